@@ -1,0 +1,172 @@
+"""ctypes binding of libcxm.so (the C ABI declared in include/cxm.h).
+
+torch is imported first so that libcxm binds to the HIP runtime torch has
+already loaded: device buffers and streams are shared with torch.  There is
+no CPU fallback: if the library is missing or no GPU is visible, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+from pathlib import Path
+
+import torch  # noqa: F401  (must precede the CDLL load)
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libcxm.so"
+
+CXM_OK = 0
+CXM_ERR_INVALID = 1
+CXM_ERR_UNSUPPORTED = 2
+CXM_ERR_HIP = 3
+CXM_ERR_WORKSPACE = 4
+
+CXM_H_NONE = 0
+CXM_H_TRANSFER = 1
+CXM_H_ASM = 2
+CXM_H_ARRAY = 3
+
+CXM_PHASE_REF32 = 0
+CXM_PHASE_ACCURATE = 1
+
+CXM_OUT_COMPLEX = 0
+CXM_OUT_INTENSITY = 1
+
+PHASE_MODES = {"ref32": CXM_PHASE_REF32, "accurate": CXM_PHASE_ACCURATE}
+
+
+class CxmError(RuntimeError):
+    """Raised when a libcxm entry point returns a non-zero status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"libcxm error {code}: {message}")
+        self.code = code
+
+
+class Prop2dDesc(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("pad_y", ctypes.c_int32),
+        ("pad_x", ctypes.c_int32),
+        ("crop_y", ctypes.c_int32),
+        ("crop_x", ctypes.c_int32),
+        ("num_z", ctypes.c_int32),
+        ("z_zip", ctypes.c_int32),
+        ("kind", ctypes.c_int32),
+        ("phase_mode", ctypes.c_int32),
+        ("out_kind", ctypes.c_int32),
+        ("remove_evanescent", ctypes.c_int32),
+        ("bandlimit", ctypes.c_int32),
+        ("num_wv", ctypes.c_int32),
+        ("dy", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("wavelength", ctypes.c_double),
+        ("n", ctypes.c_double),
+        ("ky", ctypes.c_double),
+        ("kx", ctypes.c_double),
+        ("shift_y", ctypes.c_double),
+        ("shift_x", ctypes.c_double),
+        ("z", ctypes.POINTER(ctypes.c_double)),
+        ("wavelengths", ctypes.POINTER(ctypes.c_double)),
+        ("spacing", ctypes.POINTER(ctypes.c_double)),
+        ("propagator", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _declare(lib):
+    vp, i32, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
+    dp = ctypes.POINTER(ctypes.c_double)
+    sigs = {
+        "cxm_init": (i32, [i32]),
+        "cxm_last_error": (ctypes.c_char_p, []),
+        "cxm_version": (i32, []),
+        "cxm_fft_supported": (i32, [ctypes.c_int64]),
+        "cxm_prop2d_workspace_size": (sz, [ctypes.POINTER(Prop2dDesc)]),
+        "cxm_prop2d_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Prop2dDesc)]),
+        "cxm_propagator_c64": (i32, [vp, vp, ctypes.POINTER(Prop2dDesc)]),
+        "cxm_timing_enable": (i32, [i32]),
+        "cxm_timing_collect": (i32, []),
+        "cxm_timing_get": (i32, [i32, ctypes.c_char_p, i32, dp, ctypes.POINTER(ctypes.c_int64)]),
+        "cxm_intensity_c64": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, dp]),
+        "cxm_plane_sumsq_workspace_size": (sz, [ctypes.c_int64]),
+        "cxm_plane_sumsq_c64": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, dp, vp, sz]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def lib():
+    """The loaded libcxm (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not LIB_PATH.exists():
+                    raise RuntimeError(
+                        f"{LIB_PATH} is missing; build it with `python -m chromatix_amd.build`"
+                    )
+                _lib = _declare(ctypes.CDLL(str(LIB_PATH)))
+    return _lib
+
+
+def check(code: int) -> None:
+    if code != CXM_OK:
+        msg = lib().cxm_last_error()
+        raise CxmError(code, msg.decode() if msg else "")
+
+
+_initialised: set[int] = set()
+
+
+def ensure_device(t: torch.Tensor) -> None:
+    """Require a HIP device tensor and initialise libcxm on its device."""
+    if not t.is_cuda:
+        raise RuntimeError(
+            "chromatix_amd runs on the GPU only: move the field to a HIP device "
+            "(no CPU fallback exists)"
+        )
+    dev = t.device.index if t.device.index is not None else torch.cuda.current_device()
+    if dev not in _initialised:
+        with torch.cuda.device(dev):
+            check(lib().cxm_init(dev))
+        _initialised.add(dev)
+
+
+def stream_ptr(device: torch.device | None = None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def dptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def doubles(values) -> ctypes.Array:
+    vals = [float(v) for v in values]
+    return (ctypes.c_double * max(1, len(vals)))(*vals)
+
+
+def timing_enable(on: bool = True) -> None:
+    check(lib().cxm_timing_enable(1 if on else 0))
+
+
+def timing_collect() -> dict[str, tuple[float, int]]:
+    """{kernel name: (total ms, launches)} since timing_enable(True)."""
+    n = lib().cxm_timing_collect()
+    if n < 0:
+        check(-n)
+    out = {}
+    buf = ctypes.create_string_buffer(128)
+    for i in range(n):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        check(lib().cxm_timing_get(i, buf, 128, ctypes.byref(ms), ctypes.byref(cnt)))
+        out[buf.value.decode()] = (ms.value, cnt.value)
+    return out

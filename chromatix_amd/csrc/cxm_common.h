@@ -1,0 +1,115 @@
+// Shared device helpers for the chromatix-on-MI355X kernels: complex64
+// arithmetic, double-float (f32 pair) arithmetic and launch checks.
+//
+// Everything here is gfx950 device code compiled by hipcc; there is no
+// host fallback.  Complex values are float2 (x = real, y = imaginary), the
+// same in-memory layout as complex64 in torch, numpy and XLA.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define CXM_DEV __device__ __forceinline__
+
+CXM_DEV float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+CXM_DEV float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+CXM_DEV float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+CXM_DEV float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+CXM_DEV float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+CXM_DEV float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+// multiply by -i (forward) or +i (inverse): DIR = -1 forward, +1 inverse.
+template <int DIR> CXM_DEV float2 mul_i(float2 a) {
+  if constexpr (DIR < 0) return make_float2(a.y, -a.x);
+  else return make_float2(-a.y, a.x);
+}
+CXM_DEV float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
+
+// ---------------------------------------------------------------------------
+// Double-float ("df") arithmetic: a value is hi + lo with |lo| <= ulp(hi)/2.
+// Error-free transformations after Dekker (1971) and Knuth (TAOCP vol. 2,
+// 4.2.2): two_sum and an FMA-based two_prod.  Used for the propagator phase
+// in accurate mode, where the phase reaches ~1.6e3 rad and a plain f32
+// product loses ~1e-4 rad.
+// ---------------------------------------------------------------------------
+struct df { float hi, lo; };
+
+CXM_DEV df df_make(float hi, float lo) { df r; r.hi = hi; r.lo = lo; return r; }
+CXM_DEV df two_sum(float a, float b) {
+  float s = __fadd_rn(a, b);
+  float bb = __fsub_rn(s, a);
+  float e = __fadd_rn(__fsub_rn(a, __fsub_rn(s, bb)), __fsub_rn(b, bb));
+  return df_make(s, e);
+}
+CXM_DEV df fast_two_sum(float a, float b) {  // requires |a| >= |b|
+  float s = __fadd_rn(a, b);
+  float e = __fsub_rn(b, __fsub_rn(s, a));
+  return df_make(s, e);
+}
+CXM_DEV df two_prod(float a, float b) {
+  float p = __fmul_rn(a, b);
+  float e = __builtin_fmaf(a, b, -p);
+  return df_make(p, e);
+}
+CXM_DEV df df_add(df a, df b) {
+  df s = two_sum(a.hi, b.hi);
+  df t = two_sum(a.lo, b.lo);
+  float lo = __fadd_rn(s.lo, t.hi);
+  df u = fast_two_sum(s.hi, lo);
+  lo = __fadd_rn(u.lo, t.lo);
+  return fast_two_sum(u.hi, lo);
+}
+CXM_DEV df df_neg(df a) { return df_make(-a.hi, -a.lo); }
+CXM_DEV df df_sub(df a, df b) { return df_add(a, df_neg(b)); }
+CXM_DEV df df_mul(df a, df b) {
+  df p = two_prod(a.hi, b.hi);
+  float lo = __builtin_fmaf(a.hi, b.lo, __builtin_fmaf(a.lo, b.hi, p.lo));
+  return fast_two_sum(p.hi, lo);
+}
+CXM_DEV df df_mul_f(df a, float b) {
+  df p = two_prod(a.hi, b);
+  float lo = __builtin_fmaf(a.lo, b, p.lo);
+  return fast_two_sum(p.hi, lo);
+}
+// sqrt of a non-negative df: one Newton/Karp step on the f32 root.
+CXM_DEV df df_sqrt(df a) {
+  if (a.hi <= 0.0f) return df_make(0.0f, 0.0f);
+  float s = __fsqrt_rn(a.hi);
+  df s2 = two_prod(s, s);
+  float r = __fadd_rn(__fsub_rn(__fsub_rn(a.hi, s2.hi), s2.lo), a.lo);
+  float c = __fdiv_rn(r, __fmul_rn(2.0f, s));
+  return fast_two_sum(s, c);
+}
+// Fractional part of a df value, as a single f32 in [-0.5, 0.5].  Exact
+// reduction because |hi| < 2^23 for every phase (in turns) used here.
+CXM_DEV float df_frac(df a) {
+  float r = __fsub_rn(a.hi, rintf(a.hi));
+  float f = __fadd_rn(r, a.lo);
+  return __fsub_rn(f, rintf(f));
+}
+
+// ---------------------------------------------------------------------------
+// Index helpers.
+// ---------------------------------------------------------------------------
+// numpy ifftshift along an axis of length n: y[i] = x[(i + n/2) % n].
+CXM_DEV int ishift_src(int i, int n) { int j = i + (n >> 1); return j >= n ? j - n : j; }
+// numpy fftshift: out[i] = X[(i - n/2) mod n] (n/2 = floor), so bin j lands
+// at dst = (j + n/2) % n.
+CXM_DEV int fshift_dst(int j, int n) { int d = j + (n >> 1); return d >= n ? d - n : d; }
+// Twiddle families: FFT lengths 2^a, 3*2^a and 5*2^a each read one FP64-built
+// table W_L[i] = exp(-2 pi i i / L) of the family's largest length L; a
+// length-N transform reads every (L/N)-th entry.
+namespace cxm {
+constexpr int kTwPow2 = 16384;
+constexpr int kTw3 = 3 * 4096;
+constexpr int kTw5 = 5 * 2048;
+constexpr int tw_family_len(int N) {
+  return (N % 5 == 0) ? kTw5 : (N % 3 == 0) ? kTw3 : kTwPow2;
+}
+}  // namespace cxm
+
+// signed FFT bin for natural index k (numpy fftfreq numerator).
+CXM_DEV int signed_bin(int k, int n) { return (k < (n + 1) / 2) ? k : k - n; }

@@ -1,0 +1,125 @@
+// Elementwise and per-plane reduction kernels around the propagation path:
+//   intensity  |u|^2, spectrally weighted over an innermost wavelength axis
+//              (src/chromatix/core/field.py:663-665, 735-740)
+//   plane sums of |u|^2 for power (field.py:656-661, 728-733), reduced in a
+//              fixed order (two stages, FP64 partials) so results are
+//              bitwise reproducible and independent of what else runs.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "cxm_internal.h"
+
+namespace cxm {
+
+constexpr int kMaxWv = 64;
+struct Density { int nwv; float w[kMaxWv]; };
+
+__global__ void intensity_kernel(const float2* __restrict__ u, float* __restrict__ out,
+                                 long long n_pix, Density d) {
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_pix) return;
+  if (d.nwv == 1) {
+    const float2 v = u[p];
+    out[p] = v.x * v.x + v.y * v.y;
+  } else {
+    float acc = 0.f;
+    const float2* q = u + p * d.nwv;
+    for (int w = 0; w < d.nwv; ++w) acc += d.w[w] * (q[w].x * q[w].x + q[w].y * q[w].y);
+    out[p] = acc;
+  }
+}
+
+constexpr int kSumThreads = 256;
+constexpr int kSumChunks = 64;  // blocks per plane in stage 1
+
+__global__ void __launch_bounds__(kSumThreads)
+plane_sumsq_stage1(const float2* __restrict__ u, double* __restrict__ partial,
+                   long long plane_elems, Density d) {
+  const int plane = blockIdx.y;
+  const int chunk = blockIdx.x;
+  const long long per = (plane_elems + kSumChunks - 1) / kSumChunks;
+  const long long b = chunk * per;
+  const long long e = min(plane_elems, b + per);
+  const float2* src = u + (long long)plane * plane_elems;
+  double acc = 0.0;
+  for (long long i = b + threadIdx.x; i < e; i += kSumThreads) {
+    const float2 v = src[i];
+    const double m = (double)v.x * v.x + (double)v.y * v.y;
+    acc += (d.nwv > 1) ? (double)d.w[i % d.nwv] * m : m;
+  }
+  __shared__ double s[kSumThreads];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kSumThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(long long)plane * kSumChunks + chunk] = s[0];
+}
+
+__global__ void plane_sumsq_stage2(const double* __restrict__ partial, double* __restrict__ out,
+                                   int n_planes) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_planes) return;
+  double acc = 0.0;
+  for (int c = 0; c < kSumChunks; ++c) acc += partial[(long long)p * kSumChunks + c];
+  out[p] = acc;
+}
+
+static int make_density(int num_wv, const double* density, Density& d) {
+  if (num_wv < 1 || num_wv > kMaxWv) return fail(CXM_ERR_INVALID, "num_wv must be in [1, 64]");
+  d.nwv = num_wv;
+  for (int w = 0; w < kMaxWv; ++w) d.w[w] = 0.f;
+  for (int w = 0; w < num_wv; ++w) d.w[w] = density ? (float)density[w] : 1.0f;
+  return CXM_OK;
+}
+
+}  // namespace cxm
+
+using namespace cxm;
+
+extern "C" int cxm_intensity_c64(void* stream, const void* u, void* out, int64_t n_pixels,
+                                 int32_t num_wv, const double* density) {
+  Density d;
+  int rc = make_density(num_wv, density, d);
+  if (rc) return rc;
+  if (n_pixels < 0) return fail(CXM_ERR_INVALID, "negative size");
+  if (n_pixels == 0) return CXM_OK;
+  if (!u || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  if (num_wv == 1 && density == nullptr) d.w[0] = 1.0f;
+  const long long blocks = (n_pixels + 255) / 256;
+  hipLaunchKernelGGL(intensity_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)u, (float*)out, (long long)n_pixels, d);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+extern "C" size_t cxm_plane_sumsq_workspace_size(int64_t n_planes) {
+  return (size_t)(n_planes > 0 ? n_planes : 0) * kSumChunks * sizeof(double);
+}
+
+extern "C" int cxm_plane_sumsq_c64(void* stream, const void* u, void* out, int64_t n_planes,
+                                   int64_t plane_elems, int32_t num_wv, const double* density,
+                                   void* workspace, size_t workspace_bytes) {
+  Density d;
+  int rc = make_density(num_wv, density, d);
+  if (rc) return rc;
+  if (n_planes < 0 || plane_elems < 0) return fail(CXM_ERR_INVALID, "negative size");
+  if (n_planes == 0) return CXM_OK;
+  if (n_planes > 65535) return fail(CXM_ERR_INVALID, "at most 65535 planes per call");
+  if (!u || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  if (num_wv > 1 && plane_elems % num_wv != 0)
+    return fail(CXM_ERR_INVALID, "plane_elems must be a multiple of num_wv");
+  const size_t need = cxm_plane_sumsq_workspace_size(n_planes);
+  if (!workspace || workspace_bytes < need)
+    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(plane_sumsq_stage1, dim3(kSumChunks, (unsigned)n_planes), dim3(kSumThreads), 0, s,
+                     (const float2*)u, (double*)workspace, (long long)plane_elems, d);
+  CXM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(plane_sumsq_stage2, dim3((unsigned)((n_planes + 63) / 64)), dim3(64), 0, s,
+                     (const double*)workspace, (double*)out, (int)n_planes);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}

@@ -1,0 +1,109 @@
+// Host/device shared internals of libcxm: per-plane transfer-function
+// parameters, kernel argument blocks, error plumbing and FFT-length dispatch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/cxm.h"
+#include "cxm_common.h"
+
+namespace cxm {
+
+// ----- error handling -------------------------------------------------------
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+#define CXM_HIP_CHECK(expr)                                                   \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess)                                                     \
+      return ::cxm::fail(CXM_ERR_HIP, std::string(#expr) + ": " +             \
+                                          hipGetErrorString(_e));             \
+  } while (0)
+#define CXM_LAUNCH_CHECK()                                                    \
+  do {                                                                        \
+    hipError_t _e = hipGetLastError();                                        \
+    if (_e != hipSuccess)                                                     \
+      return ::cxm::fail(CXM_ERR_HIP, std::string("kernel launch: ") +        \
+                                          hipGetErrorString(_e));             \
+  } while (0)
+
+// ----- optional kernel timing (HIP events on the launch stream) --------------
+bool timing_on();
+void* timing_start(hipStream_t s);
+void timing_stop(void* start, hipStream_t s, const std::string& name);
+
+// ----- twiddle tables (families: cxm_common.h) -------------------------------
+// device pointer to the family table for length n on the current device
+// (cxm_init must have run on it).
+const float2* twiddles_for(int n);
+int ensure_init();
+
+// ----- per-plane transfer function parameters ------------------------------
+// Everything the column kernel needs to evaluate H(ky, kx) for one output
+// plane.  f32 members hold the reference's float32 intermediates computed in
+// its operation order on the host; df pairs hold the accurate-mode
+// constants.
+struct HParam {
+  float inv_dy, inv_dx;  // fl(1/dy), fl(1/dx)                (field.py:40)
+  float ky0, kx0;        // kykx as f32
+  float c_l2;            // ASM: fl((lam/n)^2); transfer: fl(fl(fl(-pi*lam)/n)*z)
+  float c_ph;            // ASM: fl(2pi * fl(fl(|z|*n)/lam))
+  float sy, sx;          // shift_yx as f32
+  float by0, bym, bx0, bxm;  // band-limit rectangle (propagation.py:557-579)
+  // accurate mode: df constants (turns)
+  float ry_hi, ry_lo, rx_hi, rx_lo;  // 1/(Ny dy), 1/(Nx dx)
+  float kyc_hi, kyc_lo, kxc_hi, kxc_lo;  // kykx
+  float q_hi, q_lo;      // ASM: (lam/n)^2;   transfer: -lam z/(2n)
+  float a_hi, a_lo;      // ASM: |z| n / lam
+  int conj;              // ASM with z < 0
+  int pad_;
+};
+static_assert(sizeof(HParam) % 8 == 0, "HParam must stay 8-byte aligned");
+
+constexpr int kMaxHP = 32;  // parameter sets carried by value per launch
+
+struct RowArgs {
+  const float2* in;
+  float2* out_c;
+  float* out_i;
+  const float2* tw;
+  long long in_plane_stride, out_plane_stride;  // elements
+  int in_row_stride, out_row_stride;            // elements
+  int n_rows, rows_per_plane;
+  int in_off, in_len;    // FFT index of input column 0, valid input columns
+  int out_off, out_len;  // FFT indices [out_off, out_off+out_len) are stored
+  float scale;
+  int intensity;
+};
+
+struct ColArgs {
+  const float2* in;
+  float2* out;
+  const float2* tw;
+  const float2* H;       // CXM_H_ARRAY
+  long long in_plane_stride, out_plane_stride;
+  int row_stride;        // elements between rows (= number of columns Nx)
+  int ncols;
+  int n_planes;          // input planes in this launch
+  int plane0;            // global index of the first input plane (param lookup)
+  int nz;                // distances (parameter sets per wavelength)
+  int z_zip;             // 1: plane b uses distance (b / nwv) % nz, one output plane
+  int nwv;               // wavelengths cycling fastest over input planes
+  int in_off, in_len;    // FFT rows [in_off, in_off+in_len) stored at rows [0, in_len)
+  int out_off, out_len;  // FFT rows [out_off, out_off+out_len) written to rows [0, out_len)
+  int kind, phase_mode, remove_evanescent, bandlimit;
+  float scale;
+  int nhp;
+  HParam hp[kMaxHP];
+};
+
+// Host builders.
+int build_hparams(const cxm_prop2d_desc* d, int Ny, int Nx, HParam* out, int nmax);
+
+// Kernel launchers (implemented per translation unit).
+int launch_row(int N, int dir, const RowArgs& a, hipStream_t s);
+int launch_col_prop(int N, const ColArgs& a, hipStream_t s);
+bool fft_len_supported(long long n);
+
+}  // namespace cxm

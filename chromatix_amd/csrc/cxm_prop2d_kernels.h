@@ -1,0 +1,204 @@
+// Row and column pass kernels of the batched 2D propagation (templates;
+// instantiated per size group in cxm_inst_g*.hip).
+//
+// Row pass: G transforms of length N per workgroup, T = N/E threads each;
+// zero-padded load (input column x lands at FFT index x + in_off), FFT in
+// either direction, store of FFT indices [out_off, out_off + out_len) as
+// complex64 or |u|^2.
+//
+// Column pass: C columns x T threads per workgroup; thread (ty, c) owns rows
+// ty + k*T of column c and LDS holds the whole N x C strip.  Forward FFT,
+// multiply by H (evaluated on the fly, or read from an array), inverse FFT,
+// store of the requested rows.  All rows of the strip are read before any is
+// written, so in == out is allowed when nz == 1.
+#pragma once
+#include <string>
+
+#include "cxm_fft.h"
+#include "cxm_hfunc.h"
+#include "cxm_internal.h"
+#include "cxm_sizes.h"
+
+namespace cxm {
+
+template <int N, int E>
+struct RowCfg {
+  static constexpr int T = N / E;
+  static constexpr int G = (T >= 256) ? 1 : 256 / T;
+  static constexpr int THREADS = G * T;
+};
+
+template <int N, int E, int DIR>
+__global__ void __launch_bounds__((RowCfg<N, E>::THREADS))
+row_fft_kernel(RowArgs a) {
+  using Cfg = RowCfg<N, E>;
+  constexpr int T = Cfg::T;
+  constexpr int SPAN = lds_span<N, 1>();
+  __shared__ float2 lds[Cfg::G * SPAN];
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x - g * T;
+  const int row = blockIdx.x * Cfg::G + g;
+  const bool ok = row < a.n_rows;
+  const int p = ok ? row / a.rows_per_plane : 0;
+  const int r = ok ? row - p * a.rows_per_plane : 0;
+  const float2* src = a.in + p * a.in_plane_stride + (long long)r * a.in_row_stride;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int xs = t + k * T - a.in_off;
+    v[k] = (ok && xs >= 0 && xs < a.in_len) ? src[xs] : make_float2(0.f, 0.f);
+  }
+  fft<N, E, DIR, 1, true>(v, lds + g * SPAN, t, a.tw);
+  if (!ok) return;
+  const long long obase = p * a.out_plane_stride + (long long)r * a.out_row_stride;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int xo = t + k * T - a.out_off;
+    if (xo >= 0 && xo < a.out_len) {
+      const float2 w = cscale(v[k], a.scale);
+      if (a.intensity) a.out_i[obase + xo] = cabs2(w);
+      else a.out_c[obase + xo] = w;
+    }
+  }
+}
+
+template <int N, int E, int C>
+struct ColCfg {
+  static constexpr int T = N / E;
+  static constexpr int THREADS = T * C;
+};
+
+// Block -> logical strip index, grouping consecutive strips on one XCD
+// (workgroups are dealt round-robin over the 8 XCDs).
+CXM_DEV int xcd_remap(int b, int nb) {
+  if ((nb & 7) != 0) return b;
+  return (b & 7) * (nb >> 3) + (b >> 3);
+}
+
+// KIND: CXM_H_NONE / CXM_H_ARRAY (both handled by KIND == 0 with a runtime
+// switch), CXM_H_TRANSFER, CXM_H_ASM.  MODE: CXM_PHASE_*.
+template <int N, int E, int C, int KIND, int MODE>
+__global__ void __launch_bounds__((ColCfg<N, E, C>::THREADS))
+col_prop_kernel(ColArgs a) {
+  constexpr int T = N / E;
+  __shared__ float2 lds[N * C];
+  const int c = threadIdx.x % C;
+  const int ty = threadIdx.x / C;
+  const int nstrips = (a.ncols + C - 1) / C;
+  const int L = xcd_remap(blockIdx.x, gridDim.x);
+  const int p = L / nstrips;
+  const int col = (L - p * nstrips) * C + c;
+  const bool colok = col < a.ncols;
+  const float2* src = a.in + p * a.in_plane_stride + col;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int ys = ty + k * T - a.in_off;
+    v[k] = (colok && ys >= 0 && ys < a.in_len) ? src[(long long)ys * a.row_stride]
+                                              : make_float2(0.f, 0.f);
+  }
+  fft<N, E, -1, C, true>(v, lds + c, ty, a.tw);
+  const int pg = a.plane0 + p;
+  const int w = (a.nwv > 1) ? (pg % a.nwv) : 0;
+  const int kx = colok ? col : 0;
+  const int nout = a.z_zip ? 1 : a.nz;
+  for (int iz = 0; iz < nout; ++iz) {
+    float2 u[E];
+    if constexpr (KIND == CXM_H_TRANSFER || KIND == CXM_H_ASM) {
+      const int zi = a.z_zip ? (pg / a.nwv) % a.nz : iz;
+      const HParam& hp = a.hp[w * a.nz + zi];
+      const XTerms xt = x_terms(hp, kx, a.ncols, MODE);
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const float2 h = eval_H(hp, xt, ty + k * T, N, KIND, MODE, a.remove_evanescent,
+                                a.bandlimit, a.scale);
+        u[k] = cmul(v[k], h);
+      }
+    } else {
+      if (a.kind == CXM_H_ARRAY) {
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const float2 h = colok ? a.H[(long long)(ty + k * T) * a.ncols + kx]
+                                 : make_float2(0.f, 0.f);
+          u[k] = cscale(cmul(v[k], h), a.scale);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < E; ++k) u[k] = cscale(v[k], a.scale);
+      }
+    }
+    fft<N, E, 1, C, true>(u, lds + c, ty, a.tw);
+    if (colok) {
+      float2* dst = a.out + ((long long)p * nout + iz) * a.out_plane_stride + col;
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int yo = ty + k * T - a.out_off;
+        if (yo >= 0 && yo < a.out_len) dst[(long long)yo * a.row_stride] = u[k];
+      }
+    }
+  }
+}
+
+// ---- launchers --------------------------------------------------------------
+template <int N, int E>
+int launch_row_t(int dir, const RowArgs& a, hipStream_t s) {
+  using Cfg = RowCfg<N, E>;
+  const int blocks = (a.n_rows + Cfg::G - 1) / Cfg::G;
+  if (blocks == 0) return CXM_OK;
+  void* t0 = timing_start(s);
+  if (dir < 0)
+    hipLaunchKernelGGL((row_fft_kernel<N, E, -1>), dim3(blocks), dim3(Cfg::THREADS), 0, s, a);
+  else
+    hipLaunchKernelGGL((row_fft_kernel<N, E, 1>), dim3(blocks), dim3(Cfg::THREADS), 0, s, a);
+  CXM_LAUNCH_CHECK();
+  if (t0) timing_stop(t0, s, std::string(dir < 0 ? "row_fft_fwd_" : "row_fft_inv_") + std::to_string(N));
+  return CXM_OK;
+}
+
+template <int N, int E, int C>
+int launch_col_t(const ColArgs& a, hipStream_t s) {
+  using Cfg = ColCfg<N, E, C>;
+  const long long nstrips = (a.ncols + C - 1) / C;
+  const long long blocks = nstrips * a.n_planes;
+  if (blocks == 0) return CXM_OK;
+  if (blocks > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "column pass grid too large");
+  const dim3 grid((unsigned)blocks), block(Cfg::THREADS);
+  void* t0 = timing_start(s);
+  if (a.kind == CXM_H_TRANSFER) {
+    if (a.phase_mode == CXM_PHASE_REF32)
+      hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_TRANSFER, CXM_PHASE_REF32>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_TRANSFER, CXM_PHASE_ACCURATE>), grid, block, 0, s, a);
+  } else if (a.kind == CXM_H_ASM) {
+    if (a.phase_mode == CXM_PHASE_REF32)
+      hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_ASM, CXM_PHASE_REF32>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_ASM, CXM_PHASE_ACCURATE>), grid, block, 0, s, a);
+  } else {
+    hipLaunchKernelGGL((col_prop_kernel<N, E, C, 0, 0>), grid, block, 0, s, a);
+  }
+  CXM_LAUNCH_CHECK();
+  if (t0) timing_stop(t0, s, "col_prop_" + std::to_string(N));
+  return CXM_OK;
+}
+
+}  // namespace cxm
+
+// Defines launch_row_gK / launch_col_gK for one size group.
+#define CXM_DEFINE_GROUP(K, SIZES)                                            \
+  namespace cxm {                                                             \
+  int launch_row_g##K(int N, int dir, const RowArgs& a, hipStream_t s) {      \
+    switch (N) {                                                              \
+      SIZES(CXM_ROW_CASE)                                                     \
+      default: return -1;                                                     \
+    }                                                                         \
+  }                                                                           \
+  int launch_col_g##K(int N, const ColArgs& a, hipStream_t s) {               \
+    switch (N) {                                                              \
+      SIZES(CXM_COL_CASE)                                                     \
+      default: return -1;                                                     \
+    }                                                                         \
+  }                                                                           \
+  }
+#define CXM_ROW_CASE(NN, ER, EC, CC) case NN: return launch_row_t<NN, ER>(dir, a, s);
+#define CXM_COL_CASE(NN, ER, EC, CC) case NN: return launch_col_t<NN, EC, CC>(a, s);

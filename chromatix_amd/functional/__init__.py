@@ -1,0 +1,7 @@
+"""Functional API of the FFT-propagation path (mirrors chromatix.functional)."""
+from .propagation import *  # noqa: F401,F403
+from .propagation import (  # noqa: F401
+    get_phase_mode,
+    propagate_intensity,
+    set_phase_mode,
+)

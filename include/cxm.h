@@ -1,0 +1,157 @@
+/*
+ * cxm.h -- C ABI of libcxm.so, the MI355X (gfx950) implementation of the
+ * chromatix FFT-propagation hot path.
+ *
+ * Conventions for every entry point:
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
+ *     calls only enqueue work on it and never synchronise the host.
+ *   - buffers are device pointers owned by the caller; complex data is
+ *     interleaved complex64 (float re, float im), C order.
+ *   - scratch comes from the caller: query *_workspace_size(), pass a device
+ *     buffer at least that large.  NULL workspace with size 0 is allowed
+ *     when the query returned 0.
+ *   - return 0 on success, a CXM_ERR_* code otherwise; cxm_last_error()
+ *     gives a thread-local message.  No C++ exception crosses the ABI.
+ *   - host arrays inside descriptors (z lists) are read during the call only.
+ *
+ * Citations name the reference function (chromatix v0.5.0, paths relative
+ * to the chromatix repository root) whose arithmetic an entry point
+ * replaces.
+ */
+#ifndef CXM_H_
+#define CXM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CXM_OK 0
+#define CXM_ERR_INVALID 1     /* bad descriptor or argument */
+#define CXM_ERR_UNSUPPORTED 2 /* FFT length or layout not implemented */
+#define CXM_ERR_HIP 3         /* HIP runtime error */
+#define CXM_ERR_WORKSPACE 4   /* workspace too small */
+
+/* Transfer-function kinds for cxm_prop2d_c64. */
+#define CXM_H_NONE 0     /* plain fft2 -> ifft2 round trip (identity up to rounding) */
+#define CXM_H_TRANSFER 1 /* Fresnel: exp(-i pi lambda/n z |f-kykx|^2)
+                            (src/chromatix/functional/propagation.py:466-496) */
+#define CXM_H_ASM 2      /* exact angular spectrum incl. evanescent handling,
+                            shift_yx ramp and optional band limit
+                            (src/chromatix/functional/propagation.py:499-580) */
+#define CXM_H_ARRAY 3    /* caller-provided propagator, (Ny, Nx) complex64 in
+                            FFT (ifftshifted) order, shared by all planes
+                            (src/chromatix/functional/propagation.py:391-394) */
+
+/* Phase evaluation of the on-the-fly transfer function. */
+#define CXM_PHASE_REF32 0    /* float32 in the reference's operation order */
+#define CXM_PHASE_ACCURATE 1 /* double-float phase in turns, exact mod-1 reduction */
+
+/* Output kinds. */
+#define CXM_OUT_COMPLEX 0   /* complex64 field */
+#define CXM_OUT_INTENSITY 1 /* float32 |u|^2 */
+
+/*
+ * 2D Fourier-convolution propagation of a batch of planes:
+ *   out = crop( ifft2( fft2( pad(in) ) * H(z_j) ) )      for each plane, each z_j
+ * Replaces kernel_propagate's default branch with the propagator built by
+ * compute_transfer_propagator / compute_asm_propagator
+ * (src/chromatix/functional/propagation.py:168-252, 255-352, 391-394,
+ * 466-580) and, with out_kind = INTENSITY, the `.intensity` of the result
+ * (src/chromatix/core/field.py:663-665).
+ *
+ * in : batch x height x width complex64 (unpadded)
+ * out: batch x num_z x out_h x out_w (z_zip = 0) or batch x out_h x out_w
+ *      (z_zip = 1), out_h = height + 2 pad_y - 2 crop_y, complex64 or float32
+ *      by out_kind.  Plane order: input plane major, distance minor (the
+ *      reference's z-batched propagator broadcast against a single field
+ *      gives the same (z, h, w) order).
+ */
+typedef struct cxm_prop2d_desc {
+  int32_t batch;
+  int32_t height, width;
+  int32_t pad_y, pad_x;   /* zero padding per side (cval = 0) */
+  int32_t crop_y, crop_x; /* crop per side of the padded grid; "same" mode = pad */
+  int32_t num_z;          /* >= 1 */
+  int32_t z_zip;          /* 0: every plane x every z (out planes = batch*num_z);
+                             1: plane b uses z[b % num_z] (out planes = batch),
+                             the reference's broadcast of a (Z,1,1) propagator
+                             against a field whose innermost batch is Z */
+  int32_t kind;           /* CXM_H_* */
+  int32_t phase_mode;     /* CXM_PHASE_* */
+  int32_t out_kind;       /* CXM_OUT_* */
+  int32_t remove_evanescent;
+  int32_t bandlimit;
+  int32_t num_wv;         /* wavelengths; input plane p uses wavelength p % num_wv
+                             (chromatic fields passed wavelength-planar) */
+  double dy, dx;          /* sample spacing (used when spacing == NULL) */
+  double wavelength;      /* same units as dx (used when wavelengths == NULL) */
+  double n;               /* refractive index of the medium */
+  double ky, kx;          /* kykx, subtracted from the frequency grid */
+  double shift_y, shift_x;/* output-plane shift (ASM linear phase, band limit) */
+  const double* z;        /* num_z propagation distances, host memory */
+  const double* wavelengths; /* num_wv entries or NULL, host memory */
+  const double* spacing;  /* num_wv x (dy, dx) or NULL, host memory */
+  const void* propagator; /* CXM_H_ARRAY only: device (Ny, Nx) complex64 */
+} cxm_prop2d_desc;
+
+size_t cxm_prop2d_workspace_size(const cxm_prop2d_desc* d);
+int cxm_prop2d_c64(void* stream, const void* in, void* out, void* workspace,
+                   size_t workspace_bytes, const cxm_prop2d_desc* d);
+
+/*
+ * The propagator itself, materialised: out is num_wv x num_z x Ny x Nx
+ * complex64 in FFT (ifftshifted) order, Ny = height + 2 pad_y (pad/crop and
+ * batch fields of the descriptor are otherwise ignored).  Same arithmetic as
+ * the on-the-fly evaluation inside cxm_prop2d_c64.  Replaces
+ * compute_transfer_propagator / compute_asm_propagator
+ * (src/chromatix/functional/propagation.py:466-580).
+ */
+int cxm_propagator_c64(void* stream, void* out, const cxm_prop2d_desc* d);
+
+/*
+ * Intensity |u|^2 of n_pixels complex64 values, or for an innermost
+ * wavelength axis of length num_wv the density-weighted sum over it:
+ *   out[p] = sum_w density[w] |u[p*num_wv + w]|^2   (float32 out)
+ * density: num_wv host values, or NULL for weight 1.
+ * Replaces ScalarField.intensity / ChromaticScalarField.intensity
+ * (src/chromatix/core/field.py:663-665, 735-740).
+ */
+int cxm_intensity_c64(void* stream, const void* u, void* out, int64_t n_pixels,
+                      int32_t num_wv, const double* density);
+
+/*
+ * Per-plane sums of |u|^2 (float64 out, one per plane), optionally
+ * density-weighted over an innermost wavelength axis of length num_wv
+ * (plane_elems counts complex values including that axis).  Fixed
+ * reduction order: bitwise reproducible.  The reduction behind
+ * Field.power (src/chromatix/core/field.py:656-661, 728-733).
+ */
+size_t cxm_plane_sumsq_workspace_size(int64_t n_planes);
+int cxm_plane_sumsq_c64(void* stream, const void* u, void* out, int64_t n_planes,
+                        int64_t plane_elems, int32_t num_wv, const double* density,
+                        void* workspace, size_t workspace_bytes);
+
+/*
+ * Optional kernel timing: with timing enabled every kernel launch is
+ * bracketed by HIP events on its stream.  cxm_timing_collect() waits for
+ * them and returns the number of distinct kernels; cxm_timing_get(i, ...)
+ * reads kernel i's name, summed milliseconds and launch count.  No
+ * counterpart in the reference (measurement support).
+ */
+int cxm_timing_enable(int on);
+int cxm_timing_collect(void);
+int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* launches);
+
+/* Library state. */
+int cxm_init(int device);            /* build twiddle tables on `device` */
+const char* cxm_last_error(void);    /* thread-local text of the last error */
+int cxm_version(void);               /* major*10000 + minor*100 + patch */
+int cxm_fft_supported(int64_t n);    /* 1 if an FFT of length n is implemented */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CXM_H_ */

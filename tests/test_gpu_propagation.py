@@ -1,0 +1,237 @@
+"""GPU parity of the fused 2D propagation (cxm_prop2d_c64) against the oracle.
+
+ref32 mode is compared with the oracle's reference-order float32 propagator
+wrapped in exact FFTs (so the difference is the GPU's complex64 FFT
+rounding); accurate mode is compared with the FP64 oracle.  Bar: relative
+L2 <= 1e-5 for complex64.
+"""
+import numpy as np
+import pytest
+import torch
+
+import chromatix_amd as cx
+import chromatix_amd.functional as cf
+from chromatix_amd import _lib as L
+from oracle import propagation as op
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+LAM, NM = 0.532, 1.33
+
+
+def _phase_field(shape, seed=0, batch=()):
+    rng = np.random.default_rng(seed)
+    return np.exp(1j * rng.uniform(0, 2 * np.pi, size=batch + tuple(shape))).astype(np.complex64)
+
+
+def _mono(u, dx):
+    return cx.ScalarField(torch.as_tensor(u).cuda(), dx, 0.0, LAM)
+
+
+def _oracle(u, dx, z, pad, method, mode, precision, **kw):
+    return op.propagate(u, (dx, dx) if np.isscalar(dx) else dx, LAM, z, NM, pad, method=method,
+                        mode=mode, precision=precision, fft_precision="fp64", **kw)
+
+
+def _run(u, dx, z, pad, method, mode, phase_mode, **kw):
+    f = _mono(u, dx)
+    fn = cf.asm_propagate if method == "asm" else cf.transfer_propagate
+    out = fn(f, z, NM, pad, mode=mode, phase_mode=phase_mode, **kw)
+    return out.u.cpu().numpy()
+
+
+CASES = [
+    # shape, dx, z, pad, method, mode
+    ((64, 64), 0.3, 10.0, 0, "asm", "same"),
+    ((100, 60), 0.3, 25.0, (14, 2), "asm", "same"),
+    ((100, 60), 0.3, 25.0, (14, 2), "transfer", "full"),
+    ((512, 512), 0.3, 100.0, 256, "asm", "same"),  # C1 geometry
+    ((512, 512), 0.3, 100.0, 256, "transfer", "same"),
+    ((256, 256), 40 / 256, 100.0, 512, "asm", "same"),  # tests/test_propagate.py shape -> 1280^2 grid
+    ((160, 160), 0.25, 15.0, 0, "transfer", "full"),
+    ((2, 2), 0.1, 1.0, 0, "asm", "full"),
+    ((4, 8), 0.2, 3.0, (2, 0), "asm", "full"),
+    ((64, 32), 0.1, 2.0, 0, "asm", "same"),  # evanescent regime
+    ((128, 128), 0.3, -40.0, 0, "asm", "same"),  # negative z
+    ((1024, 1024), 0.3, 100.0, 0, "asm", "same"),  # headline plane
+    ((80, 2560), 0.3, 50.0, 0, "asm", "same"),
+    ((8192, 16), 0.3, 50.0, 0, "transfer", "same"),
+    ((16, 4096), 0.3, 50.0, 0, "asm", "same"),
+]
+
+
+@pytest.mark.parametrize("phase_mode", ["ref32", "accurate"])
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0][0]}x{c[0][1]}-{c[4]}-{c[5]}-z{c[2]}" for c in CASES])
+def test_propagation_matches_oracle(case, phase_mode):
+    shape, dx, z, pad, method, mode = case
+    u = _phase_field(shape, seed=hash(shape) % 1000)
+    got = _run(u, dx, z, pad, method, mode, phase_mode)
+    ref = _oracle(u, dx, z, pad, method, mode, "ref32" if phase_mode == "ref32" else "fp64")
+    err = rel_l2(got, ref)
+    print(f"{case} {phase_mode}: rel_l2 = {err:.3e}")
+    assert got.shape == ref.shape
+    assert err < TOL
+
+
+@pytest.mark.parametrize("phase_mode", ["ref32", "accurate"])
+@pytest.mark.parametrize("opts", [
+    dict(kykx=(0.4, -0.7)),
+    dict(shift_yx=(3.0, -5.5)),
+    dict(bandlimit=True),
+    dict(remove_evanescent=True),
+    dict(bandlimit=True, shift_yx=(20.0, 10.0)),
+])
+def test_asm_options(opts, phase_mode):
+    shape, dx, z = (128, 96), 0.15, 800.0
+    u = _phase_field(shape, seed=3)
+    got = _run(u, dx, z, 16, "asm", "full", phase_mode, **opts)
+    ref = _oracle(u, dx, z, 16, "asm", "full", "ref32" if phase_mode == "ref32" else "fp64", **opts)
+    err = rel_l2(got, ref)
+    print(f"{opts} {phase_mode}: rel_l2 = {err:.3e}")
+    assert err < TOL
+
+
+def test_transfer_kykx():
+    u = _phase_field((64, 64), seed=4)
+    got = _run(u, 0.3, 30.0, 0, "transfer", "same", "ref32", kykx=(0.2, 0.1))
+    ref = _oracle(u, 0.3, 30.0, 0, "transfer", "same", "ref32", kykx=(0.2, 0.1))
+    assert rel_l2(got, ref) < TOL
+
+
+def test_z_batch_outer_and_zip():
+    zs = np.array([-20.0, 5.0, 12.5, 40.0])
+    u = _phase_field((96, 64), seed=5)
+    f = _mono(u, 0.3)
+    out = cf.asm_propagate(f, zs, NM, 8, mode="same")
+    assert out.shape == (4, 96, 64)
+    ref = np.stack([_oracle(u, 0.3, z, 8, "asm", "same", "ref32") for z in zs])
+    assert rel_l2(out.u.cpu().numpy(), ref) < TOL
+    # zip: innermost batch equals the number of distances
+    ub = _phase_field((96, 64), seed=6, batch=(2, 4))
+    fb = _mono(ub, 0.3)
+    outb = cf.transfer_propagate(fb, zs, NM, 0, mode="same")
+    assert outb.shape == (2, 4, 96, 64)
+    refb = np.stack([[_oracle(ub[a, k], 0.3, zs[k], 0, "transfer", "same", "ref32") for k in range(4)]
+                     for a in range(2)])
+    assert rel_l2(outb.u.cpu().numpy(), refb) < TOL
+    # outer with a trailing singleton batch
+    u1 = _phase_field((32, 32), seed=7, batch=(3, 1))
+    o1 = cf.asm_propagate(_mono(u1, 0.3), zs, NM, 0)
+    assert o1.shape == (3, 4, 32, 32)
+
+
+def test_chromatic_fields():
+    wl = np.array([0.45, 0.532, 0.65])
+    rng = np.random.default_rng(8)
+    u = np.exp(2j * np.pi * rng.random((64, 48, 3))).astype(np.complex64)
+    dx = np.array([[0.3, 0.3], [0.31, 0.29], [0.35, 0.35]])
+    f = cx.ChromaticScalarField(torch.as_tensor(u).cuda(), dx, 0.0, cx.Spectrum(wl))
+    for pm, prec in (("ref32", "ref32"), ("accurate", "fp64")):
+        out = cf.asm_propagate(f, 30.0, NM, 8, mode="same", phase_mode=pm).u.cpu().numpy()
+        ref = np.stack([op.propagate(u[..., w], dx[w], wl[w], 30.0, NM, 8, precision=prec,
+                                     fft_precision="fp64") for w in range(3)], axis=-1)
+        assert rel_l2(out, ref) < TOL
+    # the reference tests' (wavelength, 1.0) spectrum: a chromatic field with wv = 1
+    f1 = cx.Field.build(torch.as_tensor(u[..., :1]).cuda(), 0.3, (0.532, 1.0))
+    assert isinstance(f1, cx.ChromaticScalarField)
+    o1 = cf.transfer_propagate(f1, 10.0, NM, 4, mode="same").u.cpu().numpy()
+    r1 = op.propagate(u[..., 0], (0.3, 0.3), 0.532, 10.0, NM, 4, method="transfer",
+                      precision="ref32", fft_precision="fp64")
+    assert rel_l2(o1[..., 0], r1) < TOL
+
+
+def test_chromatic_z_batch():
+    wl = np.array([0.5, 0.6])
+    zs = np.array([10.0, 20.0, 30.0])
+    rng = np.random.default_rng(9)
+    u = np.exp(2j * np.pi * rng.random((32, 32, 2))).astype(np.complex64)
+    f = cx.ChromaticScalarField(torch.as_tensor(u).cuda(), 0.25, 0.0, cx.Spectrum(wl))
+    out = cf.transfer_propagate(f, zs, NM, 0).u.cpu().numpy()
+    assert out.shape == (3, 32, 32, 2)
+    for k, z in enumerate(zs):
+        for w in range(2):
+            r = op.propagate(u[..., w], (0.25, 0.25), wl[w], z, NM, 0, method="transfer",
+                             mode="full", precision="ref32", fft_precision="fp64")
+            assert rel_l2(out[k, ..., w], r) < TOL
+
+
+def test_materialised_propagator_matches_oracle():
+    f = _mono(_phase_field((96, 80)), 0.2)
+    for pm, prec in (("ref32", "ref32"), ("accurate", "fp64")):
+        H = cf.compute_asm_propagator(f, 55.0, NM, shift_yx=(1.0, -2.0), phase_mode=pm).cpu().numpy()
+        Hr = op.asm_propagator((96, 80), (0.2, 0.2), LAM, 55.0, NM, shift_yx=(1.0, -2.0), precision=prec)
+        assert np.max(np.abs(H - Hr)) < 2e-6
+        Ht = cf.compute_transfer_propagator(f, [3.0, -8.0], NM, phase_mode=pm).cpu().numpy()
+        Htr = op.transfer_propagator((96, 80), (0.2, 0.2), LAM, [3.0, -8.0], NM, precision=prec)
+        assert Ht.shape == (2, 96, 80)
+        assert np.max(np.abs(Ht - Htr)) < 2e-6
+
+
+def test_kernel_propagate_with_array():
+    rng = np.random.default_rng(10)
+    u = (rng.standard_normal((3, 64, 40)) + 1j * rng.standard_normal((3, 64, 40))).astype(np.complex64)
+    H = np.exp(2j * np.pi * rng.random((64, 40))).astype(np.complex64)
+    f = _mono(u, 0.3)
+    out = cf.kernel_propagate(f, torch.as_tensor(H).cuda()).u.cpu().numpy()
+    ref = op.kernel_propagate(u, H)
+    assert rel_l2(out, ref) < TOL
+    # per-plane propagators broadcast against the batch
+    Hb = np.exp(2j * np.pi * rng.random((3, 64, 40))).astype(np.complex64)
+    outb = cf.kernel_propagate(f, torch.as_tensor(Hb).cuda()).u.cpu().numpy()
+    assert rel_l2(outb, op.kernel_propagate(u, Hb)) < TOL
+
+
+def test_fused_intensity_c1():
+    """Config C1: 512^2 exp(i phi) (seed 0), pad 256 -> 1024^2, intensity."""
+    u = _phase_field((512, 512), seed=0)
+    f = _mono(u, 0.3)
+    I = cf.propagate_intensity(f, 100.0, NM, 256, method="asm").cpu().numpy()
+    I2 = cf.asm_propagate(f, 100.0, NM, 256, mode="same").intensity.cpu().numpy()
+    ref32 = np.abs(_oracle(u, 0.3, 100.0, 256, "asm", "same", "ref32")) ** 2
+    ref64 = np.abs(_oracle(u, 0.3, 100.0, 256, "asm", "same", "fp64")) ** 2
+    e32 = rel_l2(I, ref32)
+    print(f"C1 intensity ref32 vs ref32-oracle {e32:.2e}; vs fp64 {rel_l2(I, ref64):.2e}")
+    assert e32 < TOL
+    assert rel_l2(I2, I) < 1e-6
+    Ia = cf.propagate_intensity(f, 100.0, NM, 256, phase_mode="accurate").cpu().numpy()
+    assert rel_l2(Ia, ref64) < TOL
+
+
+def test_intensity_and_power():
+    rng = np.random.default_rng(11)
+    u = (rng.standard_normal((2, 33, 17)) + 1j * rng.standard_normal((2, 33, 17))).astype(np.complex64)
+    f = _mono(u, (0.3, 0.2))
+    np.testing.assert_allclose(f.intensity.cpu().numpy(), np.abs(u) ** 2, rtol=1e-6)
+    p = f.power.cpu().numpy()
+    assert p.shape == (2, 1, 1)
+    np.testing.assert_allclose(p[:, 0, 0], 0.06 * np.sum(np.abs(u) ** 2, axis=(1, 2)), rtol=1e-6)
+    uc = (rng.standard_normal((5, 6, 3)) + 1j * rng.standard_normal((5, 6, 3))).astype(np.complex64)
+    fc = cx.ChromaticScalarField(torch.as_tensor(uc).cuda(), 0.5, 0.0, cx.Spectrum([0.4, 0.5, 0.6], [1, 2, 1]))
+    dens = np.array([0.25, 0.5, 0.25])
+    np.testing.assert_allclose(fc.intensity.cpu().numpy(), np.sum(dens * np.abs(uc) ** 2, -1), rtol=1e-6)
+
+
+def test_headline_batch_properties():
+    """64 x 1024^2 (the benchmark batch): spot planes vs the oracle and the
+    z / -z round trip (remove_evanescent) over the whole batch."""
+    B = 64
+    rng = np.random.default_rng(0)
+    u = torch.exp(1j * torch.as_tensor(rng.uniform(0, 2 * np.pi, (B, 1024, 1024)), dtype=torch.float32)).cuda()
+    f = _mono(u, 0.3)
+    out = cf.asm_propagate(f, 100.0, NM, 0, mode="same")
+    for b in (0, 37, B - 1):
+        ref = _oracle(u[b].cpu().numpy(), 0.3, 100.0, 0, "asm", "same", "ref32")
+        assert rel_l2(out.u[b].cpu().numpy(), ref) < TOL
+    fwd = cf.asm_propagate(f, 100.0, NM, 0, remove_evanescent=True)
+    back = cf.asm_propagate(fwd, -100.0, NM, 0, remove_evanescent=True)
+    err = (torch.linalg.vector_norm(back.u - u) / torch.linalg.vector_norm(u)).item()
+    assert err < TOL
+
+
+def test_errors_are_raised():
+    f = _mono(_phase_field((100, 100)), 0.3)
+    with pytest.raises(L.CxmError):
+        cf.asm_propagate(f, 1.0, NM, 0)  # 100 is not an implemented FFT length
+    with pytest.raises(ValueError):
+        cf.asm_propagate(_mono(_phase_field((8, 8), batch=(3,)), 0.3), [1.0, 2.0], NM, 0)
