@@ -1,0 +1,109 @@
+"""The C ABI (CPU): libcxm.so loads, exports every function include/cxm.h
+declares, its descriptor layout matches the ctypes mirror, and argument
+validation fails cleanly before any device call."""
+import ctypes
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from chromatix_amd import _lib as L
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "cxm.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cxm_\w+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = ctypes.CDLL(str(L.LIB_PATH))
+    names = declared_functions()
+    assert len(names) >= 10
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the Python binding declares a signature for each
+    L.lib()
+    for n in names:
+        assert getattr(L.lib(), n).restype is not None or n == "cxm_timing_enable"
+
+
+def test_descriptor_layout_matches_header(tmp_path):
+    fields = [f for f, _ in L.Prop2dDesc._fields_]
+    src = tmp_path / "layout.c"
+    body = "\n".join(
+        f'  printf("{f} %zu\\n", offsetof(cxm_prop2d_desc, {f}));' for f in fields
+    )
+    src.write_text(
+        "#include <stdio.h>\n#include <stddef.h>\n#include \"cxm.h\"\nint main(void){\n"
+        f"{body}\n  printf(\"SIZE %zu\\n\", sizeof(cxm_prop2d_desc));\n  return 0;\n}}\n"
+    )
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    c_off = dict(line.split() for line in out if line.strip())
+    for f in fields:
+        assert int(c_off[f]) == getattr(L.Prop2dDesc, f).offset, f
+    assert int(c_off["SIZE"]) == ctypes.sizeof(L.Prop2dDesc)
+
+
+def _desc(**kw):
+    d = L.Prop2dDesc()
+    d.batch, d.height, d.width = 1, 64, 64
+    d.num_z = 1
+    d.kind = L.CXM_H_ASM
+    d.num_wv = 1
+    d.dy = d.dx = 0.3
+    d.wavelength, d.n = 0.532, 1.33
+    z = (ctypes.c_double * 1)(10.0)
+    d.z = ctypes.cast(z, ctypes.POINTER(ctypes.c_double))
+    for k, v in kw.items():
+        setattr(d, k, v)
+    return d, z
+
+
+def test_validation_errors_without_device():
+    lib = L.lib()
+    d, keep = _desc()
+    assert lib.cxm_prop2d_workspace_size(ctypes.byref(d)) > 0
+    bad, keep2 = _desc(height=100)  # 100 is not an implemented length
+    assert lib.cxm_prop2d_workspace_size(ctypes.byref(bad)) == 0
+    rc = lib.cxm_prop2d_c64(None, None, None, None, 0, ctypes.byref(bad))
+    assert rc == L.CXM_ERR_UNSUPPORTED
+    assert b"unsupported FFT grid" in lib.cxm_last_error()
+    for kw in (dict(num_z=0), dict(kind=9), dict(phase_mode=5), dict(n=-1.0), dict(pad_y=-1),
+               dict(crop_x=40), dict(num_wv=8, num_z=8), dict(z_zip=3)):
+        dd, k3 = _desc(**kw)
+        assert lib.cxm_prop2d_c64(None, None, None, None, 0, ctypes.byref(dd)) == L.CXM_ERR_INVALID, kw
+    ok, k4 = _desc()
+    rc = lib.cxm_prop2d_c64(None, ctypes.c_void_p(16), ctypes.c_void_p(16), None, 0, ctypes.byref(ok))
+    assert rc == L.CXM_ERR_WORKSPACE
+    assert lib.cxm_intensity_c64(None, None, None, 4, 0, None) == L.CXM_ERR_INVALID
+    assert lib.cxm_plane_sumsq_c64(None, None, None, 70000, 1, 1, None, None, 0) == L.CXM_ERR_INVALID
+
+
+def test_fft_length_support():
+    lib = L.lib()
+    for n in [2, 4, 16, 64, 256, 1024, 4096, 8192, 5, 160, 1280, 2560, 5120]:
+        assert lib.cxm_fft_supported(n) == 1, n
+    for n in [3, 6, 7, 100, 1000, 2559, 16384]:
+        assert lib.cxm_fft_supported(n) == 0, n
+
+
+def test_cpu_tensors_are_refused():
+    import torch
+
+    import chromatix_amd as cx
+    import chromatix_amd.functional as cf
+
+    f = cx.ScalarField(torch.ones(8, 8, dtype=torch.complex64), 0.3, 0.0, 0.532)
+    with pytest.raises(RuntimeError, match="GPU only"):
+        cf.asm_propagate(f, 1.0, 1.33, 0)
+    with pytest.raises(RuntimeError, match="GPU only"):
+        _ = f.intensity
+    _ = np
