@@ -26,6 +26,8 @@ COMMON_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
+    # rounding-exact sequences must not be fused; FMAs are written explicitly
+    "-ffp-contract=off",
     "-Wall",
     "-Wno-unused-function",
     "-Wno-unused-variable",
@@ -37,7 +39,7 @@ COMMON_FLAGS = [
 
 
 def sources() -> list[Path]:
-    return sorted(CSRC.glob("*.hip"))
+    return sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
 
 
 def _compile(src: Path) -> Path:

@@ -4,8 +4,20 @@
 // Everything here is gfx950 device code compiled by hipcc; there is no
 // host fallback.  Complex values are float2 (x = real, y = imaginary), the
 // same in-memory layout as complex64 in torch, numpy and XLA.
+//
+// The library is compiled with -ffp-contract=off: rounding-exact sequences
+// (the reference-order float32 phase, double-float arithmetic) must not be
+// fused, so every FMA the arithmetic wants is written explicitly.
+//
+// CXM_CPU_CHECK builds these headers for the host with the shims of
+// tests/cpu_check (a test-only build that checks the phase arithmetic on the
+// CPU); the library itself is always the gfx950 build.
 #pragma once
+#ifdef CXM_CPU_CHECK
+#include "cxm_cpu_shim.h"
+#else
 #include <hip/hip_runtime.h>
+#endif
 #include <stdint.h>
 
 #define CXM_DEV __device__ __forceinline__
@@ -13,11 +25,11 @@
 CXM_DEV float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 CXM_DEV float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 CXM_DEV float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_float2(__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x));
 }
 // a * conj(b)
 CXM_DEV float2 cmulc(float2 a, float2 b) {
-  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  return make_float2(__builtin_fmaf(a.x, b.x, a.y * b.y), __builtin_fmaf(a.y, b.x, -(a.x * b.y)));
 }
 CXM_DEV float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 CXM_DEV float2 cconj(float2 a) { return make_float2(a.x, -a.y); }

@@ -55,7 +55,7 @@ CXM_DEV float2 twc(float2 a) {
   } else {
     constexpr float c = (float)cx_cos(2.0 * kPi * m / R);
     constexpr float s = (float)(DIR * cx_sin(2.0 * kPi * m / R));
-    return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+    return make_float2(__builtin_fmaf(a.x, c, -(a.y * s)), __builtin_fmaf(a.x, s, a.y * c));
   }
 }
 
@@ -86,7 +86,7 @@ template <int DIR> struct Dft<3, DIR> {
     constexpr float s = (float)(0.86602540378443864676);  // sin(2pi/3)
     float2 b = cadd(a[1], a[2]);
     float2 d = csub(a[1], a[2]);
-    float2 m = make_float2(a[0].x - 0.5f * b.x, a[0].y - 0.5f * b.y);
+    float2 m = make_float2(__builtin_fmaf(-0.5f, b.x, a[0].x), __builtin_fmaf(-0.5f, b.y, a[0].y));
     // DIR * i * s * d
     float2 r = mul_i<DIR>(cscale(d, s));
     a[0] = cadd(a[0], b);
@@ -104,10 +104,12 @@ template <int DIR> struct Dft<5, DIR> {
     float2 b1 = cadd(a[1], a[4]), d1 = csub(a[1], a[4]);
     float2 b2 = cadd(a[2], a[3]), d2 = csub(a[2], a[3]);
     float2 x0 = a[0];
-    float2 p1 = make_float2(x0.x + c1 * b1.x + c2 * b2.x, x0.y + c1 * b1.y + c2 * b2.y);
-    float2 p2 = make_float2(x0.x + c2 * b1.x + c1 * b2.x, x0.y + c2 * b1.y + c1 * b2.y);
-    float2 q1 = make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y);
-    float2 q2 = make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y);
+    const float2 p1 = make_float2(__builtin_fmaf(c2, b2.x, __builtin_fmaf(c1, b1.x, x0.x)),
+                                  __builtin_fmaf(c2, b2.y, __builtin_fmaf(c1, b1.y, x0.y)));
+    const float2 p2 = make_float2(__builtin_fmaf(c1, b2.x, __builtin_fmaf(c2, b1.x, x0.x)),
+                                  __builtin_fmaf(c1, b2.y, __builtin_fmaf(c2, b1.y, x0.y)));
+    float2 q1 = make_float2(__builtin_fmaf(s2, d2.x, s1 * d1.x), __builtin_fmaf(s2, d2.y, s1 * d1.y));
+    float2 q2 = make_float2(__builtin_fmaf(-s1, d2.x, s2 * d1.x), __builtin_fmaf(-s1, d2.y, s2 * d1.y));
     q1 = mul_i<DIR>(q1);
     q2 = mul_i<DIR>(q2);
     a[0] = cadd(x0, cadd(b1, b2));

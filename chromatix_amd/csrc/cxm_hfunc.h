@@ -16,8 +16,9 @@
 // reduces it exactly mod 1 and calls sincospif, so the error is that of a
 // correctly rounded f32 H.
 #pragma once
+#include "../../include/cxm.h"
 #include "cxm_common.h"
-#include "cxm_internal.h"
+#include "cxm_params.h"
 
 namespace cxm {
 

@@ -1,0 +1,196 @@
+// Pointwise operators applied inside the FFT pipeline passes (on load, on
+// the spectrum between two transforms, on store), so that chirps, masks,
+// phase screens and scales never cost an extra HBM pass.
+//
+// Each operator multiplies the complex value at a grid position by a factor
+// (or conjugates it).  Positions are FFT-grid indices (iy, ix) of an (ny, nx)
+// grid; physical coordinates follow field.grid of the reference,
+// g = fl(d * (i - N/2)) + origin  (src/chromatix/core/field.py:23-33).
+// Spectrum-stage operators read (iy, ix) as FFT-order bins.
+//
+// Per-plane parameters live in one float4 table per launch (kernel
+// arguments); operator k reads slot q = (plane / pdiv) % pmod at
+// table[pbase + q * pstride + j].  Phase-evaluating operators follow the
+// reference's float32 operation order (ref32) or evaluate the phase in
+// double-float turns (accurate).
+#pragma once
+#include "cxm_common.h"
+#include "cxm_hfunc.h"
+
+namespace cxm {
+
+constexpr int kMaxOps = 3;   // operators per stage
+constexpr int kPTab = 128;   // float4 parameter slots per launch
+
+enum OpKind : int {
+  OP_NONE = 0,
+  OP_SCALE = 1,   // v *= (p0.x + i p0.y)
+  OP_CONJ = 2,    // v = conj(v)
+  OP_CHIRP = 3,   // v *= exp(i ph); ph = fl(a*l2) [/ b if b != 0], l2 = |g - off|^2
+                  //   p0 = (a, b, a_lo, -), p1 = (dy, dx, oy, ox), p2 = (off_y, off_x, -, -)
+  OP_PHASE = 4,   // v *= exp(i fl(phi[y,x] * p0.x))                 (f32 array)
+  OP_THIN = 5,    // v *= exp(i 2 pi (dn + i abs) t / p0.x)           (f32 arrays)
+  OP_CARRAY = 6,  // v *= A[y,x]                                      (complex64 array)
+  OP_CIRC = 7,    // v *= (|g - off|^2 <= p0.x); p1, p2 as CHIRP
+  OP_VEC = 8,     // v *= V[i] along one axis                         (complex64 vector)
+  OP_SASDH = 9,   // SAS precompensation dH at bins (ky, kx)
+                  //   p0 = (kz | turns_hi, turns_lo, lam, n), p1 = (1/dy, 1/dx, ty, tx)
+  OP_NAMASK = 10, // v *= (fy^2 + fx^2 <= p0.x) at bins; p0 = (r2, 1/dy, 1/dx, -)
+};
+
+struct OpDesc {
+  int kind;
+  int mode;                    // CXM_PHASE_* for CHIRP / SASDH
+  int pdiv, pmod;              // slot q = (plane / pdiv) % pmod (pmod <= 1: slot 0)
+  int pbase, pstride;          // parameter table addressing
+  const void* arr0;            // PHASE: phi; THIN: dn; CARRAY / VEC: complex data
+  const void* arr1;            // THIN: absorption (may be null)
+  long long arr_plane_stride;  // elements between planes (0: broadcast)
+  int arr_row_stride;          // elements between array rows
+  int arr_rows, arr_cols;      // array extent (VEC: length in arr_cols)
+  int arr_row_off, arr_col_off;// grid index of array element (0, 0)
+  int outside_zero;            // positions outside the array: 0 (1) or unchanged (0)
+  int vec_axis;                // VEC: 0 indexes by iy, 1 by ix
+  float t;                     // THIN: thickness
+  int pad_;
+};
+
+struct OpList {
+  int n;
+  int pad_;
+  OpDesc op[kMaxOps];
+};
+
+struct OpTable {
+  float4 p[kPTab];
+};
+
+CXM_DEV float grid_coord(int i, int n, float d, float o) {
+  // fl(d * (i - n/2)) + o, as field.grid computes it in float32
+  return __fadd_rn(__fmul_rn(d, (float)i - 0.5f * (float)n), o);
+}
+
+CXM_DEV float bin_freq(int k, int n, float inv_d) {
+  // reference f_grid: fl(inv_d * fl(k_signed / n))
+  return __fmul_rn(inv_d, __fdiv_rn((float)signed_bin(k, n), (float)n));
+}
+
+CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, int iy, int ix,
+                        int ny, int nx) {
+  const int q = (o.pmod > 1) ? (plane / o.pdiv) % o.pmod : 0;
+  const float4* P = &T.p[o.pbase + q * o.pstride];
+  switch (o.kind) {
+    case OP_SCALE:
+      return cmul(v, make_float2(P[0].x, P[0].y));
+    case OP_CONJ:
+      return cconj(v);
+    case OP_CHIRP: {
+      float s, c;
+      const float4 p0 = P[0], p1 = P[1], p2 = P[2];
+      if (o.mode == CXM_PHASE_REF32) {
+        const float gy = __fsub_rn(grid_coord(iy, ny, p1.x, p1.z), p2.x);
+        const float gx = __fsub_rn(grid_coord(ix, nx, p1.y, p1.w), p2.y);
+        const float l2 = __fadd_rn(__fmul_rn(gy, gy), __fmul_rn(gx, gx));
+        float ph = __fmul_rn(p0.x, l2);
+        if (p0.y != 0.0f) ph = __fdiv_rn(ph, p0.y);
+        sincos_exact(ph, &s, &c);
+      } else {
+        // turns = (a_hi + a_lo) * |g - off|^2, g in double-float
+        const df gy = df_sub(df_add(df_mul_f(df_make(p1.x, 0.f), (float)iy - 0.5f * (float)ny),
+                                    df_make(p1.z, 0.f)), df_make(p2.x, 0.f));
+        const df gx = df_sub(df_add(df_mul_f(df_make(p1.y, 0.f), (float)ix - 0.5f * (float)nx),
+                                    df_make(p1.w, 0.f)), df_make(p2.y, 0.f));
+        const df l2 = df_add(df_mul(gy, gy), df_mul(gx, gx));
+        const float tr = df_frac(df_mul(df_make(p0.x, p0.z), l2));
+        sincospif(2.0f * tr, &s, &c);
+      }
+      return cmul(v, make_float2(c, s));
+    }
+    case OP_CIRC: {
+      const float4 p0 = P[0], p1 = P[1], p2 = P[2];
+      const float gy = __fsub_rn(grid_coord(iy, ny, p1.x, p1.z), p2.x);
+      const float gx = __fsub_rn(grid_coord(ix, nx, p1.y, p1.w), p2.y);
+      const float l2 = __fadd_rn(__fmul_rn(gy, gy), __fmul_rn(gx, gx));
+      return (l2 <= p0.x) ? v : make_float2(0.f, 0.f);
+    }
+    case OP_PHASE:
+    case OP_THIN:
+    case OP_CARRAY: {
+      const int ay = iy - o.arr_row_off, ax = ix - o.arr_col_off;
+      if (ay < 0 || ay >= o.arr_rows || ax < 0 || ax >= o.arr_cols)
+        return o.outside_zero ? make_float2(0.f, 0.f) : v;
+      const long long idx =
+          (long long)plane * o.arr_plane_stride + (long long)ay * o.arr_row_stride + ax;
+      if (o.kind == OP_CARRAY) return cmul(v, ((const float2*)o.arr0)[idx]);
+      float s, c;
+      if (o.kind == OP_PHASE) {
+        sincos_exact(__fmul_rn(((const float*)o.arr0)[idx], P[0].x), &s, &c);
+        return cmul(v, make_float2(c, s));
+      }
+      // thin sample (samples.py:126-133): exp(i 2 pi (dn + i abs) t / lam)
+      const float dn = ((const float*)o.arr0)[idx];
+      const float ab = o.arr1 ? ((const float*)o.arr1)[idx] : 0.0f;
+      const float lam = P[0].x;
+      const float re = __fdiv_rn(__fmul_rn(-__fmul_rn(kTwoPiF, ab), o.t), lam);
+      const float im = __fdiv_rn(__fmul_rn(__fmul_rn(kTwoPiF, dn), o.t), lam);
+      sincos_exact(im, &s, &c);
+      const float m = (re == 0.0f) ? 1.0f : expf(re);
+      return cmul(v, make_float2(m * c, m * s));
+    }
+    case OP_VEC: {
+      const int i = o.vec_axis ? ix - o.arr_col_off : iy - o.arr_row_off;
+      if (i < 0 || i >= o.arr_cols) return o.outside_zero ? make_float2(0.f, 0.f) : v;
+      return cmul(v, ((const float2*)o.arr0)[(long long)plane * o.arr_plane_stride + i]);
+    }
+    case OP_NAMASK: {
+      // ifftshift(|f_grid|^2 <= (NA/lam)^2) of samples.py:276-279
+      const float4 p0 = P[0];
+      const float fy = bin_freq(iy, ny, p0.y), fx = bin_freq(ix, nx, p0.z);
+      const float l2 = __fadd_rn(__fmul_rn(fy, fy), __fmul_rn(fx, fx));
+      return (l2 <= p0.x) ? v : make_float2(0.f, 0.f);
+    }
+    case OP_SASDH: {
+      // compute_sas_precompensation (src/chromatix/functional/propagation.py:85-103),
+      // ifftshifted: W * exp(i kz (H_AS - H_Fr)), s = fl(fl(lam f) / n)
+      const float4 p0 = P[0], p1 = P[1];
+      const float fy = bin_freq(iy, ny, p1.x), fx = bin_freq(ix, nx, p1.y);
+      const float sy = __fdiv_rn(__fmul_rn(p0.z, fy), p0.w);
+      const float sx = __fdiv_rn(__fmul_rn(p0.z, fx), p0.w);
+      const float sy2 = __fmul_rn(sy, sy), sx2 = __fmul_rn(sx, sx);
+      const float ty = __fadd_rn(p1.z, fabsf(sy)), tx = __fadd_rn(p1.w, fabsf(sx));
+      const bool wy = __fmul_rn(sy2, __fadd_rn(2.0f, __fdiv_rn(1.0f, __fmul_rn(ty, ty)))) <= 1.0f;
+      const bool wx = __fmul_rn(sx2, __fadd_rn(2.0f, __fdiv_rn(1.0f, __fmul_rn(tx, tx)))) <= 1.0f;
+      if (!(wy && wx)) return make_float2(0.f, 0.f);
+      const float ssum = __fadd_rn(sy2, sx2);
+      float s, c;
+      if (o.mode == CXM_PHASE_REF32) {
+        const float has = __fsqrt_rn(fmaxf(0.0f, __fsub_rn(1.0f, ssum)));
+        const float hfr = __fsub_rn(1.0f, __fdiv_rn(ssum, 2.0f));
+        sincos_exact(__fmul_rn(p0.x, __fsub_rn(has, hfr)), &s, &c);
+      } else {
+        // H_AS - H_Fr = -(q / (1 + sqrt(1 - q)))^2 / 2 without cancellation;
+        // for q > 1 H_AS is clamped to 0 as in the reference
+        float d;
+        if (ssum <= 1.0f) {
+          const float u = ssum / (1.0f + sqrtf(1.0f - ssum));
+          d = -0.5f * u * u;
+        } else {
+          d = -(1.0f - 0.5f * ssum);
+        }
+        const float tr = df_frac(df_mul_f(df_make(p0.x, p0.y), d));
+        sincospif(2.0f * tr, &s, &c);
+      }
+      return cmul(v, make_float2(c, s));
+    }
+    default:
+      return v;
+  }
+}
+
+CXM_DEV float2 apply_ops(const OpList& L, const OpTable& T, float2 v, int plane, int iy, int ix,
+                         int ny, int nx) {
+  for (int i = 0; i < L.n; ++i) v = apply_op(L.op[i], T, v, plane, iy, ix, ny, nx);
+  return v;
+}
+
+}  // namespace cxm

@@ -1,0 +1,240 @@
+// Generic one-pass FFT pipelines over rows or column strips of a batch of
+// planes.  A pass loads one line (zero padding and ifftshift folded into the
+// addressing), applies `pre` operators, runs one transform or a transform
+// pair with `mid` operators in between, applies `post` operators and stores
+// the requested window (fftshift and crop folded into the addressing), as
+// complex64 or |u|^2.  Every multi-step operation of the propagation path
+// (optical FFT, transform/SAS propagation, chirp-z transforms, the fused
+// inverse-multiply-forward step of the multislice loop) is a chain of these
+// passes; see the entry points in cxm_pipe.hip.
+#pragma once
+#include <string>
+
+#include "cxm_fft.h"
+#include "cxm_internal.h"
+#include "cxm_ops.h"
+#include "cxm_sizes.h"
+
+namespace cxm {
+
+enum Seq : int { SEQ_F = 0, SEQ_I = 1, SEQ_FI = 2, SEQ_IF = 3 };
+
+struct PipeArgs {
+  const float2* in;           // null: the loaded value is 1 (a generated source)
+  void* out;                  // complex64, or float32 when intensity
+  const float2* tw;
+  long long in_plane_stride, out_plane_stride;  // elements
+  int in_line_stride;         // row kernel: elements between input rows; col kernel: between rows too
+  int out_line_stride;
+  int n_lines;                // rows (row pass) or columns (column pass) per plane
+  int n_planes, plane0;       // planes in this launch; global index of the first
+  int grid_ny, grid_nx;       // FFT grid for operator coordinates
+  int line_off;               // grid index of line 0 (row index / column index)
+  int in_off, in_len, in_shift;     // along the transform axis
+  int out_off, out_len, out_shift;  // along the transform axis
+  int out_line_shift;         // column pass: store grid column c at fshift_dst(c)
+  int out_line_off, out_line_len;   // lines kept: grid lines [off, off + len)
+  int seq, mid_shift, intensity;
+  float scale, mid_scale;
+  OpList pre, mid, post;
+  OpTable tab;
+};
+
+// Rotate the register layout by N/2 (even N, even E): applies ifftshift to
+// the sequence held in the v[k] <-> X[t + kT] layout.
+template <int E>
+CXM_DEV void rotate_half(float2 (&v)[E]) {
+  static_assert(E % 2 == 0, "rotate_half needs an even E");
+  float2 w[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) w[k] = v[(k + E / 2) % E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) v[k] = w[k];
+}
+
+template <int N, int E, int S, bool SYNC>
+CXM_DEV void run_seq(const PipeArgs& a, float2 (&v)[E], float2* lds, int t, int plane, int line_grid,
+                     bool is_row) {
+  constexpr int T = N / E;
+  if (a.seq == SEQ_F || a.seq == SEQ_FI) fft<N, E, -1, S, SYNC>(v, lds, t, a.tw);
+  else fft<N, E, 1, S, SYNC>(v, lds, t, a.tw);
+  if (a.seq == SEQ_FI || a.seq == SEQ_IF) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int j = t + k * T;
+      float2 x = cscale(v[k], a.mid_scale);
+      if (a.mid.n) {
+        x = is_row ? apply_ops(a.mid, a.tab, x, plane, line_grid, j, a.grid_ny, a.grid_nx)
+                   : apply_ops(a.mid, a.tab, x, plane, j, line_grid, a.grid_ny, a.grid_nx);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      v[k] = x;
+    }
+    if constexpr (E % 2 == 0) {
+      if (a.mid_shift) rotate_half<E>(v);
+    }
+    if (a.seq == SEQ_FI) fft<N, E, 1, S, SYNC>(v, lds, t, a.tw);
+    else fft<N, E, -1, S, SYNC>(v, lds, t, a.tw);
+  }
+}
+
+// ---- row pass -----------------------------------------------------------------
+template <int N, int E>
+struct PipeRowCfg {
+  static constexpr int T = N / E;
+  static constexpr int G = (T >= 256) ? 1 : 256 / T;
+  static constexpr int THREADS = G * T;
+};
+
+template <int N, int E>
+__global__ void __launch_bounds__((PipeRowCfg<N, E>::THREADS))
+pipe_row_kernel(PipeArgs a) {
+  using Cfg = PipeRowCfg<N, E>;
+  constexpr int T = Cfg::T;
+  constexpr int SPAN = lds_span<N, 1>();
+  __shared__ float2 lds[Cfg::G * SPAN];
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x - g * T;
+  const long long line = (long long)blockIdx.x * Cfg::G + g;
+  const bool ok = line < (long long)a.n_planes * a.n_lines;
+  const int p = ok ? (int)(line / a.n_lines) : 0;
+  const int r = ok ? (int)(line - (long long)p * a.n_lines) : 0;
+  const int plane = a.plane0 + p;
+  const int yg = a.line_off + r;
+  const float2* src = a.in ? a.in + p * a.in_plane_stride + (long long)r * a.in_line_stride : nullptr;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = t + k * T;
+    const int xf = a.in_shift ? ishift_src(i, N) : i;
+    const int xs = xf - a.in_off;
+    const bool in = ok && xs >= 0 && xs < a.in_len;
+    float2 x = in ? (src ? src[xs] : make_float2(1.f, 0.f)) : make_float2(0.f, 0.f);
+    if (a.pre.n && in) {
+      x = apply_ops(a.pre, a.tab, x, plane, yg, xf, a.grid_ny, a.grid_nx);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    v[k] = x;
+  }
+  run_seq<N, E, 1, true>(a, v, lds + g * SPAN, t, plane, yg, true);
+  if (!ok) return;
+  const long long obase = p * a.out_plane_stride + (long long)r * a.out_line_stride;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int j = t + k * T;
+    const int xo = a.out_shift ? fshift_dst(j, N) : j;
+    const int xl = xo - a.out_off;
+    if (xl >= 0 && xl < a.out_len) {
+      float2 x = cscale(v[k], a.scale);
+      if (a.post.n) x = apply_ops(a.post, a.tab, x, plane, yg, xo, a.grid_ny, a.grid_nx);
+      if (a.intensity) ((float*)a.out)[obase + xl] = cabs2(x);
+      else ((float2*)a.out)[obase + xl] = x;
+    }
+  }
+}
+
+// ---- column pass --------------------------------------------------------------
+template <int N, int E, int C>
+struct PipeColCfg {
+  static constexpr int T = N / E;
+  static constexpr int THREADS = T * C;
+};
+
+template <int N, int E, int C>
+__global__ void __launch_bounds__((PipeColCfg<N, E, C>::THREADS))
+pipe_col_kernel(PipeArgs a) {
+  constexpr int T = N / E;
+  __shared__ float2 lds[N * C];
+  const int c = threadIdx.x % C;
+  const int ty = threadIdx.x / C;
+  const int nstrips = (a.n_lines + C - 1) / C;
+  const int nb = gridDim.x;
+  const int L = ((nb & 7) == 0) ? (int)((blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3)) : (int)blockIdx.x;
+  const int p = L / nstrips;
+  const int col = (L - p * nstrips) * C + c;
+  const bool colok = col < a.n_lines;
+  const int plane = a.plane0 + p;
+  const int xg = a.line_off + (colok ? col : 0);
+  const float2* src = a.in ? a.in + p * a.in_plane_stride + col : nullptr;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = ty + k * T;
+    const int yf = a.in_shift ? ishift_src(i, N) : i;
+    const int ys = yf - a.in_off;
+    const bool in = colok && ys >= 0 && ys < a.in_len;
+    float2 x = in ? (src ? src[(long long)ys * a.in_line_stride] : make_float2(1.f, 0.f))
+                  : make_float2(0.f, 0.f);
+    if (a.pre.n && in) {
+      x = apply_ops(a.pre, a.tab, x, plane, yf, xg, a.grid_ny, a.grid_nx);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    v[k] = x;
+  }
+  run_seq<N, E, C, true>(a, v, lds + c, ty, plane, xg, false);
+  if (!colok) return;
+  const int xo = a.out_line_shift ? fshift_dst(xg, a.grid_nx) : xg;
+  const int xl = xo - a.out_line_off;
+  if (xl < 0 || xl >= a.out_line_len) return;
+  const long long obase = p * a.out_plane_stride + xl;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int j = ty + k * T;
+    const int yo = a.out_shift ? fshift_dst(j, N) : j;
+    const int yl = yo - a.out_off;
+    if (yl >= 0 && yl < a.out_len) {
+      float2 x = cscale(v[k], a.scale);
+      if (a.post.n) x = apply_ops(a.post, a.tab, x, plane, yo, xo, a.grid_ny, a.grid_nx);
+      if (a.intensity) ((float*)a.out)[obase + (long long)yl * a.out_line_stride] = cabs2(x);
+      else ((float2*)a.out)[obase + (long long)yl * a.out_line_stride] = x;
+    }
+  }
+}
+
+template <int N, int E>
+int launch_pipe_row_t(const PipeArgs& a, hipStream_t s) {
+  using Cfg = PipeRowCfg<N, E>;
+  const long long lines = (long long)a.n_planes * a.n_lines;
+  const long long blocks = (lines + Cfg::G - 1) / Cfg::G;
+  if (blocks == 0) return CXM_OK;
+  if (blocks > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "row pass grid too large");
+  void* t0 = timing_start(s);
+  hipLaunchKernelGGL((pipe_row_kernel<N, E>), dim3((unsigned)blocks), dim3(Cfg::THREADS), 0, s, a);
+  CXM_LAUNCH_CHECK();
+  if (t0) timing_stop(t0, s, "pipe_row_" + std::to_string(N));
+  return CXM_OK;
+}
+
+template <int N, int E, int C>
+int launch_pipe_col_t(const PipeArgs& a, hipStream_t s) {
+  using Cfg = PipeColCfg<N, E, C>;
+  const long long nstrips = (a.n_lines + C - 1) / C;
+  const long long blocks = nstrips * a.n_planes;
+  if (blocks == 0) return CXM_OK;
+  if (blocks > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "column pass grid too large");
+  void* t0 = timing_start(s);
+  hipLaunchKernelGGL((pipe_col_kernel<N, E, C>), dim3((unsigned)blocks), dim3(Cfg::THREADS), 0, s, a);
+  CXM_LAUNCH_CHECK();
+  if (t0) timing_stop(t0, s, "pipe_col_" + std::to_string(N));
+  return CXM_OK;
+}
+
+}  // namespace cxm
+
+#define CXM_DEFINE_PIPE_GROUP(K, SIZES)                                       \
+  namespace cxm {                                                             \
+  int launch_pipe_row_g##K(int N, const PipeArgs& a, hipStream_t s) {         \
+    switch (N) {                                                              \
+      SIZES(CXM_PROW_CASE)                                                    \
+      default: return -1;                                                     \
+    }                                                                         \
+  }                                                                           \
+  int launch_pipe_col_g##K(int N, const PipeArgs& a, hipStream_t s) {         \
+    switch (N) {                                                              \
+      SIZES(CXM_PCOL_CASE)                                                    \
+      default: return -1;                                                     \
+    }                                                                         \
+  }                                                                           \
+  }
+#define CXM_PROW_CASE(NN, ER, EC, CC) case NN: return launch_pipe_row_t<NN, ER>(a, s);
+#define CXM_PCOL_CASE(NN, ER, EC, CC) case NN: return launch_pipe_col_t<NN, EC, CC>(a, s);

@@ -101,10 +101,10 @@ def test_transfer_kykx():
 
 def test_z_batch_outer_and_zip():
     zs = np.array([-20.0, 5.0, 12.5, 40.0])
-    u = _phase_field((96, 64), seed=5)
+    u = _phase_field((112, 48), seed=5)
     f = _mono(u, 0.3)
     out = cf.asm_propagate(f, zs, NM, 8, mode="same")
-    assert out.shape == (4, 96, 64)
+    assert out.shape == (4, 112, 48)
     ref = np.stack([_oracle(u, 0.3, z, 8, "asm", "same", "ref32") for z in zs])
     assert rel_l2(out.u.cpu().numpy(), ref) < TOL
     # zip: innermost batch equals the number of distances
