@@ -89,6 +89,7 @@ def _declare(lib):
         "cxm_prop2d_workspace_size": (sz, [ctypes.POINTER(Prop2dDesc)]),
         "cxm_prop2d_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Prop2dDesc)]),
         "cxm_propagator_c64": (i32, [vp, vp, ctypes.POINTER(Prop2dDesc)]),
+        "cxm_set_option": (i32, [i32, i32]),
         "cxm_timing_enable": (i32, [i32]),
         "cxm_timing_collect": (i32, []),
         "cxm_timing_get": (i32, [i32, ctypes.c_char_p, i32, dp, ctypes.POINTER(ctypes.c_int64)]),

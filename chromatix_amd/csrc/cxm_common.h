@@ -39,6 +39,11 @@ template <int DIR> CXM_DEV float2 mul_i(float2 a) {
   else return make_float2(-a.y, a.x);
 }
 CXM_DEV float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
+// Correctly rounded float32 square root.  (HIP's __fsqrt_rn maps to the
+// approximate native sqrt unless OCML rounded operations are enabled; the
+// builtin is correctly rounded under HIP's default
+// -fhip-fp32-correctly-rounded-divide-sqrt.)
+CXM_DEV float sqrt_rn(float x) { return __builtin_sqrtf(x); }
 
 // ---------------------------------------------------------------------------
 // Double-float ("df") arithmetic: a value is hi + lo with |lo| <= ulp(hi)/2.
@@ -89,7 +94,7 @@ CXM_DEV df df_mul_f(df a, float b) {
 // sqrt of a non-negative df: one Newton/Karp step on the f32 root.
 CXM_DEV df df_sqrt(df a) {
   if (a.hi <= 0.0f) return df_make(0.0f, 0.0f);
-  float s = __fsqrt_rn(a.hi);
+  float s = sqrt_rn(a.hi);
   df s2 = two_prod(s, s);
   float r = __fadd_rn(__fsub_rn(__fsub_rn(a.hi, s2.hi), s2.lo), a.lo);
   float c = __fdiv_rn(r, __fmul_rn(2.0f, s));
@@ -111,15 +116,26 @@ CXM_DEV int ishift_src(int i, int n) { int j = i + (n >> 1); return j >= n ? j -
 // numpy fftshift: out[i] = X[(i - n/2) mod n] (n/2 = floor), so bin j lands
 // at dst = (j + n/2) % n.
 CXM_DEV int fshift_dst(int j, int n) { int d = j + (n >> 1); return d >= n ? d - n : d; }
-// Twiddle families: FFT lengths 2^a, 3*2^a and 5*2^a each read one FP64-built
-// table W_L[i] = exp(-2 pi i i / L) of the family's largest length L; a
-// length-N transform reads every (L/N)-th entry.
+// Twiddle tables: one compact table per implemented FFT length N, holding
+// W_N[i] = exp(-2 pi i i / N) for i < N (FP64-evaluated, rounded once to
+// f32), so that a transform's twiddle reads stay within N * 8 bytes (L1
+// resident).  The tables are concatenated in the order of kTwLengths.
 namespace cxm {
-constexpr int kTwPow2 = 16384;
-constexpr int kTw3 = 3 * 4096;
-constexpr int kTw5 = 5 * 2048;
-constexpr int tw_family_len(int N) {
-  return (N % 5 == 0) ? kTw5 : (N % 3 == 0) ? kTw3 : kTwPow2;
+constexpr int kTwLengths[] = {2,  4,  8,  16,  32,  64,  128,  256,  512,  1024, 2048, 4096, 8192,
+                              5,  10, 20, 40,  80,  160, 320,  640,  1280, 2560, 5120};
+constexpr int kTwCount = (int)(sizeof(kTwLengths) / sizeof(kTwLengths[0]));
+constexpr int tw_offset(int N) {
+  int off = 0;
+  for (int i = 0; i < kTwCount; ++i) {
+    if (kTwLengths[i] == N) return off;
+    off += kTwLengths[i];
+  }
+  return -1;
+}
+constexpr int tw_total() {
+  int off = 0;
+  for (int i = 0; i < kTwCount; ++i) off += kTwLengths[i];
+  return off;
 }
 }  // namespace cxm
 

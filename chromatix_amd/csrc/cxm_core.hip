@@ -15,7 +15,7 @@ namespace cxm {
 
 namespace {
 thread_local std::string g_last_error;
-constexpr int kTwTotal = kTwPow2 + kTw3 + kTw5;
+constexpr int kTwTotal = tw_total();
 constexpr int kMaxDevices = 64;
 std::mutex g_init_mu;
 bool g_ready[kMaxDevices] = {};
@@ -27,14 +27,15 @@ __device__ __align__(256) float2 g_twiddles[kTwTotal];
 __global__ void init_twiddles_kernel(float2* tw) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= kTwTotal) return;
-  int L, k;
-  if (i < kTwPow2) { L = kTwPow2; k = i; }
-  else if (i < kTwPow2 + kTw3) { L = kTw3; k = i - kTwPow2; }
-  else { L = kTw5; k = i - kTwPow2 - kTw3; }
+  int off = 0, L = 1;
+  for (int j = 0; j < kTwCount; ++j) {
+    if (i < off + kTwLengths[j]) { L = kTwLengths[j]; break; }
+    off += kTwLengths[j];
+  }
+  const int k = i - off;
   // exp(-2 pi i k / L) in FP64, rounded once to f32
-  const double a = -2.0 * (double)k / (double)L;
   double s, c;
-  sincospi(a, &s, &c);
+  sincospi(-2.0 * (double)k / (double)L, &s, &c);
   tw[i] = make_float2((float)c, (float)s);
 }
 
@@ -77,10 +78,8 @@ int ensure_init() {
 const float2* twiddles_for(int n) {
   int dev = 0;
   (void)hipGetDevice(&dev);
-  const float2* base = g_tw_ptr[dev];
-  if (n % 5 == 0) return base + kTwPow2 + kTw3;
-  if (n % 3 == 0) return base + kTwPow2;
-  return base;
+  const int off = tw_offset(n);
+  return off < 0 ? nullptr : g_tw_ptr[dev] + off;
 }
 
 }  // namespace cxm
@@ -168,5 +167,19 @@ extern "C" int cxm_timing_get(int i, char* name, int name_len, double* total_ms,
   }
   if (total_ms) *total_ms = e.second.first;
   if (launches) *launches = e.second.second;
+  return CXM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Tuning options (launch-shape variants of the kernels; see cxm.h).
+// ---------------------------------------------------------------------------
+namespace cxm {
+namespace { int g_opts[16] = {0}; }
+int option(int key) { return (key >= 0 && key < 16) ? g_opts[key] : 0; }
+}  // namespace cxm
+
+extern "C" int cxm_set_option(int key, int value) {
+  if (key < 0 || key >= 16) return cxm::fail(CXM_ERR_INVALID, "unknown option");
+  cxm::g_opts[key] = value;
   return CXM_OK;
 }

@@ -15,8 +15,8 @@
 // W_{Ns*R}^{(j mod Ns) * r}; outputs go to (j / Ns) * Ns * R + j mod Ns + r*Ns.
 // Each radix must divide E so that every thread owns whole butterflies.
 //
-// Twiddles come from the FP64-built family table of cxm_common.h, rounded
-// once to f32.
+// Twiddles come from the per-length FP64-built table of cxm_common.h,
+// rounded once to f32.
 #pragma once
 #include "cxm_common.h"
 
@@ -208,7 +208,7 @@ CXM_DEV void stockham_stage(float2 (&v)[E], float2* lds, int t, const float2* __
   constexpr int T = N / E;
   constexpr int M = E / R;
   constexpr bool last = (Ns * R == N);
-  constexpr int twstep = (N / (Ns * R)) * (tw_family_len(N) / N);
+  constexpr int twstep = N / (Ns * R);
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int j = t + m * T;
@@ -217,9 +217,10 @@ CXM_DEV void stockham_stage(float2 (&v)[E], float2* lds, int t, const float2* __
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = v[m + r * M];
     if constexpr (Ns > 1) {
+      const float2* twp = tw;
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        float2 w = tw[k * r * twstep];
+        float2 w = twp[k * r * twstep];
         if constexpr (DIR > 0) w.y = -w.y;
         a[r] = cmul(a[r], w);
       }

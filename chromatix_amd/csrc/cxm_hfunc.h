@@ -68,10 +68,10 @@ CXM_DEV float2 eval_H(const HParam& hp, const XTerms& xt, int ky, int Ny, int ki
       const float kern = __fsub_rn(1.0f, __fmul_rn(hp.c_l2, l2));
       const float osh = __fmul_rn(kTwoPiF, __fadd_rn(__fmul_rn(fy, hp.sy), __fmul_rn(xt.fx, hp.sx)));
       if (kern >= 0.0f || remove_evanescent) {
-        const float d = __fsqrt_rn(fmaxf(kern, 0.0f));
+        const float d = sqrt_rn(fmaxf(kern, 0.0f));
         re = __fadd_rn(__fmul_rn(hp.c_ph, d), osh);
       } else {
-        const float dim = __fsqrt_rn(-kern);
+        const float dim = sqrt_rn(-kern);
         amp = scale * expf(-__fmul_rn(hp.c_ph, dim));
         re = osh;
       }

@@ -29,6 +29,9 @@ int fail(int code, const std::string& msg);
                                           hipGetErrorString(_e));             \
   } while (0)
 
+// ----- tuning options (cxm_set_option) ----------------------------------------
+int option(int key);
+
 // ----- optional kernel timing (HIP events on the launch stream) --------------
 bool timing_on();
 void* timing_start(hipStream_t s);

@@ -164,7 +164,7 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
       const float ssum = __fadd_rn(sy2, sx2);
       float s, c;
       if (o.mode == CXM_PHASE_REF32) {
-        const float has = __fsqrt_rn(fmaxf(0.0f, __fsub_rn(1.0f, ssum)));
+        const float has = sqrt_rn(fmaxf(0.0f, __fsub_rn(1.0f, ssum)));
         const float hfr = __fsub_rn(1.0f, __fdiv_rn(ssum, 2.0f));
         sincos_exact(__fmul_rn(p0.x, __fsub_rn(has, hfr)), &s, &c);
       } else {

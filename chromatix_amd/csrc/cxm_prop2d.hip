@@ -41,8 +41,14 @@ int launch_row(int N, int dir, const RowArgs& a, hipStream_t s) {
   return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
 }
 
+int launch_col_variant_1024(int v, const ColArgs& a, hipStream_t s);
+int launch_col_variant_2048(int v, const ColArgs& a, hipStream_t s);
+
 int launch_col_prop(int N, const ColArgs& a, hipStream_t s) {
   int rc;
+  const int v = option(CXM_OPT_COL_VARIANT);
+  if (v > 0 && N == 1024 && (rc = launch_col_variant_1024(v, a, s)) >= 0) return rc;
+  if (v > 0 && N == 2048 && (rc = launch_col_variant_2048(v, a, s)) >= 0) return rc;
   if ((rc = launch_col_g0(N, a, s)) >= 0) return rc;
   if ((rc = launch_col_g1(N, a, s)) >= 0) return rc;
   if ((rc = launch_col_g2(N, a, s)) >= 0) return rc;

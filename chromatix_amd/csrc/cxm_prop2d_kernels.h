@@ -87,23 +87,28 @@ col_prop_kernel(ColArgs a) {
   const int nstrips = (a.ncols + C - 1) / C;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
   const int p = L / nstrips;
-  const int col = (L - p * nstrips) * C + c;
+  const int col0 = (L - p * nstrips) * C;
+  const int col = col0 + c;
   const bool colok = col < a.ncols;
-  const float2* src = a.in + p * a.in_plane_stride + col;
-  float2 v[E];
-#pragma unroll
-  for (int k = 0; k < E; ++k) {
-    const int ys = ty + k * T - a.in_off;
-    v[k] = (colok && ys >= 0 && ys < a.in_len) ? src[(long long)ys * a.row_stride]
-                                              : make_float2(0.f, 0.f);
-  }
-  fft<N, E, -1, C, true>(v, lds + c, ty, a.tw);
+  // workgroup-uniform strip base + 32-bit lane offsets (SGPR-base addressing)
+  const float2* src = a.in + p * a.in_plane_stride + col0;
   const int pg = a.plane0 + p;
   const int w = (a.nwv > 1) ? (pg % a.nwv) : 0;
   const int kx = colok ? col : 0;
   const int nout = a.z_zip ? 1 : a.nz;
+  // One register set per pass: with several distances the strip is
+  // reloaded (from cache) and transformed again per distance, rather than
+  // keeping the forward spectrum alive next to the working copy.
   for (int iz = 0; iz < nout; ++iz) {
     float2 u[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int ys = ty + k * T - a.in_off;
+      u[k] = (colok && ys >= 0 && ys < a.in_len)
+                 ? *(const float2*)((const char*)src + (unsigned)(ys * a.row_stride + c) * 8u)
+                 : make_float2(0.f, 0.f);
+    }
+    fft<N, E, -1, C, true>(u, lds + c, ty, a.tw);
     if constexpr (KIND == CXM_H_TRANSFER || KIND == CXM_H_ASM) {
       const int zi = a.z_zip ? (pg / a.nwv) % a.nz : iz;
       const HParam& hp = a.hp[w * a.nz + zi];
@@ -112,7 +117,10 @@ col_prop_kernel(ColArgs a) {
       for (int k = 0; k < E; ++k) {
         const float2 h = eval_H(hp, xt, ty + k * T, N, KIND, MODE, a.remove_evanescent,
                                 a.bandlimit, a.scale);
-        u[k] = cmul(v[k], h);
+        u[k] = cmul(u[k], h);
+        // one H evaluation per scheduling region: the unrolled evaluations
+        // are not interleaved into a register-spilling schedule
+        __builtin_amdgcn_sched_barrier(0);
       }
     } else {
       if (a.kind == CXM_H_ARRAY) {
@@ -120,20 +128,21 @@ col_prop_kernel(ColArgs a) {
         for (int k = 0; k < E; ++k) {
           const float2 h = colok ? a.H[(long long)(ty + k * T) * a.ncols + kx]
                                  : make_float2(0.f, 0.f);
-          u[k] = cscale(cmul(v[k], h), a.scale);
+          u[k] = cscale(cmul(u[k], h), a.scale);
         }
       } else {
 #pragma unroll
-        for (int k = 0; k < E; ++k) u[k] = cscale(v[k], a.scale);
+        for (int k = 0; k < E; ++k) u[k] = cscale(u[k], a.scale);
       }
     }
     fft<N, E, 1, C, true>(u, lds + c, ty, a.tw);
     if (colok) {
-      float2* dst = a.out + ((long long)p * nout + iz) * a.out_plane_stride + col;
+      float2* dst = a.out + ((long long)p * nout + iz) * a.out_plane_stride + col0;
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const int yo = ty + k * T - a.out_off;
-        if (yo >= 0 && yo < a.out_len) dst[(long long)yo * a.row_stride] = u[k];
+        if (yo >= 0 && yo < a.out_len)
+          *(float2*)((char*)dst + (unsigned)(yo * a.row_stride + c) * 8u) = u[k];
       }
     }
   }

@@ -112,6 +112,78 @@ int cxm_prop2d_c64(void* stream, const void* in, void* out, void* workspace,
 int cxm_propagator_c64(void* stream, void* out, const cxm_prop2d_desc* d);
 
 /*
+ * Lens / far-field FFT family on a batch of planes (two passes: rows, then
+ * column strips; shifts, chirps, norms, pupils folded into the passes).
+ *
+ *   CXM_OFFT_OPTICAL   optical_fft  (src/chromatix/functional/convenience.py:9-59):
+ *                      z >= 0: fftshift(fft2(ifftshift(-i dy dx / L^2 * u)))
+ *                      z <  0: conj(fftshift(fft2(ifftshift(-i L^2/(duy dux NyNx) conj(u)))))
+ *                      with L^2 = lam z / n; optional circular pupil of diameter
+ *                      pupil_d scaled by central_wavelength/lam before it (ff_lens with
+ *                      NA, src/chromatix/functional/lenses.py:64-95).
+ *   CXM_OFFT_TRANSFORM transform_propagate (src/chromatix/functional/propagation.py:34-82):
+ *                      zero pad, input chirp exp(i pi |x|^2/L^2), 1j * optical_fft,
+ *                      output chirp on the new grid, crop.
+ *   CXM_OFFT_RAW_FWD / CXM_OFFT_RAW_INV  fft2 / ifft2 (utils/fft.py:8-35), with
+ *                      `shift` = 1 for fftshift(f(ifftshift(x))).
+ *
+ * in : batch x height x width complex64; plane p uses wavelength p % num_wv.
+ * out: batch x (height + 2 pad_y - 2 crop_y) x (width + 2 pad_x - 2 crop_x),
+ *      complex64 or float32 |u|^2.
+ */
+#define CXM_OFFT_OPTICAL 0
+#define CXM_OFFT_TRANSFORM 1
+#define CXM_OFFT_RAW_FWD 2
+#define CXM_OFFT_RAW_INV 3
+
+typedef struct cxm_optfft_desc {
+  int32_t batch;
+  int32_t height, width;
+  int32_t pad_y, pad_x;
+  int32_t crop_y, crop_x;
+  int32_t num_wv;
+  int32_t phase_mode;         /* CXM_PHASE_* for the chirps */
+  int32_t out_kind;           /* CXM_OUT_* */
+  int32_t mode;               /* CXM_OFFT_* */
+  int32_t shift;              /* RAW modes only */
+  int32_t skip_initial_phase; /* TRANSFORM */
+  int32_t skip_final_phase;   /* TRANSFORM */
+  int32_t pupil;              /* OPTICAL: apply the circular pupil */
+  int32_t reserved;
+  double z, n, pupil_d, central_wavelength;
+  double wavelength;          /* used when wavelengths == NULL */
+  double dy, dx;              /* used when spacing == NULL */
+  double oy, ox;              /* grid origin, used when origin == NULL */
+  const double* wavelengths;  /* num_wv, host */
+  const double* spacing;      /* num_wv x (dy, dx), host */
+  const double* origin;       /* num_wv x (oy, ox), host */
+} cxm_optfft_desc;
+
+size_t cxm_optical_fft_workspace_size(const cxm_optfft_desc* d);
+int cxm_optical_fft_c64(void* stream, const void* in, void* out, void* workspace,
+                        size_t workspace_bytes, const cxm_optfft_desc* d);
+
+/*
+ * Scalable angular spectrum propagation, transform_propagate_sas
+ * (src/chromatix/functional/propagation.py:85-165): pad by half the size on
+ * each side, z >= 0: kernel_propagate with the precompensation dH then
+ * transform_propagate; z < 0: the reverse order; crop.  Uses the
+ * cxm_optfft_desc fields batch, height, width, num_wv, wavelength(s),
+ * spacing, origin, z, n, phase_mode, out_kind, skip_*_phase (pads/crops are
+ * implied).  Four passes (z >= 0) or five (z < 0).
+ */
+size_t cxm_sas_workspace_size(const cxm_optfft_desc* d);
+int cxm_sas_c64(void* stream, const void* in, void* out, void* workspace,
+                size_t workspace_bytes, const cxm_optfft_desc* d);
+
+/*
+ * The SAS precompensation itself, materialised (num_wv x Ny x Nx complex64,
+ * FFT order) for the grid described by d (height/width = the padded grid):
+ * compute_sas_precompensation (src/chromatix/functional/propagation.py:85-103).
+ */
+int cxm_sas_precompensation_c64(void* stream, void* out, const cxm_optfft_desc* d);
+
+/*
  * Intensity |u|^2 of n_pixels complex64 values, or for an innermost
  * wavelength axis of length num_wv the density-weighted sum over it:
  *   out[p] = sum_w density[w] |u[p*num_wv + w]|^2   (float32 out)
@@ -144,6 +216,14 @@ int cxm_plane_sumsq_c64(void* stream, const void* u, void* out, int64_t n_planes
 int cxm_timing_enable(int on);
 int cxm_timing_collect(void);
 int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* launches);
+
+/*
+ * Launch-shape tuning.  CXM_OPT_COL_VARIANT selects an alternative column
+ * pass shape for 1024- and 2048-point columns (0 = default).  No
+ * counterpart in the reference (performance tuning).
+ */
+#define CXM_OPT_COL_VARIANT 1
+int cxm_set_option(int key, int value);
 
 /* Library state. */
 int cxm_init(int device);            /* build twiddle tables on `device` */

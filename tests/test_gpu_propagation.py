@@ -108,10 +108,10 @@ def test_z_batch_outer_and_zip():
     ref = np.stack([_oracle(u, 0.3, z, 8, "asm", "same", "ref32") for z in zs])
     assert rel_l2(out.u.cpu().numpy(), ref) < TOL
     # zip: innermost batch equals the number of distances
-    ub = _phase_field((96, 64), seed=6, batch=(2, 4))
+    ub = _phase_field((128, 64), seed=6, batch=(2, 4))
     fb = _mono(ub, 0.3)
     outb = cf.transfer_propagate(fb, zs, NM, 0, mode="same")
-    assert outb.shape == (2, 4, 96, 64)
+    assert outb.shape == (2, 4, 128, 64)
     refb = np.stack([[_oracle(ub[a, k], 0.3, zs[k], 0, "transfer", "same", "ref32") for k in range(4)]
                      for a in range(2)])
     assert rel_l2(outb.u.cpu().numpy(), refb) < TOL
@@ -157,14 +157,14 @@ def test_chromatic_z_batch():
 
 
 def test_materialised_propagator_matches_oracle():
-    f = _mono(_phase_field((96, 80)), 0.2)
+    f = _mono(_phase_field((128, 80)), 0.2)
     for pm, prec in (("ref32", "ref32"), ("accurate", "fp64")):
         H = cf.compute_asm_propagator(f, 55.0, NM, shift_yx=(1.0, -2.0), phase_mode=pm).cpu().numpy()
-        Hr = op.asm_propagator((96, 80), (0.2, 0.2), LAM, 55.0, NM, shift_yx=(1.0, -2.0), precision=prec)
+        Hr = op.asm_propagator((128, 80), (0.2, 0.2), LAM, 55.0, NM, shift_yx=(1.0, -2.0), precision=prec)
         assert np.max(np.abs(H - Hr)) < 2e-6
         Ht = cf.compute_transfer_propagator(f, [3.0, -8.0], NM, phase_mode=pm).cpu().numpy()
-        Htr = op.transfer_propagator((96, 80), (0.2, 0.2), LAM, [3.0, -8.0], NM, precision=prec)
-        assert Ht.shape == (2, 96, 80)
+        Htr = op.transfer_propagator((128, 80), (0.2, 0.2), LAM, [3.0, -8.0], NM, precision=prec)
+        assert Ht.shape == (2, 128, 80)
         assert np.max(np.abs(Ht - Htr)) < 2e-6
 
 

@@ -16,7 +16,7 @@ import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
 SIZES_H = ROOT / "chromatix_amd" / "csrc" / "cxm_sizes.h"
-TW_LEN = {"pow2": 16384, "3": 3 * 4096, "5": 5 * 2048}
+
 
 
 def configured_sizes():
@@ -29,10 +29,6 @@ def pick_radix(rem, E):
         if rem % r == 0 and E % r == 0:
             return r
     return -1
-
-
-def family_len(N):
-    return TW_LEN["5"] if N % 5 == 0 else TW_LEN["3"] if N % 3 == 0 else TW_LEN["pow2"]
 
 
 def small_dft(a, R, direction):
@@ -70,7 +66,7 @@ def lds_at(e, S):
 
 def emulate_fft(x, N, E, direction, S=1):
     T = N // E
-    L = family_len(N)
+    L = N  # one table of length N per transform length
     tw = np.exp(-2j * np.pi * np.arange(L) / L)
     span = N + N // 16 if S == 1 else N * S
     v = np.array([[x[t + k * T] for k in range(E)] for t in range(T)], dtype=complex)
@@ -80,7 +76,7 @@ def emulate_fft(x, N, E, direction, S=1):
         assert R > 0, (N, E)
         M = E // R
         last = Ns * R == N
-        twstep = (N // (Ns * R)) * (L // N)
+        twstep = N // (Ns * R)
         lds = np.full(span, np.nan + 0j)
         written = np.zeros(span, int)
         newv = v.copy()
@@ -142,6 +138,7 @@ def test_configured_schedules(cfg):
         np.testing.assert_allclose(inv, np.fft.ifft(x) * N, atol=1e-9 * N)
 
 
-def test_family_tables_cover_all_sizes():
-    for N, *_ in configured_sizes():
-        assert family_len(N) % N == 0, N
+def test_every_size_has_a_twiddle_table():
+    text = (ROOT / "chromatix_amd" / "csrc" / "cxm_common.h").read_text()
+    lengths = {int(v) for v in re.search(r"kTwLengths\[\] = \{([^}]*)\}", text).group(1).replace(" ", "").split(",")}
+    assert {c[0] for c in configured_sizes()} <= lengths
