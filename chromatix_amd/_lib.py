@@ -74,6 +74,45 @@ class Prop2dDesc(ctypes.Structure):
     ]
 
 
+CXM_OFFT_OPTICAL = 0
+CXM_OFFT_TRANSFORM = 1
+CXM_OFFT_RAW_FWD = 2
+CXM_OFFT_RAW_INV = 3
+
+
+class OptFftDesc(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("pad_y", ctypes.c_int32),
+        ("pad_x", ctypes.c_int32),
+        ("crop_y", ctypes.c_int32),
+        ("crop_x", ctypes.c_int32),
+        ("num_wv", ctypes.c_int32),
+        ("phase_mode", ctypes.c_int32),
+        ("out_kind", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("shift", ctypes.c_int32),
+        ("skip_initial_phase", ctypes.c_int32),
+        ("skip_final_phase", ctypes.c_int32),
+        ("pupil", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("z", ctypes.c_double),
+        ("n", ctypes.c_double),
+        ("pupil_d", ctypes.c_double),
+        ("central_wavelength", ctypes.c_double),
+        ("wavelength", ctypes.c_double),
+        ("dy", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("oy", ctypes.c_double),
+        ("ox", ctypes.c_double),
+        ("wavelengths", ctypes.POINTER(ctypes.c_double)),
+        ("spacing", ctypes.POINTER(ctypes.c_double)),
+        ("origin", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -90,6 +129,11 @@ def _declare(lib):
         "cxm_prop2d_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Prop2dDesc)]),
         "cxm_propagator_c64": (i32, [vp, vp, ctypes.POINTER(Prop2dDesc)]),
         "cxm_set_option": (i32, [i32, i32]),
+        "cxm_optical_fft_workspace_size": (sz, [ctypes.POINTER(OptFftDesc)]),
+        "cxm_optical_fft_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(OptFftDesc)]),
+        "cxm_sas_workspace_size": (sz, [ctypes.POINTER(OptFftDesc)]),
+        "cxm_sas_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(OptFftDesc)]),
+        "cxm_sas_precompensation_c64": (i32, [vp, vp, ctypes.POINTER(OptFftDesc)]),
         "cxm_timing_enable": (i32, [i32]),
         "cxm_timing_collect": (i32, []),
         "cxm_timing_get": (i32, [i32, ctypes.c_char_p, i32, dp, ctypes.POINTER(ctypes.c_int64)]),

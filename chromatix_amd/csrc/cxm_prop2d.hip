@@ -112,6 +112,7 @@ int validate(const cxm_prop2d_desc* d, Geom& g) {
                                           : g.plane_ws1 + (long long)nout * g.plane_ws2;
   const long long budget = (96ll << 20) / 8;
   long long ch = std::max(1ll, budget / std::max(1ll, per_plane));
+  if (option(CXM_OPT_CHUNK_PLANES) > 0) ch = option(CXM_OPT_CHUNK_PLANES);
   g.chunk = (int)std::min<long long>(ch, std::max(1, d->batch));
   return CXM_OK;
 }

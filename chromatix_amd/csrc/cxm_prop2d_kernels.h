@@ -75,9 +75,11 @@ CXM_DEV int xcd_remap(int b, int nb) {
   return (b & 7) * (nb >> 3) + (b >> 3);
 }
 
-// KIND: CXM_H_NONE / CXM_H_ARRAY (both handled by KIND == 0 with a runtime
-// switch), CXM_H_TRANSFER, CXM_H_ASM.  MODE: CXM_PHASE_*.
-template <int N, int E, int C, int KIND, int MODE>
+// KIND: CXM_H_NONE, CXM_H_ARRAY, CXM_H_TRANSFER, CXM_H_ASM.  MODE: CXM_PHASE_*.
+// MULTI: several output distances per input plane (runtime loop); the
+// single-distance instantiation has no loop, so nothing loop-invariant is
+// kept live across iterations (about 100 VGPRs instead of 170).
+template <int N, int E, int C, int KIND, int MODE, bool MULTI>
 __global__ void __launch_bounds__((ColCfg<N, E, C>::THREADS))
 col_prop_kernel(ColArgs a) {
   constexpr int T = N / E;
@@ -95,7 +97,7 @@ col_prop_kernel(ColArgs a) {
   const int pg = a.plane0 + p;
   const int w = (a.nwv > 1) ? (pg % a.nwv) : 0;
   const int kx = colok ? col : 0;
-  const int nout = a.z_zip ? 1 : a.nz;
+  const int nout = MULTI ? (a.z_zip ? 1 : a.nz) : 1;
   // One register set per pass: with several distances the strip is
   // reloaded (from cache) and transformed again per distance, rather than
   // keeping the forward spectrum alive next to the working copy.
@@ -122,18 +124,16 @@ col_prop_kernel(ColArgs a) {
         // are not interleaved into a register-spilling schedule
         __builtin_amdgcn_sched_barrier(0);
       }
-    } else {
-      if (a.kind == CXM_H_ARRAY) {
+    } else if constexpr (KIND == CXM_H_ARRAY) {
+      const float2* hcol = a.H + kx;
 #pragma unroll
-        for (int k = 0; k < E; ++k) {
-          const float2 h = colok ? a.H[(long long)(ty + k * T) * a.ncols + kx]
-                                 : make_float2(0.f, 0.f);
-          u[k] = cscale(cmul(u[k], h), a.scale);
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < E; ++k) u[k] = cscale(u[k], a.scale);
+      for (int k = 0; k < E; ++k) {
+        const float2 h = colok ? hcol[(unsigned)((ty + k * T) * a.ncols)] : make_float2(0.f, 0.f);
+        u[k] = cscale(cmul(u[k], h), a.scale);
       }
+    } else {
+#pragma unroll
+      for (int k = 0; k < E; ++k) u[k] = cscale(u[k], a.scale);
     }
     fft<N, E, 1, C, true>(u, lds + c, ty, a.tw);
     if (colok) {
@@ -173,19 +173,26 @@ int launch_col_t(const ColArgs& a, hipStream_t s) {
   if (blocks > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "column pass grid too large");
   const dim3 grid((unsigned)blocks), block(Cfg::THREADS);
   void* t0 = timing_start(s);
+  const bool multi = !a.z_zip && a.nz > 1;
+#define CXM_COL_LAUNCH(K, M)                                                                  \
+  do {                                                                                        \
+    if (multi)                                                                                \
+      hipLaunchKernelGGL((col_prop_kernel<N, E, C, K, M, true>), grid, block, 0, s, a);      \
+    else                                                                                      \
+      hipLaunchKernelGGL((col_prop_kernel<N, E, C, K, M, false>), grid, block, 0, s, a);     \
+  } while (0)
   if (a.kind == CXM_H_TRANSFER) {
-    if (a.phase_mode == CXM_PHASE_REF32)
-      hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_TRANSFER, CXM_PHASE_REF32>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_TRANSFER, CXM_PHASE_ACCURATE>), grid, block, 0, s, a);
+    if (a.phase_mode == CXM_PHASE_REF32) CXM_COL_LAUNCH(CXM_H_TRANSFER, CXM_PHASE_REF32);
+    else CXM_COL_LAUNCH(CXM_H_TRANSFER, CXM_PHASE_ACCURATE);
   } else if (a.kind == CXM_H_ASM) {
-    if (a.phase_mode == CXM_PHASE_REF32)
-      hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_ASM, CXM_PHASE_REF32>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_ASM, CXM_PHASE_ACCURATE>), grid, block, 0, s, a);
+    if (a.phase_mode == CXM_PHASE_REF32) CXM_COL_LAUNCH(CXM_H_ASM, CXM_PHASE_REF32);
+    else CXM_COL_LAUNCH(CXM_H_ASM, CXM_PHASE_ACCURATE);
+  } else if (a.kind == CXM_H_ARRAY) {
+    hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_ARRAY, 0, false>), grid, block, 0, s, a);
   } else {
-    hipLaunchKernelGGL((col_prop_kernel<N, E, C, 0, 0>), grid, block, 0, s, a);
+    hipLaunchKernelGGL((col_prop_kernel<N, E, C, CXM_H_NONE, 0, false>), grid, block, 0, s, a);
   }
+#undef CXM_COL_LAUNCH
   CXM_LAUNCH_CHECK();
   if (t0) timing_stop(t0, s, "col_prop_" + std::to_string(N));
   return CXM_OK;

@@ -5,3 +5,10 @@ from .propagation import (  # noqa: F401
     propagate_intensity,
     set_phase_mode,
 )
+from .optics import (  # noqa: F401,E402
+    compute_sas_precompensation,
+    ff_lens,
+    optical_fft,
+    transform_propagate,
+    transform_propagate_sas,
+)

@@ -433,20 +433,24 @@ def compute_padding_exact(height: int, wavelength: float, dx: float, z: float) -
 
 
 def transform_propagate(field, z, n, pad_width, cval=0, skip_initial_phase=False,
-                        skip_final_phase=False):
-    from .transform import transform_propagate as _tp
+                        skip_final_phase=False, *, phase_mode=None):
+    """Single-FFT Fresnel propagation (propagation.py:34-82); see optics.py."""
+    from .optics import transform_propagate as _tp
 
-    return _tp(field, z, n, pad_width, cval, skip_initial_phase, skip_final_phase)
-
-
-def compute_sas_precompensation(field, z, n):
-    from .transform import compute_sas_precompensation as _c
-
-    return _c(field, z, n)
+    return _tp(field, z, n, pad_width, cval, skip_initial_phase, skip_final_phase,
+               phase_mode=phase_mode)
 
 
-def transform_propagate_sas(field, z, n, cval=0, skip_initial_phase=False, skip_final_phase=False):
-    from .transform import transform_propagate_sas as _s
+def compute_sas_precompensation(field, z, n, *, phase_mode=None):
+    """SAS precompensation dH (propagation.py:85-103); see optics.py."""
+    from .optics import compute_sas_precompensation as _c
 
-    return _s(field, z, n, cval, skip_initial_phase, skip_final_phase)
+    return _c(field, z, n, phase_mode=phase_mode)
 
+
+def transform_propagate_sas(field, z, n, cval=0, skip_initial_phase=False, skip_final_phase=False,
+                            *, phase_mode=None):
+    """Scalable angular spectrum propagation (propagation.py:106-165); see optics.py."""
+    from .optics import transform_propagate_sas as _s
+
+    return _s(field, z, n, cval, skip_initial_phase, skip_final_phase, phase_mode=phase_mode)

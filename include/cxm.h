@@ -223,6 +223,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  * counterpart in the reference (performance tuning).
  */
 #define CXM_OPT_COL_VARIANT 1
+#define CXM_OPT_CHUNK_PLANES 2  /* planes per chunk of the propagation passes (0 = ~96 MiB) */
 int cxm_set_option(int key, int value);
 
 /* Library state. */

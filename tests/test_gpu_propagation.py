@@ -135,8 +135,8 @@ def test_chromatic_fields():
     # the reference tests' (wavelength, 1.0) spectrum: a chromatic field with wv = 1
     f1 = cx.Field.build(torch.as_tensor(u[..., :1]).cuda(), 0.3, (0.532, 1.0))
     assert isinstance(f1, cx.ChromaticScalarField)
-    o1 = cf.transfer_propagate(f1, 10.0, NM, 4, mode="same").u.cpu().numpy()
-    r1 = op.propagate(u[..., 0], (0.3, 0.3), 0.532, 10.0, NM, 4, method="transfer",
+    o1 = cf.transfer_propagate(f1, 10.0, NM, 8, mode="same").u.cpu().numpy()
+    r1 = op.propagate(u[..., 0], (0.3, 0.3), 0.532, 10.0, NM, 8, method="transfer",
                       precision="ref32", fft_precision="fp64")
     assert rel_l2(o1[..., 0], r1) < TOL
 
@@ -157,13 +157,15 @@ def test_chromatic_z_batch():
 
 
 def test_materialised_propagator_matches_oracle():
-    f = _mono(_phase_field((128, 80)), 0.2)
+    # dx = 0.22 keeps every bin away from the evanescent boundary |f| = n/lam,
+    # where the propagator is ill-conditioned for any implementation
+    f = _mono(_phase_field((128, 80)), 0.22)
     for pm, prec in (("ref32", "ref32"), ("accurate", "fp64")):
         H = cf.compute_asm_propagator(f, 55.0, NM, shift_yx=(1.0, -2.0), phase_mode=pm).cpu().numpy()
-        Hr = op.asm_propagator((128, 80), (0.2, 0.2), LAM, 55.0, NM, shift_yx=(1.0, -2.0), precision=prec)
+        Hr = op.asm_propagator((128, 80), (0.22, 0.22), LAM, 55.0, NM, shift_yx=(1.0, -2.0), precision=prec)
         assert np.max(np.abs(H - Hr)) < 2e-6
         Ht = cf.compute_transfer_propagator(f, [3.0, -8.0], NM, phase_mode=pm).cpu().numpy()
-        Htr = op.transfer_propagator((128, 80), (0.2, 0.2), LAM, [3.0, -8.0], NM, precision=prec)
+        Htr = op.transfer_propagator((128, 80), (0.22, 0.22), LAM, [3.0, -8.0], NM, precision=prec)
         assert Ht.shape == (2, 128, 80)
         assert np.max(np.abs(Ht - Htr)) < 2e-6
 
