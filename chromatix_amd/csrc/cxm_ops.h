@@ -34,7 +34,9 @@ enum OpKind : int {
   OP_CIRC = 7,    // v *= (|g - off|^2 <= p0.x); p1, p2 as CHIRP
   OP_VEC = 8,     // v *= V[i] along one axis                         (complex64 vector)
   OP_SASDH = 9,   // SAS precompensation dH at bins (ky, kx)
-                  //   p0 = (kz | turns_hi, turns_lo, lam, n), p1 = (1/dy, 1/dx, ty, tx)
+                  //   ref32:    p0 = (kz, -, lam, n), p1 = (1/dy, 1/dx, ty0, tx0)
+                  //   accurate: p0 = (turns hi, lo, s_step_y hi, lo),
+                  //             p1 = (s_step_x hi, lo, ty0, tx0)
   OP_NAMASK = 10, // v *= (fy^2 + fx^2 <= p0.x) at bins; p0 = (r2, 1/dy, 1/dx, -)
 };
 
@@ -153,6 +155,30 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
       // compute_sas_precompensation (src/chromatix/functional/propagation.py:85-103),
       // ifftshifted: W * exp(i kz (H_AS - H_Fr)), s = fl(fl(lam f) / n)
       const float4 p0 = P[0], p1 = P[1];
+      float s, c;
+      if (o.mode != CXM_PHASE_REF32) {
+        // FP64 path: s = k * lam / (n d N) per axis (step split hi + lo),
+        // turns = (z n / lam) * (H_AS - H_Fr) with the difference written
+        // -(q / (1 + sqrt(1 - q)))^2 / 2 to avoid cancellation; for q > 1
+        // H_AS is clamped to 0 as in the reference
+        const double sy = (double)signed_bin(iy, ny) * ((double)p0.z + (double)p0.w);
+        const double sx = (double)signed_bin(ix, nx) * ((double)p1.x + (double)p1.y);
+        const double sy2 = sy * sy, sx2 = sx * sx;
+        const double ty = (double)p1.z + fabs(sy), tx = (double)p1.w + fabs(sx);
+        if (!(sy2 * (2.0 + 1.0 / (ty * ty)) <= 1.0 && sx2 * (2.0 + 1.0 / (tx * tx)) <= 1.0))
+          return make_float2(0.f, 0.f);
+        const double q = sy2 + sx2;
+        double d;
+        if (q <= 1.0) {
+          const double u = q / (1.0 + sqrt(1.0 - q));
+          d = -0.5 * u * u;
+        } else {
+          d = -(1.0 - 0.5 * q);
+        }
+        const double tr = ((double)p0.x + (double)p0.y) * d;
+        sincospif((float)(2.0 * (tr - rint(tr))), &s, &c);
+        return cmul(v, make_float2(c, s));
+      }
       const float fy = bin_freq(iy, ny, p1.x), fx = bin_freq(ix, nx, p1.y);
       const float sy = __fdiv_rn(__fmul_rn(p0.z, fy), p0.w);
       const float sx = __fdiv_rn(__fmul_rn(p0.z, fx), p0.w);
@@ -162,24 +188,9 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
       const bool wx = __fmul_rn(sx2, __fadd_rn(2.0f, __fdiv_rn(1.0f, __fmul_rn(tx, tx)))) <= 1.0f;
       if (!(wy && wx)) return make_float2(0.f, 0.f);
       const float ssum = __fadd_rn(sy2, sx2);
-      float s, c;
-      if (o.mode == CXM_PHASE_REF32) {
-        const float has = sqrt_rn(fmaxf(0.0f, __fsub_rn(1.0f, ssum)));
-        const float hfr = __fsub_rn(1.0f, __fdiv_rn(ssum, 2.0f));
-        sincos_exact(__fmul_rn(p0.x, __fsub_rn(has, hfr)), &s, &c);
-      } else {
-        // H_AS - H_Fr = -(q / (1 + sqrt(1 - q)))^2 / 2 without cancellation;
-        // for q > 1 H_AS is clamped to 0 as in the reference
-        float d;
-        if (ssum <= 1.0f) {
-          const float u = ssum / (1.0f + sqrtf(1.0f - ssum));
-          d = -0.5f * u * u;
-        } else {
-          d = -(1.0f - 0.5f * ssum);
-        }
-        const float tr = df_frac(df_mul_f(df_make(p0.x, p0.y), d));
-        sincospif(2.0f * tr, &s, &c);
-      }
+      const float has = sqrt_rn(fmaxf(0.0f, __fsub_rn(1.0f, ssum)));
+      const float hfr = __fsub_rn(1.0f, __fdiv_rn(ssum, 2.0f));
+      sincos_exact(__fmul_rn(p0.x, __fsub_rn(has, hfr)), &s, &c);
       return cmul(v, make_float2(c, s));
     }
     default:

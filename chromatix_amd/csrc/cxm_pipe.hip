@@ -350,15 +350,18 @@ void sasdh_slots(Ops& B, OpDesc* o, const cxm_optfft_desc* d, int nwv, int Ny, i
     const float ext_y = dyf * (float)Ny, ext_x = dxf * (float)Nx;
     const float ty = ((2.0f * ext_y) / 2.0f) / std::fabs(zf);
     const float tx = ((2.0f * ext_x) / 2.0f) / std::fabs(zf);
-    float4& p0 = B.p(o, w, 0);
     if (d->phase_mode == CXM_PHASE_REF32) {
-      p0 = make_float4(kz, 0.f, lam, nf);
+      B.p(o, w, 0) = make_float4(kz, 0.f, lam, nf);
+      B.p(o, w, 1) = make_float4(1.0f / dyf, 1.0f / dxf, ty, tx);
     } else {
-      float hi, lo;
-      split(d->z * d->n / v.lam, hi, lo);
-      p0 = make_float4(hi, lo, lam, nf);
+      float th, tl, yh, yl, xh, xl;
+      split(d->z * d->n / v.lam, th, tl);
+      split(v.lam / (d->n * sy[w] * Ny), yh, yl);
+      split(v.lam / (d->n * sx[w] * Nx), xh, xl);
+      const double z_abs = std::fabs(d->z);
+      B.p(o, w, 0) = make_float4(th, tl, yh, yl);
+      B.p(o, w, 1) = make_float4(xh, xl, (float)(sy[w] * Ny / z_abs), (float)(sx[w] * Nx / z_abs));
     }
-    B.p(o, w, 1) = make_float4(1.0f / dyf, 1.0f / dxf, ty, tx);
   }
 }
 }  // namespace

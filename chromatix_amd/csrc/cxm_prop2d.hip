@@ -106,11 +106,12 @@ int validate(const cxm_prop2d_desc* d, Geom& g) {
   g.plane_in = (long long)g.Hin * g.Win;
   g.plane_ws1 = (long long)g.Hin * g.Nx;
   g.plane_ws2 = (long long)g.Hout * g.Nx;
-  // Chunk so that the workspace of one chunk stays ~<= 96 MiB (Infinity Cache resident).
+  // Chunk so that the workspace of one chunk stays <= 128 MiB, half the 256 MiB
+  // Infinity Cache: the intermediate planes are re-read from it, not from HBM.
   const int nout = d->z_zip ? 1 : d->num_z;
   const long long per_plane = (nout == 1) ? std::max(g.plane_ws1, g.plane_ws2)
                                           : g.plane_ws1 + (long long)nout * g.plane_ws2;
-  const long long budget = (96ll << 20) / 8;
+  const long long budget = (128ll << 20) / 8;
   long long ch = std::max(1ll, budget / std::max(1ll, per_plane));
   if (option(CXM_OPT_CHUNK_PLANES) > 0) ch = option(CXM_OPT_CHUNK_PLANES);
   g.chunk = (int)std::min<long long>(ch, std::max(1, d->batch));

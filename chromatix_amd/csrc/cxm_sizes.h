@@ -11,7 +11,7 @@
   X(32, 16, 16, 64) X(64, 16, 16, 32) X(128, 16, 16, 32) X(256, 16, 16, 16)   \
   X(5, 5, 5, 64) X(10, 10, 10, 64) X(20, 20, 20, 64) X(40, 20, 20, 64)        \
   X(80, 20, 20, 32) X(160, 20, 20, 32)
-#define CXM_SIZES_G1(X) X(512, 16, 16, 16) X(1024, 16, 16, 8)
+#define CXM_SIZES_G1(X) X(512, 16, 16, 16) X(1024, 16, 16, 4)
 #define CXM_SIZES_G2(X) X(2048, 16, 16, 4) X(4096, 16, 32, 4) X(8192, 32, 32, 2)
 #define CXM_SIZES_G3(X)                                                       \
   X(320, 20, 20, 16) X(640, 20, 20, 16) X(1280, 20, 20, 8) X(2560, 20, 20, 4) \
