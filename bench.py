@@ -44,6 +44,9 @@ def parse():
     p.add_argument("--phase-mode", default="ref32", choices=["ref32", "accurate"])
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--summarize", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
+                   help="write the rocprofv3 summaries of a profiled run (kernel-trace --stats, "
+                        "--pmc FETCH_SIZE, --pmc WRITE_SIZE; csv output) and exit")
     return p.parse_args()
 
 
@@ -51,22 +54,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def hbm_copy_gbps(torch, nbytes=1 << 30, reps=10):
-    """Plain device-to-device copy: read + write bytes / time."""
-    src = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+def hbm_copy_gbps(torch, L, nbytes=1 << 30, reps=10):
+    """libcxm's plain complex64 copy (8-byte lanes, like the FFT passes):
+    (read + write bytes) / kernel time, timed with the library's HIP events."""
+    n = nbytes // 8
+    src = torch.ones(n, dtype=torch.complex64, device="cuda")
     dst = torch.empty_like(src)
+    s = L.stream_ptr(src.device)
     for _ in range(3):
-        dst.copy_(src)
+        L.check(L.lib().cxm_copy_c64(s, src.data_ptr(), dst.data_ptr(), n))
     torch.cuda.synchronize()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    a.record()
+    L.timing_enable(True)
     for _ in range(reps):
-        dst.copy_(src)
-    b.record()
-    torch.cuda.synchronize()
-    ms = a.elapsed_time(b) / reps
+        L.check(L.lib().cxm_copy_c64(s, src.data_ptr(), dst.data_ptr(), n))
+    kt = L.timing_collect()
+    L.timing_enable(False)
+    ms, cnt = kt["copy_c64"]
     del src, dst
-    return 2 * nbytes / (ms * 1e-3) / 1e9
+    return 2 * nbytes / (ms / cnt * 1e-3) / 1e9
 
 
 def cpu_baseline(args):
@@ -96,8 +101,78 @@ def cpu_baseline(args):
     }
 
 
+COPY_BYTES = 1 << 30  # bytes read (and written) by one hbm_copy_gbps launch
+
+
+def timing_name(rocprof_name: str) -> str | None:
+    """rocprof kernel name -> libcxm timing name (col_prop_1024, ...)."""
+    import re
+
+    m = re.search(r"col_prop_kernel<(\d+),", rocprof_name)
+    if m:
+        return f"col_prop_{m.group(1)}"
+    m = re.search(r"row_fft_kernel<(\d+), \d+, (-?1)>", rocprof_name)
+    if m:
+        return f"row_fft_{'fwd' if m.group(2) == '-1' else 'inv'}_{m.group(1)}"
+    if "copy_kernel" in rocprof_name:
+        return "copy_c64"
+    return None
+
+
+def summarize(trace_dir: str, fetch_dir: str, write_dir: str) -> None:
+    """Kernel-time and traffic summaries of a profiled bench run.  Counter
+    values (KiB per dispatch) are scaled by what the same counter reads for
+    libcxm's copy kernel of known size (COPY_BYTES read, COPY_BYTES written)."""
+    import csv
+    import glob
+
+    def one(pattern):
+        files = glob.glob(os.path.join(pattern))
+        if not files:
+            raise SystemExit(f"no file matches {pattern}")
+        return files[0]
+
+    stats = list(csv.DictReader(open(one(os.path.join(trace_dir, "*kernel_stats.csv")))))
+    lines = ["| kernel | calls | avg us | min us | max us | % |", "|---|---|---|---|---|---|"]
+    for r in stats:
+        lines.append(f"| `{r['Name'][:80]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.1f} |")
+    counters = {}
+    for tag, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
+        for r in csv.DictReader(open(one(os.path.join(d, "*counter_collection.csv")))):
+            name = timing_name(r["Kernel_Name"])
+            if name is None or r["Counter_Name"] != tag:
+                continue
+            counters.setdefault(name, {}).setdefault(tag, []).append(float(r["Counter_Value"]) * 1024)
+    avg = {k: {t: sum(v) / len(v) for t, v in d.items()} for k, d in counters.items()}
+    cal = avg.get("copy_c64", {})
+    f_fetch = COPY_BYTES / cal["FETCH_SIZE"] if cal.get("FETCH_SIZE") else None
+    f_write = COPY_BYTES / cal["WRITE_SIZE"] if cal.get("WRITE_SIZE") else None
+    traffic = {}
+    tl = ["| kernel | FETCH_SIZE raw MiB | WRITE_SIZE raw MiB | scaled read MiB | scaled write MiB |",
+          "|---|---|---|---|---|"]
+    for k, d in sorted(avg.items()):
+        fr, wr = d.get("FETCH_SIZE"), d.get("WRITE_SIZE")
+        if fr is None or wr is None or f_fetch is None or f_write is None:
+            continue
+        rd, wt = fr * f_fetch, wr * f_write
+        traffic[k] = rd + wt
+        tl.append(f"| {k} | {fr / 2**20:.1f} | {wr / 2**20:.1f} | {rd / 2**20:.1f} | {wt / 2**20:.1f} |")
+    print("## Kernel time (rocprofv3 --kernel-trace --stats)\n")
+    print("\n".join(lines))
+    print("\n## HBM traffic per dispatch (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)\n")
+    print(f"Scale factors from the {COPY_BYTES >> 20} MiB copy kernel: read x{f_fetch}, write x{f_write}\n")
+    print("\n".join(tl))
+    (ROOT / "profiles").mkdir(exist_ok=True)
+    (ROOT / "profiles" / "traffic_m0.json").write_text(json.dumps(
+        {"bytes_per_launch": traffic, "scale_fetch": f_fetch, "scale_write": f_write}, indent=1) + "\n")
+
+
 def main():
     args = parse()
+    if args.summarize:
+        summarize(*args.summarize)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; --gpus must be 1"}))
         log("bench.py: multi-GPU runs are not part of this build (one GPU per box); use --gpus 1")
@@ -163,11 +238,11 @@ def main():
     tf = ROOT / "profiles" / "traffic_m0.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get(dom)
+            traffic = json.loads(tf.read_text())["bytes_per_launch"].get(dom)
         except Exception:
             traffic = None
 
-    copy_gbps = hbm_copy_gbps(torch)
+    copy_gbps = hbm_copy_gbps(torch, L)
     log(f"[bench] device copy: {copy_gbps:,.0f} GB/s")
 
     result = {

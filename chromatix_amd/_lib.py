@@ -113,6 +113,35 @@ class OptFftDesc(ctypes.Structure):
     ]
 
 
+CXM_GRAD_UPSTREAM = 0
+CXM_GRAD_MSE = 1
+
+
+class Psf4fDesc(ctypes.Structure):
+    _fields_ = [
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("num_z", ctypes.c_int32),
+        ("phase_mode", ctypes.c_int32),
+        ("out_kind", ctypes.c_int32),
+        ("normalize", ctypes.c_int32),
+        ("grad_kind", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("dy", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("wavelength", ctypes.c_double),
+        ("n", ctypes.c_double),
+        ("f", ctypes.c_double),
+        ("NA", ctypes.c_double),
+        ("f_tube", ctypes.c_double),
+        ("power", ctypes.c_double),
+        ("amplitude", ctypes.c_double),
+        ("offset_y", ctypes.c_double),
+        ("offset_x", ctypes.c_double),
+        ("z", ctypes.POINTER(ctypes.c_double)),
+    ]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -139,6 +168,13 @@ def _declare(lib):
         "cxm_timing_get": (i32, [i32, ctypes.c_char_p, i32, dp, ctypes.POINTER(ctypes.c_int64)]),
         "cxm_intensity_c64": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, dp]),
         "cxm_plane_sumsq_workspace_size": (sz, [ctypes.c_int64]),
+        "cxm_copy_c64": (i32, [vp, vp, vp, ctypes.c_int64]),
+        "cxm_psf4f_workspace_size": (sz, [ctypes.POINTER(Psf4fDesc)]),
+        "cxm_psf4f_fwd": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Psf4fDesc)]),
+        "cxm_psf4f_bwd": (i32, [vp, vp, vp, vp, vp, sz, ctypes.POINTER(Psf4fDesc)]),
+        "cxm_point_source_c64": (i32, [vp, vp, ctypes.POINTER(Psf4fDesc)]),
+        "cxm_phase_change_c64": (i32, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, dp]),
         "cxm_plane_sumsq_c64": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, dp, vp, sz]),
     }
     for name, (res, args) in sigs.items():

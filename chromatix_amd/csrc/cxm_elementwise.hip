@@ -4,8 +4,12 @@
 //   plane sums of |u|^2 for power (field.py:656-661, 728-733), reduced in a
 //              fixed order (two stages, FP64 partials) so results are
 //              bitwise reproducible and independent of what else runs.
+//   copy       plain complex64 copy with the 8-byte-per-lane accesses of the
+//              FFT passes: the bandwidth reference and the known-size kernel
+//              that scales the HBM traffic counters.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <string>
 
 #include "cxm_internal.h"
@@ -28,6 +32,13 @@ __global__ void intensity_kernel(const float2* __restrict__ u, float* __restrict
     for (int w = 0; w < d.nwv; ++w) acc += d.w[w] * (q[w].x * q[w].x + q[w].y * q[w].y);
     out[p] = acc;
   }
+}
+
+__global__ void __launch_bounds__(256)
+copy_kernel(const float2* __restrict__ in, float2* __restrict__ out, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = in[i];
 }
 
 constexpr int kSumThreads = 256;
@@ -92,6 +103,21 @@ extern "C" int cxm_intensity_c64(void* stream, const void* u, void* out, int64_t
   hipLaunchKernelGGL(intensity_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream,
                      (const float2*)u, (float*)out, (long long)n_pixels, d);
   CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+extern "C" int cxm_copy_c64(void* stream, const void* in, void* out, int64_t n) {
+  if (n < 0) return fail(CXM_ERR_INVALID, "negative size");
+  if (n == 0) return CXM_OK;
+  if (!in || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  hipStream_t s = (hipStream_t)stream;
+  // 8 workgroups of 256 lanes per CU, grid-stride over the rest
+  const long long blocks = std::min<long long>((n + 255) / 256, 256LL * 8);
+  void* t0 = timing_start(s);
+  hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const float2*)in,
+                     (float2*)out, (long long)n);
+  CXM_LAUNCH_CHECK();
+  if (t0) timing_stop(t0, s, "copy_c64");
   return CXM_OK;
 }
 

@@ -186,10 +186,13 @@ constexpr int num_stages(int N, int E) {
   return s;
 }
 
-// LDS address of element e of a transform: a stride S between elements and a
-// one-slot pad every 16 elements (breaks the power-of-two bank aliasing of
-// the radix-16 scatter when S == 1).
-template <int S>
+// LDS slot of element e of a transform of length N: a stride S between
+// elements (S column transforms interleaved, slot = row * S + column) and,
+// for S == 1, a one-slot pad every 16 elements that breaks the power-of-two
+// bank aliasing of the radix-16 scatter.  (An XOR row swizzle removes the
+// 16-way conflict of the column scatter too, but its address arithmetic
+// raised the column kernel from 76 to 104 VGPRs and cost more than it saved.)
+template <int N, int S>
 CXM_DEV int lds_at(int e) {
   if constexpr (S == 1) return e + (e >> 4);
   else return e * S;
@@ -237,14 +240,14 @@ CXM_DEV void stockham_stage(float2 (&v)[E], float2* lds, int t, const float2* __
     } else {
       const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lds_at<S>(base + r * Ns)] = a[r];
+      for (int r = 0; r < R; ++r) lds[lds_at<N, S>(base + r * Ns)] = a[r];
     }
   }
   if constexpr (!last) {
     if constexpr (SYNC) __syncthreads();
     else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = lds[lds_at<S>(t + e * T)];
+    for (int e = 0; e < E; ++e) v[e] = lds[lds_at<N, S>(t + e * T)];
     if constexpr (SYNC) __syncthreads();
     else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
   }

@@ -19,7 +19,7 @@
 
 namespace cxm {
 
-constexpr int kMaxOps = 3;   // operators per stage
+constexpr int kMaxOps = 4;   // operators per stage
 constexpr int kPTab = 128;   // float4 parameter slots per launch
 
 enum OpKind : int {
@@ -38,6 +38,11 @@ enum OpKind : int {
                   //   accurate: p0 = (turns hi, lo, s_step_y hi, lo),
                   //             p1 = (s_step_x hi, lo, ty0, tx0)
   OP_NAMASK = 10, // v *= (fy^2 + fx^2 <= p0.x) at bins; p0 = (r2, 1/dy, 1/dx, -)
+  OP_IGRAD = 11,  // v = 2 g v, the cotangent of v from dL/dI with I = |v|^2:
+                  //   mode 0: g = arr0[y', x'] (dL/dI given, f32)
+                  //   mode 1: g = p0.x * (|v|^2 - arr0[y', x']) (mean-squared error
+                  //           against a target, p0.x = 2 / element count)
+                  //   (y', x') = (iy, ix), each fftshifted when bit 0 / 1 of `shift` is set
 };
 
 struct OpDesc {
@@ -54,7 +59,7 @@ struct OpDesc {
   int outside_zero;            // positions outside the array: 0 (1) or unchanged (0)
   int vec_axis;                // VEC: 0 indexes by iy, 1 by ix
   float t;                     // THIN: thickness
-  int pad_;
+  int shift;                   // IGRAD: fftshift of the row (bit 0) / column (bit 1) index
 };
 
 struct OpList {
@@ -192,6 +197,14 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
       const float hfr = __fsub_rn(1.0f, __fdiv_rn(ssum, 2.0f));
       sincos_exact(__fmul_rn(p0.x, __fsub_rn(has, hfr)), &s, &c);
       return cmul(v, make_float2(c, s));
+    }
+    case OP_IGRAD: {
+      const int yy = (o.shift & 1) ? fshift_dst(iy, ny) : iy;
+      const int xx = (o.shift & 2) ? fshift_dst(ix, nx) : ix;
+      const float a = ((const float*)o.arr0)[(long long)plane * o.arr_plane_stride +
+                                              (long long)yy * o.arr_row_stride + xx];
+      const float g = (o.mode == 1) ? P[0].x * (cabs2(v) - a) : a;
+      return cscale(v, 2.0f * g);
     }
     default:
       return v;

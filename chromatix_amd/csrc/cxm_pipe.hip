@@ -8,63 +8,9 @@
 #include <cstring>
 #include <string>
 
-#include "cxm_pipe_kernels.h"
+#include "cxm_pipe_host.h"
 
 namespace cxm {
-
-int launch_pipe_row_g0(int N, const PipeArgs& a, hipStream_t s);
-int launch_pipe_row_g1(int N, const PipeArgs& a, hipStream_t s);
-int launch_pipe_row_g2(int N, const PipeArgs& a, hipStream_t s);
-int launch_pipe_row_g3(int N, const PipeArgs& a, hipStream_t s);
-int launch_pipe_col_g0(int N, const PipeArgs& a, hipStream_t s);
-int launch_pipe_col_g1(int N, const PipeArgs& a, hipStream_t s);
-int launch_pipe_col_g2(int N, const PipeArgs& a, hipStream_t s);
-int launch_pipe_col_g3(int N, const PipeArgs& a, hipStream_t s);
-
-int launch_pipe_row(int N, const PipeArgs& a, hipStream_t s) {
-  int rc;
-  if ((rc = launch_pipe_row_g0(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_row_g1(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_row_g2(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_row_g3(N, a, s)) >= 0) return rc;
-  return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
-}
-
-int launch_pipe_col(int N, const PipeArgs& a, hipStream_t s) {
-  int rc;
-  if ((rc = launch_pipe_col_g0(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_col_g1(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_col_g2(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_col_g3(N, a, s)) >= 0) return rc;
-  return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
-}
-
-// ---- operator list builder ----------------------------------------------------
-struct Ops {
-  PipeArgs& a;
-  int used = 0;
-  explicit Ops(PipeArgs& args) : a(args) {}
-  // Append an operator to `L` with `slots` parameter slots of `stride` float4s.
-  OpDesc* add(OpList& L, int kind, int slots, int stride, int mode = CXM_PHASE_REF32) {
-    if (L.n >= kMaxOps || used + slots * stride > kPTab) return nullptr;
-    OpDesc& o = L.op[L.n++];
-    memset(&o, 0, sizeof(o));
-    o.kind = kind;
-    o.mode = mode;
-    o.pdiv = 1;
-    o.pmod = slots;
-    o.pbase = used;
-    o.pstride = stride;
-    used += slots * stride;
-    return &o;
-  }
-  float4& p(const OpDesc* o, int slot, int j) { return a.tab.p[o->pbase + slot * o->pstride + j]; }
-};
-
-static void split(double v, float& hi, float& lo) {
-  hi = (float)v;
-  lo = (float)(v - (double)hi);
-}
 
 // Per-wavelength sampling of a descriptor.
 struct Wv {

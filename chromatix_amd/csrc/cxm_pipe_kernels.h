@@ -35,6 +35,7 @@ struct PipeArgs {
   int out_line_shift;         // column pass: store grid column c at fshift_dst(c)
   int out_line_off, out_line_len;   // lines kept: grid lines [off, off + len)
   int seq, mid_shift, intensity;
+  int accum;                  // phase-gradient row pass: add to `out` instead of storing
   float scale, mid_scale;
   OpList pre, mid, post;
   OpTable tab;
@@ -133,6 +134,61 @@ pipe_row_kernel(PipeArgs a) {
   }
 }
 
+// ---- phase-gradient row pass ----------------------------------------------------
+// Adjoint of a row pass whose input was generated (in == null) and multiplied
+// by a phase screen exp(i phi): for every row, every plane p of the launch,
+//   g = IFFT_row(in[p, row, :]) * scale          (unnormalised inverse, the
+//                                                  adjoint of the forward FFT)
+//   out[row, x] += sum_p Im(g[j] * conj(src_p(row, x))),  x = ishift(j) - in_off
+// where src_p = the `pre` operators applied to 1, i.e. the regenerated input
+// of the forward pass, so Im(g conj(src)) = dL/dphi (d src / d phi = i src).
+// The plane sum stays in registers; `accum` adds to an existing out.
+template <int N, int E>
+__global__ void __launch_bounds__((PipeRowCfg<N, E>::THREADS))
+pipe_row_phasegrad_kernel(PipeArgs a) {
+  using Cfg = PipeRowCfg<N, E>;
+  constexpr int T = Cfg::T;
+  constexpr int SPAN = lds_span<N, 1>();
+  __shared__ float2 lds[Cfg::G * SPAN];
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x - g * T;
+  const int r = blockIdx.x * Cfg::G + g;
+  const bool ok = r < a.n_lines;
+  const int yg = a.line_off + (ok ? r : 0);
+  float acc[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) acc[k] = 0.0f;
+  for (int p = 0; p < a.n_planes; ++p) {
+    const float2* src = a.in + p * a.in_plane_stride + (long long)(ok ? r : 0) * a.in_line_stride;
+    float2 v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) v[k] = ok ? src[t + k * T] : make_float2(0.f, 0.f);
+    fft<N, E, 1, 1, true>(v, lds + g * SPAN, t, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int j = t + k * T;
+      const int xf = a.in_shift ? ishift_src(j, N) : j;
+      const int xs = xf - a.in_off;
+      if (ok && xs >= 0 && xs < a.in_len) {
+        const float2 gv = cscale(v[k], a.scale);
+        const float2 sv = apply_ops(a.pre, a.tab, make_float2(1.f, 0.f), a.plane0 + p, yg, xf,
+                                    a.grid_ny, a.grid_nx);
+        acc[k] += gv.y * sv.x - gv.x * sv.y;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+  if (!ok) return;
+  float* dst = (float*)a.out + (long long)r * a.out_line_stride;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int j = t + k * T;
+    const int xf = a.in_shift ? ishift_src(j, N) : j;
+    const int xs = xf - a.in_off;
+    if (xs >= 0 && xs < a.in_len) dst[xs] = a.accum ? dst[xs] + acc[k] : acc[k];
+  }
+}
+
 // ---- column pass --------------------------------------------------------------
 template <int N, int E, int C>
 struct PipeColCfg {
@@ -205,6 +261,19 @@ int launch_pipe_row_t(const PipeArgs& a, hipStream_t s) {
   return CXM_OK;
 }
 
+template <int N, int E>
+int launch_pipe_phasegrad_t(const PipeArgs& a, hipStream_t s) {
+  using Cfg = PipeRowCfg<N, E>;
+  const long long blocks = (a.n_lines + Cfg::G - 1) / Cfg::G;
+  if (blocks == 0 || a.n_planes == 0) return CXM_OK;
+  void* t0 = timing_start(s);
+  hipLaunchKernelGGL((pipe_row_phasegrad_kernel<N, E>), dim3((unsigned)blocks), dim3(Cfg::THREADS), 0,
+                     s, a);
+  CXM_LAUNCH_CHECK();
+  if (t0) timing_stop(t0, s, "pipe_phasegrad_" + std::to_string(N));
+  return CXM_OK;
+}
+
 template <int N, int E, int C>
 int launch_pipe_col_t(const PipeArgs& a, hipStream_t s) {
   using Cfg = PipeColCfg<N, E, C>;
@@ -235,6 +304,13 @@ int launch_pipe_col_t(const PipeArgs& a, hipStream_t s) {
       default: return -1;                                                     \
     }                                                                         \
   }                                                                           \
+  int launch_pipe_phasegrad_g##K(int N, const PipeArgs& a, hipStream_t s) {   \
+    switch (N) {                                                              \
+      SIZES(CXM_PGRAD_CASE)                                                   \
+      default: return -1;                                                     \
+    }                                                                         \
+  }                                                                           \
   }
 #define CXM_PROW_CASE(NN, ER, EC, CC) case NN: return launch_pipe_row_t<NN, ER>(a, s);
 #define CXM_PCOL_CASE(NN, ER, EC, CC) case NN: return launch_pipe_col_t<NN, EC, CC>(a, s);
+#define CXM_PGRAD_CASE(NN, ER, EC, CC) case NN: return launch_pipe_phasegrad_t<NN, ER>(a, s);

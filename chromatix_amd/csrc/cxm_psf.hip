@@ -1,0 +1,367 @@
+// 4f widefield PSF stack (objective point source -> phase mask in the
+// pupil plane -> tube-lens far-field FFT -> intensity), its gradient with
+// respect to the phase mask, and the two elementwise building blocks
+// (objective_point_source, phase_change) on their own.
+//
+// Forward, per defocus plane (two passes, nothing materialised but one
+// row-transformed chunk):
+//   row pass:    source generated in registers (defocus chirp, pupil,
+//                amplitude / power normalisation, phase screen, lens norm),
+//                ifftshift folded into the positions, row FFT -> workspace
+//   column pass: column FFT, fftshift folded into the store, |u|^2 or u
+// Backward (three passes):
+//   row pass again -> workspace
+//   column pass: FFT, cotangent 2 dL/dI u at the output position (dL/dI
+//                read, or formed from a target for the mean-squared error),
+//                inverse FFT (the adjoint), in place
+//   row pass:    inverse FFT (adjoint), times conj(regenerated source),
+//                imaginary part summed over the defocus planes in registers
+//                -> dL/dphi
+// Reference: Optical4FSystemPSF.__call__ (src/chromatix/systems/microscopes.py:334-355)
+// composing objective_point_source (src/chromatix/functional/sources.py:122-209),
+// phase_change (src/chromatix/functional/phase_masks.py:29-68) and ff_lens
+// (src/chromatix/functional/lenses.py:64-95).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "cxm_pipe_host.h"
+
+namespace cxm {
+namespace {
+
+constexpr int kMaxZ = 32;
+
+int check_psf(const cxm_psf4f_desc* d) {
+  if (!d) return fail(CXM_ERR_INVALID, "null descriptor");
+  if (d->height <= 0 || d->width <= 0) return fail(CXM_ERR_INVALID, "invalid grid shape");
+  if (d->num_z < 1 || d->num_z > kMaxZ)
+    return fail(CXM_ERR_INVALID, "num_z must be in [1, " + std::to_string(kMaxZ) + "]");
+  if (!d->z) return fail(CXM_ERR_INVALID, "null z list");
+  if (d->phase_mode != CXM_PHASE_REF32 && d->phase_mode != CXM_PHASE_ACCURATE)
+    return fail(CXM_ERR_INVALID, "invalid phase mode");
+  if (!(d->dy > 0 && d->dx > 0 && d->wavelength > 0 && d->n > 0 && d->f > 0 && d->NA > 0))
+    return fail(CXM_ERR_INVALID, "spacing, wavelength, n, f and NA must be positive");
+  if (d->normalize && !(d->power >= 0)) return fail(CXM_ERR_INVALID, "power must be >= 0");
+  return CXM_OK;
+}
+
+// Samples inside the circular pupil |g|^2 <= r2, evaluated with the float32
+// arithmetic of OP_CIRC so the count matches the kernel's mask exactly.
+long long pupil_count(int ny, int nx, float dy, float dx, float r2) {
+  static std::mutex mu;
+  static int cny = -1, cnx = -1;
+  static float cdy = 0, cdx = 0, cr2 = 0;
+  static long long cm = 0;
+  std::lock_guard<std::mutex> lk(mu);
+  if (ny == cny && nx == cnx && dy == cdy && dx == cdx && r2 == cr2) return cm;
+  long long m = 0;
+  for (int iy = 0; iy < ny; ++iy) {
+    const float gy = dy * ((float)iy - 0.5f * (float)ny) + 0.0f;
+    const float gy2 = gy * gy;
+    for (int ix = 0; ix < nx; ++ix) {
+      const float gx = dx * ((float)ix - 0.5f * (float)nx) + 0.0f;
+      if (gy2 + gx * gx <= r2) ++m;
+    }
+  }
+  cny = ny; cnx = nx; cdy = dy; cdx = dx; cr2 = r2; cm = m;
+  return m;
+}
+
+// The point source of objective_point_source (sources.py:196-208) as
+// operators: CHIRP (defocus phase -pi (z/f) |g - offset|^2 / L^2, one slot
+// per plane), CIRC (pupil of diameter 2 f NA / n, pupils.py:17-23) and a
+// constant SCALE, -i A / L^2 times the power normalisation
+// sqrt(P / (dy dx sum |u|^2)); |u| = |A| / L^2 inside the pupil, so the sum
+// is M A^2 / L^4 for M pupil samples and the scale is independent of z.
+// `extra` multiplies the scale (the tube-lens norm of the fused PSF).
+int source_ops(Ops& B, OpList& L, const cxm_psf4f_desc* d, float2 extra) {
+  const int nz = d->num_z;
+  const int mode = d->phase_mode;
+  OpDesc* ch = B.add(L, OP_CHIRP, nz, 3, mode);
+  OpDesc* ci = B.add(L, OP_CIRC, 1, 3);
+  OpDesc* sc = B.add(L, OP_SCALE, 1, 1);
+  if (!ch || !ci || !sc) return fail(CXM_ERR_INVALID, "operator table overflow");
+  const float dyf = (float)d->dy, dxf = (float)d->dx;
+  // float32 chain of the reference: L = sqrt(lam f / n), L^2 = L * L
+  const float Lf = std::sqrt(((float)d->wavelength * (float)d->f) / (float)d->n);
+  const float L2f = Lf * Lf;
+  const double L2 = d->wavelength * d->f / d->n;
+  for (int j = 0; j < nz; ++j) {
+    float4& p0 = B.p(ch, j, 0);
+    if (mode == CXM_PHASE_REF32) {
+      p0 = make_float4((float)(-M_PI) * ((float)d->z[j] / (float)d->f), L2f, 0.f, 0.f);
+    } else {
+      float hi, lo;
+      split(-(d->z[j] / d->f) / (2.0 * L2), hi, lo);
+      p0 = make_float4(hi, 0.f, lo, 0.f);
+    }
+    B.p(ch, j, 1) = make_float4(dyf, dxf, 0.f, 0.f);
+    B.p(ch, j, 2) = make_float4((float)d->offset_y, (float)d->offset_x, 0.f, 0.f);
+  }
+  const float r = (float)(2.0 * d->f * d->NA / d->n) / 2.0f;
+  const float r2 = r * r;
+  B.p(ci, 0, 0) = make_float4(r2, 0.f, 0.f, 0.f);
+  B.p(ci, 0, 1) = make_float4(dyf, dxf, 0.f, 0.f);
+  B.p(ci, 0, 2) = make_float4(0.f, 0.f, 0.f, 0.f);
+  double amp;  // imaginary part of the (purely imaginary) source amplitude
+  if (d->normalize) {
+    const long long m = pupil_count(d->height, d->width, dyf, dxf, r2);
+    if (m == 0) return fail(CXM_ERR_INVALID, "the pupil contains no sample: power is undefined");
+    const double sgn = d->amplitude < 0 ? -1.0 : 1.0;
+    amp = -sgn * std::sqrt(d->power / (d->dy * d->dx * (double)m));
+  } else {
+    amp = -d->amplitude / L2;
+  }
+  // (i amp) * extra
+  B.p(sc, 0, 0) = make_float4((float)(-amp * extra.y), (float)(amp * extra.x), 0.f, 0.f);
+  return CXM_OK;
+}
+
+// Tube-lens norm of ff_lens / optical_fft for z = f_tube > 0:
+// -i dy dx / (lam f_tube / n) (convenience.py:43-48).
+float2 tube_norm(const cxm_psf4f_desc* d) {
+  if (d->phase_mode == CXM_PHASE_REF32) {
+    const float L2t = ((float)d->wavelength * (float)d->f_tube) / (float)d->n;
+    return make_float2(0.f, -(((float)d->dy * (float)d->dx) / L2t));
+  }
+  return make_float2(0.f, (float)(-(d->dy * d->dx) / (d->wavelength * d->f_tube / d->n)));
+}
+
+int psf_ops(Ops& B, PipeArgs& ra, const cxm_psf4f_desc* d, const float* phase) {
+  int rc = source_ops(B, ra.pre, d, tube_norm(d));
+  if (rc) return rc;
+  OpDesc* ph = B.add(ra.pre, OP_PHASE, 1, 1);
+  if (!ph) return fail(CXM_ERR_INVALID, "operator table overflow");
+  ph->arr0 = phase;
+  ph->arr_plane_stride = 0;
+  ph->arr_row_stride = d->width;
+  ph->arr_rows = d->height;
+  ph->arr_cols = d->width;
+  B.p(ph, 0, 0) = make_float4(1.0f, 0.f, 0.f, 0.f);  // spectral modulation of a mono field
+  return CXM_OK;
+}
+
+long long psf_chunk(const cxm_psf4f_desc* d) {
+  const long long per = (long long)d->height * d->width;
+  const long long budget = (128ll << 20) / 8;
+  return std::max(1ll, std::min<long long>(d->num_z, budget / std::max(1ll, per)));
+}
+
+void row_args(PipeArgs& ra, const cxm_psf4f_desc* d) {
+  ra.tw = twiddles_for(d->width);
+  ra.in = nullptr;  // generated source
+  ra.out_plane_stride = (long long)d->height * d->width;
+  ra.out_line_stride = d->width;
+  ra.n_lines = d->height;
+  ra.grid_ny = d->height;
+  ra.grid_nx = d->width;
+  ra.in_len = d->width;
+  ra.in_shift = 1;
+  ra.out_len = d->width;
+  ra.seq = SEQ_F;
+  ra.scale = 1.0f;
+  ra.mid_scale = 1.0f;
+}
+
+void col_args(PipeArgs& ca, const cxm_psf4f_desc* d) {
+  ca.tw = twiddles_for(d->height);
+  ca.in_plane_stride = (long long)d->height * d->width;
+  ca.in_line_stride = d->width;
+  ca.out_plane_stride = (long long)d->height * d->width;
+  ca.out_line_stride = d->width;
+  ca.n_lines = d->width;
+  ca.grid_ny = d->height;
+  ca.grid_nx = d->width;
+  ca.in_len = d->height;
+  ca.in_shift = 1;
+  ca.out_shift = 1;
+  ca.out_len = d->height;
+  ca.out_line_len = d->width;
+  ca.seq = SEQ_F;
+  ca.scale = 1.0f;
+  ca.mid_scale = 1.0f;
+}
+
+int psf_geom(const cxm_psf4f_desc* d) {
+  int rc = check_psf(d);
+  if (rc) return rc;
+  if (!(d->f_tube > 0)) return fail(CXM_ERR_INVALID, "f_tube must be positive");
+  if (!fft_len_supported(d->height) || !fft_len_supported(d->width))
+    return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT grid " + std::to_string(d->height) + "x" +
+                                         std::to_string(d->width));
+  return CXM_OK;
+}
+
+}  // namespace
+}  // namespace cxm
+
+using namespace cxm;
+
+extern "C" size_t cxm_psf4f_workspace_size(const cxm_psf4f_desc* d) {
+  if (psf_geom(d)) return 0;
+  return (size_t)psf_chunk(d) * d->height * d->width * sizeof(float2);
+}
+
+extern "C" int cxm_psf4f_fwd(void* stream, const float* phase, void* out, void* workspace,
+                             size_t workspace_bytes, const cxm_psf4f_desc* d) {
+  int rc = psf_geom(d);
+  if (rc) return rc;
+  if (d->out_kind != CXM_OUT_COMPLEX && d->out_kind != CXM_OUT_INTENSITY)
+    return fail(CXM_ERR_INVALID, "invalid output kind");
+  if (!phase || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  const long long chunk = psf_chunk(d);
+  const size_t need = (size_t)chunk * d->height * d->width * sizeof(float2);
+  if (!workspace || workspace_bytes < need)
+    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  PipeArgs ra, ca;
+  memset(&ra, 0, sizeof(ra));
+  memset(&ca, 0, sizeof(ca));
+  Ops rb(ra);
+  if ((rc = psf_ops(rb, ra, d, phase))) return rc;
+  row_args(ra, d);
+  col_args(ca, d);
+  ca.out_line_shift = 1;
+  ca.intensity = d->out_kind == CXM_OUT_INTENSITY;
+  const size_t esz = ca.intensity ? sizeof(float) : sizeof(float2);
+  for (long long p0 = 0; p0 < d->num_z; p0 += chunk) {
+    const int kc = (int)std::min<long long>(chunk, d->num_z - p0);
+    ra.out = workspace;
+    ra.n_planes = kc;
+    ra.plane0 = (int)p0;
+    if ((rc = launch_pipe_row(d->width, ra, s))) return rc;
+    ca.in = (const float2*)workspace;
+    ca.out = (char*)out + (size_t)p0 * ca.out_plane_stride * esz;
+    ca.n_planes = kc;
+    ca.plane0 = (int)p0;
+    if ((rc = launch_pipe_col(d->height, ca, s))) return rc;
+  }
+  return CXM_OK;
+}
+
+extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot, float* grad_phase,
+                             void* workspace, size_t workspace_bytes, const cxm_psf4f_desc* d) {
+  int rc = psf_geom(d);
+  if (rc) return rc;
+  if (d->grad_kind != CXM_GRAD_UPSTREAM && d->grad_kind != CXM_GRAD_MSE)
+    return fail(CXM_ERR_INVALID, "invalid gradient kind");
+  if (!phase || !cot || !grad_phase) return fail(CXM_ERR_INVALID, "null buffer");
+  const long long chunk = psf_chunk(d);
+  const size_t need = (size_t)chunk * d->height * d->width * sizeof(float2);
+  if (!workspace || workspace_bytes < need)
+    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  PipeArgs ra, ca, ga;
+  memset(&ra, 0, sizeof(ra));
+  memset(&ca, 0, sizeof(ca));
+  Ops rb(ra), cb(ca);
+  if ((rc = psf_ops(rb, ra, d, phase))) return rc;
+  row_args(ra, d);
+  // column pass: forward FFT, cotangent at the output position, inverse FFT
+  col_args(ca, d);
+  ca.seq = SEQ_FI;
+  OpDesc* ig = cb.add(ca.mid, OP_IGRAD, 1, 1, d->grad_kind == CXM_GRAD_MSE ? 1 : 0);
+  if (!ig) return fail(CXM_ERR_INVALID, "operator table overflow");
+  ig->arr0 = cot;
+  ig->arr_plane_stride = (long long)d->height * d->width;
+  ig->arr_row_stride = d->width;
+  ig->shift = 3;
+  const double count = (double)d->num_z * d->height * d->width;
+  cb.p(ig, 0, 0) = make_float4((float)(2.0 / count), 0.f, 0.f, 0.f);
+  // phase-gradient pass: same generated source as the row pass
+  ga = ra;
+  ga.in_plane_stride = (long long)d->height * d->width;
+  ga.in_line_stride = d->width;
+  ga.out = grad_phase;
+  ga.out_line_stride = d->width;
+  for (long long p0 = 0; p0 < d->num_z; p0 += chunk) {
+    const int kc = (int)std::min<long long>(chunk, d->num_z - p0);
+    ra.out = workspace;
+    ra.n_planes = kc;
+    ra.plane0 = (int)p0;
+    if ((rc = launch_pipe_row(d->width, ra, s))) return rc;
+    ca.in = (const float2*)workspace;
+    ca.out = workspace;  // in place: a strip is fully loaded before any store, rows only permute
+    ca.n_planes = kc;
+    ca.plane0 = (int)p0;
+    if ((rc = launch_pipe_col(d->height, ca, s))) return rc;
+    ga.in = (const float2*)workspace;
+    ga.n_planes = kc;
+    ga.plane0 = (int)p0;
+    ga.accum = p0 > 0;
+    if ((rc = launch_pipe_phasegrad(d->width, ga, s))) return rc;
+  }
+  return CXM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// objective_point_source and phase_change on their own
+// ---------------------------------------------------------------------------
+namespace cxm {
+__global__ void source_grid_kernel(float2* out, int ny, int nx, OpList L, OpTable T) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int plane = blockIdx.y;
+  if (i >= (long long)ny * nx) return;
+  const int iy = (int)(i / nx), ix = (int)(i - (long long)iy * nx);
+  out[(long long)plane * ny * nx + i] = apply_ops(L, T, make_float2(1.f, 0.f), plane, iy, ix, ny, nx);
+}
+
+constexpr int kMaxWvPhase = 64;
+struct Modulation { int nwv; float m[kMaxWvPhase]; };
+
+// out[p, y, x, w] = in[p, y, x, w] * exp(i fl(phi[y, x] * m[w]))
+__global__ void phase_change_kernel(const float2* __restrict__ in, const float* __restrict__ phi,
+                                    float2* __restrict__ out, long long n, long long plane_px,
+                                    Modulation mod) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long px = i / mod.nwv;
+  const int w = (int)(i - px * mod.nwv);
+  const float ph = __fmul_rn(phi[px % plane_px], mod.m[w]);
+  float s, c;
+  sincos_exact(ph, &s, &c);
+  out[i] = cmul(in[i], make_float2(c, s));
+}
+}  // namespace cxm
+
+extern "C" int cxm_point_source_c64(void* stream, void* out, const cxm_psf4f_desc* d) {
+  int rc = check_psf(d);
+  if (rc) return rc;
+  if (!out) return fail(CXM_ERR_INVALID, "null buffer");
+  if ((rc = ensure_init())) return rc;
+  PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  Ops b(a);
+  if ((rc = source_ops(b, a.pre, d, make_float2(1.f, 0.f)))) return rc;
+  const long long n = (long long)d->height * d->width;
+  hipLaunchKernelGGL(source_grid_kernel, dim3((unsigned)((n + 255) / 256), (unsigned)d->num_z),
+                     dim3(256), 0, (hipStream_t)stream, (float2*)out, d->height, d->width, a.pre,
+                     a.tab);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+extern "C" int cxm_phase_change_c64(void* stream, const void* in, const float* phase, void* out,
+                                    int64_t n_planes, int32_t height, int32_t width,
+                                    int32_t num_wv, const double* modulation) {
+  if (n_planes < 0 || height <= 0 || width <= 0) return fail(CXM_ERR_INVALID, "invalid shape");
+  if (num_wv < 1 || num_wv > kMaxWvPhase) return fail(CXM_ERR_INVALID, "num_wv must be in [1, 64]");
+  if (n_planes == 0) return CXM_OK;
+  if (!in || !phase || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  Modulation m;
+  m.nwv = num_wv;
+  for (int w = 0; w < kMaxWvPhase; ++w) m.m[w] = 1.0f;
+  for (int w = 0; w < num_wv; ++w) m.m[w] = modulation ? (float)modulation[w] : 1.0f;
+  const long long plane_px = (long long)height * width;
+  const long long n = n_planes * plane_px * num_wv;
+  if ((n + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "array too large");
+  hipLaunchKernelGGL(phase_change_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const float2*)in, phase, (float2*)out, n, plane_px, m);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}

@@ -12,3 +12,6 @@ from .optics import (  # noqa: F401,E402
     transform_propagate,
     transform_propagate_sas,
 )
+from .phase_masks import phase_change  # noqa: F401,E402
+from .sources import objective_point_source  # noqa: F401,E402
+from .psf import psf_4f, psf_4f_mse_grad, psf_output_spacing  # noqa: F401,E402

@@ -183,6 +183,77 @@ int cxm_sas_c64(void* stream, const void* in, void* out, void* workspace,
  */
 int cxm_sas_precompensation_c64(void* stream, void* out, const cxm_optfft_desc* d);
 
+/* Gradient kinds for cxm_psf4f_bwd. */
+#define CXM_GRAD_UPSTREAM 0 /* cot = dL/dI, num_z x height x width float32 */
+#define CXM_GRAD_MSE 1      /* cot = target; L = mean((I - target)^2) over all elements */
+
+/*
+ * 4f widefield PSF stack: a point source defocused by z_j in front of an
+ * objective (focal length f, numerical aperture NA, immersion index n),
+ * the phase mask exp(i phase) in its pupil plane, a tube lens of focal
+ * length f_tube and its back focal plane.  Per plane j:
+ *   u_j = objective_point_source(shape, (dy, dx), wavelength, z_j, f, n, NA,
+ *                                power, amplitude, offset)
+ *   out_j = ff_lens(phase_change(u_j, phase), f_tube, n)   (.intensity with
+ *           out_kind = INTENSITY)
+ * Replaces Optical4FSystemPSF.__call__ (src/chromatix/systems/microscopes.py:334-355)
+ * with objective_point_source (src/chromatix/functional/sources.py:122-209),
+ * phase_change (src/chromatix/functional/phase_masks.py:29-68), ff_lens
+ * (src/chromatix/functional/lenses.py:64-95) and .intensity
+ * (src/chromatix/core/field.py:663-665).  Monochromatic, scalar.
+ *
+ * phase: height x width float32 (device); out: num_z x height x width
+ * float32 or complex64.  Output spacing: lam f_tube / (n N dx) per axis.
+ */
+typedef struct cxm_psf4f_desc {
+  int32_t height, width;  /* pupil-plane grid */
+  int32_t num_z;          /* defocus planes, 1..32 */
+  int32_t phase_mode;     /* CXM_PHASE_* for the defocus phase */
+  int32_t out_kind;       /* forward output: CXM_OUT_INTENSITY or CXM_OUT_COMPLEX */
+  int32_t normalize;      /* 1: scale every plane to total power `power` */
+  int32_t grad_kind;      /* CXM_GRAD_* (backward only) */
+  int32_t reserved;
+  double dy, dx;          /* pupil-plane spacing */
+  double wavelength, n;   /* wavelength, immersion refractive index */
+  double f, NA;           /* objective focal length, numerical aperture */
+  double f_tube;          /* tube-lens focal length (> 0) */
+  double power, amplitude;
+  double offset_y, offset_x; /* lateral point-source offset */
+  const double* z;        /* num_z defocus distances, host memory */
+} cxm_psf4f_desc;
+
+size_t cxm_psf4f_workspace_size(const cxm_psf4f_desc* d);
+int cxm_psf4f_fwd(void* stream, const float* phase, void* out, void* workspace,
+                  size_t workspace_bytes, const cxm_psf4f_desc* d);
+
+/*
+ * Gradient of a loss of the PSF intensity stack with respect to the phase
+ * mask: grad_phase (height x width float32) = dL/dphase, from dL/dI
+ * (CXM_GRAD_UPSTREAM) or from a target stack for the mean-squared error
+ * (CXM_GRAD_MSE).  Recomputes the forward; the adjoint of what JAX's
+ * reverse mode computes through the same composition.
+ */
+int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot, float* grad_phase,
+                  void* workspace, size_t workspace_bytes, const cxm_psf4f_desc* d);
+
+/*
+ * objective_point_source (src/chromatix/functional/sources.py:122-209),
+ * materialised: out is num_z x height x width complex64 (uses the
+ * descriptor's grid, optics, z, power, amplitude and offset fields).
+ */
+int cxm_point_source_c64(void* stream, void* out, const cxm_psf4f_desc* d);
+
+/*
+ * phase_change (src/chromatix/functional/phase_masks.py:29-68):
+ *   out[p, y, x, w] = in[p, y, x, w] * exp(i phase[y, x] * modulation[w])
+ * for n_planes planes of height x width (x num_wv innermost) complex64;
+ * modulation = the spectral modulation lam0 / lam (host, NULL = 1).
+ * in == out is allowed.
+ */
+int cxm_phase_change_c64(void* stream, const void* in, const float* phase, void* out,
+                         int64_t n_planes, int32_t height, int32_t width, int32_t num_wv,
+                         const double* modulation);
+
 /*
  * Intensity |u|^2 of n_pixels complex64 values, or for an innermost
  * wavelength axis of length num_wv the density-weighted sum over it:
@@ -205,6 +276,13 @@ size_t cxm_plane_sumsq_workspace_size(int64_t n_planes);
 int cxm_plane_sumsq_c64(void* stream, const void* u, void* out, int64_t n_planes,
                         int64_t plane_elems, int32_t num_wv, const double* density,
                         void* workspace, size_t workspace_bytes);
+
+/*
+ * Plain copy of n complex64 values (device to device), with the access
+ * width of the FFT passes.  No counterpart in the reference: the bandwidth
+ * reference and the known-size kernel for scaling HBM traffic counters.
+ */
+int cxm_copy_c64(void* stream, const void* in, void* out, int64_t n);
 
 /*
  * Optional kernel timing: with timing enabled every kernel launch is

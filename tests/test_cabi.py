@@ -33,23 +33,24 @@ def test_every_declared_symbol_is_exported():
         assert getattr(L.lib(), n).restype is not None or n == "cxm_timing_enable"
 
 
-def test_descriptor_layout_matches_header(tmp_path):
-    fields = [f for f, _ in L.Prop2dDesc._fields_]
+@pytest.mark.parametrize("cls,cname", [(L.Prop2dDesc, "cxm_prop2d_desc"),
+                                       (L.OptFftDesc, "cxm_optfft_desc"),
+                                       (L.Psf4fDesc, "cxm_psf4f_desc")])
+def test_descriptor_layout_matches_header(tmp_path, cls, cname):
+    fields = [f for f, _ in cls._fields_]
     src = tmp_path / "layout.c"
-    body = "\n".join(
-        f'  printf("{f} %zu\\n", offsetof(cxm_prop2d_desc, {f}));' for f in fields
-    )
+    body = "\n".join(f'  printf("{f} %zu\\n", offsetof({cname}, {f}));' for f in fields)
     src.write_text(
         "#include <stdio.h>\n#include <stddef.h>\n#include \"cxm.h\"\nint main(void){\n"
-        f"{body}\n  printf(\"SIZE %zu\\n\", sizeof(cxm_prop2d_desc));\n  return 0;\n}}\n"
+        f"{body}\n  printf(\"SIZE %zu\\n\", sizeof({cname}));\n  return 0;\n}}\n"
     )
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
     c_off = dict(line.split() for line in out if line.strip())
     for f in fields:
-        assert int(c_off[f]) == getattr(L.Prop2dDesc, f).offset, f
-    assert int(c_off["SIZE"]) == ctypes.sizeof(L.Prop2dDesc)
+        assert int(c_off[f]) == getattr(cls, f).offset, f
+    assert int(c_off["SIZE"]) == ctypes.sizeof(cls)
 
 
 def _desc(**kw):

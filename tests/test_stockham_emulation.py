@@ -60,8 +60,11 @@ def test_small_dfts(R, direction):
     np.testing.assert_allclose(small_dft(a, R, direction), ref, atol=1e-12)
 
 
-def lds_at(e, S):
-    return e + (e >> 4) if S == 1 else e * S
+def lds_at(e, S, N):
+    """LDS slot of element e (mirror of lds_at in cxm_fft.h)."""
+    if S == 1:
+        return e + (e >> 4)
+    return e * S
 
 
 def emulate_fft(x, N, E, direction, S=1):
@@ -100,7 +103,7 @@ def emulate_fft(x, N, E, direction, S=1):
                     for r in range(R):
                         e = base + r * Ns
                         assert 0 <= e < N
-                        addr = lds_at(e, S)
+                        addr = lds_at(e, S, N)
                         assert addr < span
                         lds[addr] = a[r]
                         written[addr] += 1
@@ -109,7 +112,7 @@ def emulate_fft(x, N, E, direction, S=1):
             assert written.sum() == N
             for t in range(T):
                 for e in range(E):
-                    newv[t, e] = lds[lds_at(t + e * T, S)]
+                    newv[t, e] = lds[lds_at(t + e * T, S, N)]
             assert not np.isnan(newv).any()
         v = newv
         Ns *= R
@@ -142,3 +145,4 @@ def test_every_size_has_a_twiddle_table():
     text = (ROOT / "chromatix_amd" / "csrc" / "cxm_common.h").read_text()
     lengths = {int(v) for v in re.search(r"kTwLengths\[\] = \{([^}]*)\}", text).group(1).replace(" ", "").split(",")}
     assert {c[0] for c in configured_sizes()} <= lengths
+
