@@ -76,7 +76,9 @@ struct ColArgs {
   int out_off, out_len;  // FFT rows [out_off, out_off+out_len) written to rows [0, out_len)
   int kind, phase_mode, remove_evanescent, bandlimit;
   float scale;
+  float na_r2;           // > 0: H also masked to |f|^2 <= na_r2 (fl((NA/lam)^2), samples.py:271-280)
   int nhp;
+  int pad_;
   HParam hp[kMaxHP];
 };
 

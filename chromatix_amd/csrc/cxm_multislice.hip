@@ -1,0 +1,399 @@
+// Multislice propagation through a thick sample and its checkpointed
+// adjoint (multislice_thick_sample, src/chromatix/functional/samples.py:137-282;
+// thin_sample :85-134).
+//
+// The field stays in row-transformed form between slices, so each slice is
+// two passes over one padded plane per field:
+//   column pass : column FFT -> * H_t (on the fly) -> column IFFT   (in place)
+//   row pass    : row IFFT -> * exp(i 2 pi (dn + i abs) t / lam) -> row FFT
+// The first slice starts with a zero-padded row FFT of the input; the end is
+// the back-propagation column pass (H_{-d} times the optional NA mask) and a
+// cropping row IFFT.
+//
+// Adjoint (linear in the field for fixed dn): every pass has the same form
+// with conjugated factors; conj(H_t) = H_{-t} for the angular-spectrum
+// propagator as the reference defines it (z < 0 conjugates, the band limit
+// depends on z^2), and the adjoint of the sample multiply is conj(T).  The
+// dn gradient at slice i is k Im(g conj(u_{i+1})), where u_{i+1} is the
+// forward field after the slice, kept for one segment of k slices that is
+// recomputed from a stored state.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "cxm_pipe_host.h"
+
+namespace cxm {
+namespace {
+
+struct MsGeom {
+  int h, w, Ny, Nx, D, k, nseg;
+  long long plane;  // Ny * Nx
+};
+
+int ms_geom(const cxm_multislice_desc* d, MsGeom& g) {
+  if (!d) return fail(CXM_ERR_INVALID, "null descriptor");
+  if (d->batch < 0 || d->height <= 0 || d->width <= 0 || d->depth < 1 || d->pad_y < 0 ||
+      d->pad_x < 0)
+    return fail(CXM_ERR_INVALID, "invalid shape parameters");
+  if (d->phase_mode != CXM_PHASE_REF32 && d->phase_mode != CXM_PHASE_ACCURATE)
+    return fail(CXM_ERR_INVALID, "invalid phase mode");
+  if (d->out_kind != CXM_OUT_COMPLEX && d->out_kind != CXM_OUT_INTENSITY)
+    return fail(CXM_ERR_INVALID, "invalid output kind");
+  if (!(d->dy > 0 && d->dx > 0 && d->wavelength > 0 && d->n > 0))
+    return fail(CXM_ERR_INVALID, "spacing, wavelength and n must be positive");
+  if (d->use_na && !(d->NA > 0)) return fail(CXM_ERR_INVALID, "NA must be positive");
+  if (d->checkpoint_every < 1) return fail(CXM_ERR_INVALID, "checkpoint_every must be >= 1");
+  g.h = d->height;
+  g.w = d->width;
+  g.Ny = d->height + 2 * d->pad_y;
+  g.Nx = d->width + 2 * d->pad_x;
+  g.D = d->depth;
+  g.k = d->checkpoint_every;
+  g.nseg = (g.D + g.k - 1) / g.k;
+  g.plane = (long long)g.Ny * g.Nx;
+  if (!fft_len_supported(g.Ny) || !fft_len_supported(g.Nx))
+    return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT grid " + std::to_string(g.Ny) + "x" +
+                                         std::to_string(g.Nx));
+  return CXM_OK;
+}
+
+// Angular-spectrum parameters for distance z on the padded grid
+// (compute_asm_propagator, src/chromatix/functional/propagation.py:499-580).
+int asm_hparam(const cxm_multislice_desc* d, const MsGeom& g, double z, bool per_slice,
+               HParam& hp) {
+  cxm_prop2d_desc p;
+  memset(&p, 0, sizeof(p));
+  p.batch = 1;
+  p.height = g.Ny;
+  p.width = g.Nx;
+  p.num_z = 1;
+  p.kind = CXM_H_ASM;
+  p.phase_mode = d->phase_mode;
+  p.remove_evanescent = per_slice ? d->remove_evanescent : 0;
+  p.bandlimit = per_slice ? d->bandlimit : 0;
+  p.num_wv = 1;
+  p.dy = d->dy;
+  p.dx = d->dx;
+  p.wavelength = d->wavelength;
+  p.n = d->n;
+  p.ky = d->ky;
+  p.kx = d->kx;
+  p.z = &z;
+  return build_hparams(&p, g.Ny, g.Nx, &hp, 1);
+}
+
+// Column pass over `planes` padded planes in place, H for distance z; rows
+// [in_off, in_off + in_len) of the FFT grid are read from `in` rows
+// [0, in_len) (the rest are zero).
+int col_pass(const cxm_multislice_desc* d, const MsGeom& g, const float2* in, float2* out,
+             int planes, double z, bool per_slice, bool na, int in_off, int in_len,
+             hipStream_t s) {
+  ColArgs ca;
+  memset(&ca, 0, sizeof(ca));
+  int rc = asm_hparam(d, g, z, per_slice, ca.hp[0]);
+  if (rc) return rc;
+  ca.nhp = 1;
+  ca.in = in;
+  ca.out = out;
+  ca.tw = twiddles_for(g.Ny);
+  ca.in_plane_stride = g.plane;
+  ca.out_plane_stride = g.plane;
+  ca.row_stride = g.Nx;
+  ca.ncols = g.Nx;
+  ca.n_planes = planes;
+  ca.nz = 1;
+  ca.nwv = 1;
+  ca.in_off = in_off;
+  ca.in_len = in_len;
+  ca.out_off = 0;
+  ca.out_len = g.Ny;
+  ca.kind = CXM_H_ASM;
+  ca.phase_mode = d->phase_mode;
+  ca.remove_evanescent = per_slice ? d->remove_evanescent : 0;
+  ca.bandlimit = per_slice ? d->bandlimit : 0;
+  ca.scale = (float)(1.0 / ((double)g.Ny * (double)g.Nx));
+  if (na) {
+    const float r = (float)d->NA / (float)d->wavelength;
+    ca.na_r2 = r * r;
+  }
+  return launch_col_prop(g.Ny, ca, s);
+}
+
+// Common row-pass arguments over full padded planes.
+void row_common(PipeArgs& a, const MsGeom& g, int planes) {
+  a.tw = twiddles_for(g.Nx);
+  a.in_plane_stride = g.plane;
+  a.in_line_stride = g.Nx;
+  a.out_plane_stride = g.plane;
+  a.out_line_stride = g.Nx;
+  a.n_lines = g.Ny;
+  a.n_planes = planes;
+  a.grid_ny = g.Ny;
+  a.grid_nx = g.Nx;
+  a.in_len = g.Nx;
+  a.out_len = g.Nx;
+  a.scale = 1.0f;
+  a.mid_scale = 1.0f;
+}
+
+void slice_arrays(OpDesc* o, const cxm_multislice_desc* d, const MsGeom& g, const float* dn,
+                  const float* ab, int i) {
+  const long long sl = (long long)g.h * g.w;
+  o->arr0 = dn + i * sl;
+  o->arr1 = ab ? ab + i * sl : nullptr;
+  o->arr_plane_stride = 0;
+  o->arr_row_stride = g.w;
+  o->arr_rows = g.h;
+  o->arr_cols = g.w;
+  o->arr_row_off = d->pad_y;
+  o->arr_col_off = d->pad_x;
+  o->t = (float)d->thickness;
+}
+
+// Row pass of slice i: IFFT, thin sample, FFT, in place; optional store of
+// the field after the sample (mid_*).
+int sample_pass(const cxm_multislice_desc* d, const MsGeom& g, float2* ws, int planes,
+                const float* dn, const float* ab, int i, float2* mid_out, long long mid_pstride,
+                int mid_rstride, int y0, int x0, int mh, int mw, hipStream_t s) {
+  PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  Ops b(a);
+  OpDesc* th = b.add(a.mid, OP_THIN, 1, 1);
+  if (!th) return fail(CXM_ERR_INVALID, "operator table overflow");
+  slice_arrays(th, d, g, dn, ab, i);
+  b.p(th, 0, 0) = make_float4((float)d->wavelength, 0.f, 0.f, 0.f);
+  row_common(a, g, planes);
+  a.in = ws;
+  a.out = ws;
+  a.seq = SEQ_IF;
+  a.mid_out = mid_out;
+  a.mid_plane_stride = mid_pstride;
+  a.mid_row_stride = mid_rstride;
+  a.mid_y0 = y0;
+  a.mid_x0 = x0;
+  a.mid_h = mh;
+  a.mid_w = mw;
+  return launch_pipe_row(g.Nx, a, s);
+}
+
+// Zero-padded row FFT of `planes` input fields into ws rows [pad_y, pad_y + h).
+int input_pass(const cxm_multislice_desc* d, const MsGeom& g, const float2* in, float2* ws,
+               int planes, hipStream_t s) {
+  PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  row_common(a, g, planes);
+  a.in = in;
+  a.in_plane_stride = (long long)g.h * g.w;
+  a.in_line_stride = g.w;
+  a.n_lines = g.h;
+  a.line_off = d->pad_y;
+  a.in_off = d->pad_x;
+  a.in_len = g.w;
+  a.out = ws + (long long)d->pad_y * g.Nx;
+  a.seq = SEQ_F;
+  return launch_pipe_row(g.Nx, a, s);
+}
+
+double back_distance(const cxm_multislice_desc* d) { return -d->reverse_distance; }
+
+int copy_planes(const float2* src, float2* dst, long long n, hipStream_t s) {
+  return cxm_copy_c64((void*)s, src, dst, n);
+}
+
+}  // namespace
+}  // namespace cxm
+
+using namespace cxm;
+
+extern "C" size_t cxm_multislice_workspace_size(const cxm_multislice_desc* d) {
+  MsGeom g;
+  if (ms_geom(d, g)) return 0;
+  return (size_t)std::max(1, d->batch) * g.plane * sizeof(float2);
+}
+
+extern "C" size_t cxm_multislice_checkpoint_size(const cxm_multislice_desc* d) {
+  MsGeom g;
+  if (ms_geom(d, g)) return 0;
+  return (size_t)std::max(0, d->batch) * (g.nseg - 1) * g.plane * sizeof(float2);
+}
+
+extern "C" int cxm_multislice_fwd(void* stream, const void* in, const float* dn,
+                                  const float* absorption, void* out, void* checkpoints,
+                                  void* workspace, size_t workspace_bytes,
+                                  const cxm_multislice_desc* d) {
+  MsGeom g;
+  int rc = ms_geom(d, g);
+  if (rc) return rc;
+  if (d->batch == 0) return CXM_OK;
+  if (!in || !dn || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  const size_t need = (size_t)d->batch * g.plane * sizeof(float2);
+  if (!workspace || workspace_bytes < need)
+    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int B = d->batch;
+  float2* ws = (float2*)workspace;
+  const long long fld = (long long)g.h * g.w;
+  if ((rc = input_pass(d, g, (const float2*)in, ws, B, s))) return rc;
+  for (int i = 0; i < g.D; ++i) {
+    // the first column pass reads only the input rows (the rest is padding)
+    const bool first = i == 0;
+    if ((rc = col_pass(d, g, first ? ws + (long long)d->pad_y * g.Nx : ws, ws, B, d->thickness,
+                       true, false, first ? d->pad_y : 0, first ? g.h : g.Ny, s)))
+      return rc;
+    float2* stack = d->return_stack ? (float2*)out + (long long)i * B * fld : nullptr;
+    if ((rc = sample_pass(d, g, ws, B, dn, absorption, i, stack, fld, g.w, d->pad_y, d->pad_x,
+                          g.h, g.w, s)))
+      return rc;
+    if (checkpoints && (i + 1) % g.k == 0 && i + 1 < g.D) {
+      const int c = (i + 1) / g.k;  // state before slice c k
+      float2* dst = (float2*)checkpoints + (long long)(c - 1) * B * g.plane;
+      if ((rc = copy_planes(ws, dst, (long long)B * g.plane, s))) return rc;
+    }
+  }
+  if (d->return_stack) return CXM_OK;
+  if ((rc = col_pass(d, g, ws, ws, B, back_distance(d), false, d->use_na != 0, 0, g.Ny, s)))
+    return rc;
+  PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  row_common(a, g, B);
+  a.in = ws + (long long)d->pad_y * g.Nx;
+  a.n_lines = g.h;
+  a.line_off = d->pad_y;
+  a.out = out;
+  a.out_plane_stride = fld;
+  a.out_line_stride = g.w;
+  a.out_off = d->pad_x;
+  a.out_len = g.w;
+  a.seq = SEQ_I;
+  a.intensity = d->out_kind == CXM_OUT_INTENSITY;
+  return launch_pipe_row(g.Nx, a, s);
+}
+
+extern "C" size_t cxm_multislice_bwd_workspace_size(const cxm_multislice_desc* d) {
+  MsGeom g;
+  if (ms_geom(d, g)) return 0;
+  // forward state, adjoint state, one segment of fields, internal checkpoints
+  return (size_t)(2 + g.k + (g.nseg - 1)) * g.plane * sizeof(float2);
+}
+
+extern "C" int cxm_multislice_bwd(void* stream, const void* in, const float* dn,
+                                  const float* absorption, const void* checkpoints,
+                                  const void* cot, float* grad_dn, float* grad_absorption,
+                                  void* workspace, size_t workspace_bytes,
+                                  const cxm_multislice_desc* d) {
+  MsGeom g;
+  int rc = ms_geom(d, g);
+  if (rc) return rc;
+  if (d->return_stack) return fail(CXM_ERR_INVALID, "no adjoint for return_stack");
+  if (d->grad_kind < CXM_GRAD_UPSTREAM || d->grad_kind > CXM_GRAD_COMPLEX)
+    return fail(CXM_ERR_INVALID, "invalid gradient kind");
+  if (d->batch == 0) return CXM_OK;
+  if (!in || !dn || !cot || !grad_dn) return fail(CXM_ERR_INVALID, "null buffer");
+  const size_t need = cxm_multislice_bwd_workspace_size(d);
+  if (!workspace || workspace_bytes < need)
+    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int B = d->batch;
+  const long long fld = (long long)g.h * g.w;
+  float2* wf = (float2*)workspace;  // forward state
+  float2* wg = wf + g.plane;        // adjoint state
+  float2* seg = wg + g.plane;       // fields after each slice of a segment
+  float2* ick = seg + (long long)g.k * g.plane;  // internal checkpoints (one field)
+  const float kk = (float)(2.0 * M_PI * d->thickness / d->wavelength);
+  for (int b = 0; b < B; ++b) {
+    const float2* inb = (const float2*)in + b * fld;
+    // states before slices c k, c = 1 .. nseg-1, of field b
+    auto state = [&](int c) -> const float2* {
+      if (checkpoints) return (const float2*)checkpoints + ((long long)(c - 1) * B + b) * g.plane;
+      return ick + (long long)(c - 1) * g.plane;
+    };
+    if (!checkpoints && g.nseg > 1) {
+      if ((rc = input_pass(d, g, inb, wf, 1, s))) return rc;
+      for (int i = 0; i + 1 < (g.nseg - 1) * g.k + 1 && i < g.D; ++i) {
+        const bool first = i == 0;
+        if ((rc = col_pass(d, g, first ? wf + (long long)d->pad_y * g.Nx : wf, wf, 1,
+                           d->thickness, true, false, first ? d->pad_y : 0, first ? g.h : g.Ny, s)))
+          return rc;
+        if ((rc = sample_pass(d, g, wf, 1, dn, absorption, i, nullptr, 0, 0, 0, 0, 0, 0, s)))
+          return rc;
+        if ((i + 1) % g.k == 0 && i + 1 < g.D &&
+            (rc = copy_planes(wf, ick + (long long)((i + 1) / g.k - 1) * g.plane, g.plane, s)))
+          return rc;
+      }
+    }
+    for (int sg = g.nseg - 1; sg >= 0; --sg) {
+      const int i0 = sg * g.k, i1 = std::min(g.D, i0 + g.k);
+      // recompute the segment, keeping the field after each slice
+      if (sg == 0) {
+        if ((rc = input_pass(d, g, inb, wf, 1, s))) return rc;
+      } else if ((rc = copy_planes(state(sg), wf, g.plane, s))) {
+        return rc;
+      }
+      for (int i = i0; i < i1; ++i) {
+        const bool first = i == 0;
+        if ((rc = col_pass(d, g, first ? wf + (long long)d->pad_y * g.Nx : wf, wf, 1,
+                           d->thickness, true, false, first ? d->pad_y : 0, first ? g.h : g.Ny, s)))
+          return rc;
+        if ((rc = sample_pass(d, g, wf, 1, dn, absorption, i, seg + (long long)(i - i0) * g.plane,
+                              g.plane, g.Nx, 0, 0, g.Ny, g.Nx, s)))
+          return rc;
+      }
+      if (sg == g.nseg - 1) {
+        // output cotangent: back-propagate, inverse rows, cotangent of the
+        // cropped output, forward rows -> adjoint state; then the adjoint of
+        // the back-propagation
+        if ((rc = col_pass(d, g, wf, wf, 1, back_distance(d), false, d->use_na != 0, 0, g.Ny, s)))
+          return rc;
+        PipeArgs a;
+        memset(&a, 0, sizeof(a));
+        Ops ob(a);
+        OpDesc* ig = ob.add(a.mid, OP_IGRAD, 1, 1, d->grad_kind);
+        if (!ig) return fail(CXM_ERR_INVALID, "operator table overflow");
+        const size_t esz = d->grad_kind == CXM_GRAD_COMPLEX ? sizeof(float2) : sizeof(float);
+        ig->arr0 = (const char*)cot + (size_t)b * fld * esz;
+        ig->arr_plane_stride = 0;
+        ig->arr_row_stride = g.w;
+        ig->arr_rows = g.h;
+        ig->arr_cols = g.w;
+        ig->arr_row_off = d->pad_y;
+        ig->arr_col_off = d->pad_x;
+        ob.p(ig, 0, 0) = make_float4((float)(2.0 / ((double)B * fld)), 0.f, 0.f, 0.f);
+        row_common(a, g, 1);
+        a.in = wf;
+        a.out = wg;
+        a.seq = SEQ_IF;
+        if ((rc = launch_pipe_row(g.Nx, a, s))) return rc;
+        if ((rc = col_pass(d, g, wg, wg, 1, -back_distance(d), false, d->use_na != 0, 0, g.Ny, s)))
+          return rc;
+      }
+      // reverse through the segment
+      for (int i = i1 - 1; i >= i0; --i) {
+        PipeArgs a;
+        memset(&a, 0, sizeof(a));
+        Ops ob(a);
+        OpDesc* ta = ob.add(a.mid, OP_THINADJ, 1, 1);
+        if (!ta) return fail(CXM_ERR_INVALID, "operator table overflow");
+        slice_arrays(ta, d, g, dn, absorption, i);
+        ta->arr2 = seg + (long long)(i - i0) * g.plane;
+        ta->arr2_plane_stride = 0;
+        ta->arr2_row_stride = g.Nx;
+        ta->out0 = grad_dn + (long long)i * fld;
+        ta->out1 = grad_absorption ? grad_absorption + (long long)i * fld : nullptr;
+        ta->accum = b > 0;
+        ob.p(ta, 0, 0) = make_float4((float)d->wavelength, kk, 0.f, 0.f);
+        row_common(a, g, 1);
+        a.in = wg;
+        a.out = wg;
+        a.seq = SEQ_IF;
+        if ((rc = launch_pipe_row(g.Nx, a, s))) return rc;
+        if (i > 0 && (rc = col_pass(d, g, wg, wg, 1, -d->thickness, true, false, 0, g.Ny, s)))
+          return rc;
+      }
+    }
+  }
+  return CXM_OK;
+}

@@ -42,7 +42,12 @@ enum OpKind : int {
                   //   mode 0: g = arr0[y', x'] (dL/dI given, f32)
                   //   mode 1: g = p0.x * (|v|^2 - arr0[y', x']) (mean-squared error
                   //           against a target, p0.x = 2 / element count)
-                  //   (y', x') = (iy, ix), each fftshifted when bit 0 / 1 of `shift` is set
+                  //   mode 2: v = arr0[y', x'] (complex64 cotangent given)
+                  //   (y', x') = (iy, ix), each fftshifted when bit 0 / 1 of `shift` is
+                  //   set, minus (arr_row_off, arr_col_off); outside the array g = 0
+  OP_THINADJ = 12,// adjoint of OP_THIN at the stored forward field u = arr2 (after the
+                  //   sample): out0 (+)= k Im(v conj u), out1 (+)= -k Re(v conj u) (d/d dn,
+                  //   d/d absorption, k = 2 pi t / lam), v = conj(T) v; p0 = (lam, k, -, -)
 };
 
 struct OpDesc {
@@ -52,6 +57,12 @@ struct OpDesc {
   int pbase, pstride;          // parameter table addressing
   const void* arr0;            // PHASE: phi; THIN: dn; CARRAY / VEC: complex data
   const void* arr1;            // THIN: absorption (may be null)
+  const void* arr2;            // THINADJ: stored forward field (complex64, grid layout)
+  void* out0;                  // THINADJ: d/d dn   (f32, array layout)
+  void* out1;                  // THINADJ: d/d absorption (f32, may be null)
+  long long arr2_plane_stride; // THINADJ: elements between planes of arr2
+  int arr2_row_stride;         // THINADJ: elements between rows of arr2
+  int accum;                   // THINADJ: add to out0 / out1 instead of storing
   long long arr_plane_stride;  // elements between planes (0: broadcast)
   int arr_row_stride;          // elements between array rows
   int arr_rows, arr_cols;      // array extent (VEC: length in arr_cols)
@@ -199,12 +210,38 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
       return cmul(v, make_float2(c, s));
     }
     case OP_IGRAD: {
-      const int yy = (o.shift & 1) ? fshift_dst(iy, ny) : iy;
-      const int xx = (o.shift & 2) ? fshift_dst(ix, nx) : ix;
-      const float a = ((const float*)o.arr0)[(long long)plane * o.arr_plane_stride +
-                                              (long long)yy * o.arr_row_stride + xx];
+      const int yy = ((o.shift & 1) ? fshift_dst(iy, ny) : iy) - o.arr_row_off;
+      const int xx = ((o.shift & 2) ? fshift_dst(ix, nx) : ix) - o.arr_col_off;
+      if (yy < 0 || yy >= o.arr_rows || xx < 0 || xx >= o.arr_cols) return make_float2(0.f, 0.f);
+      const long long ai = (long long)plane * o.arr_plane_stride + (long long)yy * o.arr_row_stride + xx;
+      if (o.mode == 2) return ((const float2*)o.arr0)[ai];
+      const float a = ((const float*)o.arr0)[ai];
       const float g = (o.mode == 1) ? P[0].x * (cabs2(v) - a) : a;
       return cscale(v, 2.0f * g);
+    }
+    case OP_THINADJ: {
+      const int ay = iy - o.arr_row_off, ax = ix - o.arr_col_off;
+      if (ay < 0 || ay >= o.arr_rows || ax < 0 || ax >= o.arr_cols) return v;
+      const long long idx = (long long)ay * o.arr_row_stride + ax;
+      const float dn = ((const float*)o.arr0)[idx];
+      const float ab = o.arr1 ? ((const float*)o.arr1)[idx] : 0.0f;
+      const float lam = P[0].x, k = P[0].y;
+      const float re = __fdiv_rn(__fmul_rn(-__fmul_rn(kTwoPiF, ab), o.t), lam);
+      const float im = __fdiv_rn(__fmul_rn(__fmul_rn(kTwoPiF, dn), o.t), lam);
+      float s, c;
+      sincos_exact(im, &s, &c);
+      const float m = (re == 0.0f) ? 1.0f : expf(re);
+      const float2 u = ((const float2*)o.arr2)[(long long)plane * o.arr2_plane_stride +
+                                                (long long)iy * o.arr2_row_stride + ix];
+      const float gdn = k * (v.y * u.x - v.x * u.y);
+      float* g0 = (float*)o.out0 + idx;
+      *g0 = o.accum ? *g0 + gdn : gdn;
+      if (o.out1) {
+        const float gab = -k * (v.x * u.x + v.y * u.y);
+        float* g1 = (float*)o.out1 + idx;
+        *g1 = o.accum ? *g1 + gab : gab;
+      }
+      return cmul(v, make_float2(m * c, -m * s));
     }
     default:
       return v;

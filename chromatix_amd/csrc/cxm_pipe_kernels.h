@@ -36,6 +36,12 @@ struct PipeArgs {
   int out_line_off, out_line_len;   // lines kept: grid lines [off, off + len)
   int seq, mid_shift, intensity;
   int accum;                  // phase-gradient row pass: add to `out` instead of storing
+  // optional store of the value after the `mid` operators (row pass only),
+  // grid position (y, x) -> mid_out[plane * mid_plane_stride + (y - mid_y0) *
+  // mid_row_stride + (x - mid_x0)] for y - mid_y0 in [0, mid_h), x - mid_x0 in [0, mid_w)
+  float2* mid_out;
+  long long mid_plane_stride;
+  int mid_row_stride, mid_y0, mid_x0, mid_h, mid_w;
   float scale, mid_scale;
   OpList pre, mid, post;
   OpTable tab;
@@ -68,6 +74,12 @@ CXM_DEV void run_seq(const PipeArgs& a, float2 (&v)[E], float2* lds, int t, int 
         x = is_row ? apply_ops(a.mid, a.tab, x, plane, line_grid, j, a.grid_ny, a.grid_nx)
                    : apply_ops(a.mid, a.tab, x, plane, j, line_grid, a.grid_ny, a.grid_nx);
         __builtin_amdgcn_sched_barrier(0);
+      }
+      if (a.mid_out && is_row) {
+        const int yl = line_grid - a.mid_y0, xl = j - a.mid_x0;
+        if (yl >= 0 && yl < a.mid_h && xl >= 0 && xl < a.mid_w)
+          a.mid_out[(long long)(plane - a.plane0) * a.mid_plane_stride +
+                    (long long)yl * a.mid_row_stride + xl] = x;
       }
       v[k] = x;
     }

@@ -32,8 +32,12 @@ bool fft_len_supported(long long n) {
   }
 }
 
+int launch_row_variant_1024(int v, int dir, const RowArgs& a, hipStream_t s);
+
 int launch_row(int N, int dir, const RowArgs& a, hipStream_t s) {
   int rc;
+  const int v = option(CXM_OPT_ROW_VARIANT);
+  if (v > 0 && N == 1024 && (rc = launch_row_variant_1024(v, dir, a, s)) >= 0) return rc;
   if ((rc = launch_row_g0(N, dir, a, s)) >= 0) return rc;
   if ((rc = launch_row_g1(N, dir, a, s)) >= 0) return rc;
   if ((rc = launch_row_g2(N, dir, a, s)) >= 0) return rc;

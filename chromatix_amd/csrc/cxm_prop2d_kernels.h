@@ -115,10 +115,22 @@ col_prop_kernel(ColArgs a) {
       const int zi = a.z_zip ? (pg / a.nwv) % a.nz : iz;
       const HParam& hp = a.hp[w * a.nz + zi];
       const XTerms xt = x_terms(hp, kx, a.ncols, MODE);
+      // NA mask |f|^2 <= (NA/lam)^2 on the unshifted frequency grid,
+      // f = fl(fl(1/d) * fl(k/N)) as the reference's f_grid
+      const bool na = a.na_r2 > 0.0f;
+      float fx2 = 0.0f;
+      if (na) {
+        const float fx = __fmul_rn(hp.inv_dx, __fdiv_rn((float)signed_bin(kx, a.ncols), (float)a.ncols));
+        fx2 = __fmul_rn(fx, fx);
+      }
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const float2 h = eval_H(hp, xt, ty + k * T, N, KIND, MODE, a.remove_evanescent,
-                                a.bandlimit, a.scale);
+        float2 h = eval_H(hp, xt, ty + k * T, N, KIND, MODE, a.remove_evanescent,
+                          a.bandlimit, a.scale);
+        if (na) {
+          const float fy = __fmul_rn(hp.inv_dy, __fdiv_rn((float)signed_bin(ty + k * T, N), (float)N));
+          if (__fadd_rn(__fmul_rn(fy, fy), fx2) > a.na_r2) h = make_float2(0.f, 0.f);
+        }
         u[k] = cmul(u[k], h);
         // one H evaluation per scheduling region: the unrolled evaluations
         // are not interleaved into a register-spilling schedule

@@ -270,6 +270,8 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
   ig->arr0 = cot;
   ig->arr_plane_stride = (long long)d->height * d->width;
   ig->arr_row_stride = d->width;
+  ig->arr_rows = d->height;
+  ig->arr_cols = d->width;
   ig->shift = 3;
   const double count = (double)d->num_z * d->height * d->width;
   cb.p(ig, 0, 0) = make_float4((float)(2.0 / count), 0.f, 0.f, 0.f);

@@ -183,9 +183,11 @@ int cxm_sas_c64(void* stream, const void* in, void* out, void* workspace,
  */
 int cxm_sas_precompensation_c64(void* stream, void* out, const cxm_optfft_desc* d);
 
-/* Gradient kinds for cxm_psf4f_bwd. */
-#define CXM_GRAD_UPSTREAM 0 /* cot = dL/dI, num_z x height x width float32 */
-#define CXM_GRAD_MSE 1      /* cot = target; L = mean((I - target)^2) over all elements */
+/* Gradient kinds of the adjoint entry points (cxm_psf4f_bwd, cxm_multislice_bwd). */
+#define CXM_GRAD_UPSTREAM 0 /* cot = dL/dI of the output intensity, float32 */
+#define CXM_GRAD_MSE 1      /* cot = target intensity; L = mean((I - target)^2) over all elements */
+#define CXM_GRAD_COMPLEX 2  /* cot = dL/dRe(u) + i dL/dIm(u) of the complex output, complex64
+                               (multislice only) */
 
 /*
  * 4f widefield PSF stack: a point source defocused by z_j in front of an
@@ -255,6 +257,67 @@ int cxm_phase_change_c64(void* stream, const void* in, const float* phase, void*
                          const double* modulation);
 
 /*
+ * Multislice propagation through a thick sample (multislice_thick_sample,
+ * src/chromatix/functional/samples.py:137-282, with thin_sample :85-134):
+ * pad the field and the slices by (pad_y, pad_x); for each slice i,
+ *   u <- exp(i 2 pi (dn_i + i absorption_i) t / lam) * ifft2(H_t * fft2(u))
+ * with H_t the angular-spectrum propagator over the slice thickness t
+ * (kykx, remove_evanescent, bandlimit as compute_asm_propagator); then,
+ * unless return_stack, propagate by -reverse_distance (ASM, optional NA mask
+ * |f|^2 <= (NA/lam)^2) and crop.  Each slice costs two passes: the column
+ * pass (FFT, H, inverse FFT) and a row pass (inverse FFT, sample, FFT).
+ *
+ * in: batch x height x width complex64; dn, absorption: depth x height x
+ * width float32 (absorption may be NULL = 0); out: batch x height x width
+ * (complex64 or float32 by out_kind), or with return_stack depth x batch x
+ * height x width complex64 (the field after each slice).
+ * checkpoints (optional, for the adjoint): the row-transformed state before
+ * slices k, 2k, ... (k = checkpoint_every); size from
+ * cxm_multislice_checkpoint_size.
+ */
+typedef struct cxm_multislice_desc {
+  int32_t batch;
+  int32_t height, width;      /* unpadded field and slices */
+  int32_t depth;              /* slices */
+  int32_t pad_y, pad_x;       /* zero padding per side */
+  int32_t phase_mode;         /* CXM_PHASE_* */
+  int32_t remove_evanescent;  /* per-slice propagator */
+  int32_t bandlimit;          /* per-slice propagator */
+  int32_t use_na;             /* NA mask on the final back-propagation */
+  int32_t return_stack;
+  int32_t out_kind;           /* CXM_OUT_* of the final field */
+  int32_t checkpoint_every;   /* k >= 1 */
+  int32_t grad_kind;          /* CXM_GRAD_* (adjoint) */
+  double dy, dx, wavelength, n, thickness, NA, ky, kx;
+  double reverse_distance;    /* the reference's default is thickness * depth / 2 */
+} cxm_multislice_desc;
+
+size_t cxm_multislice_workspace_size(const cxm_multislice_desc* d);
+size_t cxm_multislice_checkpoint_size(const cxm_multislice_desc* d);
+int cxm_multislice_fwd(void* stream, const void* in, const float* dn, const float* absorption,
+                       void* out, void* checkpoints, void* workspace, size_t workspace_bytes,
+                       const cxm_multislice_desc* d);
+
+/*
+ * Gradient of a loss of the multislice output with respect to dn (and
+ * absorption): reverse sweep with checkpointing.  Segments of k slices are
+ * recomputed from the stored states (or from a forward recomputation when
+ * checkpoints is NULL), their fields kept, and the adjoint runs backwards:
+ * the cotangent of the output (from cot by grad_kind), the adjoint of the
+ * back-propagation, then per slice the adjoint row pass (gradient
+ * k Im(g conj(u)) and -k Re(g conj(u)), k = 2 pi t / lam, then conj of the
+ * sample) and the adjoint column pass (conj(H_t) = H_{-t}).  Fields of the
+ * batch are processed one after another and their gradients summed.
+ * grad_dn, grad_absorption: depth x height x width float32 (grad_absorption
+ * may be NULL).  Not available with return_stack.
+ */
+size_t cxm_multislice_bwd_workspace_size(const cxm_multislice_desc* d);
+int cxm_multislice_bwd(void* stream, const void* in, const float* dn, const float* absorption,
+                       const void* checkpoints, const void* cot, float* grad_dn,
+                       float* grad_absorption, void* workspace, size_t workspace_bytes,
+                       const cxm_multislice_desc* d);
+
+/*
  * Intensity |u|^2 of n_pixels complex64 values, or for an innermost
  * wavelength axis of length num_wv the density-weighted sum over it:
  *   out[p] = sum_w density[w] |u[p*num_wv + w]|^2   (float32 out)
@@ -296,12 +359,17 @@ int cxm_timing_collect(void);
 int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* launches);
 
 /*
- * Launch-shape tuning.  CXM_OPT_COL_VARIANT selects an alternative column
- * pass shape for 1024- and 2048-point columns (0 = default).  No
- * counterpart in the reference (performance tuning).
+ * Launch-shape tuning (0 = default for every key).  No counterpart in the
+ * reference (performance tuning).
+ *   CXM_OPT_COL_VARIANT   alternative column-pass shape for 1024- and
+ *                         2048-point columns
+ *   CXM_OPT_CHUNK_PLANES  planes per chunk of the propagation passes
+ *                         (default: a workspace of at most 128 MiB)
+ *   CXM_OPT_ROW_VARIANT   alternative row-pass shape for 1024-point rows
  */
 #define CXM_OPT_COL_VARIANT 1
-#define CXM_OPT_CHUNK_PLANES 2  /* planes per chunk of the propagation passes (0 = ~96 MiB) */
+#define CXM_OPT_CHUNK_PLANES 2
+#define CXM_OPT_ROW_VARIANT 3
 int cxm_set_option(int key, int value);
 
 /* Library state. */

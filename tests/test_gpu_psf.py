@@ -67,7 +67,7 @@ def test_psf_mse_gradient(pm, n, nz):
 
 
 def test_psf_autograd_upstream():
-    phase, dx, z, rng = _case(192, 4, pitch=6.5 * 1280 / 192)
+    phase, dx, z, rng = _case(160, 4, pitch=6.5 * 1280 / 160)
     ph = torch.as_tensor(phase).cuda().requires_grad_(True)
     I = cf.psf_4f(ph, dx, LAM, z, FOBJ, NIMM, NA, FTUBE, phase_mode="accurate")
     w = torch.as_tensor(rng.standard_normal(I.shape).astype(np.float32)).cuda()
