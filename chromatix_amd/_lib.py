@@ -142,6 +142,53 @@ class Psf4fDesc(ctypes.Structure):
     ]
 
 
+CXM_GRAD_COMPLEX = 2
+
+
+class CztDesc(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("axis", ctypes.c_int32),
+        ("m", ctypes.c_int32),
+        ("num_param", ctypes.c_int32),
+        ("mod_n", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("a", ctypes.POINTER(ctypes.c_double)),
+        ("w", ctypes.POINTER(ctypes.c_double)),
+        ("scale", ctypes.c_double),
+    ]
+
+
+class MultisliceDesc(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("depth", ctypes.c_int32),
+        ("pad_y", ctypes.c_int32),
+        ("pad_x", ctypes.c_int32),
+        ("phase_mode", ctypes.c_int32),
+        ("remove_evanescent", ctypes.c_int32),
+        ("bandlimit", ctypes.c_int32),
+        ("use_na", ctypes.c_int32),
+        ("return_stack", ctypes.c_int32),
+        ("out_kind", ctypes.c_int32),
+        ("checkpoint_every", ctypes.c_int32),
+        ("grad_kind", ctypes.c_int32),
+        ("dy", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("wavelength", ctypes.c_double),
+        ("n", ctypes.c_double),
+        ("thickness", ctypes.c_double),
+        ("NA", ctypes.c_double),
+        ("ky", ctypes.c_double),
+        ("kx", ctypes.c_double),
+        ("reverse_distance", ctypes.c_double),
+    ]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -175,6 +222,19 @@ def _declare(lib):
         "cxm_point_source_c64": (i32, [vp, vp, ctypes.POINTER(Psf4fDesc)]),
         "cxm_phase_change_c64": (i32, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_int32, dp]),
+        "cxm_thin_sample_c64": (i32, [vp, vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.c_int32, dp, ctypes.c_double]),
+        "cxm_intensity_vjp_c64": (i32, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int32, dp]),
+        "cxm_cmul_c64": (i32, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                               ctypes.c_int32, ctypes.c_int32]),
+        "cxm_czt_axis_workspace_size": (sz, [ctypes.POINTER(CztDesc)]),
+        "cxm_czt_axis_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(CztDesc)]),
+        "cxm_multislice_workspace_size": (sz, [ctypes.POINTER(MultisliceDesc)]),
+        "cxm_multislice_checkpoint_size": (sz, [ctypes.POINTER(MultisliceDesc)]),
+        "cxm_multislice_fwd": (i32, [vp, vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(MultisliceDesc)]),
+        "cxm_multislice_bwd_workspace_size": (sz, [ctypes.POINTER(MultisliceDesc)]),
+        "cxm_multislice_bwd": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, vp, sz,
+                                     ctypes.POINTER(MultisliceDesc)]),
         "cxm_plane_sumsq_c64": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, dp, vp, sz]),
     }
     for name, (res, args) in sigs.items():

@@ -1,0 +1,210 @@
+// Chirp-z transform along one axis of a batch of planes (Bluestein).
+//
+//   y_k = post_k * scale * sum_{j<n} x_j a^{-j} w^{jk},   k = 0 .. m-1
+//
+// as one pass over the planes (rows, or column strips): load the n samples
+// times a^{-j} w^{j^2/2}, FFT of length L >= n + m - 1 (the next supported
+// length; the reference uses exactly n + m - 1), times the spectrum of the
+// chirp kernel 1 / w^{(t-(n-1))^2/2}, inverse FFT, keep the window
+// [n-1, n-1+m) times w^{k^2/2} * post_k * scale.  The chirp vectors are
+// evaluated in FP64 on the device from log a, log w (the reference builds
+// them in complex64) and the kernel spectrum by one row FFT.
+// Reference: czt (src/chromatix/utils/czt.py:7-52); the CZT branch of
+// kernel_propagate (src/chromatix/functional/propagation.py:395-443) calls it
+// per axis with its modulation as `post`.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <string>
+
+#include "cxm_pipe_host.h"
+
+namespace cxm {
+namespace {
+
+using cd = std::complex<double>;
+
+int czt_len(int n, int m) {
+  const long long need = (long long)n + m - 1;
+  static const int lens[] = {
+#define CXM_LEN(N, ER, EC, C) N,
+      CXM_FOR_EACH_SIZE(CXM_LEN)
+#undef CXM_LEN
+  };
+  int best = -1;
+  for (int L : lens)
+    if (L >= need && (best < 0 || L < best)) best = L;
+  return best;
+}
+
+struct CztGeom { int n, m, L, P; };
+
+int czt_geom(const cxm_czt_desc* d, CztGeom& g) {
+  if (!d) return fail(CXM_ERR_INVALID, "null descriptor");
+  if (d->batch < 0 || d->height <= 0 || d->width <= 0 || d->m <= 0)
+    return fail(CXM_ERR_INVALID, "invalid shape parameters");
+  if (d->axis != 0 && d->axis != 1) return fail(CXM_ERR_INVALID, "axis must be 0 or 1");
+  if (d->num_param < 1 || d->num_param > 64)
+    return fail(CXM_ERR_INVALID, "num_param must be in [1, 64]");
+  if (!d->a || !d->w) return fail(CXM_ERR_INVALID, "null a / w");
+  g.n = d->axis == 0 ? d->height : d->width;
+  g.m = d->m;
+  g.P = d->num_param;
+  g.L = czt_len(g.n, g.m);
+  if (g.L < 0)
+    return fail(CXM_ERR_UNSUPPORTED, "CZT needs an FFT of length >= " + std::to_string(g.n + g.m - 1));
+  return CXM_OK;
+}
+
+struct ChirpParams {
+  double la[128], lw[128];  // log a, log w per parameter set (re, im)
+  double scale;
+  int n, m, L, P, mod_n, pad_;
+};
+
+CXM_DEV float2 cexp_d(double re, double im) {
+  const double r = exp(re);
+  return make_float2((float)(r * cos(im)), (float)(r * sin(im)));
+}
+
+// Awk2[j] = a^-j w^(j^2/2); h[t] = w^-((t-(n-1))^2/2) on [0, n+m-1), 0 beyond;
+// post[k] = scale w^(k^2/2) (w^(-N k) a^N when mod_n = N >= 0)
+__global__ void czt_chirps_kernel(float2* __restrict__ out, ChirpParams c) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nA = (long long)c.P * c.n, nH = (long long)c.P * c.L, nP = (long long)c.P * c.m;
+  if (i >= nA + nH + nP) return;
+  if (i < nA) {
+    const int p = (int)(i / c.n), j = (int)(i - (long long)p * c.n);
+    const double q = 0.5 * (double)j * (double)j;
+    out[i] = cexp_d(-c.la[2 * p] * j + c.lw[2 * p] * q, -c.la[2 * p + 1] * j + c.lw[2 * p + 1] * q);
+  } else if (i < nA + nH) {
+    const long long r = i - nA;
+    const int p = (int)(r / c.L), t = (int)(r - (long long)p * c.L);
+    if (t >= c.n + c.m - 1) {
+      out[i] = make_float2(0.f, 0.f);
+    } else {
+      const double k = (double)(t - (c.n - 1));
+      const double q = 0.5 * k * k;
+      out[i] = cexp_d(-c.lw[2 * p] * q, -c.lw[2 * p + 1] * q);
+    }
+  } else {
+    const long long r = i - nA - nH;
+    const int p = (int)(r / c.m), k = (int)(r - (long long)p * c.m);
+    const double q = 0.5 * (double)k * (double)k;
+    double er = c.lw[2 * p] * q, ei = c.lw[2 * p + 1] * q;
+    if (c.mod_n >= 0) {
+      const double N = (double)c.mod_n;
+      er += -c.lw[2 * p] * N * k + c.la[2 * p] * N;
+      ei += -c.lw[2 * p + 1] * N * k + c.la[2 * p + 1] * N;
+    }
+    const double mag = exp(er) * c.scale;
+    out[i] = make_float2((float)(mag * cos(ei)), (float)(mag * sin(ei)));
+  }
+}
+
+size_t czt_ws(const CztGeom& g) {
+  return (size_t)g.P * ((size_t)g.n + g.L + g.m) * sizeof(float2);
+}
+
+}  // namespace
+}  // namespace cxm
+
+using namespace cxm;
+
+extern "C" size_t cxm_czt_axis_workspace_size(const cxm_czt_desc* d) {
+  CztGeom g;
+  if (czt_geom(d, g)) return 0;
+  return czt_ws(g);
+}
+
+extern "C" int cxm_czt_axis_c64(void* stream, const void* in, void* out, void* workspace,
+                                size_t workspace_bytes, const cxm_czt_desc* d) {
+  CztGeom g;
+  int rc = czt_geom(d, g);
+  if (rc) return rc;
+  if (d->batch == 0) return CXM_OK;
+  if (!in || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  const size_t need = czt_ws(g);
+  if (!workspace || workspace_bytes < need)
+    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const int n = g.n, m = g.m, L = g.L, P = g.P;
+  // chirps in FP64 on the device: [Awk2 (P x n) | h (P x L) | post (P x m)]
+  ChirpParams cp;
+  memset(&cp, 0, sizeof(cp));
+  for (int p = 0; p < P; ++p) {
+    const cd a(d->a[2 * p], d->a[2 * p + 1]), w(d->w[2 * p], d->w[2 * p + 1]);
+    if (std::abs(a) == 0.0 || std::abs(w) == 0.0) return fail(CXM_ERR_INVALID, "a and w must be non-zero");
+    const cd la = std::log(a), lw = std::log(w);
+    cp.la[2 * p] = la.real(); cp.la[2 * p + 1] = la.imag();
+    cp.lw[2 * p] = lw.real(); cp.lw[2 * p + 1] = lw.imag();
+  }
+  cp.n = n; cp.m = m; cp.L = L; cp.P = P;
+  cp.mod_n = d->mod_n;
+  cp.scale = d->scale;
+  float2* wA = (float2*)workspace;
+  float2* wH = wA + (size_t)P * n;
+  float2* wP = wH + (size_t)P * L;
+  const long long tot = (long long)P * (n + L + m);
+  hipLaunchKernelGGL(czt_chirps_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                     (float2*)workspace, cp);
+  CXM_LAUNCH_CHECK();
+  // kernel spectra: P rows of length L, forward FFT in place
+  {
+    PipeArgs f;
+    memset(&f, 0, sizeof(f));
+    f.in = wH;
+    f.out = wH;
+    f.tw = twiddles_for(L);
+    f.in_plane_stride = f.out_plane_stride = (long long)P * L;
+    f.in_line_stride = f.out_line_stride = L;
+    f.n_lines = P;
+    f.n_planes = 1;
+    f.grid_ny = P;
+    f.grid_nx = L;
+    f.in_len = f.out_len = L;
+    f.seq = SEQ_F;
+    f.scale = f.mid_scale = 1.0f;
+    if ((rc = launch_pipe_row(L, f, s))) return rc;
+  }
+  PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  Ops b(a);
+  const int ax = d->axis == 1 ? 1 : 0;  // VEC index axis: x for rows, y for columns
+  OpDesc* oA = b.add(a.pre, OP_VEC, P, 0);
+  OpDesc* oH = b.add(a.mid, OP_VEC, P, 0);
+  OpDesc* oP = b.add(a.post, OP_VEC, P, 0);
+  if (!oA || !oH || !oP) return fail(CXM_ERR_INVALID, "operator table overflow");
+  oA->arr0 = wA; oA->arr_plane_stride = n; oA->arr_cols = n; oA->vec_axis = ax;
+  oH->arr0 = wH; oH->arr_plane_stride = L; oH->arr_cols = L; oH->vec_axis = ax;
+  oP->arr0 = wP; oP->arr_plane_stride = m; oP->arr_cols = m; oP->vec_axis = ax;
+  if (ax) oP->arr_col_off = n - 1; else oP->arr_row_off = n - 1;
+  a.tw = twiddles_for(L);
+  a.in = (const float2*)in;
+  a.out = out;
+  a.in_plane_stride = (long long)d->height * d->width;
+  a.n_planes = d->batch;
+  a.grid_ny = ax ? d->height : L;
+  a.grid_nx = ax ? L : d->width;
+  a.in_len = n;
+  a.out_off = n - 1;
+  a.out_len = m;
+  a.seq = SEQ_FI;
+  a.scale = 1.0f;
+  a.mid_scale = (float)(1.0 / L);
+  if (ax) {
+    a.in_line_stride = d->width;
+    a.n_lines = d->height;
+    a.out_plane_stride = (long long)d->height * m;
+    a.out_line_stride = m;
+    return launch_pipe_row(L, a, s);
+  }
+  a.in_line_stride = d->width;
+  a.n_lines = d->width;
+  a.out_plane_stride = (long long)m * d->width;
+  a.out_line_stride = d->width;
+  a.out_line_len = d->width;
+  return launch_pipe_col(L, a, s);
+}

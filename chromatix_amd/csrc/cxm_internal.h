@@ -82,6 +82,8 @@ struct ColArgs {
   HParam hp[kMaxHP];
 };
 
+static_assert(sizeof(ColArgs) <= 4096, "ColArgs exceeds the kernel-argument limit");
+
 // Host builders.
 int build_hparams(const cxm_prop2d_desc* d, int Ny, int Nx, HParam* out, int nmax);
 

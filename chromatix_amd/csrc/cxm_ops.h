@@ -32,7 +32,7 @@ enum OpKind : int {
   OP_THIN = 5,    // v *= exp(i 2 pi (dn + i abs) t / p0.x)           (f32 arrays)
   OP_CARRAY = 6,  // v *= A[y,x]                                      (complex64 array)
   OP_CIRC = 7,    // v *= (|g - off|^2 <= p0.x); p1, p2 as CHIRP
-  OP_VEC = 8,     // v *= V[i] along one axis                         (complex64 vector)
+  OP_VEC = 8,     // v *= V[q][i] along one axis, q the parameter slot (complex64 vectors)
   OP_SASDH = 9,   // SAS precompensation dH at bins (ky, kx)
                   //   ref32:    p0 = (kz, -, lam, n), p1 = (1/dy, 1/dx, ty0, tx0)
                   //   accurate: p0 = (turns hi, lo, s_step_y hi, lo),
@@ -158,7 +158,7 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
     case OP_VEC: {
       const int i = o.vec_axis ? ix - o.arr_col_off : iy - o.arr_row_off;
       if (i < 0 || i >= o.arr_cols) return o.outside_zero ? make_float2(0.f, 0.f) : v;
-      return cmul(v, ((const float2*)o.arr0)[(long long)plane * o.arr_plane_stride + i]);
+      return cmul(v, ((const float2*)o.arr0)[(long long)q * o.arr_plane_stride + i]);
     }
     case OP_NAMASK: {
       // ifftshift(|f_grid|^2 <= (NA/lam)^2) of samples.py:276-279

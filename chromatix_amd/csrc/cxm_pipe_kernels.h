@@ -47,6 +47,9 @@ struct PipeArgs {
   OpTable tab;
 };
 
+// kernels take PipeArgs by value: stay within the 4 KiB kernel-argument limit
+static_assert(sizeof(PipeArgs) <= 4096, "PipeArgs exceeds the kernel-argument limit");
+
 // Rotate the register layout by N/2 (even N, even E): applies ifftshift to
 // the sequence held in the v[k] <-> X[t + kT] layout.
 template <int E>

@@ -329,7 +329,116 @@ __global__ void phase_change_kernel(const float2* __restrict__ in, const float* 
   sincos_exact(ph, &s, &c);
   out[i] = cmul(in[i], make_float2(c, s));
 }
+struct Wavelengths { int nwv; float lam[kMaxWvPhase]; };
+
+// out = in * exp(i 2 pi (dn + i abs) t / lam) in the reference's complex64
+// order (samples.py:126-133), wavelength innermost
+__global__ void thin_sample_kernel(const float2* __restrict__ in, const float* __restrict__ dn,
+                                   const float* __restrict__ ab, float2* __restrict__ out,
+                                   long long n, long long plane_px, float t, Wavelengths wl) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long px = i / wl.nwv;
+  const int w = (int)(i - px * wl.nwv);
+  const long long q = px % plane_px;
+  const float d = dn[q];
+  const float a = ab ? ab[q] : 0.0f;
+  const float lam = wl.lam[w];
+  const float re = __fdiv_rn(__fmul_rn(-__fmul_rn(kTwoPiF, a), t), lam);
+  const float im = __fdiv_rn(__fmul_rn(__fmul_rn(kTwoPiF, d), t), lam);
+  float s, c;
+  sincos_exact(im, &s, &c);
+  const float m = (re == 0.0f) ? 1.0f : expf(re);
+  out[i] = cmul(in[i], make_float2(m * c, m * s));
+}
+
+// out[p, w] = 2 density[w] g[p] u[p, w]: the vector-Jacobian product of the
+// density-weighted intensity
+__global__ void intensity_vjp_kernel(const float2* __restrict__ u, const float* __restrict__ g,
+                                     float2* __restrict__ out, long long n, Modulation dens) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long px = i / dens.nwv;
+  const int w = (int)(i - px * dens.nwv);
+  out[i] = cscale(u[i], 2.0f * dens.m[w] * g[px]);
+}
+// out[p, y, x] = a[p, y, x] * b[p % b_planes, y', x'], (y', x') = (y, x) or,
+// with shift_b, the fftshifted position ((y - H/2) mod H, (x - W/2) mod W)
+__global__ void cmul_kernel(const float2* __restrict__ a, const float2* __restrict__ b,
+                            float2* __restrict__ out, long long n, int h, int w, int b_planes,
+                            int shift_b) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long plane_px = (long long)h * w;
+  const long long p = i / plane_px;
+  const int r = (int)(i - p * plane_px);
+  int y = r / w, x = r - (r / w) * w;
+  if (shift_b) {
+    y = (y + h - h / 2) % h;
+    x = (x + w - w / 2) % w;
+  }
+  out[i] = cmul(a[i], b[(p % b_planes) * plane_px + (long long)y * w + x]);
+}
 }  // namespace cxm
+
+extern "C" int cxm_cmul_c64(void* stream, const void* a, const void* b, void* out,
+                            int64_t n_planes, int32_t height, int32_t width, int32_t b_planes,
+                            int32_t shift_b) {
+  if (n_planes < 0 || height <= 0 || width <= 0 || b_planes < 1)
+    return fail(CXM_ERR_INVALID, "invalid shape");
+  if (n_planes == 0) return CXM_OK;
+  if (!a || !b || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  const long long n = n_planes * (long long)height * width;
+  if ((n + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "array too large");
+  hipLaunchKernelGGL(cmul_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const float2*)a, (const float2*)b, (float2*)out, n,
+                     height, width, b_planes, shift_b);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+extern "C" int cxm_thin_sample_c64(void* stream, const void* in, const float* dn,
+                                   const float* absorption, void* out, int64_t n_planes,
+                                   int32_t height, int32_t width, int32_t num_wv,
+                                   const double* wavelengths, double thickness) {
+  if (n_planes < 0 || height <= 0 || width <= 0) return fail(CXM_ERR_INVALID, "invalid shape");
+  if (num_wv < 1 || num_wv > kMaxWvPhase) return fail(CXM_ERR_INVALID, "num_wv must be in [1, 64]");
+  if (n_planes == 0) return CXM_OK;
+  if (!in || !dn || !out || !wavelengths) return fail(CXM_ERR_INVALID, "null buffer");
+  Wavelengths wl;
+  wl.nwv = num_wv;
+  for (int w = 0; w < kMaxWvPhase; ++w) wl.lam[w] = 1.0f;
+  for (int w = 0; w < num_wv; ++w) {
+    if (!(wavelengths[w] > 0)) return fail(CXM_ERR_INVALID, "wavelength must be positive");
+    wl.lam[w] = (float)wavelengths[w];
+  }
+  const long long plane_px = (long long)height * width;
+  const long long n = n_planes * plane_px * num_wv;
+  if ((n + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "array too large");
+  hipLaunchKernelGGL(thin_sample_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const float2*)in, dn, absorption, (float2*)out, n,
+                     plane_px, (float)thickness, wl);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+extern "C" int cxm_intensity_vjp_c64(void* stream, const void* u, const float* g, void* out,
+                                     int64_t n_pixels, int32_t num_wv, const double* density) {
+  if (n_pixels < 0) return fail(CXM_ERR_INVALID, "negative size");
+  if (num_wv < 1 || num_wv > kMaxWvPhase) return fail(CXM_ERR_INVALID, "num_wv must be in [1, 64]");
+  if (n_pixels == 0) return CXM_OK;
+  if (!u || !g || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  Modulation d;
+  d.nwv = num_wv;
+  for (int w = 0; w < kMaxWvPhase; ++w) d.m[w] = 1.0f;
+  for (int w = 0; w < num_wv; ++w) d.m[w] = density ? (float)density[w] : 1.0f;
+  const long long n = n_pixels * num_wv;
+  if ((n + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "array too large");
+  hipLaunchKernelGGL(intensity_vjp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const float2*)u, g, (float2*)out, n, d);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
 
 extern "C" int cxm_point_source_c64(void* stream, void* out, const cxm_psf4f_desc* d) {
   int rc = check_psf(d);

@@ -1,0 +1,92 @@
+"""The CZT branch of kernel_propagate and the output-resampling modes of
+transfer_propagate / asm_propagate (src/chromatix/functional/propagation.py:
+168-352, 395-443).
+
+u = fftshift(H * fft2(ifftshift(u_in))) is formed by the shifted raw FFT
+(two passes) and one multiply by fftshift(H); then one Bluestein pass per
+axis (cxm_czt_axis_c64) evaluates the field on the output grid, with the
+reference's per-axis modulation w^(-N k) a^N, N = (m - 1) // 2, and the
+scale prod(1 / alpha), alpha = output_dx / input_df.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .. import _lib as L
+from ..core.field import ChromaticScalarField, Field, crop, pad, shift_grid
+from ..utils.czt import czt_axis
+from .optics import fft as _fft_shifted
+from .propagation import (
+    _pair,
+    compute_asm_propagator,
+    compute_transfer_propagator,
+)
+
+__all__ = ["czt_kernel_propagate", "czt_kernel_propagate_fields"]
+
+
+def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=None,
+                         shift_yx=(0.0, 0.0), use_czt: bool = True) -> Field:
+    """kernel_propagate with an output grid of spacing ``output_dx`` and shape
+    ``output_shape`` centred on ``shift_yx`` (propagation.py:395-443)."""
+    if isinstance(field, ChromaticScalarField):
+        raise NotImplementedError("CZT propagation: monochromatic scalar fields only")
+    if not use_czt:
+        raise NotImplementedError("the scaled-convolution branch (use_czt=False with output_dx) "
+                                  "is not implemented")
+    ny, nx = field.spatial_shape
+    if output_shape is None:
+        output_shape = (ny, nx)
+    output_shape = (int(output_shape[0]), int(output_shape[1]))
+    if min(output_shape) < 2:
+        raise ValueError("output_shape must be at least 2 x 2")
+    out_dx = field.dx if output_dx is None else np.broadcast_to(
+        np.asarray(output_dx, dtype=np.float64), (2,)).copy()
+    T = field.dx * np.array([ny, nx], dtype=np.float64)  # input extent
+    df = 1.0 / T
+    H = torch.as_tensor(propagator).to(device=field.u.device, dtype=torch.complex64)
+    if H.shape[-2:] != (ny, nx):
+        raise ValueError("propagator must have the field's spatial shape")
+    H = H.reshape((-1, ny, nx)).contiguous()
+    u = field.u.contiguous()
+    L.ensure_device(u)
+    spec = _fft_shifted(u, axes=(-2, -1), shift=True)
+    n_planes = spec.numel() // (ny * nx)
+    L.check(L.lib().cxm_cmul_c64(L.stream_ptr(u.device), spec.data_ptr(), H.data_ptr(),
+                                 spec.data_ptr(), n_planes, ny, nx, H.shape[0], 1))
+    out_field = Field.empty_like(shift_grid(field, shift_yx), dx=out_dx, shape=output_shape)
+    (y0, y1), (x0, x1) = out_field.spatial_limits
+    alpha = out_field.dx / df
+    lims = ((y0, y1), (x0, x1))
+    res = spec
+    for d in range(2):
+        m = output_shape[d]
+        lo, hi = lims[d]
+        a = np.exp(-1j * 2 * np.pi / T[d] * lo)
+        w = np.exp(1j * (2 * np.pi / T[d]) * (hi - lo) / (m - 1))
+        scale = float(np.prod(1.0 / alpha)) if d == 1 else 1.0
+        res = czt_axis(res, m, a, w, d, mod_n=(m - 1) // 2, scale=scale)
+    return out_field.replace(u=res)
+
+
+def czt_kernel_propagate_fields(method: str, field: Field, z, n, pad_width, cval, kykx, shift_yx,
+                                output_dx, output_shape, use_czt, mode, remove_evanescent=False,
+                                bandlimit=False, *, phase_mode=None) -> Field:
+    """transfer_propagate / asm_propagate with output_dx or output_shape:
+    pad, propagator on the padded grid (no output shift when the CZT places
+    the window), CZT branch of kernel_propagate, crop in "same" mode
+    (propagation.py:229-252, 325-352)."""
+    py, px = _pair(pad_width)
+    padded = pad(field, (py, px), cval=cval)
+    z = float(np.asarray(z).reshape(()))
+    if method == "asm":
+        H = compute_asm_propagator(padded, z, n, kykx, bandlimit,
+                                   (0.0, 0.0) if use_czt else shift_yx,
+                                   remove_evanescent, phase_mode=phase_mode)
+    else:
+        H = compute_transfer_propagator(padded, z, n, kykx, phase_mode=phase_mode)
+    out = czt_kernel_propagate(padded, H, output_dx, output_shape, shift_yx, use_czt)
+    if mode == "same":
+        out = crop(out, (py, px))
+    return out

@@ -23,16 +23,45 @@ def _density(field) -> tuple[int, object]:
     return 1, None
 
 
+def _intensity_fwd(u, nwv, dens, out_shape):
+    out = torch.empty(out_shape, dtype=torch.float32, device=u.device)
+    L.check(L.lib().cxm_intensity_c64(L.stream_ptr(u.device), u.data_ptr(), out.data_ptr(),
+                                      out.numel(), nwv, dens))
+    return out
+
+
+class _Intensity(torch.autograd.Function):
+    """|u|^2 with its reverse-mode rule dL/du = 2 density dL/dI u
+    (cxm_intensity_vjp_c64)."""
+
+    @staticmethod
+    def forward(ctx, u, nwv, density, out_shape):
+        ctx.nwv, ctx.density = nwv, density
+        ctx.save_for_backward(u)
+        return _intensity_fwd(u, nwv, L.doubles(density) if density is not None else None,
+                              out_shape)
+
+    @staticmethod
+    def backward(ctx, g):
+        (u,) = ctx.saved_tensors
+        g = g.to(torch.float32).contiguous()
+        gu = torch.empty_like(u)
+        dens = L.doubles(ctx.density) if ctx.density is not None else None
+        L.check(L.lib().cxm_intensity_vjp_c64(L.stream_ptr(u.device), u.data_ptr(), g.data_ptr(),
+                                              gu.data_ptr(), g.numel(), ctx.nwv, dens))
+        return gu, None, None, None
+
+
 def intensity(field) -> torch.Tensor:
     u = field.u.contiguous()
     L.ensure_device(u)
-    nwv, dens = _density(field)
-    out_shape = tuple(u.shape[:-1]) if _is_chromatic(field) else tuple(u.shape)
-    out = torch.empty(out_shape, dtype=torch.float32, device=u.device)
-    n_pix = out.numel()
-    L.check(L.lib().cxm_intensity_c64(L.stream_ptr(u.device), u.data_ptr(), out.data_ptr(),
-                                      n_pix, nwv, dens))
-    return out
+    chrom = _is_chromatic(field)
+    nwv = field.spectrum.size if chrom else 1
+    out_shape = tuple(u.shape[:-1]) if chrom else tuple(u.shape)
+    if u.requires_grad and torch.is_grad_enabled():
+        return _Intensity.apply(u, nwv, field.spectrum.density if chrom else None, out_shape)
+    _, dens = _density(field)
+    return _intensity_fwd(u, nwv, dens, out_shape)
 
 
 def _is_chromatic(field) -> bool:

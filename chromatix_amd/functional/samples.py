@@ -1,0 +1,199 @@
+"""Thin and thick samples on libcxm (mirrors src/chromatix/functional/samples.py).
+
+``thin_sample`` (samples.py:85-134) is one elementwise kernel.
+``multislice_thick_sample`` (samples.py:137-282) runs the whole slice loop
+as two fused passes per slice (cxm_multislice_fwd).  With gradients
+requested for ``dn_stack`` / ``absorption_stack`` the forward also stores
+the field state every ``checkpoint_every`` slices, and the backward
+(cxm_multislice_bwd) recomputes one segment at a time from those states and
+runs the hand-written adjoint: no per-slice residuals are kept.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _lib as L
+from ..core.field import ChromaticScalarField, Field
+from .propagation import _pair, _vec2, get_phase_mode
+
+__all__ = ["thin_sample", "multislice_thick_sample", "multislice_mse_grad"]
+
+
+def thin_sample(field: Field, absorption, dn, thickness) -> Field:
+    """``field * exp(1j * 2 pi (dn + 1j * absorption) * thickness / lambda)``
+    with (H, W) ``absorption`` and ``dn`` broadcast over the batch (and the
+    wavelengths of a chromatic field)."""
+    u = field.u.contiguous()
+    L.ensure_device(u)
+    H, W = field.spatial_shape
+    dn_t = torch.as_tensor(dn, dtype=torch.float32, device=u.device).contiguous()
+    ab_t = torch.as_tensor(absorption, dtype=torch.float32, device=u.device).contiguous()
+    if dn_t.ndim != 2 or tuple(dn_t.shape) != (H, W):
+        raise ValueError(f"Refractive index must have shape {(H, W)}, got {tuple(dn_t.shape)}.")
+    if ab_t.ndim != 2 or tuple(ab_t.shape) != (H, W):
+        raise ValueError(f"Absorption must have shape {(H, W)}, got {tuple(ab_t.shape)}")
+    if np.ndim(thickness) != 0:
+        raise NotImplementedError("thin_sample: scalar thickness only")
+    nwv = field.spectrum.size if isinstance(field, ChromaticScalarField) else 1
+    n_planes = u.numel() // (H * W * nwv)
+    out = torch.empty_like(u)
+    L.check(L.lib().cxm_thin_sample_c64(L.stream_ptr(u.device), u.data_ptr(), dn_t.data_ptr(),
+                                        ab_t.data_ptr(), out.data_ptr(), n_planes, H, W, nwv,
+                                        L.doubles(field.spectrum.wavelength), float(thickness)))
+    return field.replace(u=out)
+
+
+def _desc(field: Field, depth, n, thickness, pad_width, NA, kykx, reverse_distance,
+          remove_evanescent, bandlimit, return_stack, phase_mode, checkpoint_every):
+    if isinstance(field, ChromaticScalarField):
+        raise NotImplementedError("multislice_thick_sample: monochromatic scalar fields only")
+    d = L.MultisliceDesc()
+    H, W = field.spatial_shape
+    d.batch = int(np.prod(field.shape[:-2])) if field.u.ndim > 2 else 1
+    d.height, d.width = H, W
+    d.depth = int(depth)
+    d.pad_y, d.pad_x = _pair(pad_width)
+    d.phase_mode = L.PHASE_MODES[phase_mode or get_phase_mode()]
+    d.remove_evanescent = int(bool(remove_evanescent))
+    d.bandlimit = int(bool(bandlimit))
+    d.use_na = 0 if NA is None else 1
+    d.NA = 0.0 if NA is None else float(NA)
+    d.return_stack = int(bool(return_stack))
+    d.out_kind = L.CXM_OUT_COMPLEX
+    d.checkpoint_every = int(checkpoint_every)
+    d.dy, d.dx = float(field.dx[0]), float(field.dx[1])
+    d.wavelength = float(field.spectrum.wavelength[0])
+    d.n = float(n)
+    d.thickness = float(thickness)
+    d.ky, d.kx = _vec2(kykx)
+    if reverse_distance is None:
+        # thickness_per_slice * depth / 2 (samples.py:266-267)
+        reverse_distance = float(thickness) * depth / 2
+    d.reverse_distance = float(reverse_distance)
+    return d
+
+
+def _stacks(field, absorption_stack, dn_stack):
+    dev = field.u.device
+    dn = torch.as_tensor(dn_stack, device=dev)
+    ab = torch.as_tensor(absorption_stack, device=dev)
+    if dn.shape != ab.shape:
+        raise ValueError("absorption_stack and dn_stack must have the same shape")
+    if dn.ndim != 3 or tuple(dn.shape[1:]) != field.spatial_shape:
+        raise ValueError(f"stacks must have shape (depth, {field.spatial_shape[0]}, "
+                         f"{field.spatial_shape[1]})")
+    return dn, ab
+
+
+def _fwd(u, dn, ab, d, want_ckpt):
+    dev = u.device
+    B = d.batch
+    H, W = d.height, d.width
+    if d.return_stack:
+        out = torch.empty((d.depth, B, H, W), dtype=torch.complex64, device=dev)
+    else:
+        out = torch.empty((B, H, W), dtype=torch.complex64, device=dev)
+    ws = torch.empty(L.lib().cxm_multislice_workspace_size(ctypes.byref(d)), dtype=torch.uint8,
+                     device=dev)
+    ck = None
+    if want_ckpt:
+        nb = L.lib().cxm_multislice_checkpoint_size(ctypes.byref(d))
+        ck = torch.empty(max(1, nb), dtype=torch.uint8, device=dev)
+    L.check(L.lib().cxm_multislice_fwd(L.stream_ptr(dev), u.data_ptr(), dn.data_ptr(),
+                                       ab.data_ptr(), out.data_ptr(),
+                                       None if ck is None else ck.data_ptr(), ws.data_ptr(),
+                                       ws.numel(), ctypes.byref(d)))
+    return out, ck
+
+
+def _bwd(u, dn, ab, ck, cot, d, kind):
+    dev = u.device
+    d.grad_kind = kind
+    gdn = torch.empty(dn.shape, dtype=torch.float32, device=dev)
+    gab = torch.empty(ab.shape, dtype=torch.float32, device=dev)
+    ws = torch.empty(L.lib().cxm_multislice_bwd_workspace_size(ctypes.byref(d)), dtype=torch.uint8,
+                     device=dev)
+    L.check(L.lib().cxm_multislice_bwd(L.stream_ptr(dev), u.data_ptr(), dn.data_ptr(), ab.data_ptr(),
+                                       None if ck is None else ck.data_ptr(), cot.data_ptr(),
+                                       gdn.data_ptr(), gab.data_ptr(), ws.data_ptr(), ws.numel(),
+                                       ctypes.byref(d)))
+    return gdn, gab
+
+
+class _MultisliceFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u, dn, ab, d):
+        out, ck = _fwd(u, dn, ab, d, want_ckpt=True)
+        ctx.d = d
+        ctx.ck = ck
+        ctx.save_for_backward(u, dn, ab)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        u, dn, ab = ctx.saved_tensors
+        g = g.to(torch.complex64).contiguous()
+        gdn, gab = _bwd(u, dn, ab, ctx.ck, g, ctx.d, L.CXM_GRAD_COMPLEX)
+        return None, gdn, gab, None
+
+
+def multislice_thick_sample(field: Field, absorption_stack, dn_stack, n, thickness_per_slice,
+                            pad_width, NA=None, propagator=None, kykx=(0.0, 0.0),
+                            reverse_propagate_distance=None, return_stack: bool = False,
+                            remove_evanescent: bool = False, bandlimit: bool = False, *,
+                            phase_mode=None, checkpoint_every: int = 16) -> Field:
+    """Thick sample as ``depth`` thin slices: per slice an angular-spectrum
+    propagation over ``thickness_per_slice`` then the slice's thin-sample
+    factor; afterwards propagation by ``-reverse_propagate_distance``
+    (default: half the sample thickness) through an optional NA pupil, and
+    cropping of ``pad_width``.  ``return_stack`` returns the fields after
+    every slice, (depth, ..., H, W).  Differentiable with respect to
+    ``dn_stack`` and ``absorption_stack`` (checkpointed adjoint)."""
+    if propagator is not None:
+        raise NotImplementedError("multislice_thick_sample: the propagator is evaluated in the "
+                                  "kernel; pass n, kykx, remove_evanescent, bandlimit instead")
+    dn, ab = _stacks(field, absorption_stack, dn_stack)
+    d = _desc(field, dn.shape[0], n, thickness_per_slice, pad_width, NA, kykx,
+              reverse_propagate_distance, remove_evanescent, bandlimit, return_stack, phase_mode,
+              checkpoint_every)
+    u = field.u.contiguous()
+    L.ensure_device(u)
+    batch_shape = tuple(field.shape[:-2])
+    H, W = field.spatial_shape
+    needs_grad = (dn.requires_grad or ab.requires_grad) and torch.is_grad_enabled()
+    dn32 = dn if dn.dtype == torch.float32 else dn.to(torch.float32)
+    ab32 = ab if ab.dtype == torch.float32 else ab.to(torch.float32)
+    dn32, ab32 = dn32.contiguous(), ab32.contiguous()
+    if needs_grad:
+        if return_stack:
+            raise NotImplementedError("gradients through return_stack are not implemented")
+        out = _MultisliceFn.apply(u.detach(), dn32, ab32, d)
+    else:
+        out, _ = _fwd(u, dn32.detach(), ab32.detach(), d, want_ckpt=False)
+    if return_stack:
+        return field.replace(u=out.reshape((dn.shape[0],) + batch_shape + (H, W)))
+    return field.replace(u=out.reshape(batch_shape + (H, W)))
+
+
+def multislice_mse_grad(field: Field, absorption_stack, dn_stack, target, n, thickness_per_slice,
+                        pad_width, NA=None, kykx=(0.0, 0.0), reverse_propagate_distance=None,
+                        remove_evanescent: bool = False, bandlimit: bool = False, *,
+                        phase_mode=None, checkpoint_every: int = 16, checkpoints=None):
+    """(d/d dn_stack, d/d absorption_stack) of mean((|out|^2 - target)^2) for
+    the output of multislice_thick_sample, with the cotangent formed inside
+    the adjoint.  ``checkpoints`` from :func:`multislice_checkpoints` skip the
+    forward recomputation of the stored states."""
+    dn, ab = _stacks(field, absorption_stack, dn_stack)
+    d = _desc(field, dn.shape[0], n, thickness_per_slice, pad_width, NA, kykx,
+              reverse_propagate_distance, remove_evanescent, bandlimit, False, phase_mode,
+              checkpoint_every)
+    u = field.u.contiguous()
+    L.ensure_device(u)
+    t = torch.as_tensor(target, dtype=torch.float32, device=u.device).contiguous()
+    if t.numel() != d.batch * d.height * d.width:
+        raise ValueError("target must have the output's shape")
+    return _bwd(u, dn.to(torch.float32).contiguous(), ab.to(torch.float32).contiguous(), checkpoints,
+                t, d, L.CXM_GRAD_MSE)

@@ -1,0 +1,86 @@
+"""Chirp-z transforms on libcxm (mirrors src/chromatix/utils/czt.py).
+
+``czt`` runs one Bluestein pass along an axis (cxm_czt_axis_c64): chirps in
+FP64, FFTs of the next supported length >= n + m - 1, complex64 data.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _lib as L
+
+__all__ = ["czt", "cztn", "zoomed_fft", "czt_axis"]
+
+
+def _pairs(v, count: int):
+    a = np.atleast_1d(np.asarray(v, dtype=np.complex128)).reshape(-1)
+    if a.size == 1:
+        a = np.repeat(a, count)
+    if a.size != count:
+        raise ValueError("a / w must be scalars or one value per parameter set")
+    out = np.empty(2 * count)
+    out[0::2], out[1::2] = a.real, a.imag
+    return out
+
+
+def czt_axis(x: torch.Tensor, m: int, a, w, axis: int, *, num_param: int = 1, mod_n: int = -1,
+             scale: float = 1.0) -> torch.Tensor:
+    """CZT of the planes of ``x`` (..., H, W) along axis 0 (H) or 1 (W) of
+    the last two; plane p uses parameter set p % num_param; with mod_n = N
+    the output is also multiplied by w^(-N k) a^N (the modulation of the CZT
+    branch of kernel_propagate) and by ``scale``."""
+    x = torch.as_tensor(x)
+    L.ensure_device(x)
+    x = x.to(torch.complex64).contiguous()
+    H, W = x.shape[-2:]
+    B = int(np.prod(x.shape[:-2])) if x.ndim > 2 else 1
+    d = L.CztDesc()
+    d.batch, d.height, d.width = B, H, W
+    d.axis = int(axis)
+    d.m = int(m)
+    d.num_param = int(num_param)
+    d.mod_n = int(mod_n)
+    d.scale = float(scale)
+    ab, wb = L.doubles(_pairs(a, num_param)), L.doubles(_pairs(w, num_param))
+    d.a = ctypes.cast(ab, ctypes.POINTER(ctypes.c_double))
+    d.w = ctypes.cast(wb, ctypes.POINTER(ctypes.c_double))
+    out_hw = (int(m), W) if axis == 0 else (H, int(m))
+    out = torch.empty(tuple(x.shape[:-2]) + out_hw, dtype=torch.complex64, device=x.device)
+    nbytes = L.lib().cxm_czt_axis_workspace_size(ctypes.byref(d))
+    if nbytes == 0:
+        L.check(L.lib().cxm_czt_axis_c64(None, None, None, None, 0, ctypes.byref(d)))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+    L.check(L.lib().cxm_czt_axis_c64(L.stream_ptr(x.device), x.data_ptr(), out.data_ptr(),
+                                     ws.data_ptr(), nbytes, ctypes.byref(d)))
+    return out
+
+
+def czt(x, m: int, a, w, axis: int = -1) -> torch.Tensor:
+    """Chirp-z transform along ``axis``: y_k = sum_j x_j a^-j w^(j k),
+    k < m (czt.py:7-52, Bluestein)."""
+    x = torch.as_tensor(x)
+    ax = axis % x.ndim
+    xm = x.movedim(ax, -1)
+    lead = xm.shape[:-1]
+    y = czt_axis(xm.reshape(1, -1, xm.shape[-1]), m, a, w, 1)
+    return y.reshape(tuple(lead) + (int(m),)).movedim(-1, ax)
+
+
+def cztn(x, m, a, w, axes=(-2, -1)) -> torch.Tensor:
+    """CZT along several axes in turn (czt.py:55-79)."""
+    for d, ax in enumerate(axes):
+        x = czt(x, m[d], a[d], w[d], ax)
+    return x
+
+
+def zoomed_fft(x, k_start: float, k_end: float, output_shape, axes=(-2, -1),
+               include_end: bool = True) -> torch.Tensor:
+    """Zoomed FFT between angular frequencies k_start and k_end
+    (czt.py:82-124)."""
+    renorm = tuple(m - 1 for m in output_shape) if include_end else tuple(output_shape)
+    w = tuple(np.exp(1j * (k_end - k_start) / n) for n in renorm)
+    a = np.exp(-1j * k_start)
+    return cztn(x, tuple(output_shape), tuple(a for _ in output_shape), w, axes)

@@ -257,6 +257,57 @@ int cxm_phase_change_c64(void* stream, const void* in, const float* phase, void*
                          const double* modulation);
 
 /*
+ * Chirp-z transform along one axis of a batch of height x width complex64
+ * planes (Bluestein), czt (src/chromatix/utils/czt.py:7-52):
+ *   y_k = post_k * scale * sum_{j<n} x_j a^{-j} w^{j k},  k = 0 .. m-1
+ * axis 1: along rows (n = width; out batch x height x m);
+ * axis 0: along columns (n = height; out batch x m x width).
+ * post_k = 1, or with mod_n = N >= 0 the modulation w^{-N k} a^{N} of the
+ * CZT branch of kernel_propagate (src/chromatix/functional/propagation.py:427-441).
+ * Plane p uses parameter set p % num_param (a, w: num_param complex values
+ * as (re, im) pairs, host memory).  One pass with an FFT of the next
+ * supported length >= n + m - 1.
+ */
+typedef struct cxm_czt_desc {
+  int32_t batch;
+  int32_t height, width;
+  int32_t axis;
+  int32_t m;
+  int32_t num_param;      /* 1..64 */
+  int32_t mod_n;          /* -1: post = 1 */
+  int32_t reserved;
+  const double* a;
+  const double* w;
+  double scale;
+} cxm_czt_desc;
+
+size_t cxm_czt_axis_workspace_size(const cxm_czt_desc* d);
+int cxm_czt_axis_c64(void* stream, const void* in, void* out, void* workspace,
+                     size_t workspace_bytes, const cxm_czt_desc* d);
+
+/*
+ * Elementwise product of n_planes planes of height x width complex64 with a
+ * stack of b_planes planes (plane p uses b[p % b_planes]), b optionally
+ * read fftshifted: out = a * fftshift(b).  The propagator multiply of the
+ * CZT branch of kernel_propagate, fftshift(H * fft2(...)) =
+ * fftshift(H) * fftshift(fft2(...)) (src/chromatix/functional/propagation.py:413-417).
+ * a == out is allowed.
+ */
+int cxm_cmul_c64(void* stream, const void* a, const void* b, void* out, int64_t n_planes,
+                 int32_t height, int32_t width, int32_t b_planes, int32_t shift_b);
+
+/*
+ * thin_sample (src/chromatix/functional/samples.py:85-134):
+ *   out[p, y, x, w] = in[p, y, x, w] * exp(i 2 pi (dn[y, x] + i absorption[y, x]) t / lam_w)
+ * for n_planes planes of height x width (x num_wv innermost) complex64, in
+ * the reference's complex64 operation order; absorption may be NULL (0);
+ * wavelengths: num_wv host values.  in == out is allowed.
+ */
+int cxm_thin_sample_c64(void* stream, const void* in, const float* dn, const float* absorption,
+                        void* out, int64_t n_planes, int32_t height, int32_t width,
+                        int32_t num_wv, const double* wavelengths, double thickness);
+
+/*
  * Multislice propagation through a thick sample (multislice_thick_sample,
  * src/chromatix/functional/samples.py:137-282, with thin_sample :85-134):
  * pad the field and the slices by (pad_y, pad_x); for each slice i,
@@ -327,6 +378,15 @@ int cxm_multislice_bwd(void* stream, const void* in, const float* dn, const floa
  */
 int cxm_intensity_c64(void* stream, const void* u, void* out, int64_t n_pixels,
                       int32_t num_wv, const double* density);
+
+/*
+ * Vector-Jacobian product of cxm_intensity_c64: out[p, w] = 2 density[w]
+ * g[p] u[p, w] (complex64), g = dL/dI (float32, n_pixels values).  The
+ * reverse-mode rule of .intensity (src/chromatix/core/field.py:663-665,
+ * 735-740) that JAX derives automatically.
+ */
+int cxm_intensity_vjp_c64(void* stream, const void* u, const float* g, void* out,
+                          int64_t n_pixels, int32_t num_wv, const double* density);
 
 /*
  * Per-plane sums of |u|^2 (float64 out, one per plane), optionally
