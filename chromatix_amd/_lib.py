@@ -145,6 +145,21 @@ class Psf4fDesc(ctypes.Structure):
 CXM_GRAD_COMPLEX = 2
 
 
+class MbsDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("spacing", ctypes.c_double),
+        ("k0", ctypes.c_double),
+        ("bias_re", ctypes.c_double),
+        ("bias_im", ctypes.c_double),
+        ("scale_re", ctypes.c_double),
+        ("scale_im", ctypes.c_double),
+    ]
+
+
 class CztDesc(ctypes.Structure):
     _fields_ = [
         ("batch", ctypes.c_int32),
@@ -228,6 +243,10 @@ def _declare(lib):
         "cxm_cmul_c64": (i32, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                ctypes.c_int32, ctypes.c_int32]),
         "cxm_czt_axis_workspace_size": (sz, [ctypes.POINTER(CztDesc)]),
+        "cxm_mbs_workspace_size": (sz, [ctypes.POINTER(MbsDesc)]),
+        "cxm_mbs_precondition": (i32, [vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
+        "cxm_mbs_iterate": (i32, [vp, vp, vp, vp, ctypes.c_int32, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
+        "cxm_mbs_output": (i32, [vp, vp, vp, vp, ctypes.POINTER(MbsDesc)]),
         "cxm_czt_axis_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(CztDesc)]),
         "cxm_multislice_workspace_size": (sz, [ctypes.POINTER(MultisliceDesc)]),
         "cxm_multislice_checkpoint_size": (sz, [ctypes.POINTER(MultisliceDesc)]),

@@ -1,0 +1,360 @@
+// Modified Born series (convergent Born series) solver for the vector
+// Helmholtz problem, after the preconditioned fixed-point iteration of
+// src/chromatix/experimental/modified_born_series/solver.py:42-146.
+//
+// Fields are three complex64 component volumes (component-planar
+// 3 x nx x ny x nz in the kernels; the boundary converts from the
+// reference's nx x ny x nz x 3).  With s = scale, b = bias, kb = b s + 1:
+//   V(x)  = x (s eps - kb)                                   (elementwise)
+//   G(y)  = ifftn( P_T y^ / (kb - s |K|^2) + P_L y^ / kb )   (K = k / k0)
+//   prec(y)          = -V(G(s y))
+//   prec_forward(x)  = -V(G(V(x)) + x)
+//   x <- x + prec_y - prec_forward(x) = x + prec_y + V(G(V(x)) + x)
+// One iteration: V pass, three forward FFT passes (z rows, y and x column
+// strips of the pipe kernels), the Green multiply (3x3 projection per k),
+// three inverse FFT passes, and the update pass, which also writes the FP64
+// partial sums of |r|^2 for the residual norm.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "cxm_pipe_host.h"
+
+namespace cxm {
+namespace {
+
+struct MbsGeom { int nx, ny, nz; long long vox; };
+
+int mbs_geom(const cxm_mbs_desc* d, MbsGeom& g) {
+  if (!d) return fail(CXM_ERR_INVALID, "null descriptor");
+  if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0)
+    return fail(CXM_ERR_INVALID, "invalid shape");
+  if (!(d->spacing > 0 && d->k0 > 0)) return fail(CXM_ERR_INVALID, "spacing and k0 must be positive");
+  for (int n : {d->nx, d->ny, d->nz})
+    if (!fft_len_supported(n))
+      return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(n));
+  g.nx = d->nx; g.ny = d->ny; g.nz = d->nz;
+  g.vox = (long long)d->nx * d->ny * d->nz;
+  return CXM_OK;
+}
+
+struct C2 { float re, im; };
+
+CXM_DEV float2 cmulf(float2 a, C2 b) { return make_float2(a.x * b.re - a.y * b.im, a.x * b.im + a.y * b.re); }
+
+// out[c][i] = src[c][i] * (s eps[i] - kb) * pre; src interleaved (i*3 + c)
+// when interleaved, else planar; eps may be null (factor 1 then, used for the
+// plain scaling of the source)
+__global__ void mbs_v_kernel(const float2* __restrict__ src, const float2* __restrict__ eps,
+                             float2* __restrict__ out, long long vox, int interleaved, C2 s, C2 kb,
+                             C2 pre) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vox) return;
+  C2 f = pre;
+  if (eps) {
+    const float2 e = eps[i];
+    const float2 se = make_float2(s.re * e.x - s.im * e.y - kb.re, s.re * e.y + s.im * e.x - kb.im);
+    f = C2{se.x * pre.re - se.y * pre.im, se.x * pre.im + se.y * pre.re};
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float2 v = interleaved ? src[i * 3 + c] : src[c * vox + i];
+    out[c * vox + i] = cmulf(v, f);
+  }
+}
+
+// In-place Green multiply of the spectrum Y (3 planar components):
+// Y <- P_T Y / (kb - s |K|^2) + P_L Y / kb, P_L = K K^T / |K|^2 (0 at K = 0)
+__global__ void mbs_green_kernel(float2* __restrict__ y, long long vox, int nx, int ny, int nz,
+                                 float kscale, C2 s, C2 kb) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vox) return;
+  const int iz = (int)(i % nz);
+  const int iy = (int)((i / nz) % ny);
+  const int ix = (int)(i / ((long long)nz * ny));
+  // K = 2 pi fftfreq / spacing / k0
+  const float kx = kscale * (float)signed_bin(ix, nx) / (float)nx;
+  const float ky = kscale * (float)signed_bin(iy, ny) / (float)ny;
+  const float kz = kscale * (float)signed_bin(iz, nz) / (float)nz;
+  const float k2 = kx * kx + ky * ky + kz * kz;
+  const float2 y0 = y[i], y1 = y[vox + i], y2 = y[2 * vox + i];
+  float2 p = make_float2(kx * y0.x + ky * y1.x + kz * y2.x, kx * y0.y + ky * y1.y + kz * y2.y);
+  const float inv = k2 != 0.0f ? 1.0f / k2 : 0.0f;
+  p = make_float2(p.x * inv, p.y * inv);
+  // 1 / (kb - s k2) and 1 / kb
+  const float dre = kb.re - s.re * k2, dim = kb.im - s.im * k2;
+  const float dn = dre * dre + dim * dim;
+  const C2 gt{dre / dn, -dim / dn};
+  const float bn = kb.re * kb.re + kb.im * kb.im;
+  const C2 gl{kb.re / bn, -kb.im / bn};
+  const float kk[3] = {kx, ky, kz};
+  const float2 yy[3] = {y0, y1, y2};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float2 yl = make_float2(kk[c] * p.x, kk[c] * p.y);
+    const float2 yt = make_float2(yy[c].x - yl.x, yy[c].y - yl.y);
+    const float2 a = cmulf(yt, gt), b = cmulf(yl, gl);
+    y[c * vox + i] = make_float2(a.x + b.x, a.y + b.y);
+  }
+}
+
+constexpr int kRedThreads = 256;
+
+// r = prec_y + V(g + x);  x <- x + r;  partial[block] = sum |r|^2 (FP64)
+__global__ void __launch_bounds__(kRedThreads)
+mbs_update_kernel(const float2* __restrict__ g, const float2* __restrict__ eps,
+                  const float2* __restrict__ py, float2* __restrict__ x, long long vox, C2 s, C2 kb,
+                  double* __restrict__ partial) {
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < vox;
+       i += (long long)gridDim.x * blockDim.x) {
+    const float2 e = eps[i];
+    const C2 f{s.re * e.x - s.im * e.y - kb.re, s.re * e.y + s.im * e.x - kb.im};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * vox + i;
+      const float2 xv = x[j];
+      const float2 t = make_float2(g[j].x + xv.x, g[j].y + xv.y);
+      const float2 v = cmulf(t, f);
+      const float2 r = make_float2(py[j].x + v.x, py[j].y + v.y);
+      x[j] = make_float2(xv.x + r.x, xv.y + r.y);
+      acc += (double)r.x * r.x + (double)r.y * r.y;
+    }
+  }
+  __shared__ double sh[kRedThreads];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// out[0] = sum partial (fixed order), out[1] (optional max) untouched
+__global__ void sum_partials_kernel(const double* __restrict__ partial, int n, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double acc = 0.0;
+  for (int i = 0; i < n; ++i) acc += partial[i];
+  out[0] = acc;
+}
+
+// partial[block] = max |v| over 3 planar components (as FP64)
+__global__ void __launch_bounds__(kRedThreads)
+maxabs_kernel(const float2* __restrict__ v, long long n, double* __restrict__ partial) {
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double a = sqrt((double)v[i].x * v[i].x + (double)v[i].y * v[i].y);
+    m = a > m ? a : m;
+  }
+  __shared__ double sh[kRedThreads];
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+__global__ void max_partials_kernel(const double* __restrict__ partial, int n, double* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double m = 0.0;
+  for (int i = 0; i < n; ++i) m = fmax(m, partial[i]);
+  out[0] = m;
+}
+
+// v[c][i] *= 1 / max (normalisation by max |prec_y|, read on the device)
+__global__ void scale_by_inv_kernel(float2* __restrict__ v, long long n, const double* __restrict__ m) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float f = (float)(1.0 / m[0]);
+  v[i] = make_float2(v[i].x * f, v[i].y * f);
+}
+
+// E[i*3 + c] = x[c][i] * num_scale (interleaved output)
+__global__ void mbs_output_kernel(const float2* __restrict__ x, float2* __restrict__ e, long long vox,
+                                  const double* __restrict__ num_scale) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vox) return;
+  const float f = (float)num_scale[0];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float2 v = x[c * vox + i];
+    e[i * 3 + c] = make_float2(v.x * f, v.y * f);
+  }
+}
+
+constexpr int kRedBlocks = 1024;
+
+unsigned blocks_for(long long n) { return (unsigned)std::max(1LL, std::min<long long>((n + 255) / 256, 0x7fffffffLL)); }
+
+// 3D FFT of the 3 planar component volumes in place (dir -1 forward, +1
+// inverse with the 1/(nx ny nz) scale on the last pass).
+int fft3(const MbsGeom& g, float2* v, int dir, hipStream_t s) {
+  int rc;
+  const float inv = (float)(1.0 / ((double)g.nx * g.ny * g.nz));
+  // z: contiguous rows
+  {
+    PipeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.in = v; a.out = v;
+    a.tw = twiddles_for(g.nz);
+    a.in_plane_stride = a.out_plane_stride = g.vox;
+    a.in_line_stride = a.out_line_stride = g.nz;
+    a.n_lines = g.nx * g.ny;
+    a.n_planes = 3;
+    a.grid_ny = g.nx * g.ny;
+    a.grid_nx = g.nz;
+    a.in_len = a.out_len = g.nz;
+    a.seq = dir < 0 ? SEQ_F : SEQ_I;
+    a.scale = a.mid_scale = 1.0f;
+    if ((rc = launch_pipe_row(g.nz, a, s))) return rc;
+  }
+  // y: column strips of each (ny x nz) slab, 3 nx planes
+  {
+    PipeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.in = v; a.out = v;
+    a.tw = twiddles_for(g.ny);
+    a.in_plane_stride = a.out_plane_stride = (long long)g.ny * g.nz;
+    a.in_line_stride = a.out_line_stride = g.nz;
+    a.n_lines = g.nz;
+    a.n_planes = 3 * g.nx;
+    a.grid_ny = g.ny;
+    a.grid_nx = g.nz;
+    a.in_len = a.out_len = g.ny;
+    a.out_line_len = g.nz;
+    a.seq = dir < 0 ? SEQ_F : SEQ_I;
+    a.scale = a.mid_scale = 1.0f;
+    if ((rc = launch_pipe_col(g.ny, a, s))) return rc;
+  }
+  // x: column strips of the (nx x ny nz) matrix, 3 planes
+  {
+    PipeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.in = v; a.out = v;
+    a.tw = twiddles_for(g.nx);
+    a.in_plane_stride = a.out_plane_stride = g.vox;
+    a.in_line_stride = a.out_line_stride = g.ny * g.nz;
+    a.n_lines = g.ny * g.nz;
+    a.n_planes = 3;
+    a.grid_ny = g.nx;
+    a.grid_nx = g.ny * g.nz;
+    a.in_len = a.out_len = g.nx;
+    a.out_line_len = g.ny * g.nz;
+    a.seq = dir < 0 ? SEQ_F : SEQ_I;
+    a.scale = dir < 0 ? 1.0f : inv;
+    a.mid_scale = 1.0f;
+    if ((rc = launch_pipe_col(g.nx, a, s))) return rc;
+  }
+  return CXM_OK;
+}
+
+C2 c2(double re, double im) { return C2{(float)re, (float)im}; }
+
+}  // namespace
+}  // namespace cxm
+
+using namespace cxm;
+
+// workspace: gbuf (3 vox c64) | partial (kRedBlocks f64) | scalars (4 f64)
+extern "C" size_t cxm_mbs_workspace_size(const cxm_mbs_desc* d) {
+  MbsGeom g;
+  if (mbs_geom(d, g)) return 0;
+  return (size_t)3 * g.vox * sizeof(float2) + (kRedBlocks + 4) * sizeof(double);
+}
+
+extern "C" int cxm_mbs_precondition(void* stream, const void* eps, const void* source,
+                                    void* prec_y, double* num_scale, void* workspace,
+                                    size_t workspace_bytes, const cxm_mbs_desc* d) {
+  MbsGeom g;
+  int rc = mbs_geom(d, g);
+  if (rc) return rc;
+  if (!eps || !source || !prec_y || !num_scale) return fail(CXM_ERR_INVALID, "null buffer");
+  if (!workspace || workspace_bytes < cxm_mbs_workspace_size(d))
+    return fail(CXM_ERR_WORKSPACE, "workspace too small");
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const C2 sc = c2(d->scale_re, d->scale_im);
+  const C2 kb = c2(d->bias_re * d->scale_re - d->bias_im * d->scale_im + 1.0,
+                   d->bias_re * d->scale_im + d->bias_im * d->scale_re);
+  float2* py = (float2*)prec_y;
+  double* partial = (double*)((char*)workspace + (size_t)3 * g.vox * sizeof(float2));
+  // t = s * source (planar), G(t), prec_y = -V(G(t))
+  hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)source,
+                     nullptr, py, g.vox, 1, sc, kb, sc);
+  CXM_LAUNCH_CHECK();
+  if ((rc = fft3(g, py, -1, s))) return rc;
+  const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
+  hipLaunchKernelGGL(mbs_green_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, py, g.vox, g.nx, g.ny,
+                     g.nz, kscale, sc, kb);
+  CXM_LAUNCH_CHECK();
+  if ((rc = fft3(g, py, 1, s))) return rc;
+  hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)py,
+                     (const float2*)eps, py, g.vox, 0, sc, kb, C2{-1.0f, 0.0f});
+  CXM_LAUNCH_CHECK();
+  // num_scale = max |prec_y|; prec_y /= num_scale
+  hipLaunchKernelGGL(maxabs_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, (const float2*)py,
+                     3 * g.vox, partial);
+  CXM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(max_partials_kernel, dim3(1), dim3(64), 0, s, partial, kRedBlocks, num_scale);
+  CXM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(scale_by_inv_kernel, dim3(blocks_for(3 * g.vox)), dim3(256), 0, s, py, 3 * g.vox,
+                     (const double*)num_scale);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+extern "C" int cxm_mbs_iterate(void* stream, const void* eps, const void* prec_y, void* x,
+                               int32_t iterations, double* residual_sq, void* workspace,
+                               size_t workspace_bytes, const cxm_mbs_desc* d) {
+  MbsGeom g;
+  int rc = mbs_geom(d, g);
+  if (rc) return rc;
+  if (iterations < 0) return fail(CXM_ERR_INVALID, "iterations must be >= 0");
+  if (!eps || !prec_y || !x || !residual_sq) return fail(CXM_ERR_INVALID, "null buffer");
+  if (!workspace || workspace_bytes < cxm_mbs_workspace_size(d))
+    return fail(CXM_ERR_WORKSPACE, "workspace too small");
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const C2 sc = c2(d->scale_re, d->scale_im);
+  const C2 kb = c2(d->bias_re * d->scale_re - d->bias_im * d->scale_im + 1.0,
+                   d->bias_re * d->scale_im + d->bias_im * d->scale_re);
+  float2* gb = (float2*)workspace;
+  double* partial = (double*)((char*)workspace + (size_t)3 * g.vox * sizeof(float2));
+  const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
+  for (int it = 0; it < iterations; ++it) {
+    hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)x,
+                       (const float2*)eps, gb, g.vox, 0, sc, kb, C2{1.0f, 0.0f});
+    CXM_LAUNCH_CHECK();
+    if ((rc = fft3(g, gb, -1, s))) return rc;
+    hipLaunchKernelGGL(mbs_green_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, gb, g.vox, g.nx,
+                       g.ny, g.nz, kscale, sc, kb);
+    CXM_LAUNCH_CHECK();
+    if ((rc = fft3(g, gb, 1, s))) return rc;
+    hipLaunchKernelGGL(mbs_update_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s,
+                       (const float2*)gb, (const float2*)eps, (const float2*)prec_y, (float2*)x,
+                       g.vox, sc, kb, partial);
+    CXM_LAUNCH_CHECK();
+  }
+  if (iterations > 0) {
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, s, partial, kRedBlocks, residual_sq);
+    CXM_LAUNCH_CHECK();
+  }
+  return CXM_OK;
+}
+
+extern "C" int cxm_mbs_output(void* stream, const void* x, const double* num_scale, void* e,
+                              const cxm_mbs_desc* d) {
+  MbsGeom g;
+  int rc = mbs_geom(d, g);
+  if (rc) return rc;
+  if (!x || !num_scale || !e) return fail(CXM_ERR_INVALID, "null buffer");
+  hipLaunchKernelGGL(mbs_output_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)x, (float2*)e, g.vox, num_scale);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}

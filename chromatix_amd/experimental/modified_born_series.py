@@ -1,0 +1,215 @@
+"""Modified Born series solver on libcxm (mirrors
+src/chromatix/experimental/modified_born_series/{sample,solver}.py).
+
+The preconditioned fixed-point iteration runs on the GPU (cxm_mbs_*): one
+launch sequence per iteration, V multiply, 3D FFT, Green multiply, inverse
+3D FFT and the update with an FP64 residual norm.  Setup (absorbing
+boundary, the bias/scale of get_shift_and_scale) is host code.
+
+get_shift_and_scale: the reference minimises max |eps - shift|^2 with a
+nonlinear CG to 1e-3 tolerance; here the minimiser is computed exactly, as
+the centre of the smallest circle enclosing the permittivity values in the
+complex plane (Welzl's algorithm on the distinct values).  Any shift/scale
+meeting the accretivity conditions gives the same converged field; the
+iteration count depends on it.
+"""
+from __future__ import annotations
+
+import ctypes
+import random
+
+import numpy as np
+import scipy.constants as const
+import torch
+
+from .. import _lib as L
+
+__all__ = ["Sample", "EmptySample", "Source", "add_absorbing_bc", "get_shift_and_scale", "solve",
+           "MBSResult"]
+
+
+class Sample:
+    """Permittivity on a uniform grid (sample.py:9-73), (Nx, Ny, Nz)."""
+
+    def __init__(self, permittivity, spacing: float, roi=None):
+        self.permittivity = torch.as_tensor(permittivity)
+        self.spacing = float(spacing)
+        self.ROI = roi if roi is not None else tuple(slice(0, s) for s in self.shape)
+
+    @property
+    def shape(self):
+        return tuple(self.permittivity.shape)
+
+    @property
+    def grid(self) -> np.ndarray:
+        axes = [np.linspace(0, n - 1, n) - (n // 2 if n != 1 else 0) for n in self.shape]
+        return np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1) * self.spacing
+
+    @property
+    def k_grid(self) -> np.ndarray:
+        axes = [np.fft.fftfreq(n) for n in self.shape]
+        return 2 * np.pi * np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1) / self.spacing
+
+    @property
+    def extent(self) -> np.ndarray:
+        return np.asarray(self.shape) * self.spacing
+
+    def replace(self, **kw) -> "Sample":
+        new = Sample(kw.get("permittivity", self.permittivity), kw.get("spacing", self.spacing),
+                     kw.get("ROI", self.ROI))
+        return new
+
+
+def EmptySample(shape, spacing: float, device=None) -> Sample:
+    return Sample(torch.zeros(tuple(shape), dtype=torch.complex64, device=device), spacing)
+
+
+class Source:
+    """Pre-scaled current density: field = -i / k0 1e-6 c mu0 J (sample.py:76-89)."""
+
+    def __init__(self, current_density, k0: float):
+        self.k0 = float(k0)
+        J = torch.as_tensor(current_density)
+        self.field = (J.to(torch.complex64) * complex(-1j / self.k0 * 1e-6 * const.c * const.mu_0))
+
+    @property
+    def current_density(self):
+        return self.field / complex(-1j / self.k0 * 1e-6 * const.c * const.mu_0)
+
+
+def add_absorbing_bc(sample: Sample, axis, thickness: float, max_extinction: float = 0.25,
+                     n_background: float = 1.0) -> Sample:
+    """Pad by int(thickness / spacing) per side along ``axis`` with a linear
+    extinction ramp to ``max_extinction``: eps += (n_bg + i kappa)^2 - n_bg^2
+    (sample.py:92-124)."""
+    axes = axis if isinstance(axis, tuple) else (axis,)
+    w = int(thickness / sample.spacing)
+    widths = [(w, w) if i in axes else (0, 0) for i in range(3)]
+    ext = np.pad(np.zeros(sample.shape), widths, mode="linear_ramp", end_values=max_extinction)
+    eps_bc = (n_background + 1j * ext) ** 2
+    perm = np.pad(sample.permittivity.detach().cpu().numpy().astype(np.complex128), widths,
+                  mode="constant", constant_values=n_background**2)
+    roi = tuple(slice(a, a + s) for (a, _), s in zip(widths, sample.shape))
+    out = torch.as_tensor((perm + (eps_bc - n_background**2)).astype(np.complex64),
+                          device=sample.permittivity.device)
+    return Sample(out, sample.spacing, roi)
+
+
+def _circle2(a, b):
+    c = (a + b) / 2
+    return c, abs(a - c)
+
+
+def _circle3(a, b, c):
+    ax, ay, bx, by, cx, cy = a.real, a.imag, b.real, b.imag, c.real, c.imag
+    d = 2 * (ax * (by - cy) + bx * (cy - ay) + cx * (ay - by))
+    if abs(d) < 1e-300:
+        # collinear: the widest pair
+        best = max(((a, b), (a, c), (b, c)), key=lambda p: abs(p[0] - p[1]))
+        return _circle2(*best)
+    ux = ((ax * ax + ay * ay) * (by - cy) + (bx * bx + by * by) * (cy - ay) + (cx * cx + cy * cy) * (ay - by)) / d
+    uy = ((ax * ax + ay * ay) * (cx - bx) + (bx * bx + by * by) * (ax - cx) + (cx * cx + cy * cy) * (bx - ax)) / d
+    o = complex(ux, uy)
+    return o, abs(a - o)
+
+
+def enclosing_circle(points) -> tuple[complex, float]:
+    """Smallest circle enclosing complex points (Welzl, iterative form)."""
+    pts = [complex(p) for p in np.unique(np.asarray(points, np.complex128).ravel())]
+    random.Random(0).shuffle(pts)
+    eps = 1e-12
+    c, r = pts[0], 0.0
+    for i, p in enumerate(pts):
+        if abs(p - c) <= r * (1 + eps) + eps:
+            continue
+        c, r = p, 0.0
+        for j in range(i):
+            q = pts[j]
+            if abs(q - c) <= r * (1 + eps) + eps:
+                continue
+            c, r = _circle2(p, q)
+            for k in range(j):
+                t = pts[k]
+                if abs(t - c) <= r * (1 + eps) + eps:
+                    continue
+                c, r = _circle3(p, q, t)
+    return c, r
+
+
+def get_shift_and_scale(permittivity) -> tuple[complex, complex]:
+    """(shift, scale) with shift minimising max |eps - shift| and
+    scale = 1 / (1.1 i sqrt(f)), f = max |eps - shift|^2 (solver.py:10-39)."""
+    eps = torch.as_tensor(permittivity).detach().cpu().numpy()
+    c, r = enclosing_circle(eps)
+    if r == 0:
+        r = 1e-6  # homogeneous medium: any positive radius keeps the iteration accretive
+    return c, 1.0 / (1.1j * r)
+
+
+class MBSResult:
+    def __init__(self, E, iterations, residual):
+        self.E, self.iterations, self.residual = E, iterations, residual
+
+
+def _desc(sample: Sample, k0: float, shift: complex, scale: complex):
+    d = L.MbsDesc()
+    d.nx, d.ny, d.nz = sample.shape
+    d.spacing = sample.spacing
+    d.k0 = float(k0)
+    d.bias_re, d.bias_im = shift.real, shift.imag
+    d.scale_re, d.scale_im = scale.real, scale.imag
+    return d
+
+
+def solve(sample: Sample, source: Source, initial_E=None, maxiter: int = 1000, tol: float = 1e-3,
+          implicit_diff: bool = True, use_bicgstab: bool = False, *, rtol: float | None = None,
+          check_every: int = 8, shift_scale=None, return_info: bool = False):
+    """Electric field E (Nx, Ny, Nz, 3) solving the scattering problem
+    (solver.py:101-146).  Stops when the l2 norm of the update of the
+    normalised iterate falls below ``tol`` (jaxopt's FixedPointIteration
+    criterion) or, with ``rtol``, below rtol |prec_y|; the norm is read back
+    every ``check_every`` iterations."""
+    if use_bicgstab:
+        raise NotImplementedError("use_bicgstab=True is not implemented; the fixed-point "
+                                  "iteration is")
+    eps = sample.permittivity.to(torch.complex64).contiguous()
+    L.ensure_device(eps)
+    dev = eps.device
+    S = torch.as_tensor(source.field, device=dev).to(torch.complex64).contiguous()
+    if tuple(S.shape) != sample.shape + (3,):
+        raise ValueError(f"source field must have shape {sample.shape + (3,)}")
+    shift, scale = shift_scale if shift_scale is not None else get_shift_and_scale(eps)
+    d = _desc(sample, source.k0, complex(shift), complex(scale))
+    vox = int(np.prod(sample.shape))
+    ws = torch.empty(L.lib().cxm_mbs_workspace_size(ctypes.byref(d)), dtype=torch.uint8, device=dev)
+    py = torch.empty((3,) + sample.shape, dtype=torch.complex64, device=dev)
+    num = torch.empty(2, dtype=torch.float64, device=dev)
+    s = L.stream_ptr(dev)
+    L.check(L.lib().cxm_mbs_precondition(s, eps.data_ptr(), S.data_ptr(), py.data_ptr(),
+                                         num.data_ptr(), ws.data_ptr(), ws.numel(), ctypes.byref(d)))
+    if initial_E is None:
+        x = py.clone()
+    else:
+        E0 = torch.as_tensor(initial_E, device=dev).to(torch.complex64)
+        x = (E0.permute(3, 0, 1, 2) / num[0].to(torch.float32)).contiguous()
+    if rtol is not None:
+        from ..functional.elementwise import plane_sumsq_raw
+
+        target = float(rtol) * float(np.sqrt(plane_sumsq_raw(py)))
+    else:
+        target = float(tol)
+    res = torch.empty(1, dtype=torch.float64, device=dev)
+    it, r = 0, float("inf")
+    while it < maxiter:
+        k = min(check_every, maxiter - it)
+        L.check(L.lib().cxm_mbs_iterate(s, eps.data_ptr(), py.data_ptr(), x.data_ptr(), k,
+                                        res.data_ptr(), ws.data_ptr(), ws.numel(), ctypes.byref(d)))
+        it += k
+        r = float(np.sqrt(res.item()))
+        if r < target:
+            break
+    E = torch.empty(sample.shape + (3,), dtype=torch.complex64, device=dev)
+    L.check(L.lib().cxm_mbs_output(s, x.data_ptr(), num.data_ptr(), E.data_ptr(), ctypes.byref(d)))
+    if return_info:
+        return MBSResult(E, it, r)
+    return E

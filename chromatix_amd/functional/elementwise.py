@@ -101,3 +101,15 @@ def power(field) -> torch.Tensor:
         area = float(np.prod(field.dx))
         p = s[..., None, None] * area
     return p.to(torch.float32)
+
+
+def plane_sumsq_raw(u: torch.Tensor) -> float:
+    """sum |u|^2 over the whole complex64 tensor, reduced in FP64 on the device."""
+    u = u.contiguous()
+    L.ensure_device(u)
+    out = torch.empty(1, dtype=torch.float64, device=u.device)
+    ws_bytes = L.lib().cxm_plane_sumsq_workspace_size(1)
+    ws = torch.empty(max(1, ws_bytes), dtype=torch.uint8, device=u.device)
+    L.check(L.lib().cxm_plane_sumsq_c64(L.stream_ptr(u.device), u.data_ptr(), out.data_ptr(), 1,
+                                        u.numel(), 1, None, ws.data_ptr(), ws_bytes))
+    return float(out.item())

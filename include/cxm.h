@@ -369,6 +369,39 @@ int cxm_multislice_bwd(void* stream, const void* in, const float* dn, const floa
                        const cxm_multislice_desc* d);
 
 /*
+ * Modified Born series solve (src/chromatix/experimental/modified_born_series/solver.py:42-146)
+ * of (curl curl - k0^2 eps) E = -k0^2 S for a scalar permittivity eps
+ * (nx x ny x nz complex64) and the pre-scaled source S (Source.field,
+ * sample.py:79-85; nx x ny x nz x 3 complex64), with the bias (shift) and
+ * scale of get_shift_and_scale chosen by the caller:
+ *   cxm_mbs_precondition: prec_y = prec(S) / num_scale, num_scale = max |prec(S)|
+ *                         (prec_y: 3 x nx x ny x nz component-planar; num_scale:
+ *                         one device double)
+ *   cxm_mbs_iterate:      `iterations` steps x <- x + prec_y - prec_forward(x) on the
+ *                         planar x; residual_sq (device double) = |r|^2 of the last
+ *                         step, r = x_new - x
+ *   cxm_mbs_output:       E = x num_scale, nx x ny x nz x 3
+ * Workspace: cxm_mbs_workspace_size (one field plus reduction scratch).
+ */
+typedef struct cxm_mbs_desc {
+  int32_t nx, ny, nz;     /* grid including absorbing boundaries */
+  int32_t reserved;
+  double spacing, k0;     /* grid spacing, vacuum wavenumber (same length unit) */
+  double bias_re, bias_im;
+  double scale_re, scale_im;
+} cxm_mbs_desc;
+
+size_t cxm_mbs_workspace_size(const cxm_mbs_desc* d);
+int cxm_mbs_precondition(void* stream, const void* eps, const void* source, void* prec_y,
+                         double* num_scale, void* workspace, size_t workspace_bytes,
+                         const cxm_mbs_desc* d);
+int cxm_mbs_iterate(void* stream, const void* eps, const void* prec_y, void* x, int32_t iterations,
+                    double* residual_sq, void* workspace, size_t workspace_bytes,
+                    const cxm_mbs_desc* d);
+int cxm_mbs_output(void* stream, const void* x, const double* num_scale, void* e,
+                   const cxm_mbs_desc* d);
+
+/*
  * Intensity |u|^2 of n_pixels complex64 values, or for an innermost
  * wavelength axis of length num_wv the density-weighted sum over it:
  *   out[p] = sum_w density[w] |u[p*num_wv + w]|^2   (float32 out)

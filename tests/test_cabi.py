@@ -37,7 +37,8 @@ def test_every_declared_symbol_is_exported():
                                        (L.OptFftDesc, "cxm_optfft_desc"),
                                        (L.Psf4fDesc, "cxm_psf4f_desc"),
                                        (L.MultisliceDesc, "cxm_multislice_desc"),
-                                       (L.CztDesc, "cxm_czt_desc")])
+                                       (L.CztDesc, "cxm_czt_desc"),
+                                       (L.MbsDesc, "cxm_mbs_desc")])
 def test_descriptor_layout_matches_header(tmp_path, cls, cname):
     fields = [f for f, _ in cls._fields_]
     src = tmp_path / "layout.c"
