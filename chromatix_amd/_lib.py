@@ -219,6 +219,9 @@ def _declare(lib):
         "cxm_prop2d_workspace_size": (sz, [ctypes.POINTER(Prop2dDesc)]),
         "cxm_prop2d_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Prop2dDesc)]),
         "cxm_propagator_c64": (i32, [vp, vp, ctypes.POINTER(Prop2dDesc)]),
+        "cxm_prop2d_param_bytes": (sz, []),
+        "cxm_prop2d_build_params": (i32, [ctypes.POINTER(Prop2dDesc), dp, ctypes.c_int64, vp]),
+        "cxm_prop2d_c64_params": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Prop2dDesc), vp]),
         "cxm_set_option": (i32, [i32, i32]),
         "cxm_optical_fft_workspace_size": (sz, [ctypes.POINTER(OptFftDesc)]),
         "cxm_optical_fft_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(OptFftDesc)]),

@@ -79,6 +79,9 @@ struct ColArgs {
   float na_r2;           // > 0: H also masked to |f|^2 <= na_r2 (fl((NA/lam)^2), samples.py:271-280)
   int nhp;
   int pad_;
+  // optional device table, one HParam per output plane (global input plane
+  // p, distance iz -> hp_dev[p * nout + iz]); replaces hp[] when non-null
+  const HParam* hp_dev;
   HParam hp[kMaxHP];
 };
 

@@ -137,8 +137,9 @@ extern "C" size_t cxm_prop2d_workspace_size(const cxm_prop2d_desc* d) {
   return ws_elems(d, g) * sizeof(float2);
 }
 
-extern "C" int cxm_prop2d_c64(void* stream, const void* in, void* out, void* workspace,
-                              size_t workspace_bytes, const cxm_prop2d_desc* d) {
+namespace {
+int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t workspace_bytes,
+                const cxm_prop2d_desc* d, const HParam* hp_dev) {
   Geom g;
   int rc = validate(d, g);
   if (rc) return rc;
@@ -152,7 +153,12 @@ extern "C" int cxm_prop2d_c64(void* stream, const void* in, void* out, void* wor
 
   ColArgs ca;
   memset(&ca, 0, sizeof(ca));
-  if (d->kind == CXM_H_TRANSFER || d->kind == CXM_H_ASM) {
+  if (hp_dev) {
+    if (d->kind != CXM_H_TRANSFER && d->kind != CXM_H_ASM)
+      return fail(CXM_ERR_INVALID, "a parameter table needs kind TRANSFER or ASM");
+    if (d->z_zip) return fail(CXM_ERR_INVALID, "a parameter table replaces z_zip");
+    ca.hp_dev = hp_dev;
+  } else if (d->kind == CXM_H_TRANSFER || d->kind == CXM_H_ASM) {
     if ((rc = build_hparams(d, g.Ny, g.Nx, ca.hp, kMaxHP))) return rc;
     ca.nhp = (d->num_wv > 0 ? d->num_wv : 1) * d->num_z;
   }
@@ -232,6 +238,58 @@ extern "C" int cxm_prop2d_c64(void* stream, const void* in, void* out, void* wor
     if ((rc = launch_row(g.Nx, 1, r2, s))) return rc;
   }
   return CXM_OK;
+}
+}  // namespace
+
+extern "C" int cxm_prop2d_c64(void* stream, const void* in, void* out, void* workspace,
+                              size_t workspace_bytes, const cxm_prop2d_desc* d) {
+  return prop2d_impl(stream, in, out, workspace, workspace_bytes, d, nullptr);
+}
+
+extern "C" size_t cxm_prop2d_param_bytes(void) { return sizeof(HParam); }
+
+extern "C" int cxm_prop2d_build_params(const cxm_prop2d_desc* d, const double* z_planes,
+                                       int64_t n_planes, void* host_out) {
+  Geom g;
+  cxm_prop2d_desc t = *d;
+  double z0 = 1.0;
+  t.num_z = 1;
+  t.z_zip = 0;
+  t.z = &z0;
+  int rc = validate(&t, g);
+  if (rc) return rc;
+  if (n_planes < 0 || (n_planes > 0 && (!z_planes || !host_out)))
+    return fail(CXM_ERR_INVALID, "null buffer");
+  const int nwv = d->num_wv > 0 ? d->num_wv : 1;
+  const int nout = d->num_z > 0 ? d->num_z : 1;
+  HParam* o = (HParam*)host_out;
+  for (int64_t i = 0; i < n_planes; ++i) {
+    // output plane i = (input plane p, distance iz); wavelength p % nwv
+    const int64_t p = i / nout;
+    const int w = (int)(p % nwv);
+    cxm_prop2d_desc one = t;
+    one.num_wv = 1;
+    one.z = &z_planes[i];
+    one.wavelength = d->wavelengths ? d->wavelengths[w] : d->wavelength;
+    one.wavelengths = nullptr;
+    if (d->spacing) {
+      one.dy = d->spacing[2 * w];
+      one.dx = d->spacing[2 * w + 1];
+    }
+    one.spacing = nullptr;
+    if ((rc = build_hparams(&one, g.Ny, g.Nx, &o[i], 1))) return rc;
+  }
+  return CXM_OK;
+}
+
+extern "C" int cxm_prop2d_c64_params(void* stream, const void* in, void* out, void* workspace,
+                                     size_t workspace_bytes, const cxm_prop2d_desc* d,
+                                     const void* params) {
+  if (!params) return fail(CXM_ERR_INVALID, "null parameter table");
+  cxm_prop2d_desc t = *d;
+  double z0 = 1.0;
+  t.z = &z0;  // distances come from the table
+  return prop2d_impl(stream, in, out, workspace, workspace_bytes, &t, (const HParam*)params);
 }
 
 // ---------------------------------------------------------------------------

@@ -113,7 +113,8 @@ col_prop_kernel(ColArgs a) {
     fft<N, E, -1, C, true>(u, lds + c, ty, a.tw);
     if constexpr (KIND == CXM_H_TRANSFER || KIND == CXM_H_ASM) {
       const int zi = a.z_zip ? (pg / a.nwv) % a.nz : iz;
-      const HParam& hp = a.hp[w * a.nz + zi];
+      const HParam* hpp = a.hp_dev ? a.hp_dev + ((long long)pg * nout + iz) : &a.hp[w * a.nz + zi];
+      const HParam hp = *hpp;
       const XTerms xt = x_terms(hp, kx, a.ncols, MODE);
       // NA mask |f|^2 <= (NA/lam)^2 on the unshifted frequency grid,
       // f = fl(fl(1/d) * fl(k/N)) as the reference's f_grid

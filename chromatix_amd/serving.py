@@ -1,0 +1,414 @@
+"""Request-serving loop for the propagation path: continuous batching of
+PSF / propagation requests by shape bucket with one captured HIP graph per
+bucket, SAS + CZT zoom and Born requests alongside, admission by an HBM
+budget, p50/p99 latency and throughput.
+
+Propagation requests ("a field, a phase mask, Z defocus distances ->
+intensity stack") of equal (grid, Z) share a bucket.  Each bucket keeps
+device-resident slots: B input planes, a table of B*Z transfer-function
+parameter records (cxm_prop2d_c64_params: one record per output plane, so
+every request keeps its own distances) and the B*Z output planes.  A batch
+is: per request one phase-mask kernel writing u * exp(i phi) into its slot,
+then one graph replay (parameter-table copy from pinned host memory + the
+three fused propagation passes).  Requests that arrive while batches run
+are admitted as slots and memory free up; batches of different buckets run
+on different streams and overlap on the device.
+
+SAS + CZT requests (a chromatic field -> scalable angular spectrum
+propagation -> per wavelength a CZT zoom onto a smaller output window) run
+eagerly on their own stream; Born requests (modified Born series solves,
+which read their residual back every few iterations) run on a worker
+thread with its own stream.
+
+The plain form for comparison runs every request alone, in arrival order,
+one library call per operation, waiting for each to finish.
+"""
+from __future__ import annotations
+
+import ctypes
+import threading
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .core.field import ChromaticScalarField, ScalarField
+from .core.spectrum import Spectrum
+
+__all__ = ["PropRequest", "SasCztRequest", "BornRequest", "ServingEngine", "run_plain",
+           "latency_stats"]
+
+
+@dataclass
+class PropRequest:
+    """Field u (N, N) complex64 and phase mask (N, N) float32 on the GPU,
+    propagated (angular spectrum) to each of the distances z (Z,);
+    result: intensity (Z, N, N) float32."""
+    u: torch.Tensor
+    phase: torch.Tensor
+    z: np.ndarray
+    dx: float
+    wavelength: float
+    n: float
+    arrival: float = 0.0
+    result: torch.Tensor | None = None
+    done: float | None = None
+
+
+@dataclass
+class SasCztRequest:
+    """Chromatic field (N, N, wv) propagated by z with the scalable angular
+    spectrum method, then zoomed by a CZT onto an (m, m) window of spacing
+    ``zoom_dx`` per wavelength; result (m, m, wv) complex64."""
+    u: torch.Tensor
+    dx: float
+    wavelengths: np.ndarray
+    z: float
+    n: float
+    m: int
+    zoom: float
+    arrival: float = 0.0
+    result: torch.Tensor | None = None
+    done: float | None = None
+
+
+@dataclass
+class BornRequest:
+    """Modified Born series solve of a permittivity volume with a source."""
+    permittivity: torch.Tensor
+    current: torch.Tensor
+    spacing: float
+    k0: float
+    rtol: float = 1e-4
+    arrival: float = 0.0
+    result: torch.Tensor | None = None
+    done: float | None = None
+    iterations: int = 0
+
+
+def _prop_desc(N, Z, dx, wavelength, n, batch):
+    d = L.Prop2dDesc()
+    d.batch, d.height, d.width = batch, N, N
+    d.num_z = Z
+    d.kind = L.CXM_H_ASM
+    d.phase_mode = L.CXM_PHASE_REF32
+    d.out_kind = L.CXM_OUT_INTENSITY
+    d.num_wv = 1
+    d.dy = d.dx = float(dx)
+    d.wavelength = float(wavelength)
+    d.n = float(n)
+    return d
+
+
+def _phase_into(u, phase, dst, stream_ptr):
+    N = u.shape[-1]
+    L.check(L.lib().cxm_phase_change_c64(stream_ptr, u.data_ptr(), phase.data_ptr(), dst.data_ptr(),
+                                         1, N, N, 1, None))
+
+
+class _Bucket:
+    """Graph instance for (N, Z, B): slots, parameter table, outputs."""
+
+    def __init__(self, N, Z, B, dx, wavelength, n, device, use_graph):
+        self.N, self.Z, self.B = N, Z, B
+        self.d = _prop_desc(N, Z, dx, wavelength, n, B)
+        self.rec = L.lib().cxm_prop2d_param_bytes()
+        self.slots = torch.zeros((B, N, N), dtype=torch.complex64, device=device)
+        self.out = torch.empty((B * Z, N, N), dtype=torch.float32, device=device)
+        self.params_host = torch.empty(B * Z * self.rec, dtype=torch.uint8, pin_memory=True)
+        self.params_dev = torch.empty(B * Z * self.rec, dtype=torch.uint8, device=device)
+        self.ws = torch.empty(max(1, L.lib().cxm_prop2d_workspace_size(ctypes.byref(self.d))),
+                              dtype=torch.uint8, device=device)
+        self.stream = torch.cuda.Stream(device=device)
+        self.event = torch.cuda.Event()
+        self.busy = False
+        self.requests: list = []
+        self.graph = None
+        if use_graph:
+            self._capture()
+
+    def _launch(self):
+        self.params_dev.copy_(self.params_host, non_blocking=True)
+        L.check(L.lib().cxm_prop2d_c64_params(
+            L.stream_ptr(self.slots.device), self.slots.data_ptr(), self.out.data_ptr(),
+            self.ws.data_ptr(), self.ws.numel(), ctypes.byref(self.d), self.params_dev.data_ptr()))
+
+    def _capture(self):
+        # a valid table for the warm-up and capture launches
+        self._fill_params([np.zeros(self.Z) + 1.0] * self.B)
+        with torch.cuda.stream(self.stream):
+            self._launch()  # warm-up outside capture
+        self.stream.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph, stream=self.stream):
+            self._launch()
+        self.stream.synchronize()
+
+    def _fill_params(self, zlists):
+        zs = np.zeros(self.B * self.Z)
+        for b, z in enumerate(zlists):
+            zs[b * self.Z:(b + 1) * self.Z] = z
+        zbuf = L.doubles(zs)
+        L.check(L.lib().cxm_prop2d_build_params(ctypes.byref(self.d),
+                                                ctypes.cast(zbuf, ctypes.POINTER(ctypes.c_double)),
+                                                self.B * self.Z,
+                                                ctypes.c_void_p(self.params_host.data_ptr())))
+
+    def start(self, reqs):
+        self.requests = list(reqs)
+        self._fill_params([r.z for r in reqs] + [np.ones(self.Z)] * (self.B - len(reqs)))
+        # the inputs were written on the submitting stream
+        self.stream.wait_stream(torch.cuda.current_stream(self.slots.device))
+        with torch.cuda.stream(self.stream):
+            sp = L.stream_ptr(self.slots.device)
+            for b, r in enumerate(reqs):
+                _phase_into(r.u, r.phase, self.slots[b], sp)
+            if self.graph is not None:
+                self.graph.replay()
+            else:
+                self._launch()
+            self.event.record(self.stream)
+        self.busy = True
+
+    def poll(self, keep_results: bool, now_fn):
+        if not self.busy or not self.event.query():
+            return []
+        t = now_fn()
+        done = []
+        for b, r in enumerate(self.requests):
+            if keep_results:
+                with torch.cuda.stream(self.stream):
+                    r.result = self.out[b * self.Z:(b + 1) * self.Z].clone()
+            r.done = t
+            done.append(r)
+        self.busy = False
+        self.requests = []
+        return done
+
+
+def _sas_czt(r: SasCztRequest, stream):
+    from .functional.czt_propagation import czt_kernel_propagate
+    from .functional.optics import transform_propagate_sas
+
+    with torch.cuda.stream(stream):
+        f = ChromaticScalarField(r.u, r.dx, 0.0, Spectrum(r.wavelengths))
+        s = transform_propagate_sas(f, r.z, r.n)
+        N = r.u.shape[0]
+        ones = torch.ones((N, N), dtype=torch.complex64, device=r.u.device)
+        outs = []
+        for w in range(len(r.wavelengths)):
+            fw = ScalarField(s.u[..., w].contiguous(), s.dx[w], 0.0, float(r.wavelengths[w]))
+            z = czt_kernel_propagate(fw, ones, output_dx=fw.dx / r.zoom, output_shape=(r.m, r.m))
+            outs.append(z.u)
+        return torch.stack(outs, dim=-1)
+
+
+def _born(r: BornRequest):
+    from .experimental import modified_born_series as mbs
+
+    sample = mbs.Sample(r.permittivity, r.spacing)
+    src = mbs.Source(r.current, r.k0)
+    res = mbs.solve(sample, src, maxiter=3000, rtol=r.rtol, return_info=True)
+    return res.E, res.iterations
+
+
+class ServingEngine:
+    """Continuous-batching engine (see the module docstring)."""
+
+    def __init__(self, device=None, hbm_budget: int = 64 << 30, max_batch: int = 8,
+                 use_graphs: bool = True, keep_results: bool = False):
+        self.device = torch.device(device or "cuda")
+        self.hbm_budget = int(hbm_budget)
+        self.max_batch = int(max_batch)
+        self.use_graphs = use_graphs
+        self.keep_results = keep_results
+        self.buckets: dict[tuple, list[_Bucket]] = {}
+        self.side_stream = torch.cuda.Stream(device=self.device)
+        self.t0 = 0.0
+
+    def _now(self):
+        return time.perf_counter() - self.t0
+
+    @staticmethod
+    def _bytes(r) -> int:
+        if isinstance(r, PropRequest):
+            N, Z = r.u.shape[-1], len(r.z)
+            return N * N * (8 + 4) + Z * N * N * 4
+        if isinstance(r, SasCztRequest):
+            N, W = r.u.shape[0], r.u.shape[-1]
+            return 8 * N * N * W * 12
+        n = int(np.prod(r.permittivity.shape))
+        return n * 8 * 12
+
+    def _batch_size(self, N, Z, queued):
+        cap = max(1, min(self.max_batch, (2 << 30) // max(1, Z * N * N * 4)))
+        b = 1
+        while b * 2 <= min(cap, queued):
+            b *= 2
+        return b
+
+    def _instance(self, r: PropRequest, B):
+        N, Z = r.u.shape[-1], len(r.z)
+        key = (N, Z, B, r.dx, r.wavelength, r.n)
+        insts = self.buckets.setdefault(key, [])
+        for inst in insts:
+            if not inst.busy:
+                return inst
+        if len(insts) < 2:
+            inst = _Bucket(N, Z, B, r.dx, r.wavelength, r.n, self.device, self.use_graphs)
+            insts.append(inst)
+            return inst
+        return None
+
+    def warm(self, requests):
+        """Capture the graphs the request mix will need (outside the timed run)."""
+        seen = set()
+        for r in requests:
+            if isinstance(r, PropRequest):
+                bmax = self._batch_size(r.u.shape[-1], len(r.z), self.max_batch)
+                for B in (1, 2, 4, 8):
+                    if B > bmax:
+                        break
+                    key = (r.u.shape[-1], len(r.z), B, r.dx, r.wavelength, r.n)
+                    if key not in seen:
+                        seen.add(key)
+                        self._instance(r, B)
+        torch.cuda.synchronize(self.device)
+
+    def run(self, requests):
+        """Serve ``requests`` (sorted by .arrival, seconds from start);
+        returns them with .done set (and .result with keep_results)."""
+        reqs = sorted(requests, key=lambda q: q.arrival)
+        pending = list(reqs)
+        queue: dict[tuple, list[PropRequest]] = {}
+        side: list = []
+        born_threads = []
+        in_use = 0
+        active = []
+        finished = []
+        lock = threading.Lock()
+        self.t0 = time.perf_counter()
+
+        def born_worker(r):
+            with torch.cuda.device(self.device):
+                s = torch.cuda.Stream(device=self.device)
+                s.wait_stream(torch.cuda.default_stream(self.device))
+                with torch.cuda.stream(s):
+                    E, it = _born(r)
+                    s.synchronize()
+            with lock:
+                r.result = E if self.keep_results else None
+                r.iterations = it
+                r.done = self._now()
+
+        while pending or queue or active or side or any(t.is_alive() for t in born_threads):
+            now = self._now()
+            # admit arrivals within the HBM budget
+            while pending and pending[0].arrival <= now:
+                r = pending[0]
+                need = self._bytes(r)
+                if in_use + need > self.hbm_budget and (active or side):
+                    break
+                pending.pop(0)
+                in_use += need
+                if isinstance(r, PropRequest):
+                    queue.setdefault((r.u.shape[-1], len(r.z), r.dx, r.wavelength, r.n), []).append(r)
+                elif isinstance(r, SasCztRequest):
+                    self.side_stream.wait_stream(torch.cuda.current_stream(self.device))
+                    with torch.cuda.stream(self.side_stream):
+                        r.result = _sas_czt(r, self.side_stream)
+                        ev = torch.cuda.Event()
+                        ev.record(self.side_stream)
+                    side.append((r, ev))
+                else:
+                    t = threading.Thread(target=born_worker, args=(r,), daemon=True)
+                    t.start()
+                    born_threads.append(t)
+                    active.append(("born", r, t))
+            # schedule batches
+            for key in list(queue):
+                q = queue[key]
+                while q:
+                    B = self._batch_size(key[0], key[1], len(q))
+                    inst = self._instance(q[0], B)
+                    if inst is None:
+                        break
+                    batch, queue[key] = q[:B], q[B:]
+                    q = queue[key]
+                    inst.start(batch)
+                    active.append(("prop", inst, None))
+                if not queue[key]:
+                    del queue[key]
+            # completions
+            still = []
+            for kind, obj, t in active:
+                if kind == "prop":
+                    done = obj.poll(self.keep_results, self._now)
+                    if done:
+                        for r in done:
+                            in_use -= self._bytes(r)
+                        finished += done
+                    else:
+                        still.append((kind, obj, t))
+                else:
+                    if t.is_alive():
+                        still.append((kind, obj, t))
+                    else:
+                        in_use -= self._bytes(obj)
+                        finished.append(obj)
+            active = still
+            keep = []
+            for r, ev in side:
+                if ev.query():
+                    r.done = self._now()
+                    if not self.keep_results:
+                        r.result = None
+                    in_use -= self._bytes(r)
+                    finished.append(r)
+                else:
+                    keep.append((r, ev))
+            side = keep
+            if not active and not side and not queue and pending:
+                wait = pending[0].arrival - self._now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.002))
+        torch.cuda.synchronize(self.device)
+        return reqs
+
+
+def run_plain(requests, keep_results: bool = False):
+    """Each request alone, in arrival order, one call per operation,
+    synchronised before the next starts."""
+    from .functional.propagation import asm_propagate
+    from .functional.phase_masks import phase_change
+
+    reqs = sorted(requests, key=lambda q: q.arrival)
+    t0 = time.perf_counter()
+    for r in reqs:
+        wait = r.arrival - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        if isinstance(r, PropRequest):
+            f = ScalarField(r.u, r.dx, 0.0, r.wavelength)
+            f = phase_change(f, r.phase)
+            out = asm_propagate(f, np.asarray(r.z)[:, None, None] if len(r.z) > 1 else float(r.z[0]),
+                                r.n, 0, mode="same")
+            res = out.intensity
+        elif isinstance(r, SasCztRequest):
+            res = _sas_czt(r, torch.cuda.current_stream())
+        else:
+            res, r.iterations = _born(r)
+        torch.cuda.synchronize()
+        r.done = time.perf_counter() - t0
+        r.result = res if keep_results else None
+    return reqs
+
+
+def latency_stats(reqs) -> dict:
+    lat = np.array([r.done - r.arrival for r in reqs if r.done is not None])
+    span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
+    return {"n": int(lat.size), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "throughput_rps": float(lat.size / span),
+            "span_s": float(span)}

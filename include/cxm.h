@@ -102,6 +102,23 @@ int cxm_prop2d_c64(void* stream, const void* in, void* out, void* workspace,
                    size_t workspace_bytes, const cxm_prop2d_desc* d);
 
 /*
+ * cxm_prop2d_c64 with a device table of per-output-plane transfer-function
+ * parameters instead of the descriptor's z list: output plane (p, iz) of
+ * input plane p uses params[p * num_z + iz].  The table is built on the host
+ * by cxm_prop2d_build_params (cxm_prop2d_param_bytes() bytes per plane,
+ * any distance per plane) and copied to the device by the caller, so a
+ * batch of requests with different distances runs as one launch sequence
+ * and a captured graph replays with new distances by refreshing the table.
+ * z and z_zip of the descriptor are ignored / must be 0.  No counterpart in
+ * the reference (serving support).
+ */
+size_t cxm_prop2d_param_bytes(void);
+int cxm_prop2d_build_params(const cxm_prop2d_desc* d, const double* z_planes, int64_t n_planes,
+                            void* host_out);
+int cxm_prop2d_c64_params(void* stream, const void* in, void* out, void* workspace,
+                          size_t workspace_bytes, const cxm_prop2d_desc* d, const void* params);
+
+/*
  * The propagator itself, materialised: out is num_wv x num_z x Ny x Nx
  * complex64 in FFT (ifftshifted) order, Ny = height + 2 pad_y (pad/crop and
  * batch fields of the descriptor are otherwise ignored).  Same arithmetic as

@@ -151,10 +151,9 @@ def test_c3_shape_against_oracle():
 
 def test_uniform_slab_closed_form_256_slices():
     """Plane wave, no padding, uniform dn, 256 slices of 1024^2: the output is
-    exp(i 2 pi / lam (D t n + D t dn - (D t / 2) n)) everywhere.  Chained in
-    float32, the error against this grows with depth; the bar is the error
+    exp(i 2 pi / lam (D t n + D t dn - (D t / 2) n)) everywhere.  The error
     of the same loop run in complex64 with scipy.fft (a float32 restatement
-    of the reference algorithm), not an absolute 1e-5."""
+    of the reference algorithm) is printed beside it for scale."""
     import scipy.fft as sfft
 
     from oracle.propagation import asm_propagator
@@ -179,7 +178,7 @@ def test_uniform_slab_closed_form_256_slices():
     err32 = rel_l2(v, ref)
     print(f"uniform slab D=256: GPU {err:.2e}, complex64 scipy {err32:.2e}, "
           f"|out| mean {np.abs(out).mean():.8f}")
-    assert err < 2 * err32 + 1e-6
+    assert err < TOL
 
 
 def os_workers():
