@@ -44,6 +44,10 @@ def parse():
     p.add_argument("--phase-mode", default="ref32", choices=["ref32", "accurate"])
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--config", default="m0", choices=["m0", "c1", "c2", "c3", "c4", "c5"],
+                   help="m0: the headline batched 1024^2 AS propagation (default); c1-c5: the "
+                        "other workloads (plain AS with padding, 4f PSF + gradient, multislice + "
+                        "adjoint, Born solves, serving), one JSON line each")
     p.add_argument("--summarize", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
                    help="write the rocprofv3 summaries of a profiled run (kernel-trace --stats, "
                         "--pmc FETCH_SIZE, --pmc WRITE_SIZE; csv output) and exit")
@@ -168,6 +172,277 @@ def summarize(trace_dir: str, fetch_dir: str, write_dir: str) -> None:
         {"bytes_per_launch": traffic, "scale_fetch": f_fetch, "scale_write": f_write}, indent=1) + "\n")
 
 
+
+# ---------------------------------------------------------------------------
+# Other workloads (--config c1..c5).  Each prints one JSON line in the same
+# format; value and unit are the workload's own throughput.
+# ---------------------------------------------------------------------------
+def _timed(torch, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        fn()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
+def _kernels(torch, L, fn, steps):
+    L.timing_enable(True)
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    kt = L.timing_collect()
+    L.timing_enable(False)
+    return {k: {"launches": c, "avg_us": ms / c * 1e3} for k, (ms, c) in kt.items()}
+
+
+def _result(args, metric, value, unit, ms_step, steps, warmup, dtype, workload, path_bytes,
+            kernels, cpu=None, extra=None):
+    achieved = path_bytes / (ms_step * 1e-3) / 1e9
+    r = {
+        "metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+        "config": {"workload": workload},
+        "path_roofline": {"bytes_per_step": path_bytes, "achieved": achieved,
+                          "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                          "kernels": kernels},
+        "cpu_baseline": cpu,
+    }
+    if extra:
+        r.update(extra)
+    return r
+
+
+def bench_c1(args):
+    """512^2 exp(i phi) fields, pad 256 per side (1024^2 FFT grid), intensity
+    cropped to 512^2; a batch of 64 fields per step."""
+    import torch
+
+    import chromatix_amd as cx
+    import chromatix_amd.functional as cf
+    from chromatix_amd import _lib as L
+
+    B, n, pad = 64, 512, 256
+    rng = np.random.default_rng(0)
+    u = torch.polar(torch.ones(B, n, n, device="cuda"),
+                    torch.as_tensor(rng.uniform(0, 2 * np.pi, (B, n, n)).astype(np.float32)).cuda())
+    f = cx.ScalarField(u, 0.3, 0.0, 0.532)
+    step = lambda: cf.propagate_intensity(f, 100.0, 1.33, pad, method="asm", mode="same")
+    steps, warm = max(10, args.steps // 4), args.warmup
+    ms = _timed(torch, step, steps, warm)
+    kern = _kernels(torch, L, step, 5)
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import propagation as op
+
+        cores = len(os.sched_getaffinity(0))
+        x = np.exp(1j * rng.uniform(0, 2 * np.pi, (n, n))).astype(np.complex64)
+        t0, done = time.perf_counter(), 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 8.0):
+            np.abs(op.propagate(x, (0.3, 0.3), 0.532, 100.0, 1.33, pad, precision="ref32",
+                                workers=cores)) ** 2
+            done += 1
+        el = time.perf_counter() - t0
+        cpu = {"value": done / el, "unit": "propagations/s", "cores": cores, "kind": "port",
+               "sample": f"{done} single-field propagations (oracle, ref32 propagator, complex64 "
+                         f"scipy.fft workers={cores}), {el:.1f} s"}
+    return _result(args, "512^2 AS propagations/s (pad 256, intensity)", B / (ms * 1e-3),
+                   "propagations/s", ms, steps, warm, "complex64",
+                   f"propagate_intensity batch {B} x 512^2, pad 256 -> 1024^2, z 100 um, same",
+                   B * 32 * 1024 * 1024, kern, cpu)
+
+
+def bench_c2(args):
+    """4f PSF stack (1280^2, 11 planes, z in linspace(-5, 5) um) forward and
+    the phase-mask gradient of the MSE against a target stack; one step =
+    forward + gradient."""
+    import torch
+
+    import chromatix_amd.functional as cf
+    from chromatix_amd import _lib as L
+
+    N, Z = 1280, 11
+    lam, nimm, f, NA, ft, pitch = 0.532, 1.33, 8000.0, 0.6, 200000.0, 6.5
+    dx = ft * lam / (nimm * N * pitch)
+    z = np.linspace(-5, 5, Z)
+    rng1, rng2 = np.random.default_rng(1), np.random.default_rng(2)
+    phase = torch.as_tensor(rng1.uniform(-np.pi, np.pi, (N, N)).astype(np.float32)).cuda()
+    I0 = cf.psf_4f(phase, dx, lam, z, f, nimm, NA, ft)
+    target = (I0 * torch.as_tensor(rng2.uniform(0.5, 1.5, (Z, N, N)).astype(np.float32)).cuda())
+
+    def step():
+        cf.psf_4f(phase, dx, lam, z, f, nimm, NA, ft)
+        cf.psf_4f_mse_grad(phase, target, dx, lam, z, f, nimm, NA, ft)
+
+    steps, warm = max(10, args.steps // 4), args.warmup
+    ms = _timed(torch, step, steps, warm)
+    kern = _kernels(torch, L, step, 5)
+    ms_f = _timed(torch, lambda: cf.psf_4f(phase, dx, lam, z, f, nimm, NA, ft), steps, 2)
+    b_f, b_b = Z * 20 * N * N + 4 * N * N, Z * 36 * N * N + 8 * N * N
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import psf as opsf
+
+        t0 = time.perf_counter()
+        ph = phase.cpu().numpy().astype(np.float64)
+        tg = target.cpu().numpy().astype(np.float64)
+        opsf.psf4f_mse_grad(ph, tg, (dx, dx), lam, z, f, nimm, NA, ft)
+        el = time.perf_counter() - t0
+        cpu = {"value": 1.0 / el, "unit": "steps/s", "cores": 1, "kind": "port",
+               "sample": f"one forward + MSE gradient (FP64 NumPy oracle), {el:.1f} s"}
+    return _result(args, "4f PSF stack + phase gradient steps/s (1280^2, 11 planes)",
+                   1e3 / ms, "steps/s", ms, steps, warm, "complex64",
+                   "Optical4FSystemPSF forward (intensity) + d MSE / d phase, 1280^2, Z = 11",
+                   b_f + b_b, kern, cpu, {"forward_ms": ms_f})
+
+
+def bench_c3(args):
+    """Multislice 1024^2, 256 slices of 0.5 um, forward and the Delta n
+    gradient of an intensity MSE with checkpoints every 16 slices."""
+    import torch
+
+    import chromatix_amd as cx
+    import chromatix_amd.functional as cf
+    from chromatix_amd import _lib as L
+    from chromatix_amd.functional import samples as S
+
+    N, D, k, lam, nm, t = 1024, 256, 16, 0.532, 1.33, 0.5
+    rng = np.random.default_rng(3)
+    g = torch.as_tensor(rng.standard_normal((D, N, N)).astype(np.float32)).cuda()
+    # Gaussian smoothing (sigma 2 px) along y and x by FFT, then std 0.01
+    fy = torch.fft.fftfreq(N, device="cuda")
+    ker = torch.exp(-2 * (np.pi * 2.0) ** 2 * (fy[:, None] ** 2 + fy[None, :] ** 2))
+    dn = torch.fft.ifft2(torch.fft.fft2(g) * ker).real
+    dn = (dn / dn.std() * 0.01).contiguous()
+    ab = torch.zeros_like(dn)
+    f = cx.ScalarField(torch.ones((N, N), dtype=torch.complex64, device="cuda"), 0.3, 0.0, lam)
+    out = cf.multislice_thick_sample(f, ab, dn, nm, t, 0)
+    target = (out.intensity * 1.05).contiguous()
+    d = S._desc(f, D, nm, t, 0, None, (0.0, 0.0), None, False, False, False, None, k)
+    ck_holder = {}
+
+    def fwd():
+        _, ck_holder["ck"] = S._fwd(f.u, dn, ab, d, want_ckpt=True)
+
+    def bwd():
+        cf.multislice_mse_grad(f, ab, dn, target, nm, t, 0, checkpoint_every=k,
+                               checkpoints=ck_holder["ck"])
+
+    steps, warm = 3, 1
+    ms_f = _timed(torch, fwd, steps, warm)
+    ms_b = _timed(torch, bwd, steps, warm)
+    kern = _kernels(torch, L, lambda: (fwd(), bwd()), 1)
+    b_f = D * 36 * N * N
+    b_b = D * 100 * N * N + (D // k) * 8 * N * N
+    return _result(args, "multislice forward + adjoint steps/s (1024^2, 256 slices, k = 16)",
+                   1e3 / (ms_f + ms_b), "steps/s", ms_f + ms_b, steps, warm, "complex64",
+                   "multislice_thick_sample 1024^2 x 256 slices (t 0.5 um, dn smoothed noise std "
+                   "0.01) forward + d MSE(I) / d dn, checkpoints every 16",
+                   b_f + b_b, kern, None, {"forward_ms": ms_f, "adjoint_ms": ms_b,
+                                           "checkpoint_every": k})
+
+
+def bench_c4(args):
+    """Modified Born series: 8 volumes of 128^3 (+16-voxel absorbing
+    boundary), 20 spheres each, solved to a relative residual of 1e-4."""
+    import torch
+
+    from chromatix_amd.experimental import modified_born_series as mbs
+
+    lam = 0.532
+    k0, sp = 2 * np.pi / lam, lam / 4
+    n, bc, vols = 128, 16, 8
+    times, iters = [], []
+    for v in range(vols):
+        rng = np.random.default_rng(4 + v)
+        eps = np.full((n, n, n), 1.33**2, np.complex64)
+        idx = np.indices((n, n, n), dtype=np.float32)
+        for _ in range(20):
+            c = rng.uniform(0.2 * n, 0.8 * n, 3)
+            r = rng.uniform(4, 12)
+            inside = np.sum((idx - c[:, None, None, None].astype(np.float32)) ** 2, axis=0) <= r * r
+            eps[inside] = rng.uniform(1.36, 1.40) ** 2
+        sample = mbs.add_absorbing_bc(mbs.Sample(torch.as_tensor(eps).cuda(), sp), (0, 1, 2), bc * sp,
+                                      n_background=1.33)
+        J = torch.zeros(sample.shape + (3,), dtype=torch.complex64, device="cuda")
+        J[:, :, bc + 1, 0] = 1.0
+        src = mbs.Source(J, k0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = mbs.solve(sample, src, maxiter=3000, rtol=1e-4, return_info=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        iters.append(res.iterations)
+    N3 = (n + 2 * bc) ** 3
+    per_it = sum(times) / sum(iters)
+    return _result(args, "modified Born series solves/s (128^3 + boundary, rtol 1e-4)",
+                   vols / sum(times), "solves/s", sum(times) / vols * 1e3, vols, 0, "complex64",
+                   "8 volumes 160^3 (128^3 + 16-voxel boundary), 20 spheres, plane-wave current "
+                   "sheet, to relative residual 1e-4",
+                   272 * N3 * (sum(iters) / vols), {}, None,
+                   {"iterations": iters, "ms_per_iteration": per_it * 1e3})
+
+
+def bench_c5(args):
+    """Serving: Poisson arrivals of a 70/20/10 mix (propagation / SAS+CZT /
+    Born), engine vs the plain per-request path."""
+    import torch
+
+    from chromatix_amd import serving as sv
+
+    def mix(seed, count, rate):
+        rng = np.random.default_rng(seed)
+        t, reqs = 0.0, []
+        for i in range(count):
+            t += rng.exponential(1.0 / rate)
+            kind = rng.choice(3, p=[0.7, 0.2, 0.1])
+            if kind == 0:
+                N = int(rng.choice([256, 512, 1024, 2048]))
+                Z = int(rng.choice([1, 4, 16, 32]))
+                u = torch.polar(torch.ones(N, N, device="cuda"),
+                                torch.rand(N, N, device="cuda") * 6.2832)
+                ph = (torch.rand(N, N, device="cuda") - 0.5) * 6.2832
+                reqs.append(sv.PropRequest(u, ph, rng.uniform(-20, 20, Z), 0.3, 0.532, 1.33, arrival=t))
+            elif kind == 1:
+                wl = np.linspace(0.45, 0.65, 8)
+                uc = torch.polar(torch.ones(2048, 2048, 8, device="cuda"),
+                                 torch.rand(2048, 2048, 8, device="cuda") * 6.2832)
+                reqs.append(sv.SasCztRequest(uc, 0.5, wl, float(rng.uniform(-20, 20)), 1.33, 512, 2.0,
+                                             arrival=t))
+            else:
+                n = int(rng.choice([64, 96, 128]))
+                bc = 8 if n == 64 else 16
+                eps = torch.full((n + 2 * bc,) * 3, 1.33**2, dtype=torch.complex64, device="cuda")
+                c = n // 2 + bc
+                eps[c - 6:c + 6, c - 6:c + 6, c - 6:c + 6] = 1.38**2
+                J = torch.zeros(eps.shape + (3,), dtype=torch.complex64, device="cuda")
+                J[:, :, bc + 1, 0] = 1
+                reqs.append(sv.BornRequest(eps, J, 0.133, 2 * np.pi / 0.532, 1e-4, arrival=t))
+        torch.cuda.synchronize()
+        return reqs
+
+    count, rate = 60, 40.0
+    eng = sv.ServingEngine(max_batch=8, use_graphs=True)
+    eng.warm(mix(5, count, rate))
+    served = eng.run(mix(5, count, rate))
+    st = sv.latency_stats(served)
+    plain = sv.run_plain(mix(5, count, rate))
+    sp = sv.latency_stats(plain)
+    return _result(args, "served requests/s at p99 (70% PSF/AS, 20% SAS+CZT, 10% Born)",
+                   st["throughput_rps"], "requests/s", st["span_s"] * 1e3 / st["n"], st["n"], 0,
+                   "complex64", f"{count} Poisson arrivals at {rate}/s, seed 5", 0, {}, None,
+                   {"serving": st, "plain": sp,
+                    "throughput_ratio": st["throughput_rps"] / sp["throughput_rps"]})
+
+
+CONFIGS = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}
+
+
 def main():
     args = parse()
     if args.summarize:
@@ -183,6 +458,10 @@ def main():
     import chromatix_amd as cx
     import chromatix_amd.functional as cf
     from chromatix_amd import _lib as L
+
+    if args.config != "m0":
+        print(json.dumps(CONFIGS[args.config](args)), flush=True)
+        return
 
     B, N = args.batch, args.n
     log(f"[bench] building input {B}x{N}x{N} complex64")

@@ -17,6 +17,7 @@ CXM_PIPE_GROUP_DECL(0)
 CXM_PIPE_GROUP_DECL(1)
 CXM_PIPE_GROUP_DECL(2)
 CXM_PIPE_GROUP_DECL(3)
+CXM_PIPE_GROUP_DECL(4)
 #undef CXM_PIPE_GROUP_DECL
 
 // Row pass, column pass and phase-gradient row pass of length N (over the
@@ -27,6 +28,7 @@ inline int launch_pipe_row(int N, const PipeArgs& a, hipStream_t s) {
   if ((rc = launch_pipe_row_g1(N, a, s)) >= 0) return rc;
   if ((rc = launch_pipe_row_g2(N, a, s)) >= 0) return rc;
   if ((rc = launch_pipe_row_g3(N, a, s)) >= 0) return rc;
+  if ((rc = launch_pipe_row_g4(N, a, s)) >= 0) return rc;
   return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
 }
 
@@ -36,6 +38,7 @@ inline int launch_pipe_col(int N, const PipeArgs& a, hipStream_t s) {
   if ((rc = launch_pipe_col_g1(N, a, s)) >= 0) return rc;
   if ((rc = launch_pipe_col_g2(N, a, s)) >= 0) return rc;
   if ((rc = launch_pipe_col_g3(N, a, s)) >= 0) return rc;
+  if ((rc = launch_pipe_col_g4(N, a, s)) >= 0) return rc;
   return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
 }
 
@@ -45,6 +48,7 @@ inline int launch_pipe_phasegrad(int N, const PipeArgs& a, hipStream_t s) {
   if ((rc = launch_pipe_phasegrad_g1(N, a, s)) >= 0) return rc;
   if ((rc = launch_pipe_phasegrad_g2(N, a, s)) >= 0) return rc;
   if ((rc = launch_pipe_phasegrad_g3(N, a, s)) >= 0) return rc;
+  if ((rc = launch_pipe_phasegrad_g4(N, a, s)) >= 0) return rc;
   return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
 }
 

@@ -62,6 +62,26 @@ CXM_DEV void rotate_half(float2 (&v)[E]) {
   for (int k = 0; k < E; ++k) v[k] = w[k];
 }
 
+// Operator stages.  The operator code (a switch over every operator kind)
+// is too large to be unrolled E times, and a rolled loop over the register
+// array v[] would put v in scratch memory.  So when a stage has operators,
+// each thread parks its E values in its own LDS slots (the slots the FFT's
+// first exchange reads), applies the operators in a rolled loop over those
+// slots, and takes the values back: one copy of the operator code, v stays
+// in registers.  `st` is the transform's LDS base, slot(e) = lds_at<N, S>(e).
+template <int N, int E, int S>
+CXM_DEV void stage_out(const float2 (&v)[E], float2* st, int t) {
+  constexpr int T = N / E;
+#pragma unroll
+  for (int k = 0; k < E; ++k) st[lds_at<N, S>(t + k * T)] = v[k];
+}
+template <int N, int E, int S>
+CXM_DEV void stage_in(float2 (&v)[E], const float2* st, int t) {
+  constexpr int T = N / E;
+#pragma unroll
+  for (int k = 0; k < E; ++k) v[k] = st[lds_at<N, S>(t + k * T)];
+}
+
 template <int N, int E, int S, bool SYNC>
 CXM_DEV void run_seq(const PipeArgs& a, float2 (&v)[E], float2* lds, int t, int plane, int line_grid,
                      bool is_row) {
@@ -69,22 +89,30 @@ CXM_DEV void run_seq(const PipeArgs& a, float2 (&v)[E], float2* lds, int t, int 
   if (a.seq == SEQ_F || a.seq == SEQ_FI) fft<N, E, -1, S, SYNC>(v, lds, t, a.tw);
   else fft<N, E, 1, S, SYNC>(v, lds, t, a.tw);
   if (a.seq == SEQ_FI || a.seq == SEQ_IF) {
+    if (a.mid.n || (a.mid_out && is_row)) {
+      __syncthreads();  // the FFT's last exchange has been read by every thread
+      stage_out<N, E, S>(v, lds, t);
+#pragma unroll 1
+      for (int k = 0; k < E; ++k) {
+        const int j = t + k * T;
+        float2* slot = lds + lds_at<N, S>(j);
+        float2 x = cscale(*slot, a.mid_scale);
+        if (a.mid.n)
+          x = is_row ? apply_ops(a.mid, a.tab, x, plane, line_grid, j, a.grid_ny, a.grid_nx)
+                     : apply_ops(a.mid, a.tab, x, plane, j, line_grid, a.grid_ny, a.grid_nx);
+        if (a.mid_out && is_row) {
+          const int yl = line_grid - a.mid_y0, xl = j - a.mid_x0;
+          if (yl >= 0 && yl < a.mid_h && xl >= 0 && xl < a.mid_w)
+            a.mid_out[(long long)(plane - a.plane0) * a.mid_plane_stride +
+                      (long long)yl * a.mid_row_stride + xl] = x;
+        }
+        *slot = x;
+      }
+      stage_in<N, E, S>(v, lds, t);
+      __syncthreads();  // before the next FFT scatters into other threads' slots
+    } else {
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const int j = t + k * T;
-      float2 x = cscale(v[k], a.mid_scale);
-      if (a.mid.n) {
-        x = is_row ? apply_ops(a.mid, a.tab, x, plane, line_grid, j, a.grid_ny, a.grid_nx)
-                   : apply_ops(a.mid, a.tab, x, plane, j, line_grid, a.grid_ny, a.grid_nx);
-        __builtin_amdgcn_sched_barrier(0);
-      }
-      if (a.mid_out && is_row) {
-        const int yl = line_grid - a.mid_y0, xl = j - a.mid_x0;
-        if (yl >= 0 && yl < a.mid_h && xl >= 0 && xl < a.mid_w)
-          a.mid_out[(long long)(plane - a.plane0) * a.mid_plane_stride +
-                    (long long)yl * a.mid_row_stride + xl] = x;
-      }
-      v[k] = x;
+      for (int k = 0; k < E; ++k) v[k] = cscale(v[k], a.mid_scale);
     }
     if constexpr (E % 2 == 0) {
       if (a.mid_shift) rotate_half<E>(v);
@@ -118,31 +146,58 @@ pipe_row_kernel(PipeArgs a) {
   const int plane = a.plane0 + p;
   const int yg = a.line_off + r;
   const float2* src = a.in ? a.in + p * a.in_plane_stride + (long long)r * a.in_line_stride : nullptr;
+  float2* st = lds + g * SPAN;
   float2 v[E];
-#pragma unroll
-  for (int k = 0; k < E; ++k) {
-    const int i = t + k * T;
-    const int xf = a.in_shift ? ishift_src(i, N) : i;
-    const int xs = xf - a.in_off;
-    const bool in = ok && xs >= 0 && xs < a.in_len;
-    float2 x = in ? (src ? src[xs] : make_float2(1.f, 0.f)) : make_float2(0.f, 0.f);
-    if (a.pre.n && in) {
-      x = apply_ops(a.pre, a.tab, x, plane, yg, xf, a.grid_ny, a.grid_nx);
-      __builtin_amdgcn_sched_barrier(0);
+  if (a.pre.n) {
+#pragma unroll 1
+    for (int k = 0; k < E; ++k) {
+      const int i = t + k * T;
+      const int xf = a.in_shift ? ishift_src(i, N) : i;
+      const int xs = xf - a.in_off;
+      const bool in = ok && xs >= 0 && xs < a.in_len;
+      float2 x = in ? (src ? src[xs] : make_float2(1.f, 0.f)) : make_float2(0.f, 0.f);
+      if (in) x = apply_ops(a.pre, a.tab, x, plane, yg, xf, a.grid_ny, a.grid_nx);
+      st[lds_at<N, 1>(i)] = x;
     }
-    v[k] = x;
+    stage_in<N, E, 1>(v, st, t);
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = t + k * T;
+      const int xs = (a.in_shift ? ishift_src(i, N) : i) - a.in_off;
+      const bool in = ok && xs >= 0 && xs < a.in_len;
+      v[k] = in ? (src ? src[xs] : make_float2(1.f, 0.f)) : make_float2(0.f, 0.f);
+    }
   }
-  run_seq<N, E, 1, true>(a, v, lds + g * SPAN, t, plane, yg, true);
-  if (!ok) return;
+  run_seq<N, E, 1, true>(a, v, st, t, plane, yg, true);
   const long long obase = p * a.out_plane_stride + (long long)r * a.out_line_stride;
+  if (a.post.n) {
+    __syncthreads();
+    stage_out<N, E, 1>(v, st, t);
+    if (!ok) return;
+#pragma unroll 1
+    for (int k = 0; k < E; ++k) {
+      const int j = t + k * T;
+      const int xo = a.out_shift ? fshift_dst(j, N) : j;
+      const int xl = xo - a.out_off;
+      if (xl >= 0 && xl < a.out_len) {
+        float2 x = cscale(st[lds_at<N, 1>(j)], a.scale);
+        x = apply_ops(a.post, a.tab, x, plane, yg, xo, a.grid_ny, a.grid_nx);
+        if (a.intensity) ((float*)a.out)[obase + xl] = cabs2(x);
+        else ((float2*)a.out)[obase + xl] = x;
+      }
+    }
+    return;
+  }
+  if (!ok) return;
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int j = t + k * T;
     const int xo = a.out_shift ? fshift_dst(j, N) : j;
     const int xl = xo - a.out_off;
     if (xl >= 0 && xl < a.out_len) {
-      float2 x = cscale(v[k], a.scale);
-      if (a.post.n) x = apply_ops(a.post, a.tab, x, plane, yg, xo, a.grid_ny, a.grid_nx);
+      const float2 x = cscale(v[k], a.scale);
       if (a.intensity) ((float*)a.out)[obase + xl] = cabs2(x);
       else ((float2*)a.out)[obase + xl] = x;
     }
@@ -165,33 +220,37 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
   constexpr int T = Cfg::T;
   constexpr int SPAN = lds_span<N, 1>();
   __shared__ float2 lds[Cfg::G * SPAN];
+  __shared__ float acc_lds[Cfg::G * N];  // the plane sum, one float per element
   const int g = threadIdx.x / T;
   const int t = threadIdx.x - g * T;
   const int r = blockIdx.x * Cfg::G + g;
   const bool ok = r < a.n_lines;
   const int yg = a.line_off + (ok ? r : 0);
-  float acc[E];
+  float2* st = lds + g * SPAN;
+  float* acc = acc_lds + g * N;
 #pragma unroll
-  for (int k = 0; k < E; ++k) acc[k] = 0.0f;
+  for (int k = 0; k < E; ++k) acc[t + k * T] = 0.0f;
   for (int p = 0; p < a.n_planes; ++p) {
     const float2* src = a.in + p * a.in_plane_stride + (long long)(ok ? r : 0) * a.in_line_stride;
     float2 v[E];
 #pragma unroll
     for (int k = 0; k < E; ++k) v[k] = ok ? src[t + k * T] : make_float2(0.f, 0.f);
-    fft<N, E, 1, 1, true>(v, lds + g * SPAN, t, a.tw);
-#pragma unroll
+    fft<N, E, 1, 1, true>(v, st, t, a.tw);
+    __syncthreads();
+    stage_out<N, E, 1>(v, st, t);
+#pragma unroll 1
     for (int k = 0; k < E; ++k) {
       const int j = t + k * T;
       const int xf = a.in_shift ? ishift_src(j, N) : j;
       const int xs = xf - a.in_off;
       if (ok && xs >= 0 && xs < a.in_len) {
-        const float2 gv = cscale(v[k], a.scale);
+        const float2 gv = cscale(st[lds_at<N, 1>(j)], a.scale);
         const float2 sv = apply_ops(a.pre, a.tab, make_float2(1.f, 0.f), a.plane0 + p, yg, xf,
                                     a.grid_ny, a.grid_nx);
-        acc[k] += gv.y * sv.x - gv.x * sv.y;
-        __builtin_amdgcn_sched_barrier(0);
+        acc[j] += gv.y * sv.x - gv.x * sv.y;
       }
     }
+    __syncthreads();  // before the next plane's FFT reuses the slots
   }
   if (!ok) return;
   float* dst = (float*)a.out + (long long)r * a.out_line_stride;
@@ -200,7 +259,7 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
     const int j = t + k * T;
     const int xf = a.in_shift ? ishift_src(j, N) : j;
     const int xs = xf - a.in_off;
-    if (xs >= 0 && xs < a.in_len) dst[xs] = a.accum ? dst[xs] + acc[k] : acc[k];
+    if (xs >= 0 && xs < a.in_len) dst[xs] = a.accum ? dst[xs] + acc[j] : acc[j];
   }
 }
 
@@ -227,35 +286,63 @@ pipe_col_kernel(PipeArgs a) {
   const int plane = a.plane0 + p;
   const int xg = a.line_off + (colok ? col : 0);
   const float2* src = a.in ? a.in + p * a.in_plane_stride + col : nullptr;
+  float2* st = lds + c;
   float2 v[E];
-#pragma unroll
-  for (int k = 0; k < E; ++k) {
-    const int i = ty + k * T;
-    const int yf = a.in_shift ? ishift_src(i, N) : i;
-    const int ys = yf - a.in_off;
-    const bool in = colok && ys >= 0 && ys < a.in_len;
-    float2 x = in ? (src ? src[(long long)ys * a.in_line_stride] : make_float2(1.f, 0.f))
-                  : make_float2(0.f, 0.f);
-    if (a.pre.n && in) {
-      x = apply_ops(a.pre, a.tab, x, plane, yf, xg, a.grid_ny, a.grid_nx);
-      __builtin_amdgcn_sched_barrier(0);
+  if (a.pre.n) {
+#pragma unroll 1
+    for (int k = 0; k < E; ++k) {
+      const int i = ty + k * T;
+      const int yf = a.in_shift ? ishift_src(i, N) : i;
+      const int ys = yf - a.in_off;
+      const bool in = colok && ys >= 0 && ys < a.in_len;
+      float2 x = in ? (src ? src[(long long)ys * a.in_line_stride] : make_float2(1.f, 0.f))
+                    : make_float2(0.f, 0.f);
+      if (in) x = apply_ops(a.pre, a.tab, x, plane, yf, xg, a.grid_ny, a.grid_nx);
+      st[lds_at<N, C>(i)] = x;
     }
-    v[k] = x;
+    stage_in<N, E, C>(v, st, ty);
+    __syncthreads();
+  } else {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int i = ty + k * T;
+      const int ys = (a.in_shift ? ishift_src(i, N) : i) - a.in_off;
+      const bool in = colok && ys >= 0 && ys < a.in_len;
+      v[k] = in ? (src ? src[(long long)ys * a.in_line_stride] : make_float2(1.f, 0.f))
+                : make_float2(0.f, 0.f);
+    }
   }
-  run_seq<N, E, C, true>(a, v, lds + c, ty, plane, xg, false);
-  if (!colok) return;
+  run_seq<N, E, C, true>(a, v, st, ty, plane, xg, false);
   const int xo = a.out_line_shift ? fshift_dst(xg, a.grid_nx) : xg;
   const int xl = xo - a.out_line_off;
-  if (xl < 0 || xl >= a.out_line_len) return;
+  const bool store = colok && xl >= 0 && xl < a.out_line_len;
   const long long obase = p * a.out_plane_stride + xl;
+  if (a.post.n) {
+    __syncthreads();
+    stage_out<N, E, C>(v, st, ty);
+    if (!store) return;
+#pragma unroll 1
+    for (int k = 0; k < E; ++k) {
+      const int j = ty + k * T;
+      const int yo = a.out_shift ? fshift_dst(j, N) : j;
+      const int yl = yo - a.out_off;
+      if (yl >= 0 && yl < a.out_len) {
+        float2 x = cscale(st[lds_at<N, C>(j)], a.scale);
+        x = apply_ops(a.post, a.tab, x, plane, yo, xo, a.grid_ny, a.grid_nx);
+        if (a.intensity) ((float*)a.out)[obase + (long long)yl * a.out_line_stride] = cabs2(x);
+        else ((float2*)a.out)[obase + (long long)yl * a.out_line_stride] = x;
+      }
+    }
+    return;
+  }
+  if (!store) return;
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int j = ty + k * T;
     const int yo = a.out_shift ? fshift_dst(j, N) : j;
     const int yl = yo - a.out_off;
     if (yl >= 0 && yl < a.out_len) {
-      float2 x = cscale(v[k], a.scale);
-      if (a.post.n) x = apply_ops(a.post, a.tab, x, plane, yo, xo, a.grid_ny, a.grid_nx);
+      const float2 x = cscale(v[k], a.scale);
       if (a.intensity) ((float*)a.out)[obase + (long long)yl * a.out_line_stride] = cabs2(x);
       else ((float2*)a.out)[obase + (long long)yl * a.out_line_stride] = x;
     }

@@ -42,6 +42,7 @@ int launch_row(int N, int dir, const RowArgs& a, hipStream_t s) {
   if ((rc = launch_row_g1(N, dir, a, s)) >= 0) return rc;
   if ((rc = launch_row_g2(N, dir, a, s)) >= 0) return rc;
   if ((rc = launch_row_g3(N, dir, a, s)) >= 0) return rc;
+  if ((rc = launch_row_g4(N, dir, a, s)) >= 0) return rc;
   return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
 }
 
@@ -57,6 +58,7 @@ int launch_col_prop(int N, const ColArgs& a, hipStream_t s) {
   if ((rc = launch_col_g1(N, a, s)) >= 0) return rc;
   if ((rc = launch_col_g2(N, a, s)) >= 0) return rc;
   if ((rc = launch_col_g3(N, a, s)) >= 0) return rc;
+  if ((rc = launch_col_g4(N, a, s)) >= 0) return rc;
   return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
 }
 
@@ -106,7 +108,7 @@ int validate(const cxm_prop2d_desc* d, Geom& g) {
   if (g.Hout <= 0 || g.Wout <= 0) return fail(CXM_ERR_INVALID, "crop removes the whole plane");
   if (!fft_len_supported(g.Ny) || !fft_len_supported(g.Nx))
     return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT grid " + std::to_string(g.Ny) + "x" +
-                                         std::to_string(g.Nx) + " (lengths 2^a or 5*2^a, <= 8192)");
+                                         std::to_string(g.Nx) + " (lengths 2^a <= 8192, 5*2^a <= 5120, 3*2^a <= 3072)");
   g.plane_in = (long long)g.Hin * g.Win;
   g.plane_ws1 = (long long)g.Hin * g.Nx;
   g.plane_ws2 = (long long)g.Hout * g.Nx;

@@ -17,7 +17,13 @@
   X(320, 20, 20, 16) X(640, 20, 20, 16) X(1280, 20, 20, 8) X(2560, 20, 20, 4) \
   X(5120, 20, 20, 2)
 
-#define CXM_FOR_EACH_SIZE(X) CXM_SIZES_G0(X) CXM_SIZES_G1(X) CXM_SIZES_G2(X) CXM_SIZES_G3(X)
+#define CXM_SIZES_G4(X)                                                       \
+  X(3, 3, 3, 64) X(6, 6, 6, 64) X(12, 12, 12, 64) X(24, 12, 12, 64)          \
+  X(48, 12, 12, 32) X(96, 12, 12, 32) X(192, 12, 12, 16) X(384, 24, 24, 16)  \
+  X(768, 24, 24, 8) X(1536, 24, 24, 4) X(3072, 24, 24, 2)
+
+#define CXM_FOR_EACH_SIZE(X) \
+  CXM_SIZES_G0(X) CXM_SIZES_G1(X) CXM_SIZES_G2(X) CXM_SIZES_G3(X) CXM_SIZES_G4(X)
 
 namespace cxm {
 struct RowArgs;
@@ -27,8 +33,10 @@ int launch_row_g0(int N, int dir, const RowArgs& a, hipStream_t s);
 int launch_row_g1(int N, int dir, const RowArgs& a, hipStream_t s);
 int launch_row_g2(int N, int dir, const RowArgs& a, hipStream_t s);
 int launch_row_g3(int N, int dir, const RowArgs& a, hipStream_t s);
+int launch_row_g4(int N, int dir, const RowArgs& a, hipStream_t s);
 int launch_col_g0(int N, const ColArgs& a, hipStream_t s);
 int launch_col_g1(int N, const ColArgs& a, hipStream_t s);
 int launch_col_g2(int N, const ColArgs& a, hipStream_t s);
 int launch_col_g3(int N, const ColArgs& a, hipStream_t s);
+int launch_col_g4(int N, const ColArgs& a, hipStream_t s);
 }  // namespace cxm

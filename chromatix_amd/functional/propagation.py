@@ -195,11 +195,49 @@ def _fused_propagate(field: Field, kind: int, z, n, pad_width, cval, mode, kykx=
     crop_yx = pad_yx if mode == "same" else (0, 0)
     z = np.atleast_1d(np.asarray(z, dtype=np.float64)).reshape(-1)
     plan = _Plan(field, z)
-    d, keep = _desc_common(field, kind, z, n, grid_pad, crop_yx, phase_mode, out_kind, kykx,
-                           shift_yx, remove_evanescent, bandlimit, plan)
-    out = _launch(field, d, plan, out_kind)
+    args = (field, kind, z, n, grid_pad, crop_yx, phase_mode, kykx, shift_yx, remove_evanescent,
+            bandlimit)
+    if field.u.requires_grad and torch.is_grad_enabled():
+        if out_kind != L.CXM_OUT_COMPLEX:
+            raise NotImplementedError("gradients flow through the complex output; take "
+                                      "`.intensity` of the propagated field")
+        if plan.mode == "outer":
+            raise NotImplementedError("gradients of one field propagated to several distances "
+                                      "are not implemented (batch the field over z instead)")
+        return _PropagateFn.apply(field.u, args)
+    return _propagate_planes(field.u, args, out_kind)
+
+
+def _propagate_planes(u, args, out_kind, adjoint: bool = False):
+    field, kind, z, n, grid_pad, crop_yx, phase_mode, kykx, shift_yx, rem, bl = args
+    f = field.replace(u=u)
+    if adjoint:
+        # crop_c(P_z(pad_p(u)))^H = crop_p(P_{-z}(pad_c(.))): conj(H(z)) = H(-z) for the
+        # transfer and angular-spectrum propagators as the reference defines them
+        # (propagation.py:493-496, 556), the band limit depending on z^2 only
+        z, grid_pad, crop_yx = -z, crop_yx, grid_pad
+    plan = _Plan(f, z)
+    d, keep = _desc_common(f, kind, z, n, grid_pad, crop_yx, phase_mode, out_kind, kykx,
+                           shift_yx, rem, bl, plan)
+    out = _launch(f, d, plan, out_kind)
     del keep
     return out
+
+
+class _PropagateFn(torch.autograd.Function):
+    """Fused propagation with its adjoint: the same kernels with -z and the
+    pad / crop roles exchanged (the reference gets this from JAX's FFT
+    transpose rules)."""
+
+    @staticmethod
+    def forward(ctx, u, args):
+        ctx.args = args
+        return _propagate_planes(u.detach(), args, L.CXM_OUT_COMPLEX)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _propagate_planes(g.to(torch.complex64).contiguous(), ctx.args, L.CXM_OUT_COMPLEX,
+                                 adjoint=True), None
 
 
 def _check_boundary(absorbing_boundary):

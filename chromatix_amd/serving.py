@@ -88,8 +88,12 @@ class BornRequest:
     iterations: int = 0
 
 
+_ONES = L.doubles([1.0] * 32)  # placeholder z list for size queries (the table carries the distances)
+
+
 def _prop_desc(N, Z, dx, wavelength, n, batch):
     d = L.Prop2dDesc()
+    d.z = ctypes.cast(_ONES, ctypes.POINTER(ctypes.c_double))
     d.batch, d.height, d.width = batch, N, N
     d.num_z = Z
     d.kind = L.CXM_H_ASM

@@ -93,9 +93,9 @@ def test_validation_errors_without_device():
 
 def test_fft_length_support():
     lib = L.lib()
-    for n in [2, 4, 16, 64, 256, 1024, 4096, 8192, 5, 160, 1280, 2560, 5120]:
+    for n in [2, 4, 16, 64, 256, 1024, 4096, 8192, 5, 160, 1280, 2560, 5120, 3, 96, 1536, 3072]:
         assert lib.cxm_fft_supported(n) == 1, n
-    for n in [3, 6, 7, 100, 1000, 2559, 16384]:
+    for n in [7, 9, 100, 1000, 2559, 6144, 10240, 16384]:
         assert lib.cxm_fft_supported(n) == 0, n
 
 

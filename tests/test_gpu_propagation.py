@@ -237,3 +237,33 @@ def test_errors_are_raised():
         cf.asm_propagate(f, 1.0, NM, 0)  # 100 is not an implemented FFT length
     with pytest.raises(ValueError):
         cf.asm_propagate(_mono(_phase_field((8, 8), batch=(3,)), 0.3), [1.0, 2.0], NM, 0)
+
+
+@pytest.mark.parametrize("method", ["asm", "transfer"])
+@pytest.mark.parametrize("mode", ["same", "full"])
+def test_autograd_adjoint_identity(method, mode):
+    """<P u, c> = <u, P^H c> with P^H from the autograd rule (the same
+    kernels with -z, pad and crop exchanged), checked against the FP64
+    oracle adjoint."""
+    rng = np.random.default_rng(11)
+    u = (rng.standard_normal((2, 96, 80)) + 1j * rng.standard_normal((2, 96, 80))).astype(np.complex64)
+    ut = torch.as_tensor(u).cuda().requires_grad_(True)
+    f = cx.ScalarField(ut, (0.25, 0.3), 0.0, 0.532)
+    fn = cf.asm_propagate if method == "asm" else cf.transfer_propagate
+    out = fn(f, 40.0, 1.33, (16, 24), mode=mode, phase_mode="accurate")
+    c = (rng.standard_normal(out.shape) + 1j * rng.standard_normal(out.shape)).astype(np.complex64)
+    loss = torch.sum(torch.real(torch.conj(torch.as_tensor(c).cuda()) * out.u))
+    loss.backward()
+    g = ut.grad.cpu().numpy()
+    # FP64: P = crop_c(ifft2(H fft2(pad_p(u)))), P^H c = crop_p(ifft2(conj(H) fft2(pad_c(c))))
+    py, px = 16, 24
+    cy, cx_ = (py, px) if mode == "same" else (0, 0)
+    Ny, Nx = 96 + 2 * py, 80 + 2 * px
+    H = (op.asm_propagator if method == "asm" else op.transfer_propagator)((Ny, Nx), (0.25, 0.3), 0.532, 40.0, 1.33)
+    cp = np.pad(c.astype(np.complex128), [(0, 0), (cy, cy), (cx_, cx_)])
+    adj = np.fft.ifft2(np.conj(H) * np.fft.fft2(cp), axes=(-2, -1))[..., py: Ny - py, px: Nx - px]
+    # the real-valued loss Re<c, P u> has gradient P^H c
+    assert rel_l2(g, adj) < 1e-5
+    lhs = np.vdot(c, out.u.detach().cpu().numpy())
+    rhs = np.vdot(g, u)
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
