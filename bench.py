@@ -426,18 +426,23 @@ def bench_c5(args):
         torch.cuda.synchronize()
         return reqs
 
-    count, rate = 60, 40.0
+    count = 80
     eng = sv.ServingEngine(max_batch=8, use_graphs=True)
-    eng.warm(mix(5, count, rate))
-    served = eng.run(mix(5, count, rate))
-    st = sv.latency_stats(served)
-    plain = sv.run_plain(mix(5, count, rate))
-    sp = sv.latency_stats(plain)
-    return _result(args, "served requests/s at p99 (70% PSF/AS, 20% SAS+CZT, 10% Born)",
-                   st["throughput_rps"], "requests/s", st["span_s"] * 1e3 / st["n"], st["n"], 0,
-                   "complex64", f"{count} Poisson arrivals at {rate}/s, seed 5", 0, {}, None,
-                   {"serving": st, "plain": sp,
-                    "throughput_ratio": st["throughput_rps"] / sp["throughput_rps"]})
+    eng.warm(mix(5, count, 1.0))
+    sweep = {}
+    for rate in (10.0, 40.0, 160.0):
+        st = sv.latency_stats(eng.run(mix(5, count, rate)))
+        sp = sv.latency_stats(sv.run_plain(mix(5, count, rate)))
+        sweep[str(rate)] = {"serving": st, "plain": sp,
+                            "throughput_ratio": st["throughput_rps"] / sp["throughput_rps"]}
+        log(f"[bench] rate {rate}/s: serving {st['throughput_rps']:.1f} rps p50 {st['p50_ms']:.1f} "
+            f"p99 {st['p99_ms']:.1f} ms | plain {sp['throughput_rps']:.1f} rps p50 {sp['p50_ms']:.1f} "
+            f"p99 {sp['p99_ms']:.1f} ms")
+    top = sweep["160.0"]["serving"]
+    return _result(args, "served requests/s (70% PSF/AS, 20% SAS+CZT, 10% Born), p50/p99 latency",
+                   top["throughput_rps"], "requests/s", top["span_s"] * 1e3 / top["n"], top["n"], 0,
+                   "complex64", f"{count} Poisson arrivals (seed 5) at 10, 40 and 160 per s", 0, {},
+                   None, {"sweep": sweep})
 
 
 CONFIGS = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}

@@ -12,7 +12,7 @@ namespace cxm {
 #define CXM_PIPE_GROUP_DECL(K)                                            \
   int launch_pipe_row_g##K(int N, const PipeArgs& a, hipStream_t s);     \
   int launch_pipe_col_g##K(int N, const PipeArgs& a, hipStream_t s);     \
-  int launch_pipe_phasegrad_g##K(int N, const PipeArgs& a, hipStream_t s);
+  int launch_pipe_phasegrad_g##K(int N, const PipeArgs& a, hipStream_t s, int groups);
 CXM_PIPE_GROUP_DECL(0)
 CXM_PIPE_GROUP_DECL(1)
 CXM_PIPE_GROUP_DECL(2)
@@ -22,8 +22,15 @@ CXM_PIPE_GROUP_DECL(4)
 
 // Row pass, column pass and phase-gradient row pass of length N (over the
 // size groups); CXM_ERR_UNSUPPORTED for a length no group instantiates.
+int launch_pipe_row_variant_1024(int v, const PipeArgs& a, hipStream_t s);
+
 inline int launch_pipe_row(int N, const PipeArgs& a, hipStream_t s) {
   int rc;
+  if (N == 1024) {
+    int v = option(CXM_OPT_PIPE_ROW_VARIANT);
+    if (v < 0) v = 0;
+    if (v > 0 && (rc = launch_pipe_row_variant_1024(v, a, s)) >= 0) return rc;
+  }
   if ((rc = launch_pipe_row_g0(N, a, s)) >= 0) return rc;
   if ((rc = launch_pipe_row_g1(N, a, s)) >= 0) return rc;
   if ((rc = launch_pipe_row_g2(N, a, s)) >= 0) return rc;
@@ -42,13 +49,14 @@ inline int launch_pipe_col(int N, const PipeArgs& a, hipStream_t s) {
   return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
 }
 
-inline int launch_pipe_phasegrad(int N, const PipeArgs& a, hipStream_t s) {
+// groups > 1: plane groups in parallel, partial sums at out + g * out_plane_stride
+inline int launch_pipe_phasegrad(int N, const PipeArgs& a, hipStream_t s, int groups = 1) {
   int rc;
-  if ((rc = launch_pipe_phasegrad_g0(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_phasegrad_g1(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_phasegrad_g2(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_phasegrad_g3(N, a, s)) >= 0) return rc;
-  if ((rc = launch_pipe_phasegrad_g4(N, a, s)) >= 0) return rc;
+  if ((rc = launch_pipe_phasegrad_g0(N, a, s, groups)) >= 0) return rc;
+  if ((rc = launch_pipe_phasegrad_g1(N, a, s, groups)) >= 0) return rc;
+  if ((rc = launch_pipe_phasegrad_g2(N, a, s, groups)) >= 0) return rc;
+  if ((rc = launch_pipe_phasegrad_g3(N, a, s, groups)) >= 0) return rc;
+  if ((rc = launch_pipe_phasegrad_g4(N, a, s, groups)) >= 0) return rc;
   return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
 }
 

@@ -36,6 +36,8 @@ struct PipeArgs {
   int out_line_off, out_line_len;   // lines kept: grid lines [off, off + len)
   int seq, mid_shift, intensity;
   int accum;                  // phase-gradient row pass: add to `out` instead of storing
+                              // (gridDim.y > 1: plane group y writes its partial sum to
+                              // out + y * out_plane_stride)
   // optional store of the value after the `mid` operators (row pass only),
   // grid position (y, x) -> mid_out[plane * mid_plane_stride + (y - mid_y0) *
   // mid_row_stride + (x - mid_x0)] for y - mid_y0 in [0, mid_h), x - mid_x0 in [0, mid_w)
@@ -230,7 +232,10 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
   float* acc = acc_lds + g * N;
 #pragma unroll
   for (int k = 0; k < E; ++k) acc[t + k * T] = 0.0f;
-  for (int p = 0; p < a.n_planes; ++p) {
+  const int per = (a.n_planes + (int)gridDim.y - 1) / (int)gridDim.y;
+  const int p_lo = (int)blockIdx.y * per;
+  const int p_hi = min(a.n_planes, p_lo + per);
+  for (int p = p_lo; p < p_hi; ++p) {
     const float2* src = a.in + p * a.in_plane_stride + (long long)(ok ? r : 0) * a.in_line_stride;
     float2 v[E];
 #pragma unroll
@@ -253,13 +258,15 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
     __syncthreads();  // before the next plane's FFT reuses the slots
   }
   if (!ok) return;
-  float* dst = (float*)a.out + (long long)r * a.out_line_stride;
+  float* dst = (float*)a.out + (long long)blockIdx.y * a.out_plane_stride +
+               (long long)r * a.out_line_stride;
+  const bool add = a.accum && gridDim.y == 1;
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int j = t + k * T;
     const int xf = a.in_shift ? ishift_src(j, N) : j;
     const int xs = xf - a.in_off;
-    if (xs >= 0 && xs < a.in_len) dst[xs] = a.accum ? dst[xs] + acc[j] : acc[j];
+    if (xs >= 0 && xs < a.in_len) dst[xs] = add ? dst[xs] + acc[j] : acc[j];
   }
 }
 
@@ -364,13 +371,13 @@ int launch_pipe_row_t(const PipeArgs& a, hipStream_t s) {
 }
 
 template <int N, int E>
-int launch_pipe_phasegrad_t(const PipeArgs& a, hipStream_t s) {
+int launch_pipe_phasegrad_t(const PipeArgs& a, hipStream_t s, int groups) {
   using Cfg = PipeRowCfg<N, E>;
   const long long blocks = (a.n_lines + Cfg::G - 1) / Cfg::G;
   if (blocks == 0 || a.n_planes == 0) return CXM_OK;
   void* t0 = timing_start(s);
-  hipLaunchKernelGGL((pipe_row_phasegrad_kernel<N, E>), dim3((unsigned)blocks), dim3(Cfg::THREADS), 0,
-                     s, a);
+  hipLaunchKernelGGL((pipe_row_phasegrad_kernel<N, E>), dim3((unsigned)blocks, (unsigned)groups),
+                     dim3(Cfg::THREADS), 0, s, a);
   CXM_LAUNCH_CHECK();
   if (t0) timing_stop(t0, s, "pipe_phasegrad_" + std::to_string(N));
   return CXM_OK;
@@ -406,7 +413,7 @@ int launch_pipe_col_t(const PipeArgs& a, hipStream_t s) {
       default: return -1;                                                     \
     }                                                                         \
   }                                                                           \
-  int launch_pipe_phasegrad_g##K(int N, const PipeArgs& a, hipStream_t s) {   \
+  int launch_pipe_phasegrad_g##K(int N, const PipeArgs& a, hipStream_t s, int groups) { \
     switch (N) {                                                              \
       SIZES(CXM_PGRAD_CASE)                                                   \
       default: return -1;                                                     \
@@ -415,4 +422,4 @@ int launch_pipe_col_t(const PipeArgs& a, hipStream_t s) {
   }
 #define CXM_PROW_CASE(NN, ER, EC, CC) case NN: return launch_pipe_row_t<NN, ER>(a, s);
 #define CXM_PCOL_CASE(NN, ER, EC, CC) case NN: return launch_pipe_col_t<NN, EC, CC>(a, s);
-#define CXM_PGRAD_CASE(NN, ER, EC, CC) case NN: return launch_pipe_phasegrad_t<NN, ER>(a, s);
+#define CXM_PGRAD_CASE(NN, ER, EC, CC) case NN: return launch_pipe_phasegrad_t<NN, ER>(a, s, groups);

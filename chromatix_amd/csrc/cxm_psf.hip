@@ -200,9 +200,32 @@ int psf_geom(const cxm_psf4f_desc* d) {
 
 using namespace cxm;
 
+// Plane groups of the phase-gradient pass (run in parallel, partial sums
+// reduced in a fixed order afterwards).
+constexpr int kGradGroups = 4;
+
+namespace cxm {
+namespace {
+size_t psf_ws_bytes(const cxm_psf4f_desc* d) {
+  const size_t plane = (size_t)d->height * d->width;
+  return (size_t)psf_chunk(d) * plane * sizeof(float2) + kGradGroups * plane * sizeof(float);
+}
+
+// grad[i] (+)= sum_g partial[g][i], g in order
+__global__ void sum_groups_kernel(const float* __restrict__ partial, float* __restrict__ grad,
+                                  long long n, int groups, int accum) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float acc = accum ? grad[i] : 0.0f;
+  for (int g = 0; g < groups; ++g) acc += partial[g * n + i];
+  grad[i] = acc;
+}
+}  // namespace
+}  // namespace cxm
+
 extern "C" size_t cxm_psf4f_workspace_size(const cxm_psf4f_desc* d) {
   if (psf_geom(d)) return 0;
-  return (size_t)psf_chunk(d) * d->height * d->width * sizeof(float2);
+  return psf_ws_bytes(d);
 }
 
 extern "C" int cxm_psf4f_fwd(void* stream, const float* phase, void* out, void* workspace,
@@ -251,11 +274,13 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
     return fail(CXM_ERR_INVALID, "invalid gradient kind");
   if (!phase || !cot || !grad_phase) return fail(CXM_ERR_INVALID, "null buffer");
   const long long chunk = psf_chunk(d);
-  const size_t need = (size_t)chunk * d->height * d->width * sizeof(float2);
+  const size_t need = psf_ws_bytes(d);
   if (!workspace || workspace_bytes < need)
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   if ((rc = ensure_init())) return rc;
   hipStream_t s = (hipStream_t)stream;
+  const long long plane_px = (long long)d->height * d->width;
+  float* partial = (float*)((char*)workspace + (size_t)chunk * plane_px * sizeof(float2));
   PipeArgs ra, ca, ga;
   memset(&ra, 0, sizeof(ra));
   memset(&ca, 0, sizeof(ca));
@@ -279,7 +304,8 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
   ga = ra;
   ga.in_plane_stride = (long long)d->height * d->width;
   ga.in_line_stride = d->width;
-  ga.out = grad_phase;
+  ga.out = partial;
+  ga.out_plane_stride = plane_px;
   ga.out_line_stride = d->width;
   for (long long p0 = 0; p0 < d->num_z; p0 += chunk) {
     const int kc = (int)std::min<long long>(chunk, d->num_z - p0);
@@ -295,8 +321,11 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
     ga.in = (const float2*)workspace;
     ga.n_planes = kc;
     ga.plane0 = (int)p0;
-    ga.accum = p0 > 0;
-    if ((rc = launch_pipe_phasegrad(d->width, ga, s))) return rc;
+    const int groups = std::min<int>(kGradGroups, kc);
+    if ((rc = launch_pipe_phasegrad(d->width, ga, s, groups))) return rc;
+    hipLaunchKernelGGL(sum_groups_kernel, dim3((unsigned)((plane_px + 255) / 256)), dim3(256), 0, s,
+                       (const float*)partial, grad_phase, plane_px, groups, p0 > 0 ? 1 : 0);
+    CXM_LAUNCH_CHECK();
   }
   return CXM_OK;
 }

@@ -161,6 +161,83 @@ def _desc(sample: Sample, k0: float, shift: complex, scale: complex):
     return d
 
 
+class Solver:
+    """Stepwise solve: the work is enqueued in chunks of iterations and the
+    residual comes back through pinned memory and an event, so a caller can
+    interleave other work without blocking (the serving loop does)."""
+
+    def __init__(self, sample: Sample, source: Source, initial_E=None, *, shift_scale=None,
+                 tol: float = 1e-3, rtol: float | None = None, maxiter: int = 1000):
+        eps = sample.permittivity.to(torch.complex64).contiguous()
+        L.ensure_device(eps)
+        self.eps, self.dev, self.shape = eps, eps.device, sample.shape
+        S = torch.as_tensor(source.field, device=self.dev).to(torch.complex64).contiguous()
+        if tuple(S.shape) != sample.shape + (3,):
+            raise ValueError(f"source field must have shape {sample.shape + (3,)}")
+        shift, scale = shift_scale if shift_scale is not None else get_shift_and_scale(eps)
+        self.d = _desc(sample, source.k0, complex(shift), complex(scale))
+        self.ws = torch.empty(L.lib().cxm_mbs_workspace_size(ctypes.byref(self.d)), dtype=torch.uint8,
+                              device=self.dev)
+        self.py = torch.empty((3,) + sample.shape, dtype=torch.complex64, device=self.dev)
+        self.num = torch.empty(2, dtype=torch.float64, device=self.dev)
+        s = L.stream_ptr(self.dev)
+        L.check(L.lib().cxm_mbs_precondition(s, eps.data_ptr(), S.data_ptr(), self.py.data_ptr(),
+                                             self.num.data_ptr(), self.ws.data_ptr(), self.ws.numel(),
+                                             ctypes.byref(self.d)))
+        if initial_E is None:
+            self.x = self.py.clone()
+        else:
+            E0 = torch.as_tensor(initial_E, device=self.dev).to(torch.complex64)
+            self.x = (E0.permute(3, 0, 1, 2) / self.num[0].to(torch.float32)).contiguous()
+        self.rtol, self.tol = rtol, float(tol)
+        self.maxiter = int(maxiter)
+        # [residual^2, |prec_y|^2], device and pinned host copies
+        self.res_dev = torch.zeros(2, dtype=torch.float64, device=self.dev)
+        self.res_host = torch.zeros(2, dtype=torch.float64, pin_memory=True)
+        if rtol is not None:
+            pws = torch.empty(max(1, L.lib().cxm_plane_sumsq_workspace_size(1)), dtype=torch.uint8,
+                              device=self.dev)
+            L.check(L.lib().cxm_plane_sumsq_c64(s, self.py.data_ptr(), self.res_dev[1:].data_ptr(), 1,
+                                                self.py.numel(), 1, None, pws.data_ptr(), pws.numel()))
+            self._pws = pws
+        self.target = float("nan")
+        self.event = torch.cuda.Event()
+        self.iterations, self.residual = 0, float("inf")
+        self.pending = False
+
+    def step(self, k: int) -> None:
+        """Enqueue k iterations and the residual read-back."""
+        k = min(int(k), self.maxiter - self.iterations)
+        if k <= 0:
+            return
+        L.check(L.lib().cxm_mbs_iterate(L.stream_ptr(self.dev), self.eps.data_ptr(), self.py.data_ptr(),
+                                        self.x.data_ptr(), k, self.res_dev.data_ptr(), self.ws.data_ptr(),
+                                        self.ws.numel(), ctypes.byref(self.d)))
+        self.res_host.copy_(self.res_dev, non_blocking=True)
+        self.event.record()
+        self.iterations += k
+        self.pending = True
+
+    def poll(self) -> bool:
+        """True once the last step's residual has arrived (non-blocking)."""
+        if self.pending and self.event.query():
+            r2, p2 = self.res_host.tolist()
+            self.residual = float(np.sqrt(r2))
+            self.target = self.rtol * float(np.sqrt(p2)) if self.rtol is not None else self.tol
+            self.pending = False
+        return not self.pending
+
+    @property
+    def converged(self) -> bool:
+        return self.residual < self.target or self.iterations >= self.maxiter
+
+    def result(self) -> torch.Tensor:
+        E = torch.empty(self.shape + (3,), dtype=torch.complex64, device=self.dev)
+        L.check(L.lib().cxm_mbs_output(L.stream_ptr(self.dev), self.x.data_ptr(), self.num.data_ptr(),
+                                       E.data_ptr(), ctypes.byref(self.d)))
+        return E
+
+
 def solve(sample: Sample, source: Source, initial_E=None, maxiter: int = 1000, tol: float = 1e-3,
           implicit_diff: bool = True, use_bicgstab: bool = False, *, rtol: float | None = None,
           check_every: int = 8, shift_scale=None, return_info: bool = False):
@@ -172,44 +249,15 @@ def solve(sample: Sample, source: Source, initial_E=None, maxiter: int = 1000, t
     if use_bicgstab:
         raise NotImplementedError("use_bicgstab=True is not implemented; the fixed-point "
                                   "iteration is")
-    eps = sample.permittivity.to(torch.complex64).contiguous()
-    L.ensure_device(eps)
-    dev = eps.device
-    S = torch.as_tensor(source.field, device=dev).to(torch.complex64).contiguous()
-    if tuple(S.shape) != sample.shape + (3,):
-        raise ValueError(f"source field must have shape {sample.shape + (3,)}")
-    shift, scale = shift_scale if shift_scale is not None else get_shift_and_scale(eps)
-    d = _desc(sample, source.k0, complex(shift), complex(scale))
-    vox = int(np.prod(sample.shape))
-    ws = torch.empty(L.lib().cxm_mbs_workspace_size(ctypes.byref(d)), dtype=torch.uint8, device=dev)
-    py = torch.empty((3,) + sample.shape, dtype=torch.complex64, device=dev)
-    num = torch.empty(2, dtype=torch.float64, device=dev)
-    s = L.stream_ptr(dev)
-    L.check(L.lib().cxm_mbs_precondition(s, eps.data_ptr(), S.data_ptr(), py.data_ptr(),
-                                         num.data_ptr(), ws.data_ptr(), ws.numel(), ctypes.byref(d)))
-    if initial_E is None:
-        x = py.clone()
-    else:
-        E0 = torch.as_tensor(initial_E, device=dev).to(torch.complex64)
-        x = (E0.permute(3, 0, 1, 2) / num[0].to(torch.float32)).contiguous()
-    if rtol is not None:
-        from ..functional.elementwise import plane_sumsq_raw
-
-        target = float(rtol) * float(np.sqrt(plane_sumsq_raw(py)))
-    else:
-        target = float(tol)
-    res = torch.empty(1, dtype=torch.float64, device=dev)
-    it, r = 0, float("inf")
-    while it < maxiter:
-        k = min(check_every, maxiter - it)
-        L.check(L.lib().cxm_mbs_iterate(s, eps.data_ptr(), py.data_ptr(), x.data_ptr(), k,
-                                        res.data_ptr(), ws.data_ptr(), ws.numel(), ctypes.byref(d)))
-        it += k
-        r = float(np.sqrt(res.item()))
-        if r < target:
+    sv = Solver(sample, source, initial_E, shift_scale=shift_scale, tol=tol, rtol=rtol,
+                maxiter=maxiter)
+    while True:
+        sv.step(check_every)
+        sv.event.synchronize()
+        sv.poll()
+        if sv.converged:
             break
-    E = torch.empty(sample.shape + (3,), dtype=torch.complex64, device=dev)
-    L.check(L.lib().cxm_mbs_output(s, x.data_ptr(), num.data_ptr(), E.data_ptr(), ctypes.byref(d)))
+    E = sv.result()
     if return_info:
-        return MBSResult(E, it, r)
+        return MBSResult(E, sv.iterations, sv.residual)
     return E

@@ -16,9 +16,9 @@ on different streams and overlap on the device.
 
 SAS + CZT requests (a chromatic field -> scalable angular spectrum
 propagation -> per wavelength a CZT zoom onto a smaller output window) run
-eagerly on their own stream; Born requests (modified Born series solves,
-which read their residual back every few iterations) run on a worker
-thread with its own stream.
+eagerly on their own stream; Born requests (modified Born series solves)
+advance in chunks of iterations on their own streams, their residuals read
+back through pinned memory and events, so the loop never blocks on them.
 
 The plain form for comparison runs every request alone, in arrival order,
 one library call per operation, waiting for each to finish.
@@ -26,7 +26,6 @@ one library call per operation, waiting for each to finish.
 from __future__ import annotations
 
 import ctypes
-import threading
 import time
 from dataclasses import dataclass
 
@@ -218,6 +217,42 @@ def _born(r: BornRequest):
     return res.E, res.iterations
 
 
+_BORN_CHUNK = 8  # iterations enqueued between residual checks
+
+
+class _BornJob:
+    """A Born solve advanced by the serving loop without blocking: chunks of
+    iterations on its own stream, residual read back through pinned memory."""
+
+    def __init__(self, r: BornRequest, device):
+        from .experimental import modified_born_series as mbs
+
+        self.r = r
+        self.stream = torch.cuda.Stream(device=device)
+        self.stream.wait_stream(torch.cuda.current_stream(device))
+        with torch.cuda.stream(self.stream):
+            self.sv = mbs.Solver(mbs.Sample(r.permittivity, r.spacing), mbs.Source(r.current, r.k0),
+                                 rtol=r.rtol, maxiter=3000)
+            self.sv.step(_BORN_CHUNK)
+        self.out_event = None
+
+    def advance(self) -> bool:
+        """True when finished (result enqueued and complete)."""
+        if self.out_event is not None:
+            return self.out_event.query()
+        if not self.sv.poll():
+            return False
+        with torch.cuda.stream(self.stream):
+            if self.sv.converged:
+                self.r.result = self.sv.result()
+                self.r.iterations = self.sv.iterations
+                self.out_event = torch.cuda.Event()
+                self.out_event.record(self.stream)
+            else:
+                self.sv.step(_BORN_CHUNK)
+        return False
+
+
 class ServingEngine:
     """Continuous-batching engine (see the module docstring)."""
 
@@ -278,7 +313,10 @@ class ServingEngine:
                     key = (r.u.shape[-1], len(r.z), B, r.dx, r.wavelength, r.n)
                     if key not in seen:
                         seen.add(key)
-                        self._instance(r, B)
+                        insts = self.buckets.setdefault(key, [])
+                        while len(insts) < 2:  # double-buffered per bucket
+                            insts.append(_Bucket(r.u.shape[-1], len(r.z), B, r.dx, r.wavelength, r.n,
+                                                 self.device, self.use_graphs))
         torch.cuda.synchronize(self.device)
 
     def run(self, requests):
@@ -288,32 +326,19 @@ class ServingEngine:
         pending = list(reqs)
         queue: dict[tuple, list[PropRequest]] = {}
         side: list = []
-        born_threads = []
+        born: list[_BornJob] = []
         in_use = 0
         active = []
         finished = []
-        lock = threading.Lock()
         self.t0 = time.perf_counter()
 
-        def born_worker(r):
-            with torch.cuda.device(self.device):
-                s = torch.cuda.Stream(device=self.device)
-                s.wait_stream(torch.cuda.default_stream(self.device))
-                with torch.cuda.stream(s):
-                    E, it = _born(r)
-                    s.synchronize()
-            with lock:
-                r.result = E if self.keep_results else None
-                r.iterations = it
-                r.done = self._now()
-
-        while pending or queue or active or side or any(t.is_alive() for t in born_threads):
+        while pending or queue or active or side or born:
             now = self._now()
             # admit arrivals within the HBM budget
             while pending and pending[0].arrival <= now:
                 r = pending[0]
                 need = self._bytes(r)
-                if in_use + need > self.hbm_budget and (active or side):
+                if in_use + need > self.hbm_budget and (active or side or born):
                     break
                 pending.pop(0)
                 in_use += need
@@ -327,10 +352,7 @@ class ServingEngine:
                         ev.record(self.side_stream)
                     side.append((r, ev))
                 else:
-                    t = threading.Thread(target=born_worker, args=(r,), daemon=True)
-                    t.start()
-                    born_threads.append(t)
-                    active.append(("born", r, t))
+                    born.append(_BornJob(r, self.device))
             # schedule batches
             for key in list(queue):
                 q = queue[key]
@@ -348,21 +370,25 @@ class ServingEngine:
             # completions
             still = []
             for kind, obj, t in active:
-                if kind == "prop":
-                    done = obj.poll(self.keep_results, self._now)
-                    if done:
-                        for r in done:
-                            in_use -= self._bytes(r)
-                        finished += done
-                    else:
-                        still.append((kind, obj, t))
+                done = obj.poll(self.keep_results, self._now)
+                if done:
+                    for r in done:
+                        in_use -= self._bytes(r)
+                    finished += done
                 else:
-                    if t.is_alive():
-                        still.append((kind, obj, t))
-                    else:
-                        in_use -= self._bytes(obj)
-                        finished.append(obj)
+                    still.append((kind, obj, t))
             active = still
+            keep_b = []
+            for job in born:
+                if job.advance():
+                    job.r.done = self._now()
+                    if not self.keep_results:
+                        job.r.result = None
+                    in_use -= self._bytes(job.r)
+                    finished.append(job.r)
+                else:
+                    keep_b.append(job)
+            born = keep_b
             keep = []
             for r, ev in side:
                 if ev.query():
@@ -374,7 +400,7 @@ class ServingEngine:
                 else:
                     keep.append((r, ev))
             side = keep
-            if not active and not side and not queue and pending:
+            if not active and not side and not queue and not born and pending:
                 wait = pending[0].arrival - self._now()
                 if wait > 0:
                     time.sleep(min(wait, 0.002))
@@ -411,8 +437,16 @@ def run_plain(requests, keep_results: bool = False):
 
 
 def latency_stats(reqs) -> dict:
+    """p50 / p99 latency (arrival to completion), throughput over the span
+    from the first arrival to the last completion, and per request type."""
     lat = np.array([r.done - r.arrival for r in reqs if r.done is not None])
     span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
-    return {"n": int(lat.size), "p50_ms": float(np.percentile(lat, 50) * 1e3),
-            "p99_ms": float(np.percentile(lat, 99) * 1e3), "throughput_rps": float(lat.size / span),
-            "span_s": float(span)}
+    out = {"n": int(lat.size), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+           "p99_ms": float(np.percentile(lat, 99) * 1e3), "throughput_rps": float(lat.size / span),
+           "span_s": float(span)}
+    for cls in (PropRequest, SasCztRequest, BornRequest):
+        lt = np.array([r.done - r.arrival for r in reqs if isinstance(r, cls) and r.done is not None])
+        if lt.size:
+            out[cls.__name__] = {"n": int(lt.size), "p50_ms": float(np.percentile(lt, 50) * 1e3),
+                                 "p99_ms": float(np.percentile(lt, 99) * 1e3)}
+    return out

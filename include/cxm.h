@@ -476,10 +476,13 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *   CXM_OPT_CHUNK_PLANES  planes per chunk of the propagation passes
  *                         (default: a workspace of at most 128 MiB)
  *   CXM_OPT_ROW_VARIANT   alternative row-pass shape for 1024-point rows
+ *   CXM_OPT_PIPE_ROW_VARIANT  alternative shape of the 1024-point rows of the
+ *                         operator pipelines (multislice, optics, SAS, CZT)
  */
 #define CXM_OPT_COL_VARIANT 1
 #define CXM_OPT_CHUNK_PLANES 2
 #define CXM_OPT_ROW_VARIANT 3
+#define CXM_OPT_PIPE_ROW_VARIANT 4
 int cxm_set_option(int key, int value);
 
 /* Library state. */

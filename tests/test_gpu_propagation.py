@@ -267,3 +267,17 @@ def test_autograd_adjoint_identity(method, mode):
     lhs = np.vdot(c, out.u.detach().cpu().numpy())
     rhs = np.vdot(g, u)
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(96, 192), (384, 48), (1536, 24), (3072, 12)])
+def test_three_times_power_of_two_grids(shape):
+    """Lengths 3 * 2^a (radix-3 first stage) in rows and columns, ASM, accurate."""
+    rng = np.random.default_rng(sum(shape))
+    u = np.exp(2j * np.pi * rng.random(shape)).astype(np.complex64)
+    f = cx.ScalarField(torch.as_tensor(u).cuda(), (0.3, 0.25), 0.0, 0.532)
+    out = cf.asm_propagate(f, 30.0, 1.33, 0, phase_mode="accurate").u.cpu().numpy()
+    ref = op.propagate(u, (0.3, 0.25), 0.532, 30.0, 1.33, 0)
+    assert rel_l2(out, ref) < 1e-5
+    out_t = cf.asm_propagate(cx.ScalarField(torch.as_tensor(u.T.copy()).cuda(), (0.25, 0.3), 0.0, 0.532),
+                             30.0, 1.33, 0, phase_mode="accurate").u.cpu().numpy()
+    assert rel_l2(out_t, ref.T) < 1e-5
