@@ -51,7 +51,10 @@ int launch_col_variant_2048(int v, const ColArgs& a, hipStream_t s);
 
 int launch_col_prop(int N, const ColArgs& a, hipStream_t s) {
   int rc;
-  const int v = option(CXM_OPT_COL_VARIANT);
+  int v = option(CXM_OPT_COL_VARIANT);
+  // single planes (the multislice loop): 8 values per thread and 4 columns,
+  // twice the waves of the batched default (20 -> 18 us per plane)
+  if (v == 0 && N == 1024 && a.n_planes <= 4) v = 2;
   if (v > 0 && N == 1024 && (rc = launch_col_variant_1024(v, a, s)) >= 0) return rc;
   if (v > 0 && N == 2048 && (rc = launch_col_variant_2048(v, a, s)) >= 0) return rc;
   if ((rc = launch_col_g0(N, a, s)) >= 0) return rc;

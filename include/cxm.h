@@ -469,8 +469,9 @@ int cxm_timing_collect(void);
 int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* launches);
 
 /*
- * Launch-shape tuning (0 = default for every key).  No counterpart in the
- * reference (performance tuning).
+ * Launch-shape tuning (0 = the built-in choice, which for 1024-point
+ * passes over at most a few planes picks a shape with more waves).  No
+ * counterpart in the reference (performance tuning).
  *   CXM_OPT_COL_VARIANT   alternative column-pass shape for 1024- and
  *                         2048-point columns
  *   CXM_OPT_CHUNK_PLANES  planes per chunk of the propagation passes
