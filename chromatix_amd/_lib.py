@@ -31,6 +31,16 @@ CXM_PHASE_ACCURATE = 1
 CXM_OUT_COMPLEX = 0
 CXM_OUT_INTENSITY = 1
 
+# cxm_set_option keys (launch-shape tuning and the shared H table)
+CXM_OPT_COL_VARIANT = 1
+CXM_OPT_CHUNK_PLANES = 2
+CXM_OPT_ROW_VARIANT = 3
+CXM_OPT_PIPE_ROW_VARIANT = 4
+CXM_OPT_H_TABLE = 5
+CXM_OPT_COL_ORDER = 6
+CXM_OPT_TILES = 7
+CXM_H_TABLE_MIN_PLANES = 4
+
 PHASE_MODES = {"ref32": CXM_PHASE_REF32, "accurate": CXM_PHASE_ACCURATE}
 
 

@@ -187,31 +187,83 @@ constexpr int num_stages(int N, int E) {
 }
 
 // LDS slot of element e of a transform of length N: a stride S between
-// elements (S column transforms interleaved, slot = row * S + column) and,
-// for S == 1, a one-slot pad every 16 elements that breaks the power-of-two
-// bank aliasing of the radix-16 scatter.  (An XOR row swizzle removes the
-// 16-way conflict of the column scatter too, but its address arithmetic
-// raised the column kernel from 76 to 104 VGPRs and cost more than it saved.)
+// elements (S column transforms interleaved, slot = row * S + column) and a
+// one-row pad every 16 rows, which breaks the power-of-two bank aliasing of
+// the radix-16 scatter (without it the S lanes of a row and the 16 rows a
+// wave writes land on the same banks: 2.75x the ideal LDS cycles for
+// 1024-point columns of 4).  Lengths 5*2^a and 3*2^a with S > 1 keep the
+// plain stride, which conflicts less for their radix-5/3 stages.  (An XOR
+// row swizzle removes the conflicts too, but its address arithmetic costs
+// more registers than this pad.)
+template <int N, int S>
+constexpr bool lds_padded() { return S == 1 || (N & (N - 1)) == 0; }
 template <int N, int S>
 CXM_DEV int lds_at(int e) {
-  if constexpr (S == 1) return e + (e >> 4);
+  if constexpr (lds_padded<N, S>()) return (e + (e >> 4)) * S;
   else return e * S;
 }
+// lds_at(a + b) where (a mod 16) + (b mod 16) < 16, so that the row pad
+// count of a + b is that of a plus that of b: the slot splits into a runtime
+// part and a compile-time part that becomes the instruction's immediate
+// offset (one address register per butterfly instead of one per value).
+// Callers: a = t < T, b = e * T (T a power of two); a = q * Ns * R + k with
+// k < Ns, b = r * Ns (Ns, R powers of two).
 template <int N, int S>
-constexpr int lds_span() {  // float2 slots occupied by one transform
-  if constexpr (S == 1) return N + N / 16;
+CXM_DEV int lds_at_split(int a, int b) {
+  if constexpr (lds_padded<N, S>() && (N & (N - 1)) == 0) return lds_at<N, S>(a) + lds_at<N, S>(b);
+  else return lds_at<N, S>(a + b);
+}
+template <int N, int S>
+constexpr int lds_span() {  // float2 slots occupied by S interleaved transforms
+  if constexpr (lds_padded<N, S>()) return (N + N / 16) * S;
   else return N * S;
 }
 
 // One Stockham stage.  `lds` points at this transform's element 0.
 // SYNC: 0 -> wave-local transform (T <= 64 and one transform per wave group),
 // 1 -> block-wide barrier.
+// Twiddles of the stage that starts after a product Ns of earlier radices:
+// butterfly m, input r >= 1 is multiplied by W_{Ns*R}^{(j mod Ns) * r}.  A
+// stage loads the next stage's twiddles before its LDS exchange, so that the
+// loads are in flight during the barrier rather than exposed after it.
+template <int N, int E, int Ns>
+struct StageTw {
+  static constexpr bool active = Ns > 1 && Ns < N;
+  static constexpr int R = (Ns < N) ? pick_radix(N / Ns, E) : 1;
+  static constexpr int COUNT = active ? (E / R) * (R - 1) : 1;
+  float2 w[COUNT];
+};
+
+template <int N, int E, int DIR, int Ns>
+CXM_DEV void load_stage_tw(StageTw<N, E, Ns>& st, int t, const float2* __restrict__ tw) {
+  if constexpr (StageTw<N, E, Ns>::active) {
+    constexpr int R = StageTw<N, E, Ns>::R;
+    constexpr int M = E / R;
+    constexpr int T = N / E;
+    constexpr int twstep = N / (Ns * R);
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      const int k = (t + m * T) % Ns;
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        float2 x = tw[k * r * twstep];
+        if constexpr (DIR > 0) x.y = -x.y;
+        st.w[m * (R - 1) + r - 1] = x;
+      }
+    }
+  }
+}
+
+template <int N, int E, int DIR, int Ns, int S, bool SYNC>
+CXM_DEV void stockham_from(float2 (&v)[E], float2* lds, int t, const float2* __restrict__ tw,
+                           const StageTw<N, E, Ns>& w);
+
 template <int N, int E, int DIR, int Ns, int R, int S, bool SYNC>
-CXM_DEV void stockham_stage(float2 (&v)[E], float2* lds, int t, const float2* __restrict__ tw) {
+CXM_DEV void stockham_stage(float2 (&v)[E], float2* lds, int t, const float2* __restrict__ tw,
+                            const StageTw<N, E, Ns>& w) {
   constexpr int T = N / E;
   constexpr int M = E / R;
   constexpr bool last = (Ns * R == N);
-  constexpr int twstep = N / (Ns * R);
 #pragma unroll
   for (int m = 0; m < M; ++m) {
     const int j = t + m * T;
@@ -220,13 +272,8 @@ CXM_DEV void stockham_stage(float2 (&v)[E], float2* lds, int t, const float2* __
 #pragma unroll
     for (int r = 0; r < R; ++r) a[r] = v[m + r * M];
     if constexpr (Ns > 1) {
-      const float2* twp = tw;
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        float2 w = twp[k * r * twstep];
-        if constexpr (DIR > 0) w.y = -w.y;
-        a[r] = cmul(a[r], w);
-      }
+      for (int r = 1; r < R; ++r) a[r] = cmul(a[r], w.w[m * (R - 1) + r - 1]);
     }
     if constexpr (R == 2) Dft<2, DIR>::run(a);
     else if constexpr (R == 3) Dft<3, DIR>::run(a);
@@ -238,28 +285,33 @@ CXM_DEV void stockham_stage(float2 (&v)[E], float2* lds, int t, const float2* __
 #pragma unroll
       for (int r = 0; r < R; ++r) v[m + r * M] = a[r];
     } else {
+      // base = q * Ns * R + k with k < Ns: base + r * Ns never carries past
+      // a 16-row boundary that r * Ns does not (powers of two)
       const int base = (j / Ns) * Ns * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) lds[lds_at<N, S>(base + r * Ns)] = a[r];
+      for (int r = 0; r < R; ++r) lds[lds_at_split<N, S>(base, r * Ns)] = a[r];
     }
   }
   if constexpr (!last) {
+    StageTw<N, E, Ns * R> wn;
+    load_stage_tw<N, E, DIR, Ns * R>(wn, t, tw);
     if constexpr (SYNC) __syncthreads();
     else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = lds[lds_at<N, S>(t + e * T)];
+    for (int e = 0; e < E; ++e) v[e] = lds[lds_at_split<N, S>(t, e * T)];
     if constexpr (SYNC) __syncthreads();
     else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+    stockham_from<N, E, DIR, Ns * R, S, SYNC>(v, lds, t, tw, wn);
   }
 }
 
 template <int N, int E, int DIR, int Ns, int S, bool SYNC>
-CXM_DEV void stockham_from(float2 (&v)[E], float2* lds, int t, const float2* __restrict__ tw) {
+CXM_DEV void stockham_from(float2 (&v)[E], float2* lds, int t, const float2* __restrict__ tw,
+                           const StageTw<N, E, Ns>& w) {
   if constexpr (Ns < N) {
     constexpr int R = pick_radix(N / Ns, E);
     static_assert(R > 0, "FFT length / thread split has no radix schedule");
-    stockham_stage<N, E, DIR, Ns, R, S, SYNC>(v, lds, t, tw);
-    stockham_from<N, E, DIR, Ns * R, S, SYNC>(v, lds, t, tw);
+    stockham_stage<N, E, DIR, Ns, R, S, SYNC>(v, lds, t, tw, w);
   }
 }
 
@@ -269,7 +321,8 @@ template <int N, int E, int DIR, int S, bool SYNC>
 CXM_DEV void fft(float2 (&v)[E], float2* lds, int t, const float2* __restrict__ tw) {
   static_assert(N % E == 0, "E must divide N");
   static_assert(schedule_ok(N, E), "no radix schedule for (N, E)");
-  stockham_from<N, E, DIR, 1, S, SYNC>(v, lds, t, tw);
+  const StageTw<N, E, 1> none{};
+  stockham_from<N, E, DIR, 1, S, SYNC>(v, lds, t, tw, none);
 }
 
 }  // namespace cxm

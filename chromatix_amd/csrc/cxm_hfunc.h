@@ -119,4 +119,17 @@ CXM_DEV float2 eval_H(const HParam& hp, const XTerms& xt, int ky, int Ny, int ki
   return h;
 }
 
+// Numerical-aperture mask |f|^2 <= na_r2 on the unshifted frequency grid,
+// f = fl(fl(1/d) * fl(k/N)) as the reference's f_grid
+// (src/chromatix/functional/samples.py:271-280).  na_fx2 is the
+// column-constant part.
+CXM_DEV float na_fx2(const HParam& hp, int kx, int Nx) {
+  const float fx = __fmul_rn(hp.inv_dx, __fdiv_rn((float)signed_bin(kx, Nx), (float)Nx));
+  return __fmul_rn(fx, fx);
+}
+CXM_DEV bool na_keep(const HParam& hp, int ky, int Ny, float fx2, float na_r2) {
+  const float fy = __fmul_rn(hp.inv_dy, __fdiv_rn((float)signed_bin(ky, Ny), (float)Ny));
+  return !(__fadd_rn(__fmul_rn(fy, fy), fx2) > na_r2);
+}
+
 }  // namespace cxm

@@ -19,6 +19,10 @@ int launch_col_variant_1024(int v, const ColArgs& a, hipStream_t s) {
     default: return -1;
   }
 }
+int col_variant_width_1024(int v) {
+  static const int c[8] = {0, 8, 4, 8, 4, 2, 2, 2};
+  return (v > 0 && v < 8) ? c[v] : -1;
+}
 int launch_row_variant_1024(int v, int dir, const RowArgs& a, hipStream_t s) {
   switch (v) {
     case 1: return launch_row_t<1024, 8>(dir, a, s);

@@ -14,4 +14,8 @@ int launch_col_variant_2048(int v, const ColArgs& a, hipStream_t s) {
     default: return -1;
   }
 }
+int col_variant_width_2048(int v) {
+  static const int c[5] = {0, 8, 8, 2, 4};
+  return (v > 0 && v < 5) ? c[v] : -1;
+}
 }  // namespace cxm

@@ -45,6 +45,10 @@ int ensure_init();
 
 constexpr int kMaxHP = 32;  // parameter sets carried by value per launch
 
+// Tiled planes: strips of kTileW columns, each strip's rows contiguous, so
+// that one 128-byte line holds 4 rows x 4 columns of complex64.
+constexpr int kTileW = 4;
+
 struct RowArgs {
   const float2* in;
   float2* out_c;
@@ -57,6 +61,12 @@ struct RowArgs {
   int out_off, out_len;  // FFT indices [out_off, out_off+out_len) are stored
   float scale;
   int intensity;
+  // Tiled planes (strips of kTileW columns stored one after another, see
+  // cxm_prop2d_kernels.h): element (y, x) at (x / 4) * strip_stride + y * 4 + x % 4.
+  // Only with 4 rows per workgroup, rows_per_plane % 4 == 0 and the full
+  // row range (in_off 0, in_len N, or out_off 0, out_len N).
+  int in_tiled, out_tiled;
+  long long in_strip_stride, out_strip_stride;  // elements
 };
 
 struct ColArgs {
@@ -65,7 +75,10 @@ struct ColArgs {
   const float2* tw;
   const float2* H;       // CXM_H_ARRAY
   long long in_plane_stride, out_plane_stride;
-  int row_stride;        // elements between rows (= number of columns Nx)
+  int row_stride;        // elements between rows (Nx, or kTileW for tiled planes)
+  long long strip_stride;    // elements between strips of C columns (0: C, untiled)
+  long long h_strip_stride;  // CXM_H_ARRAY: elements between strips (0: C)
+  int h_row_stride;          // CXM_H_ARRAY: elements between rows (0: ncols)
   int ncols;
   int n_planes;          // input planes in this launch
   int plane0;            // global index of the first input plane (param lookup)
@@ -78,7 +91,7 @@ struct ColArgs {
   float scale;
   float na_r2;           // > 0: H also masked to |f|^2 <= na_r2 (fl((NA/lam)^2), samples.py:271-280)
   int nhp;
-  int pad_;
+  int strip_major;       // block order: 0 plane-major, 1 strip-major (H shared by all planes)
   // optional device table, one HParam per output plane (global input plane
   // p, distance iz -> hp_dev[p * nout + iz]); replaces hp[] when non-null
   const HParam* hp_dev;

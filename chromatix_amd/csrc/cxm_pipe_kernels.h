@@ -75,13 +75,13 @@ template <int N, int E, int S>
 CXM_DEV void stage_out(const float2 (&v)[E], float2* st, int t) {
   constexpr int T = N / E;
 #pragma unroll
-  for (int k = 0; k < E; ++k) st[lds_at<N, S>(t + k * T)] = v[k];
+  for (int k = 0; k < E; ++k) st[lds_at_split<N, S>(t, k * T)] = v[k];
 }
 template <int N, int E, int S>
 CXM_DEV void stage_in(float2 (&v)[E], const float2* st, int t) {
   constexpr int T = N / E;
 #pragma unroll
-  for (int k = 0; k < E; ++k) v[k] = st[lds_at<N, S>(t + k * T)];
+  for (int k = 0; k < E; ++k) v[k] = st[lds_at_split<N, S>(t, k * T)];
 }
 
 template <int N, int E, int S, bool SYNC>
@@ -281,7 +281,7 @@ template <int N, int E, int C>
 __global__ void __launch_bounds__((PipeColCfg<N, E, C>::THREADS))
 pipe_col_kernel(PipeArgs a) {
   constexpr int T = N / E;
-  __shared__ float2 lds[N * C];
+  __shared__ float2 lds[lds_span<N, C>()];
   const int c = threadIdx.x % C;
   const int ty = threadIdx.x / C;
   const int nstrips = (a.n_lines + C - 1) / C;

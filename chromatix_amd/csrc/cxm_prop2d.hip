@@ -2,8 +2,12 @@
 //
 // One propagation of a plane is three passes over HBM:
 //   row pass    : zero-padded load -> row FFT                  -> workspace
-//   column pass : column FFT -> * H(ky,kx) (on the fly) -> column IFFT, in place
+//   column pass : column FFT -> * H(ky,kx) -> column IFFT, in place
 //   row pass    : row IFFT -> crop -> complex or |u|^2         -> output
+// H is evaluated on the fly in the column pass, or, when the whole batch
+// shares one parameter set, once per call into a table after the chunk
+// workspace (the reference likewise builds one propagator and broadcasts it
+// over the batch); the column pass then reads the table from L2.
 // The column pass reads all rows of its strip before writing any, so it runs
 // in place; the workspace holds one chunk of planes, sized to stay inside the
 // 256 MB Infinity Cache so that the intermediate round trips are served by
@@ -48,13 +52,46 @@ int launch_row(int N, int dir, const RowArgs& a, hipStream_t s) {
 
 int launch_col_variant_1024(int v, const ColArgs& a, hipStream_t s);
 int launch_col_variant_2048(int v, const ColArgs& a, hipStream_t s);
+int col_variant_width_1024(int v);
+int col_variant_width_2048(int v);
+
+// Launch-shape variant of the column pass: CXM_OPT_COL_VARIANT, or for a
+// single plane or a few (the multislice loop) 8 values per thread and 4
+// columns, twice the waves of the batched default (20 -> 18 us per plane).
+int col_variant(int N, int n_planes) {
+  int v = option(CXM_OPT_COL_VARIANT);
+  if (v == 0 && N == 1024 && n_planes <= 4) v = 2;
+  if (N == 1024 && col_variant_width_1024(v) > 0) return v;
+  if (N == 2048 && col_variant_width_2048(v) > 0) return v;
+  return 0;
+}
+
+// Columns per column-pass workgroup (the strip width C) for this launch.
+int col_width(int N, int n_planes) {
+  const int v = col_variant(N, n_planes);
+  if (v > 0) return N == 1024 ? col_variant_width_1024(v) : col_variant_width_2048(v);
+  switch (N) {
+#define CXM_CASE(NN, ER, EC, CC) case NN: return CC;
+    CXM_FOR_EACH_SIZE(CXM_CASE)
+#undef CXM_CASE
+    default: return 0;
+  }
+}
+
+// Rows per row-pass workgroup.
+int row_group(int N) {
+  if (option(CXM_OPT_ROW_VARIANT) != 0) return 0;
+  switch (N) {
+#define CXM_CASE(NN, ER, EC, CC) case NN: return (NN / ER >= 256) ? 1 : 256 / (NN / ER);
+    CXM_FOR_EACH_SIZE(CXM_CASE)
+#undef CXM_CASE
+    default: return 0;
+  }
+}
 
 int launch_col_prop(int N, const ColArgs& a, hipStream_t s) {
   int rc;
-  int v = option(CXM_OPT_COL_VARIANT);
-  // single planes (the multislice loop): 8 values per thread and 4 columns,
-  // twice the waves of the batched default (20 -> 18 us per plane)
-  if (v == 0 && N == 1024 && a.n_planes <= 4) v = 2;
+  const int v = col_variant(N, a.n_planes);
   if (v > 0 && N == 1024 && (rc = launch_col_variant_1024(v, a, s)) >= 0) return rc;
   if (v > 0 && N == 2048 && (rc = launch_col_variant_2048(v, a, s)) >= 0) return rc;
   if ((rc = launch_col_g0(N, a, s)) >= 0) return rc;
@@ -65,6 +102,50 @@ int launch_col_prop(int N, const ColArgs& a, hipStream_t s) {
   return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(N));
 }
 
+// H(ky, kx) of one parameter set over the whole Ny x Nx frequency grid, the
+// exact values col_prop_kernel would evaluate on the fly (same functions,
+// same arguments), NA mask included.  Stored in strips of W columns (the
+// column pass's strip width), each strip's rows contiguous: element (ky, kx)
+// at (kx / W) * Ny * W + ky * W + kx % W, so that a strip is read as whole
+// lines.
+template <int KIND, int MODE>
+__global__ void __launch_bounds__(256)
+h_table_kernel(HParam hp, int Ny, int Nx, int W, int remove_evanescent, int bandlimit, float scale,
+               float na_r2, float2* table) {
+  // thread i writes table element i (contiguous stores)
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const long long strip = i / ((long long)Ny * W);
+  const int rem = (int)(i - strip * Ny * W);
+  const int ky = rem / W;
+  const int kx = (int)strip * W + (rem - ky * W);
+  if (kx >= Nx) return;
+  const XTerms xt = x_terms(hp, kx, Nx, MODE);
+  float2 h = eval_H(hp, xt, ky, Ny, KIND, MODE, remove_evanescent, bandlimit, scale);
+  if (na_r2 > 0.0f && !na_keep(hp, ky, Ny, na_fx2(hp, kx, Nx), na_r2)) h = make_float2(0.f, 0.f);
+  table[i] = h;
+}
+
+int launch_h_table(const ColArgs& a, int Ny, int W, float2* table, hipStream_t s) {
+  const long long n = (long long)((a.ncols + W - 1) / W) * W * Ny;
+  const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+  void* t0 = timing_start(s);
+  const HParam& hp = a.hp[0];
+#define CXM_HT(K, M)                                                                     \
+  hipLaunchKernelGGL((h_table_kernel<K, M>), grid, block, 0, s, hp, Ny, a.ncols, W,      \
+                     a.remove_evanescent, a.bandlimit, a.scale, a.na_r2, table)
+  if (a.kind == CXM_H_TRANSFER) {
+    if (a.phase_mode == CXM_PHASE_REF32) CXM_HT(CXM_H_TRANSFER, CXM_PHASE_REF32);
+    else CXM_HT(CXM_H_TRANSFER, CXM_PHASE_ACCURATE);
+  } else {
+    if (a.phase_mode == CXM_PHASE_REF32) CXM_HT(CXM_H_ASM, CXM_PHASE_REF32);
+    else CXM_HT(CXM_H_ASM, CXM_PHASE_ACCURATE);
+  }
+#undef CXM_HT
+  CXM_LAUNCH_CHECK();
+  if (t0) timing_stop(t0, s, "h_table_" + std::to_string(Ny));
+  return CXM_OK;
+}
+
 }  // namespace cxm
 
 // ---------------------------------------------------------------------------
@@ -73,10 +154,20 @@ int launch_col_prop(int N, const ColArgs& a, hipStream_t s) {
 using namespace cxm;
 
 namespace {
+// Elements between the strips of a tiled plane with R rows: the strip plus
+// one 128-byte line.  Without the line, strips of 1024 rows lie 32 KiB
+// apart, and the 4-row groups a row-pass workgroup moves (one line in each
+// strip) all fall on the same HBM channels: measured 1.25-1.4x slower
+// passes.
+long long tile_strip_stride(int R) { return (long long)R * kTileW + 16; }
+
 struct Geom {
   int Ny, Nx, Hin, Win, Hout, Wout;
-  long long plane_in, plane_ws1, plane_ws2;  // elements
+  int Rws;                                   // rows per workspace plane: max(Hin, Hout)
+  long long plane_in, plane_ws;              // elements
   int chunk;                                 // input planes per chunk
+  int W;                                     // column-pass strip width
+  bool tiled;                                // workspace planes in strips of kTileW columns
 };
 
 int validate(const cxm_prop2d_desc* d, Geom& g) {
@@ -113,33 +204,58 @@ int validate(const cxm_prop2d_desc* d, Geom& g) {
     return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT grid " + std::to_string(g.Ny) + "x" +
                                          std::to_string(g.Nx) + " (lengths 2^a <= 8192, 5*2^a <= 5120, 3*2^a <= 3072)");
   g.plane_in = (long long)g.Hin * g.Win;
-  g.plane_ws1 = (long long)g.Hin * g.Nx;
-  g.plane_ws2 = (long long)g.Hout * g.Nx;
+  // Every workspace plane (column-pass input and output alike) has
+  // max(Hin, Hout) rows, so that with one distance the column pass works in
+  // place: a strip's output rows lie where its input rows were.
+  g.Rws = std::max(g.Hin, g.Hout);
+  g.plane_ws = (long long)g.Rws * g.Nx;
   // Chunk so that the workspace of one chunk stays <= 128 MiB, half the 256 MiB
   // Infinity Cache: the intermediate planes are re-read from it, not from HBM.
   const int nout = d->z_zip ? 1 : d->num_z;
-  const long long per_plane = (nout == 1) ? std::max(g.plane_ws1, g.plane_ws2)
-                                          : g.plane_ws1 + (long long)nout * g.plane_ws2;
+  const long long per_plane = (nout == 1 ? 1 : 1 + nout) * g.plane_ws;
   const long long budget = (128ll << 20) / 8;
   long long ch = std::max(1ll, budget / std::max(1ll, per_plane));
   if (option(CXM_OPT_CHUNK_PLANES) > 0) ch = option(CXM_OPT_CHUNK_PLANES);
   g.chunk = (int)std::min<long long>(ch, std::max(1, d->batch));
+  g.W = col_width(g.Ny, g.chunk);
+  // Tiled intermediates: the row pass moves 4 rows per workgroup, the column
+  // pass reads strips of 4 columns, and planes are whole 4-row groups.
+  g.tiled = option(CXM_OPT_TILES) != 1 && row_group(g.Nx) == kTileW && g.W == kTileW &&
+            g.Hin % kTileW == 0 && g.Hout % kTileW == 0 && g.Nx % kTileW == 0;
+  if (g.tiled) g.plane_ws = (long long)(g.Nx / kTileW) * tile_strip_stride(g.Rws);
   return CXM_OK;
 }
 
 int out_per_plane(const cxm_prop2d_desc* d) { return d->z_zip ? 1 : d->num_z; }
 
-size_t ws_elems(const cxm_prop2d_desc* d, const Geom& g) {
+// One H table for the batch: analytic kind, a single parameter set, and
+// enough planes to amortise the table pass (CXM_OPT_H_TABLE overrides).
+bool use_h_table(const cxm_prop2d_desc* d) {
+  if (d->kind != CXM_H_TRANSFER && d->kind != CXM_H_ASM) return false;
+  if ((d->num_wv > 0 ? d->num_wv : 1) * d->num_z != 1) return false;
+  const int o = option(CXM_OPT_H_TABLE);
+  if (o == 1) return false;
+  return o == 2 || d->batch >= CXM_H_TABLE_MIN_PLANES;
+}
+
+size_t chunk_elems(const cxm_prop2d_desc* d, const Geom& g) {
   const int nout = out_per_plane(d);
-  if (nout == 1) return (size_t)g.chunk * std::max(g.plane_ws1, g.plane_ws2);
-  return (size_t)g.chunk * (g.plane_ws1 + (size_t)nout * g.plane_ws2);
+  return (size_t)g.chunk * (nout == 1 ? 1 : 1 + nout) * (size_t)g.plane_ws;
+}
+
+size_t table_elems(const Geom& g) {  // whole strips of W columns
+  return (size_t)((g.Nx + g.W - 1) / g.W) * g.W * g.Ny;
+}
+
+size_t ws_elems(const cxm_prop2d_desc* d, const Geom& g, bool table) {
+  return chunk_elems(d, g) + (table ? table_elems(g) : 0);
 }
 }  // namespace
 
 extern "C" size_t cxm_prop2d_workspace_size(const cxm_prop2d_desc* d) {
   Geom g;
   if (validate(d, g) != CXM_OK) return 0;
-  return ws_elems(d, g) * sizeof(float2);
+  return ws_elems(d, g, use_h_table(d)) * sizeof(float2);
 }
 
 namespace {
@@ -150,7 +266,8 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
   if (rc) return rc;
   if (d->batch == 0) return CXM_OK;
   if (!in || !out) return fail(CXM_ERR_INVALID, "null buffer");
-  const size_t need = ws_elems(d, g) * sizeof(float2);
+  const bool table = !hp_dev && use_h_table(d);
+  const size_t need = ws_elems(d, g, table) * sizeof(float2);
   if (!workspace || workspace_bytes < need)
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   if ((rc = ensure_init())) return rc;
@@ -169,7 +286,8 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
   }
   float2* ws1 = (float2*)workspace;
   const int nz = out_per_plane(d);
-  float2* ws2 = (nz == 1) ? ws1 : ws1 + (size_t)g.chunk * g.plane_ws1;
+  float2* ws2 = (nz == 1) ? ws1 : ws1 + (size_t)g.chunk * g.plane_ws;
+  const long long tstride = tile_strip_stride(g.Rws);  // elements between tiled strips
   const float2* src = (const float2*)in;
   const long long out_plane = (long long)g.Hout * g.Wout;
 
@@ -182,7 +300,9 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
     r1.out_c = ws1;
     r1.tw = twiddles_for(g.Nx);
     r1.in_plane_stride = g.plane_in;
-    r1.out_plane_stride = g.plane_ws1;
+    r1.out_plane_stride = g.plane_ws;
+    r1.out_tiled = g.tiled;
+    r1.out_strip_stride = tstride;
     r1.in_row_stride = g.Win;
     r1.out_row_stride = g.Nx;
     r1.n_rows = kc * g.Hin;
@@ -198,9 +318,11 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
     ca.out = ws2;
     ca.tw = twiddles_for(g.Ny);
     ca.H = (const float2*)d->propagator;
-    ca.in_plane_stride = g.plane_ws1;
-    ca.out_plane_stride = g.plane_ws2;
-    ca.row_stride = g.Nx;
+    ca.in_plane_stride = g.plane_ws;
+    ca.out_plane_stride = g.plane_ws;
+    ca.row_stride = g.tiled ? kTileW : g.Nx;
+    ca.strip_stride = g.tiled ? tstride : 0;
+    if (col_width(g.Ny, kc) != g.W) return fail(CXM_ERR_INVALID, "internal: column strip width changed");
     ca.ncols = g.Nx;
     ca.n_planes = kc;
     ca.plane0 = p0;
@@ -216,13 +338,27 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
     ca.remove_evanescent = d->remove_evanescent;
     ca.bandlimit = d->bandlimit;
     ca.scale = (float)(1.0 / ((double)g.Ny * (double)g.Nx));
+    if (table) {
+      // the scale is part of the tabulated H; the column pass multiplies by
+      // the table and by 1 (exact)
+      float2* htab = ws1 + chunk_elems(d, g);
+      if (p0 == 0 && (rc = launch_h_table(ca, g.Ny, g.W, htab, s))) return rc;
+      ca.kind = CXM_H_ARRAY;
+      ca.H = htab;
+      ca.h_strip_stride = (long long)g.Ny * g.W;
+      ca.h_row_stride = g.W;
+      ca.scale = 1.0f;
+    }
+    ca.strip_major = option(CXM_OPT_COL_ORDER) == 1;
     if ((rc = launch_col_prop(g.Ny, ca, s))) return rc;
     // 3) rows: inverse FFT, crop, complex or intensity
     RowArgs r2;
     memset(&r2, 0, sizeof(r2));
     r2.in = ws2;
     r2.tw = twiddles_for(g.Nx);
-    r2.in_plane_stride = g.plane_ws2;
+    r2.in_plane_stride = g.plane_ws;
+    r2.in_tiled = g.tiled;
+    r2.in_strip_stride = tstride;
     r2.in_row_stride = g.Nx;
     r2.out_plane_stride = out_plane;
     r2.out_row_stride = g.Wout;

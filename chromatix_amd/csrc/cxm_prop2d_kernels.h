@@ -26,29 +26,104 @@ struct RowCfg {
   static constexpr int T = N / E;
   static constexpr int G = (T >= 256) ? 1 : 256 / T;
   static constexpr int THREADS = G * T;
+  // LDS row pitch: the padded transform plus 8 slots, so that the four rows
+  // of a tiled exchange fall on different banks
+  static constexpr int PITCH = lds_span<N, 1>() + 8;
 };
+
+// Tiled planes (kTileW, cxm_internal.h): the column pass reads and writes
+// its 4-column strips as whole 128-byte lines (a row-major plane gives it
+// 32-byte pieces of lines, at four times the L2 requests), and a row pass
+// moving 4 rows per workgroup exchanges whole lines through LDS.
+// Lines of the 4 rows [r0, r0+4) of a tiled plane <-> the block's LDS rows.
+// 16-byte pieces: piece c of the block = line c / 8 (strip), 8-byte pair
+// (c % 8) -> row (c % 8) / 2, columns 4 * strip + 2 * (c % 2) + {0, 1}.
+template <int N, int THREADS, int PITCH>
+CXM_DEV void tiles_to_lds(const float2* base, long long strip_stride, float2* lds) {
+  constexpr int PIECES = N / kTileW * 8;
+  static_assert(PIECES % THREADS == 0, "tile exchange shape");
+#pragma unroll
+  for (int i = 0; i < PIECES / THREADS; ++i) {
+    const int c = i * THREADS + threadIdx.x;
+    const int q = c & 7, x = (c >> 3) * kTileW + 2 * (q & 1);
+    const float4 w = *(const float4*)(base + (c >> 3) * strip_stride + 2 * q);
+    float2* d = lds + (q >> 1) * PITCH + lds_at<N, 1>(x);  // x, x+1 share a 16-slot group
+    d[0] = make_float2(w.x, w.y);
+    d[1] = make_float2(w.z, w.w);
+  }
+}
+template <int N, int THREADS, int PITCH>
+CXM_DEV void lds_to_tiles(const float2* lds, float2* base, long long strip_stride) {
+  constexpr int PIECES = N / kTileW * 8;
+  static_assert(PIECES % THREADS == 0, "tile exchange shape");
+#pragma unroll
+  for (int i = 0; i < PIECES / THREADS; ++i) {
+    const int c = i * THREADS + threadIdx.x;
+    const int q = c & 7, x = (c >> 3) * kTileW + 2 * (q & 1);
+    const float2* d = lds + (q >> 1) * PITCH + lds_at<N, 1>(x);
+    *(float4*)(base + (c >> 3) * strip_stride + 2 * q) = make_float4(d[0].x, d[0].y, d[1].x, d[1].y);
+  }
+}
 
 template <int N, int E, int DIR>
 __global__ void __launch_bounds__((RowCfg<N, E>::THREADS))
 row_fft_kernel(RowArgs a) {
   using Cfg = RowCfg<N, E>;
   constexpr int T = Cfg::T;
-  constexpr int SPAN = lds_span<N, 1>();
-  __shared__ float2 lds[Cfg::G * SPAN];
+  constexpr int G = Cfg::G;
+  constexpr int PITCH = Cfg::PITCH;
+  // transforms of at most one wave exchange through LDS without a block barrier
+  constexpr bool SYNC = T > 64;
+  __shared__ float2 lds[G * PITCH];
   const int g = threadIdx.x / T;
   const int t = threadIdx.x - g * T;
-  const int row = blockIdx.x * Cfg::G + g;
+  const int row = blockIdx.x * G + g;
   const bool ok = row < a.n_rows;
   const int p = ok ? row / a.rows_per_plane : 0;
   const int r = ok ? row - p * a.rows_per_plane : 0;
-  const float2* src = a.in + p * a.in_plane_stride + (long long)r * a.in_row_stride;
+  float2* st = lds + g * PITCH;
   float2 v[E];
+  bool loaded = false;
+  if constexpr (G == kTileW) {
+    if (a.in_tiled) {  // block-uniform: n_rows and rows_per_plane are multiples of 4
+      const int row0 = blockIdx.x * G;
+      if (row0 < a.n_rows) {
+        const int p0 = row0 / a.rows_per_plane;
+        tiles_to_lds<N, Cfg::THREADS, PITCH>(
+            a.in + p0 * a.in_plane_stride + (long long)(row0 - p0 * a.rows_per_plane) * kTileW,
+            a.in_strip_stride, lds);
+      }
+      __syncthreads();
 #pragma unroll
-  for (int k = 0; k < E; ++k) {
-    const int xs = t + k * T - a.in_off;
-    v[k] = (ok && xs >= 0 && xs < a.in_len) ? src[xs] : make_float2(0.f, 0.f);
+      for (int k = 0; k < E; ++k) v[k] = st[lds_at_split<N, 1>(t, k * T)];
+      loaded = true;
+    }
   }
-  fft<N, E, DIR, 1, true>(v, lds + g * SPAN, t, a.tw);
+  if (!loaded) {
+    const float2* src = a.in + p * a.in_plane_stride + (long long)r * a.in_row_stride;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int xs = t + k * T - a.in_off;
+      v[k] = (ok && xs >= 0 && xs < a.in_len) ? src[xs] : make_float2(0.f, 0.f);
+    }
+  }
+  fft<N, E, DIR, 1, SYNC>(v, st, t, a.tw);
+  if constexpr (G == kTileW) {
+    if (a.out_tiled) {
+      // each wave rewrites only its own row, which no other wave reads
+#pragma unroll
+      for (int k = 0; k < E; ++k) st[lds_at_split<N, 1>(t, k * T)] = cscale(v[k], a.scale);
+      __syncthreads();
+      const int row0 = blockIdx.x * G;
+      if (row0 < a.n_rows) {
+        const int p0 = row0 / a.rows_per_plane;
+        lds_to_tiles<N, Cfg::THREADS, PITCH>(
+            lds, a.out_c + p0 * a.out_plane_stride + (long long)(row0 - p0 * a.rows_per_plane) * kTileW,
+            a.out_strip_stride);
+      }
+      return;
+    }
+  }
   if (!ok) return;
   const long long obase = p * a.out_plane_stride + (long long)r * a.out_row_stride;
 #pragma unroll
@@ -83,17 +158,22 @@ template <int N, int E, int C, int KIND, int MODE, bool MULTI>
 __global__ void __launch_bounds__((ColCfg<N, E, C>::THREADS))
 col_prop_kernel(ColArgs a) {
   constexpr int T = N / E;
-  __shared__ float2 lds[N * C];
+  __shared__ float2 lds[lds_span<N, C>()];
   const int c = threadIdx.x % C;
   const int ty = threadIdx.x / C;
   const int nstrips = (a.ncols + C - 1) / C;
   const int L = xcd_remap(blockIdx.x, gridDim.x);
-  const int p = L / nstrips;
-  const int col0 = (L - p * nstrips) * C;
+  // plane-major: each XCD works through whole planes; strip-major: each XCD
+  // takes a range of strips of every plane (measured slower for 1024^2
+  // batches, with H on the fly or from a table)
+  const int p = a.strip_major ? L % a.n_planes : L / nstrips;
+  const int col0 = (a.strip_major ? L / a.n_planes : L - p * nstrips) * C;
   const int col = col0 + c;
   const bool colok = col < a.ncols;
   // workgroup-uniform strip base + 32-bit lane offsets (SGPR-base addressing)
-  const float2* src = a.in + p * a.in_plane_stride + col0;
+  const long long sstride = a.strip_stride ? a.strip_stride : C;
+  const long long soff = (long long)(col0 / C) * sstride;
+  const float2* src = a.in + p * a.in_plane_stride + soff;
   const int pg = a.plane0 + p;
   const int w = (a.nwv > 1) ? (pg % a.nwv) : 0;
   const int kx = colok ? col : 0;
@@ -116,32 +196,25 @@ col_prop_kernel(ColArgs a) {
       const HParam* hpp = a.hp_dev ? a.hp_dev + ((long long)pg * nout + iz) : &a.hp[w * a.nz + zi];
       const HParam hp = *hpp;
       const XTerms xt = x_terms(hp, kx, a.ncols, MODE);
-      // NA mask |f|^2 <= (NA/lam)^2 on the unshifted frequency grid,
-      // f = fl(fl(1/d) * fl(k/N)) as the reference's f_grid
       const bool na = a.na_r2 > 0.0f;
-      float fx2 = 0.0f;
-      if (na) {
-        const float fx = __fmul_rn(hp.inv_dx, __fdiv_rn((float)signed_bin(kx, a.ncols), (float)a.ncols));
-        fx2 = __fmul_rn(fx, fx);
-      }
+      const float fx2 = na ? na_fx2(hp, kx, a.ncols) : 0.0f;
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         float2 h = eval_H(hp, xt, ty + k * T, N, KIND, MODE, a.remove_evanescent,
                           a.bandlimit, a.scale);
-        if (na) {
-          const float fy = __fmul_rn(hp.inv_dy, __fdiv_rn((float)signed_bin(ty + k * T, N), (float)N));
-          if (__fadd_rn(__fmul_rn(fy, fy), fx2) > a.na_r2) h = make_float2(0.f, 0.f);
-        }
+        if (na && !na_keep(hp, ty + k * T, N, fx2, a.na_r2)) h = make_float2(0.f, 0.f);
         u[k] = cmul(u[k], h);
         // one H evaluation per scheduling region: the unrolled evaluations
         // are not interleaved into a register-spilling schedule
         __builtin_amdgcn_sched_barrier(0);
       }
     } else if constexpr (KIND == CXM_H_ARRAY) {
-      const float2* hcol = a.H + kx;
+      const long long hss = a.h_strip_stride ? a.h_strip_stride : C;
+      const int hrs = a.h_row_stride ? a.h_row_stride : a.ncols;
+      const float2* hcol = a.H + (long long)(col0 / C) * hss + c;
 #pragma unroll
       for (int k = 0; k < E; ++k) {
-        const float2 h = colok ? hcol[(unsigned)((ty + k * T) * a.ncols)] : make_float2(0.f, 0.f);
+        const float2 h = colok ? hcol[(unsigned)((ty + k * T) * hrs)] : make_float2(0.f, 0.f);
         u[k] = cscale(cmul(u[k], h), a.scale);
       }
     } else {
@@ -150,7 +223,7 @@ col_prop_kernel(ColArgs a) {
     }
     fft<N, E, 1, C, true>(u, lds + c, ty, a.tw);
     if (colok) {
-      float2* dst = a.out + ((long long)p * nout + iz) * a.out_plane_stride + col0;
+      float2* dst = a.out + ((long long)p * nout + iz) * a.out_plane_stride + soff;
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const int yo = ty + k * T - a.out_off;

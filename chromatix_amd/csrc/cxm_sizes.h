@@ -3,7 +3,8 @@
 //
 // X(N, E_row, E_col, C_col): FFT length, values per thread in row and column
 // transforms, columns per column-pass workgroup.  Column workgroups hold an
-// N x C_col strip of complex64 in LDS (<= 128 KiB of the 160 KiB per CU).
+// N x C_col strip of complex64 in LDS, plus one pad row every 16 rows for
+// power-of-two N (<= 136 KiB of the 160 KiB per CU).
 #pragma once
 
 #define CXM_SIZES_G0(X)                                                       \

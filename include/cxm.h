@@ -479,11 +479,29 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *   CXM_OPT_ROW_VARIANT   alternative row-pass shape for 1024-point rows
  *   CXM_OPT_PIPE_ROW_VARIANT  alternative shape of the 1024-point rows of the
  *                         operator pipelines (multislice, optics, SAS, CZT)
+ *   CXM_OPT_H_TABLE       cxm_prop2d_c64 with one parameter set (one
+ *                         distance and wavelength) for the whole batch
+ *                         evaluates H once into the workspace and the column
+ *                         pass reads it: 0 = from CXM_H_TABLE_MIN_PLANES
+ *                         planes on, 1 = never, 2 = always.  The table holds
+ *                         exactly the values the column pass would evaluate,
+ *                         so the results are bit-identical either way.
+ *   CXM_OPT_COL_ORDER     block order of the column pass of cxm_prop2d_c64:
+ *                         0 = plane by plane, 1 = strip by strip over all
+ *                         planes of a chunk
+ *   CXM_OPT_TILES         cxm_prop2d_c64 keeps its intermediate planes in
+ *                         strips of 4 columns (each strip's rows contiguous)
+ *                         where the pass shapes allow it: 0 = yes, 1 = never
+ *                         (row-major intermediates).  Same results either way.
  */
 #define CXM_OPT_COL_VARIANT 1
 #define CXM_OPT_CHUNK_PLANES 2
 #define CXM_OPT_ROW_VARIANT 3
 #define CXM_OPT_PIPE_ROW_VARIANT 4
+#define CXM_OPT_H_TABLE 5
+#define CXM_OPT_COL_ORDER 6
+#define CXM_OPT_TILES 7
+#define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 
 /* Library state. */

@@ -231,6 +231,62 @@ def test_headline_batch_properties():
     assert err < TOL
 
 
+@pytest.mark.parametrize("phase_mode", ["ref32", "accurate"])
+@pytest.mark.parametrize("method,shape,pad,opts", [
+    ("asm", (100, 60), (14, 2), {}),
+    ("asm", (128, 96), 16, dict(bandlimit=True, shift_yx=(20.0, 10.0))),
+    ("asm", (64, 32), 0, {}),  # evanescent regime
+    ("transfer", (256, 256), 0, dict(kykx=(0.2, 0.1))),
+])
+def test_h_table_is_bit_identical(method, shape, pad, opts, phase_mode):
+    """A batch sharing one parameter set reads H from a table evaluated once
+    per call; the column pass evaluating H on the fly gives the same bits."""
+    u = _phase_field(shape, seed=11, batch=(L.CXM_H_TABLE_MIN_PLANES + 1,))
+    z, dx = (2.0, 0.1) if shape == (64, 32) else (40.0, 0.3)
+    lib = L.lib()
+    try:
+        outs = []
+        for opt in (1, 0, 2):  # never, automatic (batch >= min), always
+            L.check(lib.cxm_set_option(L.CXM_OPT_H_TABLE, opt))
+            uu = u if opt != 2 else u[:1]
+            outs.append(_run(uu, dx, z, pad, method, "same", phase_mode, **opts))
+    finally:
+        L.check(lib.cxm_set_option(L.CXM_OPT_H_TABLE, 0))
+    fly, tab, tab1 = outs
+    assert np.array_equal(fly.view(np.uint32), tab.view(np.uint32))
+    assert np.array_equal(fly[:1].view(np.uint32), tab1.view(np.uint32))
+    ref = _oracle(u[2], dx, z, pad, method, "same", "ref32" if phase_mode == "ref32" else "fp64", **opts)
+    assert rel_l2(tab[2], ref) < TOL
+
+
+@pytest.mark.parametrize("shape,pad,mode,batch", [
+    ((1024, 1024), 0, "same", 3),
+    ((512, 512), 256, "full", 5),   # 512 input rows -> 1024 output rows, in place
+    ((1000, 1000), 12, "same", 2),
+    ((1022, 1022), 1, "same", 2),   # rows not a multiple of 4: row-major intermediates
+    ((1536, 1536), 0, "same", 2),
+    ((1024, 1536), 0, "same", 5),
+])
+def test_tiled_intermediates_are_bit_identical(shape, pad, mode, batch):
+    """Intermediate planes in strips of 4 columns (where the pass shapes allow
+    it) give the same bits as row-major intermediates, batch planes included
+    (their workspace planes must not overlap when input and output rows
+    differ)."""
+    u = _phase_field(shape, seed=12, batch=(batch,))
+    lib = L.lib()
+    try:
+        outs = []
+        for opt in (1, 0):  # row-major, tiled where possible
+            L.check(lib.cxm_set_option(L.CXM_OPT_TILES, opt))
+            outs.append(_run(u, 0.3, 60.0, pad, "asm", mode, "ref32"))
+    finally:
+        L.check(lib.cxm_set_option(L.CXM_OPT_TILES, 0))
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    for b in (0, batch - 1):
+        ref = _oracle(u[b], 0.3, 60.0, pad, "asm", mode, "ref32")
+        assert rel_l2(outs[1][b], ref) < TOL
+
+
 def test_errors_are_raised():
     f = _mono(_phase_field((100, 100)), 0.3)
     with pytest.raises(L.CxmError):

@@ -62,8 +62,8 @@ def test_small_dfts(R, direction):
 
 def lds_at(e, S, N):
     """LDS slot of element e (mirror of lds_at in cxm_fft.h)."""
-    if S == 1:
-        return e + (e >> 4)
+    if S == 1 or N & (N - 1) == 0:
+        return (e + (e >> 4)) * S
     return e * S
 
 
@@ -71,7 +71,7 @@ def emulate_fft(x, N, E, direction, S=1):
     T = N // E
     L = N  # one table of length N per transform length
     tw = np.exp(-2j * np.pi * np.arange(L) / L)
-    span = N + N // 16 if S == 1 else N * S
+    span = (N + N // 16) * S if S == 1 or N & (N - 1) == 0 else N * S
     v = np.array([[x[t + k * T] for k in range(E)] for t in range(T)], dtype=complex)
     Ns = 1
     while Ns < N:
@@ -105,6 +105,8 @@ def emulate_fft(x, N, E, direction, S=1):
                         assert 0 <= e < N
                         addr = lds_at(e, S, N)
                         assert addr < span
+                        if N & (N - 1) == 0:  # the kernels split the slot as base + constant
+                            assert addr == lds_at(base, S, N) + lds_at(r * Ns, S, N)
                         lds[addr] = a[r]
                         written[addr] += 1
         if not last:
@@ -113,6 +115,8 @@ def emulate_fft(x, N, E, direction, S=1):
             for t in range(T):
                 for e in range(E):
                     newv[t, e] = lds[lds_at(t + e * T, S, N)]
+                    if N & (N - 1) == 0:
+                        assert lds_at(t + e * T, S, N) == lds_at(t, S, N) + lds_at(e * T, S, N)
             assert not np.isnan(newv).any()
         v = newv
         Ns *= R
@@ -134,7 +138,8 @@ def test_configured_schedules(cfg):
         T = N // E
         assert (T * C if S == C else max(T, 256 // max(1, T) * T)) <= 1024
         if S == C:
-            assert N * C * 8 <= 160 * 1024  # the strip fits the CU's LDS
+            span = (N + N // 16) * C if N & (N - 1) == 0 else N * C
+            assert span * 8 <= 160 * 1024  # the strip fits the CU's LDS
         fwd = emulate_fft(x, N, E, -1, S)
         np.testing.assert_allclose(fwd, np.fft.fft(x), atol=1e-9 * N)
         inv = emulate_fft(x, N, E, 1, S)
