@@ -20,6 +20,7 @@ constexpr int kMaxDevices = 64;
 std::mutex g_init_mu;
 bool g_ready[kMaxDevices] = {};
 float2* g_tw_ptr[kMaxDevices] = {};
+int g_cus[kMaxDevices] = {};
 }  // namespace
 
 __device__ __align__(256) float2 g_twiddles[kTwTotal];
@@ -63,9 +64,19 @@ static int init_device(int dev) {
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (prev != dev) (void)hipSetDevice(prev);
   if (e != hipSuccess) return fail(CXM_ERR_HIP, std::string("twiddle init: ") + hipGetErrorString(e));
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  g_cus[dev] = cus;
   g_tw_ptr[dev] = (float2*)p;
   g_ready[dev] = true;
   return CXM_OK;
+}
+
+int device_cus() {
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return (dev >= 0 && dev < kMaxDevices && g_cus[dev] > 0) ? g_cus[dev] : 256;
 }
 
 int ensure_init() {
@@ -176,6 +187,7 @@ extern "C" int cxm_timing_get(int i, char* name, int name_len, double* total_ms,
 namespace cxm {
 namespace { int g_opts[16] = {0}; }
 int option(int key) { return (key >= 0 && key < 16) ? g_opts[key] : 0; }
+
 }  // namespace cxm
 
 extern "C" int cxm_set_option(int key, int value) {

@@ -254,13 +254,10 @@ CXM_DEV void load_stage_tw(StageTw<N, E, Ns>& st, int t, const float2* __restric
   }
 }
 
-template <int N, int E, int DIR, int Ns, int S, bool SYNC>
-CXM_DEV void stockham_from(float2 (&v)[E], float2* lds, int t, const float2* __restrict__ tw,
-                           const StageTw<N, E, Ns>& w);
-
-template <int N, int E, int DIR, int Ns, int R, int S, bool SYNC>
-CXM_DEV void stockham_stage(float2 (&v)[E], float2* lds, int t, const float2* __restrict__ tw,
-                            const StageTw<N, E, Ns>& w) {
+// Butterflies of one stage (twiddles applied), results to LDS unless it is
+// the last stage, which leaves them in registers in the invariant layout.
+template <int N, int E, int DIR, int Ns, int R, int S>
+CXM_DEV void stage_body(float2 (&v)[E], float2* lds, int t, const StageTw<N, E, Ns>& w) {
   constexpr int T = N / E;
   constexpr int M = E / R;
   constexpr bool last = (Ns * R == N);
@@ -292,17 +289,18 @@ CXM_DEV void stockham_stage(float2 (&v)[E], float2* lds, int t, const float2* __
       for (int r = 0; r < R; ++r) lds[lds_at_split<N, S>(base, r * Ns)] = a[r];
     }
   }
-  if constexpr (!last) {
-    StageTw<N, E, Ns * R> wn;
-    load_stage_tw<N, E, DIR, Ns * R>(wn, t, tw);
-    if constexpr (SYNC) __syncthreads();
-    else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+}
+
+// Sync, read back in the invariant layout, sync.
+template <int N, int E, int S, bool SYNC>
+CXM_DEV void stage_exchange(float2 (&v)[E], float2* lds, int t) {
+  constexpr int T = N / E;
+  if constexpr (SYNC) __syncthreads();
+  else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = lds[lds_at_split<N, S>(t, e * T)];
-    if constexpr (SYNC) __syncthreads();
-    else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
-    stockham_from<N, E, DIR, Ns * R, S, SYNC>(v, lds, t, tw, wn);
-  }
+  for (int e = 0; e < E; ++e) v[e] = lds[lds_at_split<N, S>(t, e * T)];
+  if constexpr (SYNC) __syncthreads();
+  else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
 }
 
 template <int N, int E, int DIR, int Ns, int S, bool SYNC>
@@ -311,8 +309,55 @@ CXM_DEV void stockham_from(float2 (&v)[E], float2* lds, int t, const float2* __r
   if constexpr (Ns < N) {
     constexpr int R = pick_radix(N / Ns, E);
     static_assert(R > 0, "FFT length / thread split has no radix schedule");
-    stockham_stage<N, E, DIR, Ns, R, S, SYNC>(v, lds, t, tw, w);
+    stage_body<N, E, DIR, Ns, R, S>(v, lds, t, w);
+    if constexpr (Ns * R < N) {
+      StageTw<N, E, Ns * R> wn;
+      load_stage_tw<N, E, DIR, Ns * R>(wn, t, tw);
+      stage_exchange<N, E, S, SYNC>(v, lds, t);
+      stockham_from<N, E, DIR, Ns * R, S, SYNC>(v, lds, t, tw, wn);
+    }
   }
+}
+
+// The twiddles of every stage from Ns on, held in registers (for loops that
+// transform many rows with one thread layout).
+template <int N, int E, int Ns, bool END = (Ns >= N)>
+struct AllTw;
+template <int N, int E, int Ns>
+struct AllTw<N, E, Ns, true> {};
+template <int N, int E, int Ns>
+struct AllTw<N, E, Ns, false> {
+  static constexpr int R = pick_radix(N / Ns, E);
+  StageTw<N, E, Ns> here;
+  AllTw<N, E, Ns * R> rest;
+};
+
+template <int N, int E, int DIR, int Ns>
+CXM_DEV void load_all_tw(AllTw<N, E, Ns>& a, int t, const float2* __restrict__ tw) {
+  if constexpr (Ns < N) {
+    load_stage_tw<N, E, DIR, Ns>(a.here, t, tw);
+    load_all_tw<N, E, DIR, Ns * AllTw<N, E, Ns>::R>(a.rest, t, tw);
+  }
+}
+
+template <int N, int E, int DIR, int Ns, int S, bool SYNC>
+CXM_DEV void stockham_pre(float2 (&v)[E], float2* lds, int t, const AllTw<N, E, Ns>& all) {
+  if constexpr (Ns < N) {
+    constexpr int R = AllTw<N, E, Ns>::R;
+    stage_body<N, E, DIR, Ns, R, S>(v, lds, t, all.here);
+    if constexpr (Ns * R < N) {
+      stage_exchange<N, E, S, SYNC>(v, lds, t);
+      stockham_pre<N, E, DIR, Ns * R, S, SYNC>(v, lds, t, all.rest);
+    }
+  }
+}
+
+// Full transform with twiddles loaded beforehand by load_all_tw<N, E, DIR, 1>.
+template <int N, int E, int DIR, int S, bool SYNC>
+CXM_DEV void fft_pre(float2 (&v)[E], float2* lds, int t, const AllTw<N, E, 1>& all) {
+  static_assert(N % E == 0, "E must divide N");
+  static_assert(schedule_ok(N, E), "no radix schedule for (N, E)");
+  stockham_pre<N, E, DIR, 1, S, SYNC>(v, lds, t, all);
 }
 
 // Full transform, unnormalised.  DIR = -1: forward (exp(-2 pi i nk/N)),

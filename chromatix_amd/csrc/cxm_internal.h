@@ -42,6 +42,7 @@ void timing_stop(void* start, hipStream_t s, const std::string& name);
 // (cxm_init must have run on it).
 const float2* twiddles_for(int n);
 int ensure_init();
+int device_cus();  // compute units of the current device (cxm_init must have run on it)
 
 constexpr int kMaxHP = 32;  // parameter sets carried by value per launch
 

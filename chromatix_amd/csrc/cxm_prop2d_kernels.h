@@ -12,6 +12,7 @@
 // store of the requested rows.  All rows of the strip are read before any is
 // written, so in == out is allowed when nz == 1.
 #pragma once
+#include <algorithm>
 #include <string>
 
 #include "cxm_fft.h"
@@ -137,6 +138,124 @@ row_fft_kernel(RowArgs a) {
   }
 }
 
+// Row pass as a loop over groups of 4 rows (one row per wave) for rows of
+// one wave (T == 64): the grid holds the workgroups the GPU keeps resident,
+// and each workgroup loads its next group while it transforms the current
+// one.  The twiddles of all stages are loaded once, before the first group:
+// vector-memory counters complete in order, so twiddle loads issued after a
+// prefetch would make the transform wait for the prefetched group.  Same
+// results as row_fft_kernel.
+template <int N, int E, int DIR>
+__global__ void __launch_bounds__(256)
+row_fft_loop_kernel(RowArgs a) {
+  using Cfg = RowCfg<N, E>;
+  static_assert(Cfg::G == 4 && Cfg::T == 64, "one row per wave, four rows per workgroup");
+  static_assert(N / kTileW * 8 == 128 * E, "tile pieces per thread = E / 2");
+  constexpr int T = 64, G = 4, PITCH = Cfg::PITCH;
+  __shared__ float2 lds[G * PITCH];
+  const int ngroups = (a.n_rows + G - 1) / G;
+  AllTw<N, E, 1> twr;
+  load_all_tw<N, E, DIR, 1>(twr, threadIdx.x & 63, a.tw);
+  // next group's input: E values of this thread's row, or E/2 tile pieces
+  float4 pf[E / 2];
+  // Lane-dependent addresses are the same for every group; an opaque copy
+  // of the thread index per group has them recomputed rather than held in
+  // registers across the loop.
+  auto fetch = [&](int grp, int tid) {
+    const int g = tid >> 6, t = tid & 63;
+    const int row0 = grp * G;
+    if (a.in_tiled) {
+      const int p0 = row0 / a.rows_per_plane;
+      const float2* base =
+          a.in + p0 * a.in_plane_stride + (long long)(row0 - p0 * a.rows_per_plane) * kTileW;
+#pragma unroll
+      for (int i = 0; i < E / 2; ++i) {
+        const int c = i * 256 + tid;
+        pf[i] = *(const float4*)(base + (c >> 3) * a.in_strip_stride + 2 * (c & 7));
+      }
+    } else {
+      const int row = row0 + g;
+      const bool ok = row < a.n_rows;
+      const int p = ok ? row / a.rows_per_plane : 0;
+      const int r = ok ? row - p * a.rows_per_plane : 0;
+      const float2* src = a.in + p * a.in_plane_stride + (long long)r * a.in_row_stride;
+#pragma unroll
+      for (int k = 0; k < E; k += 2) {
+        const int x0 = t + k * T - a.in_off, x1 = x0 + T;
+        const float2 u0 = (ok && x0 >= 0 && x0 < a.in_len) ? src[x0] : make_float2(0.f, 0.f);
+        const float2 u1 = (ok && x1 >= 0 && x1 < a.in_len) ? src[x1] : make_float2(0.f, 0.f);
+        pf[k / 2] = make_float4(u0.x, u0.y, u1.x, u1.y);
+      }
+    }
+  };
+  int grp = blockIdx.x;
+  if (grp < ngroups) fetch(grp, threadIdx.x);
+  for (; grp < ngroups; grp += gridDim.x) {
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int g = tid >> 6, t = tid & 63;
+    float2* st = lds + g * PITCH;
+    float2 v[E];
+    if (a.in_tiled) {
+      __syncthreads();  // the previous group's LDS reads are done
+#pragma unroll
+      for (int i = 0; i < E / 2; ++i) {
+        const int c = i * 256 + tid;
+        const int q = c & 7, x = (c >> 3) * kTileW + 2 * (q & 1);
+        float2* d = lds + (q >> 1) * PITCH + lds_at<N, 1>(x);
+        d[0] = make_float2(pf[i].x, pf[i].y);
+        d[1] = make_float2(pf[i].z, pf[i].w);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < E; ++k) v[k] = st[lds_at_split<N, 1>(t, k * T)];
+    } else {
+#pragma unroll
+      for (int k = 0; k < E; k += 2) {
+        v[k] = make_float2(pf[k / 2].x, pf[k / 2].y);
+        v[k + 1] = make_float2(pf[k / 2].z, pf[k / 2].w);
+      }
+    }
+    const int nxt = grp + gridDim.x;
+    if (nxt < ngroups) fetch(nxt, tid);
+    fft_pre<N, E, DIR, 1, false>(v, st, t, twr);
+    const int row0 = grp * G;
+    if (a.out_tiled) {
+#pragma unroll
+      for (int k = 0; k < E; ++k) st[lds_at_split<N, 1>(t, k * T)] = cscale(v[k], a.scale);
+      __syncthreads();
+      const int p0 = row0 / a.rows_per_plane;
+      float2* base =
+          a.out_c + p0 * a.out_plane_stride + (long long)(row0 - p0 * a.rows_per_plane) * kTileW;
+#pragma unroll
+      for (int i = 0; i < E / 2; ++i) {
+        const int c = i * 256 + tid;
+        const int q = c & 7, x = (c >> 3) * kTileW + 2 * (q & 1);
+        const float2* d = lds + (q >> 1) * PITCH + lds_at<N, 1>(x);
+        *(float4*)(base + (c >> 3) * a.out_strip_stride + 2 * q) =
+            make_float4(d[0].x, d[0].y, d[1].x, d[1].y);
+      }
+      __syncthreads();  // before the next group's transforms reuse the rows
+    } else {
+      const int row = row0 + g;
+      if (row < a.n_rows) {
+        const int p = row / a.rows_per_plane;
+        const int r = row - p * a.rows_per_plane;
+        const long long obase = p * a.out_plane_stride + (long long)r * a.out_row_stride;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const int xo = t + k * T - a.out_off;
+          if (xo >= 0 && xo < a.out_len) {
+            const float2 w = cscale(v[k], a.scale);
+            if (a.intensity) a.out_i[obase + xo] = cabs2(w);
+            else a.out_c[obase + xo] = w;
+          }
+        }
+      }
+    }
+  }
+}
+
 template <int N, int E, int C>
 struct ColCfg {
   static constexpr int T = N / E;
@@ -241,6 +360,31 @@ int launch_row_t(int dir, const RowArgs& a, hipStream_t s) {
   const int blocks = (a.n_rows + Cfg::G - 1) / Cfg::G;
   if (blocks == 0) return CXM_OK;
   void* t0 = timing_start(s);
+  if constexpr (Cfg::G == 4 && Cfg::T == 64) {
+    // Looping pass for rows read from caller memory (HBM latency to hide);
+    // tiled workspace input comes back faster than the loop gains.
+    const int lo = option(CXM_OPT_ROW_LOOP);
+    if (lo > 1 || (lo == 0 && !a.in_tiled)) {
+      static int resident[2] = {0, 0};  // workgroups per CU, per direction
+      int& rb = resident[dir < 0 ? 0 : 1];
+      if (rb == 0) {
+        int n = 0;
+        if (dir < 0)
+          (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, row_fft_loop_kernel<N, E, -1>, 256, 0);
+        else
+          (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, row_fft_loop_kernel<N, E, 1>, 256, 0);
+        rb = n > 0 ? n : 1;
+      }
+      const int grid = std::min(blocks, (lo > 1 ? lo : rb) * device_cus());
+      if (dir < 0)
+        hipLaunchKernelGGL((row_fft_loop_kernel<N, E, -1>), dim3(grid), dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((row_fft_loop_kernel<N, E, 1>), dim3(grid), dim3(256), 0, s, a);
+      CXM_LAUNCH_CHECK();
+      if (t0) timing_stop(t0, s, std::string(dir < 0 ? "row_fft_fwd_" : "row_fft_inv_") + std::to_string(N));
+      return CXM_OK;
+    }
+  }
   if (dir < 0)
     hipLaunchKernelGGL((row_fft_kernel<N, E, -1>), dim3(blocks), dim3(Cfg::THREADS), 0, s, a);
   else

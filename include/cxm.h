@@ -493,6 +493,15 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         strips of 4 columns (each strip's rows contiguous)
  *                         where the pass shapes allow it: 0 = yes, 1 = never
  *                         (row-major intermediates).  Same results either way.
+ *   CXM_OPT_ROW_LOOP      rows of one wavefront (1024, 1280, 1536 points):
+ *                         0 = rows read from caller memory go through a pass
+ *                         whose workgroups loop over groups of 4 rows and
+ *                         load the next group during the current transform
+ *                         (as many workgroups as stay resident), tiled
+ *                         intermediates through one group per workgroup;
+ *                         1 = one group per workgroup everywhere; k > 1 =
+ *                         the looping pass everywhere with k workgroups per
+ *                         CU.  Same results.
  */
 #define CXM_OPT_COL_VARIANT 1
 #define CXM_OPT_CHUNK_PLANES 2
@@ -501,6 +510,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_H_TABLE 5
 #define CXM_OPT_COL_ORDER 6
 #define CXM_OPT_TILES 7
+#define CXM_OPT_ROW_LOOP 8
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

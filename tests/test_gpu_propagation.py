@@ -269,19 +269,28 @@ def test_h_table_is_bit_identical(method, shape, pad, opts, phase_mode):
 ])
 def test_tiled_intermediates_are_bit_identical(shape, pad, mode, batch):
     """Intermediate planes in strips of 4 columns (where the pass shapes allow
-    it) give the same bits as row-major intermediates, batch planes included
+    it) and the looping row pass give the same bits as row-major
+    intermediates and one row group per workgroup, batch planes included
     (their workspace planes must not overlap when input and output rows
-    differ)."""
+    differ); so does the fused intensity output."""
     u = _phase_field(shape, seed=12, batch=(batch,))
     lib = L.lib()
+    # (tiles, row loop): plain, default, looping everywhere (3 per CU)
+    combos = [(1, 1), (0, 0), (0, 3), (1, 3)]
     try:
-        outs = []
-        for opt in (1, 0):  # row-major, tiled where possible
-            L.check(lib.cxm_set_option(L.CXM_OPT_TILES, opt))
+        outs, ints = [], []
+        for tiles, loop in combos:
+            L.check(lib.cxm_set_option(L.CXM_OPT_TILES, tiles))
+            L.check(lib.cxm_set_option(L.CXM_OPT_ROW_LOOP, loop))
             outs.append(_run(u, 0.3, 60.0, pad, "asm", mode, "ref32"))
+            ints.append(cf.propagate_intensity(_mono(u, 0.3), 60.0, NM, pad, method="asm",
+                                               mode=mode).cpu().numpy())
     finally:
         L.check(lib.cxm_set_option(L.CXM_OPT_TILES, 0))
-    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+        L.check(lib.cxm_set_option(L.CXM_OPT_ROW_LOOP, 0))
+    for o, i in zip(outs[1:], ints[1:]):
+        assert np.array_equal(outs[0].view(np.uint32), o.view(np.uint32))
+        assert np.array_equal(ints[0].view(np.uint32), i.view(np.uint32))
     for b in (0, batch - 1):
         ref = _oracle(u[b], 0.3, 60.0, pad, "asm", mode, "ref32")
         assert rel_l2(outs[1][b], ref) < TOL
