@@ -115,9 +115,11 @@ def timing_name(rocprof_name: str) -> str | None:
     m = re.search(r"col_prop_kernel<(\d+),", rocprof_name)
     if m:
         return f"col_prop_{m.group(1)}"
-    m = re.search(r"row_fft_kernel<(\d+), \d+, (-?1)>", rocprof_name)
+    m = re.search(r"row_fft_(?:loop_)?kernel<(\d+), \d+, (-?1)>", rocprof_name)
     if m:
         return f"row_fft_{'fwd' if m.group(2) == '-1' else 'inv'}_{m.group(1)}"
+    if "h_table_kernel" in rocprof_name:
+        return "h_table"
     if "copy_kernel" in rocprof_name:
         return "copy_c64"
     return None
