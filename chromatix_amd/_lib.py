@@ -187,6 +187,19 @@ class CztDesc(ctypes.Structure):
     ]
 
 
+class YuDesc(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("axis", ctypes.c_int32),
+        ("alpha", ctypes.c_double),
+        ("df", ctypes.c_double),
+        ("out_dx", ctypes.c_double),
+        ("x0", ctypes.c_double),
+    ]
+
+
 class MultisliceDesc(ctypes.Structure):
     _fields_ = [
         ("batch", ctypes.c_int32),
@@ -262,6 +275,11 @@ def _declare(lib):
         "cxm_mbs_iterate": (i32, [vp, vp, vp, vp, ctypes.c_int32, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_output": (i32, [vp, vp, vp, vp, ctypes.POINTER(MbsDesc)]),
         "cxm_czt_axis_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(CztDesc)]),
+        "cxm_yu_axis_workspace_size": (sz, [ctypes.POINTER(YuDesc)]),
+        "cxm_yu_axis_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(YuDesc)]),
+        "cxm_high_na_defocus_c64": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "cxm_multislice_workspace_size": (sz, [ctypes.POINTER(MultisliceDesc)]),
         "cxm_multislice_checkpoint_size": (sz, [ctypes.POINTER(MultisliceDesc)]),
         "cxm_multislice_fwd": (i32, [vp, vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(MultisliceDesc)]),

@@ -12,6 +12,11 @@
 // Reference: czt (src/chromatix/utils/czt.py:7-52); the CZT branch of
 // kernel_propagate (src/chromatix/functional/propagation.py:395-443) calls it
 // per axis with its modulation as `post`.
+//
+// The same pass also evaluates one axis of kernel_propagate's scaled
+// convolution (Yu et al. 2012, propagation.py:444-458): pre-chirp, linear
+// convolution with a finite chirp kernel (FFT of length >= 2n - 1, window
+// [s, s + n) of the full result), post-chirp.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -204,6 +209,141 @@ extern "C" int cxm_czt_axis_c64(void* stream, const void* in, void* out, void* w
   a.in_line_stride = d->width;
   a.n_lines = d->width;
   a.out_plane_stride = (long long)m * d->width;
+  a.out_line_stride = d->width;
+  a.out_line_len = d->width;
+  return launch_pipe_col(L, a, s);
+}
+
+// ---------------------------------------------------------------------------
+// Scaled convolution (Yu et al.): [pre (n) | kernel (L) | post (n)] in FP64.
+// ---------------------------------------------------------------------------
+namespace cxm {
+namespace {
+struct YuParams {
+  double alpha, df, out_dx, x0;
+  int n, L;
+};
+
+__global__ void yu_chirps_kernel(float2* __restrict__ out, YuParams c) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = c.n, L = c.L, h = c.n / 2;
+  if (i >= 2LL * n + L) return;
+  constexpr double kPi = 3.14159265358979323846;
+  if (i < n) {  // (1/alpha) exp(i pi alpha f_k^2)
+    const double f = (double)((int)i - h) * c.df;
+    const double ph = kPi * c.alpha * f * f;
+    out[i] = make_float2((float)(cos(ph) / c.alpha), (float)(sin(ph) / c.alpha));
+  } else if (i < n + L) {  // exp(-i pi alpha f_t^2), t < n; 0 beyond
+    const int t = (int)(i - n);
+    if (t >= n) {
+      out[i] = make_float2(0.f, 0.f);
+    } else {
+      const double f = (double)(t - h) * c.df;
+      const double ph = -kPi * c.alpha * f * f;
+      out[i] = make_float2((float)cos(ph), (float)sin(ph));
+    }
+  } else {  // out_dx exp(i pi x_j^2 / alpha)
+    const int j = (int)(i - n - L);
+    const double x = c.x0 + (double)j * c.out_dx;
+    const double ph = kPi * x * x / c.alpha;
+    out[i] = make_float2((float)(c.out_dx * cos(ph)), (float)(c.out_dx * sin(ph)));
+  }
+}
+
+int yu_geom(const cxm_yu_desc* d, int& n, int& L) {
+  if (!d) return fail(CXM_ERR_INVALID, "null descriptor");
+  if (d->batch < 0 || d->height <= 0 || d->width <= 0)
+    return fail(CXM_ERR_INVALID, "invalid shape parameters");
+  if (d->axis != 0 && d->axis != 1) return fail(CXM_ERR_INVALID, "axis must be 0 or 1");
+  if (!(d->alpha > 0.0) || !(d->df > 0.0) || !(d->out_dx > 0.0))
+    return fail(CXM_ERR_INVALID, "alpha, df and out_dx must be positive");
+  n = d->axis == 0 ? d->height : d->width;
+  L = czt_len(n, n);
+  if (L < 0) return fail(CXM_ERR_UNSUPPORTED, "scaled convolution needs an FFT of length >= " + std::to_string(2 * n - 1));
+  return CXM_OK;
+}
+}  // namespace
+}  // namespace cxm
+
+extern "C" size_t cxm_yu_axis_workspace_size(const cxm_yu_desc* d) {
+  int n, L;
+  if (yu_geom(d, n, L)) return 0;
+  return ((size_t)2 * n + L) * sizeof(float2);
+}
+
+extern "C" int cxm_yu_axis_c64(void* stream, const void* in, void* out, void* workspace,
+                               size_t workspace_bytes, const cxm_yu_desc* d) {
+  int n, L;
+  int rc = yu_geom(d, n, L);
+  if (rc) return rc;
+  if (d->batch == 0) return CXM_OK;
+  if (!in || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  if (in == out) return fail(CXM_ERR_INVALID, "in and out must differ");
+  const size_t need = ((size_t)2 * n + L) * sizeof(float2);
+  if (!workspace || workspace_bytes < need)
+    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  YuParams yp;
+  yp.alpha = d->alpha; yp.df = d->df; yp.out_dx = d->out_dx; yp.x0 = d->x0;
+  yp.n = n; yp.L = L;
+  float2* wA = (float2*)workspace;
+  float2* wH = wA + n;
+  float2* wP = wH + L;
+  const long long tot = 2LL * n + L;
+  hipLaunchKernelGGL(yu_chirps_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, wA, yp);
+  CXM_LAUNCH_CHECK();
+  {  // kernel spectrum: one row of length L, forward FFT in place
+    PipeArgs f;
+    memset(&f, 0, sizeof(f));
+    f.in = wH;
+    f.out = wH;
+    f.tw = twiddles_for(L);
+    f.in_plane_stride = f.out_plane_stride = L;
+    f.in_line_stride = f.out_line_stride = L;
+    f.n_lines = 1;
+    f.n_planes = 1;
+    f.grid_ny = 1;
+    f.grid_nx = L;
+    f.in_len = f.out_len = L;
+    f.seq = SEQ_F;
+    f.scale = f.mid_scale = 1.0f;
+    if ((rc = launch_pipe_row(L, f, s))) return rc;
+  }
+  PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  Ops b(a);
+  const int ax = d->axis == 1 ? 1 : 0;
+  OpDesc* oA = b.add(a.pre, OP_VEC, 1, 0);
+  OpDesc* oH = b.add(a.mid, OP_VEC, 1, 0);
+  OpDesc* oP = b.add(a.post, OP_VEC, 1, 0);
+  if (!oA || !oH || !oP) return fail(CXM_ERR_INVALID, "operator table overflow");
+  const int sw = (n - 1) / 2;  // "same" window of the full convolution
+  oA->arr0 = wA; oA->arr_plane_stride = n; oA->arr_cols = n; oA->vec_axis = ax;
+  oH->arr0 = wH; oH->arr_plane_stride = L; oH->arr_cols = L; oH->vec_axis = ax;
+  oP->arr0 = wP; oP->arr_plane_stride = n; oP->arr_cols = n; oP->vec_axis = ax;
+  if (ax) oP->arr_col_off = sw; else oP->arr_row_off = sw;
+  a.tw = twiddles_for(L);
+  a.in = (const float2*)in;
+  a.out = out;
+  a.in_plane_stride = (long long)d->height * d->width;
+  a.out_plane_stride = (long long)d->height * d->width;
+  a.n_planes = d->batch;
+  a.grid_ny = ax ? d->height : L;
+  a.grid_nx = ax ? L : d->width;
+  a.in_len = n;
+  a.out_off = sw;
+  a.out_len = n;
+  a.seq = SEQ_FI;
+  a.scale = 1.0f;
+  a.mid_scale = (float)(1.0 / L);
+  a.in_line_stride = d->width;
+  if (ax) {
+    a.n_lines = d->height;
+    a.out_line_stride = d->width;
+    return launch_pipe_row(L, a, s);
+  }
+  a.n_lines = d->width;
   a.out_line_stride = d->width;
   a.out_line_len = d->width;
   return launch_pipe_col(L, a, s);

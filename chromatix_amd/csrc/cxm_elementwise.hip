@@ -7,6 +7,8 @@
 //   copy       plain complex64 copy with the 8-byte-per-lane accesses of the
 //              FFT passes: the bandwidth reference and the known-size kernel
 //              that scales the HBM traffic counters.
+//   high-NA    the defocus factor of high_na_ff_lens
+//              (src/chromatix/functional/lenses.py:98-156), in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -146,6 +148,67 @@ extern "C" int cxm_plane_sumsq_c64(void* stream, const void* u, void* out, int64
   CXM_LAUNCH_CHECK();
   hipLaunchKernelGGL(plane_sumsq_stage2, dim3((unsigned)((n_planes + 63) / 64)), dim3(64), 0, s,
                      (const double*)workspace, (double*)out, (int)n_planes);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// high_na_ff_lens defocus: u * exp(i k sz f) / sz (0 where sz == 0) on the
+// centred frequency grid, sz = sqrt(max(1 - NA^2 |f_grid lambda / n|^2, 0)).
+// ---------------------------------------------------------------------------
+namespace cxm {
+struct HighNaParams {
+  double fy_step, fx_step;  // 1 / (N d) per axis
+  double s_scale;           // lambda / n
+  double NA2, k, f;
+  int ny, nx;
+};
+
+__global__ void high_na_defocus_kernel(const float2* __restrict__ in, float2* __restrict__ out,
+                                       long long n_total, HighNaParams c) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_total) return;
+  const long long plane = (long long)c.ny * c.nx;
+  const int r = (int)(i % plane);
+  const int y = r / c.nx, x = r - y * c.nx;
+  // fftshift(fftfreq(N)) / d = (index - N/2 rounded down) / (N d)
+  const double sy = (double)(y - c.ny / 2) * c.fy_step * c.s_scale;
+  const double sx = (double)(x - c.nx / 2) * c.fx_step * c.s_scale;
+  const double sz2 = 1.0 - c.NA2 * (sy * sy + sx * sx);
+  const double sz = sqrt(sz2 > 0.0 ? sz2 : 0.0);
+  float2 v = in[i];
+  if (sz == 0.0) {
+    out[i] = make_float2(0.f, 0.f);
+    return;
+  }
+  const double ph = c.k * sz * c.f;
+  const double re = cos(ph) / sz, im = sin(ph) / sz;
+  out[i] = make_float2((float)(v.x * re - v.y * im), (float)(v.x * im + v.y * re));
+}
+}  // namespace cxm
+
+extern "C" int cxm_high_na_defocus_c64(void* stream, const void* in, void* out, int64_t n_planes,
+                                       int32_t height, int32_t width, double dy, double dx,
+                                       double wavelength, double n, double NA, double f) {
+  if (n_planes < 0 || height <= 0 || width <= 0) return fail(CXM_ERR_INVALID, "invalid shape");
+  if (!(dy > 0.0) || !(dx > 0.0) || !(wavelength > 0.0) || !(n > 0.0))
+    return fail(CXM_ERR_INVALID, "spacing, wavelength and n must be positive");
+  if (n_planes == 0) return CXM_OK;
+  if (!in || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  HighNaParams c;
+  c.fy_step = 1.0 / ((double)height * dy);
+  c.fx_step = 1.0 / ((double)width * dx);
+  c.s_scale = wavelength / n;
+  c.NA2 = NA * NA;
+  c.k = 2.0 * 3.14159265358979323846 * n / wavelength;
+  c.f = f;
+  c.ny = height;
+  c.nx = width;
+  const long long tot = n_planes * (long long)height * width;
+  if ((tot + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "too many elements for one launch");
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(high_na_defocus_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
+                     (const float2*)in, (float2*)out, tot, c);
   CXM_LAUNCH_CHECK();
   return CXM_OK;
 }

@@ -8,6 +8,7 @@ from .propagation import (  # noqa: F401
 from .optics import (  # noqa: F401,E402
     compute_sas_precompensation,
     ff_lens,
+    high_na_ff_lens,
     optical_fft,
     transform_propagate,
     transform_propagate_sas,

@@ -1,12 +1,17 @@
-"""The CZT branch of kernel_propagate and the output-resampling modes of
-transfer_propagate / asm_propagate (src/chromatix/functional/propagation.py:
-168-352, 395-443).
+"""The output-resampling branches of kernel_propagate and the
+output_dx / output_shape modes of transfer_propagate / asm_propagate
+(src/chromatix/functional/propagation.py:168-352, 395-458).
 
 u = fftshift(H * fft2(ifftshift(u_in))) is formed by the shifted raw FFT
-(two passes) and one multiply by fftshift(H); then one Bluestein pass per
-axis (cxm_czt_axis_c64) evaluates the field on the output grid, with the
-reference's per-axis modulation w^(-N k) a^N, N = (m - 1) // 2, and the
-scale prod(1 / alpha), alpha = output_dx / input_df.
+(two passes) and one multiply by fftshift(H).  With use_czt, one Bluestein
+pass per axis (cxm_czt_axis_c64) evaluates the field on the output grid,
+with the reference's per-axis modulation w^(-N k) a^N, N = (m - 1) // 2,
+and the scale prod(1 / alpha), alpha = output_dx / input_df.  Without it
+(the scaled convolution of Yu et al. 2012), one pass per axis
+(cxm_yu_axis_c64) applies the pre-chirp (1/alpha) e^{i pi alpha f^2}, the
+"same" linear convolution with the finite chirp e^{-i pi alpha f^2} and the
+output modulation dx e^{i pi x^2 / alpha}; the output keeps the input's
+shape, as the reference's broadcast of the output-grid modulation requires.
 """
 from __future__ import annotations
 
@@ -15,7 +20,7 @@ import torch
 
 from .. import _lib as L
 from ..core.field import ChromaticScalarField, Field, crop, pad, shift_grid
-from ..utils.czt import czt_axis
+from ..utils.czt import czt_axis, scaled_conv_axis
 from .optics import fft as _fft_shifted
 from .propagation import (
     _pair,
@@ -31,16 +36,18 @@ def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=
     """kernel_propagate with an output grid of spacing ``output_dx`` and shape
     ``output_shape`` centred on ``shift_yx`` (propagation.py:395-443)."""
     if isinstance(field, ChromaticScalarField):
-        raise NotImplementedError("CZT propagation: monochromatic scalar fields only")
-    if not use_czt:
-        raise NotImplementedError("the scaled-convolution branch (use_czt=False with output_dx) "
-                                  "is not implemented")
+        raise NotImplementedError("output-grid propagation: monochromatic scalar fields only")
     ny, nx = field.spatial_shape
     if output_shape is None:
         output_shape = (ny, nx)
     output_shape = (int(output_shape[0]), int(output_shape[1]))
-    if min(output_shape) < 2:
+    if use_czt and min(output_shape) < 2:
         raise ValueError("output_shape must be at least 2 x 2")
+    if not use_czt and output_shape != (ny, nx):
+        # the reference multiplies the output-grid modulation (output_shape)
+        # with the convolution (input shape): only equal shapes broadcast
+        raise ValueError("the scaled-convolution branch keeps the input shape: output_shape "
+                         f"{output_shape} != {(ny, nx)}")
     out_dx = field.dx if output_dx is None else np.broadcast_to(
         np.asarray(output_dx, dtype=np.float64), (2,)).copy()
     T = field.dx * np.array([ny, nx], dtype=np.float64)  # input extent
@@ -56,8 +63,17 @@ def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=
     L.check(L.lib().cxm_cmul_c64(L.stream_ptr(u.device), spec.data_ptr(), H.data_ptr(),
                                  spec.data_ptr(), n_planes, ny, nx, H.shape[0], 1))
     out_field = Field.empty_like(shift_grid(field, shift_yx), dx=out_dx, shape=output_shape)
-    (y0, y1), (x0, x1) = out_field.spatial_limits
     alpha = out_field.dx / df
+    if not use_czt:
+        # first output coordinate per axis: (0 - n/2) dx + origin (field.py:23-33)
+        origin = np.broadcast_to(np.asarray(out_field.origin, dtype=np.float64), (2,))
+        res = spec
+        for d in range(2):
+            n_d = output_shape[d]
+            x0 = -(n_d / 2.0) * float(out_field.dx[d]) + float(origin[d])
+            res = scaled_conv_axis(res, d, float(alpha[d]), float(df[d]), float(out_field.dx[d]), x0)
+        return out_field.replace(u=res)
+    (y0, y1), (x0, x1) = out_field.spatial_limits
     lims = ((y0, y1), (x0, x1))
     res = spec
     for d in range(2):

@@ -7,6 +7,7 @@ Mirrors, with the reference signatures:
   compute_sas_precompensation  src/chromatix/functional/propagation.py:85-103
   transform_propagate_sas      src/chromatix/functional/propagation.py:106-165
   fft / ifft (with shift)      src/chromatix/utils/fft.py:8-35
+  high_na_ff_lens (scalar)     src/chromatix/functional/lenses.py:98-156
 
 Each call is one fused two-pass (optical FFT, transform) or four/five-pass
 (SAS) kernel chain; the new sample spacing is computed on the host in FP64.
@@ -30,6 +31,7 @@ __all__ = [
     "transform_propagate_sas",
     "fft",
     "ifft",
+    "high_na_ff_lens",
 ]
 
 
@@ -195,3 +197,39 @@ def fft(x, axes=(1, 2), shift: bool = False) -> torch.Tensor:
 def ifft(x, axes=(1, 2), shift: bool = False) -> torch.Tensor:
     """ifft2 over `axes`; with shift, fftshift(ifft2(ifftshift(x)))."""
     return _raw(x, axes, shift, inverse=True)
+
+
+def high_na_ff_lens(field: Field, f, n, NA, output_shape=None, output_dx=None) -> Field:
+    """High-NA lens a distance ``f`` after a scalar, single-wavelength, square
+    field (lenses.py:98-156): the field times the defocus factor
+    exp(i k sz f) / sz (0 where sz = 0), sz = sqrt(max(1 - NA^2 |f_grid
+    lambda / n|^2, 0)), k = 2 pi n / lambda (cxm_high_na_defocus_c64, FP64),
+    then the zoomed FFT over [-zoom pi, zoom pi] with the end point,
+    zoom = 2 NA (output_shape[0] output_dx) / ((N - 1) lambda) (two CZT
+    passes).  The output spacing is ``output_dx`` (default: the field's y
+    spacing) on both axes.  Vectorial fields (the reference's
+    cartesian_to_spherical branch) are not part of this library."""
+    from ..utils.czt import zoomed_fft
+
+    if isinstance(field, ChromaticScalarField):
+        raise NotImplementedError("high_na_ff_lens: single-wavelength scalar fields only "
+                                  "(the reference assumes one wavelength)")
+    ny, nx = field.spatial_shape
+    if output_dx is None:
+        output_dx = float(field.dx[0])
+    output_dx = float(output_dx)
+    if output_shape is None:
+        output_shape = (ny, nx)
+    output_shape = (int(output_shape[0]), int(output_shape[1]))
+    wl = float(field.spectrum.wavelength[0])
+    zoom = 2.0 * float(NA) * output_shape[0] * output_dx / ((nx - 1) * wl)
+    u = field.u
+    L.ensure_device(u)
+    u = u.to(torch.complex64).contiguous()
+    n_planes = u.numel() // (ny * nx)
+    tmp = torch.empty_like(u)
+    L.check(L.lib().cxm_high_na_defocus_c64(L.stream_ptr(u.device), u.data_ptr(), tmp.data_ptr(),
+                                            n_planes, ny, nx, float(field.dx[0]), float(field.dx[1]),
+                                            wl, float(n), float(NA), float(f)))
+    out = zoomed_fft(tmp, -zoom * np.pi, zoom * np.pi, output_shape, include_end=True)
+    return field.replace(u=out, dx=np.full(2, output_dx))

@@ -12,7 +12,7 @@ import torch
 
 from .. import _lib as L
 
-__all__ = ["czt", "cztn", "zoomed_fft", "czt_axis"]
+__all__ = ["czt", "cztn", "zoomed_fft", "czt_axis", "scaled_conv_axis"]
 
 
 def _pairs(v, count: int):
@@ -84,3 +84,30 @@ def zoomed_fft(x, k_start: float, k_end: float, output_shape, axes=(-2, -1),
     w = tuple(np.exp(1j * (k_end - k_start) / n) for n in renorm)
     a = np.exp(-1j * k_start)
     return cztn(x, tuple(output_shape), tuple(a for _ in output_shape), w, axes)
+
+
+def scaled_conv_axis(x: torch.Tensor, axis: int, alpha: float, df: float, out_dx: float,
+                     x0: float) -> torch.Tensor:
+    """One axis of kernel_propagate's scaled convolution (Yu et al. 2012,
+    propagation.py:444-458) on the planes of ``x`` (..., H, W) along axis 0
+    (H) or 1 (W) of the last two (cxm_yu_axis_c64):
+    y_j = out_dx e^{i pi x_j^2/alpha} sum_k x_k (1/alpha) e^{i pi alpha f_k^2}
+    kern[j + (n-1)//2 - k], kern[t] = e^{-i pi alpha f_t^2} on [0, n),
+    f_k = (k - n//2) df, x_j = x0 + j out_dx."""
+    x = torch.as_tensor(x)
+    L.ensure_device(x)
+    x = x.to(torch.complex64).contiguous()
+    H, W = x.shape[-2:]
+    d = L.YuDesc()
+    d.batch = int(np.prod(x.shape[:-2])) if x.ndim > 2 else 1
+    d.height, d.width = H, W
+    d.axis = int(axis)
+    d.alpha, d.df, d.out_dx, d.x0 = float(alpha), float(df), float(out_dx), float(x0)
+    out = torch.empty_like(x)
+    nbytes = L.lib().cxm_yu_axis_workspace_size(ctypes.byref(d))
+    if nbytes == 0:
+        L.check(L.lib().cxm_yu_axis_c64(None, None, None, None, 0, ctypes.byref(d)))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=x.device)
+    L.check(L.lib().cxm_yu_axis_c64(L.stream_ptr(x.device), x.data_ptr(), out.data_ptr(),
+                                    ws.data_ptr(), nbytes, ctypes.byref(d)))
+    return out

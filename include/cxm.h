@@ -303,6 +303,49 @@ int cxm_czt_axis_c64(void* stream, const void* in, void* out, void* workspace,
                      size_t workspace_bytes, const cxm_czt_desc* d);
 
 /*
+ * The scaled-convolution branch of kernel_propagate (Yu et al. 2012; used
+ * when output_dx or output_shape is given and use_czt is False,
+ * src/chromatix/functional/propagation.py:444-458), one axis of the
+ * separable form, on batch planes of height x width complex64 (the shifted
+ * spectrum fftshift(H * fft2(ifftshift(u)))):
+ *   y_j = out_dx exp(i pi x_j^2 / alpha)
+ *         sum_k u_k (1/alpha) exp(i pi alpha f_k^2) kern[j + s - k]
+ *   kern[t] = exp(-i pi alpha f_t^2) for 0 <= t < n, 0 otherwise,
+ *   f_k = (k - n/2 rounded down) df, x_j = x0 + j out_dx, s = (n - 1) / 2
+ *   rounded down (the "same" window of the full linear convolution),
+ * n the length along the axis (axis 1: rows, n = width; axis 0: columns,
+ * n = height); the output has the input's shape.  The chirps are evaluated
+ * in FP64 on the device (the reference builds them in float32); one pass
+ * with an FFT of the next supported length >= 2n - 1.  in == out is not
+ * allowed.
+ */
+typedef struct cxm_yu_desc {
+  int32_t batch;
+  int32_t height, width;
+  int32_t axis;
+  double alpha;           /* out_dx / df */
+  double df;              /* input frequency spacing along the axis */
+  double out_dx;          /* output sample spacing along the axis */
+  double x0;              /* first output coordinate (grid origin included) */
+} cxm_yu_desc;
+
+size_t cxm_yu_axis_workspace_size(const cxm_yu_desc* d);
+int cxm_yu_axis_c64(void* stream, const void* in, void* out, void* workspace,
+                    size_t workspace_bytes, const cxm_yu_desc* d);
+
+/*
+ * The scalar part of high_na_ff_lens (src/chromatix/functional/lenses.py:98-156):
+ *   out = in * defocus,  defocus = exp(i k sz f) / sz where sz != 0, else 0,
+ *   sz = sqrt(max(1 - NA^2 |f_grid lambda / n|^2, 0)), k = 2 pi n / lambda,
+ * on the centred frequency grid of height x width planes (spacings dy, dx),
+ * evaluated in FP64 (the phase k sz f reaches 1e5 rad).  The zoomed FFT
+ * that follows is two cxm_czt_axis_c64 passes.  in == out is allowed.
+ */
+int cxm_high_na_defocus_c64(void* stream, const void* in, void* out, int64_t n_planes,
+                            int32_t height, int32_t width, double dy, double dx,
+                            double wavelength, double n, double NA, double f);
+
+/*
  * Elementwise product of n_planes planes of height x width complex64 with a
  * stack of b_planes planes (plane p uses b[p % b_planes]), b optionally
  * read fftshifted: out = a * fftshift(b).  The propagator multiply of the

@@ -1,8 +1,9 @@
-"""GPU parity of the chirp-z transform (cxm_czt_axis_c64) and the CZT
-output-resampling of asm_propagate against the FP64 oracle (the defining
-sum).  Bar: relative L2 <= 1e-5.  (The reference builds its chirps in
-complex64 and is ~5e-3 from FP64 at these sizes; this build's chirps are
-FP64, so the gate is FP64.)"""
+"""GPU parity of the chirp-z transform (cxm_czt_axis_c64), the CZT and
+scaled-convolution (Yu et al.) output resampling of kernel_propagate /
+asm_propagate, and high_na_ff_lens against the FP64 oracle.  Bar: relative
+L2 <= 1e-5.  (The reference builds its chirps in complex64 and is ~5e-3
+from FP64 at these sizes; this build's chirps are FP64, so the gate is
+FP64.)"""
 import numpy as np
 import pytest
 import torch
@@ -11,6 +12,7 @@ import chromatix_amd as cx
 import chromatix_amd.functional as cf
 from chromatix_amd.utils import czt as uc
 from oracle import czt as oc
+from oracle import propagation as op
 from tests.conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -71,3 +73,61 @@ def test_asm_propagate_czt_output_grid(n, m, odx, shift):
     assert err < TOL
     np.testing.assert_allclose(out.dx, dxo)
     np.testing.assert_allclose(out.origin, org)
+
+
+@pytest.mark.parametrize("shape,odx,shift", [
+    ((256, 256), 0.45, (0.0, 0.0)),
+    ((512, 512), (0.2, 0.35), (3.0, -1.5)),
+    ((320, 384), 0.6, (-2.0, 1.0)),
+    ((1024, 1024), 0.3, (0.0, 0.0)),
+])
+def test_kernel_propagate_scaled_convolution(shape, odx, shift):
+    """kernel_propagate(..., output_dx, use_czt=False): the scaled convolution
+    of Yu et al. (propagation.py:444-458) against the FP64 oracle."""
+    rng = np.random.default_rng(shape[0] + shape[1])
+    u = np.exp(2j * np.pi * rng.random(shape)).astype(np.complex64)
+    H = op.asm_propagator(shape, (0.3, 0.3), 0.532, 40.0, 1.33)
+    f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, 0.532)
+    out = cf.kernel_propagate(f, torch.as_tensor(H.astype(np.complex64)).cuda(), output_dx=odx,
+                              shift_yx=shift, use_czt=False)
+    ref = oc.yu_kernel_propagate(u, H, (0.3, 0.3), odx, shift)
+    assert out.u.shape == shape
+    err = rel_l2(out.u.cpu().numpy(), ref)
+    print(f"scaled convolution {shape} dx_out {odx}: {err:.2e}")
+    assert err < TOL
+    np.testing.assert_allclose(out.dx, np.broadcast_to(np.asarray(odx, float), (2,)))
+    np.testing.assert_allclose(out.origin, np.asarray(shift, float))
+
+
+def test_asm_propagate_scaled_convolution_same_mode():
+    """asm_propagate with output_dx and use_czt=False: pad, propagator with
+    the output shift, scaled convolution, crop (propagation.py:325-352)."""
+    rng = np.random.default_rng(9)
+    u = np.exp(2j * np.pi * rng.random((192, 192))).astype(np.complex64)
+    f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.25, 0.0, 0.532)
+    out = cf.asm_propagate(f, 30.0, 1.33, 32, output_dx=0.4, shift_yx=(1.0, 2.0), use_czt=False,
+                           phase_mode="accurate")
+    up = np.pad(u.astype(np.complex128), 32)
+    H = op.asm_propagator(up.shape, (0.25, 0.25), 0.532, 30.0, 1.33, shift_yx=(1.0, 2.0))
+    ref = oc.yu_kernel_propagate(up, H, (0.25, 0.25), 0.4, (1.0, 2.0))[32:-32, 32:-32]
+    assert out.u.shape == (192, 192)
+    assert rel_l2(out.u.cpu().numpy(), ref) < TOL
+    with pytest.raises(ValueError):
+        cf.asm_propagate(f, 30.0, 1.33, 32, output_dx=0.4, output_shape=(128, 128), use_czt=False)
+
+
+@pytest.mark.parametrize("n,out_shape,odx,f_len,NA", [
+    (256, None, None, 100.0, 0.8),
+    (512, (200, 200), 0.15, 2000.0, 1.2),
+])
+def test_high_na_ff_lens(n, out_shape, odx, f_len, NA):
+    rng = np.random.default_rng(n)
+    u = np.exp(2j * np.pi * rng.random((n, n))).astype(np.complex64)
+    f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, 0.532)
+    out = cf.high_na_ff_lens(f, f_len, 1.33, NA, output_shape=out_shape, output_dx=odx)
+    ref, dxo = oc.high_na_ff_lens(u, (0.3, 0.3), 0.532, f_len, 1.33, NA, out_shape, odx)
+    err = rel_l2(out.u.cpu().numpy(), ref)
+    print(f"high-NA lens {n} -> {ref.shape}: {err:.2e}")
+    assert out.u.shape == ref.shape
+    assert err < TOL
+    np.testing.assert_allclose(out.dx, dxo)
