@@ -200,6 +200,29 @@ class YuDesc(ctypes.Structure):
     ]
 
 
+class FluorDesc(ctypes.Structure):
+    _fields_ = [
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("depth", ctypes.c_int32),
+        ("pad_y", ctypes.c_int32),
+        ("pad_x", ctypes.c_int32),
+        ("phase_mode", ctypes.c_int32),
+        ("remove_evanescent", ctypes.c_int32),
+        ("bandlimit", ctypes.c_int32),
+        ("dy", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("wavelength", ctypes.c_double),
+        ("n", ctypes.c_double),
+        ("thickness", ctypes.c_double),
+        ("ky", ctypes.c_double),
+        ("kx", ctypes.c_double),
+        ("intensity_scale", ctypes.c_double),
+        ("prop_fwd", ctypes.c_void_p),
+        ("prop_bwd", ctypes.c_void_p),
+    ]
+
+
 class MultisliceDesc(ctypes.Structure):
     _fields_ = [
         ("batch", ctypes.c_int32),
@@ -275,6 +298,9 @@ def _declare(lib):
         "cxm_mbs_iterate": (i32, [vp, vp, vp, vp, ctypes.c_int32, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_output": (i32, [vp, vp, vp, vp, ctypes.POINTER(MbsDesc)]),
         "cxm_czt_axis_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(CztDesc)]),
+        "cxm_fluor_multislice_workspace_size": (sz, [ctypes.POINTER(FluorDesc)]),
+        "cxm_fluor_multislice_sample_c64": (i32, [vp, vp, vp, vp, vp, vp, vp, sz,
+                                                  ctypes.POINTER(FluorDesc)]),
         "cxm_yu_axis_workspace_size": (sz, [ctypes.POINTER(YuDesc)]),
         "cxm_yu_axis_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(YuDesc)]),
         "cxm_high_na_defocus_c64": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,

@@ -1,6 +1,7 @@
 // Multislice propagation through a thick sample and its checkpointed
 // adjoint (multislice_thick_sample, src/chromatix/functional/samples.py:137-282;
-// thin_sample :85-134).
+// thin_sample :85-134), and the fluorescent Monte-Carlo variant
+// (fluorescent_multislice_thick_sample, samples.py:285-441).
 //
 // The field stays in row-transformed form between slices, so each slice is
 // two passes over one padded plane per field:
@@ -394,6 +395,253 @@ extern "C" int cxm_multislice_bwd(void* stream, const void* in, const float* dn,
           return rc;
       }
     }
+  }
+  return CXM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Fluorescent multislice, one Monte-Carlo sample (samples.py:285-441).
+// Forward: the field stays row-transformed between slices as above; slice
+// i's row pass applies the thin sample and adds f_i exp(i r_i) between the
+// row IFFT and FFT, the column pass multiplies by P(+t); the last column
+// pass stops after the column FFT and the multiply, which is fft2 of the
+// field after the last slice.  Backward per step: U *= P(-t) in place, row
+// IFFT, column IFFT storing |.|^2 of the cropped window accumulated into
+// the slice's intensity plane.
+// ---------------------------------------------------------------------------
+namespace cxm {
+namespace {
+struct FlGeom {
+  int h, w, Ny, Nx, D;
+  long long plane;
+};
+
+int fl_geom(const cxm_fluor_desc* d, FlGeom& g) {
+  if (!d) return fail(CXM_ERR_INVALID, "null descriptor");
+  if (d->height <= 0 || d->width <= 0 || d->depth < 1 || d->pad_y < 0 || d->pad_x < 0)
+    return fail(CXM_ERR_INVALID, "invalid shape parameters");
+  if (d->phase_mode != CXM_PHASE_REF32 && d->phase_mode != CXM_PHASE_ACCURATE)
+    return fail(CXM_ERR_INVALID, "invalid phase mode");
+  if (!(d->dy > 0 && d->dx > 0 && d->wavelength > 0 && d->n > 0))
+    return fail(CXM_ERR_INVALID, "spacing, wavelength and n must be positive");
+  g.h = d->height;
+  g.w = d->width;
+  g.Ny = d->height + 2 * d->pad_y;
+  g.Nx = d->width + 2 * d->pad_x;
+  g.D = d->depth;
+  g.plane = (long long)g.Ny * g.Nx;
+  if (!fft_len_supported(g.Ny) || !fft_len_supported(g.Nx))
+    return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT grid " + std::to_string(g.Ny) + "x" +
+                                         std::to_string(g.Nx));
+  return CXM_OK;
+}
+
+// The slice arrays (unpadded, centred in the padded grid) for an operator.
+void fl_arrays(OpDesc* o, const cxm_fluor_desc* d, const FlGeom& g, const float* a0,
+               const float* a1, int i) {
+  const long long sl = (long long)g.h * g.w;
+  o->arr0 = a0 + i * sl;
+  o->arr1 = a1 ? a1 + i * sl : nullptr;
+  o->arr_plane_stride = 0;
+  o->arr_row_stride = g.w;
+  o->arr_rows = g.h;
+  o->arr_cols = g.w;
+  o->arr_row_off = d->pad_y;
+  o->arr_col_off = d->pad_x;
+  o->t = (float)d->thickness;
+}
+
+// Thin sample and fluorescence of slice i as operators of list L.
+int fl_slice_ops(Ops& b, OpList& L, const cxm_fluor_desc* d, const FlGeom& g, const float* dn,
+                 const float* fl, const float* ph, int i) {
+  OpDesc* th = b.add(L, OP_THIN, 1, 1);
+  OpDesc* fo = b.add(L, OP_FLUOR, 1, 0);
+  if (!th || !fo) return fail(CXM_ERR_INVALID, "operator table overflow");
+  fl_arrays(th, d, g, dn, nullptr, i);
+  b.p(th, 0, 0) = make_float4((float)d->wavelength, 0.f, 0.f, 0.f);
+  fl_arrays(fo, d, g, fl, ph, i);
+  return CXM_OK;
+}
+
+void fl_row_common(PipeArgs& a, const FlGeom& g) {
+  a.tw = twiddles_for(g.Nx);
+  a.in_plane_stride = g.plane;
+  a.in_line_stride = g.Nx;
+  a.out_plane_stride = g.plane;
+  a.out_line_stride = g.Nx;
+  a.n_lines = g.Ny;
+  a.n_planes = 1;
+  a.grid_ny = g.Ny;
+  a.grid_nx = g.Nx;
+  a.in_len = g.Nx;
+  a.out_len = g.Nx;
+  a.scale = 1.0f;
+  a.mid_scale = 1.0f;
+}
+
+void fl_col_common(PipeArgs& a, const FlGeom& g) {
+  a.tw = twiddles_for(g.Ny);
+  a.in_plane_stride = g.plane;
+  a.in_line_stride = g.Nx;
+  a.out_plane_stride = g.plane;
+  a.out_line_stride = g.Nx;
+  a.n_lines = g.Nx;
+  a.n_planes = 1;
+  a.grid_ny = g.Ny;
+  a.grid_nx = g.Nx;
+  a.in_len = g.Ny;
+  a.out_len = g.Ny;
+  a.out_line_len = g.Nx;
+  a.scale = 1.0f;
+  a.mid_scale = 1.0f;
+}
+
+int fl_propagator(const cxm_fluor_desc* d, const FlGeom& g, double z, void* out, hipStream_t s) {
+  cxm_prop2d_desc p;
+  memset(&p, 0, sizeof(p));
+  p.batch = 1;
+  p.height = g.Ny;
+  p.width = g.Nx;
+  p.num_z = 1;
+  p.kind = CXM_H_ASM;
+  p.phase_mode = d->phase_mode;
+  p.remove_evanescent = d->remove_evanescent;
+  p.bandlimit = d->bandlimit;
+  p.num_wv = 1;
+  p.dy = d->dy;
+  p.dx = d->dx;
+  p.wavelength = d->wavelength;
+  p.n = d->n;
+  p.ky = d->ky;
+  p.kx = d->kx;
+  p.z = &z;
+  return cxm_propagator_c64((void*)s, out, &p);
+}
+}  // namespace
+}  // namespace cxm
+
+extern "C" size_t cxm_fluor_multislice_workspace_size(const cxm_fluor_desc* d) {
+  FlGeom g;
+  if (fl_geom(d, g)) return 0;
+  return (size_t)3 * g.plane * sizeof(float2);  // row-transformed field, P(+t), P(-t)
+}
+
+extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const float* fluorescence,
+                                               const float* dn, const float* phases, float* intensity,
+                                               void* workspace, size_t workspace_bytes,
+                                               const cxm_fluor_desc* d) {
+  FlGeom g;
+  int rc = fl_geom(d, g);
+  if (rc) return rc;
+  if (!state || !fluorescence || !dn || !phases || !intensity)
+    return fail(CXM_ERR_INVALID, "null buffer");
+  const size_t need = (size_t)3 * g.plane * sizeof(float2);
+  if (!workspace || workspace_bytes < need)
+    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  float2* u = (float2*)state;
+  float2* W = (float2*)workspace;
+  float2* Pf = W + g.plane;
+  float2* Pb = Pf + g.plane;
+  const float2* Pfw = (const float2*)d->prop_fwd;
+  const float2* Pbw = (const float2*)d->prop_bwd;
+  if (!Pfw) {
+    if ((rc = fl_propagator(d, g, d->thickness, Pf, s))) return rc;
+    Pfw = Pf;
+  }
+  if (!Pbw) {
+    if ((rc = fl_propagator(d, g, -d->thickness, Pb, s))) return rc;
+    Pbw = Pb;
+  }
+  const float inv_n = (float)(1.0 / ((double)g.Ny * (double)g.Nx));
+  // forward, slice 0: thin sample + fluorescence, row FFT (state -> W)
+  {
+    PipeArgs a;
+    memset(&a, 0, sizeof(a));
+    Ops b(a);
+    if ((rc = fl_slice_ops(b, a.pre, d, g, dn, fluorescence, phases, 0))) return rc;
+    fl_row_common(a, g);
+    a.in = u;
+    a.out = W;
+    a.seq = SEQ_F;
+    if ((rc = launch_pipe_row(g.Nx, a, s))) return rc;
+  }
+  for (int i = 0; i < g.D; ++i) {
+    if (i + 1 < g.D) {
+      // column FFT * P(+t) * column IFFT (normalised for the 2D IFFT), in place
+      ColArgs ca;
+      memset(&ca, 0, sizeof(ca));
+      ca.in = W;
+      ca.out = W;
+      ca.tw = twiddles_for(g.Ny);
+      ca.H = Pfw;
+      ca.in_plane_stride = ca.out_plane_stride = g.plane;
+      ca.row_stride = g.Nx;
+      ca.ncols = g.Nx;
+      ca.n_planes = 1;
+      ca.nz = 1;
+      ca.nwv = 1;
+      ca.in_len = ca.out_len = g.Ny;
+      ca.kind = CXM_H_ARRAY;
+      ca.scale = inv_n;
+      if ((rc = launch_col_prop(g.Ny, ca, s))) return rc;
+      // row IFFT, slice i+1's thin sample + fluorescence, row FFT, in place
+      PipeArgs a;
+      memset(&a, 0, sizeof(a));
+      Ops b(a);
+      if ((rc = fl_slice_ops(b, a.mid, d, g, dn, fluorescence, phases, i + 1))) return rc;
+      fl_row_common(a, g);
+      a.in = W;
+      a.out = W;
+      a.seq = SEQ_IF;
+      if ((rc = launch_pipe_row(g.Nx, a, s))) return rc;
+    } else {
+      // last slice: U = column FFT * P(+t) = fft2 of the field after the slice
+      PipeArgs a;
+      memset(&a, 0, sizeof(a));
+      Ops b(a);
+      OpDesc* op = b.add(a.post, OP_CARRAY, 1, 0);
+      if (!op) return fail(CXM_ERR_INVALID, "operator table overflow");
+      op->arr0 = Pfw;
+      op->arr_row_stride = g.Nx;
+      op->arr_rows = g.Ny;
+      op->arr_cols = g.Nx;
+      fl_col_common(a, g);
+      a.in = W;
+      a.out = u;
+      a.seq = SEQ_F;
+      if ((rc = launch_pipe_col(g.Ny, a, s))) return rc;
+    }
+  }
+  // backward sweep
+  const long long sl = (long long)g.h * g.w;
+  for (int i = 0; i < g.D; ++i) {
+    if ((rc = cxm_cmul_c64((void*)s, u, Pbw, u, 1, g.Ny, g.Nx, 1, 0))) return rc;
+    PipeArgs r;
+    memset(&r, 0, sizeof(r));
+    fl_row_common(r, g);
+    r.in = u;
+    r.out = W;
+    r.seq = SEQ_I;
+    r.scale = (float)(1.0 / (double)g.Nx);
+    if ((rc = launch_pipe_row(g.Nx, r, s))) return rc;
+    PipeArgs c;
+    memset(&c, 0, sizeof(c));
+    fl_col_common(c, g);
+    c.in = W;
+    c.out = intensity + (long long)(g.D - 1 - i) * sl;
+    c.seq = SEQ_I;
+    c.scale = (float)(1.0 / (double)g.Ny);
+    c.intensity = 2;
+    c.iscale = (float)d->intensity_scale;
+    c.out_plane_stride = sl;
+    c.out_line_stride = g.w;
+    c.out_off = d->pad_y;
+    c.out_len = g.h;
+    c.out_line_off = d->pad_x;
+    c.out_line_len = g.w;
+    if ((rc = launch_pipe_col(g.Ny, c, s))) return rc;
   }
   return CXM_OK;
 }

@@ -48,6 +48,9 @@ enum OpKind : int {
   OP_THINADJ = 12,// adjoint of OP_THIN at the stored forward field u = arr2 (after the
                   //   sample): out0 (+)= k Im(v conj u), out1 (+)= -k Re(v conj u) (d/d dn,
                   //   d/d absorption, k = 2 pi t / lam), v = conj(T) v; p0 = (lam, k, -, -)
+  OP_FLUOR = 13,  // v += f[y,x] exp(i r[y,x])  (f32 arrays: f = arr0, r = arr1; outside:
+                  //   unchanged) -- the fluorescence added before each slice's
+                  //   propagation (samples.py:388-392, complex64 order f cos r, f sin r)
 };
 
 struct OpDesc {
@@ -242,6 +245,16 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
         *g1 = o.accum ? *g1 + gab : gab;
       }
       return cmul(v, make_float2(m * c, -m * s));
+    }
+    case OP_FLUOR: {
+      const int ay = iy - o.arr_row_off, ax = ix - o.arr_col_off;
+      if (ay < 0 || ay >= o.arr_rows || ax < 0 || ax >= o.arr_cols) return v;
+      const long long idx =
+          (long long)plane * o.arr_plane_stride + (long long)ay * o.arr_row_stride + ax;
+      const float f = ((const float*)o.arr0)[idx];
+      float s, c;
+      sincosf(((const float*)o.arr1)[idx], &s, &c);
+      return make_float2(__fadd_rn(__fmul_rn(f, c), v.x), __fadd_rn(__fmul_rn(f, s), v.y));
     }
     default:
       return v;

@@ -34,7 +34,9 @@ struct PipeArgs {
   int out_off, out_len, out_shift;  // along the transform axis
   int out_line_shift;         // column pass: store grid column c at fshift_dst(c)
   int out_line_off, out_line_len;   // lines kept: grid lines [off, off + len)
-  int seq, mid_shift, intensity;
+  int seq, mid_shift;
+  int intensity;              // 0: complex store; 1: |x|^2; 2: out += iscale |x|^2
+  float iscale;
   int accum;                  // phase-gradient row pass: add to `out` instead of storing
                               // (gridDim.y > 1: plane group y writes its partial sum to
                               // out + y * out_plane_stride)
@@ -51,6 +53,18 @@ struct PipeArgs {
 
 // kernels take PipeArgs by value: stay within the 4 KiB kernel-argument limit
 static_assert(sizeof(PipeArgs) <= 4096, "PipeArgs exceeds the kernel-argument limit");
+
+// Store of one output value: complex, |x|^2, or |x|^2 accumulated.
+CXM_DEV void store_value(const PipeArgs& a, long long idx, float2 x) {
+  if (a.intensity == 2) {
+    float* q = (float*)a.out + idx;
+    *q = __fadd_rn(*q, __fmul_rn(a.iscale, cabs2(x)));
+  } else if (a.intensity) {
+    ((float*)a.out)[idx] = cabs2(x);
+  } else {
+    ((float2*)a.out)[idx] = x;
+  }
+}
 
 // Rotate the register layout by N/2 (even N, even E): applies ifftshift to
 // the sequence held in the v[k] <-> X[t + kT] layout.
@@ -186,8 +200,7 @@ pipe_row_kernel(PipeArgs a) {
       if (xl >= 0 && xl < a.out_len) {
         float2 x = cscale(st[lds_at<N, 1>(j)], a.scale);
         x = apply_ops(a.post, a.tab, x, plane, yg, xo, a.grid_ny, a.grid_nx);
-        if (a.intensity) ((float*)a.out)[obase + xl] = cabs2(x);
-        else ((float2*)a.out)[obase + xl] = x;
+        store_value(a, obase + xl, x);
       }
     }
     return;
@@ -200,8 +213,7 @@ pipe_row_kernel(PipeArgs a) {
     const int xl = xo - a.out_off;
     if (xl >= 0 && xl < a.out_len) {
       const float2 x = cscale(v[k], a.scale);
-      if (a.intensity) ((float*)a.out)[obase + xl] = cabs2(x);
-      else ((float2*)a.out)[obase + xl] = x;
+      store_value(a, obase + xl, x);
     }
   }
 }
@@ -336,8 +348,7 @@ pipe_col_kernel(PipeArgs a) {
       if (yl >= 0 && yl < a.out_len) {
         float2 x = cscale(st[lds_at<N, C>(j)], a.scale);
         x = apply_ops(a.post, a.tab, x, plane, yo, xo, a.grid_ny, a.grid_nx);
-        if (a.intensity) ((float*)a.out)[obase + (long long)yl * a.out_line_stride] = cabs2(x);
-        else ((float2*)a.out)[obase + (long long)yl * a.out_line_stride] = x;
+        store_value(a, obase + (long long)yl * a.out_line_stride, x);
       }
     }
     return;
@@ -350,8 +361,7 @@ pipe_col_kernel(PipeArgs a) {
     const int yl = yo - a.out_off;
     if (yl >= 0 && yl < a.out_len) {
       const float2 x = cscale(v[k], a.scale);
-      if (a.intensity) ((float*)a.out)[obase + (long long)yl * a.out_line_stride] = cabs2(x);
-      else ((float2*)a.out)[obase + (long long)yl * a.out_line_stride] = x;
+      store_value(a, obase + (long long)yl * a.out_line_stride, x);
     }
   }
 }

@@ -16,4 +16,9 @@ from .optics import (  # noqa: F401,E402
 from .phase_masks import phase_change  # noqa: F401,E402
 from .sources import objective_point_source  # noqa: F401,E402
 from .psf import psf_4f, psf_4f_mse_grad, psf_output_spacing  # noqa: F401,E402
-from .samples import multislice_mse_grad, multislice_thick_sample, thin_sample  # noqa: F401,E402
+from .samples import (  # noqa: F401,E402
+    fluorescent_multislice_thick_sample,
+    multislice_mse_grad,
+    multislice_thick_sample,
+    thin_sample,
+)

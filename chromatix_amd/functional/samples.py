@@ -7,6 +7,10 @@ requested for ``dn_stack`` / ``absorption_stack`` the forward also stores
 the field state every ``checkpoint_every`` slices, and the backward
 (cxm_multislice_bwd) recomputes one segment at a time from those states and
 runs the hand-written adjoint: no per-slice residuals are kept.
+``fluorescent_multislice_thick_sample`` (samples.py:285-441) runs each
+Monte-Carlo sample as one library call (cxm_fluor_multislice_sample_c64):
+two fused passes per slice forward, three per slice for the backward
+intensity sweep.
 """
 from __future__ import annotations
 
@@ -19,7 +23,8 @@ from .. import _lib as L
 from ..core.field import ChromaticScalarField, Field
 from .propagation import _pair, _vec2, get_phase_mode
 
-__all__ = ["thin_sample", "multislice_thick_sample", "multislice_mse_grad"]
+__all__ = ["thin_sample", "multislice_thick_sample", "multislice_mse_grad",
+           "fluorescent_multislice_thick_sample"]
 
 
 def thin_sample(field: Field, absorption, dn, thickness) -> Field:
@@ -197,3 +202,100 @@ def multislice_mse_grad(field: Field, absorption_stack, dn_stack, target, n, thi
         raise ValueError("target must have the output's shape")
     return _bwd(u, dn.to(torch.float32).contiguous(), ab.to(torch.float32).contiguous(), checkpoints,
                 t, d, L.CXM_GRAD_MSE)
+
+
+def fluorescent_multislice_thick_sample(field: Field, fluorescence_stack, dn_stack, n,
+                                        thickness_per_slice, pad_width, key=0, num_samples: int = 1,
+                                        propagator_forward=None, propagator_backward=None,
+                                        kykx=(0.0, 0.0), remove_evanescent: bool = False,
+                                        bandlimit: bool = False, *, random_phases=None,
+                                        phase_mode=None) -> torch.Tensor:
+    """Scattered fluorescence intensity (depth, H, W) of a thick, phase-only,
+    fluorescent sample (samples.py:285-441), by Monte-Carlo over random
+    emitter phases.
+
+    Per sample: the field goes forward through the slices (thin sample, plus
+    fluorescence * exp(i r) with r uniform in [0, 2 pi), then an
+    angular-spectrum step of +thickness), is Fourier transformed, and is
+    propagated back step by step (-thickness), the intensity of each step
+    accumulating into its slice.  As in the reference, the field a sample
+    ends with (its backward-swept spectrum) is where the next sample starts,
+    and the stack is averaged over the samples.
+
+    ``key``: an integer seed or a ``torch.Generator`` for the random phases
+    (drawn on the device with torch; JAX's key stream is not reproduced).
+    ``random_phases``: optional (num_samples, depth, H, W) phases to use
+    instead.  ``propagator_forward`` / ``propagator_backward``: optional
+    (H + 2 pad, W + 2 pad) propagators in FFT order.
+    """
+    if isinstance(field, ChromaticScalarField):
+        raise NotImplementedError("fluorescent multislice: monochromatic scalar fields only")
+    u = field.u
+    L.ensure_device(u)
+    H, W = field.spatial_shape
+    if u.numel() != H * W:
+        raise NotImplementedError("fluorescent multislice: one field (the reference's intensity "
+                                  "stack has no batch axis)")
+    dev = u.device
+    fl = torch.as_tensor(fluorescence_stack, dtype=torch.float32, device=dev).contiguous()
+    dn = torch.as_tensor(dn_stack, dtype=torch.float32, device=dev).contiguous()
+    if fl.ndim != 3 or tuple(fl.shape) != tuple(dn.shape) or tuple(fl.shape[1:]) != (H, W):
+        raise ValueError("fluorescence_stack and dn_stack must both have shape (depth, H, W) "
+                         f"with (H, W) = {(H, W)}")
+    D = fl.shape[0]
+    S = int(num_samples)
+    if S < 1:
+        raise ValueError("num_samples must be >= 1")
+    if isinstance(pad_width, (tuple, list)) or np.ndim(pad_width) != 0:
+        raise ValueError("pad_width must be an integer (the reference pads both axes equally)")
+    pad = int(pad_width)
+    Ny, Nx = H + 2 * pad, W + 2 * pad
+    if random_phases is not None:
+        ph_all = torch.as_tensor(random_phases, dtype=torch.float32, device=dev)
+        if tuple(ph_all.shape) != (S, D, H, W):
+            raise ValueError(f"random_phases must have shape {(S, D, H, W)}")
+    else:
+        gen = key if isinstance(key, torch.Generator) else None
+        if gen is None:
+            gen = torch.Generator(device=dev)
+            gen.manual_seed(int(key))
+        ph_all = None
+    d = L.FluorDesc()
+    d.height, d.width, d.depth = H, W, D
+    d.pad_y = d.pad_x = pad
+    d.phase_mode = L.PHASE_MODES[phase_mode or get_phase_mode()]
+    d.remove_evanescent = int(bool(remove_evanescent))
+    d.bandlimit = int(bool(bandlimit))
+    d.dy, d.dx = float(field.dx[0]), float(field.dx[1])
+    d.wavelength = float(field.spectrum.wavelength[0])
+    d.n = float(n)
+    d.thickness = float(thickness_per_slice)
+    d.ky, d.kx = _vec2(kykx)
+    d.intensity_scale = 1.0 / S
+    keep = []
+    for name, P in (("prop_fwd", propagator_forward), ("prop_bwd", propagator_backward)):
+        if P is not None:
+            Pt = torch.as_tensor(P).to(device=dev, dtype=torch.complex64).reshape(-1)
+            if Pt.numel() != Ny * Nx:
+                raise ValueError(f"propagators must have the padded shape {(Ny, Nx)}")
+            Pt = Pt.contiguous()
+            keep.append(Pt)
+            setattr(d, name, Pt.data_ptr())
+    state = torch.zeros((Ny, Nx), dtype=torch.complex64, device=dev)
+    state[pad:pad + H, pad:pad + W] = u.reshape(H, W)
+    out = torch.zeros((D, H, W), dtype=torch.float32, device=dev)
+    nbytes = L.lib().cxm_fluor_multislice_workspace_size(ctypes.byref(d))
+    if nbytes == 0:
+        L.check(L.lib().cxm_fluor_multislice_sample_c64(None, None, None, None, None, None, None, 0,
+                                                         ctypes.byref(d)))
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    stream = L.stream_ptr(dev)
+    for smp in range(S):
+        if ph_all is not None:
+            ph = ph_all[smp].contiguous()
+        else:
+            ph = torch.rand((D, H, W), generator=gen, device=dev, dtype=torch.float32) * (2 * np.pi)
+        L.check(L.lib().cxm_fluor_multislice_sample_c64(stream, state.data_ptr(), fl.data_ptr(),
+                                                         dn.data_ptr(), ph.data_ptr(), out.data_ptr(),
+                                                         ws.data_ptr(), nbytes, ctypes.byref(d)))
+    return out

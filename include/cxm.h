@@ -410,6 +410,43 @@ int cxm_multislice_fwd(void* stream, const void* in, const float* dn, const floa
                        const cxm_multislice_desc* d);
 
 /*
+ * Fluorescent multislice (fluorescent_multislice_thick_sample,
+ * src/chromatix/functional/samples.py:285-441), one Monte-Carlo sample on
+ * one padded plane.  `state` (Ny x Nx complex64, Ny = height + 2 pad_y,
+ * Nx = width + 2 pad_x) holds the field the sample starts from -- the padded
+ * input field for the first sample -- and on return the field the
+ * reference carries into the next sample (the spectrum after the backward
+ * sweep).  Forward, slice i = 0 .. depth-1:
+ *   u = u * exp(i 2 pi dn_i t / lam) + f_i exp(i r_i)   (f, r, dn: depth x
+ *       height x width float32, zero outside the centred unpadded window)
+ *   u = ifft2(fft2(u) * P(+t))
+ * then U = fft2(u); backward, i = 0 .. depth-1:
+ *   U = U * P(-t);  intensity[depth-1-i] += intensity_scale * |crop(ifft2(U))|^2
+ * P(z): the angular-spectrum propagator of the padded grid (remove_evanescent,
+ * bandlimit as given), or the caller's prop_fwd / prop_bwd (Ny x Nx, FFT
+ * order, device) when non-null.  intensity: depth x height x width float32,
+ * accumulated.
+ */
+typedef struct cxm_fluor_desc {
+  int32_t height, width;      /* unpadded field and slices */
+  int32_t depth;
+  int32_t pad_y, pad_x;
+  int32_t phase_mode;         /* CXM_PHASE_* of the propagators */
+  int32_t remove_evanescent;
+  int32_t bandlimit;
+  double dy, dx, wavelength, n, thickness, ky, kx;
+  double intensity_scale;     /* 1 / num_samples */
+  const void* prop_fwd;       /* optional caller propagators */
+  const void* prop_bwd;
+} cxm_fluor_desc;
+
+size_t cxm_fluor_multislice_workspace_size(const cxm_fluor_desc* d);
+int cxm_fluor_multislice_sample_c64(void* stream, void* state, const float* fluorescence,
+                                    const float* dn, const float* phases, float* intensity,
+                                    void* workspace, size_t workspace_bytes,
+                                    const cxm_fluor_desc* d);
+
+/*
  * Gradient of a loss of the multislice output with respect to dn (and
  * absorption): reverse sweep with checkpointing.  Segments of k slices are
  * recomputed from the stored states (or from a forward recomputation when

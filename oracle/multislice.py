@@ -4,6 +4,8 @@
 Follows (paths relative to the chromatix repository root):
   thin_sample              src/chromatix/functional/samples.py:85-134
   multislice_thick_sample  src/chromatix/functional/samples.py:137-282
+  fluorescent_multislice_thick_sample
+                           src/chromatix/functional/samples.py:285-441
   compute_asm_propagator   src/chromatix/functional/propagation.py:499-580
 The adjoint is written out by hand (the reference differentiates with JAX
 reverse mode through fori_loop) and pinned by finite differences in
@@ -119,3 +121,36 @@ class Multislice:
             gab[i] = -k * np.sum(np.real(prod).reshape((-1, self.Ny, self.Nx)), axis=0)
             g = _prop(np.conj(T[i]) * g, np.conj(self.H))
         return self._crop(gdn), self._crop(gab)
+
+
+def fluorescent_multislice(u, fluorescence, dn, spacing, wavelength, n, thickness, pad, phases,
+                           kykx=(0.0, 0.0), remove_evanescent=False, bandlimit=False,
+                           prop_fwd=None, prop_bwd=None):
+    """fluorescent_multislice_thick_sample (samples.py:285-441) in FP64 with
+    the random phases given, (num_samples, depth, H, W).  The steps and the
+    carry of the field from one sample into the next are the reference's."""
+    u = np.asarray(u, np.complex128)
+    H, W = u.shape[-2:]
+    D = fluorescence.shape[0]
+    S = phases.shape[0]
+    field = _pad2(u.reshape(H, W), pad, pad)
+    Ny, Nx = field.shape
+    dnp = _pad2(np.asarray(dn, np.float64), pad, pad)
+    if prop_fwd is None:
+        prop_fwd = asm_propagator((Ny, Nx), spacing, wavelength, thickness, n, kykx,
+                                  bandlimit=bandlimit, remove_evanescent=remove_evanescent)
+    if prop_bwd is None:
+        prop_bwd = asm_propagator((Ny, Nx), spacing, wavelength, -thickness, n, kykx,
+                                  bandlimit=bandlimit, remove_evanescent=remove_evanescent)
+    out = np.zeros((D, H, W))
+    for smp in range(S):
+        fl = _pad2(np.asarray(fluorescence, np.float64) * np.exp(1j * phases[smp]), pad, pad)
+        for i in range(D):
+            field = field * np.exp(1j * 2 * np.pi * dnp[i] * thickness / wavelength)
+            field = np.fft.ifft2(np.fft.fft2(fl[i] + field) * prop_fwd)
+        field = np.fft.fft2(field)
+        for i in range(D):
+            field = field * prop_bwd
+            fi = np.fft.ifft2(field)[pad:pad + H, pad:pad + W]
+            out[D - 1 - i] += np.abs(fi) ** 2
+    return out / S

@@ -39,6 +39,7 @@ def test_every_declared_symbol_is_exported():
                                        (L.MultisliceDesc, "cxm_multislice_desc"),
                                        (L.CztDesc, "cxm_czt_desc"),
                                        (L.YuDesc, "cxm_yu_desc"),
+                                       (L.FluorDesc, "cxm_fluor_desc"),
                                        (L.MbsDesc, "cxm_mbs_desc")])
 def test_descriptor_layout_matches_header(tmp_path, cls, cname):
     fields = [f for f, _ in cls._fields_]

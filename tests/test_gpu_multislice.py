@@ -8,6 +8,8 @@ import torch
 
 import chromatix_amd as cx
 import chromatix_amd.functional as cf
+from oracle import multislice as om
+from oracle import propagation as op
 from oracle.multislice import Multislice, sample_factor
 from tests.conftest import rel_l2
 
@@ -185,3 +187,67 @@ def os_workers():
     import os
 
     return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def _fluor_case(H, D, seed):
+    rng = np.random.default_rng(seed)
+    u = np.exp(2j * np.pi * 0.05 * rng.standard_normal((H, H))).astype(np.complex64)
+    fl = rng.random((D, H, H)).astype(np.float32)
+    dn = (0.02 * rng.standard_normal((D, H, H))).astype(np.float32)
+    return u, fl, dn, rng
+
+
+@pytest.mark.parametrize("samples", [1, 2])
+def test_fluorescent_multislice_matches_oracle(samples):
+    """fluorescent_multislice_thick_sample with given random phases against
+    the FP64 oracle (accurate propagators), including the reference's carry
+    of the field from one Monte-Carlo sample into the next."""
+    H, D, pad, t = 256, 6, 32, 1.5
+    u, fl, dn, rng = _fluor_case(H, D, samples)
+    ph = rng.uniform(0, 2 * np.pi, (samples, D, H, H)).astype(np.float32)
+    f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, 0.532)
+    got = cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, t, pad, num_samples=samples,
+                                                 random_phases=ph, phase_mode="accurate")
+    ref = om.fluorescent_multislice(u, fl.astype(np.float64), dn.astype(np.float64), (0.3, 0.3),
+                                    0.532, 1.33, t, pad, ph.astype(np.float64))
+    err = rel_l2(got.cpu().numpy(), ref)
+    print(f"fluorescent multislice {H}^2 x {D}, {samples} sample(s): {err:.2e}")
+    assert got.shape == (D, H, H)
+    assert err < TOL
+
+
+def test_fluorescent_multislice_ref32_and_given_propagators():
+    """Default (ref32) propagators against the oracle run with the
+    reference-order float32 propagators, and caller-supplied propagators
+    used as given."""
+    H, D, pad, t = 192, 4, 32, 2.0
+    u, fl, dn, rng = _fluor_case(H, D, 7)
+    ph = rng.uniform(0, 2 * np.pi, (1, D, H, H)).astype(np.float32)
+    f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.25, 0.0, 0.532)
+    Ny = H + 2 * pad
+    Pf = op.asm_propagator((Ny, Ny), (0.25, 0.25), 0.532, t, 1.33, precision="ref32")
+    Pb = op.asm_propagator((Ny, Ny), (0.25, 0.25), 0.532, -t, 1.33, precision="ref32")
+    ref = om.fluorescent_multislice(u, fl.astype(np.float64), dn.astype(np.float64), (0.25, 0.25),
+                                    0.532, 1.33, t, pad, ph.astype(np.float64), prop_fwd=Pf, prop_bwd=Pb)
+    got = cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, t, pad, random_phases=ph)
+    assert rel_l2(got.cpu().numpy(), ref) < TOL
+    # the same propagators passed in (complex64) give the same result
+    got2 = cf.fluorescent_multislice_thick_sample(
+        f, fl, dn, 1.33, t, pad, random_phases=ph,
+        propagator_forward=torch.as_tensor(Pf.astype(np.complex64)).cuda(),
+        propagator_backward=torch.as_tensor(Pb.astype(np.complex64)).cuda())
+    assert rel_l2(got2.cpu().numpy(), ref) < TOL
+
+
+def test_fluorescent_multislice_seeded_monte_carlo():
+    """Phases drawn on the device: the same seed gives the same stack, another
+    seed another one; the stack is non-negative."""
+    H, D = 128, 3
+    u, fl, dn, _ = _fluor_case(H, D, 3)
+    f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, 0.532)
+    a = cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, 1.0, 16, key=5, num_samples=3)
+    b = cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, 1.0, 16, key=5, num_samples=3)
+    c = cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, 1.0, 16, key=6, num_samples=3)
+    assert torch.equal(a, b)
+    assert not torch.equal(a, c)
+    assert bool((a >= 0).all()) and bool(torch.isfinite(a).all())

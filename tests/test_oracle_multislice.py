@@ -4,6 +4,8 @@ the stack/final-field consistency."""
 import numpy as np
 import pytest
 
+from oracle import multislice as om
+from oracle import propagation as op
 from oracle.multislice import Multislice, sample_factor
 
 LAM, NM, T = 0.532, 1.33, 0.5
@@ -75,3 +77,46 @@ def test_ref32_sample_factor_is_close_to_fp64():
     a = sample_factor(dn, ab, T, LAM)
     b = sample_factor(dn, ab, T, LAM, "ref32")
     assert 0 < np.max(np.abs(a - b)) < 1e-6
+
+
+def test_fluorescent_multislice_without_fluorescence_is_a_z_stack():
+    """No emitters and no index change: the backward sweep visits the field
+    propagated to each slice depth, so slice j holds |u propagated by j t|^2
+    (the plain angular-spectrum propagation of the oracle, padded grid)."""
+    rng = np.random.default_rng(1)
+    H = W = 24
+    pad, D, t = 4, 5, 1.5
+    u = np.exp(2j * np.pi * rng.random((H, W)))
+    zero = np.zeros((D, H, W))
+    phases = rng.uniform(0, 2 * np.pi, (1, D, H, W))
+    got = om.fluorescent_multislice(u, zero, zero, (0.3, 0.3), 0.532, 1.33, t, pad, phases)
+    up = np.pad(u, pad)
+    for j in range(D):
+        Hj = op.asm_propagator(up.shape, (0.3, 0.3), 0.532, j * t, 1.33) if j else 1.0
+        ref = np.abs(np.fft.ifft2(np.fft.fft2(up) * Hj)[pad:-pad, pad:-pad]) ** 2
+        np.testing.assert_allclose(got[j], ref, rtol=1e-9, atol=1e-12)
+
+
+def test_fluorescent_multislice_point_emitter_is_linear_and_phase_blind():
+    """With no incident field, one emitter of amplitude a at slice k is a
+    point source added before slice k's step: slice j sees |a|^2 |impulse
+    propagated by (j - k) t|^2 (back-propagated for j < k) whatever its
+    random phase (the grid has no evanescent bins, so the +t and -t steps
+    compose to one propagation)."""
+    H = W = 16
+    pad, D, t, k = 4, 4, 2.0, 1
+    fl = np.zeros((D, H, W))
+    fl[k, 7, 9] = 3.0
+    dn = np.zeros((D, H, W))
+    u0 = np.zeros((H, W), complex)
+    for phase in (0.0, 2.1):
+        phases = np.full((1, D, H, W), phase)
+        got = om.fluorescent_multislice(u0, fl, dn, (0.3, 0.3), 0.532, 1.33, t, pad, phases)
+        src = np.zeros((H + 2 * pad, W + 2 * pad), complex)
+        src[7 + pad, 9 + pad] = 3.0
+        for j in range(D):
+            # D - k forward steps of t from the emitter, D - j backward steps
+            z = (j - k) * t
+            Hz = op.asm_propagator(src.shape, (0.3, 0.3), 0.532, z, 1.33) if z else 1.0
+            ref = np.abs(np.fft.ifft2(np.fft.fft2(src) * Hz)[pad:-pad, pad:-pad]) ** 2
+            np.testing.assert_allclose(got[j], ref, rtol=1e-8, atol=1e-12 * ref.max())
