@@ -44,10 +44,12 @@ def parse():
     p.add_argument("--phase-mode", default="ref32", choices=["ref32", "accurate"])
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--config", default="m0", choices=["m0", "c1", "c2", "c3", "c4", "c5"],
-                   help="m0: the headline batched 1024^2 AS propagation (default); c1-c5: the "
+    p.add_argument("--config", default="m0",
+                   choices=["m0", "c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8"],
+                   help="m0: the headline batched 1024^2 AS propagation (default); c1-c8: the "
                         "other workloads (plain AS with padding, 4f PSF + gradient, multislice + "
-                        "adjoint, Born solves, serving), one JSON line each")
+                        "adjoint, Born solves, serving, scaled-convolution propagation, high-NA "
+                        "lens, fluorescent multislice), one JSON line each")
     p.add_argument("--summarize", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
                    help="write the rocprofv3 summaries of a profiled run (kernel-trace --stats, "
                         "--pmc FETCH_SIZE, --pmc WRITE_SIZE; csv output) and exit")
@@ -176,7 +178,7 @@ def summarize(trace_dir: str, fetch_dir: str, write_dir: str) -> None:
 
 
 # ---------------------------------------------------------------------------
-# Other workloads (--config c1..c5).  Each prints one JSON line in the same
+# Other workloads (--config c1..c8).  Each prints one JSON line in the same
 # format; value and unit are the workload's own throughput.
 # ---------------------------------------------------------------------------
 def _timed(torch, fn, steps, warmup):
@@ -447,7 +449,90 @@ def bench_c5(args):
                    None, {"sweep": sweep})
 
 
-CONFIGS = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5}
+def bench_c6(args):
+    """kernel_propagate's scaled-convolution branch (Yu et al.): 16 planes of
+    1024^2, ASM propagator, output spacing 0.45 um (input 0.3 um)."""
+    import torch
+
+    import chromatix_amd as cx
+    import chromatix_amd.functional as cf
+    from chromatix_amd import _lib as L
+
+    B, N = 16, 1024
+    rng = np.random.default_rng(0)
+    u = torch.polar(torch.ones(B, N, N, device="cuda"),
+                    torch.as_tensor(rng.uniform(0, 2 * np.pi, (B, N, N)).astype(np.float32)).cuda())
+    f = cx.ScalarField(u, 0.3, 0.0, 0.532)
+    H = cf.compute_asm_propagator(f, 100.0, 1.33)
+    step = lambda: cf.kernel_propagate(f, H, output_dx=0.45, use_czt=False)
+    steps, warm = max(10, args.steps // 10), args.warmup
+    ms = _timed(torch, step, steps, warm)
+    kern = _kernels(torch, L, step, 3)
+    # per plane: shifted fft2 (two passes), propagator multiply, two axis passes
+    per_plane = 32 * N * N + 24 * N * N + 32 * N * N
+    return _result(args, "scaled-convolution propagations/s (1024^2 -> 1024^2 at 1.5x spacing)",
+                   B / (ms * 1e-3), "propagations/s", ms, steps, warm, "complex64",
+                   f"kernel_propagate(use_czt=False, output_dx 0.45) batch {B} x {N}^2, ASM z 100 um",
+                   B * per_plane, kern)
+
+
+def bench_c7(args):
+    """high_na_ff_lens: 16 planes of 1024^2 -> 512^2 (NA 0.8, f 100 um)."""
+    import torch
+
+    import chromatix_amd as cx
+    import chromatix_amd.functional as cf
+    from chromatix_amd import _lib as L
+
+    B, N, M = 16, 1024, 512
+    rng = np.random.default_rng(0)
+    u = torch.polar(torch.ones(B, N, N, device="cuda"),
+                    torch.as_tensor(rng.uniform(0, 2 * np.pi, (B, N, N)).astype(np.float32)).cuda())
+    f = cx.ScalarField(u, 0.3, 0.0, 0.532)
+    step = lambda: cf.high_na_ff_lens(f, 100.0, 1.33, 0.8, output_shape=(M, M), output_dx=0.2)
+    steps, warm = max(10, args.steps // 10), args.warmup
+    ms = _timed(torch, step, steps, warm)
+    kern = _kernels(torch, L, step, 3)
+    per_plane = 16 * N * N + (8 * N * N + 8 * M * N) + (8 * M * N + 8 * M * M)
+    return _result(args, "high-NA lens fields/s (1024^2 -> 512^2)", B / (ms * 1e-3), "fields/s", ms,
+                   steps, warm, "complex64",
+                   f"high_na_ff_lens batch {B} x {N}^2 -> {M}^2, NA 0.8, f 100 um, output dx 0.2 um",
+                   B * per_plane, kern)
+
+
+def bench_c8(args):
+    """fluorescent_multislice_thick_sample: 512^2 sample, pad 64 (640^2
+    grid), 64 slices, 4 Monte-Carlo samples per step."""
+    import torch
+
+    import chromatix_amd as cx
+    import chromatix_amd.functional as cf
+    from chromatix_amd import _lib as L
+
+    N, pad, D, S = 512, 64, 64, 4
+    Np = N + 2 * pad
+    rng = np.random.default_rng(0)
+    u = torch.ones(N, N, dtype=torch.complex64, device="cuda")
+    fl = torch.as_tensor(rng.random((D, N, N)).astype(np.float32)).cuda()
+    dn = torch.as_tensor((0.01 * rng.standard_normal((D, N, N))).astype(np.float32)).cuda()
+    f = cx.ScalarField(u, 0.3, 0.0, 0.532)
+    step = lambda: cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, 0.5, pad, key=1,
+                                                          num_samples=S)
+    steps, warm = 3, 1
+    ms = _timed(torch, step, steps, warm)
+    kern = _kernels(torch, L, step, 1)
+    # per sample and slice: forward row + column pass (32 Np^2) and the slice
+    # arrays (12 N^2); backward multiply (24 Np^2), row and column inverse
+    # passes (24 Np^2) and the intensity accumulation (8 N^2)
+    per = S * D * (80 * Np * Np + 20 * N * N)
+    return _result(args, "fluorescent multislice stacks/s (512^2, 64 slices, 4 samples)",
+                   1.0 / (ms * 1e-3), "stacks/s", ms, steps, warm, "complex64",
+                   f"fluorescent_multislice_thick_sample {N}^2 pad {pad}, {D} slices, {S} samples",
+                   per, kern, None, {"ms_per_sample_slice": ms / (S * D)})
+
+
+CONFIGS = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
+           "c6": bench_c6, "c7": bench_c7, "c8": bench_c8}
 
 
 def main():

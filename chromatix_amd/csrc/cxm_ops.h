@@ -136,27 +136,36 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
     }
     case OP_PHASE:
     case OP_THIN:
-    case OP_CARRAY: {
+    case OP_CARRAY:
+    case OP_FLUOR: {
       const int ay = iy - o.arr_row_off, ax = ix - o.arr_col_off;
       if (ay < 0 || ay >= o.arr_rows || ax < 0 || ax >= o.arr_cols)
         return o.outside_zero ? make_float2(0.f, 0.f) : v;
       const long long idx =
           (long long)plane * o.arr_plane_stride + (long long)ay * o.arr_row_stride + ax;
       if (o.kind == OP_CARRAY) return cmul(v, ((const float2*)o.arr0)[idx]);
-      float s, c;
+      // one sine/cosine site for the three phase-factor operators (another
+      // inlined copy tips the pipeline kernels into a stack frame in scratch)
+      float ang, m = 1.0f, f = 0.0f;
       if (o.kind == OP_PHASE) {
-        sincos_exact(__fmul_rn(((const float*)o.arr0)[idx], P[0].x), &s, &c);
-        return cmul(v, make_float2(c, s));
+        ang = __fmul_rn(((const float*)o.arr0)[idx], P[0].x);
+      } else if (o.kind == OP_THIN) {
+        // thin sample (samples.py:126-133): exp(i 2 pi (dn + i abs) t / lam)
+        const float dn = ((const float*)o.arr0)[idx];
+        const float ab = o.arr1 ? ((const float*)o.arr1)[idx] : 0.0f;
+        const float lam = P[0].x;
+        const float re = __fdiv_rn(__fmul_rn(-__fmul_rn(kTwoPiF, ab), o.t), lam);
+        ang = __fdiv_rn(__fmul_rn(__fmul_rn(kTwoPiF, dn), o.t), lam);
+        m = (re == 0.0f) ? 1.0f : expf(re);
+      } else {  // fluorescence f exp(i r), added
+        f = ((const float*)o.arr0)[idx];
+        ang = ((const float*)o.arr1)[idx];
       }
-      // thin sample (samples.py:126-133): exp(i 2 pi (dn + i abs) t / lam)
-      const float dn = ((const float*)o.arr0)[idx];
-      const float ab = o.arr1 ? ((const float*)o.arr1)[idx] : 0.0f;
-      const float lam = P[0].x;
-      const float re = __fdiv_rn(__fmul_rn(-__fmul_rn(kTwoPiF, ab), o.t), lam);
-      const float im = __fdiv_rn(__fmul_rn(__fmul_rn(kTwoPiF, dn), o.t), lam);
-      sincos_exact(im, &s, &c);
-      const float m = (re == 0.0f) ? 1.0f : expf(re);
-      return cmul(v, make_float2(m * c, m * s));
+      float sn, cs;
+      sincos_exact(ang, &sn, &cs);
+      if (o.kind == OP_FLUOR)
+        return make_float2(__fadd_rn(__fmul_rn(f, cs), v.x), __fadd_rn(__fmul_rn(f, sn), v.y));
+      return cmul(v, make_float2(m * cs, m * sn));
     }
     case OP_VEC: {
       const int i = o.vec_axis ? ix - o.arr_col_off : iy - o.arr_row_off;
@@ -245,16 +254,6 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
         *g1 = o.accum ? *g1 + gab : gab;
       }
       return cmul(v, make_float2(m * c, -m * s));
-    }
-    case OP_FLUOR: {
-      const int ay = iy - o.arr_row_off, ax = ix - o.arr_col_off;
-      if (ay < 0 || ay >= o.arr_rows || ax < 0 || ax >= o.arr_cols) return v;
-      const long long idx =
-          (long long)plane * o.arr_plane_stride + (long long)ay * o.arr_row_stride + ax;
-      const float f = ((const float*)o.arr0)[idx];
-      float s, c;
-      sincosf(((const float*)o.arr1)[idx], &s, &c);
-      return make_float2(__fadd_rn(__fmul_rn(f, c), v.x), __fadd_rn(__fmul_rn(f, s), v.y));
     }
     default:
       return v;
