@@ -223,6 +223,24 @@ class FluorDesc(ctypes.Structure):
     ]
 
 
+class FconvDesc(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int32),
+        ("depth", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("k_planes", ctypes.c_int32),
+        ("k_depth", ctypes.c_int32),
+        ("k_height", ctypes.c_int32),
+        ("k_width", ctypes.c_int32),
+        ("real_input", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+        ("out_depth", ctypes.c_int32),
+        ("out_height", ctypes.c_int32),
+        ("out_width", ctypes.c_int32),
+    ]
+
+
 class MultisliceDesc(ctypes.Structure):
     _fields_ = [
         ("batch", ctypes.c_int32),
@@ -306,6 +324,10 @@ def _declare(lib):
         "cxm_high_na_defocus_c64": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "cxm_fconv_workspace_size": (sz, [ctypes.POINTER(FconvDesc)]),
+        "cxm_fconv_c64": (i32, [vp, vp, vp, vp, vp, sz, ctypes.POINTER(FconvDesc)]),
+        "cxm_sum_axis_f32": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]),
+        "cxm_shot_noise_approx_f32": (i32, [vp, vp, vp, vp, ctypes.c_int64]),
         "cxm_multislice_workspace_size": (sz, [ctypes.POINTER(MultisliceDesc)]),
         "cxm_multislice_checkpoint_size": (sz, [ctypes.POINTER(MultisliceDesc)]),
         "cxm_multislice_fwd": (i32, [vp, vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(MultisliceDesc)]),

@@ -9,6 +9,8 @@
 //              that scales the HBM traffic counters.
 //   high-NA    the defocus factor of high_na_ff_lens
 //              (src/chromatix/functional/lenses.py:98-156), in FP64.
+//   sensor     sum over an axis (reduce_axis, functional/sensors.py:79-80)
+//              and approximate shot noise (ops/noise.py:10-16).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -209,6 +211,57 @@ extern "C" int cxm_high_na_defocus_c64(void* stream, const void* in, void* out, 
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(high_na_defocus_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
                      (const float2*)in, (float2*)out, tot, c);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Sensor: sum over the middle axis of (outer, n_sum, inner) in index order,
+// and approximate shot noise max(x + sqrt(x) n, 0).
+// ---------------------------------------------------------------------------
+namespace cxm {
+__global__ void sum_axis_kernel(const float* __restrict__ in, float* __restrict__ out, long long outer,
+                                long long n_sum, long long inner) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= outer * inner) return;
+  const long long o = i / inner, j = i - o * inner;
+  const float* p = in + o * n_sum * inner + j;
+  float acc = 0.0f;
+  for (long long k = 0; k < n_sum; ++k) acc = __fadd_rn(acc, p[k * inner]);
+  out[i] = acc;
+}
+
+__global__ void shot_noise_approx_kernel(const float* __restrict__ x, const float* __restrict__ nrm,
+                                         float* __restrict__ out, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = x[i];
+  const float y = __fadd_rn(v, __fmul_rn(sqrtf(v), nrm[i]));
+  out[i] = y > 0.0f ? y : 0.0f;
+}
+}  // namespace cxm
+
+extern "C" int cxm_sum_axis_f32(void* stream, const float* in, float* out, int64_t outer,
+                                int64_t n_sum, int64_t inner) {
+  if (outer < 0 || n_sum < 0 || inner < 0) return fail(CXM_ERR_INVALID, "negative size");
+  const long long tot = outer * inner;
+  if (tot == 0) return CXM_OK;
+  if (!in || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  if ((tot + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "too many elements for one launch");
+  hipLaunchKernelGGL(sum_axis_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, in, out, (long long)outer, (long long)n_sum, (long long)inner);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+extern "C" int cxm_shot_noise_approx_f32(void* stream, const float* image, const float* normal,
+                                         float* out, int64_t n) {
+  if (n < 0) return fail(CXM_ERR_INVALID, "negative size");
+  if (n == 0) return CXM_OK;
+  if (!image || !normal || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  if ((n + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "too many elements for one launch");
+  hipLaunchKernelGGL(shot_noise_approx_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, image, normal, out, (long long)n);
   CXM_LAUNCH_CHECK();
   return CXM_OK;
 }

@@ -61,7 +61,8 @@ struct RowArgs {
   int in_off, in_len;    // FFT index of input column 0, valid input columns
   int out_off, out_len;  // FFT indices [out_off, out_off+out_len) are stored
   float scale;
-  int intensity;
+  int intensity;         // store: 0 complex, 1 |u|^2, 2 real part (float32 out_i)
+  int in_real;           // input rows are float32 (imaginary part 0)
   // Tiled planes (strips of kTileW columns stored one after another, see
   // cxm_prop2d_kernels.h): element (y, x) at (x / 4) * strip_stride + y * 4 + x % 4.
   // Only with 4 rows per workgroup, rows_per_plane % 4 == 0 and the full
@@ -80,6 +81,9 @@ struct ColArgs {
   long long strip_stride;    // elements between strips of C columns (0: C, untiled)
   long long h_strip_stride;  // CXM_H_ARRAY: elements between strips (0: C)
   int h_row_stride;          // CXM_H_ARRAY: elements between rows (0: ncols)
+  int h_planes;              // CXM_H_ARRAY: planes of H (0 or 1: one for all input planes;
+                             //   else input plane p uses H plane p % h_planes)
+  long long h_plane_stride;  // CXM_H_ARRAY: elements between H planes
   int ncols;
   int n_planes;          // input planes in this launch
   int plane0;            // global index of the first input plane (param lookup)

@@ -66,6 +66,25 @@ CXM_DEV void lds_to_tiles(const float2* lds, float2* base, long long strip_strid
   }
 }
 
+// Element xs of an input row: complex64, or float32 with imaginary part 0.
+CXM_DEV float2 row_load(const RowArgs& a, const float2* row, int xs) {
+  if (a.in_real) return make_float2(((const float*)row)[xs], 0.f);
+  return row[xs];
+}
+
+// Input row r of plane p (complex64 or float32 elements).
+CXM_DEV const float2* row_ptr(const RowArgs& a, int p, int r) {
+  const long long off = p * a.in_plane_stride + (long long)r * a.in_row_stride;
+  return a.in_real ? (const float2*)((const float*)a.in + off) : a.in + off;
+}
+
+// Store of one output value: complex, |u|^2 or the real part.
+CXM_DEV void row_store(const RowArgs& a, long long idx, float2 w) {
+  if (a.intensity == 1) a.out_i[idx] = cabs2(w);
+  else if (a.intensity == 2) a.out_i[idx] = w.x;
+  else a.out_c[idx] = w;
+}
+
 template <int N, int E, int DIR>
 __global__ void __launch_bounds__((RowCfg<N, E>::THREADS))
 row_fft_kernel(RowArgs a) {
@@ -101,11 +120,11 @@ row_fft_kernel(RowArgs a) {
     }
   }
   if (!loaded) {
-    const float2* src = a.in + p * a.in_plane_stride + (long long)r * a.in_row_stride;
+    const float2* src = row_ptr(a, p, r);
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const int xs = t + k * T - a.in_off;
-      v[k] = (ok && xs >= 0 && xs < a.in_len) ? src[xs] : make_float2(0.f, 0.f);
+      v[k] = (ok && xs >= 0 && xs < a.in_len) ? row_load(a, src, xs) : make_float2(0.f, 0.f);
     }
   }
   fft<N, E, DIR, 1, SYNC>(v, st, t, a.tw);
@@ -131,9 +150,7 @@ row_fft_kernel(RowArgs a) {
   for (int k = 0; k < E; ++k) {
     const int xo = t + k * T - a.out_off;
     if (xo >= 0 && xo < a.out_len) {
-      const float2 w = cscale(v[k], a.scale);
-      if (a.intensity) a.out_i[obase + xo] = cabs2(w);
-      else a.out_c[obase + xo] = w;
+      row_store(a, obase + xo, cscale(v[k], a.scale));
     }
   }
 }
@@ -178,12 +195,14 @@ row_fft_loop_kernel(RowArgs a) {
       const bool ok = row < a.n_rows;
       const int p = ok ? row / a.rows_per_plane : 0;
       const int r = ok ? row - p * a.rows_per_plane : 0;
-      const float2* src = a.in + p * a.in_plane_stride + (long long)r * a.in_row_stride;
+      const float2* src = row_ptr(a, p, r);
 #pragma unroll
       for (int k = 0; k < E; k += 2) {
         const int x0 = t + k * T - a.in_off, x1 = x0 + T;
-        const float2 u0 = (ok && x0 >= 0 && x0 < a.in_len) ? src[x0] : make_float2(0.f, 0.f);
-        const float2 u1 = (ok && x1 >= 0 && x1 < a.in_len) ? src[x1] : make_float2(0.f, 0.f);
+        const float2 u0 =
+            (ok && x0 >= 0 && x0 < a.in_len) ? row_load(a, src, x0) : make_float2(0.f, 0.f);
+        const float2 u1 =
+            (ok && x1 >= 0 && x1 < a.in_len) ? row_load(a, src, x1) : make_float2(0.f, 0.f);
         pf[k / 2] = make_float4(u0.x, u0.y, u1.x, u1.y);
       }
     }
@@ -246,9 +265,7 @@ row_fft_loop_kernel(RowArgs a) {
         for (int k = 0; k < E; ++k) {
           const int xo = t + k * T - a.out_off;
           if (xo >= 0 && xo < a.out_len) {
-            const float2 w = cscale(v[k], a.scale);
-            if (a.intensity) a.out_i[obase + xo] = cabs2(w);
-            else a.out_c[obase + xo] = w;
+            row_store(a, obase + xo, cscale(v[k], a.scale));
           }
         }
       }
@@ -330,7 +347,8 @@ col_prop_kernel(ColArgs a) {
     } else if constexpr (KIND == CXM_H_ARRAY) {
       const long long hss = a.h_strip_stride ? a.h_strip_stride : C;
       const int hrs = a.h_row_stride ? a.h_row_stride : a.ncols;
-      const float2* hcol = a.H + (long long)(col0 / C) * hss + c;
+      const float2* hcol = a.H + (long long)(col0 / C) * hss + c +
+                           (a.h_planes > 1 ? (long long)(pg % a.h_planes) * a.h_plane_stride : 0);
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const float2 h = colok ? hcol[(unsigned)((ty + k * T) * hrs)] : make_float2(0.f, 0.f);

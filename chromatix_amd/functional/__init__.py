@@ -22,3 +22,4 @@ from .samples import (  # noqa: F401,E402
     multislice_thick_sample,
     thin_sample,
 )
+from .sensors import basic_sensor  # noqa: F401,E402

@@ -1,0 +1,154 @@
+"""Fourier convolution and shot noise on libcxm (mirrors chromatix.ops).
+
+``fourier_convolution`` (src/chromatix/ops/ops.py:9-86) runs as one library
+call (cxm_fconv_c64): the kernel spectra once, then zero-padded forward
+transforms, the spectral product and the inverse transforms on the
+propagation passes.  The transform lengths are the implemented FFT lengths
+at or above n + k - 1 per axis (the linear convolution); the reference's
+n + k or power-of-two shapes give the same linear convolution, so results
+agree to float32 rounding, and the "full" output keeps the reference's
+shape (zeros past the linear convolution).
+
+``approximate_shot_noise`` / ``shot_noise`` (src/chromatix/ops/noise.py:
+9-45) take a ``torch.Generator`` or an integer seed in place of a JAX
+PRNG key: the distributions are the reference's, the samples are torch's.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+__all__ = ["fourier_convolution", "approximate_shot_noise", "shot_noise", "next_order"]
+
+
+def next_order(val: int) -> int:
+    """Smallest power of two >= val (src/chromatix/utils/utils.py:11-12)."""
+    return int(2 ** math.ceil(math.log2(val)))
+
+
+def _kernel_planes(kb: tuple[int, ...], ob: tuple[int, ...]) -> int | None:
+    """Kernel volumes when output volume b can use kernel volume b % k:
+    kernel batch dims equal the output's over a trailing run and are 1
+    before it.  None when the kernel must be broadcast explicitly."""
+    kb = (1,) * (len(ob) - len(kb)) + tuple(kb)
+    for s in range(len(ob) + 1):  # s: first dim of the trailing run
+        if all(d == 1 for d in kb[:s]) and tuple(kb[s:]) == tuple(ob[s:]):
+            return int(np.prod(ob[s:], dtype=np.int64))
+    return None
+
+
+def fourier_convolution(image, kernel, *, axes=(0, 1), fast_fft_shape: bool = True,
+                        mode: str = "same") -> torch.Tensor:
+    """Convolution of ``image`` with ``kernel`` over two or three contiguous
+    ``axes`` by the Fourier convolution theorem, zero-padded against
+    circular wrap.  ``mode`` "same" crops to the image's shape (window
+    starting at (k - 1) // 2 per axis, as the reference), "full" returns the
+    reference's transform shape (n + k per axis, rounded up to a power of two
+    when ``fast_fft_shape``).  Real inputs give a float32 result (the
+    reference's rfftn / irfftn), complex ones complex64.  Dimensions outside
+    ``axes`` broadcast; in "same" mode they are cropped to the image's, as
+    the reference's slices do."""
+    image = torch.as_tensor(image)
+    L.ensure_device(image)
+    kernel = torch.as_tensor(kernel, device=image.device)
+    if image.ndim != kernel.ndim:
+        raise ValueError(f"Input ({image.ndim}D) and kernel ({kernel.ndim}D) must have same "
+                         "number of dimensions")
+    if mode not in ("same", "full"):
+        raise ValueError(f"mode must be 'same' or 'full', got {mode!r}")
+    nd = image.ndim
+    axes = [a + nd if a < 0 else a for a in axes]
+    for i in range(len(axes) - 1):
+        if axes[i + 1] != axes[i] + 1:
+            raise ValueError("Axes to convolve over must be contiguous")
+    na = len(axes)
+    if na not in (2, 3):
+        raise NotImplementedError("fourier_convolution: convolution over two or three axes")
+    cplx = image.is_complex() or kernel.is_complex()
+    dt = torch.complex64 if cplx else torch.float32
+    perm = [i for i in range(nd) if i not in axes] + axes
+    img = image.permute(perm).to(dt)
+    ker = kernel.permute(perm).to(dt)
+    ib, kb = tuple(img.shape[:-na]), tuple(ker.shape[:-na])
+    isp, ksp = tuple(img.shape[-na:]), tuple(ker.shape[-na:])
+    bshape = tuple(torch.broadcast_shapes(ib, kb))
+    if mode == "same":
+        # the reference slices every non-convolved dim to the image's length
+        ob = ib
+        ker = ker[tuple(slice(0, i) if k > 1 else slice(None) for i, k in zip(ib, kb))]
+        kb = tuple(ker.shape[:-na])
+        out_sp = isp
+    else:
+        ob = bshape
+        padded = [i + k for i, k in zip(isp, ksp)]
+        out_sp = tuple(next_order(p) for p in padded) if fast_fft_shape else tuple(padded)
+    if ib != ob:
+        img = img.expand(ob + isp)
+    k_planes = _kernel_planes(kb, ob)
+    if k_planes is None:
+        ker = ker.expand(ob + ksp)
+        k_planes = int(np.prod(ob, dtype=np.int64))
+    img = img.contiguous()
+    ker = ker.contiguous()
+    batch = int(np.prod(ob, dtype=np.int64))
+    d = L.FconvDesc()
+    d.batch = batch
+    (d.depth, d.height, d.width) = (1,) * (3 - na) + isp
+    d.k_planes = k_planes
+    (d.k_depth, d.k_height, d.k_width) = (1,) * (3 - na) + ksp
+    d.real_input = 0 if cplx else 1
+    d.mode = 0 if mode == "same" else 1
+    (d.out_depth, d.out_height, d.out_width) = (1,) * (3 - na) + out_sp
+    lib = L.lib()
+    ws_bytes = lib.cxm_fconv_workspace_size(d)
+    if ws_bytes == 0:
+        L.check(lib.cxm_fconv_c64(None, None, None, None, None, 0, d))
+    if mode == "full":
+        # entries past the transform used here are not written
+        out = torch.zeros(ob + out_sp, dtype=dt, device=img.device)
+    else:
+        out = torch.empty(ob + out_sp, dtype=dt, device=img.device)
+    if batch > 0:
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=img.device)
+        L.check(lib.cxm_fconv_c64(L.stream_ptr(img.device), img.data_ptr(), ker.data_ptr(),
+                                  out.data_ptr(), ws.data_ptr(), ws_bytes, d))
+    inv = [0] * nd
+    for i, p in enumerate(perm):
+        inv[p] = i
+    return out.permute(inv)
+
+
+def _generator(key, device) -> torch.Generator:
+    if isinstance(key, torch.Generator):
+        return key
+    if key is None:
+        raise ValueError("Must provide a PRNG key to generate noise")
+    g = torch.Generator(device=device)
+    g.manual_seed(int(key))
+    return g
+
+
+def approximate_shot_noise(key, image) -> torch.Tensor:
+    """max(image + sqrt(image) * N(0, 1), 0) (noise.py:9-16); ``key`` a
+    torch.Generator on the image's device or an integer seed."""
+    x = torch.as_tensor(image)
+    L.ensure_device(x)
+    x = x.to(torch.float32).contiguous()
+    normal = torch.randn(x.shape, generator=_generator(key, x.device), device=x.device,
+                         dtype=torch.float32)
+    out = torch.empty_like(x)
+    L.check(L.lib().cxm_shot_noise_approx_f32(L.stream_ptr(x.device), x.data_ptr(),
+                                              normal.data_ptr(), out.data_ptr(), x.numel()))
+    return out
+
+
+def shot_noise(key, image) -> torch.Tensor:
+    """Poisson samples with mean ``image``, as float32 (noise.py:38-45)."""
+    x = torch.as_tensor(image)
+    L.ensure_device(x)
+    x = x.to(torch.float32)
+    return torch.poisson(x, generator=_generator(key, x.device))

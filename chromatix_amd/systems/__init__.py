@@ -1,2 +1,2 @@
-"""System-level caller of the PSF path (mirrors src/chromatix/systems/microscopes.py)."""
-from .microscopes import Optical4FSystemPSF  # noqa: F401
+"""System-level callers (mirrors src/chromatix/systems/microscopes.py)."""
+from .microscopes import Microscope, Optical4FSystemPSF  # noqa: F401

@@ -1,9 +1,15 @@
-"""Optical4FSystemPSF (src/chromatix/systems/microscopes.py:300-365) on the
-fused libcxm PSF kernels.
+"""Microscope (src/chromatix/systems/microscopes.py:20-297) and
+Optical4FSystemPSF (microscopes.py:300-365) on libcxm.
 
-The reference module takes a ``Microscope`` for the objective's focal
-length ``f``, ``n``, ``NA``, ``spectrum`` and ``padding_ratio``; any object
-with those attributes works here.
+``Microscope.image`` convolves the sample with the PSF intensity
+(``ops.fourier_convolution``, one library call) and hands the image to the
+sensor without resampling.  ``Microscope.__call__`` computes the PSF,
+crops its padding and passes it to the sensor's ``resample``; tapering the
+PSF edges (``taper_width`` > 0) and resampling to a different pitch are not
+implemented.
+
+``Optical4FSystemPSF`` takes any object with the objective's ``f``, ``n``,
+``NA``, ``spectrum`` and ``padding_ratio`` (a ``Microscope`` has them).
 """
 from __future__ import annotations
 
@@ -13,6 +19,76 @@ import torch
 from ..core.field import ScalarField
 from ..core.spectrum import MonoSpectrum, Spectrum
 from ..functional.psf import psf_4f
+from ..ops import fourier_convolution
+
+
+class Microscope:
+    """Microscope with a planewise spatially invariant PSF: images are the
+    sample convolved with the PSF intensity, seen by ``sensor``.
+
+    system_psf(microscope, *args, **kwargs) returns a Field or an intensity
+    tensor; ``convolution_axes`` (-2, -1) convolves plane by plane, (-3, -2,
+    -1) in 3D; ``fast_fft_shape`` is accepted for the reference's interface
+    (the transform lengths here are the implemented FFT lengths at or above
+    the linear-convolution size either way)."""
+
+    def __init__(self, system_psf, sensor, f, n, NA, spectrum, padding_ratio=0, taper_width=0,
+                 convolution_axes=(-2, -1), fast_fft_shape=True):
+        self.system_psf = system_psf
+        self.sensor = sensor
+        self.f = f
+        self.n = n
+        self.NA = NA
+        self.spectrum = Spectrum.build(spectrum)
+        self.padding_ratio = padding_ratio
+        self.taper_width = taper_width
+        self.convolution_axes = tuple(convolution_axes)
+        self.fast_fft_shape = fast_fft_shape
+
+    def __call__(self, sample, *args, key=None, **kwargs):
+        """PSF for ``args`` / ``kwargs``, then ``image(sample, psf)``."""
+        system_psf = self.psf(*args, **kwargs)
+        psf = self._process_psf(system_psf)
+        return self.image(sample, psf, key=key)
+
+    def psf(self, *args, **kwargs):
+        """PSF of the system (microscopes.py:225-227)."""
+        return self.system_psf(self, *args, **kwargs)
+
+    def _process_psf(self, system_psf):
+        """Intensity, padding crop and resampling to the sensor
+        (microscopes.py:229-269)."""
+        if isinstance(system_psf, ScalarField):
+            psf = system_psf.intensity
+            spacing = system_psf.central_rectangular_dx
+        else:
+            psf = torch.as_tensor(system_psf)
+            spacing = None
+        shape = tuple(psf.shape[-2:])
+        if self.padding_ratio:
+            unpadded = tuple(int(s / (1.0 + self.padding_ratio)) for s in shape)
+            padding = tuple(s - u for s, u in zip(shape, unpadded))
+        else:
+            unpadded, padding = shape, (0, 0)
+        if spacing is None:
+            spacing = np.asarray(self.sensor.spacing, np.float64) * np.array(
+                [self.sensor.shape[0] / (shape[0] - padding[0]),
+                 self.sensor.shape[1] / (shape[1] - padding[1])])
+        if self.padding_ratio and self.padding_ratio > 0:
+            # center_crop by padding // 2 on each side (utils.py:26-41)
+            cy, cx = padding[0] // 2, padding[1] // 2
+            psf = psf[..., cy: shape[0] - cy, cx: shape[1] - cx]
+        if self.taper_width > 0:
+            raise NotImplementedError("Microscope: PSF edge tapering is not implemented")
+        return self.sensor.resample(psf, spacing)
+
+    def image(self, sample, psf, key=None):
+        """Image of ``sample`` through ``psf`` (both already at the sensor's
+        pixels): Fourier convolution over ``convolution_axes``, then the
+        sensor without resampling (microscopes.py:271-297)."""
+        image = fourier_convolution(sample, psf, axes=self.convolution_axes,
+                                    fast_fft_shape=self.fast_fft_shape)
+        return self.sensor(image, self.sensor.spacing, resample=False, key=key)
 
 
 class Optical4FSystemPSF:

@@ -410,6 +410,55 @@ int cxm_multislice_fwd(void* stream, const void* in, const float* dn, const floa
                        const cxm_multislice_desc* d);
 
 /*
+ * Fourier convolution over the last two or three axes (fourier_convolution,
+ * src/chromatix/ops/ops.py:9-86): out = ifftn(fftn(image) * fftn(kernel))
+ * with both zero-padded to at least (depth + k_depth - 1) x (height +
+ * k_height - 1) x (width + k_width - 1) (the next implemented FFT lengths:
+ * the linear convolution, which the reference's larger transform shapes
+ * also give), then
+ *   mode 0 ("same"): along each axis the window [(k - 1) / 2, + n) of the
+ *                    linear convolution (k kernel, n image length; integer
+ *                    halves rounded down);
+ *   mode 1 ("full"): the window [0, out_n) of the reference's transform
+ *                    shape; entries beyond the transform used here are not
+ *                    written (the caller zeroes them).
+ * depth == k_depth == 1 is the two-axis convolution.  Image volume b uses
+ * kernel volume b % k_planes.  real_input: image, kernel and out are float32
+ * and out is the real part (the reference's rfftn / irfftn); otherwise
+ * complex64.  Layouts are C-contiguous (batch, depth, height, width).
+ */
+typedef struct cxm_fconv_desc {
+  int32_t batch;                     /* image volumes */
+  int32_t depth, height, width;      /* image volume */
+  int32_t k_planes;                  /* kernel volumes */
+  int32_t k_depth, k_height, k_width;
+  int32_t real_input;
+  int32_t mode;                      /* 0 same, 1 full */
+  int32_t out_depth, out_height, out_width;  /* full mode output volume */
+} cxm_fconv_desc;
+
+size_t cxm_fconv_workspace_size(const cxm_fconv_desc* d);
+int cxm_fconv_c64(void* stream, const void* image, const void* kernel, void* out, void* workspace,
+                  size_t workspace_bytes, const cxm_fconv_desc* d);
+
+/*
+ * Sum over the middle axis of an (outer, n_sum, inner) float32 array into
+ * (outer, inner), in index order (the sensor's reduce_axis,
+ * src/chromatix/functional/sensors.py:79-80).
+ */
+int cxm_sum_axis_f32(void* stream, const float* in, float* out, int64_t outer, int64_t n_sum,
+                     int64_t inner);
+
+/*
+ * Approximate shot noise (approximate_shot_noise, src/chromatix/ops/noise.py:10-16):
+ *   out = max(image + sqrt(image) * normal, 0)
+ * with the standard-normal samples supplied by the caller.  in == out is
+ * allowed.
+ */
+int cxm_shot_noise_approx_f32(void* stream, const float* image, const float* normal, float* out,
+                              int64_t n);
+
+/*
  * Fluorescent multislice (fluorescent_multislice_thick_sample,
  * src/chromatix/functional/samples.py:285-441), one Monte-Carlo sample on
  * one padded plane.  `state` (Ny x Nx complex64, Ny = height + 2 pad_y,

@@ -1,0 +1,199 @@
+"""GPU parity of fourier_convolution (cxm_fconv_c64), the sensor kernels
+(cxm_sum_axis_f32, cxm_shot_noise_approx_f32) and Microscope.image against
+the oracle (FP64).  Bar: relative L2 <= 1e-5; the reference tests' shapes
+also meet their own criterion (allclose to scipy, atol 1e-3)."""
+import numpy as np
+import pytest
+import scipy.signal
+import torch
+
+import chromatix_amd.functional as cf
+from chromatix_amd.elements import BasicSensor
+from chromatix_amd.ops import approximate_shot_noise, fourier_convolution, shot_noise
+from chromatix_amd.systems import Microscope
+from oracle import convolution as oc
+from tests.conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _arr(rng, shape, cplx=False):
+    x = rng.standard_normal(shape)
+    if cplx:
+        x = x + 1j * rng.standard_normal(shape)
+        return x.astype(np.complex64)
+    return x.astype(np.float32)
+
+
+def _gpu(a, k, **kw):
+    return fourier_convolution(torch.as_tensor(a).cuda(), torch.as_tensor(k).cuda(), **kw).cpu().numpy()
+
+
+@pytest.mark.parametrize("shapes,fast", [
+    (((512, 512), (256, 256)), True),     # tests/test_convolution.py:8-16
+    (((513, 513), (253, 253)), False),    # tests/test_convolution.py:30-40
+    (((37, 53), (9, 12)), True),
+    (((64, 64), (64, 64)), True),
+    (((30, 100), (101, 7)), False),
+    (((1, 40), (1, 5)), True),
+])
+def test_same_real(shapes, fast):
+    rng = np.random.default_rng(0)
+    (H, W), (h, w) = shapes
+    a, k = _arr(rng, (H, W)), _arr(rng, (h, w))
+    print("fconv same real", shapes, fast, flush=True)
+    out = _gpu(a, k, fast_fft_shape=fast)
+    ref = oc.fourier_convolution(a, k, fast_fft_shape=fast)
+    assert out.dtype == np.float32 and out.shape == (H, W)
+    assert rel_l2(out, ref) < TOL
+    if H >= 512:
+        sp = scipy.signal.convolve(a.astype(np.float64), k.astype(np.float64), method="fft", mode="same")
+        assert np.allclose(sp, out, atol=1e-3)
+
+
+@pytest.mark.parametrize("ci,ck", [(True, False), (False, True), (True, True)])
+def test_same_complex(ci, ck):
+    rng = np.random.default_rng(1)
+    a, k = _arr(rng, (3, 70, 45), ci), _arr(rng, (3, 11, 8), ck)
+    out = _gpu(a, k, axes=(-2, -1))
+    ref = oc.fourier_convolution(a, k, axes=(-2, -1))
+    assert out.dtype == np.complex64
+    assert rel_l2(out, ref) < TOL
+
+
+@pytest.mark.parametrize("ishape,kshape", [
+    ((3, 40, 50), (1, 9, 9)),        # one kernel for every plane
+    ((3, 40, 50), (3, 9, 9)),        # one kernel per plane
+    ((2, 3, 40, 50), (1, 3, 7, 5)),  # kernel b % 3
+    ((2, 3, 40, 50), (2, 1, 7, 5)),  # explicit broadcast
+    ((1, 40, 50), (4, 9, 9)),        # same mode keeps kernel 0 (the reference's crop)
+])
+def test_batch_broadcast(ishape, kshape):
+    rng = np.random.default_rng(2)
+    a, k = _arr(rng, ishape), _arr(rng, kshape)
+    out = _gpu(a, k, axes=(-2, -1))
+    ref = oc.fourier_convolution(a, k, axes=(-2, -1))
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) < TOL
+
+
+@pytest.mark.parametrize("fast", [True, False])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_full_mode(fast, cplx):
+    rng = np.random.default_rng(3)
+    a, k = _arr(rng, (2, 33, 60), cplx), _arr(rng, (1, 16, 5), cplx)
+    out = _gpu(a, k, axes=(1, 2), mode="full", fast_fft_shape=fast)
+    ref = oc.fourier_convolution(a, k, axes=(1, 2), mode="full", fast_fft_shape=fast)
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) < TOL
+    # past the linear convolution: zeros (to rounding)
+    assert np.abs(out[:, 33 + 16 - 1:, :]).max() < 1e-5 * np.abs(out).max()
+
+
+def test_default_axes_channels_last():
+    rng = np.random.default_rng(4)
+    a, k = _arr(rng, (40, 50, 3)), _arr(rng, (9, 7, 3))
+    out = _gpu(a, k)
+    ref = oc.fourier_convolution(a, k)
+    assert out.shape == (40, 50, 3)
+    assert rel_l2(out, ref) < TOL
+
+
+@pytest.mark.parametrize("ishape,kshape,mode,cplx", [
+    ((5, 512, 512), (3, 256, 256), "same", False),  # tests/test_convolution.py:19-27
+    ((5, 40, 50), (3, 7, 9), "same", True),
+    ((4, 30, 20), (6, 5, 5), "full", False),
+    ((2, 5, 32, 24), (1, 3, 8, 8), "same", False),
+    ((2, 5, 32, 24), (2, 3, 8, 8), "same", True),
+])
+def test_three_axes(ishape, kshape, mode, cplx):
+    rng = np.random.default_rng(5)
+    a, k = _arr(rng, ishape, cplx), _arr(rng, kshape, cplx)
+    ax = (-3, -2, -1)
+    print("fconv 3d", ishape, kshape, mode, flush=True)
+    out = _gpu(a, k, axes=ax, mode=mode)
+    ref = oc.fourier_convolution(a, k, axes=ax, mode=mode)
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) < TOL
+    if ishape == (5, 512, 512):
+        sp = scipy.signal.convolve(a.astype(np.float64), k.astype(np.float64), method="fft", mode="same")
+        assert np.allclose(sp, out, atol=1e-3)
+
+
+def test_empty_batch_and_errors():
+    z = fourier_convolution(torch.zeros(0, 8, 8, device="cuda"), torch.ones(1, 3, 3, device="cuda"),
+                            axes=(-2, -1))
+    assert tuple(z.shape) == (0, 8, 8)
+    x = torch.ones(8, 8, device="cuda")
+    with pytest.raises(ValueError):
+        fourier_convolution(x, torch.ones(1, 3, 3, device="cuda"))
+    with pytest.raises(ValueError):
+        fourier_convolution(torch.ones(2, 8, 8, device="cuda"), torch.ones(2, 3, 3, device="cuda"),
+                            axes=(0, 2))
+    with pytest.raises(ValueError):
+        fourier_convolution(x, x, mode="valid")
+    with pytest.raises(RuntimeError):
+        fourier_convolution(torch.ones(8, 8), torch.ones(3, 3))
+
+
+def test_sum_axis_and_approximate_noise():
+    rng = np.random.default_rng(6)
+    x = (rng.random((4, 7, 33, 20)) * 50).astype(np.float32)
+    xt = torch.as_tensor(x).cuda()
+    for ax in (0, 1, -1):
+        s = cf.basic_sensor(xt, reduce_axis=ax).cpu().numpy()
+        assert rel_l2(s, x.astype(np.float64).sum(axis=ax)) < 1e-6
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    out = approximate_shot_noise(g, xt).cpu().numpy()
+    g.manual_seed(11)
+    nrm = torch.randn(xt.shape, generator=g, device="cuda").cpu().numpy()
+    ref = oc.approximate_shot_noise(x, nrm)
+    assert rel_l2(out, ref) < 1e-6
+    assert (out >= 0).all()
+    # same seed, same samples through basic_sensor
+    s = cf.basic_sensor(xt, "approximate", reduce_axis=1, noise_key=5).cpu().numpy()
+    g.manual_seed(5)
+    nrm = torch.randn((4, 33, 20), generator=g, device="cuda").cpu().numpy()
+    ref = oc.basic_sensor(x, "approximate", reduce_axis=1, normal=nrm)
+    assert rel_l2(s, ref) < 1e-6
+
+
+def test_poisson_noise_statistics():
+    lam = torch.full((512, 512), 100.0, device="cuda")
+    y = shot_noise(3, lam)
+    assert y.dtype == torch.float32
+    m, v = float(y.mean()), float(y.var())
+    # 2^18 samples: standard errors of mean and variance ~0.02 and ~0.3
+    assert abs(m - 100.0) < 0.2 and abs(v - 100.0) < 3.0
+    assert torch.equal(y, y.round())
+
+
+def test_microscope_image_planewise():
+    rng = np.random.default_rng(7)
+    D, H, W = 4, 96, 80
+    sample = rng.random((D, H, W)).astype(np.float32)
+    psf = rng.random((D, 31, 31)).astype(np.float32)
+    sensor = BasicSensor((H, W), 0.5, reduce_axis=0)
+    m = Microscope(lambda mic: psf, sensor, f=5.0, n=1.33, NA=0.8, spectrum=0.532)
+    img = m.image(torch.as_tensor(sample).cuda(), torch.as_tensor(psf).cuda()).cpu().numpy()
+    ref = oc.microscope_image(sample, psf, reduce_axis=0)
+    assert img.shape == (H, W)
+    assert rel_l2(img, ref) < TOL
+
+
+def test_microscope_call_crops_padding():
+    rng = np.random.default_rng(8)
+    H, W = 64, 64
+    sample = rng.random((H, W)).astype(np.float32)
+    padded = rng.random((80, 80)).astype(np.float32)  # padding_ratio 0.25: 64 -> 80
+
+    def system_psf(mic):
+        return torch.as_tensor(padded).cuda()
+
+    sensor = BasicSensor((H, W), 0.5, resampling_method=None)
+    m = Microscope(system_psf, sensor, f=5.0, n=1.33, NA=0.8, spectrum=0.532, padding_ratio=0.25)
+    img = m(torch.as_tensor(sample).cuda()).cpu().numpy()
+    ref = oc.fourier_convolution(sample, padded[8:72, 8:72], axes=(-2, -1))
+    assert rel_l2(img, ref) < TOL
