@@ -45,11 +45,11 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--config", default="m0",
-                   choices=["m0", "c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8"],
-                   help="m0: the headline batched 1024^2 AS propagation (default); c1-c8: the "
+                   choices=["m0", "c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8", "c9"],
+                   help="m0: the headline batched 1024^2 AS propagation (default); c1-c9: the "
                         "other workloads (plain AS with padding, 4f PSF + gradient, multislice + "
                         "adjoint, Born solves, serving, scaled-convolution propagation, high-NA "
-                        "lens, fluorescent multislice), one JSON line each")
+                        "lens, fluorescent multislice, Microscope imaging), one JSON line each")
     p.add_argument("--summarize", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
                    help="write the rocprofv3 summaries of a profiled run (kernel-trace --stats, "
                         "--pmc FETCH_SIZE, --pmc WRITE_SIZE; csv output) and exit")
@@ -531,8 +531,51 @@ def bench_c8(args):
                    per, kern, None, {"ms_per_sample_slice": ms / (S * D)})
 
 
+def bench_c9(args):
+    """Microscope imaging of a 128-plane volume (the reference's single-GPU
+    imaging example, examples/parallel_imaging.py:10-57): Optical4FSystemPSF
+    at 1536^2 (spacing 0.3 um, zero phase mask, tube lens f 100 um), 128
+    defocus planes z in linspace(-4, 4) um, objective f 100 um, n 1.33,
+    NA 0.8, lambda 0.532 um; a volume of ones convolved plane by plane with
+    the PSF intensity, summed over the planes by the sensor."""
+    import torch
+
+    from chromatix_amd import _lib as L
+    from chromatix_amd.elements import BasicSensor
+    from chromatix_amd.systems import Microscope, Optical4FSystemPSF
+
+    D, N = 128, 1536
+    shape = (N, N)
+    mic = Microscope(Optical4FSystemPSF(shape, 0.3, 100.0, torch.zeros(shape, device="cuda")),
+                     BasicSensor(shape, 0.3, resampling_method=None, reduce_axis=0),
+                     f=100.0, n=1.33, NA=0.8, spectrum=0.532)
+    volume = torch.ones((D, N, N), device="cuda")
+    z = np.linspace(-4, 4, D)
+    step = lambda: mic(volume, z)
+    steps, warm = 5, 1
+    ms = _timed(torch, step, steps, warm)
+    psf = mic._process_psf(mic.psf(z))
+    img_step = lambda: mic.image(volume, psf)
+    ms_img = _timed(torch, img_step, steps, warm)
+    kern = _kernels(torch, L, step, 1)
+    # fourier_convolution per plane (H = W = 1536, L = 3072 transform):
+    # kernel spectrum (row pass 4HW + 8HL, column pass 8HL + 8LL), image
+    # (row 4HW + 8HL, column 8HL + spectrum 8LL + 8HL, inverse row 8HL + 4HW)
+    # and the plane sum (4HW)
+    Lc = 3072
+    conv_bytes = D * (16 * N * N + 48 * N * Lc + 16 * Lc * Lc)
+    return _result(args, "Microscope images/s (128 x 1536^2 4f PSF + planewise Fourier convolution "
+                         "+ plane sum)", 1.0 / (ms * 1e-3), "images/s", ms, steps, warm, "float32",
+                   f"Microscope(Optical4FSystemPSF {N}^2, {D} planes) on a {D}x{N}^2 volume, "
+                   "BasicSensor reduce_axis 0", conv_bytes, kern, None,
+                   {"ms_per_image_conv_and_sensor": ms_img,
+                    "conv_roofline_frac": conv_bytes / (ms_img * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                    "reference_published_ms": {"value": 172.86, "hardware": "1x NVIDIA A100",
+                                               "source": "docs/parallelism.md:182"}})
+
+
 CONFIGS = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
-           "c6": bench_c6, "c7": bench_c7, "c8": bench_c8}
+           "c6": bench_c6, "c7": bench_c7, "c8": bench_c8, "c9": bench_c9}
 
 
 def main():

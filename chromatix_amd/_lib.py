@@ -315,6 +315,11 @@ def _declare(lib):
         "cxm_mbs_precondition": (i32, [vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_iterate": (i32, [vp, vp, vp, vp, ctypes.c_int32, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_output": (i32, [vp, vp, vp, vp, ctypes.POINTER(MbsDesc)]),
+        "cxm_mbs_bicgstab_state_size": (sz, [ctypes.POINTER(MbsDesc)]),
+        "cxm_mbs_bicgstab_init": (i32, [vp, vp, vp, vp, vp, sz, ctypes.c_double, ctypes.c_double, vp,
+                                        vp, sz, ctypes.POINTER(MbsDesc)]),
+        "cxm_mbs_bicgstab_iterate": (i32, [vp, vp, vp, vp, sz, ctypes.c_int32, vp, vp, sz,
+                                           ctypes.POINTER(MbsDesc)]),
         "cxm_czt_axis_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(CztDesc)]),
         "cxm_fluor_multislice_workspace_size": (sz, [ctypes.POINTER(FluorDesc)]),
         "cxm_fluor_multislice_sample_c64": (i32, [vp, vp, vp, vp, vp, vp, vp, sz,

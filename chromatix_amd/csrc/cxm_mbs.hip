@@ -255,6 +255,333 @@ int fft3(const MbsGeom& g, float2* v, int dir, hipStream_t s) {
 
 C2 c2(double re, double im) { return C2{(float)re, (float)im}; }
 
+// ---- BiCGStab on the preconditioned system ---------------------------------
+// Solves prec_forward(x) = prec_y with the stabilised bi-conjugate gradient
+// iteration of jax.scipy.sparse.linalg.bicgstab (no preconditioner M; the
+// solver calls it as spa.bicgstab(prec_forward, prec_y, x, maxiter, tol),
+// solver.py:141-143).  With A = prec_forward, b = prec_y, per iteration:
+//   rho' = <rh, r>, beta = rho'/rho alpha/omega, p = r + beta (p - omega q),
+//   q = A p, alpha' = rho' / <rh, q>, s = r - alpha' q,
+//   early = |s|^2 < atol2, t = A s, omega' = <t, s> / <t, t>,
+//   x += alpha' p (+ omega' s unless early), r = s (- omega' t unless early),
+//   k = -11 if omega' == 0 or alpha' == 0, -10 if rho' == 0, else k + 1,
+// while |r|^2 > atol2 = max(tol^2 |b|^2, atol^2) and k >= 0 (<a, b> conjugates
+// a).  The scalars live on the device (FP64), so chunks of iterations are
+// enqueued without host round trips; once stopped, every kernel of a later
+// iteration leaves the vectors untouched.
+struct BiScal {
+  double rho[2], alpha[2], omega[2];  // values of the last completed iteration
+  double beta[2], rho_n[2], alpha_n[2], omega_n[2];  // this iteration
+  double rr[2];                       // <rh, r> of the current r
+  double rs, atol2, bs, k, early, done;
+};
+
+CXM_DEV void cmul_d(const double* a, const double* b, double* o) {
+  const double re = a[0] * b[0] - a[1] * b[1], im = a[0] * b[1] + a[1] * b[0];
+  o[0] = re;
+  o[1] = im;
+}
+CXM_DEV void cdiv_d(const double* a, const double* b, double* o) {
+  const double d = b[0] * b[0] + b[1] * b[1];
+  const double re = (a[0] * b[0] + a[1] * b[1]) / d, im = (a[1] * b[0] - a[0] * b[1]) / d;
+  o[0] = re;
+  o[1] = im;
+}
+CXM_DEV float2 cf(const double* a) { return make_float2((float)a[0], (float)a[1]); }
+CXM_DEV float2 cmul2(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+CXM_DEV float2 cadd2(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+CXM_DEV float2 csub2(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// conj(a) b accumulated in FP64
+CXM_DEV void cdot_acc(float2 a, float2 b, double& re, double& im) {
+  re += (double)a.x * b.x + (double)a.y * b.y;
+  im += (double)a.x * b.y - (double)a.y * b.x;
+}
+// V factor at voxel i: s eps - kb
+CXM_DEV float2 vfac(const float2* eps, long long i, C2 s, C2 kb) {
+  const float2 e = eps[i];
+  return make_float2(s.re * e.x - s.im * e.y - kb.re, s.re * e.y + s.im * e.x - kb.im);
+}
+
+// Block sums of NV doubles to partial[block * 4 + j] (fixed order).
+template <int NV>
+CXM_DEV void block_sums(double (&v)[NV], double* partial) {
+  __shared__ double sh[NV][kRedThreads];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] = v[j];
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) partial[blockIdx.x * 4 + j] = sh[j][0];
+  }
+}
+
+#define BI_LOOP(i) \
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < vox; \
+       i += (long long)gridDim.x * blockDim.x)
+
+// r = b - A x = b + V(g + x); rh = p = q = r; partials |r|^2, |b|^2
+__global__ void __launch_bounds__(kRedThreads)
+bi_r0_kernel(const float2* __restrict__ g, const float2* __restrict__ eps, const float2* __restrict__ b,
+             const float2* __restrict__ x, float2* __restrict__ r, float2* __restrict__ rh,
+             float2* __restrict__ p, float2* __restrict__ q, long long vox, C2 s, C2 kb,
+             double* __restrict__ partial) {
+  double acc[2] = {0.0, 0.0};
+  BI_LOOP(i) {
+    const float2 f = vfac(eps, i, s, kb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * vox + i;
+      const float2 bv = b[j];
+      const float2 rv = cadd2(bv, cmul2(cadd2(g[j], x[j]), f));
+      r[j] = rv;
+      rh[j] = rv;
+      p[j] = rv;
+      q[j] = rv;
+      acc[0] += (double)rv.x * rv.x + (double)rv.y * rv.y;
+      acc[1] += (double)bv.x * bv.x + (double)bv.y * bv.y;
+    }
+  }
+  block_sums<2>(acc, partial);
+}
+
+// partial sums of NV values -> out[0..NV) (one block, fixed order)
+template <int NV>
+CXM_DEV void sum_partials(const double* partial, double (&out)[NV]) {
+  __shared__ double sh[NV][kRedThreads];
+  double a[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) a[j] = 0.0;
+  for (int b = threadIdx.x; b < kRedBlocks; b += kRedThreads) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) a[j] += partial[b * 4 + j];
+  }
+#pragma unroll
+  for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] = a[j];
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < NV; ++j) out[j] = sh[j][0];
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+bi_init_scalars_kernel(const double* __restrict__ partial, BiScal* st, double tol, double atol,
+                       double* __restrict__ status) {
+  double v[2];
+  sum_partials<2>(partial, v);
+  if (threadIdx.x != 0) return;
+  st->rho[0] = st->alpha[0] = st->omega[0] = 1.0;
+  st->rho[1] = st->alpha[1] = st->omega[1] = 0.0;
+  st->rs = v[0];
+  st->bs = v[1];
+  st->rr[0] = v[0];  // rh = r
+  st->rr[1] = 0.0;
+  st->atol2 = fmax(tol * tol * v[1], atol * atol);
+  st->k = 0.0;
+  st->early = 0.0;
+  st->done = (st->rs > st->atol2) ? 0.0 : 1.0;
+  status[0] = st->rs;
+  status[1] = st->k;
+  status[2] = st->done;
+  status[3] = st->bs;
+}
+
+// rho' = <rh, r>, beta = rho'/rho alpha/omega
+__global__ void bi_beta_kernel(BiScal* st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || st->done != 0.0) return;
+  st->rho_n[0] = st->rr[0];
+  st->rho_n[1] = st->rr[1];
+  double t[2], u[2];
+  cdiv_d(st->rho_n, st->rho, t);
+  cdiv_d(st->alpha, st->omega, u);
+  cmul_d(t, u, st->beta);
+}
+
+// p = r + beta (p - omega q);  g = V(p)
+__global__ void __launch_bounds__(kRedThreads)
+bi_p_kernel(const float2* __restrict__ r, float2* __restrict__ p, const float2* __restrict__ q,
+            const float2* __restrict__ eps, float2* __restrict__ g, long long vox, C2 s, C2 kb,
+            const BiScal* st) {
+  if (st->done != 0.0) return;
+  const float2 beta = cf(st->beta), omega = cf(st->omega);
+  BI_LOOP(i) {
+    const float2 f = vfac(eps, i, s, kb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * vox + i;
+      const float2 pv = cadd2(r[j], cmul2(beta, csub2(p[j], cmul2(omega, q[j]))));
+      p[j] = pv;
+      g[j] = cmul2(pv, f);
+    }
+  }
+}
+
+// q = A p = -V(g + p); partial <rh, q>
+__global__ void __launch_bounds__(kRedThreads)
+bi_q_kernel(const float2* __restrict__ g, const float2* __restrict__ p, const float2* __restrict__ rh,
+            const float2* __restrict__ eps, float2* __restrict__ q, long long vox, C2 s, C2 kb,
+            const BiScal* st, double* __restrict__ partial) {
+  if (st->done != 0.0) return;
+  double acc[2] = {0.0, 0.0};
+  BI_LOOP(i) {
+    const float2 f = vfac(eps, i, s, kb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * vox + i;
+      const float2 v = cmul2(cadd2(g[j], p[j]), f);
+      const float2 qv = make_float2(-v.x, -v.y);
+      q[j] = qv;
+      cdot_acc(rh[j], qv, acc[0], acc[1]);
+    }
+  }
+  block_sums<2>(acc, partial);
+}
+
+// alpha' = rho' / <rh, q>
+__global__ void __launch_bounds__(kRedThreads)
+bi_alpha_kernel(const double* __restrict__ partial, BiScal* st) {
+  if (st->done != 0.0) return;
+  double v[2];
+  sum_partials<2>(partial, v);
+  if (threadIdx.x == 0) cdiv_d(st->rho_n, v, st->alpha_n);
+}
+
+// s = r - alpha' q (into r);  g = V(s);  partial |s|^2
+__global__ void __launch_bounds__(kRedThreads)
+bi_s_kernel(float2* __restrict__ r, const float2* __restrict__ q, const float2* __restrict__ eps,
+            float2* __restrict__ g, long long vox, C2 s, C2 kb, const BiScal* st,
+            double* __restrict__ partial) {
+  if (st->done != 0.0) return;
+  const float2 alpha = cf(st->alpha_n);
+  double acc[1] = {0.0};
+  BI_LOOP(i) {
+    const float2 f = vfac(eps, i, s, kb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * vox + i;
+      const float2 sv = csub2(r[j], cmul2(alpha, q[j]));
+      r[j] = sv;
+      g[j] = cmul2(sv, f);
+      acc[0] += (double)sv.x * sv.x + (double)sv.y * sv.y;
+    }
+  }
+  block_sums<1>(acc, partial);
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+bi_early_kernel(const double* __restrict__ partial, BiScal* st) {
+  if (st->done != 0.0) return;
+  double v[1];
+  sum_partials<1>(partial, v);
+  if (threadIdx.x == 0) st->early = (v[0] < st->atol2) ? 1.0 : 0.0;
+}
+
+// t = A s = -V(g + s) (into g); partials <t, s>, <t, t>
+__global__ void __launch_bounds__(kRedThreads)
+bi_t_kernel(float2* __restrict__ g, const float2* __restrict__ sv_, const float2* __restrict__ eps,
+            long long vox, C2 s, C2 kb, const BiScal* st, double* __restrict__ partial) {
+  if (st->done != 0.0) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+  BI_LOOP(i) {
+    const float2 f = vfac(eps, i, s, kb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * vox + i;
+      const float2 sv = sv_[j];
+      const float2 v = cmul2(cadd2(g[j], sv), f);
+      const float2 tv = make_float2(-v.x, -v.y);
+      g[j] = tv;
+      cdot_acc(tv, sv, acc[0], acc[1]);
+      acc[2] += (double)tv.x * tv.x + (double)tv.y * tv.y;
+    }
+  }
+  block_sums<3>(acc, partial);
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+bi_omega_kernel(const double* __restrict__ partial, BiScal* st) {
+  if (st->done != 0.0) return;
+  double v[3];
+  sum_partials<3>(partial, v);
+  if (threadIdx.x == 0) {
+    const double tt[2] = {v[2], 0.0};
+    cdiv_d(v, tt, st->omega_n);
+  }
+}
+
+// x += alpha' p (+ omega' s);  r = s (- omega' t);  partials |r|^2, <rh, r>
+__global__ void __launch_bounds__(kRedThreads)
+bi_x_kernel(float2* __restrict__ x, const float2* __restrict__ p, float2* __restrict__ r,
+            const float2* __restrict__ t, const float2* __restrict__ rh, long long vox,
+            const BiScal* st, double* __restrict__ partial) {
+  if (st->done != 0.0) return;
+  const float2 alpha = cf(st->alpha_n), omega = cf(st->omega_n);
+  const bool early = st->early != 0.0;
+  double acc[3] = {0.0, 0.0, 0.0};
+  BI_LOOP(i) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long j = c * vox + i;
+      const float2 sv = r[j];
+      float2 xv = cadd2(x[j], cmul2(alpha, p[j]));
+      float2 rv = sv;
+      if (!early) {
+        xv = cadd2(xv, cmul2(omega, sv));
+        rv = csub2(sv, cmul2(omega, t[j]));
+      }
+      x[j] = xv;
+      r[j] = rv;
+      acc[0] += (double)rv.x * rv.x + (double)rv.y * rv.y;
+      cdot_acc(rh[j], rv, acc[1], acc[2]);
+    }
+  }
+  block_sums<3>(acc, partial);
+}
+
+__global__ void __launch_bounds__(kRedThreads)
+bi_end_kernel(const double* __restrict__ partial, BiScal* st, double* __restrict__ status) {
+  if (st->done == 0.0) {
+    double v[3];
+    sum_partials<3>(partial, v);
+    if (threadIdx.x == 0) {
+      st->rs = v[0];
+      st->rr[0] = v[1];
+      st->rr[1] = v[2];
+      double k = st->k + 1.0;
+      if ((st->omega_n[0] == 0.0 && st->omega_n[1] == 0.0) ||
+          (st->alpha_n[0] == 0.0 && st->alpha_n[1] == 0.0))
+        k = -11.0;
+      if (st->rho_n[0] == 0.0 && st->rho_n[1] == 0.0) k = -10.0;
+      st->k = k;
+      for (int j = 0; j < 2; ++j) {
+        st->rho[j] = st->rho_n[j];
+        st->alpha[j] = st->alpha_n[j];
+        st->omega[j] = st->omega_n[j];
+      }
+      st->done = (st->rs > st->atol2 && k >= 0.0) ? 0.0 : 1.0;
+    }
+  }
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    status[0] = st->rs;
+    status[1] = st->k;
+    status[2] = st->done;
+    status[3] = st->bs;
+  }
+}
+#undef BI_LOOP
+
 }  // namespace
 }  // namespace cxm
 
@@ -356,5 +683,137 @@ extern "C" int cxm_mbs_output(void* stream, const void* x, const double* num_sca
   hipLaunchKernelGGL(mbs_output_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, (hipStream_t)stream,
                      (const float2*)x, (float2*)e, g.vox, num_scale);
   CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+// ---- BiCGStab entry points -----------------------------------------------------
+namespace cxm {
+namespace {
+// state: r | rh | p | q (3 vox c64 each) | BiScal (256 B) | partials (kRedBlocks x 4 f64)
+struct BiLayout {
+  float2 *r, *rh, *p, *q;
+  BiScal* st;
+  double* partial;
+};
+size_t bi_state_bytes(const MbsGeom& g) {
+  return (size_t)4 * 3 * g.vox * sizeof(float2) + 256 + (size_t)kRedBlocks * 4 * sizeof(double);
+}
+BiLayout bi_layout(const MbsGeom& g, void* state) {
+  BiLayout b;
+  const long long n = 3 * g.vox;
+  b.r = (float2*)state;
+  b.rh = b.r + n;
+  b.p = b.rh + n;
+  b.q = b.p + n;
+  b.st = (BiScal*)(b.q + n);
+  b.partial = (double*)((char*)b.st + 256);
+  return b;
+}
+static_assert(sizeof(BiScal) <= 256, "BiCGStab scalars exceed their slot");
+
+// g <- G(g): forward 3D FFT, Green multiply, inverse 3D FFT
+int green_apply(const MbsGeom& g, const cxm_mbs_desc* d, float2* gb, C2 sc, C2 kb, hipStream_t s) {
+  int rc;
+  if ((rc = fft3(g, gb, -1, s))) return rc;
+  const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
+  hipLaunchKernelGGL(mbs_green_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, gb, g.vox, g.nx, g.ny,
+                     g.nz, kscale, sc, kb);
+  CXM_LAUNCH_CHECK();
+  return fft3(g, gb, 1, s);
+}
+
+int bi_check(const cxm_mbs_desc* d, MbsGeom& g, const void* eps, const void* x, void* state,
+             size_t state_bytes, double* status, void* workspace, size_t workspace_bytes) {
+  int rc = mbs_geom(d, g);
+  if (rc) return rc;
+  if (!eps || !x || !state || !status) return fail(CXM_ERR_INVALID, "null buffer");
+  if (state_bytes < bi_state_bytes(g)) return fail(CXM_ERR_WORKSPACE, "BiCGStab state too small");
+  if (!workspace || workspace_bytes < cxm_mbs_workspace_size(d))
+    return fail(CXM_ERR_WORKSPACE, "workspace too small");
+  return ensure_init();
+}
+}  // namespace
+}  // namespace cxm
+
+extern "C" size_t cxm_mbs_bicgstab_state_size(const cxm_mbs_desc* d) {
+  MbsGeom g;
+  if (mbs_geom(d, g)) return 0;
+  return bi_state_bytes(g);
+}
+
+extern "C" int cxm_mbs_bicgstab_init(void* stream, const void* eps, const void* prec_y, const void* x,
+                                     void* state, size_t state_bytes, double tol, double atol,
+                                     double* status, void* workspace, size_t workspace_bytes,
+                                     const cxm_mbs_desc* d) {
+  MbsGeom g;
+  int rc = bi_check(d, g, eps, x, state, state_bytes, status, workspace, workspace_bytes);
+  if (rc) return rc;
+  if (!prec_y) return fail(CXM_ERR_INVALID, "null buffer");
+  if (!(tol >= 0.0) || !(atol >= 0.0)) return fail(CXM_ERR_INVALID, "tol and atol must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  const C2 sc = c2(d->scale_re, d->scale_im);
+  const C2 kb = c2(d->bias_re * d->scale_re - d->bias_im * d->scale_im + 1.0,
+                   d->bias_re * d->scale_im + d->bias_im * d->scale_re);
+  float2* gb = (float2*)workspace;
+  const BiLayout b = bi_layout(g, state);
+  // r0 = b - A x0 = b + V(G(V(x0)) + x0)
+  hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)x,
+                     (const float2*)eps, gb, g.vox, 0, sc, kb, C2{1.0f, 0.0f});
+  CXM_LAUNCH_CHECK();
+  if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
+  hipLaunchKernelGGL(bi_r0_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, (const float2*)gb,
+                     (const float2*)eps, (const float2*)prec_y, (const float2*)x, b.r, b.rh, b.p, b.q,
+                     g.vox, sc, kb, b.partial);
+  CXM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(bi_init_scalars_kernel, dim3(1), dim3(kRedThreads), 0, s, (const double*)b.partial,
+                     b.st, tol, atol, status);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+extern "C" int cxm_mbs_bicgstab_iterate(void* stream, const void* eps, void* x, void* state,
+                                        size_t state_bytes, int32_t iterations, double* status,
+                                        void* workspace, size_t workspace_bytes, const cxm_mbs_desc* d) {
+  MbsGeom g;
+  int rc = bi_check(d, g, eps, x, state, state_bytes, status, workspace, workspace_bytes);
+  if (rc) return rc;
+  if (iterations < 0) return fail(CXM_ERR_INVALID, "iterations must be >= 0");
+  hipStream_t s = (hipStream_t)stream;
+  const C2 sc = c2(d->scale_re, d->scale_im);
+  const C2 kb = c2(d->bias_re * d->scale_re - d->bias_im * d->scale_im + 1.0,
+                   d->bias_re * d->scale_im + d->bias_im * d->scale_re);
+  float2* gb = (float2*)workspace;
+  const BiLayout b = bi_layout(g, state);
+  const float2* ep = (const float2*)eps;
+  const dim3 rg(kRedBlocks), rt(kRedThreads);
+  for (int it = 0; it < iterations; ++it) {
+    hipLaunchKernelGGL(bi_beta_kernel, dim3(1), dim3(1), 0, s, b.st);
+    CXM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bi_p_kernel, rg, rt, 0, s, (const float2*)b.r, b.p, (const float2*)b.q, ep, gb,
+                       g.vox, sc, kb, (const BiScal*)b.st);
+    CXM_LAUNCH_CHECK();
+    if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
+    hipLaunchKernelGGL(bi_q_kernel, rg, rt, 0, s, (const float2*)gb, (const float2*)b.p,
+                       (const float2*)b.rh, ep, b.q, g.vox, sc, kb, (const BiScal*)b.st, b.partial);
+    CXM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bi_alpha_kernel, dim3(1), rt, 0, s, (const double*)b.partial, b.st);
+    CXM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bi_s_kernel, rg, rt, 0, s, b.r, (const float2*)b.q, ep, gb, g.vox, sc, kb,
+                       (const BiScal*)b.st, b.partial);
+    CXM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bi_early_kernel, dim3(1), rt, 0, s, (const double*)b.partial, b.st);
+    CXM_LAUNCH_CHECK();
+    if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
+    hipLaunchKernelGGL(bi_t_kernel, rg, rt, 0, s, gb, (const float2*)b.r, ep, g.vox, sc, kb,
+                       (const BiScal*)b.st, b.partial);
+    CXM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bi_omega_kernel, dim3(1), rt, 0, s, (const double*)b.partial, b.st);
+    CXM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bi_x_kernel, rg, rt, 0, s, (float2*)x, (const float2*)b.p, b.r, (const float2*)gb,
+                       (const float2*)b.rh, g.vox, (const BiScal*)b.st, b.partial);
+    CXM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bi_end_kernel, dim3(1), rt, 0, s, (const double*)b.partial, b.st, status);
+    CXM_LAUNCH_CHECK();
+  }
   return CXM_OK;
 }

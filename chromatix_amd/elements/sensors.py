@@ -37,7 +37,7 @@ class BasicSensor:
             return shape != self.shape
         same = np.allclose(np.broadcast_to(np.asarray(input_spacing, np.float64), (2,)),
                            np.broadcast_to(np.asarray(self.spacing, np.float64), (2,)),
-                           rtol=1e-12, atol=0.0)
+                           rtol=1e-9, atol=0.0)
         return not (same and shape == self.shape)
 
     def __call__(self, sensor_input, input_spacing=None, resample: bool = True, key=None):

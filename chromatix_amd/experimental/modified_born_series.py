@@ -3,8 +3,12 @@ src/chromatix/experimental/modified_born_series/{sample,solver}.py).
 
 The preconditioned fixed-point iteration runs on the GPU (cxm_mbs_*): one
 launch sequence per iteration, V multiply, 3D FFT, Green multiply, inverse
-3D FFT and the update with an FP64 residual norm.  Setup (absorbing
-boundary, the bias/scale of get_shift_and_scale) is host code.
+3D FFT and the update with an FP64 residual norm.  ``use_bicgstab=True``
+runs BiCGStab on the same preconditioned operator (cxm_mbs_bicgstab_*):
+two operator applications per iteration, the dot products reduced in FP64
+into device scalars, no host round trip inside a chunk of iterations.
+Setup (absorbing boundary, the bias/scale of get_shift_and_scale) is host
+code.
 
 get_shift_and_scale: the reference minimises max |eps - shift|^2 with a
 nonlinear CG to 1e-3 tolerance; here the minimiser is computed exactly, as
@@ -25,7 +29,7 @@ import torch
 from .. import _lib as L
 
 __all__ = ["Sample", "EmptySample", "Source", "add_absorbing_bc", "get_shift_and_scale", "solve",
-           "MBSResult"]
+           "MBSResult", "Solver", "BiCGStabSolver"]
 
 
 class Sample:
@@ -238,17 +242,85 @@ class Solver:
         return E
 
 
+class BiCGStabSolver(Solver):
+    """Stepwise BiCGStab on the preconditioned system (solve(...,
+    use_bicgstab=True), solver.py:141-143): the iteration of
+    jax.scipy.sparse.linalg.bicgstab without a preconditioner, its scalars
+    kept on the device; it stops once |r| <= max(tol |prec_y|, atol) or on
+    breakdown, and the steps enqueued after that change nothing."""
+
+    def __init__(self, sample: Sample, source: Source, initial_E=None, *, shift_scale=None,
+                 tol: float = 1e-3, atol: float = 0.0, maxiter: int = 1000):
+        super().__init__(sample, source, initial_E, shift_scale=shift_scale, tol=tol,
+                         maxiter=maxiter)
+        lib = L.lib()
+        nbytes = lib.cxm_mbs_bicgstab_state_size(ctypes.byref(self.d))
+        self.state = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+        # [|r|^2, k, stopped, |prec_y|^2]
+        self.status_dev = torch.zeros(4, dtype=torch.float64, device=self.dev)
+        self.status_host = torch.zeros(4, dtype=torch.float64, pin_memory=True)
+        L.check(lib.cxm_mbs_bicgstab_init(L.stream_ptr(self.dev), self.eps.data_ptr(), self.py.data_ptr(),
+                                          self.x.data_ptr(), self.state.data_ptr(), nbytes, float(tol),
+                                          float(atol), self.status_dev.data_ptr(), self.ws.data_ptr(),
+                                          self.ws.numel(), ctypes.byref(self.d)))
+        self.k, self.stopped = 0, False
+        self._readback()
+
+    def _readback(self):
+        self.status_host.copy_(self.status_dev, non_blocking=True)
+        self.event.record()
+        self.pending = True
+
+    def step(self, k: int) -> None:
+        k = min(int(k), self.maxiter - self.iterations)
+        if k <= 0 or self.stopped:
+            return
+        L.check(L.lib().cxm_mbs_bicgstab_iterate(L.stream_ptr(self.dev), self.eps.data_ptr(),
+                                                 self.x.data_ptr(), self.state.data_ptr(),
+                                                 self.state.numel(), k, self.status_dev.data_ptr(),
+                                                 self.ws.data_ptr(), self.ws.numel(),
+                                                 ctypes.byref(self.d)))
+        self.iterations += k
+        self._readback()
+
+    def poll(self) -> bool:
+        if self.pending and self.event.query():
+            rs, k, stopped, _ = self.status_host.tolist()
+            self.residual = float(np.sqrt(rs))
+            self.k, self.stopped = int(k), bool(stopped)
+            self.pending = False
+        return not self.pending
+
+    @property
+    def converged(self) -> bool:
+        return self.stopped or self.iterations >= self.maxiter
+
+
 def solve(sample: Sample, source: Source, initial_E=None, maxiter: int = 1000, tol: float = 1e-3,
           implicit_diff: bool = True, use_bicgstab: bool = False, *, rtol: float | None = None,
           check_every: int = 8, shift_scale=None, return_info: bool = False):
     """Electric field E (Nx, Ny, Nz, 3) solving the scattering problem
-    (solver.py:101-146).  Stops when the l2 norm of the update of the
-    normalised iterate falls below ``tol`` (jaxopt's FixedPointIteration
-    criterion) or, with ``rtol``, below rtol |prec_y|; the norm is read back
-    every ``check_every`` iterations."""
+    (solver.py:101-146).  Fixed-point iteration (default): stops when the l2
+    norm of the update of the normalised iterate falls below ``tol``
+    (jaxopt's FixedPointIteration criterion) or, with ``rtol``, below
+    rtol |prec_y|.  ``use_bicgstab``: BiCGStab until |r| <= tol |prec_y|
+    (jax.scipy.sparse.linalg.bicgstab's criterion) or breakdown.  The norm
+    is read back every ``check_every`` iterations."""
     if use_bicgstab:
-        raise NotImplementedError("use_bicgstab=True is not implemented; the fixed-point "
-                                  "iteration is")
+        if rtol is not None:
+            raise ValueError("rtol applies to the fixed-point iteration; BiCGStab's tol is relative")
+        sv = BiCGStabSolver(sample, source, initial_E, shift_scale=shift_scale, tol=tol,
+                            maxiter=maxiter)
+        sv.event.synchronize()
+        sv.poll()
+        while not sv.converged:
+            sv.step(check_every)
+            sv.event.synchronize()
+            sv.poll()
+        E = sv.result()
+        if return_info:
+            return MBSResult(E, sv.k, sv.residual)
+        return E
     sv = Solver(sample, source, initial_E, shift_scale=shift_scale, tol=tol, rtol=rtol,
                 maxiter=maxiter)
     while True:

@@ -47,10 +47,42 @@ def _workspace(d, device) -> torch.Tensor:
     return torch.empty(nbytes, dtype=torch.uint8, device=device)
 
 
-def _forward(phase, d, out_kind) -> torch.Tensor:
+# defocus planes per library call (the per-plane parameters travel with the
+# launch); longer z lists run in chunks
+MAX_Z_PER_CALL = 32
+
+
+def _z_chunks(args):
+    """(plane slice, args with that chunk's z) for the z list in ``args``."""
+    z = args[2]
+    zl = np.atleast_1d(np.asarray(z, np.float64)).reshape(-1)
+    if zl.size <= MAX_Z_PER_CALL:
+        return [(slice(0, zl.size), args)]
+    return [(slice(i, min(i + MAX_Z_PER_CALL, zl.size)),
+             args[:2] + (zl[i: i + MAX_Z_PER_CALL],) + args[3:])
+            for i in range(0, zl.size, MAX_Z_PER_CALL)]
+
+
+def _forward_all(phase, args, out_kind) -> torch.Tensor:
+    """Forward over every z chunk into one (Z, H, W) output."""
+    chunks = _z_chunks(args)
+    if len(chunks) == 1:
+        d, keep, zs = _desc(phase, *args)
+        return _forward(phase, d, out_kind)
+    H, W = phase.shape[-2:]
+    dtype = torch.float32 if out_kind == L.CXM_OUT_INTENSITY else torch.complex64
+    out = torch.empty((chunks[-1][0].stop, H, W), dtype=dtype, device=phase.device)
+    for sl, a in chunks:
+        d, keep, zs = _desc(phase, *a)
+        _forward(phase, d, out_kind, out[sl])
+    return out
+
+
+def _forward(phase, d, out_kind, out=None) -> torch.Tensor:
     d.out_kind = out_kind
     dtype = torch.float32 if out_kind == L.CXM_OUT_INTENSITY else torch.complex64
-    out = torch.empty((d.num_z, d.height, d.width), dtype=dtype, device=phase.device)
+    if out is None:
+        out = torch.empty((d.num_z, d.height, d.width), dtype=dtype, device=phase.device)
     ws = _workspace(d, phase.device)
     L.check(L.lib().cxm_psf4f_fwd(L.stream_ptr(phase.device), phase.data_ptr(), out.data_ptr(),
                                   ws.data_ptr(), ws.numel(), ctypes.byref(d)))
@@ -72,16 +104,20 @@ def _backward(phase, cot, d, kind) -> torch.Tensor:
 class _PsfIntensity(torch.autograd.Function):
     @staticmethod
     def forward(ctx, phase, args):
-        d, keep, zs = _desc(phase, *args)
         ctx.args = args
         ctx.save_for_backward(phase)
-        return _forward(phase, d, L.CXM_OUT_INTENSITY)
+        return _forward_all(phase, args, L.CXM_OUT_INTENSITY)
 
     @staticmethod
     def backward(ctx, g):
         (phase,) = ctx.saved_tensors
-        d, keep, zs = _desc(phase, *ctx.args)
-        return _backward(phase, g, d, L.CXM_GRAD_UPSTREAM), None
+        grad = None
+        g = g.reshape((-1,) + tuple(phase.shape[-2:]))
+        for sl, a in _z_chunks(ctx.args):
+            d, keep, zs = _desc(phase, *a)
+            gc = _backward(phase, g[sl], d, L.CXM_GRAD_UPSTREAM)
+            grad = gc if grad is None else grad.add_(gc)
+        return grad, None
 
 
 def _phase_tensor(phase) -> torch.Tensor:
@@ -115,12 +151,11 @@ def psf_4f(phase, spacing, wavelength, z, f, n, NA, f_tube, power=1.0, amplitude
         if ph_c.requires_grad:
             out = _PsfIntensity.apply(ph_c, args)
         else:
-            d, keep, zs = _desc(ph_c, *args)
-            out = _forward(ph_c, d, L.CXM_OUT_INTENSITY)
+            out = _forward_all(ph_c, args, L.CXM_OUT_INTENSITY)
         return out[0] if single else out
     if output == "field":
         d, keep, zs = _desc(ph_c, *args)
-        u = _forward(ph_c.detach(), d, L.CXM_OUT_COMPLEX)
+        u = _forward_all(ph_c.detach(), args, L.CXM_OUT_COMPLEX)
         du = psf_output_spacing(ph_c.shape, (d.dy, d.dx), wavelength, f_tube, n)
         return ScalarField(u[0] if single else u, du, 0.0, float(wavelength))
     raise ValueError('output must be "intensity" or "field"')
@@ -131,7 +166,16 @@ def psf_4f_mse_grad(phase, target, spacing, wavelength, z, f, n, NA, f_tube, pow
     """d/dphase of mean((psf_4f(phase, ...) - target)^2), fused: the forward
     is recomputed inside the adjoint and the intensity never stored."""
     ph = _phase_tensor(phase).detach().contiguous()
-    d, keep, zs = _desc(ph, spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset,
-                        phase_mode)
-    t = torch.as_tensor(target, device=ph.device).reshape(d.num_z, d.height, d.width)
-    return _backward(ph, t, d, L.CXM_GRAD_MSE)
+    args = (spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode)
+    chunks = _z_chunks(args)
+    nz = chunks[-1][0].stop
+    t = torch.as_tensor(target, device=ph.device).reshape((nz,) + tuple(ph.shape[-2:]))
+    grad = None
+    for sl, a in chunks:
+        d, keep, zs = _desc(ph, *a)
+        gc = _backward(ph, t[sl], d, L.CXM_GRAD_MSE)
+        if len(chunks) > 1:
+            # the library's mean is over its chunk's planes
+            gc.mul_((sl.stop - sl.start) / nz)
+        grad = gc if grad is None else grad.add_(gc)
+    return grad

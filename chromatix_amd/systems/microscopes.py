@@ -47,23 +47,28 @@ class Microscope:
 
     def __call__(self, sample, *args, key=None, **kwargs):
         """PSF for ``args`` / ``kwargs``, then ``image(sample, psf)``."""
-        system_psf = self.psf(*args, **kwargs)
-        psf = self._process_psf(system_psf)
+        if isinstance(self.system_psf, Optical4FSystemPSF):
+            # the fused intensity kernels: |field|^2 without storing the field
+            # (the field's spacing is the requested output spacing)
+            psf = self.system_psf.intensity(self, *args, **kwargs)
+            psf = self._process_psf(psf, spacing=self.system_psf.spacing)
+        else:
+            psf = self._process_psf(self.psf(*args, **kwargs))
         return self.image(sample, psf, key=key)
 
     def psf(self, *args, **kwargs):
         """PSF of the system (microscopes.py:225-227)."""
         return self.system_psf(self, *args, **kwargs)
 
-    def _process_psf(self, system_psf):
+    def _process_psf(self, system_psf, spacing=None):
         """Intensity, padding crop and resampling to the sensor
-        (microscopes.py:229-269)."""
+        (microscopes.py:229-269); ``spacing``: that of an intensity
+        ``system_psf`` when known."""
         if isinstance(system_psf, ScalarField):
             psf = system_psf.intensity
             spacing = system_psf.central_rectangular_dx
         else:
             psf = torch.as_tensor(system_psf)
-            spacing = None
         shape = tuple(psf.shape[-2:])
         if self.padding_ratio:
             unpadded = tuple(int(s / (1.0 + self.padding_ratio)) for s in shape)

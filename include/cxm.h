@@ -548,6 +548,28 @@ int cxm_mbs_output(void* stream, const void* x, const double* num_scale, void* e
                    const cxm_mbs_desc* d);
 
 /*
+ * BiCGStab on the same preconditioned system (solve(..., use_bicgstab=True),
+ * solver.py:141-143, which calls jax.scipy.sparse.linalg.bicgstab(prec_forward,
+ * prec_y, x0, maxiter, tol) without a preconditioner): x solves
+ * prec_forward(x) = prec_y; stop when |r|^2 <= max(tol^2 |prec_y|^2, atol^2) or on
+ * breakdown (k = -10: <rh, r> = 0; k = -11: alpha or omega = 0).
+ *   cxm_mbs_bicgstab_init:    r = prec_y - prec_forward(x), rh = p = q = r,
+ *                             rho = alpha = omega = 1, k = 0
+ *   cxm_mbs_bicgstab_iterate: `iterations` steps on the device; steps after the
+ *                             stop leave x and the state unchanged
+ * state: cxm_mbs_bicgstab_state_size bytes (four fields and the FP64 scalars),
+ * owned by the caller between calls.  status: 4 device doubles written by every
+ * call: |r|^2, k, stopped (0/1), |prec_y|^2.  x: 3 x nx x ny x nz planar.
+ */
+size_t cxm_mbs_bicgstab_state_size(const cxm_mbs_desc* d);
+int cxm_mbs_bicgstab_init(void* stream, const void* eps, const void* prec_y, const void* x,
+                          void* state, size_t state_bytes, double tol, double atol, double* status,
+                          void* workspace, size_t workspace_bytes, const cxm_mbs_desc* d);
+int cxm_mbs_bicgstab_iterate(void* stream, const void* eps, void* x, void* state, size_t state_bytes,
+                             int32_t iterations, double* status, void* workspace,
+                             size_t workspace_bytes, const cxm_mbs_desc* d);
+
+/*
  * Intensity |u|^2 of n_pixels complex64 values, or for an innermost
  * wavelength axis of length num_wv the density-weighted sum over it:
  *   out[p] = sum_w density[w] |u[p*num_wv + w]|^2   (float32 out)

@@ -3,6 +3,12 @@
 Follows (paths relative to the chromatix repository root):
   Sample.k_grid, add_absorbing_bc, Source  src/chromatix/experimental/modified_born_series/sample.py:9-124
   get_shift_and_scale, precondition, solve src/chromatix/experimental/modified_born_series/solver.py:10-146
+  solve(..., use_bicgstab=True)            solver.py:141-143, which calls
+      jax.scipy.sparse.linalg.bicgstab (JAX >= 0.4.1, a dependency absent
+      here): ``bicgstab`` restates its published iteration (van der Vorst's
+      BiCGStab, no preconditioner M, stop on |r|^2 <= max(tol^2 |b|^2, atol^2)
+      or breakdown) and is pinned against scipy.sparse.linalg.bicgstab's
+      iterates and numpy.linalg.solve on dense systems.
 Fields are (Nx, Ny, Nz, 3) complex128.  get_shift_and_scale minimises
 max |eps - s|^2 directly (scipy Nelder-Mead from the reference's start
 point [1.6, 0.25]); the solution is pinned by the residual of the PDE it
@@ -89,3 +95,51 @@ class MBS:
             El = np.where(self.K2 != 0, self.K / self.K2 * proj, 0.0)
         curl = np.fft.ifftn(self.K2 * (Ef - El), axes=ax3)
         return np.linalg.norm(curl - self.eps[..., None] * E + S) / np.linalg.norm(S)
+
+
+def bicgstab(A, b, x0, maxiter, tol=1e-5, atol=0.0):
+    """BiCGStab of jax.scipy.sparse.linalg.bicgstab in FP64, M = identity.
+    Returns (x, k, |r|^2): k iterations run, -10 on rho breakdown, -11 on
+    alpha / omega breakdown."""
+    def vdot(a, c):
+        return np.vdot(a.ravel(), c.ravel())
+
+    b = np.asarray(b, np.complex128)
+    x = np.asarray(x0, np.complex128).copy()
+    atol2 = max(tol**2 * vdot(b, b).real, atol**2)
+    r = b - A(x)
+    rh, p, q = r.copy(), r.copy(), r.copy()
+    rho = alpha = omega = 1.0 + 0j
+    k = 0
+    rs = vdot(r, r).real
+    while rs > atol2 and k < maxiter and k >= 0:
+        rho_n = vdot(rh, r)
+        beta = rho_n / rho * alpha / omega
+        p = r + beta * (p - omega * q)
+        q = A(p)
+        alpha_n = rho_n / vdot(rh, q)
+        s = r - alpha_n * q
+        early = vdot(s, s).real < atol2
+        t = A(s)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            omega_n = vdot(t, s) / vdot(t, t)
+        if early:
+            x = x + alpha_n * p
+            r = s
+        else:
+            x = x + alpha_n * p + omega_n * s
+            r = s - omega_n * t
+        k = -11 if (omega_n == 0 or alpha_n == 0) else k + 1
+        k = -10 if rho_n == 0 else k
+        rho, alpha, omega = rho_n, alpha_n, omega_n
+        rs = vdot(r, r).real
+    return x, k, rs
+
+
+def solve_bicgstab(eps, S, spacing, k0, shift, scale, maxiter=1000, tol=1e-3):
+    """solve(..., use_bicgstab=True) (solver.py:101-146): E = x num_scale,
+    x from BiCGStab on prec_forward(x) = prec_y from x0 = prec_y."""
+    m = MBS(eps, spacing, k0, shift, scale)
+    py, num = m.prec_y(np.asarray(S, np.complex128))
+    x, k, rs = bicgstab(m.prec_forward, py, py, maxiter, tol)
+    return x * num, k, rs

@@ -81,3 +81,71 @@ def test_c4_volume_converges():
     print(f"mbs 160^3: {res.iterations} iterations, residual {res.residual:.2e}, PDE {pde:.2e}")
     assert res.iterations < 3000
     assert pde < 1e-2
+
+
+@pytest.mark.parametrize("iters", [1, 3, 6])
+def test_bicgstab_iterates_match_oracle(iters):
+    # solve(use_bicgstab=True) with maxiter k and tol 0 runs exactly k
+    # BiCGStab iterations: the iterate matches the FP64 oracle's
+    eps, spacing, J = _setup(24, 4, 2, 2)  # 32^3 with the boundary
+    S = om.source_field(J, K0)
+    shift, scale = om.get_shift_and_scale(eps)
+    m = om.MBS(eps, spacing, K0, shift, scale)
+    py, num = m.prec_y(S)
+    x_ref, k_ref, _ = om.bicgstab(m.prec_forward, py, py, iters, tol=0.0)
+    assert k_ref == iters
+    sample = mbs.Sample(torch.as_tensor(eps.astype(np.complex64)).cuda(), spacing)
+    src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
+    res = mbs.solve(sample, src, maxiter=iters, tol=0.0, shift_scale=(shift, scale), use_bicgstab=True,
+                    return_info=True, check_every=2)
+    assert res.iterations == iters
+    err = rel_l2(res.E.cpu().numpy(), x_ref * num)
+    print(f"bicgstab 32^3 after {iters} iterations: {err:.2e}")
+    assert err < 1e-5
+
+
+def test_bicgstab_converged_solution():
+    # tol 1e-3 (the reference's default): stops at the oracle's iteration
+    # count (+-1 for float32 rounding near the threshold), the field solves
+    # the Helmholtz equation and equals the converged fixed point
+    eps, spacing, J = _setup(32, 4, 3, 3)
+    S = om.source_field(J, K0)
+    shift, scale = om.get_shift_and_scale(eps)
+    E_ref, k_ref, _ = om.solve_bicgstab(eps, S, spacing, K0, shift, scale, maxiter=500, tol=1e-3)
+    sample = mbs.Sample(torch.as_tensor(eps.astype(np.complex64)).cuda(), spacing)
+    src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
+    res = mbs.solve(sample, src, maxiter=500, tol=1e-3, shift_scale=(shift, scale), use_bicgstab=True,
+                    return_info=True)
+    assert abs(res.iterations - k_ref) <= 1
+    E = res.E.cpu().numpy().astype(np.complex128)
+    assert rel_l2(E, E_ref) < 1e-4
+    res5 = mbs.solve(sample, src, maxiter=500, tol=1e-5, shift_scale=(shift, scale), use_bicgstab=True,
+                     return_info=True)
+    m = om.MBS(eps, spacing, K0, shift, scale)
+    E5 = res5.E.cpu().numpy().astype(np.complex128)
+    assert m.pde_residual(E5, S) < 1e-4
+    E_fp, _, _ = om.solve_bicgstab(eps, S, spacing, K0, shift, scale, maxiter=500, tol=1e-9)
+    assert rel_l2(E5, E_fp) < 1e-4
+    print(f"bicgstab 40^3: {res.iterations} iterations (oracle {k_ref}) at tol 1e-3, "
+          f"{res5.iterations} at 1e-5")
+
+
+def test_bicgstab_stops_and_keeps_state():
+    # extra enqueued steps after the stop change nothing
+    eps, spacing, J = _setup(24, 4, 2, 4)
+    sample = mbs.Sample(torch.as_tensor(eps.astype(np.complex64)).cuda(), spacing)
+    src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
+    sv = mbs.BiCGStabSolver(sample, src, tol=1e-3, maxiter=400)
+    while True:
+        sv.step(4)
+        sv.event.synchronize()
+        sv.poll()
+        if sv.converged:
+            break
+    assert sv.stopped and 0 < sv.k <= sv.iterations
+    x1 = sv.x.clone()
+    sv.stopped = False  # force more steps past the device-side stop
+    sv.step(3)
+    sv.event.synchronize()
+    sv.poll()
+    assert sv.stopped and torch.equal(sv.x, x1)

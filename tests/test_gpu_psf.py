@@ -66,6 +66,32 @@ def test_psf_mse_gradient(pm, n, nz):
     assert err < TOL
 
 
+def test_psf_more_planes_than_one_call():
+    # 70 planes run as chunks of 32: intensity, field, autograd and the
+    # MSE gradient (whose mean covers all planes) against the oracle
+    n, nz = 96, 70
+    phase, dx, z, rng = _case(n, nz, pitch=6.5 * 1280 / n)
+    args = ((dx, dx), LAM, z, FOBJ, NIMM, NA, FTUBE)
+    I_ref = op.psf4f(phase.astype(np.float64), *args)
+    ph = torch.as_tensor(phase).cuda()
+    I = cf.psf_4f(ph, dx, LAM, z, FOBJ, NIMM, NA, FTUBE, phase_mode="accurate")
+    assert tuple(I.shape) == (nz, n, n)
+    assert rel_l2(I.cpu().numpy(), I_ref) < TOL
+    fld = cf.psf_4f(ph, dx, LAM, z, FOBJ, NIMM, NA, FTUBE, phase_mode="accurate", output="field")
+    assert rel_l2(np.abs(fld.u.cpu().numpy()) ** 2, I_ref) < TOL
+    target = (I_ref * rng.uniform(0.5, 1.5, I_ref.shape)).astype(np.float32)
+    _, g_ref = op.psf4f_mse_grad(phase.astype(np.float64), target.astype(np.float64), *args)
+    g = cf.psf_4f_mse_grad(ph, torch.as_tensor(target).cuda(), dx, LAM, z, FOBJ, NIMM, NA, FTUBE,
+                           phase_mode="accurate").cpu().numpy()
+    assert rel_l2(g, g_ref) < TOL
+    phr = ph.clone().requires_grad_(True)
+    I2 = cf.psf_4f(phr, dx, LAM, z, FOBJ, NIMM, NA, FTUBE, phase_mode="accurate")
+    w = torch.as_tensor(rng.standard_normal(I2.shape).astype(np.float32)).cuda()
+    (I2 * w).sum().backward()
+    g_ref = op.psf4f_grad(phase.astype(np.float64), w.cpu().numpy().astype(np.float64), *args)
+    assert rel_l2(phr.grad.cpu().numpy(), g_ref) < TOL
+
+
 def test_psf_autograd_upstream():
     phase, dx, z, rng = _case(160, 4, pitch=6.5 * 1280 / 160)
     ph = torch.as_tensor(phase).cuda().requires_grad_(True)

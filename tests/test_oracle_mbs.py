@@ -57,3 +57,62 @@ def test_absorbing_boundary_ramp():
     kappa = np.array([0.3, 0.2, 0.1])
     np.testing.assert_allclose(eps[:3, 0, 0], 1 + ((1 + 1j * kappa) ** 2 - 1), atol=1e-12)
     np.testing.assert_allclose(eps[3:7, 0, 0], 1.0)
+
+
+def _dense(n=40, seed=3):
+    rng = np.random.default_rng(seed)
+    A = np.eye(n) + 0.3 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    return A, b
+
+
+def test_bicgstab_iterates_match_scipy():
+    # the same recurrences as scipy.sparse.linalg.bicgstab (rtilde = r0,
+    # first p = r0): iterates agree after every iteration count
+    import scipy.sparse.linalg as spl
+
+    A, b = _dense()
+    x0 = np.zeros_like(b)
+    for k in range(1, 7):
+        x, kk, _ = om.bicgstab(lambda v: A @ v, b, x0, maxiter=k, tol=0.0)
+        xs, info = spl.bicgstab(A, b, x0=x0, rtol=0.0, atol=0.0, maxiter=k)
+        assert kk == k
+        np.testing.assert_allclose(x, xs, rtol=0, atol=1e-11 * np.linalg.norm(b))
+
+
+def test_bicgstab_converges_to_the_dense_solution():
+    A, b = _dense(seed=4)
+    x, k, rs = om.bicgstab(lambda v: A @ v, b, np.zeros_like(b), maxiter=200, tol=1e-12)
+    assert 0 < k < 200 and rs <= 1e-24 * np.vdot(b, b).real
+    np.testing.assert_allclose(x, np.linalg.solve(A, b), rtol=0, atol=1e-10 * np.linalg.norm(b))
+
+
+def test_bicgstab_closed_forms():
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(16) + 1j * rng.standard_normal(16)
+    # A = I: q = p = r0, alpha = 1, s = 0 -> the early exit, x = b after one step
+    x, k, rs = om.bicgstab(lambda v: v, b, np.zeros_like(b), maxiter=10, tol=1e-6)
+    np.testing.assert_allclose(x, b, atol=1e-14)
+    assert k == 1 and rs == 0.0
+    # an exact start runs no iteration; b = 0 from x0 = 0 likewise (|r|^2 = 0
+    # is not above atol2 = 0)
+    A, _ = _dense(16, seed=6)
+    xe = rng.standard_normal(16) + 1j * rng.standard_normal(16)
+    x, k, _ = om.bicgstab(lambda v: A @ v, A @ xe, xe, maxiter=10, tol=1e-6)
+    assert k == 0 and np.array_equal(x, xe)
+    z = np.zeros(16, complex)
+    x, k, rs = om.bicgstab(lambda v: A @ v, z, z, maxiter=10, tol=0.5)
+    assert k == 0 and rs == 0.0 and not x.any()
+
+
+def test_bicgstab_solves_the_helmholtz_equation():
+    eps, spacing, S = _problem(n=20, bc=3)
+    shift, scale = om.get_shift_and_scale(eps)
+    E, k, rs = om.solve_bicgstab(eps, S, spacing, K0, shift, scale, maxiter=300, tol=1e-9)
+    m = om.MBS(eps, spacing, K0, shift, scale)
+    assert k > 0
+    assert m.pde_residual(E, S) < 1e-6
+    # the fixed point of the default solver is the same field
+    py, num = m.prec_y(S)
+    x, _ = m.iterate(py.copy(), py, 600)
+    assert np.linalg.norm(E - x * num) / np.linalg.norm(x * num) < 1e-6
