@@ -45,11 +45,12 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--config", default="m0",
-                   choices=["m0", "c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8", "c9"],
-                   help="m0: the headline batched 1024^2 AS propagation (default); c1-c9: the "
+                   choices=["m0", "c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8", "c9", "c10"],
+                   help="m0: the headline batched 1024^2 AS propagation (default); c1-c10: the "
                         "other workloads (plain AS with padding, 4f PSF + gradient, multislice + "
                         "adjoint, Born solves, serving, scaled-convolution propagation, high-NA "
-                        "lens, fluorescent multislice, Microscope imaging), one JSON line each")
+                        "lens, fluorescent multislice, Microscope imaging, Born solves by "
+                        "BiCGStab), one JSON line each")
     p.add_argument("--summarize", nargs=3, metavar=("TRACE_DIR", "FETCH_DIR", "WRITE_DIR"),
                    help="write the rocprofv3 summaries of a profiled run (kernel-trace --stats, "
                         "--pmc FETCH_SIZE, --pmc WRITE_SIZE; csv output) and exit")
@@ -351,17 +352,10 @@ def bench_c3(args):
                                            "checkpoint_every": k})
 
 
-def bench_c4(args):
-    """Modified Born series: 8 volumes of 128^3 (+16-voxel absorbing
-    boundary), 20 spheres each, solved to a relative residual of 1e-4."""
-    import torch
-
-    from chromatix_amd.experimental import modified_born_series as mbs
-
-    lam = 0.532
-    k0, sp = 2 * np.pi / lam, lam / 4
-    n, bc, vols = 128, 16, 8
-    times, iters = [], []
+def _mbs_volumes(torch, mbs, n, bc, vols, k0, sp):
+    """The c4 volumes: 128^3 of n = 1.33 with 20 spheres (radius 4-12
+    voxels, n in U[1.36, 1.40], seeded), a 16-voxel absorbing boundary and
+    an x-polarised plane current sheet."""
     for v in range(vols):
         rng = np.random.default_rng(4 + v)
         eps = np.full((n, n, n), 1.33**2, np.complex64)
@@ -375,21 +369,56 @@ def bench_c4(args):
                                       n_background=1.33)
         J = torch.zeros(sample.shape + (3,), dtype=torch.complex64, device="cuda")
         J[:, :, bc + 1, 0] = 1.0
-        src = mbs.Source(J, k0)
+        yield sample, mbs.Source(J, k0)
+
+
+def bench_c4(args, use_bicgstab=False):
+    """Modified Born series: 8 volumes of 128^3 (+16-voxel absorbing
+    boundary), 20 spheres each, solved to a relative residual of 1e-4
+    (|prec_y - prec_forward(x)| <= 1e-4 |prec_y|: the fixed point's update
+    norm, BiCGStab's residual)."""
+    import torch
+
+    from chromatix_amd.experimental import modified_born_series as mbs
+
+    lam = 0.532
+    k0, sp = 2 * np.pi / lam, lam / 4
+    n, bc, vols = 128, 16, 8
+    times, iters = [], []
+    for sample, src in _mbs_volumes(torch, mbs, n, bc, vols, k0, sp):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = mbs.solve(sample, src, maxiter=3000, rtol=1e-4, return_info=True)
+        if use_bicgstab:
+            res = mbs.solve(sample, src, maxiter=3000, tol=1e-4, use_bicgstab=True, return_info=True)
+        else:
+            res = mbs.solve(sample, src, maxiter=3000, rtol=1e-4, return_info=True)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
         iters.append(res.iterations)
     N3 = (n + 2 * bc) ** 3
     per_it = sum(times) / sum(iters)
+    if use_bicgstab:
+        # per iteration: two operator applications (V folded into the vector
+        # kernels; 3D FFT = 3 passes of 48 B/voxel each way, Green 48 B) and
+        # the five vector kernels (128 + 104 + 104 + 80 + 168 B/voxel)
+        per_iter_bytes = 2 * (2 * 144 + 48) + 584
+        return _result(args, "modified Born series BiCGStab solves/s (128^3 + boundary, tol 1e-4)",
+                       vols / sum(times), "solves/s", sum(times) / vols * 1e3, vols, 0, "complex64",
+                       "8 volumes 160^3 (128^3 + 16-voxel boundary), 20 spheres, plane-wave current "
+                       "sheet, BiCGStab to relative residual 1e-4",
+                       per_iter_bytes * N3 * (sum(iters) / vols), {}, None,
+                       {"iterations": iters, "ms_per_iteration": per_it * 1e3})
     return _result(args, "modified Born series solves/s (128^3 + boundary, rtol 1e-4)",
                    vols / sum(times), "solves/s", sum(times) / vols * 1e3, vols, 0, "complex64",
                    "8 volumes 160^3 (128^3 + 16-voxel boundary), 20 spheres, plane-wave current "
                    "sheet, to relative residual 1e-4",
                    272 * N3 * (sum(iters) / vols), {}, None,
                    {"iterations": iters, "ms_per_iteration": per_it * 1e3})
+
+
+def bench_c10(args):
+    """bench_c4's volumes solved with BiCGStab (solve(use_bicgstab=True))."""
+    return bench_c4(args, use_bicgstab=True)
 
 
 def bench_c5(args):
@@ -575,7 +604,7 @@ def bench_c9(args):
 
 
 CONFIGS = {"c1": bench_c1, "c2": bench_c2, "c3": bench_c3, "c4": bench_c4, "c5": bench_c5,
-           "c6": bench_c6, "c7": bench_c7, "c8": bench_c8, "c9": bench_c9}
+           "c6": bench_c6, "c7": bench_c7, "c8": bench_c8, "c9": bench_c9, "c10": bench_c10}
 
 
 def main():
