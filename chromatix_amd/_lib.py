@@ -161,7 +161,7 @@ class MbsDesc(ctypes.Structure):
         ("nx", ctypes.c_int32),
         ("ny", ctypes.c_int32),
         ("nz", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("eps_tensor", ctypes.c_int32),
         ("spacing", ctypes.c_double),
         ("k0", ctypes.c_double),
         ("bias_re", ctypes.c_double),

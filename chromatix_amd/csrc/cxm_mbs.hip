@@ -5,7 +5,8 @@
 // Fields are three complex64 component volumes (component-planar
 // 3 x nx x ny x nz in the kernels; the boundary converts from the
 // reference's nx x ny x nz x 3).  With s = scale, b = bias, kb = b s + 1:
-//   V(x)  = x (s eps - kb)                                   (elementwise)
+//   V(x)  = (s eps - kb) x    (eps a scalar per voxel, or a 3x3 matrix per
+//                              voxel: the tensorial extension)
 //   G(y)  = ifftn( P_T y^ / (kb - s |K|^2) + P_L y^ / kb )   (K = k / k0)
 //   prec(y)          = -V(G(s y))
 //   prec_forward(x)  = -V(G(V(x)) + x)
@@ -31,6 +32,7 @@ int mbs_geom(const cxm_mbs_desc* d, MbsGeom& g) {
   if (d->nx <= 0 || d->ny <= 0 || d->nz <= 0)
     return fail(CXM_ERR_INVALID, "invalid shape");
   if (!(d->spacing > 0 && d->k0 > 0)) return fail(CXM_ERR_INVALID, "spacing and k0 must be positive");
+  if (d->eps_tensor != 0 && d->eps_tensor != 1) return fail(CXM_ERR_INVALID, "eps_tensor must be 0 or 1");
   for (int n : {d->nx, d->ny, d->nz})
     if (!fft_len_supported(n))
       return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(n));
@@ -43,25 +45,50 @@ struct C2 { float re, im; };
 
 CXM_DEV float2 cmulf(float2 a, C2 b) { return make_float2(a.x * b.re - a.y * b.im, a.x * b.im + a.y * b.re); }
 
-// out[c][i] = src[c][i] * (s eps[i] - kb) * pre; src interleaved (i*3 + c)
-// when interleaved, else planar; eps may be null (factor 1 then, used for the
-// plain scaling of the source)
-__global__ void mbs_v_kernel(const float2* __restrict__ src, const float2* __restrict__ eps,
-                             float2* __restrict__ out, long long vox, int interleaved, C2 s, C2 kb,
-                             C2 pre) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= vox) return;
-  C2 f = pre;
-  if (eps) {
+// V applied to the field vector v at voxel i: o = (s eps - kb) v, eps a
+// scalar (eps[i]) or, with `tensor`, a 3x3 matrix (eps[(3 a + b) vox + i]).
+CXM_DEV void apply_v(const float2* __restrict__ eps, int tensor, long long i, long long vox, C2 s,
+                     C2 kb, const float2 (&v)[3], float2 (&o)[3]) {
+  if (!tensor) {
     const float2 e = eps[i];
-    const float2 se = make_float2(s.re * e.x - s.im * e.y - kb.re, s.re * e.y + s.im * e.x - kb.im);
-    f = C2{se.x * pre.re - se.y * pre.im, se.x * pre.im + se.y * pre.re};
+    const C2 f{s.re * e.x - s.im * e.y - kb.re, s.re * e.y + s.im * e.x - kb.im};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = cmulf(v[c], f);
+    return;
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const float2 v = interleaved ? src[i * 3 + c] : src[c * vox + i];
-    out[c * vox + i] = cmulf(v, f);
+  for (int a = 0; a < 3; ++a) {
+    float2 acc = cmulf(v[a], C2{-kb.re, -kb.im});
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const float2 e = eps[(long long)(3 * a + b) * vox + i];
+      const C2 se{s.re * e.x - s.im * e.y, s.re * e.y + s.im * e.x};
+      const float2 t = cmulf(v[b], se);
+      acc = make_float2(acc.x + t.x, acc.y + t.y);
+    }
+    o[a] = acc;
   }
+}
+
+// out[c][i] = pre * V(src)[c][i]; src interleaved (i*3 + c) when
+// interleaved, else planar; eps may be null (out = pre * src then, used for
+// the plain scaling of the source)
+__global__ void mbs_v_kernel(const float2* __restrict__ src, const float2* __restrict__ eps,
+                             float2* __restrict__ out, long long vox, int interleaved, int tensor, C2 s,
+                             C2 kb, C2 pre) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vox) return;
+  float2 v[3], o[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v[c] = interleaved ? src[i * 3 + c] : src[c * vox + i];
+  if (eps) {
+    apply_v(eps, tensor, i, vox, s, kb, v, o);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o[c] = v[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[c * vox + i] = cmulf(o[c], pre);
 }
 
 // In-place Green multiply of the spectrum Y (3 planar components):
@@ -104,21 +131,24 @@ constexpr int kRedThreads = 256;
 // r = prec_y + V(g + x);  x <- x + r;  partial[block] = sum |r|^2 (FP64)
 __global__ void __launch_bounds__(kRedThreads)
 mbs_update_kernel(const float2* __restrict__ g, const float2* __restrict__ eps,
-                  const float2* __restrict__ py, float2* __restrict__ x, long long vox, C2 s, C2 kb,
-                  double* __restrict__ partial) {
+                  const float2* __restrict__ py, float2* __restrict__ x, long long vox, int tensor,
+                  C2 s, C2 kb, double* __restrict__ partial) {
   double acc = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < vox;
        i += (long long)gridDim.x * blockDim.x) {
-    const float2 e = eps[i];
-    const C2 f{s.re * e.x - s.im * e.y - kb.re, s.re * e.y + s.im * e.x - kb.im};
+    float2 xv[3], t[3], v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xv[c] = x[c * vox + i];
+      const float2 gv = g[c * vox + i];
+      t[c] = make_float2(gv.x + xv[c].x, gv.y + xv[c].y);
+    }
+    apply_v(eps, tensor, i, vox, s, kb, t, v);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long j = c * vox + i;
-      const float2 xv = x[j];
-      const float2 t = make_float2(g[j].x + xv.x, g[j].y + xv.y);
-      const float2 v = cmulf(t, f);
-      const float2 r = make_float2(py[j].x + v.x, py[j].y + v.y);
-      x[j] = make_float2(xv.x + r.x, xv.y + r.y);
+      const float2 r = make_float2(py[j].x + v[c].x, py[j].y + v[c].y);
+      x[j] = make_float2(xv[c].x + r.x, xv[c].y + r.y);
       acc += (double)r.x * r.x + (double)r.y * r.y;
     }
   }
@@ -296,11 +326,6 @@ CXM_DEV void cdot_acc(float2 a, float2 b, double& re, double& im) {
   re += (double)a.x * b.x + (double)a.y * b.y;
   im += (double)a.x * b.y - (double)a.y * b.x;
 }
-// V factor at voxel i: s eps - kb
-CXM_DEV float2 vfac(const float2* eps, long long i, C2 s, C2 kb) {
-  const float2 e = eps[i];
-  return make_float2(s.re * e.x - s.im * e.y - kb.re, s.re * e.y + s.im * e.x - kb.im);
-}
 
 // Block sums of NV doubles to partial[block * 4 + j] (fixed order).
 template <int NV>
@@ -330,16 +355,19 @@ CXM_DEV void block_sums(double (&v)[NV], double* partial) {
 __global__ void __launch_bounds__(kRedThreads)
 bi_r0_kernel(const float2* __restrict__ g, const float2* __restrict__ eps, const float2* __restrict__ b,
              const float2* __restrict__ x, float2* __restrict__ r, float2* __restrict__ rh,
-             float2* __restrict__ p, float2* __restrict__ q, long long vox, C2 s, C2 kb,
+             float2* __restrict__ p, float2* __restrict__ q, long long vox, int tensor, C2 s, C2 kb,
              double* __restrict__ partial) {
   double acc[2] = {0.0, 0.0};
   BI_LOOP(i) {
-    const float2 f = vfac(eps, i, s, kb);
+    float2 t[3], v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) t[c] = cadd2(g[c * vox + i], x[c * vox + i]);
+    apply_v(eps, tensor, i, vox, s, kb, t, v);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long j = c * vox + i;
       const float2 bv = b[j];
-      const float2 rv = cadd2(bv, cmul2(cadd2(g[j], x[j]), f));
+      const float2 rv = cadd2(bv, v[c]);
       r[j] = rv;
       rh[j] = rv;
       p[j] = rv;
@@ -412,36 +440,40 @@ __global__ void bi_beta_kernel(BiScal* st) {
 // p = r + beta (p - omega q);  g = V(p)
 __global__ void __launch_bounds__(kRedThreads)
 bi_p_kernel(const float2* __restrict__ r, float2* __restrict__ p, const float2* __restrict__ q,
-            const float2* __restrict__ eps, float2* __restrict__ g, long long vox, C2 s, C2 kb,
-            const BiScal* st) {
+            const float2* __restrict__ eps, float2* __restrict__ g, long long vox, int tensor, C2 s,
+            C2 kb, const BiScal* st) {
   if (st->done != 0.0) return;
   const float2 beta = cf(st->beta), omega = cf(st->omega);
   BI_LOOP(i) {
-    const float2 f = vfac(eps, i, s, kb);
+    float2 pv[3], v[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long j = c * vox + i;
-      const float2 pv = cadd2(r[j], cmul2(beta, csub2(p[j], cmul2(omega, q[j]))));
-      p[j] = pv;
-      g[j] = cmul2(pv, f);
+      pv[c] = cadd2(r[j], cmul2(beta, csub2(p[j], cmul2(omega, q[j]))));
+      p[j] = pv[c];
     }
+    apply_v(eps, tensor, i, vox, s, kb, pv, v);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[c * vox + i] = v[c];
   }
 }
 
 // q = A p = -V(g + p); partial <rh, q>
 __global__ void __launch_bounds__(kRedThreads)
 bi_q_kernel(const float2* __restrict__ g, const float2* __restrict__ p, const float2* __restrict__ rh,
-            const float2* __restrict__ eps, float2* __restrict__ q, long long vox, C2 s, C2 kb,
-            const BiScal* st, double* __restrict__ partial) {
+            const float2* __restrict__ eps, float2* __restrict__ q, long long vox, int tensor, C2 s,
+            C2 kb, const BiScal* st, double* __restrict__ partial) {
   if (st->done != 0.0) return;
   double acc[2] = {0.0, 0.0};
   BI_LOOP(i) {
-    const float2 f = vfac(eps, i, s, kb);
+    float2 t[3], w[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) t[c] = cadd2(g[c * vox + i], p[c * vox + i]);
+    apply_v(eps, tensor, i, vox, s, kb, t, w);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long j = c * vox + i;
-      const float2 v = cmul2(cadd2(g[j], p[j]), f);
-      const float2 qv = make_float2(-v.x, -v.y);
+      const float2 qv = make_float2(-w[c].x, -w[c].y);
       q[j] = qv;
       cdot_acc(rh[j], qv, acc[0], acc[1]);
     }
@@ -461,21 +493,23 @@ bi_alpha_kernel(const double* __restrict__ partial, BiScal* st) {
 // s = r - alpha' q (into r);  g = V(s);  partial |s|^2
 __global__ void __launch_bounds__(kRedThreads)
 bi_s_kernel(float2* __restrict__ r, const float2* __restrict__ q, const float2* __restrict__ eps,
-            float2* __restrict__ g, long long vox, C2 s, C2 kb, const BiScal* st,
+            float2* __restrict__ g, long long vox, int tensor, C2 s, C2 kb, const BiScal* st,
             double* __restrict__ partial) {
   if (st->done != 0.0) return;
   const float2 alpha = cf(st->alpha_n);
   double acc[1] = {0.0};
   BI_LOOP(i) {
-    const float2 f = vfac(eps, i, s, kb);
+    float2 sv[3], v[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long j = c * vox + i;
-      const float2 sv = csub2(r[j], cmul2(alpha, q[j]));
-      r[j] = sv;
-      g[j] = cmul2(sv, f);
-      acc[0] += (double)sv.x * sv.x + (double)sv.y * sv.y;
+      sv[c] = csub2(r[j], cmul2(alpha, q[j]));
+      r[j] = sv[c];
+      acc[0] += (double)sv[c].x * sv[c].x + (double)sv[c].y * sv[c].y;
     }
+    apply_v(eps, tensor, i, vox, s, kb, sv, v);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) g[c * vox + i] = v[c];
   }
   block_sums<1>(acc, partial);
 }
@@ -491,17 +525,19 @@ bi_early_kernel(const double* __restrict__ partial, BiScal* st) {
 // t = A s = -V(g + s) (into g); partials <t, s>, <t, t>
 __global__ void __launch_bounds__(kRedThreads)
 bi_t_kernel(float2* __restrict__ g, const float2* __restrict__ sv_, const float2* __restrict__ eps,
-            long long vox, C2 s, C2 kb, const BiScal* st, double* __restrict__ partial) {
+            long long vox, int tensor, C2 s, C2 kb, const BiScal* st, double* __restrict__ partial) {
   if (st->done != 0.0) return;
   double acc[3] = {0.0, 0.0, 0.0};
   BI_LOOP(i) {
-    const float2 f = vfac(eps, i, s, kb);
+    float2 u[3], w[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c] = cadd2(g[c * vox + i], sv_[c * vox + i]);
+    apply_v(eps, tensor, i, vox, s, kb, u, w);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const long long j = c * vox + i;
       const float2 sv = sv_[j];
-      const float2 v = cmul2(cadd2(g[j], sv), f);
-      const float2 tv = make_float2(-v.x, -v.y);
+      const float2 tv = make_float2(-w[c].x, -w[c].y);
       g[j] = tv;
       cdot_acc(tv, sv, acc[0], acc[1]);
       acc[2] += (double)tv.x * tv.x + (double)tv.y * tv.y;
@@ -612,7 +648,7 @@ extern "C" int cxm_mbs_precondition(void* stream, const void* eps, const void* s
   double* partial = (double*)((char*)workspace + (size_t)3 * g.vox * sizeof(float2));
   // t = s * source (planar), G(t), prec_y = -V(G(t))
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)source,
-                     nullptr, py, g.vox, 1, sc, kb, sc);
+                     nullptr, py, g.vox, 1, 0, sc, kb, sc);
   CXM_LAUNCH_CHECK();
   if ((rc = fft3(g, py, -1, s))) return rc;
   const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
@@ -621,7 +657,7 @@ extern "C" int cxm_mbs_precondition(void* stream, const void* eps, const void* s
   CXM_LAUNCH_CHECK();
   if ((rc = fft3(g, py, 1, s))) return rc;
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)py,
-                     (const float2*)eps, py, g.vox, 0, sc, kb, C2{-1.0f, 0.0f});
+                     (const float2*)eps, py, g.vox, 0, d->eps_tensor, sc, kb, C2{-1.0f, 0.0f});
   CXM_LAUNCH_CHECK();
   // num_scale = max |prec_y|; prec_y /= num_scale
   hipLaunchKernelGGL(maxabs_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, (const float2*)py,
@@ -655,7 +691,7 @@ extern "C" int cxm_mbs_iterate(void* stream, const void* eps, const void* prec_y
   const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
   for (int it = 0; it < iterations; ++it) {
     hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)x,
-                       (const float2*)eps, gb, g.vox, 0, sc, kb, C2{1.0f, 0.0f});
+                       (const float2*)eps, gb, g.vox, 0, d->eps_tensor, sc, kb, C2{1.0f, 0.0f});
     CXM_LAUNCH_CHECK();
     if ((rc = fft3(g, gb, -1, s))) return rc;
     hipLaunchKernelGGL(mbs_green_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, gb, g.vox, g.nx,
@@ -664,7 +700,7 @@ extern "C" int cxm_mbs_iterate(void* stream, const void* eps, const void* prec_y
     if ((rc = fft3(g, gb, 1, s))) return rc;
     hipLaunchKernelGGL(mbs_update_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s,
                        (const float2*)gb, (const float2*)eps, (const float2*)prec_y, (float2*)x,
-                       g.vox, sc, kb, partial);
+                       g.vox, d->eps_tensor, sc, kb, partial);
     CXM_LAUNCH_CHECK();
   }
   if (iterations > 0) {
@@ -758,12 +794,12 @@ extern "C" int cxm_mbs_bicgstab_init(void* stream, const void* eps, const void* 
   const BiLayout b = bi_layout(g, state);
   // r0 = b - A x0 = b + V(G(V(x0)) + x0)
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)x,
-                     (const float2*)eps, gb, g.vox, 0, sc, kb, C2{1.0f, 0.0f});
+                     (const float2*)eps, gb, g.vox, 0, d->eps_tensor, sc, kb, C2{1.0f, 0.0f});
   CXM_LAUNCH_CHECK();
   if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
   hipLaunchKernelGGL(bi_r0_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, (const float2*)gb,
                      (const float2*)eps, (const float2*)prec_y, (const float2*)x, b.r, b.rh, b.p, b.q,
-                     g.vox, sc, kb, b.partial);
+                     g.vox, d->eps_tensor, sc, kb, b.partial);
   CXM_LAUNCH_CHECK();
   hipLaunchKernelGGL(bi_init_scalars_kernel, dim3(1), dim3(kRedThreads), 0, s, (const double*)b.partial,
                      b.st, tol, atol, status);
@@ -790,22 +826,23 @@ extern "C" int cxm_mbs_bicgstab_iterate(void* stream, const void* eps, void* x, 
     hipLaunchKernelGGL(bi_beta_kernel, dim3(1), dim3(1), 0, s, b.st);
     CXM_LAUNCH_CHECK();
     hipLaunchKernelGGL(bi_p_kernel, rg, rt, 0, s, (const float2*)b.r, b.p, (const float2*)b.q, ep, gb,
-                       g.vox, sc, kb, (const BiScal*)b.st);
+                       g.vox, d->eps_tensor, sc, kb, (const BiScal*)b.st);
     CXM_LAUNCH_CHECK();
     if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
     hipLaunchKernelGGL(bi_q_kernel, rg, rt, 0, s, (const float2*)gb, (const float2*)b.p,
-                       (const float2*)b.rh, ep, b.q, g.vox, sc, kb, (const BiScal*)b.st, b.partial);
+                       (const float2*)b.rh, ep, b.q, g.vox, d->eps_tensor, sc, kb, (const BiScal*)b.st,
+                       b.partial);
     CXM_LAUNCH_CHECK();
     hipLaunchKernelGGL(bi_alpha_kernel, dim3(1), rt, 0, s, (const double*)b.partial, b.st);
     CXM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bi_s_kernel, rg, rt, 0, s, b.r, (const float2*)b.q, ep, gb, g.vox, sc, kb,
-                       (const BiScal*)b.st, b.partial);
+    hipLaunchKernelGGL(bi_s_kernel, rg, rt, 0, s, b.r, (const float2*)b.q, ep, gb, g.vox, d->eps_tensor, sc,
+                       kb, (const BiScal*)b.st, b.partial);
     CXM_LAUNCH_CHECK();
     hipLaunchKernelGGL(bi_early_kernel, dim3(1), rt, 0, s, (const double*)b.partial, b.st);
     CXM_LAUNCH_CHECK();
     if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
-    hipLaunchKernelGGL(bi_t_kernel, rg, rt, 0, s, gb, (const float2*)b.r, ep, g.vox, sc, kb,
-                       (const BiScal*)b.st, b.partial);
+    hipLaunchKernelGGL(bi_t_kernel, rg, rt, 0, s, gb, (const float2*)b.r, ep, g.vox, d->eps_tensor, sc,
+                       kb, (const BiScal*)b.st, b.partial);
     CXM_LAUNCH_CHECK();
     hipLaunchKernelGGL(bi_omega_kernel, dim3(1), rt, 0, s, (const double*)b.partial, b.st);
     CXM_LAUNCH_CHECK();

@@ -33,16 +33,25 @@ __all__ = ["Sample", "EmptySample", "Source", "add_absorbing_bc", "get_shift_and
 
 
 class Sample:
-    """Permittivity on a uniform grid (sample.py:9-73), (Nx, Ny, Nz)."""
+    """Permittivity on a uniform grid (sample.py:9-73): (Nx, Ny, Nz), or
+    (3, 3, Nx, Ny, Nz) for a permittivity tensor per voxel (the layout the
+    reference's solver docstrings give for anisotropic materials)."""
 
     def __init__(self, permittivity, spacing: float, roi=None):
         self.permittivity = torch.as_tensor(permittivity)
+        if self.permittivity.ndim not in (3, 5) or (
+                self.permittivity.ndim == 5 and tuple(self.permittivity.shape[:2]) != (3, 3)):
+            raise ValueError("permittivity must be (Nx, Ny, Nz) or (3, 3, Nx, Ny, Nz)")
         self.spacing = float(spacing)
         self.ROI = roi if roi is not None else tuple(slice(0, s) for s in self.shape)
 
     @property
+    def tensor(self) -> bool:
+        return self.permittivity.ndim == 5
+
+    @property
     def shape(self):
-        return tuple(self.permittivity.shape)
+        return tuple(self.permittivity.shape[-3:])
 
     @property
     def grid(self) -> np.ndarray:
@@ -91,11 +100,18 @@ def add_absorbing_bc(sample: Sample, axis, thickness: float, max_extinction: flo
     widths = [(w, w) if i in axes else (0, 0) for i in range(3)]
     ext = np.pad(np.zeros(sample.shape), widths, mode="linear_ramp", end_values=max_extinction)
     eps_bc = (n_background + 1j * ext) ** 2
-    perm = np.pad(sample.permittivity.detach().cpu().numpy().astype(np.complex128), widths,
-                  mode="constant", constant_values=n_background**2)
+    eps = sample.permittivity.detach().cpu().numpy().astype(np.complex128)
     roi = tuple(slice(a, a + s) for (a, _), s in zip(widths, sample.shape))
-    out = torch.as_tensor((perm + (eps_bc - n_background**2)).astype(np.complex64),
-                          device=sample.permittivity.device)
+    if sample.tensor:
+        # isotropic background n^2 I with the extinction ramp on the diagonal
+        perm = np.pad(eps, [(0, 0), (0, 0)] + widths, mode="constant", constant_values=0.0)
+        for a in range(3):
+            perm[a, a] = np.pad(eps[a, a], widths, mode="constant", constant_values=n_background**2)
+            perm[a, a] += eps_bc - n_background**2
+    else:
+        perm = np.pad(eps, widths, mode="constant", constant_values=n_background**2)
+        perm = perm + (eps_bc - n_background**2)
+    out = torch.as_tensor(perm.astype(np.complex64), device=sample.permittivity.device)
     return Sample(out, sample.spacing, roi)
 
 
@@ -140,11 +156,41 @@ def enclosing_circle(points) -> tuple[complex, float]:
     return c, r
 
 
+def _tensor_shift(eps: np.ndarray) -> tuple[complex, float]:
+    """Shift c minimising max over voxels of the spectral norm
+    ||eps - c I||_2 and that norm, for (3, 3, ...) permittivities: the centre
+    of the circle enclosing every eigenvalue when all the matrices are
+    normal (then the norm is the largest |eigenvalue - c|), refined by a
+    Nelder-Mead minimisation of the convex maximum otherwise."""
+    from scipy.optimize import minimize
+
+    E = np.unique(eps.reshape(9, -1).T, axis=0).reshape(-1, 3, 3)
+    c0, _ = enclosing_circle(np.linalg.eigvals(E))
+    I3 = np.eye(3)
+
+    def norm(c):
+        return float(np.max(np.linalg.norm(E - c * I3, ord=2, axis=(1, 2))))
+
+    comm = E @ np.conj(np.swapaxes(E, 1, 2)) - np.conj(np.swapaxes(E, 1, 2)) @ E
+    scale = max(1.0, float(np.max(np.abs(E))) ** 2)
+    if float(np.max(np.abs(comm))) <= 1e-12 * scale:
+        return c0, norm(c0)
+    res = minimize(lambda p: norm(complex(p[0], p[1])), np.array([c0.real, c0.imag]),
+                   method="Nelder-Mead", options={"xatol": 1e-10, "fatol": 1e-12, "maxiter": 4000})
+    c = complex(res.x[0], res.x[1])
+    return (c, norm(c)) if norm(c) < norm(c0) else (c0, norm(c0))
+
+
 def get_shift_and_scale(permittivity) -> tuple[complex, complex]:
     """(shift, scale) with shift minimising max |eps - shift| and
-    scale = 1 / (1.1 i sqrt(f)), f = max |eps - shift|^2 (solver.py:10-39)."""
+    scale = 1 / (1.1 i sqrt(f)), f = max |eps - shift|^2 (solver.py:10-39);
+    for a (3, 3, ...) permittivity tensor |.| is the spectral norm of
+    eps - shift I (the norm the reference's note asks for)."""
     eps = torch.as_tensor(permittivity).detach().cpu().numpy()
-    c, r = enclosing_circle(eps)
+    if eps.ndim == 5:
+        c, r = _tensor_shift(eps.astype(np.complex128))
+    else:
+        c, r = enclosing_circle(eps)
     if r == 0:
         r = 1e-6  # homogeneous medium: any positive radius keeps the iteration accretive
     return c, 1.0 / (1.1j * r)
@@ -158,6 +204,7 @@ class MBSResult:
 def _desc(sample: Sample, k0: float, shift: complex, scale: complex):
     d = L.MbsDesc()
     d.nx, d.ny, d.nz = sample.shape
+    d.eps_tensor = 1 if sample.tensor else 0
     d.spacing = sample.spacing
     d.k0 = float(k0)
     d.bias_re, d.bias_im = shift.real, shift.imag

@@ -517,7 +517,9 @@ int cxm_multislice_bwd(void* stream, const void* in, const float* dn, const floa
 /*
  * Modified Born series solve (src/chromatix/experimental/modified_born_series/solver.py:42-146)
  * of (curl curl - k0^2 eps) E = -k0^2 S for a scalar permittivity eps
- * (nx x ny x nz complex64) and the pre-scaled source S (Source.field,
+ * (nx x ny x nz complex64) or a 3x3 permittivity tensor per voxel (eps_tensor;
+ * the tensorial permittivity the reference lists as future work,
+ * modified_born_series/README.md) and the pre-scaled source S (Source.field,
  * sample.py:79-85; nx x ny x nz x 3 complex64), with the bias (shift) and
  * scale of get_shift_and_scale chosen by the caller:
  *   cxm_mbs_precondition: prec_y = prec(S) / num_scale, num_scale = max |prec(S)|
@@ -531,7 +533,8 @@ int cxm_multislice_bwd(void* stream, const void* in, const float* dn, const floa
  */
 typedef struct cxm_mbs_desc {
   int32_t nx, ny, nz;     /* grid including absorbing boundaries */
-  int32_t reserved;
+  int32_t eps_tensor;     /* 0: eps nx x ny x nz; 1: eps 3 x 3 x nx x ny x nz (eps_ab
+                             at (3 a + b) voxels + i), V(x) = (s eps - kb) x per voxel */
   double spacing, k0;     /* grid spacing, vacuum wavenumber (same length unit) */
   double bias_re, bias_im;
   double scale_re, scale_im;

@@ -9,6 +9,11 @@ Follows (paths relative to the chromatix repository root):
       BiCGStab, no preconditioner M, stop on |r|^2 <= max(tol^2 |b|^2, atol^2)
       or breakdown) and is pinned against scipy.sparse.linalg.bicgstab's
       iterates and numpy.linalg.solve on dense systems.
+  tensorial permittivity (3, 3, Nx, Ny, Nz): listed as future work in
+      modified_born_series/README.md; here V(x) = (s eps - kb I) x per voxel
+      and the shift minimises the largest spectral norm of eps - shift I,
+      pinned by the PDE residual with the tensor and by eps = e I reducing to
+      the scalar problem (tests/test_oracle_mbs.py).
 Fields are (Nx, Ny, Nz, 3) complex128.  get_shift_and_scale minimises
 max |eps - s|^2 directly (scipy Nelder-Mead from the reference's start
 point [1.6, 0.25]); the solution is pinned by the residual of the PDE it
@@ -54,15 +59,53 @@ def get_shift_and_scale(eps):
     return shift, 1 / (1.1j * np.sqrt(loss(res.x)))
 
 
+def get_shift_and_scale_tensor(eps):
+    """Tensor permittivity: shift minimising max ||eps - shift I||_2 (direct
+    Nelder-Mead over the distinct matrices from the mean diagonal)."""
+    E = np.unique(np.asarray(eps, np.complex128).reshape(9, -1).T, axis=0).reshape(-1, 3, 3)
+
+    def loss(p):
+        return np.max(np.linalg.norm(E - (p[0] + 1j * p[1]) * np.eye(3), ord=2, axis=(1, 2)))
+
+    c0 = np.mean(np.trace(E, axis1=1, axis2=2)) / 3
+    res = minimize(loss, np.array([c0.real, c0.imag]), method="Nelder-Mead",
+                   options={"xatol": 1e-12, "fatol": 1e-14, "maxiter": 20000})
+    return res.x[0] + 1j * res.x[1], 1 / (1.1j * loss(res.x))
+
+
+def add_absorbing_bc_tensor(eps, spacing, axis, thickness, max_extinction=0.25, n_background=1.0):
+    """add_absorbing_bc for (3, 3, ...) permittivities: background n^2 I,
+    extinction ramp on the diagonal."""
+    eps = np.asarray(eps, np.complex128)
+    diag = [add_absorbing_bc(eps[a, a], spacing, axis, thickness, max_extinction, n_background)
+            for a in range(3)]
+    out = np.zeros((3, 3) + diag[0].shape, np.complex128)
+    axes = axis if isinstance(axis, tuple) else (axis,)
+    w = int(thickness / spacing)
+    widths = [(w, w) if i in axes else (0, 0) for i in range(3)]
+    for a in range(3):
+        for b in range(3):
+            out[a, b] = diag[a] if a == b else np.pad(eps[a, b], widths)
+    return out
+
+
 class MBS:
     def __init__(self, eps, spacing, k0, shift, scale):
         self.eps = np.asarray(eps, np.complex128)
-        self.K = k_grid(self.eps.shape, spacing) / k0
+        self.tensor = self.eps.ndim == 5
+        shape = self.eps.shape[-3:]
+        self.K = k_grid(shape, spacing) / k0
         self.K2 = np.sum(self.K**2, axis=-1, keepdims=True)
         self.scale, self.kb = scale, shift * scale + 1
 
+    def eps_mul(self, E):
+        """eps E per voxel (scalar or 3x3)."""
+        if self.tensor:
+            return np.einsum("abxyz,xyzb->xyza", self.eps, E)
+        return self.eps[..., None] * E
+
     def V(self, x):
-        return x * (self.scale * self.eps[..., None] - self.kb)
+        return self.scale * self.eps_mul(x) - self.kb * x
 
     def G(self, y):
         yf = np.fft.fftn(y, axes=ax3)
@@ -94,7 +137,7 @@ class MBS:
         with np.errstate(invalid="ignore", divide="ignore"):
             El = np.where(self.K2 != 0, self.K / self.K2 * proj, 0.0)
         curl = np.fft.ifftn(self.K2 * (Ef - El), axes=ax3)
-        return np.linalg.norm(curl - self.eps[..., None] * E + S) / np.linalg.norm(S)
+        return np.linalg.norm(curl - self.eps_mul(E) + S) / np.linalg.norm(S)
 
 
 def bicgstab(A, b, x0, maxiter, tol=1e-5, atol=0.0):

@@ -149,3 +149,59 @@ def test_bicgstab_stops_and_keeps_state():
     sv.event.synchronize()
     sv.poll()
     assert sv.stopped and torch.equal(sv.x, x1)
+
+
+def _aniso_gpu(n=24, bc=4, seed=7):
+    from tests.test_oracle_mbs import _aniso
+
+    T, spacing, S = _aniso(n=n, bc=bc, seed=seed)
+    J = np.zeros(T.shape[2:] + (3,), np.complex128)
+    J[:, :, bc + 1, 0] = 1.0
+    return T, spacing, J, om.source_field(J, K0)
+
+
+@pytest.mark.parametrize("bicg", [False, True])
+def test_tensor_iterates_match_oracle(bicg):
+    T, spacing, J, S = _aniso_gpu()
+    shift, scale = om.get_shift_and_scale_tensor(T)
+    m = om.MBS(T, spacing, K0, shift, scale)
+    py, num = m.prec_y(S)
+    iters = 4 if bicg else 40
+    if bicg:
+        x_ref, _, _ = om.bicgstab(m.prec_forward, py, py, iters, tol=0.0)
+    else:
+        x_ref, _ = m.iterate(py.copy(), py, iters)
+    sample = mbs.Sample(torch.as_tensor(T.astype(np.complex64)).cuda(), spacing)
+    assert sample.tensor
+    src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
+    res = mbs.solve(sample, src, maxiter=iters, tol=0.0, shift_scale=(shift, scale), use_bicgstab=bicg,
+                    return_info=True)
+    err = rel_l2(res.E.cpu().numpy(), x_ref * num)
+    print(f"tensor eps 32^3, {'bicgstab' if bicg else 'fixed point'} {iters} iterations: {err:.2e}")
+    assert err < 1e-5
+
+
+def test_identity_tensor_matches_scalar_on_gpu():
+    eps, spacing, J = _setup(24, 4, 2, 5)
+    T = np.zeros((3, 3) + eps.shape, np.complex64)
+    for a in range(3):
+        T[a, a] = eps
+    shift, scale = om.get_shift_and_scale(eps)
+    src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
+    out = []
+    for e in (eps.astype(np.complex64), T):
+        sample = mbs.Sample(torch.as_tensor(e).cuda(), spacing)
+        out.append(mbs.solve(sample, src, maxiter=30, tol=0.0, shift_scale=(shift, scale)).cpu().numpy())
+    assert rel_l2(out[1], out[0]) < 1e-6
+
+
+def test_tensor_converged_solution_solves_helmholtz():
+    T, spacing, J, S = _aniso_gpu(n=32, bc=4, seed=9)
+    sample = mbs.Sample(torch.as_tensor(T.astype(np.complex64)).cuda(), spacing)
+    src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
+    res = mbs.solve(sample, src, maxiter=400, tol=1e-5, use_bicgstab=True, return_info=True)
+    shift, scale = mbs.get_shift_and_scale(sample.permittivity)
+    m = om.MBS(T, spacing, K0, shift, scale)
+    pde = m.pde_residual(res.E.cpu().numpy().astype(np.complex128), S)
+    print(f"tensor eps 40^3 bicgstab: {res.iterations} iterations, PDE residual {pde:.2e}")
+    assert pde < 1e-4

@@ -116,3 +116,84 @@ def test_bicgstab_solves_the_helmholtz_equation():
     py, num = m.prec_y(S)
     x, _ = m.iterate(py.copy(), py, 600)
     assert np.linalg.norm(E - x * num) / np.linalg.norm(x * num) < 1e-6
+
+
+def _aniso(n=20, bc=3, seed=7, normal=True):
+    """Birefringent sphere (ordinary 1.36, extraordinary 1.40 along a tilted
+    axis) in n = 1.33, as a 3x3 permittivity per voxel."""
+    rng = np.random.default_rng(seed)
+    spacing = LAM / 4
+    g = sum((np.arange(n) - n // 2).reshape([-1 if i == a else 1 for i in range(3)]) ** 2
+            for a in range(3))
+    ax = rng.standard_normal(3)
+    ax /= np.linalg.norm(ax)
+    eps_in = 1.36**2 * np.eye(3) + (1.40**2 - 1.36**2) * np.outer(ax, ax)
+    if not normal:
+        eps_in = eps_in + 0.03 * np.triu(np.ones((3, 3)), 1)
+    T = np.zeros((3, 3, n, n, n), np.complex128)
+    for a in range(3):
+        T[a, a] = 1.33**2
+    T[:, :, g <= 5**2] = eps_in[:, :, None]
+    T = om.add_absorbing_bc_tensor(T, spacing, (0, 1, 2), bc * spacing, n_background=1.33)
+    J = np.zeros(T.shape[2:] + (3,), np.complex128)
+    J[:, :, bc + 1, 0] = 1.0
+    return T, spacing, om.source_field(J, K0)
+
+
+def test_identity_tensor_reduces_to_scalar():
+    eps, spacing, S = _problem(n=16, bc=2)
+    T = np.zeros((3, 3) + eps.shape, np.complex128)
+    for a in range(3):
+        T[a, a] = eps
+    shift, scale = om.get_shift_and_scale(eps)
+    ms, mt = om.MBS(eps, spacing, K0, shift, scale), om.MBS(T, spacing, K0, shift, scale)
+    py, _ = ms.prec_y(S)
+    xs, _ = ms.iterate(py.copy(), py, 10)
+    xt, _ = mt.iterate(py.copy(), mt.prec_y(S)[0], 10)
+    assert np.abs(xs - xt).max() < 1e-12 * np.abs(xs).max()
+    st, sct = om.get_shift_and_scale_tensor(T)
+    assert abs(st - shift) < 1e-5 and abs(abs(1 / sct) - abs(1 / scale)) < 1e-6
+
+
+def test_tensor_solution_solves_the_helmholtz_equation():
+    T, spacing, S = _aniso()
+    shift, scale = om.get_shift_and_scale_tensor(T)
+    # the shifted potential is a contraction: ||s (eps - shift I)|| = 1 / 1.1
+    E = T.reshape(9, -1).T.reshape(-1, 3, 3)
+    norms = np.linalg.norm(E - shift * np.eye(3), ord=2, axis=(1, 2))
+    np.testing.assert_allclose(abs(scale) * norms.max(), 1 / 1.1, rtol=1e-9)
+    Ef, k, rs = om.solve_bicgstab(T, S, spacing, K0, shift, scale, maxiter=400, tol=1e-10)
+    m = om.MBS(T, spacing, K0, shift, scale)
+    assert k > 0 and m.pde_residual(Ef, S) < 1e-7
+    # the fixed point converges to the same field
+    py, num = m.prec_y(S)
+    x, r = m.iterate(py.copy(), py, 1500)
+    assert np.linalg.norm(x * num - Ef) / np.linalg.norm(Ef) < 1e-6
+
+
+def test_host_tensor_shift_matches_direct_minimisation():
+    import torch
+
+    from chromatix_amd.experimental import modified_born_series as mbs
+
+    for normal in (True, False):
+        T, spacing, _ = _aniso(n=12, bc=2, normal=normal)
+        c, sc = mbs.get_shift_and_scale(torch.as_tensor(T))
+        c_ref, sc_ref = om.get_shift_and_scale_tensor(T)
+        # both minimise the same convex maximum: equal optimal values
+        np.testing.assert_allclose(abs(1 / sc), abs(1 / sc_ref), rtol=1e-6)
+        assert abs(c - c_ref) < 1e-3
+
+
+def test_host_tensor_absorbing_boundary_matches_oracle():
+    import torch
+
+    from chromatix_amd.experimental import modified_born_series as mbs
+
+    rng = np.random.default_rng(8)
+    T = rng.standard_normal((3, 3, 4, 5, 6)) + 1j * rng.standard_normal((3, 3, 4, 5, 6))
+    s = mbs.add_absorbing_bc(mbs.Sample(torch.as_tensor(T.astype(np.complex64)), 0.25), (0, 2), 0.75,
+                             max_extinction=0.3, n_background=1.2)
+    ref = om.add_absorbing_bc_tensor(T, 0.25, (0, 2), 0.75, max_extinction=0.3, n_background=1.2)
+    assert s.shape == ref.shape[2:] and s.tensor
+    np.testing.assert_allclose(s.permittivity.numpy(), ref, atol=1e-6)
