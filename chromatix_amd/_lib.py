@@ -316,6 +316,8 @@ def _declare(lib):
         "cxm_mbs_iterate": (i32, [vp, vp, vp, vp, ctypes.c_int32, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_output": (i32, [vp, vp, vp, vp, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_bicgstab_state_size": (sz, [ctypes.POINTER(MbsDesc)]),
+        "cxm_enclosing_circle_workspace_size": (sz, [ctypes.c_int64]),
+        "cxm_enclosing_circle_c64": (i32, [vp, vp, ctypes.c_int64, dp, vp, sz]),
         "cxm_mbs_bicgstab_init": (i32, [vp, vp, vp, vp, vp, sz, ctypes.c_double, ctypes.c_double, vp,
                                         vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_bicgstab_iterate": (i32, [vp, vp, vp, vp, sz, ctypes.c_int32, vp, vp, sz,

@@ -17,8 +17,10 @@
 // partial sums of |r|^2 for the residual norm.
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "cxm_pipe_host.h"
 
@@ -852,5 +854,252 @@ extern "C" int cxm_mbs_bicgstab_iterate(void* stream, const void* eps, void* x, 
     hipLaunchKernelGGL(bi_end_kernel, dim3(1), rt, 0, s, (const double*)b.partial, b.st, status);
     CXM_LAUNCH_CHECK();
   }
+  return CXM_OK;
+}
+
+// ---- smallest enclosing circle of the permittivity values --------------------
+// get_shift_and_scale (solver.py:10-39) minimises max |eps - shift| over the
+// complex shift: the centre of the smallest circle enclosing the values.  The
+// device finds the extreme values in 8 directions (their octagon lies inside
+// the convex hull), keeps only the values outside that octagon as candidates,
+// and the host runs Welzl's algorithm on the candidates and the 8 extremes;
+// a last device pass takes the radius as max |eps - centre| over every value.
+namespace cxm {
+namespace {
+constexpr int kEcBlocks = 1024;
+constexpr int kEcDirs = 8;
+
+CXM_DEV double ec_proj(double x, double y, int d) {
+  switch (d) {
+    case 0: return x;
+    case 1: return x + y;
+    case 2: return y;
+    case 3: return y - x;
+    case 4: return -x;
+    case 5: return -x - y;
+    case 6: return -y;
+    default: return x - y;
+  }
+}
+
+// partial[(block * 8 + d) * 2] = best projection, [.. + 1] = its index (as double)
+__global__ void __launch_bounds__(256)
+ec_extreme_kernel(const float2* __restrict__ v, long long n, double* __restrict__ partial) {
+  double best[kEcDirs];
+  long long arg[kEcDirs];
+#pragma unroll
+  for (int d = 0; d < kEcDirs; ++d) {
+    best[d] = -INFINITY;
+    arg[d] = -1;
+  }
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double x = v[i].x, y = v[i].y;
+#pragma unroll
+    for (int d = 0; d < kEcDirs; ++d) {
+      const double p = ec_proj(x, y, d);
+      if (p > best[d]) {
+        best[d] = p;
+        arg[d] = i;
+      }
+    }
+  }
+  __shared__ double sb[256];
+  __shared__ long long sa[256];
+  for (int d = 0; d < kEcDirs; ++d) {
+    sb[threadIdx.x] = best[d];
+    sa[threadIdx.x] = arg[d];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w && sb[threadIdx.x + w] > sb[threadIdx.x]) {
+        sb[threadIdx.x] = sb[threadIdx.x + w];
+        sa[threadIdx.x] = sa[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      partial[(blockIdx.x * kEcDirs + d) * 2] = sb[0];
+      partial[(blockIdx.x * kEcDirs + d) * 2 + 1] = (double)sa[0];
+    }
+    __syncthreads();
+  }
+}
+
+// ext[2 d], ext[2 d + 1]: the extreme value in direction d; count = 0
+__global__ void ec_extreme_final_kernel(const float2* __restrict__ v, const double* __restrict__ partial,
+                                        double* __restrict__ ext, int* __restrict__ count) {
+  const int d = threadIdx.x;
+  if (d >= kEcDirs) return;
+  double best = -INFINITY;
+  long long arg = 0;
+  for (int b = 0; b < kEcBlocks; ++b) {
+    const double p = partial[(b * kEcDirs + d) * 2];
+    if (p > best) {
+      best = p;
+      arg = (long long)partial[(b * kEcDirs + d) * 2 + 1];
+    }
+  }
+  ext[2 * d] = v[arg].x;
+  ext[2 * d + 1] = v[arg].y;
+  if (d == 0) *count = 0;
+}
+
+// values strictly outside the octagon of the extremes (counter-clockwise
+// in direction order) are appended to cand
+__global__ void __launch_bounds__(256)
+ec_filter_kernel(const float2* __restrict__ v, long long n, const double* __restrict__ ext,
+                 float2* __restrict__ cand, int* __restrict__ count) {
+  double px[kEcDirs], py[kEcDirs];
+#pragma unroll
+  for (int d = 0; d < kEcDirs; ++d) {
+    px[d] = ext[2 * d];
+    py[d] = ext[2 * d + 1];
+  }
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double x = v[i].x, y = v[i].y;
+    bool outside = false;
+#pragma unroll
+    for (int d = 0; d < kEcDirs; ++d) {
+      const int e = (d + 1) % kEcDirs;
+      const double ex = px[e] - px[d], ey = py[e] - py[d];
+      if (ex == 0.0 && ey == 0.0) continue;
+      if (ex * (y - py[d]) - ey * (x - px[d]) < 0.0) outside = true;
+    }
+    if (outside) {
+      const int k = atomicAdd(count, 1);
+      cand[k] = v[i];
+    }
+  }
+}
+
+// partial[block] = max |v - c|^2
+__global__ void __launch_bounds__(256)
+ec_radius_kernel(const float2* __restrict__ v, long long n, double cx, double cy,
+                 double* __restrict__ partial) {
+  double m = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double dx = (double)v[i].x - cx, dy = (double)v[i].y - cy;
+    m = fmax(m, dx * dx + dy * dy);
+  }
+  __shared__ double sh[256];
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+struct Pt { double x, y; };
+struct Circ { Pt c; double r2; };
+
+bool in_circ(const Circ& c, const Pt& p) {
+  const double dx = p.x - c.c.x, dy = p.y - c.c.y;
+  return dx * dx + dy * dy <= c.r2 * (1.0 + 1e-12) + 1e-300;
+}
+Circ circ2(const Pt& a, const Pt& b) {
+  const Pt c{(a.x + b.x) / 2, (a.y + b.y) / 2};
+  const double dx = a.x - c.x, dy = a.y - c.y;
+  return Circ{c, dx * dx + dy * dy};
+}
+Circ circ3(const Pt& a, const Pt& b, const Pt& c) {
+  const double d = 2 * (a.x * (b.y - c.y) + b.x * (c.y - a.y) + c.x * (a.y - b.y));
+  if (std::fabs(d) < 1e-300) {  // collinear: the widest pair
+    Circ best = circ2(a, b), t = circ2(a, c);
+    if (t.r2 > best.r2) best = t;
+    t = circ2(b, c);
+    if (t.r2 > best.r2) best = t;
+    return best;
+  }
+  const double a2 = a.x * a.x + a.y * a.y, b2 = b.x * b.x + b.y * b.y, c2_ = c.x * c.x + c.y * c.y;
+  const Pt o{(a2 * (b.y - c.y) + b2 * (c.y - a.y) + c2_ * (a.y - b.y)) / d,
+             (a2 * (c.x - b.x) + b2 * (a.x - c.x) + c2_ * (b.x - a.x)) / d};
+  const double dx = a.x - o.x, dy = a.y - o.y;
+  return Circ{o, dx * dx + dy * dy};
+}
+
+// Welzl's algorithm, iterative form, on points in a fixed shuffled order
+Circ welzl(std::vector<Pt>& p) {
+  uint64_t s = 0x9e3779b97f4a7c15ull;
+  for (size_t i = p.size(); i > 1; --i) {  // deterministic shuffle
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    std::swap(p[i - 1], p[s % i]);
+  }
+  Circ c{p[0], 0.0};
+  for (size_t i = 1; i < p.size(); ++i) {
+    if (in_circ(c, p[i])) continue;
+    c = Circ{p[i], 0.0};
+    for (size_t j = 0; j < i; ++j) {
+      if (in_circ(c, p[j])) continue;
+      c = circ2(p[i], p[j]);
+      for (size_t k = 0; k < j; ++k)
+        if (!in_circ(c, p[k])) c = circ3(p[i], p[j], p[k]);
+    }
+  }
+  return c;
+}
+}  // namespace
+}  // namespace cxm
+
+extern "C" size_t cxm_enclosing_circle_workspace_size(int64_t n) {
+  if (n <= 0) return 0;
+  return (size_t)n * sizeof(float2) + (size_t)kEcBlocks * kEcDirs * 2 * sizeof(double) +
+         (size_t)kEcDirs * 2 * sizeof(double) + 256;
+}
+
+extern "C" int cxm_enclosing_circle_c64(void* stream, const void* values, int64_t n, double* out,
+                                        void* workspace, size_t workspace_bytes) {
+  if (n <= 0) return fail(CXM_ERR_INVALID, "no values");
+  if (n > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "more than 2^31 - 1 values");
+  if (!values || !out) return fail(CXM_ERR_INVALID, "null buffer");
+  if (!workspace || workspace_bytes < cxm_enclosing_circle_workspace_size(n))
+    return fail(CXM_ERR_WORKSPACE, "workspace too small");
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const float2* v = (const float2*)values;
+  float2* cand = (float2*)workspace;
+  double* partial = (double*)(cand + n);
+  double* ext = partial + (size_t)kEcBlocks * kEcDirs * 2;
+  int* count = (int*)(ext + kEcDirs * 2);
+  hipLaunchKernelGGL(ec_extreme_kernel, dim3(kEcBlocks), dim3(256), 0, s, v, (long long)n, partial);
+  CXM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(ec_extreme_final_kernel, dim3(1), dim3(64), 0, s, v, (const double*)partial, ext,
+                     count);
+  CXM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(ec_filter_kernel, dim3(kEcBlocks), dim3(256), 0, s, v, (long long)n,
+                     (const double*)ext, cand, count);
+  CXM_LAUNCH_CHECK();
+  int nc = 0;
+  double hext[kEcDirs * 2];
+  CXM_HIP_CHECK(hipMemcpyAsync(&nc, count, sizeof(int), hipMemcpyDeviceToHost, s));
+  CXM_HIP_CHECK(hipMemcpyAsync(hext, ext, sizeof(hext), hipMemcpyDeviceToHost, s));
+  CXM_HIP_CHECK(hipStreamSynchronize(s));
+  std::vector<float2> hc((size_t)nc);
+  if (nc > 0) {
+    CXM_HIP_CHECK(hipMemcpyAsync(hc.data(), cand, (size_t)nc * sizeof(float2), hipMemcpyDeviceToHost, s));
+    CXM_HIP_CHECK(hipStreamSynchronize(s));
+  }
+  std::vector<Pt> pts;
+  pts.reserve((size_t)nc + kEcDirs);
+  for (int d = 0; d < kEcDirs; ++d) pts.push_back(Pt{hext[2 * d], hext[2 * d + 1]});
+  for (const float2& q : hc) pts.push_back(Pt{(double)q.x, (double)q.y});
+  const Circ c = welzl(pts);
+  hipLaunchKernelGGL(ec_radius_kernel, dim3(kEcBlocks), dim3(256), 0, s, v, (long long)n, c.c.x, c.c.y,
+                     partial);
+  CXM_LAUNCH_CHECK();
+  std::vector<double> hp(kEcBlocks);
+  CXM_HIP_CHECK(hipMemcpyAsync(hp.data(), partial, kEcBlocks * sizeof(double), hipMemcpyDeviceToHost, s));
+  CXM_HIP_CHECK(hipStreamSynchronize(s));
+  double r2 = 0.0;
+  for (double x : hp) r2 = std::max(r2, x);
+  out[0] = c.c.x;
+  out[1] = c.c.y;
+  out[2] = std::sqrt(std::max(r2, c.r2));
   return CXM_OK;
 }

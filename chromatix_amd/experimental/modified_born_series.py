@@ -133,9 +133,19 @@ def _circle3(a, b, c):
     return o, abs(a - o)
 
 
+def _distinct(values: np.ndarray) -> np.ndarray:
+    """Distinct values of a piecewise-constant volume: runs of equal
+    neighbours collapse first (cheap), then the sort of np.unique runs on
+    what is left."""
+    v = np.ascontiguousarray(values).ravel()
+    if v.size > 1:
+        v = v[np.concatenate(([True], v[1:] != v[:-1]))]
+    return np.unique(v)
+
+
 def enclosing_circle(points) -> tuple[complex, float]:
     """Smallest circle enclosing complex points (Welzl, iterative form)."""
-    pts = [complex(p) for p in np.unique(np.asarray(points, np.complex128).ravel())]
+    pts = [complex(p) for p in _distinct(np.asarray(points)).astype(np.complex128)]
     random.Random(0).shuffle(pts)
     eps = 1e-12
     c, r = pts[0], 0.0
@@ -164,7 +174,10 @@ def _tensor_shift(eps: np.ndarray) -> tuple[complex, float]:
     Nelder-Mead minimisation of the convex maximum otherwise."""
     from scipy.optimize import minimize
 
-    E = np.unique(eps.reshape(9, -1).T, axis=0).reshape(-1, 3, 3)
+    rows = eps.reshape(9, -1).T
+    if rows.shape[0] > 1:
+        rows = rows[np.concatenate(([True], np.any(rows[1:] != rows[:-1], axis=1)))]
+    E = np.unique(rows, axis=0).reshape(-1, 3, 3)
     c0, _ = enclosing_circle(np.linalg.eigvals(E))
     I3 = np.eye(3)
 
@@ -186,11 +199,23 @@ def get_shift_and_scale(permittivity) -> tuple[complex, complex]:
     scale = 1 / (1.1 i sqrt(f)), f = max |eps - shift|^2 (solver.py:10-39);
     for a (3, 3, ...) permittivity tensor |.| is the spectral norm of
     eps - shift I (the norm the reference's note asks for)."""
-    eps = torch.as_tensor(permittivity).detach().cpu().numpy()
-    if eps.ndim == 5:
-        c, r = _tensor_shift(eps.astype(np.complex128))
+    t = torch.as_tensor(permittivity).detach()
+    if t.ndim != 5 and t.is_cuda:
+        # on the device: cxm_enclosing_circle_c64 (candidates pruned on the GPU)
+        L.ensure_device(t)
+        v = t.to(torch.complex64).contiguous()
+        ws = torch.empty(L.lib().cxm_enclosing_circle_workspace_size(v.numel()), dtype=torch.uint8,
+                         device=v.device)
+        out = (ctypes.c_double * 3)()
+        L.check(L.lib().cxm_enclosing_circle_c64(L.stream_ptr(v.device), v.data_ptr(), v.numel(), out,
+                                                 ws.data_ptr(), ws.numel()))
+        c, r = complex(out[0], out[1]), float(out[2])
     else:
-        c, r = enclosing_circle(eps)
+        eps = t.cpu().numpy()
+        if eps.ndim == 5:
+            c, r = _tensor_shift(eps.astype(np.complex128))
+        else:
+            c, r = enclosing_circle(eps)
     if r == 0:
         r = 1e-6  # homogeneous medium: any positive radius keeps the iteration accretive
     return c, 1.0 / (1.1j * r)

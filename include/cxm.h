@@ -564,6 +564,17 @@ int cxm_mbs_output(void* stream, const void* x, const double* num_scale, void* e
  * owned by the caller between calls.  status: 4 device doubles written by every
  * call: |r|^2, k, stopped (0/1), |prec_y|^2.  x: 3 x nx x ny x nz planar.
  */
+/*
+ * Smallest circle enclosing n complex64 values (the shift of
+ * get_shift_and_scale, solver.py:10-39, which minimises max |eps - shift|):
+ * out[0], out[1] = centre (re, im), out[2] = max |value - centre|.  The device
+ * prunes the values inside the octagon of the 8 directional extremes, the
+ * host runs Welzl's algorithm on the rest.  Synchronises `stream`.
+ */
+size_t cxm_enclosing_circle_workspace_size(int64_t n);
+int cxm_enclosing_circle_c64(void* stream, const void* values, int64_t n, double* out,
+                             void* workspace, size_t workspace_bytes);
+
 size_t cxm_mbs_bicgstab_state_size(const cxm_mbs_desc* d);
 int cxm_mbs_bicgstab_init(void* stream, const void* eps, const void* prec_y, const void* x,
                           void* state, size_t state_bytes, double tol, double atol, double* status,

@@ -205,3 +205,28 @@ def test_tensor_converged_solution_solves_helmholtz():
     pde = m.pde_residual(res.E.cpu().numpy().astype(np.complex128), S)
     print(f"tensor eps 40^3 bicgstab: {res.iterations} iterations, PDE residual {pde:.2e}")
     assert pde < 1e-4
+
+
+@pytest.mark.parametrize("case", ["spheres", "random", "single", "collinear", "ring"])
+def test_device_enclosing_circle_matches_host(case):
+    rng = np.random.default_rng(11)
+    if case == "spheres":
+        eps, _, _ = _setup(40, 6, 5, 6)
+    elif case == "random":
+        eps = (rng.standard_normal((30, 40, 50)) + 1j * rng.standard_normal((30, 40, 50))) * 0.1 + 1.8
+    elif case == "single":
+        eps = np.full((8, 8, 8), 1.33**2 + 0.01j)
+    elif case == "collinear":
+        eps = (1.7 + 0.1j) + (0.3 - 0.2j) * rng.random((16, 16, 16))
+    else:  # values on a circle: every value is a hull vertex
+        eps = 1.8 + 0.2j + 0.05 * np.exp(2j * np.pi * rng.random((20, 20, 20)))
+    e32 = eps.astype(np.complex64)
+    c_h, r_h = mbs.enclosing_circle(e32)
+    c_d, s_d = mbs.get_shift_and_scale(torch.as_tensor(e32).cuda())
+    r_d = abs(1 / s_d) / 1.1
+    if r_h == 0:
+        r_h = 1e-6
+    assert abs(c_d - c_h) <= 1e-9 * max(1.0, abs(c_h))
+    np.testing.assert_allclose(r_d, r_h, rtol=1e-9)
+    # every value lies in the circle
+    assert np.max(np.abs(e32.astype(np.complex128) - c_d)) <= r_d * (1 + 1e-12) + 1e-15
