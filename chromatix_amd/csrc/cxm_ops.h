@@ -260,9 +260,20 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
   }
 }
 
+// Operators whose factor is always finite map 0 to 0: once a value is 0
+// (outside a pupil, in the zero padding) they are skipped, which saves
+// their sine/cosine.
+CXM_DEV bool maps_zero_to_zero(int kind) {
+  return kind == OP_SCALE || kind == OP_CONJ || kind == OP_CHIRP || kind == OP_PHASE ||
+         kind == OP_CIRC || kind == OP_SASDH || kind == OP_NAMASK;
+}
+
 CXM_DEV float2 apply_ops(const OpList& L, const OpTable& T, float2 v, int plane, int iy, int ix,
                          int ny, int nx) {
-  for (int i = 0; i < L.n; ++i) v = apply_op(L.op[i], T, v, plane, iy, ix, ny, nx);
+  for (int i = 0; i < L.n; ++i) {
+    if (v.x == 0.0f && v.y == 0.0f && maps_zero_to_zero(L.op[i].kind)) continue;
+    v = apply_op(L.op[i], T, v, plane, iy, ix, ny, nx);
+  }
   return v;
 }
 

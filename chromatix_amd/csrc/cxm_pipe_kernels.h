@@ -54,6 +54,15 @@ struct PipeArgs {
 // kernels take PipeArgs by value: stay within the 4 KiB kernel-argument limit
 static_assert(sizeof(PipeArgs) <= 4096, "PipeArgs exceeds the kernel-argument limit");
 
+// A value the caller knows to be the same in every lane of the wavefront
+// (UNIFORM), made visibly so to the compiler: scalar registers, scalar loads
+// and branches for what depends on it.
+template <bool UNIFORM>
+CXM_DEV int wave_uniform(int x) {
+  if constexpr (UNIFORM) return __builtin_amdgcn_readfirstlane(x);
+  return x;
+}
+
 // Store of one output value: complex, |x|^2, or |x|^2 accumulated.
 CXM_DEV void store_value(const PipeArgs& a, long long idx, float2 x) {
   if (a.intensity == 2) {
@@ -159,8 +168,10 @@ pipe_row_kernel(PipeArgs a) {
   const bool ok = line < (long long)a.n_planes * a.n_lines;
   const int p = ok ? (int)(line / a.n_lines) : 0;
   const int r = ok ? (int)(line - (long long)p * a.n_lines) : 0;
-  const int plane = a.plane0 + p;
-  const int yg = a.line_off + r;
+  // whole wavefronts per row (T a multiple of 64): plane and row are uniform
+  // per wave, so the operators' parameter lookups stay scalar
+  const int plane = wave_uniform<T % 64 == 0>(a.plane0 + p);
+  const int yg = wave_uniform<T % 64 == 0>(a.line_off + r);
   const float2* src = a.in ? a.in + p * a.in_plane_stride + (long long)r * a.in_line_stride : nullptr;
   float2* st = lds + g * SPAN;
   float2 v[E];
@@ -224,9 +235,13 @@ pipe_row_kernel(PipeArgs a) {
 //   g = IFFT_row(in[p, row, :]) * scale          (unnormalised inverse, the
 //                                                  adjoint of the forward FFT)
 //   out[row, x] += sum_p Im(g[j] * conj(src_p(row, x))),  x = ishift(j) - in_off
-// where src_p = the `pre` operators applied to 1, i.e. the regenerated input
-// of the forward pass, so Im(g conj(src)) = dL/dphi (d src / d phi = i src).
-// The plane sum stays in registers; `accum` adds to an existing out.
+// where src_p = the `pre` operators applied to 1, then the `mid` operators,
+// i.e. the regenerated input of the forward pass, so Im(g conj(src)) =
+// dL/dphi (d src / d phi = i src).  `mid` holds the operators that are the
+// same for every plane (all operators multiply): with w = mid(1),
+//   sum_p Im(g_p conj(w pre_p(1))) = Im(conj(w) sum_p g_p conj(pre_p(1))),
+// so they are evaluated once per element, after the plane sum.  The sum is
+// kept in LDS; `accum` adds to an existing out.
 template <int N, int E>
 __global__ void __launch_bounds__((PipeRowCfg<N, E>::THREADS))
 pipe_row_phasegrad_kernel(PipeArgs a) {
@@ -234,16 +249,16 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
   constexpr int T = Cfg::T;
   constexpr int SPAN = lds_span<N, 1>();
   __shared__ float2 lds[Cfg::G * SPAN];
-  __shared__ float acc_lds[Cfg::G * N];  // the plane sum, one float per element
+  __shared__ float2 acc_lds[Cfg::G * N];  // the plane sum of g conj(pre(1)), per element
   const int g = threadIdx.x / T;
   const int t = threadIdx.x - g * T;
   const int r = blockIdx.x * Cfg::G + g;
   const bool ok = r < a.n_lines;
-  const int yg = a.line_off + (ok ? r : 0);
+  const int yg = wave_uniform<T % 64 == 0>(a.line_off + (ok ? r : 0));
   float2* st = lds + g * SPAN;
-  float* acc = acc_lds + g * N;
+  float2* acc = acc_lds + g * N;
 #pragma unroll
-  for (int k = 0; k < E; ++k) acc[t + k * T] = 0.0f;
+  for (int k = 0; k < E; ++k) acc[t + k * T] = make_float2(0.0f, 0.0f);
   const int per = (a.n_planes + (int)gridDim.y - 1) / (int)gridDim.y;
   const int p_lo = (int)blockIdx.y * per;
   const int p_hi = min(a.n_planes, p_lo + per);
@@ -264,7 +279,8 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
         const float2 gv = cscale(st[lds_at<N, 1>(j)], a.scale);
         const float2 sv = apply_ops(a.pre, a.tab, make_float2(1.f, 0.f), a.plane0 + p, yg, xf,
                                     a.grid_ny, a.grid_nx);
-        acc[j] += gv.y * sv.x - gv.x * sv.y;
+        const float2 c = acc[j];
+        acc[j] = make_float2(c.x + (gv.x * sv.x + gv.y * sv.y), c.y + (gv.y * sv.x - gv.x * sv.y));
       }
     }
     __syncthreads();  // before the next plane's FFT reuses the slots
@@ -273,12 +289,19 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
   float* dst = (float*)a.out + (long long)blockIdx.y * a.out_plane_stride +
                (long long)r * a.out_line_stride;
   const bool add = a.accum && gridDim.y == 1;
-#pragma unroll
+#pragma unroll 1
   for (int k = 0; k < E; ++k) {
     const int j = t + k * T;
     const int xf = a.in_shift ? ishift_src(j, N) : j;
     const int xs = xf - a.in_off;
-    if (xs >= 0 && xs < a.in_len) dst[xs] = add ? dst[xs] + acc[j] : acc[j];
+    if (xs >= 0 && xs < a.in_len) {
+      const float2 c = acc[j];
+      const float2 w = a.mid.n ? apply_ops(a.mid, a.tab, make_float2(1.f, 0.f), a.plane0, yg, xf,
+                                           a.grid_ny, a.grid_nx)
+                               : make_float2(1.f, 0.f);
+      const float v = w.x * c.y - w.y * c.x;  // Im(conj(w) c)
+      dst[xs] = add ? dst[xs] + v : v;
+    }
   }
 }
 

@@ -80,8 +80,10 @@ long long pupil_count(int ny, int nx, float dy, float dx, float r2) {
 int source_ops(Ops& B, OpList& L, const cxm_psf4f_desc* d, float2 extra) {
   const int nz = d->num_z;
   const int mode = d->phase_mode;
-  OpDesc* ch = B.add(L, OP_CHIRP, nz, 3, mode);
+  // the pupil first: outside it the value is 0 and the chirp, scale and
+  // phase operators are skipped (the product is the same, the mask is exact)
   OpDesc* ci = B.add(L, OP_CIRC, 1, 3);
+  OpDesc* ch = B.add(L, OP_CHIRP, nz, 3, mode);
   OpDesc* sc = B.add(L, OP_SCALE, 1, 1);
   if (!ch || !ci || !sc) return fail(CXM_ERR_INVALID, "operator table overflow");
   const float dyf = (float)d->dy, dxf = (float)d->dx;
@@ -144,10 +146,15 @@ int psf_ops(Ops& B, PipeArgs& ra, const cxm_psf4f_desc* d, const float* phase) {
   return CXM_OK;
 }
 
+// Planes per chunk: at most 160 MiB of complex64 intermediates (they stay in
+// the 256 MiB Infinity Cache), split evenly so that no chunk is a lone
+// latency-bound plane (11 planes of 1280^2: one chunk, not 10 + 1).
 long long psf_chunk(const cxm_psf4f_desc* d) {
   const long long per = (long long)d->height * d->width;
-  const long long budget = (128ll << 20) / 8;
-  return std::max(1ll, std::min<long long>(d->num_z, budget / std::max(1ll, per)));
+  const long long budget = (160ll << 20) / 8;
+  const long long cap = std::max(1ll, std::min<long long>(d->num_z, budget / std::max(1ll, per)));
+  const long long n = (d->num_z + cap - 1) / cap;
+  return (d->num_z + n - 1) / n;
 }
 
 void row_args(PipeArgs& ra, const cxm_psf4f_desc* d) {
@@ -300,8 +307,17 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
   ig->shift = 3;
   const double count = (double)d->num_z * d->height * d->width;
   cb.p(ig, 0, 0) = make_float4((float)(2.0 / count), 0.f, 0.f, 0.f);
-  // phase-gradient pass: same generated source as the row pass
+  // phase-gradient pass: same generated source as the row pass, split into
+  // the operators that change with the plane (pupil, defocus chirp; `pre`)
+  // and those that do not (scale, phase mask; `mid`, once per element)
   ga = ra;
+  ga.pre.n = 0;
+  ga.mid.n = 0;
+  for (int i = 0; i < ra.pre.n; ++i) {
+    const OpDesc& o = ra.pre.op[i];
+    OpList& dst = (o.kind == OP_CIRC || o.kind == OP_CHIRP) ? ga.pre : ga.mid;
+    dst.op[dst.n++] = o;
+  }
   ga.in_plane_stride = (long long)d->height * d->width;
   ga.in_line_stride = d->width;
   ga.out = partial;
