@@ -15,7 +15,7 @@
 #define CXM_SIZES_G1(X) X(512, 16, 16, 16) X(1024, 16, 16, 4)
 #define CXM_SIZES_G2(X) X(2048, 16, 16, 4) X(4096, 16, 32, 4) X(8192, 32, 32, 2)
 #define CXM_SIZES_G3(X)                                                       \
-  X(320, 20, 20, 16) X(640, 20, 20, 16) X(1280, 20, 20, 4) X(2560, 20, 20, 4) \
+  X(320, 20, 20, 16) X(640, 20, 20, 8) X(1280, 20, 20, 4) X(2560, 20, 20, 4) \
   X(5120, 20, 20, 2)
 
 #define CXM_SIZES_G4(X)                                                       \
