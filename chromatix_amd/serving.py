@@ -75,13 +75,16 @@ class SasCztRequest:
 
 @dataclass
 class BornRequest:
-    """Modified Born series solve of a permittivity volume with a source."""
+    """Modified Born series solve of a permittivity volume with a source, to
+    |prec_y - prec_forward(x)| <= rtol |prec_y|, by the fixed-point iteration
+    or (use_bicgstab) BiCGStab."""
     permittivity: torch.Tensor
     current: torch.Tensor
     spacing: float
     k0: float
     rtol: float = 1e-4
     arrival: float = 0.0
+    use_bicgstab: bool = False
     result: torch.Tensor | None = None
     done: float | None = None
     iterations: int = 0
@@ -213,7 +216,10 @@ def _born(r: BornRequest):
 
     sample = mbs.Sample(r.permittivity, r.spacing)
     src = mbs.Source(r.current, r.k0)
-    res = mbs.solve(sample, src, maxiter=3000, rtol=r.rtol, return_info=True)
+    if r.use_bicgstab:
+        res = mbs.solve(sample, src, maxiter=3000, tol=r.rtol, use_bicgstab=True, return_info=True)
+    else:
+        res = mbs.solve(sample, src, maxiter=3000, rtol=r.rtol, return_info=True)
     return res.E, res.iterations
 
 
@@ -231,8 +237,11 @@ class _BornJob:
         self.stream = torch.cuda.Stream(device=device)
         self.stream.wait_stream(torch.cuda.current_stream(device))
         with torch.cuda.stream(self.stream):
-            self.sv = mbs.Solver(mbs.Sample(r.permittivity, r.spacing), mbs.Source(r.current, r.k0),
-                                 rtol=r.rtol, maxiter=3000)
+            sample, src = mbs.Sample(r.permittivity, r.spacing), mbs.Source(r.current, r.k0)
+            if r.use_bicgstab:
+                self.sv = mbs.BiCGStabSolver(sample, src, tol=r.rtol, maxiter=3000)
+            else:
+                self.sv = mbs.Solver(sample, src, rtol=r.rtol, maxiter=3000)
             self.sv.step(_BORN_CHUNK)
         self.out_event = None
 
@@ -245,7 +254,7 @@ class _BornJob:
         with torch.cuda.stream(self.stream):
             if self.sv.converged:
                 self.r.result = self.sv.result()
-                self.r.iterations = self.sv.iterations
+                self.r.iterations = getattr(self.sv, "k", self.sv.iterations)
                 self.out_event = torch.cuda.Event()
                 self.out_event.record(self.stream)
             else:

@@ -33,6 +33,8 @@ def _mix(seed=0):
     J[:, :, 3, 0] = 1
     reqs.append(sv.BornRequest(torch.as_tensor(eps).cuda(), torch.as_tensor(J).cuda(), 0.133,
                                2 * np.pi / 0.532, rtol=1e-4, arrival=0.004))
+    reqs.append(sv.BornRequest(torch.as_tensor(eps).cuda(), torch.as_tensor(J).cuda(), 0.133,
+                               2 * np.pi / 0.532, rtol=1e-4, arrival=0.005, use_bicgstab=True))
     return reqs
 
 
