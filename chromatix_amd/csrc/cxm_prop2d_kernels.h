@@ -80,8 +80,7 @@ CXM_DEV const float2* row_ptr(const RowArgs& a, int p, int r) {
 
 // Store of one output value: complex, |u|^2 or the real part.
 CXM_DEV void row_store(const RowArgs& a, long long idx, float2 w) {
-  if (a.intensity == 1) a.out_i[idx] = cabs2(w);
-  else if (a.intensity == 2) a.out_i[idx] = w.x;
+  if (a.intensity) a.out_i[idx] = (a.intensity == 2) ? w.x : cabs2(w);
   else a.out_c[idx] = w;
 }
 
@@ -195,14 +194,12 @@ row_fft_loop_kernel(RowArgs a) {
       const bool ok = row < a.n_rows;
       const int p = ok ? row / a.rows_per_plane : 0;
       const int r = ok ? row - p * a.rows_per_plane : 0;
-      const float2* src = row_ptr(a, p, r);
+      const float2* src = a.in + p * a.in_plane_stride + (long long)r * a.in_row_stride;
 #pragma unroll
       for (int k = 0; k < E; k += 2) {
         const int x0 = t + k * T - a.in_off, x1 = x0 + T;
-        const float2 u0 =
-            (ok && x0 >= 0 && x0 < a.in_len) ? row_load(a, src, x0) : make_float2(0.f, 0.f);
-        const float2 u1 =
-            (ok && x1 >= 0 && x1 < a.in_len) ? row_load(a, src, x1) : make_float2(0.f, 0.f);
+        const float2 u0 = (ok && x0 >= 0 && x0 < a.in_len) ? src[x0] : make_float2(0.f, 0.f);
+        const float2 u1 = (ok && x1 >= 0 && x1 < a.in_len) ? src[x1] : make_float2(0.f, 0.f);
         pf[k / 2] = make_float4(u0.x, u0.y, u1.x, u1.y);
       }
     }
@@ -265,7 +262,9 @@ row_fft_loop_kernel(RowArgs a) {
         for (int k = 0; k < E; ++k) {
           const int xo = t + k * T - a.out_off;
           if (xo >= 0 && xo < a.out_len) {
-            row_store(a, obase + xo, cscale(v[k], a.scale));
+            const float2 w = cscale(v[k], a.scale);
+            if (a.intensity) a.out_i[obase + xo] = cabs2(w);
+            else a.out_c[obase + xo] = w;
           }
         }
       }
@@ -381,8 +380,10 @@ int launch_row_t(int dir, const RowArgs& a, hipStream_t s) {
   if constexpr (Cfg::G == 4 && Cfg::T == 64) {
     // Looping pass for rows read from caller memory (HBM latency to hide);
     // tiled workspace input comes back faster than the loop gains.
+    // (complex64 input and complex or |u|^2 output only: real input rows and
+    // real-part stores take the plain pass)
     const int lo = option(CXM_OPT_ROW_LOOP);
-    if (lo > 1 || (lo == 0 && !a.in_tiled)) {
+    if (!a.in_real && a.intensity != 2 && (lo > 1 || (lo == 0 && !a.in_tiled))) {
       static int resident[2] = {0, 0};  // workgroups per CU, per direction
       int& rb = resident[dir < 0 ? 0 : 1];
       if (rb == 0) {
