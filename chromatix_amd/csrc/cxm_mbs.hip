@@ -130,6 +130,114 @@ __global__ void mbs_green_kernel(float2* __restrict__ y, long long vox, int nx, 
 
 constexpr int kRedThreads = 256;
 
+// x pass of the Green's operator, fused: for a strip of C columns of the
+// (nx x ny nz) matrix, the three components of each column are transformed
+// along x by three groups of threads (3 C column transforms interleaved in
+// LDS), exchanged through LDS, multiplied by the Green's projection at
+// every (kx, ky, kz) (the arithmetic of mbs_green_kernel, each thread
+// writing its own component), transformed back along x and stored with
+// `scale`; the y and z passes stay pipeline passes.  One HBM round trip
+// instead of three (forward x pass, Green multiply, inverse x pass).
+template <int N, int E, int C>
+__global__ void __launch_bounds__((N / E) * 3 * C)
+mbs_xgreen_kernel(float2* __restrict__ v, const float2* __restrict__ tw, long long vox, int ncols,
+                  int ny, int nz, float kscale, C2 s, C2 kb, float scale) {
+  constexpr int T = N / E;
+  constexpr int CC = 3 * C;
+  __shared__ float2 lds[lds_span<N, CC>()];
+  const int cc = threadIdx.x % CC;
+  const int ty = threadIdx.x / CC;
+  const int q = cc / C;  // component of this thread's column
+  const int c = cc - q * C;
+  const int nb = gridDim.x;
+  const int L = ((nb & 7) == 0) ? (int)((blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3)) : (int)blockIdx.x;
+  const int col = L * C + c;
+  const bool colok = col < ncols;
+  float2* vq = v + q * vox + col;
+  float2 u[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k)
+    u[k] = colok ? vq[(long long)(ty + k * T) * ncols] : make_float2(0.f, 0.f);
+  fft<N, E, -1, CC, true>(u, lds + cc, ty, tw);
+  __syncthreads();  // the transform's last exchange has been read by every thread
+  stage_out<N, E, CC>(u, lds + cc, ty);
+  __syncthreads();
+  const int iy = colok ? col / nz : 0;
+  const int iz = colok ? col - iy * nz : 0;
+  const float ky = kscale * (float)signed_bin(iy, ny) / (float)ny;
+  const float kz = kscale * (float)signed_bin(iz, nz) / (float)nz;
+  const float bn = kb.re * kb.re + kb.im * kb.im;
+  const C2 gl{kb.re / bn, -kb.im / bn};
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = ty + k * T;
+    const float kx = kscale * (float)signed_bin(i, N) / (float)N;
+    const float k2 = kx * kx + ky * ky + kz * kz;
+    const float2 y0 = lds[c + lds_at<N, CC>(i)];
+    const float2 y1 = lds[C + c + lds_at<N, CC>(i)];
+    const float2 y2 = lds[2 * C + c + lds_at<N, CC>(i)];
+    float2 p = make_float2(kx * y0.x + ky * y1.x + kz * y2.x, kx * y0.y + ky * y1.y + kz * y2.y);
+    const float inv = k2 != 0.0f ? 1.0f / k2 : 0.0f;
+    p = make_float2(p.x * inv, p.y * inv);
+    const float dre = kb.re - s.re * k2, dim = kb.im - s.im * k2;
+    const float dn = dre * dre + dim * dim;
+    const C2 gt{dre / dn, -dim / dn};
+    const float kq = q == 0 ? kx : (q == 1 ? ky : kz);
+    const float2 yq = q == 0 ? y0 : (q == 1 ? y1 : y2);
+    const float2 yl = make_float2(kq * p.x, kq * p.y);
+    const float2 yt = make_float2(yq.x - yl.x, yq.y - yl.y);
+    const float2 ga = cmulf(yt, gt), gb = cmulf(yl, gl);
+    u[k] = make_float2(ga.x + gb.x, ga.y + gb.y);
+  }
+  // the inverse transform's first stage writes this thread's column before
+  // any barrier: every thread must have read the other components first
+  __syncthreads();
+  fft<N, E, 1, CC, true>(u, lds + cc, ty, tw);
+  if (!colok) return;
+#pragma unroll
+  for (int k = 0; k < E; ++k)
+    vq[(long long)(ty + k * T) * ncols] = make_float2(u[k].x * scale, u[k].y * scale);
+}
+
+// (length, values per thread, columns per workgroup; 3x as many column
+// transforms): at most 32 KiB of LDS per workgroup (at 160: 8 columns
+// measured 0.377 ms per c4 iteration, 16 columns 0.431, the three
+// separate launches 0.423)
+#define CXM_XGREEN_SIZES(X)                                                   \
+  X(8, 8, 32) X(16, 16, 32) X(32, 16, 32) X(64, 16, 16) X(128, 16, 8)          \
+  X(256, 16, 4) X(20, 20, 32) X(40, 20, 32) X(80, 20, 16) X(160, 20, 8)        \
+  X(12, 12, 32) X(24, 12, 32) X(48, 12, 16) X(96, 12, 8) X(192, 12, 4)
+
+bool xgreen_supported(int n) {
+  switch (n) {
+#define CXM_XG_CASE(N, E, C) case N:
+    CXM_XGREEN_SIZES(CXM_XG_CASE)
+#undef CXM_XG_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+int launch_xgreen(int n, float2* v, long long vox, int ncols, int ny, int nz, float kscale, C2 s, C2 kb,
+                  float scale, hipStream_t st) {
+  switch (n) {
+#define CXM_XG_LAUNCH(N, E, C)                                                                        \
+  case N: {                                                                                           \
+    const long long blocks = ((long long)ncols + (C)-1) / (C);                                        \
+    if (blocks > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "fused Green pass grid too large");     \
+    hipLaunchKernelGGL((mbs_xgreen_kernel<N, E, C>), dim3((unsigned)blocks), dim3((N / E) * 3 * (C)), 0, \
+                       st, v, twiddles_for(N), vox, ncols, ny, nz, kscale, s, kb, scale);             \
+    CXM_LAUNCH_CHECK();                                                                               \
+    return CXM_OK;                                                                                    \
+  }
+    CXM_XGREEN_SIZES(CXM_XG_LAUNCH)
+#undef CXM_XG_LAUNCH
+    default:
+      return fail(CXM_ERR_UNSUPPORTED, "no fused Green pass for length " + std::to_string(n));
+  }
+}
+
 // r = prec_y + V(g + x);  x <- x + r;  partial[block] = sum |r|^2 (FP64)
 __global__ void __launch_bounds__(kRedThreads)
 mbs_update_kernel(const float2* __restrict__ g, const float2* __restrict__ eps,
@@ -223,66 +331,88 @@ constexpr int kRedBlocks = 1024;
 
 unsigned blocks_for(long long n) { return (unsigned)std::max(1LL, std::min<long long>((n + 255) / 256, 0x7fffffffLL)); }
 
-// 3D FFT of the 3 planar component volumes in place (dir -1 forward, +1
-// inverse with the 1/(nx ny nz) scale on the last pass).
-int fft3(const MbsGeom& g, float2* v, int dir, hipStream_t s) {
+// One axis of the 3D FFT of the 3 planar component volumes, in place
+// (dir -1 forward, +1 inverse, unnormalised; `scale` on the output).
+int pass_z(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
+  // z: contiguous rows
+  PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.in = v; a.out = v;
+  a.tw = twiddles_for(g.nz);
+  a.in_plane_stride = a.out_plane_stride = g.vox;
+  a.in_line_stride = a.out_line_stride = g.nz;
+  a.n_lines = g.nx * g.ny;
+  a.n_planes = 3;
+  a.grid_ny = g.nx * g.ny;
+  a.grid_nx = g.nz;
+  a.in_len = a.out_len = g.nz;
+  a.seq = dir < 0 ? SEQ_F : SEQ_I;
+  a.scale = scale;
+  a.mid_scale = 1.0f;
+  return launch_pipe_row(g.nz, a, s);
+}
+
+int pass_y(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
+  // y: column strips of each (ny x nz) slab, 3 nx planes
+  PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.in = v; a.out = v;
+  a.tw = twiddles_for(g.ny);
+  a.in_plane_stride = a.out_plane_stride = (long long)g.ny * g.nz;
+  a.in_line_stride = a.out_line_stride = g.nz;
+  a.n_lines = g.nz;
+  a.n_planes = 3 * g.nx;
+  a.grid_ny = g.ny;
+  a.grid_nx = g.nz;
+  a.in_len = a.out_len = g.ny;
+  a.out_line_len = g.nz;
+  a.seq = dir < 0 ? SEQ_F : SEQ_I;
+  a.scale = scale;
+  a.mid_scale = 1.0f;
+  return launch_pipe_col(g.ny, a, s);
+}
+
+int pass_x(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
+  // x: column strips of the (nx x ny nz) matrix, 3 planes
+  PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.in = v; a.out = v;
+  a.tw = twiddles_for(g.nx);
+  a.in_plane_stride = a.out_plane_stride = g.vox;
+  a.in_line_stride = a.out_line_stride = g.ny * g.nz;
+  a.n_lines = g.ny * g.nz;
+  a.n_planes = 3;
+  a.grid_ny = g.nx;
+  a.grid_nx = g.ny * g.nz;
+  a.in_len = a.out_len = g.nx;
+  a.out_line_len = g.ny * g.nz;
+  a.seq = dir < 0 ? SEQ_F : SEQ_I;
+  a.scale = scale;
+  a.mid_scale = 1.0f;
+  return launch_pipe_col(g.nx, a, s);
+}
+
+// g <- G(g) = ifftn(P_T g^ / (kb - s |K|^2) + P_L g^ / kb): z and y passes
+// forward, the fused x pass (x FFT, Green multiply, inverse x FFT, the
+// 1/(nx ny nz) scale) where its length has one, else the x pass, the Green
+// kernel and the inverse x pass; then the y and z passes inverse.
+int green_apply(const MbsGeom& g, const cxm_mbs_desc* d, float2* gb, C2 sc, C2 kb, hipStream_t s) {
   int rc;
   const float inv = (float)(1.0 / ((double)g.nx * g.ny * g.nz));
-  // z: contiguous rows
-  {
-    PipeArgs a;
-    memset(&a, 0, sizeof(a));
-    a.in = v; a.out = v;
-    a.tw = twiddles_for(g.nz);
-    a.in_plane_stride = a.out_plane_stride = g.vox;
-    a.in_line_stride = a.out_line_stride = g.nz;
-    a.n_lines = g.nx * g.ny;
-    a.n_planes = 3;
-    a.grid_ny = g.nx * g.ny;
-    a.grid_nx = g.nz;
-    a.in_len = a.out_len = g.nz;
-    a.seq = dir < 0 ? SEQ_F : SEQ_I;
-    a.scale = a.mid_scale = 1.0f;
-    if ((rc = launch_pipe_row(g.nz, a, s))) return rc;
+  const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
+  if ((rc = pass_z(g, gb, -1, 1.0f, s))) return rc;
+  if ((rc = pass_y(g, gb, -1, 1.0f, s))) return rc;
+  if (xgreen_supported(g.nx) && option(CXM_OPT_MBS_XGREEN) != 2) {
+    if ((rc = launch_xgreen(g.nx, gb, g.vox, g.ny * g.nz, g.ny, g.nz, kscale, sc, kb, inv, s))) return rc;
+  } else {
+    if ((rc = pass_x(g, gb, -1, 1.0f, s))) return rc;
+    hipLaunchKernelGGL(mbs_green_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, gb, g.vox, g.nx, g.ny,
+                       g.nz, kscale, sc, kb);
+    CXM_LAUNCH_CHECK();
+    if ((rc = pass_x(g, gb, 1, inv, s))) return rc;
   }
-  // y: column strips of each (ny x nz) slab, 3 nx planes
-  {
-    PipeArgs a;
-    memset(&a, 0, sizeof(a));
-    a.in = v; a.out = v;
-    a.tw = twiddles_for(g.ny);
-    a.in_plane_stride = a.out_plane_stride = (long long)g.ny * g.nz;
-    a.in_line_stride = a.out_line_stride = g.nz;
-    a.n_lines = g.nz;
-    a.n_planes = 3 * g.nx;
-    a.grid_ny = g.ny;
-    a.grid_nx = g.nz;
-    a.in_len = a.out_len = g.ny;
-    a.out_line_len = g.nz;
-    a.seq = dir < 0 ? SEQ_F : SEQ_I;
-    a.scale = a.mid_scale = 1.0f;
-    if ((rc = launch_pipe_col(g.ny, a, s))) return rc;
-  }
-  // x: column strips of the (nx x ny nz) matrix, 3 planes
-  {
-    PipeArgs a;
-    memset(&a, 0, sizeof(a));
-    a.in = v; a.out = v;
-    a.tw = twiddles_for(g.nx);
-    a.in_plane_stride = a.out_plane_stride = g.vox;
-    a.in_line_stride = a.out_line_stride = g.ny * g.nz;
-    a.n_lines = g.ny * g.nz;
-    a.n_planes = 3;
-    a.grid_ny = g.nx;
-    a.grid_nx = g.ny * g.nz;
-    a.in_len = a.out_len = g.nx;
-    a.out_line_len = g.ny * g.nz;
-    a.seq = dir < 0 ? SEQ_F : SEQ_I;
-    a.scale = dir < 0 ? 1.0f : inv;
-    a.mid_scale = 1.0f;
-    if ((rc = launch_pipe_col(g.nx, a, s))) return rc;
-  }
-  return CXM_OK;
+  if ((rc = pass_y(g, gb, 1, 1.0f, s))) return rc;
+  return pass_z(g, gb, 1, 1.0f, s);
 }
 
 C2 c2(double re, double im) { return C2{(float)re, (float)im}; }
@@ -652,12 +782,7 @@ extern "C" int cxm_mbs_precondition(void* stream, const void* eps, const void* s
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)source,
                      nullptr, py, g.vox, 1, 0, sc, kb, sc);
   CXM_LAUNCH_CHECK();
-  if ((rc = fft3(g, py, -1, s))) return rc;
-  const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
-  hipLaunchKernelGGL(mbs_green_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, py, g.vox, g.nx, g.ny,
-                     g.nz, kscale, sc, kb);
-  CXM_LAUNCH_CHECK();
-  if ((rc = fft3(g, py, 1, s))) return rc;
+  if ((rc = green_apply(g, d, py, sc, kb, s))) return rc;
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)py,
                      (const float2*)eps, py, g.vox, 0, d->eps_tensor, sc, kb, C2{-1.0f, 0.0f});
   CXM_LAUNCH_CHECK();
@@ -690,16 +815,11 @@ extern "C" int cxm_mbs_iterate(void* stream, const void* eps, const void* prec_y
                    d->bias_re * d->scale_im + d->bias_im * d->scale_re);
   float2* gb = (float2*)workspace;
   double* partial = (double*)((char*)workspace + (size_t)3 * g.vox * sizeof(float2));
-  const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
   for (int it = 0; it < iterations; ++it) {
     hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)x,
                        (const float2*)eps, gb, g.vox, 0, d->eps_tensor, sc, kb, C2{1.0f, 0.0f});
     CXM_LAUNCH_CHECK();
-    if ((rc = fft3(g, gb, -1, s))) return rc;
-    hipLaunchKernelGGL(mbs_green_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, gb, g.vox, g.nx,
-                       g.ny, g.nz, kscale, sc, kb);
-    CXM_LAUNCH_CHECK();
-    if ((rc = fft3(g, gb, 1, s))) return rc;
+    if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
     hipLaunchKernelGGL(mbs_update_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s,
                        (const float2*)gb, (const float2*)eps, (const float2*)prec_y, (float2*)x,
                        g.vox, d->eps_tensor, sc, kb, partial);
@@ -749,16 +869,6 @@ BiLayout bi_layout(const MbsGeom& g, void* state) {
 }
 static_assert(sizeof(BiScal) <= 256, "BiCGStab scalars exceed their slot");
 
-// g <- G(g): forward 3D FFT, Green multiply, inverse 3D FFT
-int green_apply(const MbsGeom& g, const cxm_mbs_desc* d, float2* gb, C2 sc, C2 kb, hipStream_t s) {
-  int rc;
-  if ((rc = fft3(g, gb, -1, s))) return rc;
-  const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
-  hipLaunchKernelGGL(mbs_green_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, gb, g.vox, g.nx, g.ny,
-                     g.nz, kscale, sc, kb);
-  CXM_LAUNCH_CHECK();
-  return fft3(g, gb, 1, s);
-}
 
 int bi_check(const cxm_mbs_desc* d, MbsGeom& g, const void* eps, const void* x, void* state,
              size_t state_bytes, double* status, void* workspace, size_t workspace_bytes) {

@@ -667,6 +667,10 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         1 = one group per workgroup everywhere; k > 1 =
  *                         the looping pass everywhere with k workgroups per
  *                         CU.  Same results.
+ *   CXM_OPT_MBS_XGREEN    modified Born series: 0 = the x-axis FFT, the
+ *                         Green's projection and the inverse x-axis FFT run
+ *                         as one pass where the x length has one; 2 = as
+ *                         three launches.  Equal to float32 rounding.
  */
 #define CXM_OPT_COL_VARIANT 1
 #define CXM_OPT_CHUNK_PLANES 2
@@ -676,6 +680,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_COL_ORDER 6
 #define CXM_OPT_TILES 7
 #define CXM_OPT_ROW_LOOP 8
+#define CXM_OPT_MBS_XGREEN 9
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

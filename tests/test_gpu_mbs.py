@@ -230,3 +230,23 @@ def test_device_enclosing_circle_matches_host(case):
     np.testing.assert_allclose(r_d, r_h, rtol=1e-9)
     # every value lies in the circle
     assert np.max(np.abs(e32.astype(np.complex128) - c_d)) <= r_d * (1 + 1e-12) + 1e-15
+
+
+@pytest.mark.parametrize("n,bc", [(24, 4), (40, 4), (64, 8)])
+def test_fused_green_pass_matches_separate_passes(n, bc):
+    # the fused x-axis FFT + Green + inverse pass against the three
+    # launches (CXM_OPT_MBS_XGREEN 2): the same operator to float32 rounding
+    from chromatix_amd import _lib as L
+
+    eps, spacing, J = _setup(n, bc, 2, 12)
+    sample = mbs.Sample(torch.as_tensor(eps.astype(np.complex64)).cuda(), spacing)
+    src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
+    shift, scale = om.get_shift_and_scale(eps)
+    out = []
+    try:
+        for opt in (0, 2):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_MBS_XGREEN, opt))
+            out.append(mbs.solve(sample, src, maxiter=12, tol=0.0, shift_scale=(shift, scale)).cpu().numpy())
+    finally:
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_MBS_XGREEN, 0))
+    assert rel_l2(out[0], out[1]) < 1e-6
