@@ -273,11 +273,19 @@ mbs_update_kernel(const float2* __restrict__ g, const float2* __restrict__ eps,
 }
 
 // out[0] = sum partial (fixed order), out[1] (optional max) untouched
-__global__ void sum_partials_kernel(const double* __restrict__ partial, int n, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// (one block of kRedThreads: strided partial sums, then a fixed tree)
+__global__ void __launch_bounds__(kRedThreads)
+sum_partials_kernel(const double* __restrict__ partial, int n, double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
   double acc = 0.0;
-  for (int i = 0; i < n; ++i) acc += partial[i];
-  out[0] = acc;
+  for (int i = threadIdx.x; i < n; i += kRedThreads) acc += partial[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 // partial[block] = max |v| over 3 planar components (as FP64)
@@ -299,11 +307,18 @@ maxabs_kernel(const float2* __restrict__ v, long long n, double* __restrict__ pa
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-__global__ void max_partials_kernel(const double* __restrict__ partial, int n, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__global__ void __launch_bounds__(kRedThreads)
+max_partials_kernel(const double* __restrict__ partial, int n, double* __restrict__ out) {
+  __shared__ double sh[kRedThreads];
   double m = 0.0;
-  for (int i = 0; i < n; ++i) m = fmax(m, partial[i]);
-  out[0] = m;
+  for (int i = threadIdx.x; i < n; i += kRedThreads) m = fmax(m, partial[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
 }
 
 // v[c][i] *= 1 / max (normalisation by max |prec_y|, read on the device)
@@ -790,7 +805,7 @@ extern "C" int cxm_mbs_precondition(void* stream, const void* eps, const void* s
   hipLaunchKernelGGL(maxabs_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, (const float2*)py,
                      3 * g.vox, partial);
   CXM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(max_partials_kernel, dim3(1), dim3(64), 0, s, partial, kRedBlocks, num_scale);
+  hipLaunchKernelGGL(max_partials_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, kRedBlocks, num_scale);
   CXM_LAUNCH_CHECK();
   hipLaunchKernelGGL(scale_by_inv_kernel, dim3(blocks_for(3 * g.vox)), dim3(256), 0, s, py, 3 * g.vox,
                      (const double*)num_scale);
@@ -826,7 +841,8 @@ extern "C" int cxm_mbs_iterate(void* stream, const void* eps, const void* prec_y
     CXM_LAUNCH_CHECK();
   }
   if (iterations > 0) {
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(64), 0, s, partial, kRedBlocks, residual_sq);
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, kRedBlocks,
+                       residual_sq);
     CXM_LAUNCH_CHECK();
   }
   return CXM_OK;
@@ -1036,22 +1052,45 @@ ec_extreme_kernel(const float2* __restrict__ v, long long n, double* __restrict_
 }
 
 // ext[2 d], ext[2 d + 1]: the extreme value in direction d; count = 0
-__global__ void ec_extreme_final_kernel(const float2* __restrict__ v, const double* __restrict__ partial,
-                                        double* __restrict__ ext, int* __restrict__ count) {
-  const int d = threadIdx.x;
-  if (d >= kEcDirs) return;
+// (256 threads: 32 per direction scan the block partials, then the best of
+// each 32 with ties to the earlier block, as a serial scan would pick)
+__global__ void __launch_bounds__(256)
+ec_extreme_final_kernel(const float2* __restrict__ v, const double* __restrict__ partial,
+                        double* __restrict__ ext, int* __restrict__ count) {
+  const int d = threadIdx.x / 32, l = threadIdx.x % 32;
   double best = -INFINITY;
-  long long arg = 0;
-  for (int b = 0; b < kEcBlocks; ++b) {
+  int bb = -1;
+  for (int b = l; b < kEcBlocks; b += 32) {
     const double p = partial[(b * kEcDirs + d) * 2];
     if (p > best) {
       best = p;
-      arg = (long long)partial[(b * kEcDirs + d) * 2 + 1];
+      bb = b;
     }
   }
-  ext[2 * d] = v[arg].x;
-  ext[2 * d + 1] = v[arg].y;
-  if (d == 0) *count = 0;
+  __shared__ double sv[256];
+  __shared__ int sb[256];
+  sv[threadIdx.x] = best;
+  sb[threadIdx.x] = bb;
+  __syncthreads();
+  for (int w = 16; w > 0; w >>= 1) {
+    if (l < w) {
+      const double o = sv[threadIdx.x + w];
+      const int ob = sb[threadIdx.x + w];
+      const int mb = sb[threadIdx.x];
+      if (o > sv[threadIdx.x] || (o == sv[threadIdx.x] && ob >= 0 && (mb < 0 || ob < mb))) {
+        sv[threadIdx.x] = o;
+        sb[threadIdx.x] = ob;
+      }
+    }
+    __syncthreads();
+  }
+  if (l == 0) {
+    const int b = sb[threadIdx.x];
+    const long long arg = b >= 0 ? (long long)partial[(b * kEcDirs + d) * 2 + 1] : 0;
+    ext[2 * d] = v[arg < 0 ? 0 : arg].x;
+    ext[2 * d + 1] = v[arg < 0 ? 0 : arg].y;
+  }
+  if (threadIdx.x == 0) *count = 0;
 }
 
 // values strictly outside the octagon of the extremes (counter-clockwise
@@ -1179,7 +1218,7 @@ extern "C" int cxm_enclosing_circle_c64(void* stream, const void* values, int64_
   int* count = (int*)(ext + kEcDirs * 2);
   hipLaunchKernelGGL(ec_extreme_kernel, dim3(kEcBlocks), dim3(256), 0, s, v, (long long)n, partial);
   CXM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(ec_extreme_final_kernel, dim3(1), dim3(64), 0, s, v, (const double*)partial, ext,
+  hipLaunchKernelGGL(ec_extreme_final_kernel, dim3(1), dim3(256), 0, s, v, (const double*)partial, ext,
                      count);
   CXM_LAUNCH_CHECK();
   hipLaunchKernelGGL(ec_filter_kernel, dim3(kEcBlocks), dim3(256), 0, s, v, (long long)n,
