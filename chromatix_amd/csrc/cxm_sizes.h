@@ -21,7 +21,7 @@
 #define CXM_SIZES_G4(X)                                                       \
   X(3, 3, 3, 64) X(6, 6, 6, 64) X(12, 12, 12, 64) X(24, 12, 12, 64)          \
   X(48, 12, 12, 32) X(96, 12, 12, 32) X(192, 12, 12, 16) X(384, 24, 24, 16)  \
-  X(768, 24, 24, 8) X(1536, 24, 24, 4) X(3072, 24, 24, 2)
+  X(768, 24, 24, 8) X(1536, 24, 24, 4) X(3072, 24, 24, 4)
 
 #define CXM_FOR_EACH_SIZE(X) \
   CXM_SIZES_G0(X) CXM_SIZES_G1(X) CXM_SIZES_G2(X) CXM_SIZES_G3(X) CXM_SIZES_G4(X)
