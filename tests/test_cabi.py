@@ -40,7 +40,8 @@ def test_every_declared_symbol_is_exported():
                                        (L.CztDesc, "cxm_czt_desc"),
                                        (L.YuDesc, "cxm_yu_desc"),
                                        (L.FluorDesc, "cxm_fluor_desc"),
-                                       (L.MbsDesc, "cxm_mbs_desc")])
+                                       (L.MbsDesc, "cxm_mbs_desc"),
+                                       (L.FconvDesc, "cxm_fconv_desc")])
 def test_descriptor_layout_matches_header(tmp_path, cls, cname):
     fields = [f for f, _ in cls._fields_]
     src = tmp_path / "layout.c"
@@ -113,3 +114,33 @@ def test_cpu_tensors_are_refused():
     with pytest.raises(RuntimeError, match="GPU only"):
         _ = f.intensity
     _ = np
+
+
+def test_validation_errors_of_the_imaging_and_born_entries():
+    lib = L.lib()
+    d = L.FconvDesc()
+    d.batch, d.depth, d.height, d.width = 2, 1, 40, 50
+    d.k_planes, d.k_depth, d.k_height, d.k_width = 1, 1, 9, 9
+    d.real_input = 1
+    assert lib.cxm_fconv_workspace_size(ctypes.byref(d)) > 0
+    d.mode = 2
+    assert lib.cxm_fconv_workspace_size(ctypes.byref(d)) == 0
+    assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_INVALID
+    d.mode, d.height = 0, 9000  # beyond the implemented transform lengths
+    assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_UNSUPPORTED
+    d.height, d.mode = 40, 1  # full mode needs its output shape
+    assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_INVALID
+    assert lib.cxm_sum_axis_f32(None, None, None, -1, 1, 1) == L.CXM_ERR_INVALID
+    assert lib.cxm_sum_axis_f32(None, None, None, 0, 4, 4) == L.CXM_OK  # nothing to do
+    assert lib.cxm_shot_noise_approx_f32(None, None, None, None, -1) == L.CXM_ERR_INVALID
+    assert lib.cxm_shot_noise_approx_f32(None, None, None, None, 0) == L.CXM_OK
+    out = (ctypes.c_double * 3)()
+    assert lib.cxm_enclosing_circle_c64(None, None, 0, out, None, 0) == L.CXM_ERR_INVALID
+    assert lib.cxm_enclosing_circle_workspace_size(0) == 0
+    m = L.MbsDesc()
+    m.nx = m.ny = m.nz = 32
+    m.spacing, m.k0 = 0.133, 11.8
+    assert lib.cxm_mbs_bicgstab_state_size(ctypes.byref(m)) > 0
+    m.eps_tensor = 3
+    assert lib.cxm_mbs_bicgstab_state_size(ctypes.byref(m)) == 0
+    assert lib.cxm_mbs_workspace_size(ctypes.byref(m)) == 0
