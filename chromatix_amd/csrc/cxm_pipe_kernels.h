@@ -40,6 +40,9 @@ struct PipeArgs {
   int accum;                  // phase-gradient row pass: add to `out` instead of storing
                               // (gridDim.y > 1: plane group y writes its partial sum to
                               // out + y * out_plane_stride)
+  int partial_complex;        // phase-gradient row pass: store the complex plane sum
+                              // sum_p g conj(pre(1)) (complex64 out) and leave the
+                              // `mid` operators and Im() to the caller's reduction
   // optional store of the value after the `mid` operators (row pass only),
   // grid position (y, x) -> mid_out[plane * mid_plane_stride + (y - mid_y0) *
   // mid_row_stride + (x - mid_x0)] for y - mid_y0 in [0, mid_h), x - mid_x0 in [0, mid_w)
@@ -249,16 +252,18 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
   constexpr int T = Cfg::T;
   constexpr int SPAN = lds_span<N, 1>();
   __shared__ float2 lds[Cfg::G * SPAN];
-  __shared__ float2 acc_lds[Cfg::G * N];  // the plane sum of g conj(pre(1)), per element
   const int g = threadIdx.x / T;
   const int t = threadIdx.x - g * T;
   const int r = blockIdx.x * Cfg::G + g;
   const bool ok = r < a.n_lines;
   const int yg = wave_uniform<T % 64 == 0>(a.line_off + (ok ? r : 0));
   float2* st = lds + g * SPAN;
-  float2* acc = acc_lds + g * N;
+  // the plane sum of g conj(pre(1)) stays in registers, element t + k T in
+  // acc[k]; each plane's products pass through this thread's own LDS slots
+  // (the operator loop is rolled), so LDS holds one row per thread group
+  float2 acc[E];
 #pragma unroll
-  for (int k = 0; k < E; ++k) acc[t + k * T] = make_float2(0.0f, 0.0f);
+  for (int k = 0; k < E; ++k) acc[k] = make_float2(0.0f, 0.0f);
   const int per = (a.n_planes + (int)gridDim.y - 1) / (int)gridDim.y;
   const int p_lo = (int)blockIdx.y * per;
   const int p_hi = min(a.n_planes, p_lo + per);
@@ -275,17 +280,37 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
       const int j = t + k * T;
       const int xf = a.in_shift ? ishift_src(j, N) : j;
       const int xs = xf - a.in_off;
+      float2 prod = make_float2(0.f, 0.f);
       if (ok && xs >= 0 && xs < a.in_len) {
         const float2 gv = cscale(st[lds_at<N, 1>(j)], a.scale);
         const float2 sv = apply_ops(a.pre, a.tab, make_float2(1.f, 0.f), a.plane0 + p, yg, xf,
                                     a.grid_ny, a.grid_nx);
-        const float2 c = acc[j];
-        acc[j] = make_float2(c.x + (gv.x * sv.x + gv.y * sv.y), c.y + (gv.y * sv.x - gv.x * sv.y));
+        prod = make_float2(gv.x * sv.x + gv.y * sv.y, gv.y * sv.x - gv.x * sv.y);
       }
+      st[lds_at<N, 1>(j)] = prod;
+    }
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const float2 q = st[lds_at<N, 1>(t + k * T)];
+      acc[k] = make_float2(acc[k].x + q.x, acc[k].y + q.y);
     }
     __syncthreads();  // before the next plane's FFT reuses the slots
   }
   if (!ok) return;
+  if (a.partial_complex) {
+    float2* cdst = (float2*)a.out + (long long)blockIdx.y * a.out_plane_stride +
+                   (long long)r * a.out_line_stride;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int j = t + k * T;
+      const int xs = (a.in_shift ? ishift_src(j, N) : j) - a.in_off;
+      if (xs >= 0 && xs < a.in_len) cdst[xs] = acc[k];
+    }
+    return;
+  }
+  // own slots only from here on
+#pragma unroll
+  for (int k = 0; k < E; ++k) st[lds_at<N, 1>(t + k * T)] = acc[k];
   float* dst = (float*)a.out + (long long)blockIdx.y * a.out_plane_stride +
                (long long)r * a.out_line_stride;
   const bool add = a.accum && gridDim.y == 1;
@@ -295,7 +320,7 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
     const int xf = a.in_shift ? ishift_src(j, N) : j;
     const int xs = xf - a.in_off;
     if (xs >= 0 && xs < a.in_len) {
-      const float2 c = acc[j];
+      const float2 c = st[lds_at<N, 1>(j)];
       const float2 w = a.mid.n ? apply_ops(a.mid, a.tab, make_float2(1.f, 0.f), a.plane0, yg, xf,
                                            a.grid_ny, a.grid_nx)
                                : make_float2(1.f, 0.f);

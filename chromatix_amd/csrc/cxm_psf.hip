@@ -207,25 +207,36 @@ int psf_geom(const cxm_psf4f_desc* d) {
 
 using namespace cxm;
 
-// Plane groups of the phase-gradient pass (run in parallel, partial sums
-// reduced in a fixed order afterwards).
-constexpr int kGradGroups = 4;
+// Plane groups of the phase-gradient pass (run in parallel, complex partial
+// sums reduced in a fixed order afterwards, where the plane-invariant
+// operators are applied once per element).
+constexpr int kGradGroups = 12;
 
 namespace cxm {
 namespace {
 size_t psf_ws_bytes(const cxm_psf4f_desc* d) {
   const size_t plane = (size_t)d->height * d->width;
-  return (size_t)psf_chunk(d) * plane * sizeof(float2) + kGradGroups * plane * sizeof(float);
+  return (size_t)psf_chunk(d) * plane * sizeof(float2) + kGradGroups * plane * sizeof(float2);
 }
 
-// grad[i] (+)= sum_g partial[g][i], g in order
-__global__ void sum_groups_kernel(const float* __restrict__ partial, float* __restrict__ grad,
-                                  long long n, int groups, int accum) {
+// grad[i] (+)= Im(conj(w) sum_g partial[g][i]), g in order, w = the
+// plane-invariant operators at grid position i
+__global__ void sum_groups_kernel(const float2* __restrict__ partial, float* __restrict__ grad,
+                                  int ny, int nx, int groups, int accum, int plane0, OpList mid,
+                                  OpTable tab) {
+  const long long n = (long long)ny * nx;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float acc = accum ? grad[i] : 0.0f;
-  for (int g = 0; g < groups; ++g) acc += partial[g * n + i];
-  grad[i] = acc;
+  float2 c = make_float2(0.f, 0.f);
+  for (int g = 0; g < groups; ++g) {
+    const float2 q = partial[g * n + i];
+    c = make_float2(c.x + q.x, c.y + q.y);
+  }
+  const int iy = (int)(i / nx), ix = (int)(i - (long long)iy * nx);
+  const float2 w = mid.n ? apply_ops(mid, tab, make_float2(1.f, 0.f), plane0, iy, ix, ny, nx)
+                         : make_float2(1.f, 0.f);
+  const float v = w.x * c.y - w.y * c.x;
+  grad[i] = accum ? grad[i] + v : v;
 }
 }  // namespace
 }  // namespace cxm
@@ -287,7 +298,7 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
   if ((rc = ensure_init())) return rc;
   hipStream_t s = (hipStream_t)stream;
   const long long plane_px = (long long)d->height * d->width;
-  float* partial = (float*)((char*)workspace + (size_t)chunk * plane_px * sizeof(float2));
+  float2* partial = (float2*)((char*)workspace + (size_t)chunk * plane_px * sizeof(float2));
   PipeArgs ra, ca, ga;
   memset(&ra, 0, sizeof(ra));
   memset(&ca, 0, sizeof(ca));
@@ -320,6 +331,7 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
   }
   ga.in_plane_stride = (long long)d->height * d->width;
   ga.in_line_stride = d->width;
+  ga.partial_complex = 1;
   ga.out = partial;
   ga.out_plane_stride = plane_px;
   ga.out_line_stride = d->width;
@@ -340,7 +352,8 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
     const int groups = std::min<int>(kGradGroups, kc);
     if ((rc = launch_pipe_phasegrad(d->width, ga, s, groups))) return rc;
     hipLaunchKernelGGL(sum_groups_kernel, dim3((unsigned)((plane_px + 255) / 256)), dim3(256), 0, s,
-                       (const float*)partial, grad_phase, plane_px, groups, p0 > 0 ? 1 : 0);
+                       (const float2*)partial, grad_phase, d->height, d->width, groups, p0 > 0 ? 1 : 0,
+                       (int)p0, ga.mid, ga.tab);
     CXM_LAUNCH_CHECK();
   }
   return CXM_OK;
