@@ -69,6 +69,7 @@ struct RowArgs {
   // row range (in_off 0, in_len N, or out_off 0, out_len N).
   int in_tiled, out_tiled;
   long long in_strip_stride, out_strip_stride;  // elements
+  int nt_in, nt_out;     // non-temporal loads of `in` / stores of the output
 };
 
 struct ColArgs {

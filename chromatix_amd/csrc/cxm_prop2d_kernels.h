@@ -78,10 +78,30 @@ CXM_DEV const float2* row_ptr(const RowArgs& a, int p, int r) {
   return a.in_real ? (const float2*)((const float*)a.in + off) : a.in + off;
 }
 
+// Non-temporal (streaming) load and store of one complex64 value.
+typedef float cxm_f2v __attribute__((ext_vector_type(2)));
+CXM_DEV float2 nt_load2(const float2* p) {
+  const cxm_f2v v = __builtin_nontemporal_load((const cxm_f2v*)p);
+  return make_float2(v.x, v.y);
+}
+CXM_DEV void nt_store2(float2* p, float2 w) {
+  cxm_f2v v;
+  v.x = w.x;
+  v.y = w.y;
+  __builtin_nontemporal_store(v, (cxm_f2v*)p);
+}
+
 // Store of one output value: complex, |u|^2 or the real part.
 CXM_DEV void row_store(const RowArgs& a, long long idx, float2 w) {
-  if (a.intensity) a.out_i[idx] = (a.intensity == 2) ? w.x : cabs2(w);
-  else a.out_c[idx] = w;
+  if (a.intensity) {
+    const float x = (a.intensity == 2) ? w.x : cabs2(w);
+    if (a.nt_out) __builtin_nontemporal_store(x, a.out_i + idx);
+    else a.out_i[idx] = x;
+  } else if (a.nt_out) {
+    nt_store2(a.out_c + idx, w);
+  } else {
+    a.out_c[idx] = w;
+  }
 }
 
 template <int N, int E, int DIR>
@@ -198,8 +218,15 @@ row_fft_loop_kernel(RowArgs a) {
 #pragma unroll
       for (int k = 0; k < E; k += 2) {
         const int x0 = t + k * T - a.in_off, x1 = x0 + T;
-        const float2 u0 = (ok && x0 >= 0 && x0 < a.in_len) ? src[x0] : make_float2(0.f, 0.f);
-        const float2 u1 = (ok && x1 >= 0 && x1 < a.in_len) ? src[x1] : make_float2(0.f, 0.f);
+        const bool in0 = ok && x0 >= 0 && x0 < a.in_len, in1 = ok && x1 >= 0 && x1 < a.in_len;
+        float2 u0 = make_float2(0.f, 0.f), u1 = make_float2(0.f, 0.f);
+        if (a.nt_in) {
+          if (in0) u0 = nt_load2(src + x0);
+          if (in1) u1 = nt_load2(src + x1);
+        } else {
+          if (in0) u0 = src[x0];
+          if (in1) u1 = src[x1];
+        }
         pf[k / 2] = make_float4(u0.x, u0.y, u1.x, u1.y);
       }
     }
