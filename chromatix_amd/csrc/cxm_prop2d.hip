@@ -370,7 +370,7 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
     r2.out_off = d->crop_x;
     r2.out_len = g.Wout;
     r2.scale = 1.0f;
-    r2.nt_out = (option(CXM_OPT_NT_IO) == 2 || option(CXM_OPT_NT_IO) == 3);
+    r2.nt_out = (option(CXM_OPT_NT_IO) == 0 || option(CXM_OPT_NT_IO) == 2);
     const size_t out_off_planes = (size_t)p0 * nz;
     if (d->out_kind == CXM_OUT_INTENSITY) {
       r2.intensity = 1;

@@ -669,10 +669,11 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         CU.  Same results.
  *   CXM_OPT_NT_IO         cxm_prop2d_c64, cache hints for the planes read
  *                         and written once: 0 = non-temporal (streaming)
- *                         loads of the input, so that the intermediates keep
- *                         the Infinity Cache (measured +9% on 64 x 1024^2);
- *                         1 = no hints; 2 = non-temporal stores of the output
- *                         only; 3 = both (measured slower).  Same results.
+ *                         loads of the input and stores of the output, so
+ *                         that the intermediates keep the Infinity Cache
+ *                         (measured +11% on 64 x 1024^2); 1 = no hints;
+ *                         2 = streaming stores only; 3 = streaming loads
+ *                         only.  Same results.
  *   CXM_OPT_MBS_XGREEN    modified Born series: 0 = the x-axis FFT, the
  *                         Green's projection and the inverse x-axis FFT run
  *                         as one pass where the x length has one; 2 = as
