@@ -104,9 +104,10 @@ size_t fc_ws(const cxm_fconv_desc* d, const FcGeom& g) {
 // Row FFT of `planes` input planes of `rows` x `cols` (float32 or complex64,
 // zero-padded to Lx) into rows of Lx elements, planes `out_plane` apart.
 int row_forward(const void* in, int real, long long in_plane, int rows, int cols, float2* out,
-                long long out_plane, int Lx, int planes, hipStream_t s) {
+                long long out_plane, int Lx, int planes, hipStream_t s, int nt = 0) {
   RowArgs r;
   memset(&r, 0, sizeof(r));
+  r.nt_in = nt;
   r.in = (const float2*)in;
   r.in_real = real;
   r.out_c = out;
@@ -206,7 +207,8 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
     const char* img = (const char*)image + (size_t)v0 * in_vol * esz;
     // forward transforms of the rows (and, for volumes, the columns)
     if (!g.vol) {
-      if ((rc = row_forward(img, d->real_input, in_plane, d->height, d->width, ws, g.vol_ws, Lx, kc, s)))
+      // image planes are read once: streaming loads
+      if ((rc = row_forward(img, d->real_input, in_plane, d->height, d->width, ws, g.vol_ws, Lx, kc, s, 1)))
         return rc;
     } else {
       for (int v = 0; v < kc; ++v) {
@@ -263,6 +265,7 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
     r.out_len = g.x.keep;
     r.scale = 1.0f;
     r.intensity = d->real_input ? 2 : 0;
+    r.nt_out = 1;  // written once
     r.out_plane_stride = out_plane;
     r.rows_per_plane = g.y.keep;
     if (!g.vol) {

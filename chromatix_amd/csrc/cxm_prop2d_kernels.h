@@ -66,9 +66,18 @@ CXM_DEV void lds_to_tiles(const float2* lds, float2* base, long long strip_strid
   }
 }
 
-// Element xs of an input row: complex64, or float32 with imaginary part 0.
+// Element xs of an input row: complex64, or float32 with imaginary part 0
+// (streaming loads when nt_in).
 CXM_DEV float2 row_load(const RowArgs& a, const float2* row, int xs) {
-  if (a.in_real) return make_float2(((const float*)row)[xs], 0.f);
+  if (a.in_real) {
+    const float* q = (const float*)row + xs;
+    return make_float2(a.nt_in ? __builtin_nontemporal_load(q) : *q, 0.f);
+  }
+  if (a.nt_in) {
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    const f2v v = __builtin_nontemporal_load((const f2v*)(row + xs));
+    return make_float2(v.x, v.y);
+  }
   return row[xs];
 }
 
