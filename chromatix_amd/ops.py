@@ -43,8 +43,8 @@ def _kernel_planes(kb: tuple[int, ...], ob: tuple[int, ...]) -> int | None:
 
 def fourier_convolution(image, kernel, *, axes=(0, 1), fast_fft_shape: bool = True,
                         mode: str = "same") -> torch.Tensor:
-    """Convolution of ``image`` with ``kernel`` over two or three contiguous
-    ``axes`` by the Fourier convolution theorem, zero-padded against
+    """Convolution of ``image`` with ``kernel`` over one, two or three
+    contiguous ``axes`` by the Fourier convolution theorem, zero-padded against
     circular wrap.  ``mode`` "same" crops to the image's shape (window
     starting at (k - 1) // 2 per axis, as the reference), "full" returns the
     reference's transform shape (n + k per axis, rounded up to a power of two
@@ -66,8 +66,16 @@ def fourier_convolution(image, kernel, *, axes=(0, 1), fast_fft_shape: bool = Tr
         if axes[i + 1] != axes[i] + 1:
             raise ValueError("Axes to convolve over must be contiguous")
     na = len(axes)
+    if na == 1:
+        # one axis: the two-axis convolution with a unit axis before it
+        ax = axes[0]
+        out = fourier_convolution(image.unsqueeze(ax), kernel.unsqueeze(ax), axes=(ax, ax + 1),
+                                  fast_fft_shape=fast_fft_shape, mode=mode)
+        if mode == "full":
+            out = out.narrow(ax, 0, 1)
+        return out.squeeze(ax)
     if na not in (2, 3):
-        raise NotImplementedError("fourier_convolution: convolution over two or three axes")
+        raise NotImplementedError("fourier_convolution: convolution over one, two or three axes")
     cplx = image.is_complex() or kernel.is_complex()
     dt = torch.complex64 if cplx else torch.float32
     perm = [i for i in range(nd) if i not in axes] + axes

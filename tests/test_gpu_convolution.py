@@ -219,3 +219,21 @@ def test_microscope_with_4f_psf_end_to_end():
     ref = oc.microscope_image(volume, psf, reduce_axis=0)
     assert img.shape == (n, n)
     assert rel_l2(img, ref) < TOL
+
+
+@pytest.mark.parametrize("mode", ["same", "full"])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_one_axis(mode, cplx):
+    rng = np.random.default_rng(10)
+    a, k = _arr(rng, (3, 200), cplx), _arr(rng, (3, 31), cplx)
+    out = _gpu(a, k, axes=(1,), mode=mode)
+    ref = oc.fourier_convolution(a, k, axes=(1,), mode=mode)
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) < TOL
+    # the default axes (0, 1) of a 2D array against axes (-1,) of a 3D one
+    b = _arr(rng, (2, 5, 64), cplx)
+    kk = _arr(rng, (1, 1, 9), cplx)
+    out = _gpu(b, kk, axes=(-1,), mode=mode)
+    ref = oc.fourier_convolution(b, kk, axes=(-1,), mode=mode)
+    assert out.shape == ref.shape
+    assert rel_l2(out, ref) < TOL
