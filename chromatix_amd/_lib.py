@@ -42,6 +42,7 @@ CXM_OPT_TILES = 7
 CXM_OPT_ROW_LOOP = 8
 CXM_OPT_MBS_XGREEN = 9
 CXM_OPT_NT_IO = 10
+CXM_OPT_FCONV_PAIR = 11
 CXM_H_TABLE_MIN_PLANES = 4
 
 PHASE_MODES = {"ref32": CXM_PHASE_REF32, "accurate": CXM_PHASE_ACCURATE}

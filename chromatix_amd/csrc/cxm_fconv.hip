@@ -17,6 +17,7 @@
 #include <string>
 
 #include "cxm_pipe_host.h"
+#include "cxm_prop2d_kernels.h"
 
 namespace cxm {
 namespace {
@@ -101,6 +102,208 @@ size_t fc_ws(const cxm_fconv_desc* d, const FcGeom& g) {
   return ((size_t)d->k_planes * g.kvol + (size_t)g.chunk * g.vol_ws) * sizeof(float2);
 }
 
+// ---- real image and kernel in one complex plane ------------------------------
+// For real image a and real kernel k of a plane, z = a + i s k (s a power of
+// two that brings s k to the scale of a, so that neither spectrum loses
+// relative precision to the other) has Z = A + i s K with A, K Hermitian, so
+// at every frequency f, with Z2 = conj(Z(-f)):
+//   A = (Z + Z2) / 2,  s K = (Z - Z2) / (2 i),  A K = (Z^2 - Z2^2) / (4 i s).
+// One forward transform of z replaces those of a and k (and the kernel
+// spectra table); the product is Hermitian, so the real part of its inverse
+// is the convolution.  The column pass pairs columns kx and -kx in one
+// workgroup to reach Z(-f).
+
+constexpr int kSsBlocks = 64;  // partial sums per plane
+
+// partial[(plane * kSsBlocks + b)] = sum of squares over a slice of the plane
+__global__ void __launch_bounds__(256)
+plane_ss_kernel(const float* __restrict__ x, long long plane_elems, double* __restrict__ partial) {
+  const int plane = blockIdx.y;
+  const float* px = x + (long long)plane * plane_elems;
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < plane_elems; i += (long long)kSsBlocks * 256)
+    acc += (double)px[i] * px[i];
+  __shared__ double sh[256];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(long long)plane * kSsBlocks + blockIdx.x] = sh[0];
+}
+
+// scale[b] = 2^round(log2(|a_b| / |k_{b % k_planes}|)) (1 if either is 0)
+__global__ void pair_scale_kernel(const double* __restrict__ pa, const double* __restrict__ pk,
+                                  int batch, int k_planes, float* __restrict__ scale) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  double sa = 0.0, sk = 0.0;
+  for (int j = 0; j < kSsBlocks; ++j) sa += pa[(long long)b * kSsBlocks + j];
+  const int kb = b % k_planes;
+  for (int j = 0; j < kSsBlocks; ++j) sk += pk[(long long)kb * kSsBlocks + j];
+  float s = 1.0f;
+  if (sa > 0.0 && sk > 0.0) {
+    const int e = (int)lrint(0.25 * log2(sa / sk) * 2.0);  // round(log2(sqrt(sa / sk)))
+    s = ldexpf(1.0f, max(-100, min(100, e)));
+  }
+  scale[b] = s;
+}
+
+struct PairRowArgs {
+  const float* img;          // batch x height x width
+  const float* ker;          // k_planes x k_height x k_width
+  const float* scale;        // per image plane
+  float2* out;               // planes of rows x L
+  const float2* tw;
+  long long img_plane, ker_plane, out_plane;
+  int height, width, k_height, k_width, k_planes;
+  int rows;                  // rows per plane = max(height, k_height)
+  int plane0, n_rows;        // first image plane, rows in this launch
+};
+
+template <int N, int E>
+__global__ void __launch_bounds__((RowCfg<N, E>::THREADS))
+pair_row_kernel(PairRowArgs a) {
+  using Cfg = RowCfg<N, E>;
+  constexpr int T = Cfg::T;
+  constexpr int G = Cfg::G;
+  constexpr bool SYNC = T > 64;
+  __shared__ float2 lds[G * Cfg::PITCH];
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x - g * T;
+  const int row = blockIdx.x * G + g;
+  const bool ok = row < a.n_rows;
+  const int p = ok ? row / a.rows : 0;
+  const int r = ok ? row - p * a.rows : 0;
+  const int b = a.plane0 + p;
+  const float s = a.scale[b];
+  const float* ir = a.img + (long long)b * a.img_plane + (long long)r * a.width;
+  const float* kr = a.ker + (long long)(b % a.k_planes) * a.ker_plane + (long long)r * a.k_width;
+  const bool iok = ok && r < a.height, kok = ok && r < a.k_height;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int x = t + k * T;
+    const float re = (iok && x < a.width) ? __builtin_nontemporal_load(ir + x) : 0.f;
+    const float im = (kok && x < a.k_width) ? kr[x] * s : 0.f;
+    v[k] = make_float2(re, im);
+  }
+  fft<N, E, -1, 1, SYNC>(v, lds + g * Cfg::PITCH, t, a.tw);
+  if (!ok) return;
+  float2* o = a.out + (long long)p * a.out_plane + (long long)r * N;
+#pragma unroll
+  for (int k = 0; k < E; ++k) o[t + k * T] = v[k];
+}
+
+struct PairColArgs {
+  float2* buf;               // planes of L_y x L_x, in place
+  const float2* tw;
+  const float* scale;        // per image plane
+  long long plane_stride;
+  int lx;                    // columns (even)
+  int in_len;                // rows holding data
+  int out_off, out_len;      // rows kept
+  int n_planes, plane0;
+  float norm;                // 1 / (L_y L_x)
+};
+
+// C slots per workgroup: slot s < C/2 holds column first(j), slot s + C/2
+// column second(j) of pair j = j0 + s; pair 0 is (0, L/2), each its own
+// mirror, pair j > 0 is (j, L - j).
+template <int N, int E, int C>
+__global__ void __launch_bounds__((N / E) * C)
+pair_col_kernel(PairColArgs a) {
+  static_assert(C % 2 == 0, "column pairs");
+  constexpr int T = N / E;
+  constexpr int H = C / 2;
+  __shared__ float2 lds[lds_span<N, C>()];
+  const int c = threadIdx.x % C;
+  const int ty = threadIdx.x / C;
+  const int npairs = a.lx / 2;
+  const int nblk = (npairs + H - 1) / H;
+  const int p = blockIdx.x / nblk;
+  const int pair = (blockIdx.x - p * nblk) * H + (c % H);
+  const bool second = c >= H;
+  const bool valid = pair < npairs;
+  const int col = !valid ? 0 : (pair == 0 ? (second ? a.lx / 2 : 0) : (second ? a.lx - pair : pair));
+  const int partner = (pair == 0) ? c : (second ? c - H : c + H);
+  float2* colp = a.buf + (long long)p * a.plane_stride + col;
+  float2 u[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = ty + k * T;
+    u[k] = (valid && i < a.in_len) ? colp[(long long)i * a.lx] : make_float2(0.f, 0.f);
+  }
+  fft<N, E, -1, C, true>(u, lds + c, ty, a.tw);
+  __syncthreads();  // the transform's last exchange has been read by every thread
+  stage_out<N, E, C>(u, lds + c, ty);
+  __syncthreads();
+  const float f = a.norm / (4.0f * a.scale[a.plane0 + p]);
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = ty + k * T;
+    const int mi = i == 0 ? 0 : N - i;
+    const float2 z = u[k];
+    const float2 w = lds[partner + lds_at<N, C>(mi)];
+    const float2 z2 = make_float2(w.x, -w.y);  // conj(Z(-f))
+    // D = z^2 - z2^2;  A K = D / (4 i s) = (D.y, -D.x) / (4 s)
+    const float dx = (z.x * z.x - z.y * z.y) - (z2.x * z2.x - z2.y * z2.y);
+    const float dy = 2.0f * (z.x * z.y - z2.x * z2.y);
+    u[k] = make_float2(dy * f, -dx * f);
+  }
+  __syncthreads();  // the partner values are read before the inverse transform writes LDS
+  fft<N, E, 1, C, true>(u, lds + c, ty, a.tw);
+  if (!valid) return;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int yl = ty + k * T - a.out_off;
+    if (yl >= 0 && yl < a.out_len) colp[(long long)yl * a.lx] = u[k];
+  }
+}
+
+#define CXM_PAIR_LAUNCH(N, ER, EC, C)                                                           \
+  case N: {                                                                                     \
+    if (row) {                                                                                  \
+      using Cfg = RowCfg<N, ER>;                                                                \
+      const long long blocks = ((long long)ra->n_rows + Cfg::G - 1) / Cfg::G;                   \
+      hipLaunchKernelGGL((pair_row_kernel<N, ER>), dim3((unsigned)blocks), dim3(Cfg::THREADS), 0, \
+                         s, *ra);                                                               \
+      CXM_LAUNCH_CHECK();                                                                       \
+      return CXM_OK;                                                                            \
+    }                                                                                           \
+    if constexpr ((C) % 2 == 0) {                                                               \
+      const long long blocks = (long long)ca->n_planes * ((ca->lx / 2 + (C) / 2 - 1) / ((C) / 2)); \
+      hipLaunchKernelGGL((pair_col_kernel<N, EC, C>), dim3((unsigned)blocks),                   \
+                         dim3((N / (EC)) * (C)), 0, s, *ca);                                    \
+      CXM_LAUNCH_CHECK();                                                                       \
+      return CXM_OK;                                                                            \
+    } else {                                                                                    \
+      return fail(CXM_ERR_UNSUPPORTED, "no paired column pass for length " + std::to_string(N)); \
+    }                                                                                           \
+  }
+
+// row: the paired row pass of length n (ra); else the paired column pass (ca)
+int launch_pair(int n, bool row, const PairRowArgs* ra, const PairColArgs* ca, hipStream_t s) {
+  switch (n) {
+    CXM_FOR_EACH_SIZE(CXM_PAIR_LAUNCH)
+    default:
+      return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT length " + std::to_string(n));
+  }
+}
+#undef CXM_PAIR_LAUNCH
+
+// paired passes need an even row length (column pairs kx, -kx) and an even
+// number of columns per column-pass workgroup at the column length
+bool pair_supported(int ly, int lx) {
+  bool ok_y = false;
+#define CXM_PAIR_OK(N, ER, EC, C) \
+  if (N == ly && (C) % 2 == 0) ok_y = true;
+  CXM_FOR_EACH_SIZE(CXM_PAIR_OK)
+#undef CXM_PAIR_OK
+  return ok_y && lx % 2 == 0;
+}
+
 // Row FFT of `planes` input planes of `rows` x `cols` (float32 or complex64,
 // zero-padded to Lx) into rows of Lx elements, planes `out_plane` apart.
 int row_forward(const void* in, int real, long long in_plane, int rows, int cols, float2* out,
@@ -153,10 +356,41 @@ int col_pass(float2* buf, long long plane_stride, int lines, int line_stride, in
 
 using namespace cxm;
 
+namespace cxm {
+namespace {
+// The paired path: real two-axis convolution with an even row length.
+bool use_pair(const cxm_fconv_desc* d, const FcGeom& g) {
+  return d->real_input && !g.vol && option(CXM_OPT_FCONV_PAIR) != 1 && d->batch <= 65535 &&
+         d->k_planes <= 65535 && pair_supported(g.y.L, g.x.L);
+}
+struct PairGeom {
+  int rz;         // rows holding data: max(height, k_height)
+  int rws;        // rows per workspace plane
+  int chunk;
+  size_t ws_off;  // bytes before the planes: scales and partial sums
+};
+PairGeom pair_geom(const cxm_fconv_desc* d, const FcGeom& g) {
+  PairGeom q;
+  q.rz = std::max(d->height, d->k_height);
+  q.rws = std::max(q.rz, g.y.keep);
+  const long long per = (long long)q.rws * g.x.L;
+  q.chunk = (int)std::max(1LL, std::min<long long>(std::max(1, d->batch), ((128LL << 20) / 8) / per));
+  const size_t scales = ((size_t)std::max(1, d->batch) * sizeof(float) + 255) / 256 * 256;
+  const size_t partial = (size_t)(std::max(1, d->batch) + d->k_planes) * kSsBlocks * sizeof(double);
+  q.ws_off = scales + (partial + 255) / 256 * 256;
+  return q;
+}
+size_t pair_ws(const cxm_fconv_desc* d, const FcGeom& g) {
+  const PairGeom q = pair_geom(d, g);
+  return q.ws_off + (size_t)q.chunk * q.rws * g.x.L * sizeof(float2);
+}
+}  // namespace
+}  // namespace cxm
+
 extern "C" size_t cxm_fconv_workspace_size(const cxm_fconv_desc* d) {
   FcGeom g;
   if (fc_geom(d, g)) return 0;
-  return fc_ws(d, g);
+  return use_pair(d, g) ? pair_ws(d, g) : fc_ws(d, g);
 }
 
 extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel, void* out,
@@ -166,10 +400,89 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
   if (rc) return rc;
   if (d->batch == 0) return CXM_OK;
   if (!image || !kernel || !out) return fail(CXM_ERR_INVALID, "null buffer");
-  if (!workspace || workspace_bytes < fc_ws(d, g))
-    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(fc_ws(d, g)) + " bytes");
+  const bool pair = use_pair(d, g);
+  const size_t need = pair ? pair_ws(d, g) : fc_ws(d, g);
+  if (!workspace || workspace_bytes < need)
+    return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   if ((rc = ensure_init())) return rc;
   hipStream_t s = (hipStream_t)stream;
+  if (pair) {
+    const PairGeom q = pair_geom(d, g);
+    float* scale = (float*)workspace;
+    double* pa = (double*)((char*)workspace +
+                           ((size_t)d->batch * sizeof(float) + 255) / 256 * 256);
+    double* pk = pa + (size_t)d->batch * kSsBlocks;
+    float2* ws = (float2*)((char*)workspace + q.ws_off);
+    const long long in_plane = (long long)d->height * d->width;
+    const long long k_plane = (long long)d->k_height * d->k_width;
+    hipLaunchKernelGGL(plane_ss_kernel, dim3(kSsBlocks, d->batch), dim3(256), 0, s, (const float*)image,
+                       in_plane, pa);
+    CXM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(plane_ss_kernel, dim3(kSsBlocks, d->k_planes), dim3(256), 0, s,
+                       (const float*)kernel, k_plane, pk);
+    CXM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(pair_scale_kernel, dim3((d->batch + 255) / 256), dim3(256), 0, s,
+                       (const double*)pa, (const double*)pk, d->batch, d->k_planes, scale);
+    CXM_LAUNCH_CHECK();
+    const int Ly = g.y.L, Lx = g.x.L;
+    const long long wplane = (long long)q.rws * Lx;
+    const long long out_plane = (long long)g.out_h * g.out_w;
+    for (int p0 = 0; p0 < d->batch; p0 += q.chunk) {
+      const int kc = std::min(q.chunk, d->batch - p0);
+      PairRowArgs ra;
+      memset(&ra, 0, sizeof(ra));
+      ra.img = (const float*)image;
+      ra.ker = (const float*)kernel;
+      ra.scale = scale;
+      ra.out = ws;
+      ra.tw = twiddles_for(Lx);
+      ra.img_plane = in_plane;
+      ra.ker_plane = k_plane;
+      ra.out_plane = wplane;
+      ra.height = d->height;
+      ra.width = d->width;
+      ra.k_height = d->k_height;
+      ra.k_width = d->k_width;
+      ra.k_planes = d->k_planes;
+      ra.rows = q.rz;
+      ra.plane0 = p0;
+      ra.n_rows = kc * q.rz;
+      if ((rc = launch_pair(Lx, true, &ra, nullptr, s))) return rc;
+      PairColArgs ca;
+      memset(&ca, 0, sizeof(ca));
+      ca.buf = ws;
+      ca.tw = twiddles_for(Ly);
+      ca.scale = scale;
+      ca.plane_stride = wplane;
+      ca.lx = Lx;
+      ca.in_len = q.rz;
+      ca.out_off = g.y.off;
+      ca.out_len = g.y.keep;
+      ca.n_planes = kc;
+      ca.plane0 = p0;
+      ca.norm = (float)(1.0 / ((double)Ly * (double)Lx));
+      if ((rc = launch_pair(Ly, false, nullptr, &ca, s))) return rc;
+      RowArgs r;
+      memset(&r, 0, sizeof(r));
+      r.in = ws;
+      r.tw = twiddles_for(Lx);
+      r.in_plane_stride = wplane;
+      r.in_row_stride = Lx;
+      r.out_plane_stride = out_plane;
+      r.out_row_stride = g.out_w;
+      r.n_rows = kc * g.y.keep;
+      r.rows_per_plane = g.y.keep;
+      r.in_len = Lx;
+      r.out_off = g.x.off;
+      r.out_len = g.x.keep;
+      r.scale = 1.0f;
+      r.intensity = 2;
+      r.nt_out = 1;
+      r.out_i = (float*)out + (size_t)p0 * out_plane;
+      if ((rc = launch_row(Lx, 1, r, s))) return rc;
+    }
+    return CXM_OK;
+  }
   const int Ld = g.d.L, Ly = g.y.L, Lx = g.x.L;
   const long long lplane = (long long)Ly * Lx;
   const size_t esz = d->real_input ? sizeof(float) : sizeof(float2);

@@ -674,6 +674,12 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         (measured +11% on 64 x 1024^2); 1 = no hints;
  *                         2 = streaming stores only; 3 = streaming loads
  *                         only.  Same results.
+ *   CXM_OPT_FCONV_PAIR    cxm_fconv_c64 with real input over two axes: 0 = the
+ *                         image and kernel planes transformed together as one
+ *                         complex plane (kernel scaled by a power of two to the
+ *                         image's norm) and their spectra separated at the
+ *                         product (even row lengths), 1 = separate kernel
+ *                         spectra.  Equal to float32 rounding.
  *   CXM_OPT_MBS_XGREEN    modified Born series: 0 = the x-axis FFT, the
  *                         Green's projection and the inverse x-axis FFT run
  *                         as one pass where the x length has one; 2 = as
@@ -689,6 +695,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_ROW_LOOP 8
 #define CXM_OPT_MBS_XGREEN 9
 #define CXM_OPT_NT_IO 10
+#define CXM_OPT_FCONV_PAIR 11
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

@@ -237,3 +237,28 @@ def test_one_axis(mode, cplx):
     ref = oc.fourier_convolution(b, kk, axes=(-1,), mode=mode)
     assert out.shape == ref.shape
     assert rel_l2(out, ref) < TOL
+
+
+@pytest.mark.parametrize("scale_k", [1.0, 1e-7, 3e5])
+@pytest.mark.parametrize("shapes", [((3, 96, 80), (3, 17, 33)), ((2, 64, 64), (1, 64, 64)),
+                                    ((1, 5, 3), (1, 2, 3))])
+def test_paired_real_path_matches_separate_spectra(scale_k, shapes):
+    # real image and kernel transformed as one complex plane (the kernel
+    # scaled by a power of two to the image's norm) against the path with
+    # separate kernel spectra (CXM_OPT_FCONV_PAIR 1), and both against FP64,
+    # with kernels many orders of magnitude smaller or larger than the image
+    from chromatix_amd import _lib as L
+
+    rng = np.random.default_rng(11)
+    a = _arr(rng, shapes[0])
+    k = (_arr(rng, shapes[1]) * scale_k).astype(np.float32)
+    ref = oc.fourier_convolution(a, k, axes=(-2, -1))
+    outs = []
+    try:
+        for opt in (0, 1):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_FCONV_PAIR, opt))
+            outs.append(_gpu(a, k, axes=(-2, -1)))
+    finally:
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_FCONV_PAIR, 0))
+    for o in outs:
+        assert rel_l2(o, ref) < TOL
