@@ -150,37 +150,80 @@ __global__ void pair_scale_kernel(const double* __restrict__ pa, const double* _
   scale[b] = s;
 }
 
+// Pair-tiled planes.  Slot 2 j + side of a row holds column j (side 0) and
+// column L - j (side 1) for pair j > 0; pair 0 is columns 0 and L / 2.  The
+// slots go in strips of kTileW: element (y, slot) at
+// (slot / 4) * strip_stride + y * 4 + slot % 4, so a column pass over the
+// slots of a strip reads and writes one contiguous block, and the row
+// passes move 4 rows per workgroup (whole 128-byte lines of 4 rows x 4
+// slots) through LDS.
+CXM_DEV int slot_col(int slot, int L) {
+  const int j = slot >> 1;
+  if (j == 0) return (slot & 1) ? L / 2 : 0;
+  return (slot & 1) ? L - j : j;
+}
+
+// 4 rows of N-point row transforms per workgroup fit the LDS
+template <int N, int E>
+constexpr bool pair_rows_fit() {
+  return 4 * (N / E) <= 1024 && 4 * RowCfg<N, E>::PITCH * (int)sizeof(float2) <= 160 * 1024;
+}
+
+// LDS rows (natural column order) <-> the pair tiles of 4 rows [r0, r0+4):
+// piece c = 16 bytes = row (c % 8) / 2, slots 4 (c / 8) + 2 (c % 2) + {0, 1}
+// (columns j and its mirror of pair j = 2 (c / 8) + c % 2).
+template <int N, int PITCH, bool TO_TILES>
+CXM_DEV void pair_tiles_exchange(float2* lds, float2* base, long long strip_stride, int nthreads) {
+  constexpr int PIECES = (N + 3) / 4 * 8;
+  for (int c = threadIdx.x; c < PIECES; c += nthreads) {
+    const int q = c & 7, j = 2 * (c >> 3) + (q & 1);
+    if (j >= N / 2) continue;  // the second half of the last strip when N % 4 == 2
+    const int x0 = j == 0 ? 0 : j, x1 = j == 0 ? N / 2 : N - j;
+    float2* d = lds + (q >> 1) * PITCH;
+    float4* g = (float4*)(base + (c >> 3) * strip_stride + 2 * q);
+    if constexpr (TO_TILES) {
+      const float2 a = d[lds_at<N, 1>(x0)], b = d[lds_at<N, 1>(x1)];
+      *g = make_float4(a.x, a.y, b.x, b.y);
+    } else {
+      const float4 w = *g;
+      d[lds_at<N, 1>(x0)] = make_float2(w.x, w.y);
+      d[lds_at<N, 1>(x1)] = make_float2(w.z, w.w);
+    }
+  }
+}
+
 struct PairRowArgs {
   const float* img;          // batch x height x width
   const float* ker;          // k_planes x k_height x k_width
   const float* scale;        // per image plane
-  float2* out;               // planes of rows x L
+  float2* out;               // pair-tiled planes of `rows` rows
   const float2* tw;
-  long long img_plane, ker_plane, out_plane;
+  long long img_plane, ker_plane, out_plane, strip_stride;
   int height, width, k_height, k_width, k_planes;
-  int rows;                  // rows per plane = max(height, k_height)
-  int plane0, n_rows;        // first image plane, rows in this launch
+  int rows;                  // rows per plane (a multiple of 4, >= height, k_height)
+  int plane0, n_rows;        // first image plane, rows in this launch (a multiple of 4)
 };
 
+// Forward row transforms of z = a + i s k, 4 rows per workgroup, stored
+// pair-tiled.
 template <int N, int E>
-__global__ void __launch_bounds__((RowCfg<N, E>::THREADS))
+__global__ void __launch_bounds__(4 * (N / E))
 pair_row_kernel(PairRowArgs a) {
-  using Cfg = RowCfg<N, E>;
-  constexpr int T = Cfg::T;
-  constexpr int G = Cfg::G;
+  constexpr int T = N / E;
+  constexpr int PITCH = RowCfg<N, E>::PITCH;
   constexpr bool SYNC = T > 64;
-  __shared__ float2 lds[G * Cfg::PITCH];
+  __shared__ float2 lds[4 * PITCH];
   const int g = threadIdx.x / T;
   const int t = threadIdx.x - g * T;
-  const int row = blockIdx.x * G + g;
-  const bool ok = row < a.n_rows;
-  const int p = ok ? row / a.rows : 0;
-  const int r = ok ? row - p * a.rows : 0;
+  const int row0 = blockIdx.x * 4;
+  const int p = row0 / a.rows;
+  const int r = row0 - p * a.rows + g;
   const int b = a.plane0 + p;
   const float s = a.scale[b];
   const float* ir = a.img + (long long)b * a.img_plane + (long long)r * a.width;
   const float* kr = a.ker + (long long)(b % a.k_planes) * a.ker_plane + (long long)r * a.k_width;
-  const bool iok = ok && r < a.height, kok = ok && r < a.k_height;
+  const bool iok = r < a.height, kok = r < a.k_height;
+  float2* st = lds + g * PITCH;
   float2 v[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) {
@@ -189,51 +232,48 @@ pair_row_kernel(PairRowArgs a) {
     const float im = (kok && x < a.k_width) ? kr[x] * s : 0.f;
     v[k] = make_float2(re, im);
   }
-  fft<N, E, -1, 1, SYNC>(v, lds + g * Cfg::PITCH, t, a.tw);
-  if (!ok) return;
-  float2* o = a.out + (long long)p * a.out_plane + (long long)r * N;
+  fft<N, E, -1, 1, SYNC>(v, st, t, a.tw);
+  __syncthreads();  // every wave of the row has read the transform's last exchange
 #pragma unroll
-  for (int k = 0; k < E; ++k) o[t + k * T] = v[k];
+  for (int k = 0; k < E; ++k) st[lds_at_split<N, 1>(t, k * T)] = v[k];
+  __syncthreads();
+  pair_tiles_exchange<N, PITCH, true>(lds, a.out + p * a.out_plane + (long long)(row0 - p * a.rows) * kTileW,
+                                      a.strip_stride, 4 * T);
 }
 
 struct PairColArgs {
-  float2* buf;               // planes of L_y x L_x, in place
+  float2* buf;               // pair-tiled planes, in place
   const float2* tw;
   const float* scale;        // per image plane
-  long long plane_stride;
-  int lx;                    // columns (even)
+  long long plane_stride, strip_stride;
+  int lx;                    // columns (even) = slots
   int in_len;                // rows holding data
   int out_off, out_len;      // rows kept
   int n_planes, plane0;
   float norm;                // 1 / (L_y L_x)
 };
 
-// C slots per workgroup: slot s < C/2 holds column first(j), slot s + C/2
-// column second(j) of pair j = j0 + s; pair 0 is (0, L/2), each its own
-// mirror, pair j > 0 is (j, L - j).
+// C consecutive slots per workgroup (C even: both columns of each pair);
+// slot 2 j + side pairs with its neighbour, pair 0's columns with themselves.
 template <int N, int E, int C>
 __global__ void __launch_bounds__((N / E) * C)
 pair_col_kernel(PairColArgs a) {
   static_assert(C % 2 == 0, "column pairs");
   constexpr int T = N / E;
-  constexpr int H = C / 2;
   __shared__ float2 lds[lds_span<N, C>()];
   const int c = threadIdx.x % C;
   const int ty = threadIdx.x / C;
-  const int npairs = a.lx / 2;
-  const int nblk = (npairs + H - 1) / H;
+  const int nblk = (a.lx + C - 1) / C;
   const int p = blockIdx.x / nblk;
-  const int pair = (blockIdx.x - p * nblk) * H + (c % H);
-  const bool second = c >= H;
-  const bool valid = pair < npairs;
-  const int col = !valid ? 0 : (pair == 0 ? (second ? a.lx / 2 : 0) : (second ? a.lx - pair : pair));
-  const int partner = (pair == 0) ? c : (second ? c - H : c + H);
-  float2* colp = a.buf + (long long)p * a.plane_stride + col;
+  const int slot = (blockIdx.x - p * nblk) * C + c;
+  const bool valid = slot < a.lx;
+  const int partner = (slot >> 1) == 0 ? c : (c ^ 1);
+  float2* colp = a.buf + (long long)p * a.plane_stride + (long long)(slot >> 2) * a.strip_stride + (slot & 3);
   float2 u[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int i = ty + k * T;
-    u[k] = (valid && i < a.in_len) ? colp[(long long)i * a.lx] : make_float2(0.f, 0.f);
+    u[k] = (valid && i < a.in_len) ? colp[(long long)i * kTileW] : make_float2(0.f, 0.f);
   }
   fft<N, E, -1, C, true>(u, lds + c, ty, a.tw);
   __syncthreads();  // the transform's last exchange has been read by every thread
@@ -258,33 +298,80 @@ pair_col_kernel(PairColArgs a) {
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int yl = ty + k * T - a.out_off;
-    if (yl >= 0 && yl < a.out_len) colp[(long long)yl * a.lx] = u[k];
+    if (yl >= 0 && yl < a.out_len) colp[(long long)yl * kTileW] = u[k];
   }
 }
 
+struct PairInvArgs {
+  const float2* in;          // pair-tiled planes of `rows` rows
+  float* out;                // out_len-wide rows, `keep` rows per plane
+  const float2* tw;
+  long long in_plane, strip_stride, out_plane;
+  int rows;                  // rows per workspace plane group (a multiple of 4)
+  int keep;                  // rows stored per plane
+  int out_w, out_off, out_len;
+  int n_rows;                // planes in this launch x rows (a multiple of 4)
+};
+
+// Inverse row transforms of pair-tiled rows, 4 rows per workgroup: the real
+// part of columns [out_off, out_off + out_len), streaming stores.
+template <int N, int E>
+__global__ void __launch_bounds__(4 * (N / E))
+pair_inv_kernel(PairInvArgs a) {
+  constexpr int T = N / E;
+  constexpr int PITCH = RowCfg<N, E>::PITCH;
+  constexpr bool SYNC = T > 64;
+  __shared__ float2 lds[4 * PITCH];
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x - g * T;
+  const int row0 = blockIdx.x * 4;
+  const int p = row0 / a.rows;
+  const int r = row0 - p * a.rows + g;
+  pair_tiles_exchange<N, PITCH, false>(lds, (float2*)a.in + p * a.in_plane + (long long)(row0 - p * a.rows) * kTileW,
+                                       a.strip_stride, 4 * T);
+  __syncthreads();
+  float2* st = lds + g * PITCH;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) v[k] = st[lds_at_split<N, 1>(t, k * T)];
+  fft<N, E, 1, 1, SYNC>(v, st, t, a.tw);
+  if (r >= a.keep) return;
+  float* o = a.out + p * a.out_plane + (long long)r * a.out_w - a.out_off;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int x = t + k * T;
+    if (x >= a.out_off && x < a.out_off + a.out_len) __builtin_nontemporal_store(v[k].x, o + x);
+  }
+}
+
+enum PairPass { PAIR_ROW, PAIR_COL, PAIR_INV };
+
 #define CXM_PAIR_LAUNCH(N, ER, EC, C)                                                           \
   case N: {                                                                                     \
-    if (row) {                                                                                  \
-      using Cfg = RowCfg<N, ER>;                                                                \
-      const long long blocks = ((long long)ra->n_rows + Cfg::G - 1) / Cfg::G;                   \
-      hipLaunchKernelGGL((pair_row_kernel<N, ER>), dim3((unsigned)blocks), dim3(Cfg::THREADS), 0, \
-                         s, *ra);                                                               \
+    if (pass == PAIR_COL) {                                                                     \
+      if constexpr ((C) % 2 == 0) {                                                             \
+        const long long blocks = (long long)ca->n_planes * ((ca->lx + (C) - 1) / (C));          \
+        hipLaunchKernelGGL((pair_col_kernel<N, EC, C>), dim3((unsigned)blocks),                 \
+                           dim3((N / (EC)) * (C)), 0, s, *ca);                                  \
+        CXM_LAUNCH_CHECK();                                                                     \
+        return CXM_OK;                                                                          \
+      }                                                                                         \
+    } else if constexpr (pair_rows_fit<N, ER>() && N % 2 == 0) {                                \
+      if (pass == PAIR_ROW) {                                                                   \
+        hipLaunchKernelGGL((pair_row_kernel<N, ER>), dim3((unsigned)(ra->n_rows / 4)),          \
+                           dim3(4 * (N / (ER))), 0, s, *ra);                                    \
+      } else {                                                                                  \
+        hipLaunchKernelGGL((pair_inv_kernel<N, ER>), dim3((unsigned)(ia->n_rows / 4)),          \
+                           dim3(4 * (N / (ER))), 0, s, *ia);                                    \
+      }                                                                                         \
       CXM_LAUNCH_CHECK();                                                                       \
       return CXM_OK;                                                                            \
     }                                                                                           \
-    if constexpr ((C) % 2 == 0) {                                                               \
-      const long long blocks = (long long)ca->n_planes * ((ca->lx / 2 + (C) / 2 - 1) / ((C) / 2)); \
-      hipLaunchKernelGGL((pair_col_kernel<N, EC, C>), dim3((unsigned)blocks),                   \
-                         dim3((N / (EC)) * (C)), 0, s, *ca);                                    \
-      CXM_LAUNCH_CHECK();                                                                       \
-      return CXM_OK;                                                                            \
-    } else {                                                                                    \
-      return fail(CXM_ERR_UNSUPPORTED, "no paired column pass for length " + std::to_string(N)); \
-    }                                                                                           \
+    return fail(CXM_ERR_UNSUPPORTED, "no paired pass for length " + std::to_string(N));         \
   }
 
-// row: the paired row pass of length n (ra); else the paired column pass (ca)
-int launch_pair(int n, bool row, const PairRowArgs* ra, const PairColArgs* ca, hipStream_t s) {
+int launch_pair(int n, PairPass pass, const PairRowArgs* ra, const PairColArgs* ca,
+                const PairInvArgs* ia, hipStream_t s) {
   switch (n) {
     CXM_FOR_EACH_SIZE(CXM_PAIR_LAUNCH)
     default:
@@ -293,15 +380,16 @@ int launch_pair(int n, bool row, const PairRowArgs* ra, const PairColArgs* ca, h
 }
 #undef CXM_PAIR_LAUNCH
 
-// paired passes need an even row length (column pairs kx, -kx) and an even
-// number of columns per column-pass workgroup at the column length
+// paired passes need an even row length whose 4-row transforms fit the LDS
+// and an even number of columns per column-pass workgroup at the column length
 bool pair_supported(int ly, int lx) {
-  bool ok_y = false;
-#define CXM_PAIR_OK(N, ER, EC, C) \
-  if (N == ly && (C) % 2 == 0) ok_y = true;
+  bool ok_y = false, ok_x = false;
+#define CXM_PAIR_OK(N, ER, EC, C)                     \
+  if (N == ly && (C) % 2 == 0) ok_y = true;           \
+  if (N == lx && N % 2 == 0 && pair_rows_fit<N, ER>()) ok_x = true;
   CXM_FOR_EACH_SIZE(CXM_PAIR_OK)
 #undef CXM_PAIR_OK
-  return ok_y && lx % 2 == 0;
+  return ok_y && ok_x;
 }
 
 // Row FFT of `planes` input planes of `rows` x `cols` (float32 or complex64,
@@ -365,16 +453,22 @@ bool use_pair(const cxm_fconv_desc* d, const FcGeom& g) {
 }
 struct PairGeom {
   int rz;         // rows holding data: max(height, k_height)
+  int rows_f;     // rows per plane of the forward row pass (rz rounded up to 4)
+  int rows_i;     // rows per plane of the inverse row pass (kept rows rounded up to 4)
   int rws;        // rows per workspace plane
+  long long strip, plane;  // elements per strip and per workspace plane
   int chunk;
   size_t ws_off;  // bytes before the planes: scales and partial sums
 };
 PairGeom pair_geom(const cxm_fconv_desc* d, const FcGeom& g) {
   PairGeom q;
   q.rz = std::max(d->height, d->k_height);
-  q.rws = std::max(q.rz, g.y.keep);
-  const long long per = (long long)q.rws * g.x.L;
-  q.chunk = (int)std::max(1LL, std::min<long long>(std::max(1, d->batch), ((128LL << 20) / 8) / per));
+  q.rows_f = (q.rz + 3) / 4 * 4;
+  q.rows_i = (g.y.keep + 3) / 4 * 4;
+  q.rws = std::max(q.rows_f, q.rows_i);
+  q.strip = (long long)q.rws * kTileW;
+  q.plane = q.strip * ((g.x.L + kTileW - 1) / kTileW);
+  q.chunk = (int)std::max(1LL, std::min<long long>(std::max(1, d->batch), ((128LL << 20) / 8) / q.plane));
   const size_t scales = ((size_t)std::max(1, d->batch) * sizeof(float) + 255) / 256 * 256;
   const size_t partial = (size_t)(std::max(1, d->batch) + d->k_planes) * kSsBlocks * sizeof(double);
   q.ws_off = scales + (partial + 255) / 256 * 256;
@@ -382,7 +476,7 @@ PairGeom pair_geom(const cxm_fconv_desc* d, const FcGeom& g) {
 }
 size_t pair_ws(const cxm_fconv_desc* d, const FcGeom& g) {
   const PairGeom q = pair_geom(d, g);
-  return q.ws_off + (size_t)q.chunk * q.rws * g.x.L * sizeof(float2);
+  return q.ws_off + (size_t)q.chunk * q.plane * sizeof(float2);
 }
 }  // namespace
 }  // namespace cxm
@@ -425,7 +519,6 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
                        (const double*)pa, (const double*)pk, d->batch, d->k_planes, scale);
     CXM_LAUNCH_CHECK();
     const int Ly = g.y.L, Lx = g.x.L;
-    const long long wplane = (long long)q.rws * Lx;
     const long long out_plane = (long long)g.out_h * g.out_w;
     for (int p0 = 0; p0 < d->batch; p0 += q.chunk) {
       const int kc = std::min(q.chunk, d->batch - p0);
@@ -438,22 +531,24 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
       ra.tw = twiddles_for(Lx);
       ra.img_plane = in_plane;
       ra.ker_plane = k_plane;
-      ra.out_plane = wplane;
+      ra.out_plane = q.plane;
+      ra.strip_stride = q.strip;
       ra.height = d->height;
       ra.width = d->width;
       ra.k_height = d->k_height;
       ra.k_width = d->k_width;
       ra.k_planes = d->k_planes;
-      ra.rows = q.rz;
+      ra.rows = q.rows_f;
       ra.plane0 = p0;
-      ra.n_rows = kc * q.rz;
-      if ((rc = launch_pair(Lx, true, &ra, nullptr, s))) return rc;
+      ra.n_rows = kc * q.rows_f;
+      if ((rc = launch_pair(Lx, PAIR_ROW, &ra, nullptr, nullptr, s))) return rc;
       PairColArgs ca;
       memset(&ca, 0, sizeof(ca));
       ca.buf = ws;
       ca.tw = twiddles_for(Ly);
       ca.scale = scale;
-      ca.plane_stride = wplane;
+      ca.plane_stride = q.plane;
+      ca.strip_stride = q.strip;
       ca.lx = Lx;
       ca.in_len = q.rz;
       ca.out_off = g.y.off;
@@ -461,25 +556,22 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
       ca.n_planes = kc;
       ca.plane0 = p0;
       ca.norm = (float)(1.0 / ((double)Ly * (double)Lx));
-      if ((rc = launch_pair(Ly, false, nullptr, &ca, s))) return rc;
-      RowArgs r;
-      memset(&r, 0, sizeof(r));
-      r.in = ws;
-      r.tw = twiddles_for(Lx);
-      r.in_plane_stride = wplane;
-      r.in_row_stride = Lx;
-      r.out_plane_stride = out_plane;
-      r.out_row_stride = g.out_w;
-      r.n_rows = kc * g.y.keep;
-      r.rows_per_plane = g.y.keep;
-      r.in_len = Lx;
-      r.out_off = g.x.off;
-      r.out_len = g.x.keep;
-      r.scale = 1.0f;
-      r.intensity = 2;
-      r.nt_out = 1;
-      r.out_i = (float*)out + (size_t)p0 * out_plane;
-      if ((rc = launch_row(Lx, 1, r, s))) return rc;
+      if ((rc = launch_pair(Ly, PAIR_COL, nullptr, &ca, nullptr, s))) return rc;
+      PairInvArgs ia;
+      memset(&ia, 0, sizeof(ia));
+      ia.in = ws;
+      ia.out = (float*)out + (size_t)p0 * out_plane;
+      ia.tw = twiddles_for(Lx);
+      ia.in_plane = q.plane;
+      ia.strip_stride = q.strip;
+      ia.out_plane = out_plane;
+      ia.rows = q.rows_i;
+      ia.keep = g.y.keep;
+      ia.out_w = g.out_w;
+      ia.out_off = g.x.off;
+      ia.out_len = g.x.keep;
+      ia.n_rows = kc * q.rows_i;
+      if ((rc = launch_pair(Lx, PAIR_INV, nullptr, nullptr, &ia, s))) return rc;
     }
     return CXM_OK;
   }
