@@ -603,12 +603,14 @@ def bench_c9(args):
     img_step = lambda: mic.image(volume, psf)
     ms_img = _timed(torch, img_step, steps, warm)
     kern = _kernels(torch, L, step, 1)
-    # fourier_convolution per plane (H = W = 1536, L = 3072 transform):
-    # kernel spectrum (row pass 4HW + 8HL, column pass 8HL + 8LL), image
-    # (row 4HW + 8HL, column 8HL + spectrum 8LL + 8HL, inverse row 8HL + 4HW)
-    # and the plane sum (4HW)
+    # the planewise convolution summed over the planes (H = W = 1536 sample
+    # and PSF planes, 3072-point transforms, 1536-row planes of 3072 column
+    # slots in the workspace), per plane: the row pass reads the sample and
+    # PSF planes (4HW + 4HW) and writes 8HL; the column pass reads 8HL and
+    # reads and writes the running sum (8HL + 8HL); once per image the
+    # inverse row pass reads the sum (8HL) and writes the image (4HW)
     Lc = 3072
-    conv_bytes = D * (16 * N * N + 48 * N * Lc + 16 * Lc * Lc)
+    conv_bytes = D * (8 * N * N + 32 * N * Lc) + 8 * N * Lc + 4 * N * N
     return _result(args, "Microscope images/s (128 x 1536^2 4f PSF + planewise Fourier convolution "
                          "+ plane sum)", 1.0 / (ms * 1e-3), "images/s", ms, steps, warm, "float32",
                    f"Microscope(Optical4FSystemPSF {N}^2, {D} planes) on a {D}x{N}^2 volume, "

@@ -241,6 +241,7 @@ class FconvDesc(ctypes.Structure):
         ("out_depth", ctypes.c_int32),
         ("out_height", ctypes.c_int32),
         ("out_width", ctypes.c_int32),
+        ("sum_batch", ctypes.c_int32),
     ]
 
 

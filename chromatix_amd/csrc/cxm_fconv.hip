@@ -78,6 +78,7 @@ int fc_geom(const cxm_fconv_desc* d, FcGeom& g) {
   if (d->mode == 1 && (d->out_depth <= 0 || d->out_height <= 0 || d->out_width <= 0))
     return fail(CXM_ERR_INVALID, "invalid full-mode output shape");
   g.vol = !(d->depth == 1 && d->k_depth == 1);
+  if (d->sum_batch != 0 && d->sum_batch != 1) return fail(CXM_ERR_INVALID, "sum_batch must be 0 or 1");
   int rc;
   if ((rc = axis_geom(d->depth, d->k_depth, d->mode, d->out_depth, g.vol, g.d))) return rc;
   if ((rc = axis_geom(d->height, d->k_height, d->mode, d->out_height, true, g.y))) return rc;
@@ -251,25 +252,16 @@ struct PairColArgs {
   int out_off, out_len;      // rows kept
   int n_planes, plane0;
   float norm;                // 1 / (L_y L_x)
+  float2* acc;               // sum mode: pair-tiled plane of the kept rows
+  int first;                 // sum mode: acc is written, not added to
 };
 
-// C consecutive slots per workgroup (C even: both columns of each pair);
-// slot 2 j + side pairs with its neighbour, pair 0's columns with themselves.
+// Column pass of one plane p: forward transform of the slot's column, the
+// product A K of the pair, inverse transform, in u.
 template <int N, int E, int C>
-__global__ void __launch_bounds__((N / E) * C)
-pair_col_kernel(PairColArgs a) {
-  static_assert(C % 2 == 0, "column pairs");
+CXM_DEV void pair_col_plane(const PairColArgs& a, int p, const float2* colp, bool valid, int c,
+                            int ty, int partner, float2* lds, float2 (&u)[E]) {
   constexpr int T = N / E;
-  __shared__ float2 lds[lds_span<N, C>()];
-  const int c = threadIdx.x % C;
-  const int ty = threadIdx.x / C;
-  const int nblk = (a.lx + C - 1) / C;
-  const int p = blockIdx.x / nblk;
-  const int slot = (blockIdx.x - p * nblk) * C + c;
-  const bool valid = slot < a.lx;
-  const int partner = (slot >> 1) == 0 ? c : (c ^ 1);
-  float2* colp = a.buf + (long long)p * a.plane_stride + (long long)(slot >> 2) * a.strip_stride + (slot & 3);
-  float2 u[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int i = ty + k * T;
@@ -294,11 +286,47 @@ pair_col_kernel(PairColArgs a) {
   }
   __syncthreads();  // the partner values are read before the inverse transform writes LDS
   fft<N, E, 1, C, true>(u, lds + c, ty, a.tw);
+}
+
+// C consecutive slots per workgroup (C even: both columns of each pair);
+// slot 2 j + side pairs with its neighbour, pair 0's columns with themselves.
+// One workgroup per (plane, C slots), the kept rows stored in place; SUM:
+// one plane per launch (n_planes 1), its kept rows written to acc (first)
+// or added to it, so that launches in plane order sum the planes in order.
+template <int N, int E, int C, bool SUM>
+__global__ void __launch_bounds__((N / E) * C)
+pair_col_kernel(PairColArgs a) {
+  static_assert(C % 2 == 0, "column pairs");
+  constexpr int T = N / E;
+  __shared__ float2 lds[lds_span<N, C>()];
+  const int c = threadIdx.x % C;
+  const int ty = threadIdx.x / C;
+  const int nblk = (a.lx + C - 1) / C;
+  const int slot = (SUM ? blockIdx.x : blockIdx.x % nblk) * C + c;
+  const bool valid = slot < a.lx;
+  const int partner = (slot >> 1) == 0 ? c : (c ^ 1);
+  const long long soff = (long long)(slot >> 2) * a.strip_stride + (slot & 3);
+  float2 u[E];
+  const int p = SUM ? 0 : blockIdx.x / nblk;
+  float2* colp = a.buf + (long long)p * a.plane_stride + soff;
+  pair_col_plane<N, E, C>(a, p, colp, valid, c, ty, partner, lds, u);
   if (!valid) return;
+  if constexpr (SUM) {
+    float2* accp = a.acc + soff;
 #pragma unroll
-  for (int k = 0; k < E; ++k) {
-    const int yl = ty + k * T - a.out_off;
-    if (yl >= 0 && yl < a.out_len) colp[(long long)yl * kTileW] = u[k];
+    for (int k = 0; k < E; ++k) {
+      const int yl = ty + k * T - a.out_off;
+      if (yl >= 0 && yl < a.out_len) {
+        float2* q = accp + (long long)yl * kTileW;
+        *q = a.first ? u[k] : make_float2(q->x + u[k].x, q->y + u[k].y);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int yl = ty + k * T - a.out_off;
+      if (yl >= 0 && yl < a.out_len) colp[(long long)yl * kTileW] = u[k];
+    }
   }
 }
 
@@ -350,9 +378,14 @@ enum PairPass { PAIR_ROW, PAIR_COL, PAIR_INV };
   case N: {                                                                                     \
     if (pass == PAIR_COL) {                                                                     \
       if constexpr ((C) % 2 == 0) {                                                             \
-        const long long blocks = (long long)ca->n_planes * ((ca->lx + (C) - 1) / (C));          \
-        hipLaunchKernelGGL((pair_col_kernel<N, EC, C>), dim3((unsigned)blocks),                 \
-                           dim3((N / (EC)) * (C)), 0, s, *ca);                                  \
+        const long long nblk = (ca->lx + (C) - 1) / (C);                                        \
+        if (ca->acc)                                                                            \
+          hipLaunchKernelGGL((pair_col_kernel<N, EC, C, true>), dim3((unsigned)nblk),           \
+                             dim3((N / (EC)) * (C)), 0, s, *ca);                                \
+        else                                                                                    \
+          hipLaunchKernelGGL((pair_col_kernel<N, EC, C, false>),                                \
+                             dim3((unsigned)(ca->n_planes * nblk)), dim3((N / (EC)) * (C)), 0, s, \
+                             *ca);                                                              \
         CXM_LAUNCH_CHECK();                                                                     \
         return CXM_OK;                                                                          \
       }                                                                                         \
@@ -476,14 +509,22 @@ PairGeom pair_geom(const cxm_fconv_desc* d, const FcGeom& g) {
 }
 size_t pair_ws(const cxm_fconv_desc* d, const FcGeom& g) {
   const PairGeom q = pair_geom(d, g);
-  return q.ws_off + (size_t)q.chunk * q.plane * sizeof(float2);
+  return q.ws_off + (size_t)(q.chunk + (d->sum_batch ? 1 : 0)) * q.plane * sizeof(float2);
+}
+// the batch sum runs on the paired path only
+int check_sum(const cxm_fconv_desc* d, const FcGeom& g) {
+  if (d->sum_batch && !use_pair(d, g))
+    return fail(CXM_ERR_UNSUPPORTED,
+                "sum_batch needs real two-axis input with paired transform lengths (and the paired "
+                "path enabled)");
+  return CXM_OK;
 }
 }  // namespace
 }  // namespace cxm
 
 extern "C" size_t cxm_fconv_workspace_size(const cxm_fconv_desc* d) {
   FcGeom g;
-  if (fc_geom(d, g)) return 0;
+  if (fc_geom(d, g) || check_sum(d, g)) return 0;
   return use_pair(d, g) ? pair_ws(d, g) : fc_ws(d, g);
 }
 
@@ -491,7 +532,7 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
                              void* workspace, size_t workspace_bytes, const cxm_fconv_desc* d) {
   FcGeom g;
   int rc = fc_geom(d, g);
-  if (rc) return rc;
+  if (rc || (rc = check_sum(d, g))) return rc;
   if (d->batch == 0) return CXM_OK;
   if (!image || !kernel || !out) return fail(CXM_ERR_INVALID, "null buffer");
   const bool pair = use_pair(d, g);
@@ -507,6 +548,7 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
                            ((size_t)d->batch * sizeof(float) + 255) / 256 * 256);
     double* pk = pa + (size_t)d->batch * kSsBlocks;
     float2* ws = (float2*)((char*)workspace + q.ws_off);
+    float2* acc = ws + (size_t)q.chunk * q.plane;  // sum_batch: the summed kept rows
     const long long in_plane = (long long)d->height * d->width;
     const long long k_plane = (long long)d->k_height * d->k_width;
     hipLaunchKernelGGL(plane_ss_kernel, dim3(kSsBlocks, d->batch), dim3(256), 0, s, (const float*)image,
@@ -556,6 +598,18 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
       ca.n_planes = kc;
       ca.plane0 = p0;
       ca.norm = (float)(1.0 / ((double)Ly * (double)Lx));
+      if (d->sum_batch) {
+        // one launch per plane, in plane order (the sum's order)
+        ca.acc = acc;
+        ca.n_planes = 1;
+        for (int p = 0; p < kc; ++p) {
+          ca.buf = ws + p * q.plane;
+          ca.plane0 = p0 + p;
+          ca.first = p0 + p == 0;
+          if ((rc = launch_pair(Ly, PAIR_COL, nullptr, &ca, nullptr, s))) return rc;
+        }
+        continue;
+      }
       if ((rc = launch_pair(Ly, PAIR_COL, nullptr, &ca, nullptr, s))) return rc;
       PairInvArgs ia;
       memset(&ia, 0, sizeof(ia));
@@ -571,6 +625,23 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
       ia.out_off = g.x.off;
       ia.out_len = g.x.keep;
       ia.n_rows = kc * q.rows_i;
+      if ((rc = launch_pair(Lx, PAIR_INV, nullptr, nullptr, &ia, s))) return rc;
+    }
+    if (d->sum_batch) {
+      PairInvArgs ia;
+      memset(&ia, 0, sizeof(ia));
+      ia.in = acc;
+      ia.out = (float*)out;
+      ia.tw = twiddles_for(Lx);
+      ia.in_plane = q.plane;
+      ia.strip_stride = q.strip;
+      ia.out_plane = out_plane;
+      ia.rows = q.rows_i;
+      ia.keep = g.y.keep;
+      ia.out_w = g.out_w;
+      ia.out_off = g.x.off;
+      ia.out_len = g.x.keep;
+      ia.n_rows = q.rows_i;
       if ((rc = launch_pair(Lx, PAIR_INV, nullptr, nullptr, &ia, s))) return rc;
     }
     return CXM_OK;

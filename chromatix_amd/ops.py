@@ -22,7 +22,8 @@ import torch
 
 from . import _lib as L
 
-__all__ = ["fourier_convolution", "approximate_shot_noise", "shot_noise", "next_order"]
+__all__ = ["fourier_convolution", "planewise_convolution_sum", "approximate_shot_noise",
+           "shot_noise", "next_order"]
 
 
 def next_order(val: int) -> int:
@@ -139,6 +140,41 @@ def _generator(key, device) -> torch.Generator:
     g.manual_seed(int(key))
     return g
 
+
+def planewise_convolution_sum(image, kernel) -> torch.Tensor | None:
+    """``fourier_convolution(image, kernel, axes=(-2, -1)).sum(0)`` for a real
+    (D, H, W) ``image`` and a real (D, h, w) or (1, h, w) ``kernel`` in one
+    library call (cxm_fconv_c64 with sum_batch): the sum over the planes is
+    taken on the column transforms' results, so one inverse row transform
+    serves every plane (a Microscope's planewise convolution summed by its
+    sensor, src/chromatix/systems/microscopes.py:271-297 and
+    src/chromatix/functional/sensors.py:79-80).  None where the library's
+    summed path does not cover the input (complex values, an odd row
+    transform length); the (H, W) result otherwise."""
+    image = torch.as_tensor(image)
+    L.ensure_device(image)
+    kernel = torch.as_tensor(kernel, device=image.device)
+    if (image.ndim != 3 or kernel.ndim != 3 or image.is_complex() or kernel.is_complex()
+            or kernel.shape[0] not in (1, image.shape[0]) or image.shape[0] == 0):
+        return None
+    img = image.to(torch.float32).contiguous()
+    ker = kernel.to(torch.float32).contiguous()
+    d = L.FconvDesc()
+    d.batch = img.shape[0]
+    d.depth, d.height, d.width = 1, img.shape[1], img.shape[2]
+    d.k_planes = ker.shape[0]
+    d.k_depth, d.k_height, d.k_width = 1, ker.shape[1], ker.shape[2]
+    d.real_input = 1
+    d.sum_batch = 1
+    lib = L.lib()
+    ws_bytes = lib.cxm_fconv_workspace_size(d)
+    if ws_bytes == 0:
+        return None
+    out = torch.empty(tuple(img.shape[1:]), dtype=torch.float32, device=img.device)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=img.device)
+    L.check(lib.cxm_fconv_c64(L.stream_ptr(img.device), img.data_ptr(), ker.data_ptr(),
+                              out.data_ptr(), ws.data_ptr(), ws_bytes, d))
+    return out
 
 def approximate_shot_noise(key, image) -> torch.Tensor:
     """max(image + sqrt(image) * N(0, 1), 0) (noise.py:9-16); ``key`` a

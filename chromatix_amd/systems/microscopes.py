@@ -3,7 +3,9 @@ Optical4FSystemPSF (microscopes.py:300-365) on libcxm.
 
 ``Microscope.image`` convolves the sample with the PSF intensity
 (``ops.fourier_convolution``, one library call) and hands the image to the
-sensor without resampling.  ``Microscope.__call__`` computes the PSF,
+sensor without resampling; when a BasicSensor sums the planes of a (D, H, W)
+sample, the sum is taken inside the convolution
+(``ops.planewise_convolution_sum``).  ``Microscope.__call__`` computes the PSF,
 crops its padding and passes it to the sensor's ``resample``; tapering the
 PSF edges (``taper_width`` > 0) and resampling to a different pitch are not
 implemented.
@@ -18,8 +20,10 @@ import torch
 
 from ..core.field import ScalarField
 from ..core.spectrum import MonoSpectrum, Spectrum
+from ..elements.sensors import BasicSensor
 from ..functional.psf import psf_4f
-from ..ops import fourier_convolution
+from ..functional.sensors import basic_sensor
+from ..ops import fourier_convolution, planewise_convolution_sum
 
 
 class Microscope:
@@ -91,9 +95,26 @@ class Microscope:
         """Image of ``sample`` through ``psf`` (both already at the sensor's
         pixels): Fourier convolution over ``convolution_axes``, then the
         sensor without resampling (microscopes.py:271-297)."""
+        summed = self._summed_image(sample, psf)
+        if summed is not None:
+            return basic_sensor(summed, self.sensor.shot_noise_mode, None, None,
+                                self.sensor.reduce_parallel_axis_name,
+                                input_spacing=self.sensor.spacing, noise_key=key)
         image = fourier_convolution(sample, psf, axes=self.convolution_axes,
                                     fast_fft_shape=self.fast_fft_shape)
         return self.sensor(image, self.sensor.spacing, resample=False, key=key)
+
+    def _summed_image(self, sample, psf):
+        """The planewise convolution of a (D, H, W) sample already summed over
+        the planes (``planewise_convolution_sum``) when a BasicSensor sums
+        axis 0 of it; None otherwise."""
+        sample = torch.as_tensor(sample)
+        axes = tuple(a % sample.ndim for a in self.convolution_axes) if sample.ndim else ()
+        if (not isinstance(self.sensor, BasicSensor) or sample.ndim != 3 or axes != (1, 2)
+                or not isinstance(self.sensor.reduce_axis, int) or self.sensor.reduce_axis % 3 != 0
+                or not sample.is_cuda):
+            return None
+        return planewise_convolution_sum(sample, psf)
 
 
 class Optical4FSystemPSF:

@@ -435,6 +435,11 @@ typedef struct cxm_fconv_desc {
   int32_t real_input;
   int32_t mode;                      /* 0 same, 1 full */
   int32_t out_depth, out_height, out_width;  /* full mode output volume */
+  int32_t sum_batch;                 /* 1: out is one volume, the sum over the batch of
+                                        the convolutions (real two-axis input with an even
+                                        row length only: the sensor's plane sum,
+                                        src/chromatix/functional/sensors.py:79-80, fused
+                                        into the transforms) */
 } cxm_fconv_desc;
 
 size_t cxm_fconv_workspace_size(const cxm_fconv_desc* d);

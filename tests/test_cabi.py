@@ -130,6 +130,15 @@ def test_validation_errors_of_the_imaging_and_born_entries():
     assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_UNSUPPORTED
     d.height, d.mode = 40, 1  # full mode needs its output shape
     assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_INVALID
+    d.mode, d.sum_batch = 0, 2
+    assert lib.cxm_fconv_workspace_size(ctypes.byref(d)) == 0
+    assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_INVALID
+    d.sum_batch = 1  # the batch sum: real two-axis input only
+    assert lib.cxm_fconv_workspace_size(ctypes.byref(d)) > 0
+    d.real_input = 0
+    assert lib.cxm_fconv_workspace_size(ctypes.byref(d)) == 0
+    assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_UNSUPPORTED
+    d.real_input, d.sum_batch = 1, 0
     assert lib.cxm_sum_axis_f32(None, None, None, -1, 1, 1) == L.CXM_ERR_INVALID
     assert lib.cxm_sum_axis_f32(None, None, None, 0, 4, 4) == L.CXM_OK  # nothing to do
     assert lib.cxm_shot_noise_approx_f32(None, None, None, None, -1) == L.CXM_ERR_INVALID
