@@ -374,20 +374,37 @@ pair_inv_kernel(PairInvArgs a) {
 
 enum PairPass { PAIR_ROW, PAIR_COL, PAIR_INV };
 
+// Values per thread of the paired column pass: half the propagation
+// column pass's for lengths 3 * 2^a >= 512 (twice the threads at half the
+// registers each: 13.7 against 14.7 ms for 128 planes of 3072-point
+// transforms), where that stays within 1024 threads; else 0 (the table's).
+template <int N, int EC, int C>
+constexpr int pair_col_half_e() {
+  if constexpr (EC % 2 != 0 || EC / 2 < 2 || N % (EC / 2) != 0 || (N / (EC / 2)) * C > 1024 || N < 512 ||
+                N % 3 != 0)
+    return 0;
+  else
+    return schedule_ok(N, EC / 2) ? EC / 2 : 0;
+}
+
+template <int N, int E, int C>
+int launch_pair_col(const PairColArgs* ca, hipStream_t s) {
+  const long long nblk = (ca->lx + C - 1) / C;
+  if (ca->acc)
+    hipLaunchKernelGGL((pair_col_kernel<N, E, C, true>), dim3((unsigned)nblk), dim3((N / E) * C), 0, s, *ca);
+  else
+    hipLaunchKernelGGL((pair_col_kernel<N, E, C, false>), dim3((unsigned)(ca->n_planes * nblk)),
+                       dim3((N / E) * C), 0, s, *ca);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
 #define CXM_PAIR_LAUNCH(N, ER, EC, C)                                                           \
   case N: {                                                                                     \
     if (pass == PAIR_COL) {                                                                     \
       if constexpr ((C) % 2 == 0) {                                                             \
-        const long long nblk = (ca->lx + (C) - 1) / (C);                                        \
-        if (ca->acc)                                                                            \
-          hipLaunchKernelGGL((pair_col_kernel<N, EC, C, true>), dim3((unsigned)nblk),           \
-                             dim3((N / (EC)) * (C)), 0, s, *ca);                                \
-        else                                                                                    \
-          hipLaunchKernelGGL((pair_col_kernel<N, EC, C, false>),                                \
-                             dim3((unsigned)(ca->n_planes * nblk)), dim3((N / (EC)) * (C)), 0, s, \
-                             *ca);                                                              \
-        CXM_LAUNCH_CHECK();                                                                     \
-        return CXM_OK;                                                                          \
+        constexpr int EH = pair_col_half_e<N, EC, C>();                                         \
+        return launch_pair_col<N, (EH > 0 ? EH : EC), C>(ca, s);                                \
       }                                                                                         \
     } else if constexpr (pair_rows_fit<N, ER>() && N % 2 == 0) {                                \
       if (pass == PAIR_ROW) {                                                                   \
