@@ -387,6 +387,29 @@ constexpr int pair_col_half_e() {
     return schedule_ok(N, EC / 2) ? EC / 2 : 0;
 }
 
+// Values per thread of the paired row passes: half the propagation row
+// pass's for lengths 3 * 2^a >= 512 (13.5 against 13.75 ms for 128 planes
+// of 3072-point transforms), where 4 rows still fit; else 0 (the table's).
+template <int N, int ER>
+constexpr int pair_row_half_e() {
+  if constexpr (ER % 2 != 0 || ER / 2 < 2 || N % (ER / 2) != 0 || N < 512 || N % 3 != 0)
+    return 0;
+  else if constexpr (!pair_rows_fit<N, ER / 2>())
+    return 0;
+  else
+    return schedule_ok(N, ER / 2) ? ER / 2 : 0;
+}
+
+template <int N, int E>
+int launch_pair_rows(PairPass pass, const PairRowArgs* ra, const PairInvArgs* ia, hipStream_t s) {
+  if (pass == PAIR_ROW)
+    hipLaunchKernelGGL((pair_row_kernel<N, E>), dim3((unsigned)(ra->n_rows / 4)), dim3(4 * (N / E)), 0, s, *ra);
+  else
+    hipLaunchKernelGGL((pair_inv_kernel<N, E>), dim3((unsigned)(ia->n_rows / 4)), dim3(4 * (N / E)), 0, s, *ia);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
 template <int N, int E, int C>
 int launch_pair_col(const PairColArgs* ca, hipStream_t s) {
   const long long nblk = (ca->lx + C - 1) / C;
@@ -407,15 +430,8 @@ int launch_pair_col(const PairColArgs* ca, hipStream_t s) {
         return launch_pair_col<N, (EH > 0 ? EH : EC), C>(ca, s);                                \
       }                                                                                         \
     } else if constexpr (pair_rows_fit<N, ER>() && N % 2 == 0) {                                \
-      if (pass == PAIR_ROW) {                                                                   \
-        hipLaunchKernelGGL((pair_row_kernel<N, ER>), dim3((unsigned)(ra->n_rows / 4)),          \
-                           dim3(4 * (N / (ER))), 0, s, *ra);                                    \
-      } else {                                                                                  \
-        hipLaunchKernelGGL((pair_inv_kernel<N, ER>), dim3((unsigned)(ia->n_rows / 4)),          \
-                           dim3(4 * (N / (ER))), 0, s, *ia);                                    \
-      }                                                                                         \
-      CXM_LAUNCH_CHECK();                                                                       \
-      return CXM_OK;                                                                            \
+      constexpr int RH = pair_row_half_e<N, ER>();                                              \
+      return launch_pair_rows<N, (RH > 0 ? RH : ER)>(pass, ra, ia, s);                          \
     }                                                                                           \
     return fail(CXM_ERR_UNSUPPORTED, "no paired pass for length " + std::to_string(N));         \
   }

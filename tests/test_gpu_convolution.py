@@ -271,7 +271,8 @@ def test_one_axis(mode, cplx):
 @pytest.mark.parametrize("scale_k", [1.0, 1e-7, 3e5])
 @pytest.mark.parametrize("shapes", [((3, 96, 80), (3, 17, 33)), ((2, 64, 64), (1, 64, 64)),
                                     ((1, 5, 3), (1, 2, 3)), ((2, 7, 4), (1, 3, 3)),
-                                    ((1, 2600, 4), (1, 2521, 3)), ((2, 700, 40), (2, 61, 9))])
+                                    ((1, 2600, 4), (1, 2521, 3)), ((2, 700, 40), (2, 61, 9)),
+                                    ((1, 40, 700), (1, 9, 61))])
 def test_paired_real_path_matches_separate_spectra(scale_k, shapes):
     # real image and kernel transformed as one complex plane (the kernel
     # scaled by a power of two to the image's norm) against the path with
@@ -281,7 +282,8 @@ def test_paired_real_path_matches_separate_spectra(scale_k, shapes):
     # length: separate spectra only), 10 x 6 (a half-filled last strip of
     # column pairs, row counts not multiples of 4), 5120 x 6 (two columns per
     # column-pass workgroup), 768 x 48 (a 3 * 2^a column length, the column
-    # pass at half the propagation's values per thread)
+    # pass at half the propagation's values per thread), 48 x 768 (the row
+    # passes at half the values per thread)
     from chromatix_amd import _lib as L
 
     rng = np.random.default_rng(11)
