@@ -185,11 +185,11 @@ def test_microscope_image_planewise():
 
 
 @pytest.mark.parametrize("shapes", [((5, 96, 80), (5, 17, 33)), ((3, 7, 4), (1, 3, 3)),
-                                    ((16, 1000, 1000), (16, 41, 41))])
+                                    ((24, 1000, 1000), (24, 41, 41))])
 def test_planewise_convolution_sum(shapes):
     # the plane sum taken inside the convolution (one inverse row transform
     # for all planes) against the oracle's planewise convolutions summed in
-    # FP64; 16 planes of 1280 x 1280 transforms run as two chunks (13 + 3
+    # FP64; 24 planes of 1280 x 1280 transforms run as two chunks (12 + 12
     # planes), the second added to the first's sum
     rng = np.random.default_rng(12)
     a = rng.random(shapes[0]).astype(np.float32)
