@@ -12,6 +12,11 @@
 //   row IFFT, cropped.
 // The kernel spectra K = fftn(kernel) on the (Ld x) Ly x Lx grid are
 // computed once per call by the same row / column / depth passes.
+//
+// Real input over two axes with an even row length (the default) takes the
+// paired path instead: image and kernel planes as one complex transform on
+// pair-tiled planes (see the paired passes below), optionally summed over
+// the batch before the inverse row transform (sum_batch).
 #include <algorithm>
 #include <cstring>
 #include <string>
