@@ -97,3 +97,105 @@ def test_point_source_power_and_pupil():
     g = op.grid((256, 256), (0.3, 0.3))
     outside = g[..., 0] ** 2 + g[..., 1] ** 2 > (D / 2) ** 2
     assert np.all(u[:, outside] == 0) and np.all(np.abs(u[:, ~outside]) > 0)
+
+
+# ---- SAS against the exact angular-spectrum integral ------------------------------
+def _exact_as_gaussian(r, z, w, lam, n, fmax, m=10001):
+    """Angular-spectrum field of u0 = exp(-|r|^2 / w^2) at distance z, by the
+    Hankel transform of its closed-form spectrum pi w^2 exp(-pi^2 w^2 rho^2):
+    u(r) = 2 pi int rho U(rho) exp(i 2 pi |z| sqrt((n/lam)^2 - rho^2)) J0(2 pi rho r) drho
+    (conjugated phase for z < 0), Simpson's rule on [0, fmax]."""
+    from scipy.integrate import simpson
+    from scipy.special import j0
+
+    rho = np.linspace(0.0, fmax, m)
+    U = np.pi * w**2 * np.exp(-((np.pi * w * rho) ** 2))
+    H = np.exp(2j * np.pi * abs(z) * np.sqrt((n / lam) ** 2 - rho**2 + 0j))
+    if z < 0:
+        H = np.conj(H)
+    return 2 * np.pi * simpson(j0(2 * np.pi * np.outer(r, rho)) * (rho * U * H)[None, :], x=rho, axis=1)
+
+
+def _sas_vs_exact(n_px, dx, w, z, lam, n):
+    """rel. L2 of transform_propagate_sas of a sampled Gaussian against the
+    exact angular-spectrum field on the SAS output grid.  The reference's
+    Fresnel transform leaves out the constant exp(i k |z|) / i (conjugated for
+    z < 0), so the SAS output times that constant is the physical field."""
+    y = (np.arange(n_px) - n_px / 2) * dx
+    u = np.exp(-(y[:, None] ** 2 + y[None, :] ** 2) / w**2)
+    out, du = oo.transform_propagate_sas(u, (dx, dx), lam, z, n)
+    yo = (np.arange(n_px) - n_px / 2) * np.atleast_1d(du)[0]
+    R = np.sqrt(yo[:, None] ** 2 + yo[None, :] ** 2)
+    radii, inv = np.unique(np.round(R, 12), return_inverse=True)
+    exact = _exact_as_gaussian(radii, z, w, lam, n, min(1 / (2 * dx), 12 / (np.pi * w)))
+    exact = exact[inv].reshape(R.shape)
+    c = np.exp(2j * np.pi * n * abs(z) / lam) / 1j
+    c = c if z >= 0 else np.conj(c)
+    return rel_l2(out * c, exact)
+
+
+# (pixels, dx, waist, z, lambda, n): paraxial and wide-angle forward cases,
+# and a back-propagation whose output grid resolves the beam
+_SAS_CASES = [(64, 0.5, 2.0, 100.0, 0.532, 1.0), (64, 0.25, 1.0, 40.0, 0.532, 1.33),
+              (64, 0.5, 3.0, -100.0, 0.532, 1.0)]
+
+
+@pytest.mark.parametrize("case", _SAS_CASES)
+def test_sas_matches_exact_angular_spectrum(case):
+    assert _sas_vs_exact(*case) < 1e-8
+
+
+def test_sas_exact_pin_fails_on_planted_mistakes(monkeypatch):
+    """The same comparison with the precompensation's sign flipped
+    (exp(-i kz (H_AS - H_Fr))) or left out (plain Fresnel transform) fails."""
+    orig = oo.sas_precompensation
+    monkeypatch.setattr(oo, "sas_precompensation", lambda *a, **k: np.conj(orig(*a, **k)))
+    assert min(_sas_vs_exact(*c) for c in _SAS_CASES[:2]) > 1e-2
+    monkeypatch.setattr(oo, "sas_precompensation", lambda *a, **k: np.ones_like(orig(*a, **k)))
+    assert min(_sas_vs_exact(*c) for c in _SAS_CASES[:2]) > 5e-3
+
+
+def _sas_window_error(prec):
+    """Bins where the precompensation's window W differs from the band
+    |s| <= s* per axis, s* the root of s^2 (2 + 1/t^2) = 1 with
+    t = extent/|z| + |s| (propagation.py:85-103: L = 2 * extent, t = L/2/|z| + |s|),
+    found by bisection; anisotropic grids, both signs of z."""
+    from scipy.optimize import brentq
+
+    bad = 0
+    for shape, sp_, lam, n, z in (((96, 128), (0.2, 0.15), 0.532, 1.33, 30.0),
+                                  ((128, 64), (0.25, 0.3), 0.45, 1.0, -12.0),
+                                  ((64, 64), (0.18, 0.18), 0.65, 1.33, 80.0)):
+        dH = prec(shape, sp_, lam, z, n)
+        keep = []
+        for nn, d in zip(shape, sp_):
+            a = nn * d / abs(z)
+            s_star = brentq(lambda s: s * s * (2 + 1 / (a + s) ** 2) - 1, 1e-12, 1.0)
+            keep.append(np.abs(lam * np.fft.fftfreq(nn, d) / n) <= s_star)
+        want = keep[0][:, None] & keep[1][None, :]
+        assert 0 < want.sum() < want.size
+        bad += int(np.sum((np.abs(dH) > 0) != want))
+    return bad
+
+
+def test_sas_window_edge():
+    assert _sas_window_error(oo.sas_precompensation) == 0
+    # planted mistakes: L = extent instead of 2 * extent (same as doubling z
+    # in t), and the spacings of the two axes exchanged
+    assert _sas_window_error(lambda sh, sp_, lam, z, n: oo.sas_precompensation(sh, sp_, lam, 2 * z, n)) > 0
+    assert _sas_window_error(
+        lambda sh, sp_, lam, z, n: oo.sas_precompensation(sh, sp_[::-1], lam, z, n)) > 0
+
+
+def test_sas_window_keeps_only_frequencies_that_stay_inside_the_padding():
+    """Physical sufficiency of W: the precompensation moves frequency s by
+    |z| |s| (1/sqrt(1 - s^2) - 1) sideways (its group delay); every kept
+    frequency moves less than the padded extent L/2."""
+    for shape, sp_, z in (((128, 128), (0.15, 0.15), 20.0), ((96, 64), (0.2, 0.18), -9.0)):
+        dH = oo.sas_precompensation(shape, sp_, 0.532, z, 1.33)
+        for ax, (nn, d) in enumerate(zip(shape, sp_)):
+            s = np.abs(0.532 * np.fft.fftfreq(nn, d) / 1.33)
+            kept = (np.abs(dH) > 0).any(axis=1 - ax)
+            shift = abs(z) * s[kept] * (1 / np.sqrt(1 - s[kept] ** 2) - 1)
+            assert np.all(shift <= nn * d)
+            assert not kept.all()  # the window is active in these cases
