@@ -288,6 +288,8 @@ def _declare(lib):
         "cxm_prop2d_workspace_size": (sz, [ctypes.POINTER(Prop2dDesc)]),
         "cxm_prop2d_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Prop2dDesc)]),
         "cxm_propagator_c64": (i32, [vp, vp, ctypes.POINTER(Prop2dDesc)]),
+        "cxm_noalloc_scratch_bytes": (sz, []),
+        "cxm_prop2d_c64_noalloc": (i32, [vp, vp, vp, ctypes.POINTER(Prop2dDesc)]),
         "cxm_prop2d_param_bytes": (sz, []),
         "cxm_prop2d_build_params": (i32, [ctypes.POINTER(Prop2dDesc), dp, ctypes.c_int64, vp]),
         "cxm_prop2d_c64_params": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Prop2dDesc), vp]),

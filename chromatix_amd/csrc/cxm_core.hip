@@ -20,10 +20,17 @@ constexpr int kMaxDevices = 64;
 std::mutex g_init_mu;
 bool g_ready[kMaxDevices] = {};
 float2* g_tw_ptr[kMaxDevices] = {};
+float2* g_scratch_ptr[kMaxDevices] = {};
 int g_cus[kMaxDevices] = {};
 }  // namespace
 
 __device__ __align__(256) float2 g_twiddles[kTwTotal];
+
+// Workspace of the allocation-free entry points (cxm_prop2d_c64_noalloc):
+// kNoallocScratchBytes per GPU, held for the life of the process.  In host
+// code `scratch` names a host-side stand-in, not device memory: kernels get
+// the per-device address taken by hipGetSymbolAddress in init_device.
+__device__ __align__(256) float2 scratch[kNoallocScratchBytes / sizeof(float2)];
 
 __global__ void init_twiddles_kernel(float2* tw) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,6 +66,12 @@ static int init_device(int dev) {
     if (prev != dev) (void)hipSetDevice(prev);
     return fail(CXM_ERR_HIP, std::string("hipGetSymbolAddress(twiddles): ") + hipGetErrorString(e));
   }
+  void* sp = nullptr;
+  e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) {
+    if (prev != dev) (void)hipSetDevice(prev);
+    return fail(CXM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  }
   hipLaunchKernelGGL(init_twiddles_kernel, dim3((kTwTotal + 255) / 256), dim3(256), 0, 0, (float2*)p);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -69,6 +82,7 @@ static int init_device(int dev) {
     cus = 256;
   g_cus[dev] = cus;
   g_tw_ptr[dev] = (float2*)p;
+  g_scratch_ptr[dev] = (float2*)sp;
   g_ready[dev] = true;
   return CXM_OK;
 }
@@ -86,6 +100,12 @@ int ensure_init() {
   return init_device(dev);
 }
 
+float2* noalloc_scratch() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  return g_scratch_ptr[dev];
+}
+
 const float2* twiddles_for(int n) {
   int dev = 0;
   (void)hipGetDevice(&dev);
@@ -98,6 +118,7 @@ const float2* twiddles_for(int n) {
 extern "C" int cxm_init(int device) { return cxm::init_device(device); }
 extern "C" const char* cxm_last_error(void) { return cxm::g_last_error.c_str(); }
 extern "C" int cxm_version(void) { return 100; }
+extern "C" size_t cxm_noalloc_scratch_bytes(void) { return cxm::kNoallocScratchBytes; }
 extern "C" int cxm_fft_supported(int64_t n) { return cxm::fft_len_supported(n) ? 1 : 0; }
 
 // ---------------------------------------------------------------------------

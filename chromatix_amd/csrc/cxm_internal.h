@@ -42,6 +42,10 @@ void timing_stop(void* start, hipStream_t s, const std::string& name);
 // (cxm_init must have run on it).
 const float2* twiddles_for(int n);
 int ensure_init();
+// Static device workspace of the allocation-free entry points: its size, and
+// its address on the current device (ensure_init must have run on it).
+constexpr size_t kNoallocScratchBytes = 64ull << 20;
+float2* noalloc_scratch();
 int device_cus();  // compute units of the current device (cxm_init must have run on it)
 
 constexpr int kMaxHP = 32;  // parameter sets carried by value per launch

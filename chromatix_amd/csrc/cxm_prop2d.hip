@@ -170,7 +170,12 @@ struct Geom {
   bool tiled;                                // workspace planes in strips of kTileW columns
 };
 
-int validate(const cxm_prop2d_desc* d, Geom& g) {
+bool use_h_table(const cxm_prop2d_desc* d);
+size_t table_elems(const Geom& g);
+
+// limit_bytes > 0: the chunk is the largest whose workspace (planes and the
+// H table) fits in limit_bytes; CXM_ERR_UNSUPPORTED when one plane does not.
+int validate(const cxm_prop2d_desc* d, Geom& g, size_t limit_bytes = 0) {
   if (!d) return fail(CXM_ERR_INVALID, "null descriptor");
   if (d->batch < 0 || d->height <= 0 || d->width <= 0 || d->pad_y < 0 || d->pad_x < 0 ||
       d->crop_y < 0 || d->crop_x < 0 || d->num_z < 1)
@@ -216,14 +221,23 @@ int validate(const cxm_prop2d_desc* d, Geom& g) {
   const long long budget = (128ll << 20) / 8;
   long long ch = std::max(1ll, budget / std::max(1ll, per_plane));
   if (option(CXM_OPT_CHUNK_PLANES) > 0) ch = option(CXM_OPT_CHUNK_PLANES);
-  g.chunk = (int)std::min<long long>(ch, std::max(1, d->batch));
-  g.W = col_width(g.Ny, g.chunk);
-  // Tiled intermediates: the row pass moves 4 rows per workgroup, the column
-  // pass reads strips of 4 columns, and planes are whole 4-row groups.
-  g.tiled = option(CXM_OPT_TILES) != 1 && row_group(g.Nx) == kTileW && g.W == kTileW &&
-            g.Hin % kTileW == 0 && g.Hout % kTileW == 0 && g.Nx % kTileW == 0;
-  if (g.tiled) g.plane_ws = (long long)(g.Nx / kTileW) * tile_strip_stride(g.Rws);
-  return CXM_OK;
+  const long long plane_rm = g.plane_ws;  // row-major workspace plane
+  for (ch = std::min<long long>(ch, std::max(1, d->batch)); ch >= 1; --ch) {
+    g.chunk = (int)ch;
+    g.W = col_width(g.Ny, g.chunk);
+    // Tiled intermediates: the row pass moves 4 rows per workgroup, the column
+    // pass reads strips of 4 columns, and planes are whole 4-row groups.
+    g.tiled = option(CXM_OPT_TILES) != 1 && row_group(g.Nx) == kTileW && g.W == kTileW &&
+              g.Hin % kTileW == 0 && g.Hout % kTileW == 0 && g.Nx % kTileW == 0;
+    g.plane_ws = g.tiled ? (long long)(g.Nx / kTileW) * tile_strip_stride(g.Rws) : plane_rm;
+    if (limit_bytes == 0) return CXM_OK;
+    const size_t need = ((size_t)g.chunk * (nout == 1 ? 1 : 1 + nout) * (size_t)g.plane_ws +
+                         (use_h_table(d) ? table_elems(g) : 0)) * sizeof(float2);
+    if (need <= limit_bytes) return CXM_OK;
+  }
+  return fail(CXM_ERR_UNSUPPORTED, "one plane of a " + std::to_string(g.Ny) + "x" +
+                                       std::to_string(g.Nx) + " grid needs more workspace than the " +
+                                       std::to_string(limit_bytes >> 20) + " MiB static scratch");
 }
 
 int out_per_plane(const cxm_prop2d_desc* d) { return d->z_zip ? 1 : d->num_z; }
@@ -259,18 +273,25 @@ extern "C" size_t cxm_prop2d_workspace_size(const cxm_prop2d_desc* d) {
 }
 
 namespace {
+// noalloc: the workspace is the library's static scratch array (workspace
+// and workspace_bytes are ignored) and the chunk is sized to fit it.
 int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t workspace_bytes,
-                const cxm_prop2d_desc* d, const HParam* hp_dev) {
+                const cxm_prop2d_desc* d, const HParam* hp_dev, bool noalloc = false) {
   Geom g;
-  int rc = validate(d, g);
+  int rc = validate(d, g, noalloc ? kNoallocScratchBytes : 0);
   if (rc) return rc;
   if (d->batch == 0) return CXM_OK;
   if (!in || !out) return fail(CXM_ERR_INVALID, "null buffer");
   const bool table = !hp_dev && use_h_table(d);
   const size_t need = ws_elems(d, g, table) * sizeof(float2);
-  if (!workspace || workspace_bytes < need)
+  if (!noalloc && (!workspace || workspace_bytes < need))
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   if ((rc = ensure_init())) return rc;
+  if (noalloc) {
+    workspace = noalloc_scratch();
+    if (!workspace || need > kNoallocScratchBytes)
+      return fail(CXM_ERR_WORKSPACE, "static scratch unavailable or too small");
+  }
   hipStream_t s = (hipStream_t)stream;
 
   ColArgs ca;
@@ -387,6 +408,14 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
 extern "C" int cxm_prop2d_c64(void* stream, const void* in, void* out, void* workspace,
                               size_t workspace_bytes, const cxm_prop2d_desc* d) {
   return prop2d_impl(stream, in, out, workspace, workspace_bytes, d, nullptr);
+}
+
+// The same passes with the workspace in the library's static scratch array:
+// no allocation by the caller or the library.  Calls that overlap on one GPU
+// share the array and race.
+extern "C" int cxm_prop2d_c64_noalloc(void* stream, const void* in, void* out,
+                                      const cxm_prop2d_desc* d) {
+  return prop2d_impl(stream, in, out, nullptr, 0, d, nullptr, true);
 }
 
 extern "C" size_t cxm_prop2d_param_bytes(void) { return sizeof(HParam); }

@@ -102,6 +102,21 @@ int cxm_prop2d_c64(void* stream, const void* in, void* out, void* workspace,
                    size_t workspace_bytes, const cxm_prop2d_desc* d);
 
 /*
+ * cxm_prop2d_c64 without a workspace: the passes use the library's static
+ * device scratch array of cxm_noalloc_scratch_bytes() (64 MiB) per GPU, and
+ * neither the caller nor the library allocates memory during the call.  The
+ * chunk of planes per pass sequence is sized to fit the array; a grid one
+ * plane of which (with the shared H table) does not fit is rejected with
+ * CXM_ERR_UNSUPPORTED (2048^2 fits, 4096^2 does not).  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (two streams, two
+ * host threads) race and must be serialised by the caller.  No counterpart
+ * in the reference: for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).
+ */
+size_t cxm_noalloc_scratch_bytes(void);
+int cxm_prop2d_c64_noalloc(void* stream, const void* in, void* out, const cxm_prop2d_desc* d);
+
+/*
  * cxm_prop2d_c64 with a device table of per-output-plane transfer-function
  * parameters instead of the descriptor's z list: output plane (p, iz) of
  * input plane p uses params[p * num_z + iz].  The table is built on the host
