@@ -393,6 +393,19 @@ def ensure_device(t: torch.Tensor) -> None:
         _initialised.add(dev)
 
 
+def refuse_grad(what: str, *tensors) -> None:
+    """Raise where an entry point without an adjoint would receive an input
+    that requires grad: its output would carry no grad_fn and the gradient
+    would be lost without an error."""
+    if not torch.is_grad_enabled():
+        return
+    for t in tensors:
+        if isinstance(t, torch.Tensor) and t.requires_grad:
+            raise NotImplementedError(
+                f"{what}: gradients are not implemented for this entry point; "
+                "call it under torch.no_grad() or detach the input")
+
+
 def stream_ptr(device: torch.device | None = None) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 

@@ -139,7 +139,11 @@ plane_ss_kernel(const float* __restrict__ x, long long plane_elems, double* __re
   if (threadIdx.x == 0) partial[(long long)plane * kSsBlocks + blockIdx.x] = sh[0];
 }
 
-// scale[b] = 2^round(log2(|a_b| / |k_{b % k_planes}|)) (1 if either is 0)
+// scale[b] = 2^round(log2(|a_b| / |k_{b % k_planes}|)) and, after the batch's
+// scales, scale[batch + b] = 1 / (4 scale[b]) (exact: a power of two).  Both
+// are 0 when either plane is all zeros: the product A K of that pair is
+// exactly 0, and the column pass then stores exact zeros (the shared
+// transform would leave a rounding residual of the nonzero plane there).
 __global__ void pair_scale_kernel(const double* __restrict__ pa, const double* __restrict__ pk,
                                   int batch, int k_planes, float* __restrict__ scale) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -148,12 +152,14 @@ __global__ void pair_scale_kernel(const double* __restrict__ pa, const double* _
   for (int j = 0; j < kSsBlocks; ++j) sa += pa[(long long)b * kSsBlocks + j];
   const int kb = b % k_planes;
   for (int j = 0; j < kSsBlocks; ++j) sk += pk[(long long)kb * kSsBlocks + j];
-  float s = 1.0f;
+  float s = 0.0f, q = 0.0f;
   if (sa > 0.0 && sk > 0.0) {
-    const int e = (int)lrint(0.25 * log2(sa / sk) * 2.0);  // round(log2(sqrt(sa / sk)))
-    s = ldexpf(1.0f, max(-100, min(100, e)));
+    const int e = max(-100, min(100, (int)lrint(0.25 * log2(sa / sk) * 2.0)));  // round(log2(sqrt(sa / sk)))
+    s = ldexpf(1.0f, e);
+    q = ldexpf(1.0f, -e - 2);
   }
   scale[b] = s;
+  scale[batch + b] = q;
 }
 
 // Pair-tiled planes.  Slot 2 j + side of a row holds column j (side 0) and
@@ -250,7 +256,7 @@ pair_row_kernel(PairRowArgs a) {
 struct PairColArgs {
   float2* buf;               // pair-tiled planes, in place
   const float2* tw;
-  const float* scale;        // per image plane
+  const float* inv4s;        // per image plane: 1 / (4 scale), 0 for a zero plane
   long long plane_stride, strip_stride;
   int lx;                    // columns (even) = slots
   int in_len;                // rows holding data
@@ -276,7 +282,7 @@ CXM_DEV void pair_col_plane(const PairColArgs& a, int p, const float2* colp, boo
   __syncthreads();  // the transform's last exchange has been read by every thread
   stage_out<N, E, C>(u, lds + c, ty);
   __syncthreads();
-  const float f = a.norm / (4.0f * a.scale[a.plane0 + p]);
+  const float f = a.norm * a.inv4s[a.plane0 + p];  // 0 for a zero plane: exact zeros
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int i = ty + k * T;
@@ -547,7 +553,7 @@ PairGeom pair_geom(const cxm_fconv_desc* d, const FcGeom& g) {
   const long long cap = std::max(1LL, std::min<long long>(nb, ((160LL << 20) / 8) / q.plane));
   const long long nchunks = (nb + cap - 1) / cap;
   q.chunk = (int)((nb + nchunks - 1) / nchunks);
-  const size_t scales = ((size_t)std::max(1, d->batch) * sizeof(float) + 255) / 256 * 256;
+  const size_t scales = ((size_t)std::max(1, d->batch) * 2 * sizeof(float) + 255) / 256 * 256;
   const size_t partial = (size_t)(std::max(1, d->batch) + d->k_planes) * kSsBlocks * sizeof(double);
   q.ws_off = scales + (partial + 255) / 256 * 256;
   return q;
@@ -590,7 +596,7 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
     const PairGeom q = pair_geom(d, g);
     float* scale = (float*)workspace;
     double* pa = (double*)((char*)workspace +
-                           ((size_t)d->batch * sizeof(float) + 255) / 256 * 256);
+                           ((size_t)d->batch * 2 * sizeof(float) + 255) / 256 * 256);
     double* pk = pa + (size_t)d->batch * kSsBlocks;
     float2* ws = (float2*)((char*)workspace + q.ws_off);
     float2* acc = ws + (size_t)q.chunk * q.plane;  // sum_batch: the summed kept rows
@@ -633,7 +639,7 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
       memset(&ca, 0, sizeof(ca));
       ca.buf = ws;
       ca.tw = twiddles_for(Ly);
-      ca.scale = scale;
+      ca.inv4s = scale + d->batch;
       ca.plane_stride = q.plane;
       ca.strip_stride = q.strip;
       ca.lx = Lx;

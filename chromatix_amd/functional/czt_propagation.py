@@ -35,6 +35,7 @@ def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=
                          shift_yx=(0.0, 0.0), use_czt: bool = True) -> Field:
     """kernel_propagate with an output grid of spacing ``output_dx`` and shape
     ``output_shape`` centred on ``shift_yx`` (propagation.py:395-443)."""
+    L.refuse_grad("kernel_propagate with an output grid", field.u, propagator)
     if isinstance(field, ChromaticScalarField):
         raise NotImplementedError("output-grid propagation: monochromatic scalar fields only")
     ny, nx = field.spatial_shape
@@ -93,6 +94,7 @@ def czt_kernel_propagate_fields(method: str, field: Field, z, n, pad_width, cval
     pad, propagator on the padded grid (no output shift when the CZT places
     the window), CZT branch of kernel_propagate, crop in "same" mode
     (propagation.py:229-252, 325-352)."""
+    L.refuse_grad(f"{method}_propagate with an output grid", field.u)
     py, px = _pair(pad_width)
     padded = pad(field, (py, px), cval=cval)
     z = float(np.asarray(z).reshape(()))

@@ -97,6 +97,7 @@ def _new_spacing(field: Field, z: float, n: float, grid: tuple[int, int]) -> np.
 
 def optical_fft(field: Field, z, n, *, phase_mode=None) -> Field:
     """Far-field / lens FFT with the sampling change du = df |lam z / n|."""
+    L.refuse_grad("optical_fft", field.u)
     z = float(z)
     d, keep = _desc(field, L.CXM_OFFT_OPTICAL, z, n, phase_mode)
     u = _run(field, d, "optical_fft", field.spatial_shape)
@@ -107,6 +108,7 @@ def ff_lens(field: Field, f, n, NA=None, inverse: bool = False, *, phase_mode=No
     """Thin lens a distance f after the field, observed in its back focal
     plane: optional circular pupil of diameter 2 f NA / n, then optical_fft
     with f (or -f when inverse)."""
+    L.refuse_grad("ff_lens", field.u)
     f = float(f)
     z = -f if inverse else f
     d, keep = _desc(field, L.CXM_OFFT_OPTICAL, z, n, phase_mode)
@@ -121,6 +123,7 @@ def transform_propagate(field: Field, z, n, pad_width, cval: float = 0,
                         skip_initial_phase: bool = False, skip_final_phase: bool = False, *,
                         phase_mode=None) -> Field:
     """Single-FFT Fresnel propagation (SFT-FR) with the sampling change."""
+    L.refuse_grad("transform_propagate", field.u)
     py, px = _pair(pad_width)
     if cval != 0:
         from ..core.field import pad as _pad
@@ -147,6 +150,7 @@ def transform_propagate_sas(field: Field, z, n, cval: float = 0,
                             phase_mode=None) -> Field:
     """Scalable angular spectrum propagation (padding factor 2, as the
     reference supports)."""
+    L.refuse_grad("transform_propagate_sas", field.u)
     if cval != 0:
         raise NotImplementedError("transform_propagate_sas with cval != 0")
     z = float(z)
@@ -176,6 +180,7 @@ def compute_sas_precompensation(field: Field, z, n, *, phase_mode=None) -> torch
 
 def _raw(x: torch.Tensor, axes, shift: bool, inverse: bool) -> torch.Tensor:
     x = torch.as_tensor(x)
+    L.refuse_grad("fft / ifft", x)
     ax = tuple(a % x.ndim for a in axes)
     if len(ax) != 2:
         raise ValueError("fft/ifft take two axes")
@@ -211,6 +216,7 @@ def high_na_ff_lens(field: Field, f, n, NA, output_shape=None, output_dx=None) -
     cartesian_to_spherical branch) are not part of this library."""
     from ..utils.czt import zoomed_fft
 
+    L.refuse_grad("high_na_ff_lens", field.u)
     if isinstance(field, ChromaticScalarField):
         raise NotImplementedError("high_na_ff_lens: single-wavelength scalar fields only "
                                   "(the reference assumes one wavelength)")

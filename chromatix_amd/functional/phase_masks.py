@@ -13,6 +13,7 @@ __all__ = ["phase_change"]
 def phase_change(field: Field, phase, spectrally_modulate: bool = True) -> Field:
     """Perturbs ``field`` by ``phase`` (radians, (H, W), broadcast over the
     batch and, scaled by the spectral modulation, over wavelengths)."""
+    L.refuse_grad("phase_change", field.u, phase)
     u = field.u.contiguous()
     L.ensure_device(u)
     ph = torch.as_tensor(phase, dtype=torch.float32, device=u.device).contiguous()

@@ -31,6 +31,7 @@ def thin_sample(field: Field, absorption, dn, thickness) -> Field:
     """``field * exp(1j * 2 pi (dn + 1j * absorption) * thickness / lambda)``
     with (H, W) ``absorption`` and ``dn`` broadcast over the batch (and the
     wavelengths of a chromatic field)."""
+    L.refuse_grad("thin_sample", field.u, absorption, dn)
     u = field.u.contiguous()
     L.ensure_device(u)
     H, W = field.spatial_shape

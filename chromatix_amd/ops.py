@@ -54,6 +54,7 @@ def fourier_convolution(image, kernel, *, axes=(0, 1), fast_fft_shape: bool = Tr
     ``axes`` broadcast; in "same" mode they are cropped to the image's, as
     the reference's slices do."""
     image = torch.as_tensor(image)
+    L.refuse_grad("fourier_convolution", image, kernel)
     L.ensure_device(image)
     kernel = torch.as_tensor(kernel, device=image.device)
     if image.ndim != kernel.ndim:
@@ -152,6 +153,7 @@ def planewise_convolution_sum(image, kernel) -> torch.Tensor | None:
     summed path does not cover the input (complex values, an odd row
     transform length); the (H, W) result otherwise."""
     image = torch.as_tensor(image)
+    L.refuse_grad("planewise_convolution_sum", image, kernel)
     L.ensure_device(image)
     kernel = torch.as_tensor(kernel, device=image.device)
     if (image.ndim != 3 or kernel.ndim != 3 or image.is_complex() or kernel.is_complex()

@@ -33,6 +33,7 @@ def czt_axis(x: torch.Tensor, m: int, a, w, axis: int, *, num_param: int = 1, mo
     the output is also multiplied by w^(-N k) a^N (the modulation of the CZT
     branch of kernel_propagate) and by ``scale``."""
     x = torch.as_tensor(x)
+    L.refuse_grad("czt", x)
     L.ensure_device(x)
     x = x.to(torch.complex64).contiguous()
     H, W = x.shape[-2:]
@@ -95,6 +96,7 @@ def scaled_conv_axis(x: torch.Tensor, axis: int, alpha: float, df: float, out_dx
     kern[j + (n-1)//2 - k], kern[t] = e^{-i pi alpha f_t^2} on [0, n),
     f_k = (k - n//2) df, x_j = x0 + j out_dx."""
     x = torch.as_tensor(x)
+    L.refuse_grad("kernel_propagate (scaled convolution)", x)
     L.ensure_device(x)
     x = x.to(torch.complex64).contiguous()
     H, W = x.shape[-2:]

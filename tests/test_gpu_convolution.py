@@ -299,3 +299,46 @@ def test_paired_real_path_matches_separate_spectra(scale_k, shapes):
         L.check(L.lib().cxm_set_option(L.CXM_OPT_FCONV_PAIR, 0))
     for o in outs:
         assert rel_l2(o, ref) < TOL
+
+
+def _band_limited_psf(n, radius):
+    """|ifft2(circular pupil)|^2, centred: a PSF whose spectrum (the pupil's
+    autocorrelation) vanishes beyond 2 * radius bins."""
+    f = np.fft.fftfreq(n) * n
+    pupil = (f[:, None] ** 2 + f[None, :] ** 2) <= radius**2
+    return np.fft.fftshift(np.abs(np.fft.ifft2(pupil)) ** 2)
+
+
+@pytest.mark.parametrize("scale_k", [1.0, 3e5])
+def test_paired_real_path_zero_and_sparse_planes(scale_k):
+    # planes the shared transform of the paired path handles worst: an
+    # all-zero image plane, an all-zero kernel plane (the pair's product is
+    # exactly 0), and a sparse image (five point emitters) under a
+    # band-limited PSF; each plane against FP64 on its own, both paths
+    from chromatix_amd import _lib as L
+
+    rng = np.random.default_rng(13)
+    a = np.zeros((4, 96, 80), np.float32)
+    a[0] = rng.random((96, 80))
+    a[2] = rng.random((96, 80))
+    for y, x in rng.integers(5, 75, size=(5, 2)):
+        a[3, y, x] = rng.uniform(0.5, 2.0)
+    k = np.zeros((4, 33, 33), np.float32)
+    k[1] = rng.random((33, 33))
+    k[2] = rng.random((33, 33))
+    k[3] = _band_limited_psf(33, 6)
+    k = (k * scale_k).astype(np.float32)
+    a[1] = 0.0  # image plane 1 empty (kernel plane 1 is not)
+    ref = oc.fourier_convolution(a, k, axes=(-2, -1))
+    try:
+        for opt in (0, 1):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_FCONV_PAIR, opt))
+            out = _gpu(a, k, axes=(-2, -1))
+            assert np.all(out[0] == 0) and np.all(out[1] == 0), f"pair path {opt}: zero planes"
+            for p in (2, 3):
+                assert rel_l2(out[p], ref[p]) < TOL, (opt, p)
+            if opt == 0:  # the summed convolution runs on the paired path only
+                s = planewise_convolution_sum(torch.as_tensor(a).cuda(), torch.as_tensor(k).cuda())
+                assert rel_l2(s.cpu().numpy(), ref.sum(0)) < TOL
+    finally:
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_FCONV_PAIR, 0))

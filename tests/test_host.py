@@ -130,3 +130,35 @@ def test_absorbing_boundary_validation():
     f = cx.ScalarField(torch.zeros(4, 4, dtype=torch.complex64), 0.3, 0.0, 0.5)
     with pytest.raises(AssertionError):
         cf.kernel_propagate(f, torch.ones(4, 4), absorbing_boundary="box")
+
+
+def test_entry_points_without_adjoints_refuse_inputs_that_require_grad():
+    """An entry point without an autograd rule raises for an input that
+    requires grad (its output would carry no grad_fn), before touching the
+    device."""
+    from chromatix_amd.ops import fourier_convolution, planewise_convolution_sum
+    from chromatix_amd.utils.czt import czt, scaled_conv_axis
+
+    u = torch.ones((8, 8), dtype=torch.complex64, requires_grad=True)
+    f = cx.ScalarField(u, 0.3, 0.0, 0.532)
+    ph = torch.zeros((8, 8), requires_grad=True)
+    calls = [
+        lambda: cf.optical_fft(f, 100.0, 1.33),
+        lambda: cf.ff_lens(f, 100.0, 1.33),
+        lambda: cf.transform_propagate(f, 100.0, 1.33, 2),
+        lambda: cf.transform_propagate_sas(f, 100.0, 1.33),
+        lambda: cf.high_na_ff_lens(f, 100.0, 1.33, 0.5),
+        lambda: cf.asm_propagate(f, 100.0, 1.33, 0, output_dx=0.2, output_shape=(4, 4)),
+        lambda: cf.phase_change(cx.ScalarField(torch.ones((8, 8), dtype=torch.complex64), 0.3,
+                                               0.0, 0.532), ph),
+        lambda: cf.thin_sample(f, torch.zeros((8, 8)), torch.zeros((8, 8)), 0.5),
+        lambda: czt(u, 4, 1.0, 1.0),
+        lambda: scaled_conv_axis(u, 0, 1.0, 1.0, 1.0, 0.0),
+        lambda: fourier_convolution(u.real, ph, axes=(0, 1)),
+        lambda: planewise_convolution_sum(u.real[None], ph[None]),
+    ]
+    for i, call in enumerate(calls):
+        with pytest.raises(NotImplementedError, match="gradients are not implemented"):
+            call()
+    with torch.no_grad():  # nothing is refused without grad mode
+        L.refuse_grad("x", u)
