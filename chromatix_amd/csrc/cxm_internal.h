@@ -116,6 +116,10 @@ int build_hparams(const cxm_prop2d_desc* d, int Ny, int Nx, HParam* out, int nma
 // Kernel launchers (implemented per translation unit).
 int launch_row(int N, int dir, const RowArgs& a, hipStream_t s);
 int launch_col_prop(int N, const ColArgs& a, hipStream_t s);
+// persistent column pass for one parameter set (cxm_prop2d_persist.hip);
+// -1 when N has none
+bool col_persist_available(int N);
+int launch_col_persist(int N, const ColArgs& a, hipStream_t s);
 bool fft_len_supported(long long n);
 
 }  // namespace cxm

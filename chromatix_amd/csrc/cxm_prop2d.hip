@@ -242,11 +242,26 @@ int validate(const cxm_prop2d_desc* d, Geom& g, size_t limit_bytes = 0) {
 
 int out_per_plane(const cxm_prop2d_desc* d) { return d->z_zip ? 1 : d->num_z; }
 
-// One H table for the batch: analytic kind, a single parameter set, and
-// enough planes to amortise the table pass (CXM_OPT_H_TABLE overrides).
-bool use_h_table(const cxm_prop2d_desc* d) {
+// Analytic kind, a single parameter set, and enough planes to share H.
+bool one_param_set(const cxm_prop2d_desc* d) {
   if (d->kind != CXM_H_TRANSFER && d->kind != CXM_H_ASM) return false;
-  if ((d->num_wv > 0 ? d->num_wv : 1) * d->num_z != 1) return false;
+  return (d->num_wv > 0 ? d->num_wv : 1) * d->num_z == 1;
+}
+
+// The persistent column pass (H per strip in registers) for such a call,
+// where the column length has one and no launch-shape option is set.
+bool use_persist(const cxm_prop2d_desc* d) {
+  if (!one_param_set(d) || d->batch < CXM_H_TABLE_MIN_PLANES) return false;
+  if (option(CXM_OPT_COL_PERSIST) == 1 || option(CXM_OPT_H_TABLE) != 0 ||
+      option(CXM_OPT_COL_VARIANT) != 0 || option(CXM_OPT_COL_ORDER) != 0)
+    return false;
+  return col_persist_available(d->height + 2 * d->pad_y);
+}
+
+// Otherwise one H table for the batch, evaluated once per call and read by
+// the per-plane column pass (CXM_OPT_H_TABLE overrides).
+bool use_h_table(const cxm_prop2d_desc* d) {
+  if (!one_param_set(d) || use_persist(d)) return false;
   const int o = option(CXM_OPT_H_TABLE);
   if (o == 1) return false;
   return o == 2 || d->batch >= CXM_H_TABLE_MIN_PLANES;
@@ -283,6 +298,7 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
   if (d->batch == 0) return CXM_OK;
   if (!in || !out) return fail(CXM_ERR_INVALID, "null buffer");
   const bool table = !hp_dev && use_h_table(d);
+  const bool persist = !hp_dev && use_persist(d);
   const size_t need = ws_elems(d, g, table) * sizeof(float2);
   if (!noalloc && (!workspace || workspace_bytes < need))
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
@@ -372,7 +388,11 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
       ca.scale = 1.0f;
     }
     ca.strip_major = option(CXM_OPT_COL_ORDER) == 1;
-    if ((rc = launch_col_prop(g.Ny, ca, s))) return rc;
+    if (persist) {
+      if ((rc = launch_col_persist(g.Ny, ca, s))) return rc;
+    } else if ((rc = launch_col_prop(g.Ny, ca, s))) {
+      return rc;
+    }
     // 3) rows: inverse FFT, crop, complex or intensity
     RowArgs r2;
     memset(&r2, 0, sizeof(r2));

@@ -694,6 +694,18 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         (measured +11% on 64 x 1024^2); 1 = no hints;
  *                         2 = streaming stores only; 3 = streaming loads
  *                         only.  Same results.
+ *   CXM_OPT_COL_PERSIST   cxm_prop2d_c64 with one parameter set and
+ *                         512/1024/2048-point columns: 0 = the persistent
+ *                         column pass (H of a strip evaluated once per
+ *                         workgroup, no H table), 1 = the per-plane column
+ *                         pass (with the H table), n > 1 = the persistent
+ *                         pass with n workgroups per CU
+ *   CXM_OPT_COL_PERSIST_E 8: the 1024-point persistent column pass with 8
+ *                         values per thread (default 16)
+ *   CXM_OPT_COL_PERSIST_MODE  the 1024-point persistent column pass: 0 =
+ *                         twiddles in LDS and the next item's values loaded
+ *                         during the current transforms, 1 = neither,
+ *                         2 = LDS twiddles only, 3 = the early load only
  *   CXM_OPT_FCONV_PAIR    cxm_fconv_c64 with real input over two axes: 0 = the
  *                         image and kernel planes transformed together as one
  *                         complex plane (kernel scaled by a power of two to the
@@ -716,6 +728,9 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_MBS_XGREEN 9
 #define CXM_OPT_NT_IO 10
 #define CXM_OPT_FCONV_PAIR 11
+#define CXM_OPT_COL_PERSIST 12
+#define CXM_OPT_COL_PERSIST_E 13
+#define CXM_OPT_COL_PERSIST_MODE 14
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

@@ -346,3 +346,40 @@ def test_three_times_power_of_two_grids(shape):
     out_t = cf.asm_propagate(cx.ScalarField(torch.as_tensor(u.T.copy()).cuda(), (0.25, 0.3), 0.0, 0.532),
                              30.0, 1.33, 0, phase_mode="accurate").u.cpu().numpy()
     assert rel_l2(out_t, ref.T) < 1e-5
+
+
+@pytest.mark.parametrize("phase_mode", ["ref32", "accurate"])
+@pytest.mark.parametrize("method,shape,pad,batch,opts", [
+    ("asm", (960, 200), (32, 28), 5, dict(bandlimit=True, shift_yx=(20.0, 10.0))),  # 1024 x 256
+    ("transfer", (512, 80), 0, 4, dict(kykx=(0.2, 0.1))),                          # 512 x 80
+    ("asm", (2048, 40), 0, 6, {}),                                                   # 2048 x 40
+    ("asm", (1024, 1024), 0, 7, {}),                                                 # tiled planes
+    ("asm", (1024, 32), 0, 5, dict(dx=0.1, z=2.0)),                                  # evanescent
+])
+def test_persistent_column_pass_is_bit_identical(method, shape, pad, batch, opts, phase_mode):
+    """A call with one parameter set runs the persistent column pass (H of a
+    strip evaluated once per workgroup, the chunk's strip x plane items dealt
+    in equal ranges); its bits equal those of the per-plane pass evaluating
+    H on the fly, in every launch shape of the 1024-point pass."""
+    opts = dict(opts)
+    dx, z = opts.pop("dx", 0.3), opts.pop("z", 40.0)
+    u = _phase_field(shape, seed=13, batch=(batch,))
+    lib = L.lib()
+    runs = [{}, {L.CXM_OPT_COL_PERSIST: 1, L.CXM_OPT_H_TABLE: 1}, {L.CXM_OPT_COL_PERSIST: 3}]
+    if shape[0] + 2 * (pad if np.isscalar(pad) else pad[0]) == 1024:
+        runs += [{L.CXM_OPT_COL_PERSIST_MODE: m} for m in (1, 2, 3)]
+    keys = (L.CXM_OPT_COL_PERSIST, L.CXM_OPT_H_TABLE, L.CXM_OPT_COL_PERSIST_MODE)
+    outs = []
+    try:
+        for r in runs:
+            for k in keys:
+                L.check(lib.cxm_set_option(k, r.get(k, 0)))
+            outs.append(_run(u, dx, z, pad, method, "same", phase_mode, **opts))
+    finally:
+        for k in keys:
+            L.check(lib.cxm_set_option(k, 0))
+    for o in outs[1:]:
+        assert np.array_equal(outs[0].view(np.uint32), o.view(np.uint32))
+    ref = _oracle(u[batch - 1], dx, z, pad, method, "same",
+                  "ref32" if phase_mode == "ref32" else "fp64", **opts)
+    assert rel_l2(outs[0][batch - 1], ref) < TOL
