@@ -81,30 +81,50 @@ def hbm_copy_gbps(torch, L, nbytes=1 << 30, reps=10):
     return 2 * nbytes / (ms / cnt * 1e-3) / 1e9
 
 
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args):
-    """Reference path restated in NumPy/SciPy (oracle, 'port'), host cores."""
+    """Reference path restated in NumPy/SciPy (oracle, 'port'), on all host
+    cores and (a shorter sample) on one worker."""
     from oracle import propagation as op
 
     cores = len(os.sched_getaffinity(0))
     rng = np.random.default_rng(0)
     sample_planes = 4
     u = np.exp(1j * rng.uniform(0, 2 * np.pi, (sample_planes, args.n, args.n))).astype(np.complex64)
-    op.propagate(u[:1], (0.3, 0.3), 0.532, 100.0, 1.33, 0, precision="ref32", workers=cores)
-    done, t0 = 0, time.perf_counter()
-    while True:
-        op.propagate(u, (0.3, 0.3), 0.532, 100.0, 1.33, 0, precision="ref32", workers=cores)
-        done += sample_planes
-        el = time.perf_counter() - t0
-        if el >= args.cpu_seconds:
-            break
+
+    def rate(workers, seconds, per_call):
+        op.propagate(u[:1], (0.3, 0.3), 0.532, 100.0, 1.33, 0, precision="ref32", workers=workers)
+        done, t0 = 0, time.perf_counter()
+        while True:
+            op.propagate(u[:per_call], (0.3, 0.3), 0.532, 100.0, 1.33, 0, precision="ref32",
+                         workers=workers)
+            done += per_call
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                return done, el
+
+    done, el = rate(cores, args.cpu_seconds, sample_planes)
+    done1, el1 = rate(1, max(2.0, args.cpu_seconds / 3), 1)
     return {
         "value": done / el,
         "unit": "propagations/s",
         "cores": cores,
         "kind": "port",
+        "cpu_model": _cpu_model(),
+        "value_1_worker": done1 / el1,
         "sample": f"{done} planes of {args.n}^2 in calls of {sample_planes} "
                   f"(asm_propagate, reference-order float32 propagator, complex64 "
-                  f"scipy.fft workers={cores}), {el:.1f} s",
+                  f"scipy.fft workers={cores}), {el:.1f} s; 1 worker: {done1} planes, "
+                  f"{el1:.1f} s",
     }
 
 
@@ -115,7 +135,7 @@ def timing_name(rocprof_name: str) -> str | None:
     """rocprof kernel name -> libcxm timing name (col_prop_1024, ...)."""
     import re
 
-    m = re.search(r"col_prop_kernel<(\d+),", rocprof_name)
+    m = re.search(r"col_prop_(?:persist_)?kernel<(\d+),", rocprof_name)
     if m:
         return f"col_prop_{m.group(1)}"
     m = re.search(r"row_fft_(?:loop_)?kernel<(\d+), \d+, (-?1)>", rocprof_name)
@@ -734,6 +754,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
             "traffic": traffic,
+            "traffic_source": "FETCH_SIZE + WRITE_SIZE per dispatch of this kernel from a "
+                              "rocprofv3 --pmc run of this workload (profiles/traffic_m0.json), "
+                              "each scaled by the counter's reading for a copy kernel of known "
+                              "size; not measured in this run",
             "algorithmic_bytes_per_launch": dom_bytes,
             "avg_launch_us": dom_avg_s * 1e6,
         },
