@@ -193,6 +193,25 @@ class CztDesc(ctypes.Structure):
     ]
 
 
+class ZoomDesc(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("m_y", ctypes.c_int32),
+        ("m_x", ctypes.c_int32),
+        ("mod_n_y", ctypes.c_int32),
+        ("mod_n_x", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("a_y", ctypes.c_double * 2),
+        ("w_y", ctypes.c_double * 2),
+        ("a_x", ctypes.c_double * 2),
+        ("w_x", ctypes.c_double * 2),
+        ("scale_y", ctypes.c_double),
+        ("scale_x", ctypes.c_double),
+    ]
+
+
 class YuDesc(ctypes.Structure):
     _fields_ = [
         ("batch", ctypes.c_int32),
@@ -291,6 +310,11 @@ def _declare(lib):
         "cxm_prop2d_workspace_size": (sz, [ctypes.POINTER(Prop2dDesc)]),
         "cxm_prop2d_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Prop2dDesc)]),
         "cxm_propagator_c64": (i32, [vp, vp, ctypes.POINTER(Prop2dDesc)]),
+        "cxm_zoom_dft_workspace_size": (sz, [ctypes.POINTER(ZoomDesc)]),
+        "cxm_zoom_dft_mfma_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(ZoomDesc)]),
+        "cxm_cgemm_mfma_c64": (i32, [vp, vp, vp, vp, i32, i32, i32, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_int64, i32]),
         "cxm_noalloc_scratch_bytes": (sz, []),
         "cxm_prop2d_c64_noalloc": (i32, [vp, vp, vp, ctypes.POINTER(Prop2dDesc)]),
         "cxm_prop2d_param_bytes": (sz, []),

@@ -20,7 +20,7 @@ import torch
 
 from .. import _lib as L
 from ..core.field import ChromaticScalarField, Field, crop, pad, shift_grid
-from ..utils.czt import czt_axis, scaled_conv_axis
+from ..utils.czt import czt_axis, scaled_conv_axis, use_zoom_mfma, zoom_dft
 from .optics import fft as _fft_shifted
 from .propagation import (
     _pair,
@@ -76,14 +76,23 @@ def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=
         return out_field.replace(u=res)
     (y0, y1), (x0, x1) = out_field.spatial_limits
     lims = ((y0, y1), (x0, x1))
-    res = spec
+    pars = []
     for d in range(2):
         m = output_shape[d]
         lo, hi = lims[d]
         a = np.exp(-1j * 2 * np.pi / T[d] * lo)
         w = np.exp(1j * (2 * np.pi / T[d]) * (hi - lo) / (m - 1))
         scale = float(np.prod(1.0 / alpha)) if d == 1 else 1.0
-        res = czt_axis(res, m, a, w, d, mod_n=(m - 1) // 2, scale=scale)
+        pars.append((m, a, w, (m - 1) // 2, scale))
+    if use_zoom_mfma(output_shape):
+        # a small window: both CZTs with their modulation as two complex GEMMs
+        res = zoom_dft(spec, output_shape, (pars[0][1], pars[1][1]), (pars[0][2], pars[1][2]),
+                       mod_n=(pars[0][3], pars[1][3]), scale=(pars[0][4], pars[1][4]))
+        return out_field.replace(u=res)
+    res = spec
+    for d in range(2):
+        m, a, w, nmod, scale = pars[d]
+        res = czt_axis(res, m, a, w, d, mod_n=nmod, scale=scale)
     return out_field.replace(u=res)
 
 

@@ -318,6 +318,45 @@ int cxm_czt_axis_c64(void* stream, const void* in, void* out, void* workspace,
                      size_t workspace_bytes, const cxm_czt_desc* d);
 
 /*
+ * Matrix-DFT zoom on MFMA: the chirp-z transform of every height x width
+ * plane along both axes, y[ky][kx] = sum_{jy,jx} x[jy][jx] A_y[ky][jy]
+ * A_x[kx][jx] with A_d[k][j] = a_d^-j w_d^(j k), times w_d^(-N_d k) a_d^N_d
+ * scale_d when mod_n_d = N_d >= 0 (mod_n_d < 0: no modulation, scale_d
+ * still applies) -- the values of a cxm_czt_axis_c64 pass along each axis.
+ * Computed as two complex GEMMs (X A_x^T, then A_y (X A_x^T)) on FP32
+ * matrix cores (v_mfma_f32_16x16x4f32); the matrices are built in FP64.
+ * For small output windows this contraction (8 (m n^2 + m^2 n) flops per
+ * plane) is faster than Bluestein's three FFTs per line.  Replaces cztn /
+ * zoomed_fft (src/chromatix/utils/czt.py:55-124) and the CZT pair of
+ * kernel_propagate (src/chromatix/functional/propagation.py:427-441) for
+ * such windows.
+ * in : batch x height x width complex64;  out: batch x m_y x m_x complex64.
+ */
+typedef struct cxm_zoom_desc {
+  int32_t batch, height, width, m_y, m_x;
+  int32_t mod_n_y, mod_n_x;   /* output modulation index N per axis, < 0: none */
+  int32_t reserved;
+  double a_y[2], w_y[2];      /* complex (re, im) CZT parameters along the height */
+  double a_x[2], w_x[2];      /* along the width */
+  double scale_y, scale_x;    /* per-axis output scales */
+} cxm_zoom_desc;
+
+size_t cxm_zoom_dft_workspace_size(const cxm_zoom_desc* d);
+int cxm_zoom_dft_mfma_c64(void* stream, const void* in, void* out, void* workspace,
+                          size_t workspace_bytes, const cxm_zoom_desc* d);
+
+/*
+ * Batched complex64 GEMM on FP32 matrix cores, C[z] = A[z] B[z] (row-major,
+ * leading dimensions in elements, batch strides in elements, 0 = shared),
+ * the contraction of cxm_zoom_dft_mfma_c64.  No counterpart in the
+ * reference (its matrix products are XLA dots); exposed for callers with
+ * their own DFT matrices.
+ */
+int cxm_cgemm_mfma_c64(void* stream, const void* A, const void* B, void* C, int32_t M, int32_t N,
+                       int32_t K, int64_t lda, int64_t ldb, int64_t ldc, int64_t stride_a,
+                       int64_t stride_b, int64_t stride_c, int32_t batch);
+
+/*
  * The scaled-convolution branch of kernel_propagate (Yu et al. 2012; used
  * when output_dx or output_shape is given and use_czt is False,
  * src/chromatix/functional/propagation.py:444-458), one axis of the

@@ -41,7 +41,8 @@ def test_every_declared_symbol_is_exported():
                                        (L.YuDesc, "cxm_yu_desc"),
                                        (L.FluorDesc, "cxm_fluor_desc"),
                                        (L.MbsDesc, "cxm_mbs_desc"),
-                                       (L.FconvDesc, "cxm_fconv_desc")])
+                                       (L.FconvDesc, "cxm_fconv_desc"),
+                                       (L.ZoomDesc, "cxm_zoom_desc")])
 def test_descriptor_layout_matches_header(tmp_path, cls, cname):
     fields = [f for f, _ in cls._fields_]
     src = tmp_path / "layout.c"

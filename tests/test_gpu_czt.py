@@ -53,14 +53,20 @@ def test_czt_generic_axis_and_parameter_sets():
         assert rel_l2(got[p], ref) < TOL
 
 
-def test_zoomed_fft():
+@pytest.mark.parametrize("path", ["mfma", "bluestein"])
+def test_zoomed_fft(path, monkeypatch):
+    monkeypatch.setenv("CXM_ZOOM_PATH", path)
     x = _x((64, 48), 3)
     got = uc.zoomed_fft(torch.as_tensor(x).cuda(), 0.4, 2.1, (40, 30)).cpu().numpy()
     assert rel_l2(got, oc.zoomed_fft(x, 0.4, 2.1, (40, 30))) < TOL
 
 
-@pytest.mark.parametrize("n,m,odx,shift", [(256, 64, 0.2, (3.0, -1.5)), (2048, 512, 0.35, (0.0, 0.0))])
-def test_asm_propagate_czt_output_grid(n, m, odx, shift):
+@pytest.mark.parametrize("path", ["default", "mfma", "bluestein"])
+@pytest.mark.parametrize("n,m,odx,shift", [(256, 64, 0.2, (3.0, -1.5)), (2048, 512, 0.35, (0.0, 0.0)),
+                                           (2048, 64, 0.35, (1.0, -2.0))])
+def test_asm_propagate_czt_output_grid(n, m, odx, shift, path, monkeypatch):
+    if path != "default":
+        monkeypatch.setenv("CXM_ZOOM_PATH", path)
     rng = np.random.default_rng(4)
     u = np.exp(2j * np.pi * rng.random((n, n))).astype(np.complex64)
     f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.5, 0.0, 0.532)
@@ -124,7 +130,9 @@ def test_asm_propagate_scaled_convolution_same_mode():
     # reference's formula is unbounded (NA 1.2 puts bin (256, 192) on it)
     (512, (200, 200), 0.15, 2000.0, 1.08),
 ])
-def test_high_na_ff_lens(n, out_shape, odx, f_len, NA):
+@pytest.mark.parametrize("path", ["mfma", "bluestein"])
+def test_high_na_ff_lens(n, out_shape, odx, f_len, NA, path, monkeypatch):
+    monkeypatch.setenv("CXM_ZOOM_PATH", path)
     rng = np.random.default_rng(n)
     u = np.exp(2j * np.pi * rng.random((n, n))).astype(np.complex64)
     f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, 0.532)
