@@ -44,12 +44,13 @@ def sources() -> list[Path]:
 
 
 # Column-pass instantiations that are known to spill: the long columns
-# (2048-8192, 640-5120 with 20 values per thread, 3072 with 4 columns per
+# (2048-8192, 640-10240 with 20 values per thread, 3072 with 4 columns per
 # workgroup) whose H evaluation and transform do not fit the register
 # budget, and the 1024-point 8-column launch-shape variants for several
 # distances.  Anything else that spills fails the build.
 _KNOWN_SPILLS = re.compile(
-    r"col_prop_kernelILi(640|1280|2048|2560|3072|4096|5120|8192)E"
+    r"col_prop_kernelILi(640|1280|2048|2560|3072|4096|5120|8192|10240)E"
+    r"|pipe_row_phasegrad_kernelILi10240E"
     r"|col_prop_kernelILi1024ELi(8|16)ELi8E.*Lb1E")
 
 

@@ -122,7 +122,7 @@ CXM_DEV int fshift_dst(int j, int n) { int d = j + (n >> 1); return d >= n ? d -
 // resident).  The tables are concatenated in the order of kTwLengths.
 namespace cxm {
 constexpr int kTwLengths[] = {2,  4,  8,  16,  32,  64,  128,  256,  512,  1024, 2048, 4096, 8192,
-                              5,  10, 20, 40,  80,  160, 320,  640,  1280, 2560, 5120,
+                              5,  10, 20, 40,  80,  160, 320,  640,  1280, 2560, 5120, 10240,
                               3,  6,  12, 24,  48,  96,  192,  384,  768,  1536, 3072};
 constexpr int kTwCount = (int)(sizeof(kTwLengths) / sizeof(kTwLengths[0]));
 constexpr int tw_offset(int N) {

@@ -168,10 +168,14 @@ struct Geom {
   int chunk;                                 // input planes per chunk
   int W;                                     // column-pass strip width
   bool tiled;                                // workspace planes in strips of kTileW columns
+  bool generic;                              // a grid length without a fused FFT: chirp-z passes
+  long long gen_plane;                       // generic path: workspace elements per input plane
+  size_t gen_czt_ws;                         // generic path: bytes of the chirp-z workspace
 };
 
 bool use_h_table(const cxm_prop2d_desc* d);
 size_t table_elems(const Geom& g);
+int generic_geom(const cxm_prop2d_desc* d, Geom& g, size_t limit_bytes);
 
 // limit_bytes > 0: the chunk is the largest whose workspace (planes and the
 // H table) fits in limit_bytes; CXM_ERR_UNSUPPORTED when one plane does not.
@@ -205,10 +209,9 @@ int validate(const cxm_prop2d_desc* d, Geom& g, size_t limit_bytes = 0) {
   g.Hout = g.Ny - 2 * d->crop_y;
   g.Wout = g.Nx - 2 * d->crop_x;
   if (g.Hout <= 0 || g.Wout <= 0) return fail(CXM_ERR_INVALID, "crop removes the whole plane");
-  if (!fft_len_supported(g.Ny) || !fft_len_supported(g.Nx))
-    return fail(CXM_ERR_UNSUPPORTED, "unsupported FFT grid " + std::to_string(g.Ny) + "x" +
-                                         std::to_string(g.Nx) + " (lengths 2^a <= 8192, 5*2^a <= 5120, 3*2^a <= 3072)");
+  g.generic = !fft_len_supported(g.Ny) || !fft_len_supported(g.Nx);
   g.plane_in = (long long)g.Hin * g.Win;
+  if (g.generic) return generic_geom(d, g, limit_bytes);
   // Every workspace plane (column-pass input and output alike) has
   // max(Hin, Hout) rows, so that with one distance the column pass works in
   // place: a strip's output rows lie where its input rows were.
@@ -250,8 +253,12 @@ bool one_param_set(const cxm_prop2d_desc* d) {
 
 // The persistent column pass (H per strip in registers) for such a call,
 // where the column length has one and no launch-shape option is set.
+bool fused_grid(const cxm_prop2d_desc* d) {
+  return fft_len_supported(d->height + 2 * d->pad_y) && fft_len_supported(d->width + 2 * d->pad_x);
+}
+
 bool use_persist(const cxm_prop2d_desc* d) {
-  if (!one_param_set(d) || d->batch < CXM_H_TABLE_MIN_PLANES) return false;
+  if (!fused_grid(d) || !one_param_set(d) || d->batch < CXM_H_TABLE_MIN_PLANES) return false;
   if (option(CXM_OPT_COL_PERSIST) == 1 || option(CXM_OPT_H_TABLE) != 0 ||
       option(CXM_OPT_COL_VARIANT) != 0 || option(CXM_OPT_COL_ORDER) != 0)
     return false;
@@ -261,7 +268,7 @@ bool use_persist(const cxm_prop2d_desc* d) {
 // Otherwise one H table for the batch, evaluated once per call and read by
 // the per-plane column pass (CXM_OPT_H_TABLE overrides).
 bool use_h_table(const cxm_prop2d_desc* d) {
-  if (!one_param_set(d) || use_persist(d)) return false;
+  if (!fused_grid(d) || !one_param_set(d) || use_persist(d)) return false;
   const int o = option(CXM_OPT_H_TABLE);
   if (o == 1) return false;
   return o == 2 || d->batch >= CXM_H_TABLE_MIN_PLANES;
@@ -277,6 +284,7 @@ size_t table_elems(const Geom& g) {  // whole strips of W columns
 }
 
 size_t ws_elems(const cxm_prop2d_desc* d, const Geom& g, bool table) {
+  if (g.generic) return (size_t)g.chunk * g.gen_plane + (g.gen_czt_ws + 7) / 8;
   return chunk_elems(d, g) + (table ? table_elems(g) : 0);
 }
 }  // namespace
@@ -286,6 +294,189 @@ extern "C" size_t cxm_prop2d_workspace_size(const cxm_prop2d_desc* d) {
   if (validate(d, g) != CXM_OK) return 0;
   return ws_elems(d, g, use_h_table(d)) * sizeof(float2);
 }
+
+// ---------------------------------------------------------------------------
+// Grids with a length that has no fused FFT (any length the reference's
+// jnp.fft accepts, e.g. the 4604- or 1856-point grids of its padding
+// calculators, propagation.py:583-656).  The 2D DFT pair runs as chirp-z
+// passes along each axis (cxm_czt_axis_c64, Bluestein with FP64 chirps):
+//   forward, rows then columns:  X_k = sum_j x_j w^(j k), w = e^(-2 pi i / N),
+//     over the unpadded input samples only (the zero padding adds nothing);
+//   H(ky, kx) evaluated per bin (the functions of the fused column pass);
+//   inverse, columns then rows:  y_j = sum_k Z_k a^(-k) w^(-k j), j < N - 2 c,
+//     a = e^(-2 pi i (c - p) / N): the pad offset p of the forward sum and the
+//     crop offset c of the output, both a factor w^(+-k .) per bin, meet in a.
+// The 1/(Ny Nx) of the inverse is part of H's scale, as in the fused passes.
+// Lengths are limited by the chirp-z FFT: n + N - 1 per axis must fit the
+// largest supported transform (10240).
+// ---------------------------------------------------------------------------
+namespace cxm {
+template <int KIND, int MODE>
+__global__ void __launch_bounds__(256)
+apply_H_kernel(const float2* __restrict__ in, float2* __restrict__ out, long long plane, int Ny,
+               int Nx, int iz, ColArgs a) {
+  const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+  const int p = blockIdx.y;
+  if (i >= plane) return;
+  const int ky = (int)(i / Nx), kx = (int)(i - (long long)ky * Nx);
+  const float2 v = in[p * plane + i];
+  float2 h;
+  if constexpr (KIND == CXM_H_TRANSFER || KIND == CXM_H_ASM) {
+    const int pg = a.plane0 + p;
+    const int w = (a.nwv > 1) ? (pg % a.nwv) : 0;
+    const int zi = a.z_zip ? (pg / a.nwv) % a.nz : iz;
+    const HParam& hp = a.hp[w * a.nz + zi];
+    h = eval_H(hp, x_terms(hp, kx, Nx, MODE), ky, Ny, KIND, MODE, a.remove_evanescent,
+               a.bandlimit, a.scale);
+  } else if constexpr (KIND == CXM_H_ARRAY) {
+    h = cscale(a.H[i], a.scale);
+  } else {
+    h = make_float2(a.scale, 0.f);
+  }
+  out[p * plane + i] = cmul(v, h);
+}
+}  // namespace cxm
+
+namespace {
+// Chirp-z passes of one chunk: 0 rows forward, 1 columns forward,
+// 2 columns inverse, 3 rows inverse.
+struct GenCzt {
+  double a[4][2], w[4][2];
+  cxm_czt_desc c[4];
+};
+
+void gen_czt(const cxm_prop2d_desc* d, const Geom& g, int kc, GenCzt& z) {
+  const double tau = 2.0 * M_PI;
+  auto set = [&](int i, int h, int w, int axis, int m, double a_ang, double w_ang) {
+    z.a[i][0] = std::cos(a_ang);
+    z.a[i][1] = std::sin(a_ang);
+    z.w[i][0] = std::cos(w_ang);
+    z.w[i][1] = std::sin(w_ang);
+    memset(&z.c[i], 0, sizeof(z.c[i]));
+    z.c[i].batch = kc;
+    z.c[i].height = h;
+    z.c[i].width = w;
+    z.c[i].axis = axis;
+    z.c[i].m = m;
+    z.c[i].num_param = 1;
+    z.c[i].mod_n = -1;
+    z.c[i].a = z.a[i];
+    z.c[i].w = z.w[i];
+    z.c[i].scale = 1.0;
+  };
+  set(0, g.Hin, g.Win, 1, g.Nx, 0.0, -tau / g.Nx);
+  set(1, g.Hin, g.Nx, 0, g.Ny, 0.0, -tau / g.Ny);
+  set(2, g.Ny, g.Nx, 0, g.Hout, -tau * (d->crop_y - d->pad_y) / g.Ny, tau / g.Ny);
+  set(3, g.Hout, g.Nx, 1, g.Wout, -tau * (d->crop_x - d->pad_x) / g.Nx, tau / g.Nx);
+}
+
+int generic_geom(const cxm_prop2d_desc* d, Geom& g, size_t limit_bytes) {
+  if (limit_bytes)
+    return fail(CXM_ERR_UNSUPPORTED, "FFT grid " + std::to_string(g.Ny) + "x" +
+                                         std::to_string(g.Nx) +
+                                         " has no fused transform; the allocation-free entry "
+                                         "covers fused grids only");
+  const int nout = d->z_zip ? 1 : d->num_z;
+  const bool inten = d->out_kind == CXM_OUT_INTENSITY;
+  g.gen_plane = (long long)g.Hin * g.Nx + (long long)g.Ny * g.Nx * (nout > 1 ? 2 : 1) +
+                (long long)g.Hout * g.Nx + (inten ? (long long)g.Hout * g.Wout : 0);
+  const long long budget = (256ll << 20) / 8;
+  g.chunk = nout > 1 ? 1
+                     : (int)std::max(1ll, std::min<long long>(std::max(1, d->batch),
+                                                              budget / g.gen_plane));
+  g.W = 0;
+  g.tiled = false;
+  g.plane_ws = 0;
+  GenCzt z;
+  gen_czt(d, g, g.chunk, z);
+  g.gen_czt_ws = 0;
+  for (int i = 0; i < 4; ++i) {
+    const size_t b = cxm_czt_axis_workspace_size(&z.c[i]);
+    if (b == 0)
+      return fail(CXM_ERR_UNSUPPORTED,
+                  "FFT grid " + std::to_string(g.Ny) + "x" + std::to_string(g.Nx) +
+                      ": the chirp-z passes of a grid without a fused transform need "
+                      "n + N - 1 <= 10240 per axis");
+    g.gen_czt_ws = std::max(g.gen_czt_ws, b);
+  }
+  return CXM_OK;
+}
+
+int launch_apply_H(const float2* in, float2* out, int kc, const Geom& g, int iz, const ColArgs& a,
+                   hipStream_t s) {
+  const long long plane = (long long)g.Ny * g.Nx;
+  const dim3 grid((unsigned)((plane + 255) / 256), (unsigned)kc), block(256);
+#define CXM_AH(K, M) hipLaunchKernelGGL((apply_H_kernel<K, M>), grid, block, 0, s, in, out, plane, g.Ny, g.Nx, iz, a)
+  if (a.kind == CXM_H_TRANSFER) {
+    if (a.phase_mode == CXM_PHASE_REF32) CXM_AH(CXM_H_TRANSFER, CXM_PHASE_REF32);
+    else CXM_AH(CXM_H_TRANSFER, CXM_PHASE_ACCURATE);
+  } else if (a.kind == CXM_H_ASM) {
+    if (a.phase_mode == CXM_PHASE_REF32) CXM_AH(CXM_H_ASM, CXM_PHASE_REF32);
+    else CXM_AH(CXM_H_ASM, CXM_PHASE_ACCURATE);
+  } else if (a.kind == CXM_H_ARRAY) {
+    CXM_AH(CXM_H_ARRAY, 0);
+  } else {
+    CXM_AH(CXM_H_NONE, 0);
+  }
+#undef CXM_AH
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+int generic_impl(void* stream, const void* in, void* out, void* workspace,
+                 const cxm_prop2d_desc* d, const Geom& g) {
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  ColArgs ca;
+  memset(&ca, 0, sizeof(ca));
+  if (d->kind == CXM_H_TRANSFER || d->kind == CXM_H_ASM) {
+    if ((rc = build_hparams(d, g.Ny, g.Nx, ca.hp, kMaxHP))) return rc;
+    ca.nhp = (d->num_wv > 0 ? d->num_wv : 1) * d->num_z;
+  }
+  ca.H = (const float2*)d->propagator;
+  ca.nz = d->num_z;
+  ca.z_zip = d->z_zip;
+  ca.nwv = d->num_wv > 0 ? d->num_wv : 1;
+  ca.kind = d->kind;
+  ca.phase_mode = d->phase_mode;
+  ca.remove_evanescent = d->remove_evanescent;
+  ca.bandlimit = d->bandlimit;
+  ca.scale = (float)(1.0 / ((double)g.Ny * (double)g.Nx));
+  const int nout = out_per_plane(d);
+  const bool inten = d->out_kind == CXM_OUT_INTENSITY;
+  const long long plane_f = (long long)g.Ny * g.Nx;
+  const long long out_plane = (long long)g.Hout * g.Wout;
+  float2* A1 = (float2*)workspace;
+  float2* S = A1 + (size_t)g.chunk * g.Hin * g.Nx;
+  float2* T = S + (size_t)g.chunk * plane_f;
+  float2* U = nout > 1 ? T + (size_t)g.chunk * plane_f : T;
+  float2* V = U + (size_t)g.chunk * g.Hout * g.Nx;
+  void* cws = (void*)(A1 + (size_t)g.chunk * g.gen_plane);
+  for (int p0 = 0; p0 < d->batch; p0 += g.chunk) {
+    const int kc = std::min(g.chunk, d->batch - p0);
+    GenCzt z;
+    gen_czt(d, g, kc, z);
+    const float2* src = (const float2*)in + (size_t)p0 * g.plane_in;
+    if ((rc = cxm_czt_axis_c64(s, src, A1, cws, g.gen_czt_ws, &z.c[0]))) return rc;
+    if ((rc = cxm_czt_axis_c64(s, A1, S, cws, g.gen_czt_ws, &z.c[1]))) return rc;
+    ca.plane0 = p0;
+    for (int iz = 0; iz < nout; ++iz) {
+      float2* Z = nout > 1 ? T : S;
+      if ((rc = launch_apply_H(S, Z, kc, g, iz, ca, s))) return rc;
+      if ((rc = cxm_czt_axis_c64(s, Z, U, cws, g.gen_czt_ws, &z.c[2]))) return rc;
+      const size_t o = ((size_t)p0 * nout + iz) * out_plane;
+      if (inten) {
+        if ((rc = cxm_czt_axis_c64(s, U, V, cws, g.gen_czt_ws, &z.c[3]))) return rc;
+        if ((rc = cxm_intensity_c64(s, V, (float*)out + o, (int64_t)kc * out_plane, 1, nullptr)))
+          return rc;
+      } else if ((rc = cxm_czt_axis_c64(s, U, (float2*)out + o, cws, g.gen_czt_ws, &z.c[3]))) {
+        return rc;
+      }
+    }
+  }
+  return CXM_OK;
+}
+}  // namespace
 
 namespace {
 // noalloc: the workspace is the library's static scratch array (workspace
@@ -302,6 +493,16 @@ int prop2d_impl(void* stream, const void* in, void* out, void* workspace, size_t
   const size_t need = ws_elems(d, g, table) * sizeof(float2);
   if (!noalloc && (!workspace || workspace_bytes < need))
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  if (g.generic) {
+    if (noalloc)
+      return fail(CXM_ERR_UNSUPPORTED, "grid lengths without a fused FFT are not available "
+                                       "without a workspace");
+    if (hp_dev) return fail(CXM_ERR_UNSUPPORTED, "a parameter table needs a fused FFT grid");
+    if (!workspace || workspace_bytes < need)
+      return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+    if ((rc = ensure_init())) return rc;
+    return generic_impl(stream, in, out, workspace, d, g);
+  }
   if ((rc = ensure_init())) return rc;
   if (noalloc) {
     workspace = noalloc_scratch();

@@ -16,7 +16,7 @@
 #define CXM_SIZES_G2(X) X(2048, 16, 16, 4) X(4096, 16, 32, 4) X(8192, 32, 32, 2)
 #define CXM_SIZES_G3(X)                                                       \
   X(320, 20, 20, 16) X(640, 20, 20, 8) X(1280, 20, 20, 4) X(2560, 20, 20, 4) \
-  X(5120, 20, 20, 2)
+  X(5120, 20, 20, 2) X(10240, 20, 20, 1)
 
 #define CXM_SIZES_G4(X)                                                       \
   X(3, 3, 3, 64) X(6, 6, 6, 64) X(12, 12, 12, 64) X(24, 12, 12, 64)          \

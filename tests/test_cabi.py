@@ -79,11 +79,18 @@ def test_validation_errors_without_device():
     lib = L.lib()
     d, keep = _desc()
     assert lib.cxm_prop2d_workspace_size(ctypes.byref(d)) > 0
-    bad, keep2 = _desc(height=100)  # 100 is not an implemented length
+    # a length without a fused transform runs as chirp-z passes
+    gen, keep2 = _desc(height=100)
+    assert lib.cxm_prop2d_workspace_size(ctypes.byref(gen)) > 0
+    # ... up to n + N - 1 per axis within the largest transform (10240)
+    bad, keep3 = _desc(height=6000)
     assert lib.cxm_prop2d_workspace_size(ctypes.byref(bad)) == 0
     rc = lib.cxm_prop2d_c64(None, None, None, None, 0, ctypes.byref(bad))
     assert rc == L.CXM_ERR_UNSUPPORTED
-    assert b"unsupported FFT grid" in lib.cxm_last_error()
+    assert b"chirp-z passes" in lib.cxm_last_error()
+    # and not without a workspace
+    rc = lib.cxm_prop2d_c64_noalloc(None, ctypes.c_void_p(16), ctypes.c_void_p(16), ctypes.byref(gen))
+    assert rc == L.CXM_ERR_UNSUPPORTED
     for kw in (dict(num_z=0), dict(kind=9), dict(phase_mode=5), dict(n=-1.0), dict(pad_y=-1),
                dict(crop_x=40), dict(num_wv=8, num_z=8), dict(z_zip=3)):
         dd, k3 = _desc(**kw)
@@ -97,9 +104,10 @@ def test_validation_errors_without_device():
 
 def test_fft_length_support():
     lib = L.lib()
-    for n in [2, 4, 16, 64, 256, 1024, 4096, 8192, 5, 160, 1280, 2560, 5120, 3, 96, 1536, 3072]:
+    for n in [2, 4, 16, 64, 256, 1024, 4096, 8192, 5, 160, 1280, 2560, 5120, 10240, 3, 96, 1536,
+              3072]:
         assert lib.cxm_fft_supported(n) == 1, n
-    for n in [7, 9, 100, 1000, 2559, 6144, 10240, 16384]:
+    for n in [7, 9, 100, 1000, 2559, 6144, 20480, 16384]:
         assert lib.cxm_fft_supported(n) == 0, n
 
 
@@ -127,7 +135,7 @@ def test_validation_errors_of_the_imaging_and_born_entries():
     d.mode = 2
     assert lib.cxm_fconv_workspace_size(ctypes.byref(d)) == 0
     assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_INVALID
-    d.mode, d.height = 0, 9000  # beyond the implemented transform lengths
+    d.mode, d.height = 0, 12000  # beyond the implemented transform lengths
     assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_UNSUPPORTED
     d.height, d.mode = 40, 1  # full mode needs its output shape
     assert lib.cxm_fconv_c64(None, None, None, None, None, 0, ctypes.byref(d)) == L.CXM_ERR_INVALID

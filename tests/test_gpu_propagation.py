@@ -297,9 +297,9 @@ def test_tiled_intermediates_are_bit_identical(shape, pad, mode, batch):
 
 
 def test_errors_are_raised():
-    f = _mono(_phase_field((100, 100)), 0.3)
+    f = _mono(_phase_field((6000, 8)), 0.3)
     with pytest.raises(L.CxmError):
-        cf.asm_propagate(f, 1.0, NM, 0)  # 100 is not an implemented FFT length
+        cf.asm_propagate(f, 1.0, NM, 0)  # 6000 + 6000 - 1 exceeds the chirp-z transform
     with pytest.raises(ValueError):
         cf.asm_propagate(_mono(_phase_field((8, 8), batch=(3,)), 0.3), [1.0, 2.0], NM, 0)
 
@@ -383,3 +383,87 @@ def test_persistent_column_pass_is_bit_identical(method, shape, pad, batch, opts
     ref = _oracle(u[batch - 1], dx, z, pad, method, "same",
                   "ref32" if phase_mode == "ref32" else "fp64", **opts)
     assert rel_l2(outs[0][batch - 1], ref) < TOL
+
+
+# ---- grids of any length: chirp-z passes ------------------------------------------
+@pytest.mark.parametrize("phase_mode", ["ref32", "accurate"])
+@pytest.mark.parametrize("method,shape,pad,mode,z", [
+    ("asm", (100, 100), 0, "same", 10.0),
+    ("transfer", (100, 100), 0, "full", -7.0),
+    ("asm", (90, 70), (7, 13), "same", 25.0),     # 104 x 96: one axis fused-length, one not
+    ("asm", (60, 44), (3, 5), "full", 12.0),
+])
+def test_arbitrary_grid_lengths(method, shape, pad, mode, z, phase_mode):
+    """Grid lengths without a fused transform (100, 104, 66, 54, ...) run the
+    2D DFT pair as chirp-z passes; same results as the oracle's exact FFTs."""
+    u = _phase_field(shape, seed=21)
+    got = _run(u, 0.3, z, pad, method, mode, phase_mode)
+    ref = _oracle(u, 0.3, z, pad, method, mode, "ref32" if phase_mode == "ref32" else "fp64")
+    assert got.shape == ref.shape
+    err = rel_l2(got, ref)
+    print(f"{method} {shape} pad {pad} {mode}: {err:.2e}")
+    assert err < TOL
+
+
+@pytest.mark.parametrize("px,z,calc,method", [
+    (512, 100.0, "exact", "asm"),        # 4604-point grid
+    (512, 100.0, "transfer", "transfer"),  # 1856
+    (256, 100.0, "exact", "asm"),        # 4860
+    (1024, 50.0, "exact", "asm"),        # 7836
+])
+def test_reference_padding_calculator_grids(px, z, calc, method):
+    """The grids the reference's own padding calculators produce
+    (compute_padding_exact / compute_padding_transfer, propagation.py:583-656)
+    for a 0.3 um, 532 nm field, propagated in "same" mode."""
+    fn = cf.compute_padding_exact if calc == "exact" else cf.compute_padding_transfer
+    pad = fn(px, LAM, 0.3, z)
+    print(f"{px} px z {z}: pad {pad} -> grid {px + 2 * pad}", flush=True)
+    assert not L.lib().cxm_fft_supported(px + 2 * pad)
+    u = _phase_field((px, px), seed=22)
+    got = _run(u, 0.3, z, pad, method, "same", "accurate")
+    ref = op.propagate(u, (0.3, 0.3), LAM, z, NM, pad, method=method, mode="same",
+                       precision="fp64", workers=16)
+    err = rel_l2(got, ref)
+    print(f"  rel_l2 vs FP64 = {err:.2e}")
+    assert err < TOL
+
+
+def test_arbitrary_grid_batches_distances_and_wavelengths():
+    zs = np.array([-20.0, 5.0, 12.5])
+    u = _phase_field((100, 60), seed=23, batch=(3,))
+    out = cf.asm_propagate(_mono(u, 0.3), zs, NM, (9, 3), mode="same")  # zip: plane b at zs[b]
+    for b in range(3):
+        rb = _oracle(u[b], 0.3, zs[b], (9, 3), "asm", "same", "ref32")
+        assert rel_l2(out.u[b].cpu().numpy(), rb) < TOL
+    ub = _phase_field((100, 60), seed=24)
+    o2 = cf.asm_propagate(_mono(ub, 0.3), zs, NM, (9, 3), mode="same")           # outer
+    assert o2.shape == (3, 100, 60)
+    for k, zk in enumerate(zs):
+        assert rel_l2(o2.u[k].cpu().numpy(), _oracle(ub, 0.3, zk, (9, 3), "asm", "same", "ref32")) < TOL
+    wl = np.array([0.45, 0.65])
+    uc = np.exp(2j * np.pi * np.random.default_rng(25).random((70, 50, 2))).astype(np.complex64)
+    fc = cx.ChromaticScalarField(torch.as_tensor(uc).cuda(), 0.3, 0.0, cx.Spectrum(wl))
+    oc = cf.transfer_propagate(fc, 15.0, NM, 5, mode="same").u.cpu().numpy()
+    for w in range(2):
+        r = op.propagate(uc[..., w], (0.3, 0.3), wl[w], 15.0, NM, 5, method="transfer",
+                         precision="ref32", fft_precision="fp64")
+        assert rel_l2(oc[..., w], r) < TOL
+    # intensity output of the same grid
+    I = cf.propagate_intensity(_mono(ub, 0.3), 12.5, NM, (9, 3))
+    ref = np.abs(_oracle(ub, 0.3, 12.5, (9, 3), "asm", "same", "ref32")) ** 2
+    assert rel_l2(I.cpu().numpy().reshape(ref.shape), ref) < TOL
+
+
+
+def test_arbitrary_grid_adjoint_identity():
+    """<P u, c> = <u, P^H c> on a 100 x 90 grid (chirp-z passes both ways)."""
+    rng = np.random.default_rng(26)
+    u = torch.as_tensor(_phase_field((80, 70), seed=27)).cuda().requires_grad_(True)
+    c = torch.as_tensor((rng.standard_normal((80, 70)) + 1j * rng.standard_normal((80, 70)))
+                        .astype(np.complex64)).cuda()
+    f = cx.ScalarField(u, 0.3, 0.0, LAM)
+    out = cf.asm_propagate(f, 30.0, NM, (10, 10), mode="same", remove_evanescent=True)
+    lhs = torch.vdot(out.u.reshape(-1), c.reshape(-1))
+    (g,) = torch.autograd.grad((out.u * c.conj()).real.sum(), u)
+    rhs = torch.vdot(u.detach().reshape(-1), g.reshape(-1))
+    assert abs((lhs - rhs).item()) / abs(lhs.item()) < 1e-5
