@@ -315,6 +315,8 @@ def _declare(lib):
         "cxm_cgemm_mfma_c64": (i32, [vp, vp, vp, vp, i32, i32, i32, ctypes.c_int64, ctypes.c_int64,
                                      ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
                                      ctypes.c_int64, i32]),
+        "cxm_modulation_vjp_c64": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int64, i32,
+                                         i32, i32, dp, ctypes.c_double]),
         "cxm_noalloc_scratch_bytes": (sz, []),
         "cxm_prop2d_c64_noalloc": (i32, [vp, vp, vp, ctypes.POINTER(Prop2dDesc)]),
         "cxm_prop2d_param_bytes": (sz, []),

@@ -410,6 +410,63 @@ __global__ void thin_sample_kernel(const float2* __restrict__ in, const float* _
   out[i] = cmul(in[i], make_float2(m * c, m * s));
 }
 
+// The modulation factor of phase_change (kind 0: exp(i fl(phi m_w))) or
+// thin_sample (kind 1: exp(i 2 pi (dn + i abs) t / lam_w)) at element i, as
+// the forward kernels evaluate it.
+CXM_DEV float2 modulation_at(int kind, const float* a, const float* b, long long q, int w,
+                             const Modulation& m, const Wavelengths& wl, float t) {
+  float s, c;
+  if (kind == 0) {
+    sincos_exact(__fmul_rn(a[q], m.m[w]), &s, &c);
+    return make_float2(c, s);
+  }
+  const float d = a[q], ab = b ? b[q] : 0.0f, lam = wl.lam[w];
+  const float re = __fdiv_rn(__fmul_rn(-__fmul_rn(kTwoPiF, ab), t), lam);
+  const float im = __fdiv_rn(__fmul_rn(__fmul_rn(kTwoPiF, d), t), lam);
+  sincos_exact(im, &s, &c);
+  const float mg = (re == 0.0f) ? 1.0f : expf(re);
+  return make_float2(mg * c, mg * s);
+}
+
+// grad_u[i] = g[i] conj(T[i]) (the input cotangent of out = in T)
+__global__ void modulation_vjp_u_kernel(int kind, const float2* __restrict__ g,
+                                        const float* __restrict__ a, const float* __restrict__ b,
+                                        float2* __restrict__ gu, long long n, long long plane_px,
+                                        Modulation m, Wavelengths wl, float t) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long px = i / m.nwv;
+  const int w = (int)(i - px * m.nwv);
+  const float2 T = modulation_at(kind, a, b, px % plane_px, w, m, wl, t);
+  const float2 v = g[i];
+  gu[i] = make_float2(v.x * T.x + v.y * T.y, v.y * T.x - v.x * T.y);
+}
+
+// Per pixel q of the (H, W) parameter arrays, over every plane and
+// wavelength in order: ga[q] = sum k_w Im(g conj(y)), gb[q] = -sum k_w
+// Re(g conj(y)), with k_w = m_w (phase) or 2 pi t / lam_w (thin sample):
+// d out / d phi = i m_w out, d out / d dn = i k_w out, d out / d abs = -k_w out.
+__global__ void modulation_vjp_param_kernel(const float2* __restrict__ g,
+                                            const float2* __restrict__ y, float* __restrict__ ga,
+                                            float* __restrict__ gb, long long n_planes,
+                                            long long plane_px, Modulation k) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= plane_px) return;
+  double si = 0.0, sr = 0.0;
+  for (long long p = 0; p < n_planes; ++p) {
+    const long long base = (p * plane_px + q) * k.nwv;
+    for (int w = 0; w < k.nwv; ++w) {
+      const float2 gv = g[base + w], yv = y[base + w];
+      // g conj(y)
+      const float zr = gv.x * yv.x + gv.y * yv.y, zi = gv.y * yv.x - gv.x * yv.y;
+      si += (double)k.m[w] * zi;
+      sr += (double)k.m[w] * zr;
+    }
+  }
+  if (ga) ga[q] = (float)si;
+  if (gb) gb[q] = (float)(-sr);
+}
+
 // out[p, w] = 2 density[w] g[p] u[p, w]: the vector-Jacobian product of the
 // density-weighted intensity
 __global__ void intensity_vjp_kernel(const float2* __restrict__ u, const float* __restrict__ g,
@@ -532,5 +589,54 @@ extern "C" int cxm_phase_change_c64(void* stream, const void* in, const float* p
   hipLaunchKernelGGL(phase_change_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, (const float2*)in, phase, (float2*)out, n, plane_px, m);
   CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+// Vector-Jacobian product of cxm_phase_change_c64 (kind 0) and
+// cxm_thin_sample_c64 (kind 1): with g the cotangent of out (dL/dRe + i
+// dL/dIm) and y = out the forward result,
+//   grad_in = g conj(T),  grad_a = sum k_w Im(g conj y),  grad_b = -sum k_w Re(g conj y),
+// where a is the phase (k_w = modulation) or dn (k_w = 2 pi t / lam_w) and b
+// the absorption.  Any output pointer may be null.  params: modulation per
+// wavelength (kind 0) or wavelengths (kind 1).
+extern "C" int cxm_modulation_vjp_c64(void* stream, int32_t kind, const void* g, const void* y,
+                                      const float* a, const float* b, void* grad_in,
+                                      float* grad_a, float* grad_b, int64_t n_planes,
+                                      int32_t height, int32_t width, int32_t num_wv,
+                                      const double* params, double thickness) {
+  if (kind != 0 && kind != 1) return fail(CXM_ERR_INVALID, "kind must be 0 (phase) or 1 (thin)");
+  if (n_planes < 0 || height <= 0 || width <= 0) return fail(CXM_ERR_INVALID, "invalid shape");
+  if (num_wv < 1 || num_wv > kMaxWvPhase) return fail(CXM_ERR_INVALID, "num_wv must be in [1, 64]");
+  if (!g || !a || (!y && (grad_a || grad_b)) || (kind == 1 && !params))
+    return fail(CXM_ERR_INVALID, "null buffer");
+  Modulation m, k;
+  Wavelengths wl;
+  m.nwv = k.nwv = wl.nwv = num_wv;
+  for (int w = 0; w < kMaxWvPhase; ++w) m.m[w] = k.m[w] = wl.lam[w] = 1.0f;
+  for (int w = 0; w < num_wv; ++w) {
+    if (kind == 0) {
+      m.m[w] = k.m[w] = params ? (float)params[w] : 1.0f;
+    } else {
+      if (!(params[w] > 0)) return fail(CXM_ERR_INVALID, "wavelength must be positive");
+      wl.lam[w] = (float)params[w];
+      k.m[w] = (float)(2.0 * M_PI * thickness / params[w]);
+    }
+  }
+  const long long plane_px = (long long)height * width;
+  const long long n = n_planes * plane_px * num_wv;
+  if ((n + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "array too large");
+  hipStream_t s = (hipStream_t)stream;
+  if (grad_in && n > 0) {
+    hipLaunchKernelGGL(modulation_vjp_u_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       kind, (const float2*)g, a, b, (float2*)grad_in, n, plane_px, m, wl,
+                       (float)thickness);
+    CXM_LAUNCH_CHECK();
+  }
+  if (grad_a || grad_b) {
+    hipLaunchKernelGGL(modulation_vjp_param_kernel, dim3((unsigned)((plane_px + 255) / 256)),
+                       dim3(256), 0, s, (const float2*)g, (const float2*)y, grad_a, grad_b,
+                       (long long)n_planes, plane_px, k);
+    CXM_LAUNCH_CHECK();
+  }
   return CXM_OK;
 }

@@ -27,16 +27,51 @@ __all__ = ["thin_sample", "multislice_thick_sample", "multislice_mse_grad",
            "fluorescent_multislice_thick_sample"]
 
 
+def _thin_fwd(u, dn, ab, meta):
+    n_planes, H, W, nwv, wl, t = meta
+    out = torch.empty_like(u)
+    L.check(L.lib().cxm_thin_sample_c64(L.stream_ptr(u.device), u.data_ptr(), dn.data_ptr(),
+                                        ab.data_ptr(), out.data_ptr(), n_planes, H, W, nwv,
+                                        L.doubles(wl), t))
+    return out
+
+
+class _ThinSampleFn(torch.autograd.Function):
+    """thin_sample with its vector-Jacobian product (cxm_modulation_vjp_c64):
+    grad_u = g conj(T), grad_dn = sum k Im(g conj(out)), grad_abs = -sum k
+    Re(g conj(out)), k = 2 pi t / lambda."""
+
+    @staticmethod
+    def forward(ctx, u, dn, ab, meta):
+        out = _thin_fwd(u.detach(), dn.detach(), ab.detach(), meta)
+        ctx.meta = meta
+        ctx.save_for_backward(dn, ab, out)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        dn, ab, out = ctx.saved_tensors
+        n_planes, H, W, nwv, wl, t = ctx.meta
+        g = g.to(torch.complex64).contiguous()
+        gu = torch.empty_like(g) if ctx.needs_input_grad[0] else None
+        gdn = torch.empty_like(dn) if ctx.needs_input_grad[1] else None
+        gab = torch.empty_like(ab) if ctx.needs_input_grad[2] else None
+        L.check(L.lib().cxm_modulation_vjp_c64(
+            L.stream_ptr(g.device), 1, g.data_ptr(), out.data_ptr(), dn.data_ptr(), ab.data_ptr(),
+            L.dptr(gu), L.dptr(gdn), L.dptr(gab), n_planes, H, W, nwv, L.doubles(wl), t))
+        return gu, gdn, gab, None
+
+
 def thin_sample(field: Field, absorption, dn, thickness) -> Field:
     """``field * exp(1j * 2 pi (dn + 1j * absorption) * thickness / lambda)``
     with (H, W) ``absorption`` and ``dn`` broadcast over the batch (and the
-    wavelengths of a chromatic field)."""
-    L.refuse_grad("thin_sample", field.u, absorption, dn)
+    wavelengths of a chromatic field).  Differentiable in the field, dn and
+    the absorption."""
     u = field.u.contiguous()
     L.ensure_device(u)
     H, W = field.spatial_shape
-    dn_t = torch.as_tensor(dn, dtype=torch.float32, device=u.device).contiguous()
-    ab_t = torch.as_tensor(absorption, dtype=torch.float32, device=u.device).contiguous()
+    dn_t = torch.as_tensor(dn, device=u.device).to(torch.float32).contiguous()
+    ab_t = torch.as_tensor(absorption, device=u.device).to(torch.float32).contiguous()
     if dn_t.ndim != 2 or tuple(dn_t.shape) != (H, W):
         raise ValueError(f"Refractive index must have shape {(H, W)}, got {tuple(dn_t.shape)}.")
     if ab_t.ndim != 2 or tuple(ab_t.shape) != (H, W):
@@ -44,12 +79,11 @@ def thin_sample(field: Field, absorption, dn, thickness) -> Field:
     if np.ndim(thickness) != 0:
         raise NotImplementedError("thin_sample: scalar thickness only")
     nwv = field.spectrum.size if isinstance(field, ChromaticScalarField) else 1
-    n_planes = u.numel() // (H * W * nwv)
-    out = torch.empty_like(u)
-    L.check(L.lib().cxm_thin_sample_c64(L.stream_ptr(u.device), u.data_ptr(), dn_t.data_ptr(),
-                                        ab_t.data_ptr(), out.data_ptr(), n_planes, H, W, nwv,
-                                        L.doubles(field.spectrum.wavelength), float(thickness)))
-    return field.replace(u=out)
+    meta = (u.numel() // (H * W * nwv), H, W, nwv,
+            [float(w) for w in np.atleast_1d(field.spectrum.wavelength)], float(thickness))
+    if torch.is_grad_enabled() and (u.requires_grad or dn_t.requires_grad or ab_t.requires_grad):
+        return field.replace(u=_ThinSampleFn.apply(u, dn_t, ab_t, meta))
+    return field.replace(u=_thin_fwd(u, dn_t, ab_t, meta))
 
 
 def _desc(field: Field, depth, n, thickness, pad_width, NA, kykx, reverse_distance,

@@ -289,6 +289,25 @@ int cxm_phase_change_c64(void* stream, const void* in, const float* phase, void*
                          const double* modulation);
 
 /*
+ * Vector-Jacobian product of cxm_phase_change_c64 (kind 0) and
+ * cxm_thin_sample_c64 (kind 1), the reverse-mode rules JAX derives for
+ * phase_change / thin_sample (src/chromatix/functional/phase_masks.py:29-68,
+ * src/chromatix/functional/samples.py:85-134).  g is the cotangent of the
+ * output (dL/dRe + i dL/dIm), y the forward output:
+ *   grad_in = g conj(T)
+ *   grad_a[y, x] = sum over planes and wavelengths of k_w Im(g conj(y))
+ *   grad_b[y, x] = - sum k_w Re(g conj(y))
+ * a = phase (k_w = modulation[w]) or dn (k_w = 2 pi thickness / lam_w),
+ * b = absorption (kind 1).  params: modulation (kind 0, NULL = 1) or the
+ * wavelengths (kind 1).  Output pointers may be NULL (not computed).  The
+ * per-pixel sums run in a fixed order (deterministic).
+ */
+int cxm_modulation_vjp_c64(void* stream, int32_t kind, const void* g, const void* y,
+                           const float* a, const float* b, void* grad_in, float* grad_a,
+                           float* grad_b, int64_t n_planes, int32_t height, int32_t width,
+                           int32_t num_wv, const double* params, double thickness);
+
+/*
  * Chirp-z transform along one axis of a batch of height x width complex64
  * planes (Bluestein), czt (src/chromatix/utils/czt.py:7-52):
  *   y_k = post_k * scale * sum_{j<n} x_j a^{-j} w^{j k},  k = 0 .. m-1

@@ -149,9 +149,6 @@ def test_entry_points_without_adjoints_refuse_inputs_that_require_grad():
         lambda: cf.transform_propagate_sas(f, 100.0, 1.33),
         lambda: cf.high_na_ff_lens(f, 100.0, 1.33, 0.5),
         lambda: cf.asm_propagate(f, 100.0, 1.33, 0, output_dx=0.2, output_shape=(4, 4)),
-        lambda: cf.phase_change(cx.ScalarField(torch.ones((8, 8), dtype=torch.complex64), 0.3,
-                                               0.0, 0.532), ph),
-        lambda: cf.thin_sample(f, torch.zeros((8, 8)), torch.zeros((8, 8)), 0.5),
         lambda: czt(u, 4, 1.0, 1.0),
         lambda: scaled_conv_axis(u, 0, 1.0, 1.0, 1.0, 0.0),
         lambda: fourier_convolution(u.real, ph, axes=(0, 1)),
