@@ -114,7 +114,7 @@ class OptFftDesc(ctypes.Structure):
         ("skip_initial_phase", ctypes.c_int32),
         ("skip_final_phase", ctypes.c_int32),
         ("pupil", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("adjoint", ctypes.c_int32),
         ("z", ctypes.c_double),
         ("n", ctypes.c_double),
         ("pupil_d", ctypes.c_double),

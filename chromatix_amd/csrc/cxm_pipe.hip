@@ -138,15 +138,40 @@ int opt_geom(const cxm_optfft_desc* d, OptGeom& g) {
 }
 }  // namespace
 
+// The pass geometry of a call: the descriptor itself, or for the adjoint
+// (input = the forward output, output = the forward input) the descriptor
+// with heights, widths, pads and crops exchanged.
+cxm_optfft_desc pass_desc(const cxm_optfft_desc* d) {
+  cxm_optfft_desc e = *d;
+  if (d->adjoint) {
+    e.height = d->height + 2 * d->pad_y - 2 * d->crop_y;
+    e.width = d->width + 2 * d->pad_x - 2 * d->crop_x;
+    e.pad_y = d->crop_y;
+    e.pad_x = d->crop_x;
+    e.crop_y = d->pad_y;
+    e.crop_x = d->pad_x;
+  }
+  return e;
+}
+
 extern "C" size_t cxm_optical_fft_workspace_size(const cxm_optfft_desc* d) {
   OptGeom g;
-  if (opt_geom(d, g)) return 0;
-  return (size_t)g.chunk * d->height * g.Nx * sizeof(float2);
+  if (!d) return 0;
+  const cxm_optfft_desc e = pass_desc(d);
+  if (opt_geom(&e, g)) return 0;
+  return (size_t)g.chunk * e.height * g.Nx * sizeof(float2);
 }
 
 extern "C" int cxm_optical_fft_c64(void* stream, const void* in, void* out, void* workspace,
-                                   size_t workspace_bytes, const cxm_optfft_desc* d) {
+                                   size_t workspace_bytes, const cxm_optfft_desc* d0) {
   OptGeom g;
+  if (!d0) return fail(CXM_ERR_INVALID, "null descriptor");
+  if (d0->adjoint != 0 && d0->adjoint != 1) return fail(CXM_ERR_INVALID, "adjoint must be 0 or 1");
+  if (d0->adjoint && d0->out_kind != CXM_OUT_COMPLEX)
+    return fail(CXM_ERR_INVALID, "the adjoint has a complex output");
+  const cxm_optfft_desc e = pass_desc(d0);
+  const cxm_optfft_desc* d = &e;
+  const bool adj = d0->adjoint != 0;
   int rc = opt_geom(d, g);
   if (rc) return rc;
   if (d->batch == 0) return CXM_OK;
@@ -167,7 +192,44 @@ extern "C" int cxm_optical_fft_c64(void* stream, const void* in, void* out, void
   memset(&ra, 0, sizeof(ra));
   memset(&ca, 0, sizeof(ca));
   Ops rb(ra), cb(ca);
-  if (optical) {
+  if (optical && adj) {
+    // adjoint chain: conj(Q2) and sigma on load, the DFT of the other
+    // direction (unnormalised), conj(Q1) and the pupil on store
+    OpDesc *q2 = nullptr, *sc = nullptr, *q1 = nullptr, *pup = nullptr;
+    if (transform && !d->skip_final_phase) q2 = rb.add(ra.pre, OP_CHIRP, nwv, 3, mode);
+    sc = rb.add(ra.pre, OP_SCALE, nwv, 1);
+    if (transform && !d->skip_initial_phase) q1 = cb.add(ca.post, OP_CHIRP, nwv, 3, mode);
+    if (d->mode == CXM_OFFT_OPTICAL && d->pupil) pup = cb.add(ca.post, OP_CIRC, nwv, 3);
+    if (!sc || (transform && !d->skip_final_phase && !q2) ||
+        (transform && !d->skip_initial_phase && !q1) || (d->mode == CXM_OFFT_OPTICAL && d->pupil && !pup))
+      return fail(CXM_ERR_INVALID, "operator table overflow");
+    for (int w = 0; w < nwv; ++w) {
+      const Wv v = wv_of(d, w);
+      const OptConsts c = opt_consts(v, d->z, d->n, g.Ny, g.Nx, mode);
+      // sigma = conj(c) (z >= 0) or c (z < 0), times -i for the transform
+      float2 sg = neg ? c.norm : make_float2(c.norm.x, -c.norm.y);
+      if (transform) sg = make_float2(sg.y, -sg.x);
+      rb.p(sc, w, 0) = make_float4(sg.x, sg.y, 0.f, 0.f);
+      if (q2) {
+        chirp_slot(rb, q2, w, c, c.du_y, c.du_x, c.du_y32, c.du_x32, v, mode);
+        float4& p0 = rb.p(q2, w, 0);
+        p0.x = -p0.x;
+        p0.z = -p0.z;
+      }
+      if (q1) {
+        chirp_slot(cb, q1, w, c, v.dy, v.dx, (float)v.dy, (float)v.dx, v, mode);
+        float4& p0 = cb.p(q1, w, 0);
+        p0.x = -p0.x;
+        p0.z = -p0.z;
+      }
+      if (pup) {
+        const float r = ((float)d->central_wavelength / (float)v.lam) * (float)d->pupil_d / 2.0f;
+        cb.p(pup, w, 0) = make_float4(r * r, 0.f, 0.f, 0.f);
+        cb.p(pup, w, 1) = make_float4((float)v.dy, (float)v.dx, (float)v.oy, (float)v.ox);
+        cb.p(pup, w, 2) = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+  } else if (optical) {
     OpDesc *pup = nullptr, *q1 = nullptr, *sc = nullptr;
     if (d->mode == CXM_OFFT_OPTICAL && d->pupil) pup = rb.add(ra.pre, OP_CIRC, nwv, 3);
     if (transform && !d->skip_initial_phase) q1 = rb.add(ra.pre, OP_CHIRP, nwv, 3, mode);
@@ -195,7 +257,13 @@ extern "C" int cxm_optical_fft_c64(void* stream, const void* in, void* out, void
       if (q2) chirp_slot(cb, q2, w, c, c.du_y, c.du_x, c.du_y32, c.du_x32, v, mode);
     }
   }
-  const bool inv = d->mode == CXM_OFFT_RAW_INV;
+  // transform direction: the forward DFT, or for the raw inverse and the
+  // adjoint of the forward modes (z >= 0) the inverse one; the adjoint of
+  // the raw inverse is the forward DFT scaled by 1/N, that of the raw
+  // forward the inverse DFT without 1/N
+  const bool inv = adj ? ((optical && !neg) || d->mode == CXM_OFFT_RAW_FWD)
+                       : d->mode == CXM_OFFT_RAW_INV;
+  const bool scale_n = adj ? d->mode == CXM_OFFT_RAW_INV : d->mode == CXM_OFFT_RAW_INV;
   // row pass
   ra.tw = twiddles_for(g.Nx);
   ra.in_plane_stride = (long long)d->height * d->width;
@@ -235,7 +303,7 @@ extern "C" int cxm_optical_fft_c64(void* stream, const void* in, void* out, void
   ca.out_line_len = g.Wout;
   ca.seq = inv ? SEQ_I : SEQ_F;
   ca.intensity = d->out_kind == CXM_OUT_INTENSITY;
-  ca.scale = inv ? (float)(1.0 / ((double)g.Ny * g.Nx)) : 1.0f;
+  ca.scale = scale_n ? (float)(1.0 / ((double)g.Ny * g.Nx)) : 1.0f;
   ca.mid_scale = 1.0f;
   // plane parameters cycle over wavelengths
   for (OpList* L : {&ra.pre, &ra.mid, &ra.post})

@@ -84,6 +84,35 @@ def _run(field: Field, d, entry: str, out_hw: tuple[int, int]) -> torch.Tensor:
     return plan.unplanar(out)
 
 
+class _OpticalFn(torch.autograd.Function):
+    """An optical-FFT-family call with its adjoint: the same descriptor with
+    `adjoint` set (the conjugated chirps, scale and pupil around the DFT of
+    the other direction, pads and crops exchanged) applied to the output's
+    cotangent.  The reference gets these rules from JAX's FFT transpose."""
+
+    @staticmethod
+    def forward(ctx, u, field, d, keep, entry, out_hw):
+        ctx.field, ctx.d, ctx.keep, ctx.entry = field, d, keep, entry
+        ctx.in_hw = tuple(field.spatial_shape)
+        return _run(field.replace(u=u.detach()), d, entry, out_hw)
+
+    @staticmethod
+    def backward(ctx, g):
+        da = L.OptFftDesc.from_buffer_copy(ctx.d)
+        da.adjoint = 1
+        fg = ctx.field.replace(u=g.to(torch.complex64).contiguous())
+        return _run(fg, da, ctx.entry, ctx.in_hw), None, None, None, None, None
+
+
+def _apply(field: Field, d, keep, entry: str, out_hw) -> torch.Tensor:
+    if torch.is_grad_enabled() and field.u.requires_grad:
+        if d.out_kind != L.CXM_OUT_COMPLEX:
+            raise NotImplementedError("gradients flow through the complex output; take "
+                                      "`.intensity` of the returned field")
+        return _OpticalFn.apply(field.u, field, d, keep, entry, tuple(out_hw))
+    return _run(field, d, entry, out_hw)
+
+
 def _new_spacing(field: Field, z: float, n: float, grid: tuple[int, int]) -> np.ndarray:
     """du = df * |lam z / n| with df = 1 / (N dx) on the (padded) grid
     (convenience.py:33-41)."""
@@ -96,11 +125,11 @@ def _new_spacing(field: Field, z: float, n: float, grid: tuple[int, int]) -> np.
 
 
 def optical_fft(field: Field, z, n, *, phase_mode=None) -> Field:
-    """Far-field / lens FFT with the sampling change du = df |lam z / n|."""
-    L.refuse_grad("optical_fft", field.u)
+    """Far-field / lens FFT with the sampling change du = df |lam z / n|
+    (differentiable in the field)."""
     z = float(z)
     d, keep = _desc(field, L.CXM_OFFT_OPTICAL, z, n, phase_mode)
-    u = _run(field, d, "optical_fft", field.spatial_shape)
+    u = _apply(field, d, keep, "optical_fft", field.spatial_shape)
     return field.replace(u=u, dx=_new_spacing(field, z, n, field.spatial_shape))
 
 
@@ -108,22 +137,21 @@ def ff_lens(field: Field, f, n, NA=None, inverse: bool = False, *, phase_mode=No
     """Thin lens a distance f after the field, observed in its back focal
     plane: optional circular pupil of diameter 2 f NA / n, then optical_fft
     with f (or -f when inverse)."""
-    L.refuse_grad("ff_lens", field.u)
     f = float(f)
     z = -f if inverse else f
     d, keep = _desc(field, L.CXM_OFFT_OPTICAL, z, n, phase_mode)
     if NA is not None:
         d.pupil = 1
         d.pupil_d = 2.0 * f * float(NA) / float(n)
-    u = _run(field, d, "optical_fft", field.spatial_shape)
+    u = _apply(field, d, keep, "optical_fft", field.spatial_shape)
     return field.replace(u=u, dx=_new_spacing(field, z, n, field.spatial_shape))
 
 
 def transform_propagate(field: Field, z, n, pad_width, cval: float = 0,
                         skip_initial_phase: bool = False, skip_final_phase: bool = False, *,
                         phase_mode=None) -> Field:
-    """Single-FFT Fresnel propagation (SFT-FR) with the sampling change."""
-    L.refuse_grad("transform_propagate", field.u)
+    """Single-FFT Fresnel propagation (SFT-FR) with the sampling change
+    (differentiable in the field)."""
     py, px = _pair(pad_width)
     if cval != 0:
         from ..core.field import pad as _pad
@@ -141,7 +169,7 @@ def transform_propagate(field: Field, z, n, pad_width, cval: float = 0,
     ny, nx = field.spatial_shape
     grid = (ny + 2 * grid_pad[0], nx + 2 * grid_pad[1])
     out_hw = (grid[0] - 2 * py, grid[1] - 2 * px)
-    u = _run(field, d, "optical_fft", out_hw)
+    u = _apply(field, d, keep, "optical_fft", out_hw)
     return field.replace(u=u, dx=_new_spacing(field, z, n, grid))
 
 
@@ -180,7 +208,6 @@ def compute_sas_precompensation(field: Field, z, n, *, phase_mode=None) -> torch
 
 def _raw(x: torch.Tensor, axes, shift: bool, inverse: bool) -> torch.Tensor:
     x = torch.as_tensor(x)
-    L.refuse_grad("fft / ifft", x)
     ax = tuple(a % x.ndim for a in axes)
     if len(ax) != 2:
         raise ValueError("fft/ifft take two axes")
@@ -190,7 +217,7 @@ def _raw(x: torch.Tensor, axes, shift: bool, inverse: bool) -> torch.Tensor:
     f = ScalarField(xm, 1.0, 0.0, 1.0)
     d, keep = _desc(f, L.CXM_OFFT_RAW_INV if inverse else L.CXM_OFFT_RAW_FWD, 0.0, 1.0, None)
     d.shift = int(bool(shift))
-    out = _run(f, d, "optical_fft", f.spatial_shape)
+    out = _apply(f, d, keep, "optical_fft", f.spatial_shape)
     return out.movedim((-2, -1), ax)
 
 

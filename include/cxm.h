@@ -181,7 +181,8 @@ typedef struct cxm_optfft_desc {
   int32_t skip_initial_phase; /* TRANSFORM */
   int32_t skip_final_phase;   /* TRANSFORM */
   int32_t pupil;              /* OPTICAL: apply the circular pupil */
-  int32_t reserved;
+  int32_t adjoint;            /* 1: the adjoint of the call the descriptor describes: input
+                                 = its output shape, output = its input shape (complex) */
   double z, n, pupil_d, central_wavelength;
   double wavelength;          /* used when wavelengths == NULL */
   double dy, dx;              /* used when spacing == NULL */

@@ -8,6 +8,7 @@ import torch
 
 import chromatix_amd as cx
 import chromatix_amd.functional as cf
+import chromatix_amd.utils.fft as cfft
 from tests.conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -31,7 +32,7 @@ def _grads_cpu(fn, inputs, c):
           .requires_grad_(True) for x in inputs]
     out = fn(*ts)
     loss = (out * torch.as_tensor(c.astype(np.complex128)).conj()).real.sum()
-    return [g.numpy() for g in torch.autograd.grad(loss, ts)]
+    return [g.resolve_conj().numpy() for g in torch.autograd.grad(loss, ts)]
 
 
 @pytest.mark.parametrize("chromatic", [False, True])
@@ -87,3 +88,78 @@ def test_thin_sample_gradients(chromatic):
     gc = _grads_cpu(cpu, [u, dn, ab], c)
     for a, b in zip(gg, gc):
         assert rel_l2(a, b) < TOL
+
+
+def _adjoint_identity(make_field, fwd, in_shape, out_shape, seed):
+    """<A x, y> against <x, A^H y> with A^H from the autograd rule."""
+    rng = np.random.default_rng(seed)
+    x = torch.as_tensor(_rand_c(rng, in_shape)).cuda().requires_grad_(True)
+    y = torch.as_tensor(_rand_c(rng, out_shape)).cuda()
+    out = fwd(make_field(x))
+    assert tuple(out.shape) == tuple(out_shape)
+    lhs = torch.vdot(out.reshape(-1), y.reshape(-1))
+    (g,) = torch.autograd.grad((out * y.conj()).real.sum(), x)
+    rhs = torch.vdot(x.detach().reshape(-1), g.reshape(-1))
+    return abs((lhs - rhs).item()) / abs(lhs.item())
+
+
+WL3 = np.array([0.45, 0.532, 0.65])
+
+
+@pytest.mark.parametrize("chromatic", [False, True])
+@pytest.mark.parametrize("z", [150.0, -90.0])
+@pytest.mark.parametrize("kind", ["optical_fft", "ff_lens_na", "ff_lens_inverse", "transform",
+                                  "transform_skip", "fft_shift", "ifft"])
+def test_optical_family_adjoint_identity(kind, z, chromatic):
+    H, W = 96, 80
+    in_shape = (2, H, W, 3) if chromatic else (2, H, W)
+
+    def mk(x):
+        if chromatic:
+            return cx.ChromaticScalarField(x, 0.3, 0.0, cx.Spectrum(WL3))
+        return cx.ScalarField(x, 0.3, 0.0, 0.532)
+
+    out_shape = in_shape
+    if kind == "optical_fft":
+        fwd = lambda f: cf.optical_fft(f, z, 1.33).u
+    elif kind == "ff_lens_na":
+        fwd = lambda f: cf.ff_lens(f, abs(z), 1.33, NA=0.3).u
+    elif kind == "ff_lens_inverse":
+        fwd = lambda f: cf.ff_lens(f, abs(z), 1.33, inverse=True).u
+    elif kind in ("transform", "transform_skip"):
+        skip = kind == "transform_skip"
+        fwd = lambda f: cf.transform_propagate(f, z, 1.33, (16, 24), skip_initial_phase=skip).u
+    else:
+        if chromatic:
+            pytest.skip("raw transforms act on arrays")
+        if kind == "fft_shift":
+            fwd = lambda f: cfft.fft(f.u, axes=(-2, -1), shift=True)
+        else:
+            fwd = lambda f: cfft.ifft(f.u, axes=(-2, -1), shift=False)
+    err = _adjoint_identity(mk, fwd, in_shape, out_shape, 7)
+    print(f"{kind} z {z} chromatic {chromatic}: {err:.2e}")
+    assert err < TOL
+
+
+@pytest.mark.parametrize("z", [150.0, -90.0])
+def test_optical_fft_gradient_matches_autodiff_of_the_formula(z):
+    """convenience.py:43-58: z >= 0: fftshift(fft2(ifftshift(-i prod(dx)/L^2 u)));
+    z < 0: conj(fftshift(fft2(ifftshift(-i L^2/(prod(du) N) conj(u))))),
+    differentiated by torch in complex128 on the CPU."""
+    rng = np.random.default_rng(8)
+    H, W, dx, lam, n = 64, 48, 0.3, 0.532, 1.33
+    u, c = _rand_c(rng, (H, W)), _rand_c(rng, (H, W))
+    L2 = lam * z / n
+    du = (abs(L2) / (H * dx), abs(L2) / (W * dx))
+
+    def cpu(x):
+        ax = (-2, -1)
+        if z >= 0:
+            v = -1j * dx * dx / L2 * x
+            return torch.fft.fftshift(torch.fft.fft2(torch.fft.ifftshift(v, dim=ax)), dim=ax)
+        v = -1j * L2 / (du[0] * du[1] * H * W) * x.conj()
+        return torch.fft.fftshift(torch.fft.fft2(torch.fft.ifftshift(v, dim=ax)), dim=ax).conj()
+
+    gg = _grads_gpu(lambda x: cf.optical_fft(cx.ScalarField(x, dx, 0.0, lam), z, n).u, [u], c)
+    gc = _grads_cpu(cpu, [u], c)
+    assert rel_l2(gg[0], gc[0]) < TOL
