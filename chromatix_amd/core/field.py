@@ -385,7 +385,7 @@ def pad(field: Field, pad_width, cval: float = 0) -> Field:
         u = torch.where(mask, torch.as_tensor(complex(cval), dtype=u.dtype, device=u.device), u)
     if isinstance(field, ChromaticScalarField):
         u = u.movedim(-3, -1)
-    return field.replace(u=u.contiguous())
+    return field.replace(u=u.resolve_conj().contiguous())
 
 
 def crop(field: Field, crop_width) -> Field:
@@ -395,7 +395,7 @@ def crop(field: Field, crop_width) -> Field:
     ny, nx = field.spatial_shape
     sl[field.ndim + field.y] = slice(cy, ny - cy)
     sl[field.ndim + field.x] = slice(cx, nx - cx)
-    return field.replace(u=field.u[tuple(sl)].contiguous())
+    return field.replace(u=field.u[tuple(sl)].resolve_conj().contiguous())
 
 
 def shift_grid(field: Field, shift_yx) -> Field:

@@ -203,7 +203,7 @@ def get_shift_and_scale(permittivity) -> tuple[complex, complex]:
     if t.ndim != 5 and t.is_cuda:
         # on the device: cxm_enclosing_circle_c64 (candidates pruned on the GPU)
         L.ensure_device(t)
-        v = t.to(torch.complex64).contiguous()
+        v = t.to(torch.complex64).resolve_conj().contiguous()
         ws = torch.empty(L.lib().cxm_enclosing_circle_workspace_size(v.numel()), dtype=torch.uint8,
                          device=v.device)
         out = (ctypes.c_double * 3)()
@@ -244,10 +244,10 @@ class Solver:
 
     def __init__(self, sample: Sample, source: Source, initial_E=None, *, shift_scale=None,
                  tol: float = 1e-3, rtol: float | None = None, maxiter: int = 1000):
-        eps = sample.permittivity.to(torch.complex64).contiguous()
+        eps = sample.permittivity.to(torch.complex64).resolve_conj().contiguous()
         L.ensure_device(eps)
         self.eps, self.dev, self.shape = eps, eps.device, sample.shape
-        S = torch.as_tensor(source.field, device=self.dev).to(torch.complex64).contiguous()
+        S = torch.as_tensor(source.field, device=self.dev).to(torch.complex64).resolve_conj().contiguous()
         if tuple(S.shape) != sample.shape + (3,):
             raise ValueError(f"source field must have shape {sample.shape + (3,)}")
         shift, scale = shift_scale if shift_scale is not None else get_shift_and_scale(eps)
@@ -264,7 +264,7 @@ class Solver:
             self.x = self.py.clone()
         else:
             E0 = torch.as_tensor(initial_E, device=self.dev).to(torch.complex64)
-            self.x = (E0.permute(3, 0, 1, 2) / self.num[0].to(torch.float32)).contiguous()
+            self.x = (E0.permute(3, 0, 1, 2) / self.num[0].to(torch.float32)).resolve_conj().contiguous()
         self.rtol, self.tol = rtol, float(tol)
         self.maxiter = int(maxiter)
         # [residual^2, |prec_y|^2], device and pinned host copies

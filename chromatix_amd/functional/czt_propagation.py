@@ -56,8 +56,8 @@ def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=
     H = torch.as_tensor(propagator).to(device=field.u.device, dtype=torch.complex64)
     if H.shape[-2:] != (ny, nx):
         raise ValueError("propagator must have the field's spatial shape")
-    H = H.reshape((-1, ny, nx)).contiguous()
-    u = field.u.contiguous()
+    H = H.reshape((-1, ny, nx)).resolve_conj().contiguous()
+    u = field.u.resolve_conj().contiguous()
     L.ensure_device(u)
     spec = _fft_shifted(u, axes=(-2, -1), shift=True)
     n_planes = spec.numel() // (ny * nx)

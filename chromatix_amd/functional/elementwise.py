@@ -44,7 +44,7 @@ class _Intensity(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         (u,) = ctx.saved_tensors
-        g = g.to(torch.float32).contiguous()
+        g = g.to(torch.float32).resolve_conj().contiguous()
         gu = torch.empty_like(u)
         dens = L.doubles(ctx.density) if ctx.density is not None else None
         L.check(L.lib().cxm_intensity_vjp_c64(L.stream_ptr(u.device), u.data_ptr(), g.data_ptr(),
@@ -53,7 +53,7 @@ class _Intensity(torch.autograd.Function):
 
 
 def intensity(field) -> torch.Tensor:
-    u = field.u.contiguous()
+    u = field.u.resolve_conj().contiguous()
     L.ensure_device(u)
     chrom = _is_chromatic(field)
     nwv = field.spectrum.size if chrom else 1
@@ -73,7 +73,7 @@ def _is_chromatic(field) -> bool:
 def plane_sumsq(field) -> torch.Tensor:
     """FP64 sum over (y, x[, wv]) of density-weighted |u|^2, one per batch
     element (shape = batch shape)."""
-    u = field.u.contiguous()
+    u = field.u.resolve_conj().contiguous()
     L.ensure_device(u)
     nwv, dens = _density(field)
     chrom = _is_chromatic(field)
@@ -105,7 +105,7 @@ def power(field) -> torch.Tensor:
 
 def plane_sumsq_raw(u: torch.Tensor) -> float:
     """sum |u|^2 over the whole complex64 tensor, reduced in FP64 on the device."""
-    u = u.contiguous()
+    u = u.resolve_conj().contiguous()
     L.ensure_device(u)
     out = torch.empty(1, dtype=torch.float64, device=u.device)
     ws_bytes = L.lib().cxm_plane_sumsq_workspace_size(1)

@@ -100,7 +100,7 @@ class _OpticalFn(torch.autograd.Function):
     def backward(ctx, g):
         da = L.OptFftDesc.from_buffer_copy(ctx.d)
         da.adjoint = 1
-        fg = ctx.field.replace(u=g.to(torch.complex64).contiguous())
+        fg = ctx.field.replace(u=g.to(torch.complex64).resolve_conj().contiguous())
         return _run(fg, da, ctx.entry, ctx.in_hw), None, None, None, None, None
 
 
@@ -258,7 +258,7 @@ def high_na_ff_lens(field: Field, f, n, NA, output_shape=None, output_dx=None) -
     zoom = 2.0 * float(NA) * output_shape[0] * output_dx / ((nx - 1) * wl)
     u = field.u
     L.ensure_device(u)
-    u = u.to(torch.complex64).contiguous()
+    u = u.to(torch.complex64).resolve_conj().contiguous()
     n_planes = u.numel() // (ny * nx)
     tmp = torch.empty_like(u)
     L.check(L.lib().cxm_high_na_defocus_c64(L.stream_ptr(u.device), u.data_ptr(), tmp.data_ptr(),

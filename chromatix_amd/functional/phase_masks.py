@@ -33,7 +33,7 @@ class _PhaseChangeFn(torch.autograd.Function):
     def backward(ctx, g):
         ph, out = ctx.saved_tensors
         n_planes, H, W, nwv, mod = ctx.meta
-        g = g.to(torch.complex64).contiguous()
+        g = g.to(torch.complex64).resolve_conj().contiguous()
         gu = torch.empty_like(g) if ctx.needs_input_grad[0] else None
         gph = torch.empty_like(ph) if ctx.needs_input_grad[1] else None
         L.check(L.lib().cxm_modulation_vjp_c64(
@@ -46,9 +46,9 @@ def phase_change(field: Field, phase, spectrally_modulate: bool = True) -> Field
     """Perturbs ``field`` by ``phase`` (radians, (H, W), broadcast over the
     batch and, scaled by the spectral modulation, over wavelengths).
     Differentiable in the field and the phase."""
-    u = field.u.contiguous()
+    u = field.u.resolve_conj().contiguous()
     L.ensure_device(u)
-    ph = torch.as_tensor(phase, device=u.device).to(torch.float32).contiguous()
+    ph = torch.as_tensor(phase, device=u.device).to(torch.float32).resolve_conj().contiguous()
     H, W = field.spatial_shape
     if tuple(ph.shape) != (H, W):
         raise ValueError("Phase must have same height and width as incoming ``Field``.")

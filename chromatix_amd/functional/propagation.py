@@ -101,7 +101,7 @@ class _Plan:
         if self.chrom:
             u = u.movedim(-1, -3)
         h, w = u.shape[-2], u.shape[-1]
-        return u.reshape(-1, h, w).contiguous()
+        return u.reshape(-1, h, w).resolve_conj().contiguous()
 
     def out_planes(self, n_in: int) -> int:
         return n_in * self.nz if self.mode == "outer" else n_in
@@ -236,7 +236,7 @@ class _PropagateFn(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g):
-        return _propagate_planes(g.to(torch.complex64).contiguous(), ctx.args, L.CXM_OUT_COMPLEX,
+        return _propagate_planes(g.to(torch.complex64).resolve_conj().contiguous(), ctx.args, L.CXM_OUT_COMPLEX,
                                  adjoint=True), None
 
 
@@ -368,7 +368,7 @@ def kernel_propagate(
         raise ValueError(f"propagator spatial shape {tuple(H.shape[-2:])} != field {(ny, nx)}")
     lead = H.shape[:-2]
     if all(s == 1 for s in lead):
-        out = _kernel_propagate_array(field, H.reshape(ny, nx).contiguous())
+        out = _kernel_propagate_array(field, H.reshape(ny, nx).resolve_conj().contiguous())
     else:
         out = _kernel_propagate_broadcast(field, H)
     return shift_grid(field.replace(u=out), shift_yx)
@@ -401,7 +401,7 @@ def _kernel_propagate_broadcast(field: Field, H: torch.Tensor) -> torch.Tensor:
 
             w = idx[-1]
             mono = ScalarField(ub[idx], field.dx[w], 0.0, float(field.spectrum.wavelength[w]))
-        outs[idx] = _kernel_propagate_array(mono, Hb[idx].contiguous())
+        outs[idx] = _kernel_propagate_array(mono, Hb[idx].resolve_conj().contiguous())
     return outs.movedim(-3, -1) if chrom else outs
 
 

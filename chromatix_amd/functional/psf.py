@@ -91,7 +91,7 @@ def _forward(phase, d, out_kind, out=None) -> torch.Tensor:
 
 def _backward(phase, cot, d, kind) -> torch.Tensor:
     d.grad_kind = kind
-    cot = cot.to(torch.float32).contiguous()
+    cot = cot.to(torch.float32).resolve_conj().contiguous()
     if tuple(cot.shape) != (d.num_z, d.height, d.width):
         raise ValueError(f"cotangent / target must have shape {(d.num_z, d.height, d.width)}")
     grad = torch.empty((d.height, d.width), dtype=torch.float32, device=phase.device)
@@ -144,7 +144,7 @@ def psf_4f(phase, spacing, wavelength, z, f, n, NA, f_tube, power=1.0, amplitude
     if ph.ndim != 2:
         raise ValueError("phase must be a (height, width) array")
     L.ensure_device(ph)
-    ph_c = ph.contiguous()
+    ph_c = ph.resolve_conj().contiguous()
     args = (spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode)
     single = np.ndim(np.asarray(z)) == 0 or np.asarray(z).size == 1
     if output == "intensity":
@@ -165,7 +165,7 @@ def psf_4f_mse_grad(phase, target, spacing, wavelength, z, f, n, NA, f_tube, pow
                     amplitude=1.0, offset=(0.0, 0.0), *, phase_mode=None) -> torch.Tensor:
     """d/dphase of mean((psf_4f(phase, ...) - target)^2), fused: the forward
     is recomputed inside the adjoint and the intensity never stored."""
-    ph = _phase_tensor(phase).detach().contiguous()
+    ph = _phase_tensor(phase).detach().resolve_conj().contiguous()
     args = (spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode)
     chunks = _z_chunks(args)
     nz = chunks[-1][0].stop

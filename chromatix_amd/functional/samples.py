@@ -52,7 +52,7 @@ class _ThinSampleFn(torch.autograd.Function):
     def backward(ctx, g):
         dn, ab, out = ctx.saved_tensors
         n_planes, H, W, nwv, wl, t = ctx.meta
-        g = g.to(torch.complex64).contiguous()
+        g = g.to(torch.complex64).resolve_conj().contiguous()
         gu = torch.empty_like(g) if ctx.needs_input_grad[0] else None
         gdn = torch.empty_like(dn) if ctx.needs_input_grad[1] else None
         gab = torch.empty_like(ab) if ctx.needs_input_grad[2] else None
@@ -67,11 +67,11 @@ def thin_sample(field: Field, absorption, dn, thickness) -> Field:
     with (H, W) ``absorption`` and ``dn`` broadcast over the batch (and the
     wavelengths of a chromatic field).  Differentiable in the field, dn and
     the absorption."""
-    u = field.u.contiguous()
+    u = field.u.resolve_conj().contiguous()
     L.ensure_device(u)
     H, W = field.spatial_shape
-    dn_t = torch.as_tensor(dn, device=u.device).to(torch.float32).contiguous()
-    ab_t = torch.as_tensor(absorption, device=u.device).to(torch.float32).contiguous()
+    dn_t = torch.as_tensor(dn, device=u.device).to(torch.float32).resolve_conj().contiguous()
+    ab_t = torch.as_tensor(absorption, device=u.device).to(torch.float32).resolve_conj().contiguous()
     if dn_t.ndim != 2 or tuple(dn_t.shape) != (H, W):
         raise ValueError(f"Refractive index must have shape {(H, W)}, got {tuple(dn_t.shape)}.")
     if ab_t.ndim != 2 or tuple(ab_t.shape) != (H, W):
@@ -175,7 +175,7 @@ class _MultisliceFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         u, dn, ab = ctx.saved_tensors
-        g = g.to(torch.complex64).contiguous()
+        g = g.to(torch.complex64).resolve_conj().contiguous()
         gdn, gab = _bwd(u, dn, ab, ctx.ck, g, ctx.d, L.CXM_GRAD_COMPLEX)
         return None, gdn, gab, None
 
@@ -199,14 +199,14 @@ def multislice_thick_sample(field: Field, absorption_stack, dn_stack, n, thickne
     d = _desc(field, dn.shape[0], n, thickness_per_slice, pad_width, NA, kykx,
               reverse_propagate_distance, remove_evanescent, bandlimit, return_stack, phase_mode,
               checkpoint_every)
-    u = field.u.contiguous()
+    u = field.u.resolve_conj().contiguous()
     L.ensure_device(u)
     batch_shape = tuple(field.shape[:-2])
     H, W = field.spatial_shape
     needs_grad = (dn.requires_grad or ab.requires_grad) and torch.is_grad_enabled()
     dn32 = dn if dn.dtype == torch.float32 else dn.to(torch.float32)
     ab32 = ab if ab.dtype == torch.float32 else ab.to(torch.float32)
-    dn32, ab32 = dn32.contiguous(), ab32.contiguous()
+    dn32, ab32 = dn32.resolve_conj().contiguous(), ab32.resolve_conj().contiguous()
     if needs_grad:
         if return_stack:
             raise NotImplementedError("gradients through return_stack are not implemented")
@@ -230,12 +230,12 @@ def multislice_mse_grad(field: Field, absorption_stack, dn_stack, target, n, thi
     d = _desc(field, dn.shape[0], n, thickness_per_slice, pad_width, NA, kykx,
               reverse_propagate_distance, remove_evanescent, bandlimit, False, phase_mode,
               checkpoint_every)
-    u = field.u.contiguous()
+    u = field.u.resolve_conj().contiguous()
     L.ensure_device(u)
-    t = torch.as_tensor(target, dtype=torch.float32, device=u.device).contiguous()
+    t = torch.as_tensor(target, dtype=torch.float32, device=u.device).resolve_conj().contiguous()
     if t.numel() != d.batch * d.height * d.width:
         raise ValueError("target must have the output's shape")
-    return _bwd(u, dn.to(torch.float32).contiguous(), ab.to(torch.float32).contiguous(), checkpoints,
+    return _bwd(u, dn.to(torch.float32).resolve_conj().contiguous(), ab.to(torch.float32).resolve_conj().contiguous(), checkpoints,
                 t, d, L.CXM_GRAD_MSE)
 
 
@@ -272,8 +272,8 @@ def fluorescent_multislice_thick_sample(field: Field, fluorescence_stack, dn_sta
         raise NotImplementedError("fluorescent multislice: one field (the reference's intensity "
                                   "stack has no batch axis)")
     dev = u.device
-    fl = torch.as_tensor(fluorescence_stack, dtype=torch.float32, device=dev).contiguous()
-    dn = torch.as_tensor(dn_stack, dtype=torch.float32, device=dev).contiguous()
+    fl = torch.as_tensor(fluorescence_stack, dtype=torch.float32, device=dev).resolve_conj().contiguous()
+    dn = torch.as_tensor(dn_stack, dtype=torch.float32, device=dev).resolve_conj().contiguous()
     if fl.ndim != 3 or tuple(fl.shape) != tuple(dn.shape) or tuple(fl.shape[1:]) != (H, W):
         raise ValueError("fluorescence_stack and dn_stack must both have shape (depth, H, W) "
                          f"with (H, W) = {(H, W)}")
@@ -313,7 +313,7 @@ def fluorescent_multislice_thick_sample(field: Field, fluorescence_stack, dn_sta
             Pt = torch.as_tensor(P).to(device=dev, dtype=torch.complex64).reshape(-1)
             if Pt.numel() != Ny * Nx:
                 raise ValueError(f"propagators must have the padded shape {(Ny, Nx)}")
-            Pt = Pt.contiguous()
+            Pt = Pt.resolve_conj().contiguous()
             keep.append(Pt)
             setattr(d, name, Pt.data_ptr())
     state = torch.zeros((Ny, Nx), dtype=torch.complex64, device=dev)
@@ -327,7 +327,7 @@ def fluorescent_multislice_thick_sample(field: Field, fluorescence_stack, dn_sta
     stream = L.stream_ptr(dev)
     for smp in range(S):
         if ph_all is not None:
-            ph = ph_all[smp].contiguous()
+            ph = ph_all[smp].resolve_conj().contiguous()
         else:
             ph = torch.rand((D, H, W), generator=gen, device=dev, dtype=torch.float32) * (2 * np.pi)
         L.check(L.lib().cxm_fluor_multislice_sample_c64(stream, state.data_ptr(), fl.data_ptr(),

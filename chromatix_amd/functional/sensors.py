@@ -18,16 +18,36 @@ from ..ops import approximate_shot_noise, shot_noise
 __all__ = ["basic_sensor", "sum_axis"]
 
 
+class _SumAxisFn(torch.autograd.Function):
+    """sum_axis with its reverse-mode rule (the cotangent broadcast along
+    the summed axis)."""
+
+    @staticmethod
+    def forward(ctx, x, ax):
+        ctx.shape, ctx.ax = tuple(x.shape), ax
+        return _sum_axis(x.detach(), ax)
+
+    @staticmethod
+    def backward(ctx, g):
+        return g.unsqueeze(ctx.ax).expand(ctx.shape).resolve_conj().contiguous(), None
+
+
 def sum_axis(x: torch.Tensor, axis: int) -> torch.Tensor:
     """Sum of a float32 device tensor over ``axis`` (terms added in index
-    order, one thread per output element)."""
+    order, one thread per output element); differentiable."""
     x = torch.as_tensor(x)
     L.ensure_device(x)
-    x = x.to(torch.float32).contiguous()
+    x = x.to(torch.float32).resolve_conj().contiguous()
     nd = x.ndim
     ax = axis + nd if axis < 0 else axis
     if not 0 <= ax < nd:
         raise ValueError(f"axis {axis} out of range for {nd} dimensions")
+    if torch.is_grad_enabled() and x.requires_grad:
+        return _SumAxisFn.apply(x, ax)
+    return _sum_axis(x, ax)
+
+
+def _sum_axis(x: torch.Tensor, ax: int) -> torch.Tensor:
     shape = tuple(x.shape)
     outer = 1
     for s in shape[:ax]:

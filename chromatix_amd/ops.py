@@ -54,7 +54,6 @@ def fourier_convolution(image, kernel, *, axes=(0, 1), fast_fft_shape: bool = Tr
     ``axes`` broadcast; in "same" mode they are cropped to the image's, as
     the reference's slices do."""
     image = torch.as_tensor(image)
-    L.refuse_grad("fourier_convolution", image, kernel)
     L.ensure_device(image)
     kernel = torch.as_tensor(kernel, device=image.device)
     if image.ndim != kernel.ndim:
@@ -102,34 +101,97 @@ def fourier_convolution(image, kernel, *, axes=(0, 1), fast_fft_shape: bool = Tr
     if k_planes is None:
         ker = ker.expand(ob + ksp)
         k_planes = int(np.prod(ob, dtype=np.int64))
-    img = img.contiguous()
-    ker = ker.contiguous()
+    img = img.resolve_conj().contiguous()
+    ker = ker.resolve_conj().contiguous()
+    meta = (na, mode, ob, isp, ksp, out_sp, k_planes, cplx)
+    if torch.is_grad_enabled() and (img.requires_grad or ker.requires_grad):
+        out = _FconvFn.apply(img, ker, meta)
+    else:
+        out = _fconv_core(img, ker, meta)
+    inv = [0] * nd
+    for i, p in enumerate(perm):
+        inv[p] = i
+    return out.permute(inv)
+
+
+def _fconv_core(img, ker, meta) -> torch.Tensor:
+    """One cxm_fconv_c64 call on contiguous (batch..., spatial...) planes;
+    kernel plane b % k_planes serves output plane b."""
+    na, mode, ob, isp, ksp, out_sp, k_planes, cplx = meta
+    dt = torch.complex64 if cplx else torch.float32
+    img = img.detach().to(dt).resolve_conj().contiguous()
+    ker = ker.detach().to(dt).resolve_conj().contiguous()
     batch = int(np.prod(ob, dtype=np.int64))
     d = L.FconvDesc()
     d.batch = batch
-    (d.depth, d.height, d.width) = (1,) * (3 - na) + isp
+    (d.depth, d.height, d.width) = (1,) * (3 - na) + tuple(isp)
     d.k_planes = k_planes
-    (d.k_depth, d.k_height, d.k_width) = (1,) * (3 - na) + ksp
+    (d.k_depth, d.k_height, d.k_width) = (1,) * (3 - na) + tuple(ksp)
     d.real_input = 0 if cplx else 1
     d.mode = 0 if mode == "same" else 1
-    (d.out_depth, d.out_height, d.out_width) = (1,) * (3 - na) + out_sp
+    (d.out_depth, d.out_height, d.out_width) = (1,) * (3 - na) + tuple(out_sp)
     lib = L.lib()
     ws_bytes = lib.cxm_fconv_workspace_size(d)
     if ws_bytes == 0:
         L.check(lib.cxm_fconv_c64(None, None, None, None, None, 0, d))
     if mode == "full":
         # entries past the transform used here are not written
-        out = torch.zeros(ob + out_sp, dtype=dt, device=img.device)
+        out = torch.zeros(tuple(ob) + tuple(out_sp), dtype=dt, device=img.device)
     else:
-        out = torch.empty(ob + out_sp, dtype=dt, device=img.device)
+        out = torch.empty(tuple(ob) + tuple(out_sp), dtype=dt, device=img.device)
     if batch > 0:
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=img.device)
         L.check(lib.cxm_fconv_c64(L.stream_ptr(img.device), img.data_ptr(), ker.data_ptr(),
                                   out.data_ptr(), ws.data_ptr(), ws_bytes, d))
-    inv = [0] * nd
-    for i, p in enumerate(perm):
-        inv[p] = i
-    return out.permute(inv)
+    return out
+
+
+class _FconvFn(torch.autograd.Function):
+    """fourier_convolution with its reverse-mode rule.  With the output
+    window starting at s per axis (k - 1) // 2 ("same") or 0 ("full"), and
+    a flipped, conjugated copy of the other operand:
+      grad_image[j]  = (g * conj(flip(kernel)))_full[j + K - 1 - s]
+      grad_kernel[i] = (g * conj(flip(image)))_full[i + N - 1 - s]
+    both full linear convolutions by the same library call, the kernel's
+    summed over the output planes that share it (the reference gets these
+    from JAX's FFT transpose rules)."""
+
+    @staticmethod
+    def forward(ctx, img, ker, meta):
+        ctx.meta = meta
+        ctx.save_for_backward(img, ker)
+        return _fconv_core(img, ker, meta)
+
+    @staticmethod
+    def backward(ctx, g):
+        img, ker = ctx.saved_tensors
+        na, mode, ob, isp, ksp, out_sp, k_planes, cplx = ctx.meta
+        sp = tuple(range(len(ob), len(ob) + na))
+        starts = [(k - 1) // 2 if mode == "same" else 0 for k in ksp]
+        gsp = tuple(g.shape[len(ob):])
+        dt = torch.complex64 if cplx else torch.float32
+        g = g.to(dt).resolve_conj().contiguous()
+        gi = gk = None
+        if ctx.needs_input_grad[0]:
+            kf = torch.flip(ker.detach(), sp)
+            kf = torch.conj_physical(kf) if cplx else kf
+            full = _fconv_core(g, kf, (na, "full", ob, gsp, ksp,
+                                       tuple(a + b for a, b in zip(gsp, ksp)), k_planes, cplx))
+            for d, (n, k, s) in enumerate(zip(isp, ksp, starts)):
+                full = full.narrow(len(ob) + d, k - 1 - s, n)
+            gi = full.resolve_conj().contiguous()
+        if ctx.needs_input_grad[1]:
+            af = torch.flip(img.detach(), sp)
+            af = torch.conj_physical(af) if cplx else af
+            batch = int(np.prod(ob, dtype=np.int64))
+            full = _fconv_core(g, af, (na, "full", ob, gsp, isp,
+                                       tuple(a + b for a, b in zip(gsp, isp)), batch, cplx))
+            for d, (n, k, s) in enumerate(zip(isp, ksp, starts)):
+                full = full.narrow(len(ob) + d, n - 1 - s, k)
+            # kernel plane q serves output planes q, q + k_planes, ...
+            full = full.reshape((batch // k_planes, k_planes) + tuple(ksp)).sum(0)
+            gk = full.reshape(ker.shape)
+        return gi, gk, None
 
 
 def _generator(key, device) -> torch.Generator:
@@ -159,8 +221,8 @@ def planewise_convolution_sum(image, kernel) -> torch.Tensor | None:
     if (image.ndim != 3 or kernel.ndim != 3 or image.is_complex() or kernel.is_complex()
             or kernel.shape[0] not in (1, image.shape[0]) or image.shape[0] == 0):
         return None
-    img = image.to(torch.float32).contiguous()
-    ker = kernel.to(torch.float32).contiguous()
+    img = image.to(torch.float32).resolve_conj().contiguous()
+    ker = kernel.to(torch.float32).resolve_conj().contiguous()
     d = L.FconvDesc()
     d.batch = img.shape[0]
     d.depth, d.height, d.width = 1, img.shape[1], img.shape[2]
@@ -181,9 +243,10 @@ def planewise_convolution_sum(image, kernel) -> torch.Tensor | None:
 def approximate_shot_noise(key, image) -> torch.Tensor:
     """max(image + sqrt(image) * N(0, 1), 0) (noise.py:9-16); ``key`` a
     torch.Generator on the image's device or an integer seed."""
+    L.refuse_grad("approximate_shot_noise", image)
     x = torch.as_tensor(image)
     L.ensure_device(x)
-    x = x.to(torch.float32).contiguous()
+    x = x.to(torch.float32).resolve_conj().contiguous()
     normal = torch.randn(x.shape, generator=_generator(key, x.device), device=x.device,
                          dtype=torch.float32)
     out = torch.empty_like(x)
@@ -194,6 +257,7 @@ def approximate_shot_noise(key, image) -> torch.Tensor:
 
 def shot_noise(key, image) -> torch.Tensor:
     """Poisson samples with mean ``image``, as float32 (noise.py:38-45)."""
+    L.refuse_grad("shot_noise", image)
     x = torch.as_tensor(image)
     L.ensure_device(x)
     x = x.to(torch.float32)

@@ -205,7 +205,7 @@ def _sas_czt(r: SasCztRequest, stream):
         ones = torch.ones((N, N), dtype=torch.complex64, device=r.u.device)
         outs = []
         for w in range(len(r.wavelengths)):
-            fw = ScalarField(s.u[..., w].contiguous(), s.dx[w], 0.0, float(r.wavelengths[w]))
+            fw = ScalarField(s.u[..., w].resolve_conj().contiguous(), s.dx[w], 0.0, float(r.wavelengths[w]))
             z = czt_kernel_propagate(fw, ones, output_dx=fw.dx / r.zoom, output_shape=(r.m, r.m))
             outs.append(z.u)
         return torch.stack(outs, dim=-1)

@@ -109,6 +109,8 @@ class Microscope:
         the planes (``planewise_convolution_sum``) when a BasicSensor sums
         axis 0 of it; None otherwise."""
         sample = torch.as_tensor(sample)
+        if torch.is_grad_enabled() and (sample.requires_grad or torch.as_tensor(psf).requires_grad):
+            return None  # the differentiable convolution and sensor sum instead
         axes = tuple(a % sample.ndim for a in self.convolution_axes) if sample.ndim else ()
         if (not isinstance(self.sensor, BasicSensor) or sample.ndim != 3 or axes != (1, 2)
                 or not isinstance(self.sensor.reduce_axis, int) or self.sensor.reduce_axis % 3 != 0

@@ -35,7 +35,7 @@ def czt_axis(x: torch.Tensor, m: int, a, w, axis: int, *, num_param: int = 1, mo
     x = torch.as_tensor(x)
     L.refuse_grad("czt", x)
     L.ensure_device(x)
-    x = x.to(torch.complex64).contiguous()
+    x = x.to(torch.complex64).resolve_conj().contiguous()
     H, W = x.shape[-2:]
     B = int(np.prod(x.shape[:-2])) if x.ndim > 2 else 1
     d = L.CztDesc()
@@ -102,7 +102,7 @@ def zoom_dft(x, m, a, w, mod_n=(-1, -1), scale=(1.0, 1.0)) -> torch.Tensor:
     x = torch.as_tensor(x)
     L.refuse_grad("zoom_dft", x)
     L.ensure_device(x)
-    x = x.to(torch.complex64).contiguous()
+    x = x.to(torch.complex64).resolve_conj().contiguous()
     H, W = x.shape[-2:]
     d = L.ZoomDesc()
     d.batch = int(np.prod(x.shape[:-2])) if x.ndim > 2 else 1
@@ -158,7 +158,7 @@ def scaled_conv_axis(x: torch.Tensor, axis: int, alpha: float, df: float, out_dx
     x = torch.as_tensor(x)
     L.refuse_grad("kernel_propagate (scaled convolution)", x)
     L.ensure_device(x)
-    x = x.to(torch.complex64).contiguous()
+    x = x.to(torch.complex64).resolve_conj().contiguous()
     H, W = x.shape[-2:]
     d = L.YuDesc()
     d.batch = int(np.prod(x.shape[:-2])) if x.ndim > 2 else 1

@@ -163,3 +163,82 @@ def test_optical_fft_gradient_matches_autodiff_of_the_formula(z):
     gg = _grads_gpu(lambda x: cf.optical_fft(cx.ScalarField(x, dx, 0.0, lam), z, n).u, [u], c)
     gc = _grads_cpu(cpu, [u], c)
     assert rel_l2(gg[0], gc[0]) < TOL
+
+
+def _conv_ref(a, k, axes, mode):
+    """Linear convolution over `axes` by FFT in complex128 (torch, CPU), the
+    window of the reference's fourier_convolution (start (k-1)//2 in "same",
+    0 in "full" with the n + k transform shape)."""
+    sz = [a.shape[ax] + k.shape[ax] - 1 for ax in axes]
+    A = torch.fft.fftn(a, s=sz, dim=axes)
+    Kf = torch.fft.fftn(k, s=sz, dim=axes)
+    full = torch.fft.ifftn(A * Kf, s=sz, dim=axes)
+    if mode == "same":
+        for ax in axes:
+            full = full.narrow(ax, (k.shape[ax] - 1) // 2, a.shape[ax])
+    else:
+        full = torch.nn.functional.pad(full, sum([[0, 1] for _ in reversed(axes)], []))
+    return full
+
+
+@pytest.mark.parametrize("ishape,kshape,axes,mode,cplx", [
+    ((3, 40, 36), (3, 9, 12), (1, 2), "same", False),
+    ((3, 40, 36), (1, 8, 7), (1, 2), "same", False),    # kernel broadcast over the batch
+    ((2, 30, 28), (2, 5, 6), (1, 2), "full", True),
+    ((12, 14, 10), (3, 4, 5), (0, 1, 2), "same", True),
+])
+def test_fourier_convolution_gradients(ishape, kshape, axes, mode, cplx):
+    from chromatix_amd.ops import fourier_convolution
+
+    rng = np.random.default_rng(11)
+    a = _rand_c(rng, ishape) if cplx else rng.standard_normal(ishape).astype(np.float32)
+    k = _rand_c(rng, kshape) if cplx else rng.standard_normal(kshape).astype(np.float32)
+    out_shape = tuple(fourier_convolution(torch.as_tensor(a).cuda(), torch.as_tensor(k).cuda(),
+                                          axes=axes, mode=mode, fast_fft_shape=False).shape)
+    c = _rand_c(rng, out_shape) if cplx else rng.standard_normal(out_shape).astype(np.float32)
+
+    def gpu(a_, k_):
+        return fourier_convolution(a_, k_, axes=axes, mode=mode, fast_fft_shape=False)
+
+    def cpu(a_, k_):
+        r = _conv_ref(a_, k_, axes, mode)
+        return r if cplx else r.real
+
+    gg = _grads_gpu(gpu, [a, k], c)
+    gc = _grads_cpu(cpu, [a, k], c)
+    for x, y in zip(gg, gc):
+        assert x.shape == y.shape
+        assert rel_l2(x, y) < TOL
+
+
+def test_microscope_phase_mask_gradient_finite_differences():
+    """Microscope(Optical4FSystemPSF) image -> mean squared error against a
+    target; d loss / d phase along a random direction against a central
+    difference of the loss (the reference differentiates this chain with
+    JAX through fourier_convolution, microscopes.py:190-297)."""
+    from chromatix_amd.elements import BasicSensor
+    from chromatix_amd.systems import Microscope, Optical4FSystemPSF
+
+    rng = np.random.default_rng(12)
+    n, D = 64, 3
+    z = np.linspace(-2, 2, D)
+    phase0 = torch.as_tensor(rng.uniform(-np.pi, np.pi, (n, n)).astype(np.float32)).cuda()
+    volume = torch.as_tensor(rng.random((D, n, n)).astype(np.float32)).cuda()
+    target = torch.as_tensor(rng.random((n, n)).astype(np.float32)).cuda()
+    sensor = BasicSensor((n, n), 0.3, resampling_method=None, reduce_axis=0)
+
+    def loss(ph):
+        m = Microscope(Optical4FSystemPSF((n, n), 0.3, 100.0, ph), sensor, f=100.0, n=1.33,
+                       NA=0.8, spectrum=0.532)
+        img = m(volume, z)
+        return ((img - target) ** 2).mean()
+
+    ph = phase0.clone().requires_grad_(True)
+    (g,) = torch.autograd.grad(loss(ph), ph)
+    v = torch.as_tensor(rng.standard_normal((n, n)).astype(np.float32)).cuda()
+    eps = 1e-2
+    with torch.no_grad():
+        fd = (loss(phase0 + eps * v) - loss(phase0 - eps * v)).item() / (2 * eps)
+    ad = float((g * v).sum().item())
+    print(f"microscope d loss: autograd {ad:.6e}  central difference {fd:.6e}")
+    assert abs(ad - fd) <= 2e-3 * abs(fd)

@@ -467,3 +467,19 @@ def test_arbitrary_grid_adjoint_identity():
     (g,) = torch.autograd.grad((out.u * c.conj()).real.sum(), u)
     rhs = torch.vdot(u.detach().reshape(-1), g.reshape(-1))
     assert abs((lhs - rhs).item()) / abs(lhs.item()) < 1e-5
+
+
+def test_lazily_conjugated_inputs_are_read_conjugated():
+    """torch's conj() is a view with a conjugate bit over the same memory;
+    the library reads device pointers, so the wrappers resolve it first."""
+    from chromatix_amd.ops import fourier_convolution
+
+    u = _phase_field((64, 48), seed=31)
+    ut = torch.as_tensor(u).cuda()
+    got = cf.asm_propagate(cx.ScalarField(ut.conj(), 0.3, 0.0, LAM), 20.0, NM, 8, mode="same")
+    ref = _oracle(np.conj(u), 0.3, 20.0, 8, "asm", "same", "ref32")
+    assert rel_l2(got.u.cpu().numpy(), ref) < TOL
+    k = torch.as_tensor(_phase_field((5, 7), seed=32)).cuda()
+    a = fourier_convolution(ut.conj(), k.conj(), axes=(0, 1))
+    b = fourier_convolution(torch.conj_physical(ut), torch.conj_physical(k), axes=(0, 1))
+    assert torch.equal(a, b)

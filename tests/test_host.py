@@ -148,7 +148,6 @@ def test_entry_points_without_adjoints_refuse_inputs_that_require_grad():
         lambda: cf.asm_propagate(f, 100.0, 1.33, 0, output_dx=0.2, output_shape=(4, 4)),
         lambda: czt(u, 4, 1.0, 1.0),
         lambda: scaled_conv_axis(u, 0, 1.0, 1.0, 1.0, 0.0),
-        lambda: fourier_convolution(u.real, ph, axes=(0, 1)),
         lambda: planewise_convolution_sum(u.real[None], ph[None]),
     ]
     for i, call in enumerate(calls):
