@@ -177,8 +177,7 @@ def transform_propagate_sas(field: Field, z, n, cval: float = 0,
                             skip_initial_phase: bool = False, skip_final_phase: bool = False, *,
                             phase_mode=None) -> Field:
     """Scalable angular spectrum propagation (padding factor 2, as the
-    reference supports)."""
-    L.refuse_grad("transform_propagate_sas", field.u)
+    reference supports); differentiable in the field."""
     if cval != 0:
         raise NotImplementedError("transform_propagate_sas with cval != 0")
     z = float(z)
@@ -186,9 +185,91 @@ def transform_propagate_sas(field: Field, z, n, cval: float = 0,
     d.skip_initial_phase = int(bool(skip_initial_phase))
     d.skip_final_phase = int(bool(skip_final_phase))
     ny, nx = field.spatial_shape
-    u = _run(field, d, "sas", (ny, nx))
     grid = (ny + 2 * (ny // 2), nx + 2 * (nx // 2))
+    if torch.is_grad_enabled() and field.u.requires_grad:
+        u = _SasFn.apply(field.u, field, d, keep, float(n), phase_mode)
+    else:
+        u = _run(field, d, "sas", (ny, nx))
     return field.replace(u=u, dx=_new_spacing(field, z, n, grid))
+
+
+def _spatial_axes(field: Field) -> tuple[int, int]:
+    return (-3, -2) if isinstance(field, ChromaticScalarField) else (-2, -1)
+
+
+def _sas_conj_dh_mul(field: Field, v: torch.Tensor, z: float, n: float, spacing,
+                     phase_mode) -> torch.Tensor:
+    """ifft2(conj(dH) fft2(v)) on v's (padded) grid with the SAS
+    precompensation of `spacing` (conj(dH(z)) = dH(-z): the window depends on
+    |z| only): the adjoint of the precompensation step."""
+    ax = _spatial_axes(field)
+    ny, nx = v.shape[ax[0]], v.shape[ax[1]]
+    chrom = isinstance(field, ChromaticScalarField)
+    d, keep = _desc(field, L.CXM_OFFT_TRANSFORM, -z, n, phase_mode)
+    d.height, d.width = ny, nx
+    sp = L.doubles(np.asarray(spacing, np.float64).reshape(-1))
+    keep.append(sp)
+    if chrom:
+        d.spacing = ctypes.cast(sp, ctypes.POINTER(ctypes.c_double))
+    else:
+        d.dy, d.dx = float(sp[0]), float(sp[1])
+    nwv = d.num_wv
+    dh = torch.empty((nwv, ny, nx), dtype=torch.complex64, device=v.device)
+    L.check(L.lib().cxm_sas_precompensation_c64(L.stream_ptr(v.device), dh.data_ptr(),
+                                                ctypes.byref(d)))
+    spec = fft(v, axes=ax, shift=False)
+    pl = spec.movedim(-1, -3).resolve_conj().contiguous() if chrom else spec.resolve_conj().contiguous()
+    L.check(L.lib().cxm_cmul_c64(L.stream_ptr(v.device), pl.data_ptr(), dh.data_ptr(),
+                                 pl.data_ptr(), pl.numel() // (ny * nx), ny, nx, nwv, 0))
+    spec = pl.movedim(-3, -1) if chrom else pl
+    return ifft(spec, axes=ax, shift=False)
+
+
+def _sas_adjoint(field: Field, d, g: torch.Tensor, n: float, phase_mode) -> torch.Tensor:
+    """Adjoint of transform_propagate_sas (propagation.py:106-165) on the
+    cotangent g of its output: z >= 0: crop(K^H(T^H(pad(g)))), z < 0:
+    crop(T^H(K'^H(pad(g)))), with T the transform step on the padded grid
+    (optical-FFT adjoint), K the precompensation on the input grid and K'
+    on the transformed grid."""
+    ny, nx = field.spatial_shape
+    py, px = ny // 2, nx // 2
+    Ny, Nx = ny + 2 * py, nx + 2 * px
+    ax = _spatial_axes(field)
+    z = float(d.z)
+    chrom = isinstance(field, ChromaticScalarField)
+    dt = L.OptFftDesc.from_buffer_copy(d)
+    dt.height, dt.width = Ny, Nx
+    dt.pad_y = dt.pad_x = 0
+    dt.adjoint = 1
+    g = g.to(torch.complex64).resolve_conj().contiguous()
+
+    def crop(v):
+        return v.narrow(ax[0] % v.ndim, py, ny).narrow(ax[1] % v.ndim, px, nx)
+
+    if z >= 0:
+        dt.crop_y, dt.crop_x = py, px
+        v = _run(field.replace(u=g), dt, "optical_fft", (Ny, Nx))
+        v = _sas_conj_dh_mul(field, v, z, n, field.dx, phase_mode)
+        return crop(v).resolve_conj().contiguous()
+    pads = [0, 0] * (1 if chrom else 0) + [px, px, py, py]
+    v = torch.nn.functional.pad(g, pads)
+    du = _new_spacing(field, z, n, (Ny, Nx))
+    v = _sas_conj_dh_mul(field, v, z, n, du, phase_mode)
+    dt.crop_y = dt.crop_x = 0
+    v = _run(field.replace(u=v.resolve_conj().contiguous()), dt, "optical_fft", (Ny, Nx))
+    return crop(v).resolve_conj().contiguous()
+
+
+class _SasFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, u, field, d, keep, n, phase_mode):
+        ctx.field, ctx.d, ctx.keep, ctx.n, ctx.pm = field, d, keep, n, phase_mode
+        ny, nx = field.spatial_shape
+        return _run(field.replace(u=u.detach()), d, "sas", (ny, nx))
+
+    @staticmethod
+    def backward(ctx, g):
+        return _sas_adjoint(ctx.field, ctx.d, g, ctx.n, ctx.pm), None, None, None, None, None
 
 
 def compute_sas_precompensation(field: Field, z, n, *, phase_mode=None) -> torch.Tensor:

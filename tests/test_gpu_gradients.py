@@ -109,7 +109,7 @@ WL3 = np.array([0.45, 0.532, 0.65])
 @pytest.mark.parametrize("chromatic", [False, True])
 @pytest.mark.parametrize("z", [150.0, -90.0])
 @pytest.mark.parametrize("kind", ["optical_fft", "ff_lens_na", "ff_lens_inverse", "transform",
-                                  "transform_skip", "fft_shift", "ifft"])
+                                  "transform_skip", "fft_shift", "ifft", "sas", "sas_skip"])
 def test_optical_family_adjoint_identity(kind, z, chromatic):
     H, W = 96, 80
     in_shape = (2, H, W, 3) if chromatic else (2, H, W)
@@ -126,6 +126,9 @@ def test_optical_family_adjoint_identity(kind, z, chromatic):
         fwd = lambda f: cf.ff_lens(f, abs(z), 1.33, NA=0.3).u
     elif kind == "ff_lens_inverse":
         fwd = lambda f: cf.ff_lens(f, abs(z), 1.33, inverse=True).u
+    elif kind in ("sas", "sas_skip"):
+        skip = kind == "sas_skip"
+        fwd = lambda f: cf.transform_propagate_sas(f, z, 1.33, skip_final_phase=skip).u
     elif kind in ("transform", "transform_skip"):
         skip = kind == "transform_skip"
         fwd = lambda f: cf.transform_propagate(f, z, 1.33, (16, 24), skip_initial_phase=skip).u

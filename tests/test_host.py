@@ -143,7 +143,6 @@ def test_entry_points_without_adjoints_refuse_inputs_that_require_grad():
     f = cx.ScalarField(u, 0.3, 0.0, 0.532)
     ph = torch.zeros((8, 8), requires_grad=True)
     calls = [
-        lambda: cf.transform_propagate_sas(f, 100.0, 1.33),
         lambda: cf.high_na_ff_lens(f, 100.0, 1.33, 0.5),
         lambda: cf.asm_propagate(f, 100.0, 1.33, 0, output_dx=0.2, output_shape=(4, 4)),
         lambda: czt(u, 4, 1.0, 1.0),
