@@ -130,6 +130,57 @@ class MBS:
             x = x + r
         return x, r
 
+    def adjoint(self) -> "MBS":
+        """The operator set of the conjugated problem: eps -> eps^H per voxel,
+        scale -> conj(scale), kb -> conj(kb).  Its G is the adjoint of this
+        G (the Fourier multipliers conjugate, the projections are real
+        symmetric) and its prec_forward = -V^H (G^H V^H + I) is the adjoint
+        of -V (G V + I): the operator the implicit differentiation of solve
+        (jaxopt FixedPointIteration, implicit_diff=True, solver.py:136-142)
+        solves with."""
+        o = MBS.__new__(MBS)
+        o.eps = np.conj(np.swapaxes(self.eps, 0, 1)) if self.tensor else np.conj(self.eps)
+        o.tensor, o.K, o.K2 = self.tensor, self.K, self.K2
+        o.scale, o.kb = np.conj(self.scale), np.conj(self.kb)
+        return o
+
+    def vjp(self, S, E, Ebar, tol=1e-12, maxiter=20000):
+        """Reverse-mode rule of E = solve(eps, S) (scalar eps) for the
+        cotangent Ebar (dL/dRe E + i dL/dIm E).  With A = prec_forward and
+        b = -V G(scale S) (E = A^-1 b; the num_scale normalisation cancels):
+          A^H lam = Ebar  (the same iteration on the adjoint operators),
+          mu = G^H(V^H lam),  w = G(V E) + E - G(scale S),
+          eps_bar = conj(scale sum_c (conj(lam_c) w_c + conj(mu_c) E_c)),
+          S_bar = -conj(scale) mu.
+        Returns (eps_bar, S_bar, lam)."""
+        if self.tensor:
+            raise NotImplementedError("vjp: scalar permittivity")
+        adj = self.adjoint()
+        lam = np.asarray(Ebar, np.complex128).copy()
+        nb = np.linalg.norm(Ebar)
+        for _ in range(maxiter):
+            r = Ebar - adj.prec_forward(lam)
+            lam = lam + r
+            if np.linalg.norm(r) <= tol * nb:
+                break
+        mu = adj.G(adj.V(lam))
+        w = self.G(self.V(E)) + E - self.G(S * self.scale)
+        eps_bar = np.conj(self.scale * np.sum(np.conj(lam) * w + np.conj(mu) * E, axis=-1))
+        S_bar = -np.conj(self.scale) * mu
+        return eps_bar, S_bar, lam
+
+    def solve_exact(self, S, tol=1e-13, maxiter=20000):
+        """E = A^-1 b by the fixed-point iteration run to `tol` (FP64)."""
+        b = -self.V(self.G(S * self.scale))
+        x = b.copy()
+        nb = np.linalg.norm(b)
+        for _ in range(maxiter):
+            r = b - self.prec_forward(x)
+            x = x + r
+            if np.linalg.norm(r) <= tol * nb:
+                break
+        return x
+
     def pde_residual(self, E, S):
         """|(|K|^2 P_T - eps) E + S| / |S|, the equation the fixed point solves."""
         Ef = np.fft.fftn(E, axes=ax3)
