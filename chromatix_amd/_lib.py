@@ -357,6 +357,8 @@ def _declare(lib):
                                         vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_bicgstab_iterate": (i32, [vp, vp, vp, vp, sz, ctypes.c_int32, vp, vp, sz,
                                            ctypes.POINTER(MbsDesc)]),
+        "cxm_mbs_vjp_workspace_size": (sz, [ctypes.POINTER(MbsDesc)]),
+        "cxm_mbs_vjp": (i32, [vp, vp, vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_czt_axis_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(CztDesc)]),
         "cxm_fluor_multislice_workspace_size": (sz, [ctypes.POINTER(FluorDesc)]),
         "cxm_fluor_multislice_sample_c64": (i32, [vp, vp, vp, vp, vp, vp, vp, sz,

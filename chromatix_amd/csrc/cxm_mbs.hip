@@ -1252,3 +1252,105 @@ extern "C" int cxm_enclosing_circle_c64(void* stream, const void* values, int64_
   out[2] = std::sqrt(std::max(r2, c.r2));
   return CXM_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Reverse-mode rule of the solve (the implicit differentiation of
+// solver.py:136-142, jaxopt FixedPointIteration with implicit_diff=True).
+// With A = prec_forward = -V (G V + I) and b = -V G(s S), E = A^-1 b (the
+// num_scale normalisation cancels).  Given lam solving A^H lam = Ebar (the
+// same iteration on the conjugated problem: A^H = -V^H (G^H V^H + I), V^H
+// and G^H the operators of conj(eps), conj(s), conj(kb)):
+//   mu = G^H(V^H lam),  w = G(V E) + E - G(s S),
+//   eps_bar = conj(s sum_c (conj(lam_c) w_c + conj(mu_c) E_c)),
+//   S_bar = -conj(s) mu.
+// ---------------------------------------------------------------------------
+namespace cxm {
+namespace {
+// out[c][i] = conj(s eps[i] - kb) lam[c][i] (planar)
+__global__ void mbs_vconj_kernel(const float2* __restrict__ lam, const float2* __restrict__ eps,
+                                 float2* __restrict__ out, long long vox, C2 s, C2 kb) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vox) return;
+  const float2 e = eps[i];
+  const C2 f{s.re * e.x - s.im * e.y - kb.re, -(s.re * e.y + s.im * e.x - kb.im)};
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[c * vox + i] = cmulf(lam[c * vox + i], f);
+}
+
+// grad_eps[i] = conj(s sum_c (conj(lam_c) (gve_c + E_c - gss_c) + conj(mu_c) E_c)),
+// grad_src[i][c] = -conj(s) mu_c; E and grad_src interleaved, the rest planar
+__global__ void mbs_vjp_kernel(const float2* __restrict__ gve, const float2* __restrict__ gss,
+                               const float2* __restrict__ E, const float2* __restrict__ lam,
+                               const float2* __restrict__ mu, float2* __restrict__ grad_eps,
+                               float2* __restrict__ grad_src, long long vox, C2 s) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= vox) return;
+  float ar = 0.f, ai = 0.f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float2 e = E[i * 3 + c], l = lam[c * vox + i], m = mu[c * vox + i];
+    const float2 a = gve[c * vox + i], b = gss[c * vox + i];
+    const float wr = a.x + e.x - b.x, wi = a.y + e.y - b.y;
+    // conj(l) w + conj(m) e
+    ar += l.x * wr + l.y * wi + m.x * e.x + m.y * e.y;
+    ai += l.x * wi - l.y * wr + m.x * e.y - m.y * e.x;
+    if (grad_src) {
+      // -conj(s) m
+      grad_src[i * 3 + c] = make_float2(-(s.re * m.x + s.im * m.y), -(s.re * m.y - s.im * m.x));
+    }
+  }
+  if (grad_eps) {
+    // conj(s acc)
+    grad_eps[i] = make_float2(s.re * ar - s.im * ai, -(s.re * ai + s.im * ar));
+  }
+}
+}  // namespace
+}  // namespace cxm
+
+extern "C" size_t cxm_mbs_vjp_workspace_size(const cxm_mbs_desc* d) {
+  MbsGeom g;
+  if (mbs_geom(d, g)) return 0;
+  return (size_t)9 * g.vox * sizeof(float2);
+}
+
+extern "C" int cxm_mbs_vjp(void* stream, const void* eps, const void* source, const void* E,
+                           const void* lam, void* grad_eps, void* grad_source, void* workspace,
+                           size_t workspace_bytes, const cxm_mbs_desc* d) {
+  MbsGeom g;
+  int rc = mbs_geom(d, g);
+  if (rc) return rc;
+  if (d->eps_tensor) return fail(CXM_ERR_UNSUPPORTED, "the solve's gradient: scalar permittivity only");
+  if (!eps || !source || !E || !lam) return fail(CXM_ERR_INVALID, "null buffer");
+  if (!workspace || workspace_bytes < cxm_mbs_vjp_workspace_size(d))
+    return fail(CXM_ERR_WORKSPACE, "workspace too small");
+  if (!grad_eps && !grad_source) return CXM_OK;
+  if ((rc = ensure_init())) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const C2 sc = c2(d->scale_re, d->scale_im);
+  const C2 kb = c2(d->bias_re * d->scale_re - d->bias_im * d->scale_im + 1.0,
+                   d->bias_re * d->scale_im + d->bias_im * d->scale_re);
+  const C2 scc{sc.re, -sc.im}, kbc{kb.re, -kb.im};
+  float2* gve = (float2*)workspace;
+  float2* gss = gve + 3 * g.vox;
+  float2* mu = gss + 3 * g.vox;
+  // G(V E)
+  hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)E,
+                     (const float2*)eps, gve, g.vox, 1, 0, sc, kb, C2{1.0f, 0.0f});
+  CXM_LAUNCH_CHECK();
+  if ((rc = green_apply(g, d, gve, sc, kb, s))) return rc;
+  // G(s S)
+  hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)source,
+                     nullptr, gss, g.vox, 1, 0, sc, kb, sc);
+  CXM_LAUNCH_CHECK();
+  if ((rc = green_apply(g, d, gss, sc, kb, s))) return rc;
+  // mu = G^H(V^H lam)
+  hipLaunchKernelGGL(mbs_vconj_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)lam,
+                     (const float2*)eps, mu, g.vox, sc, kb);
+  CXM_LAUNCH_CHECK();
+  if ((rc = green_apply(g, d, mu, scc, kbc, s))) return rc;
+  hipLaunchKernelGGL(mbs_vjp_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)gve,
+                     (const float2*)gss, (const float2*)E, (const float2*)lam, (const float2*)mu,
+                     (float2*)grad_eps, (float2*)grad_source, g.vox, sc);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}

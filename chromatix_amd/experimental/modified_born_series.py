@@ -368,16 +368,9 @@ class BiCGStabSolver(Solver):
         return self.stopped or self.iterations >= self.maxiter
 
 
-def solve(sample: Sample, source: Source, initial_E=None, maxiter: int = 1000, tol: float = 1e-3,
-          implicit_diff: bool = True, use_bicgstab: bool = False, *, rtol: float | None = None,
-          check_every: int = 8, shift_scale=None, return_info: bool = False):
-    """Electric field E (Nx, Ny, Nz, 3) solving the scattering problem
-    (solver.py:101-146).  Fixed-point iteration (default): stops when the l2
-    norm of the update of the normalised iterate falls below ``tol``
-    (jaxopt's FixedPointIteration criterion) or, with ``rtol``, below
-    rtol |prec_y|.  ``use_bicgstab``: BiCGStab until |r| <= tol |prec_y|
-    (jax.scipy.sparse.linalg.bicgstab's criterion) or breakdown.  The norm
-    is read back every ``check_every`` iterations."""
+def _run(sample: Sample, source: Source, initial_E, maxiter, tol, use_bicgstab, rtol, check_every,
+         shift_scale):
+    """The forward solve; returns the stepped solver and E."""
     if use_bicgstab:
         if rtol is not None:
             raise ValueError("rtol applies to the fixed-point iteration; BiCGStab's tol is relative")
@@ -389,10 +382,7 @@ def solve(sample: Sample, source: Source, initial_E=None, maxiter: int = 1000, t
             sv.step(check_every)
             sv.event.synchronize()
             sv.poll()
-        E = sv.result()
-        if return_info:
-            return MBSResult(E, sv.k, sv.residual)
-        return E
+        return sv, sv.result()
     sv = Solver(sample, source, initial_E, shift_scale=shift_scale, tol=tol, rtol=rtol,
                 maxiter=maxiter)
     while True:
@@ -401,7 +391,152 @@ def solve(sample: Sample, source: Source, initial_E=None, maxiter: int = 1000, t
         sv.poll()
         if sv.converged:
             break
-    E = sv.result()
+    return sv, sv.result()
+
+
+def _adjoint_desc(d):
+    """The desc of the conjugated problem: bias and scale conjugated (kb
+    follows), the operators of A^H = -V^H (G^H V^H + I)."""
+    a = L.MbsDesc.from_buffer_copy(d)
+    a.bias_im, a.scale_im = -d.bias_im, -d.scale_im
+    return a
+
+
+def _adjoint_solve(eps, d, Ebar, *, tol, rtol, maxiter, use_bicgstab, check_every):
+    """lam (3, Nx, Ny, Nz planar) solving A^H lam = Ebar, A the solve's
+    prec_forward: the same iteration as the forward solve on conj(eps) and
+    the conjugated desc, with the right-hand side normalised to max |.| = 1
+    as the forward normalises prec_y, and the forward's stopping rule."""
+    dev = eps.device
+    lib = L.lib()
+    da = _adjoint_desc(d)
+    eps_c = torch.conj_physical(eps).contiguous()
+    b = Ebar.permute(3, 0, 1, 2).contiguous()
+    norm = float(b.abs().max())
+    if norm == 0.0:
+        return torch.zeros_like(b)
+    if not np.isfinite(norm):
+        raise ValueError("the gradient of the solve's output is not finite")
+    py = b / norm
+    x = py.clone()
+    s = L.stream_ptr(dev)
+    ws = torch.empty(lib.cxm_mbs_workspace_size(ctypes.byref(da)), dtype=torch.uint8, device=dev)
+    it = 0
+    if use_bicgstab:
+        nbytes = lib.cxm_mbs_bicgstab_state_size(ctypes.byref(da))
+        state = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        status = torch.zeros(4, dtype=torch.float64, device=dev)
+        L.check(lib.cxm_mbs_bicgstab_init(s, eps_c.data_ptr(), py.data_ptr(), x.data_ptr(),
+                                          state.data_ptr(), nbytes, float(tol), 0.0, status.data_ptr(),
+                                          ws.data_ptr(), ws.numel(), ctypes.byref(da)))
+        while it < maxiter and not bool(status[2].item()):
+            k = min(check_every, maxiter - it)
+            L.check(lib.cxm_mbs_bicgstab_iterate(s, eps_c.data_ptr(), x.data_ptr(), state.data_ptr(),
+                                                 nbytes, k, status.data_ptr(), ws.data_ptr(), ws.numel(),
+                                                 ctypes.byref(da)))
+            it += k
+    else:
+        target = float(tol) if rtol is None else float(rtol) * float(torch.linalg.vector_norm(py))
+        res = torch.zeros(1, dtype=torch.float64, device=dev)
+        while it < maxiter:
+            k = min(check_every, maxiter - it)
+            L.check(lib.cxm_mbs_iterate(s, eps_c.data_ptr(), py.data_ptr(), x.data_ptr(), k,
+                                        res.data_ptr(), ws.data_ptr(), ws.numel(), ctypes.byref(da)))
+            it += k
+            if float(res.item()) ** 0.5 < target:
+                break
+    return x * norm
+
+
+class _BornSolveFn(torch.autograd.Function):
+    """E = solve(eps, S) with the implicit reverse-mode rule: the adjoint
+    system A^H lam = Ebar solved by the forward's own iteration, then
+    cxm_mbs_vjp for the permittivity and source gradients.  This is the
+    gradient jaxopt's implicit differentiation of the fixed point
+    (solver.py:136-142) and jax's custom_linear_solve rule of bicgstab
+    (solver.py:143-144) both produce; the shift and scale of
+    get_shift_and_scale are constants of it (the converged E does not depend
+    on them)."""
+
+    @staticmethod
+    def forward(ctx, eps, S, cfg):
+        sample = Sample(eps, cfg["spacing"], cfg["roi"])
+        src = Source.__new__(Source)
+        src.k0, src.field = cfg["k0"], S
+        sv, E = _run(sample, src, cfg["initial_E"], cfg["maxiter"], cfg["tol"], cfg["use_bicgstab"],
+                     cfg["rtol"], cfg["check_every"], cfg["shift_scale"])
+        Sc = S.to(device=sv.dev, dtype=torch.complex64).resolve_conj().contiguous()
+        ctx.save_for_backward(sv.eps, Sc, E)
+        ctx.d, ctx.cfg, cfg["solver"] = sv.d, cfg, sv
+        ctx.src_device = S.device
+        return E
+
+    @staticmethod
+    def backward(ctx, gE):
+        eps, S, E = ctx.saved_tensors
+        cfg = ctx.cfg
+        ge = gs = None
+        if gE is None or not (ctx.needs_input_grad[0] or ctx.needs_input_grad[1]):
+            return None, None, None
+        gE = gE.to(torch.complex64).resolve_conj().contiguous()
+        tol = cfg["adjoint_tol"] if cfg["adjoint_tol"] is not None else cfg["tol"]
+        rtol = None if cfg["use_bicgstab"] else cfg["rtol"]
+        if cfg["adjoint_tol"] is not None and rtol is not None:
+            rtol = cfg["adjoint_tol"]
+        lam = _adjoint_solve(eps, ctx.d, gE, tol=tol, rtol=rtol, maxiter=cfg["maxiter"],
+                             use_bicgstab=cfg["use_bicgstab"], check_every=cfg["check_every"])
+        lib = L.lib()
+        if ctx.needs_input_grad[0]:
+            ge = torch.empty_like(eps)
+        if ctx.needs_input_grad[1]:
+            gs = torch.empty_like(S)
+        ws = torch.empty(lib.cxm_mbs_vjp_workspace_size(ctypes.byref(ctx.d)), dtype=torch.uint8,
+                         device=eps.device)
+        L.check(lib.cxm_mbs_vjp(L.stream_ptr(eps.device), eps.data_ptr(), S.data_ptr(), E.data_ptr(),
+                                lam.data_ptr(), ge.data_ptr() if ge is not None else None,
+                                gs.data_ptr() if gs is not None else None, ws.data_ptr(), ws.numel(),
+                                ctypes.byref(ctx.d)))
+        if gs is not None and gs.device != ctx.src_device:
+            gs = gs.to(ctx.src_device)
+        return ge, gs, None
+
+
+def solve(sample: Sample, source: Source, initial_E=None, maxiter: int = 1000, tol: float = 1e-3,
+          implicit_diff: bool = True, use_bicgstab: bool = False, *, rtol: float | None = None,
+          check_every: int = 8, shift_scale=None, return_info: bool = False,
+          adjoint_tol: float | None = None):
+    """Electric field E (Nx, Ny, Nz, 3) solving the scattering problem
+    (solver.py:101-146).  Fixed-point iteration (default): stops when the l2
+    norm of the update of the normalised iterate falls below ``tol``
+    (jaxopt's FixedPointIteration criterion) or, with ``rtol``, below
+    rtol |prec_y|.  ``use_bicgstab``: BiCGStab until |r| <= tol |prec_y|
+    (jax.scipy.sparse.linalg.bicgstab's criterion) or breakdown.  The norm
+    is read back every ``check_every`` iterations.
+
+    Differentiable in the permittivity and the source field (scalar
+    permittivity) by implicit differentiation: the backward pass solves the
+    adjoint system with the same iteration and stopping rule (tolerance
+    ``adjoint_tol`` when given) -- the gradient jaxopt's implicit_diff=True
+    and the bicgstab rule of jax give.  ``initial_E`` gets no gradient (as
+    under implicit differentiation); implicit_diff=False, the reference's
+    differentiation through the unrolled iteration, is refused."""
+    eps = sample.permittivity
+    S = torch.as_tensor(source.field)
+    if torch.is_grad_enabled() and (eps.requires_grad or S.requires_grad):
+        if not implicit_diff:
+            raise NotImplementedError(
+                "solve: differentiation through the unrolled iteration (implicit_diff=False) is "
+                "not implemented; use implicit_diff=True")
+        if sample.tensor:
+            raise NotImplementedError("solve: gradients for a permittivity tensor are not implemented")
+        cfg = dict(spacing=sample.spacing, roi=sample.ROI, k0=source.k0, initial_E=initial_E,
+                   maxiter=maxiter, tol=tol, use_bicgstab=use_bicgstab, rtol=rtol,
+                   check_every=check_every, shift_scale=shift_scale, adjoint_tol=adjoint_tol)
+        E = _BornSolveFn.apply(eps.to(torch.complex64), S, cfg)
+        sv = cfg["solver"]
+    else:
+        sv, E = _run(sample, source, initial_E, maxiter, tol, use_bicgstab, rtol, check_every,
+                     shift_scale)
     if return_info:
-        return MBSResult(E, sv.iterations, sv.residual)
+        return MBSResult(E, sv.k if use_bicgstab else sv.iterations, sv.residual)
     return E

@@ -643,6 +643,35 @@ int cxm_mbs_output(void* stream, const void* x, const double* num_scale, void* e
  * owned by the caller between calls.  status: 4 device doubles written by every
  * call: |r|^2, k, stopped (0/1), |prec_y|^2.  x: 3 x nx x ny x nz planar.
  */
+size_t cxm_mbs_bicgstab_state_size(const cxm_mbs_desc* d);
+int cxm_mbs_bicgstab_init(void* stream, const void* eps, const void* prec_y, const void* x,
+                          void* state, size_t state_bytes, double tol, double atol, double* status,
+                          void* workspace, size_t workspace_bytes, const cxm_mbs_desc* d);
+int cxm_mbs_bicgstab_iterate(void* stream, const void* eps, void* x, void* state, size_t state_bytes,
+                             int32_t iterations, double* status, void* workspace,
+                             size_t workspace_bytes, const cxm_mbs_desc* d);
+
+/*
+ * Reverse-mode rule of the solve: the implicit differentiation that
+ * solve(..., implicit_diff=True) gets from jaxopt (solver.py:136-142).  With
+ * A = prec_forward and b = prec(S) (E = A^-1 b x num_scale / num_scale), the
+ * caller solves the adjoint system A^H lam = Ebar -- cxm_mbs_iterate or the
+ * BiCGStab entries on conj(eps) with the desc's bias and scale conjugated,
+ * right-hand side Ebar (3 x nx x ny x nz planar) -- and this entry returns
+ *   grad_eps[i]       = conj(s sum_c (conj(lam_c) w_c + conj(mu_c) E_c)),
+ *   grad_source[i][c] = -conj(s) mu_c,
+ * mu = G^H V^H lam, w = G V E + E - G (s S), s the desc's scale, G the Green's
+ * function of the desc, V = s eps - kb; E, S, grad_source nx x ny x nz x 3
+ * interleaved, lam planar.  grad_eps and grad_source may be NULL.  Gradients
+ * follow the conjugate-Wirtinger convention of torch: for a real loss L,
+ * grad = dL/dRe + i dL/dIm.  Scalar permittivity only (eps_tensor = 0).
+ * Workspace: cxm_mbs_vjp_workspace_size (three fields).
+ */
+size_t cxm_mbs_vjp_workspace_size(const cxm_mbs_desc* d);
+int cxm_mbs_vjp(void* stream, const void* eps, const void* source, const void* e, const void* lam,
+                void* grad_eps, void* grad_source, void* workspace, size_t workspace_bytes,
+                const cxm_mbs_desc* d);
+
 /*
  * Smallest circle enclosing n complex64 values (the shift of
  * get_shift_and_scale, solver.py:10-39, which minimises max |eps - shift|):
@@ -653,14 +682,6 @@ int cxm_mbs_output(void* stream, const void* x, const double* num_scale, void* e
 size_t cxm_enclosing_circle_workspace_size(int64_t n);
 int cxm_enclosing_circle_c64(void* stream, const void* values, int64_t n, double* out,
                              void* workspace, size_t workspace_bytes);
-
-size_t cxm_mbs_bicgstab_state_size(const cxm_mbs_desc* d);
-int cxm_mbs_bicgstab_init(void* stream, const void* eps, const void* prec_y, const void* x,
-                          void* state, size_t state_bytes, double tol, double atol, double* status,
-                          void* workspace, size_t workspace_bytes, const cxm_mbs_desc* d);
-int cxm_mbs_bicgstab_iterate(void* stream, const void* eps, void* x, void* state, size_t state_bytes,
-                             int32_t iterations, double* status, void* workspace,
-                             size_t workspace_bytes, const cxm_mbs_desc* d);
 
 /*
  * Intensity |u|^2 of n_pixels complex64 values, or for an innermost

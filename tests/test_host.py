@@ -154,3 +154,17 @@ def test_entry_points_without_adjoints_refuse_inputs_that_require_grad():
             call()
     with torch.no_grad():  # nothing is refused without grad mode
         L.refuse_grad("x", u)
+
+
+def test_born_solve_refuses_unsupported_gradients():
+    """solve differentiates implicitly (scalar permittivity); the unrolled
+    rule and the permittivity tensor raise before touching the device."""
+    from chromatix_amd.experimental import modified_born_series as mbs
+
+    eps = torch.ones((4, 4, 4), dtype=torch.complex64, requires_grad=True)
+    src = mbs.Source(torch.zeros((4, 4, 4, 3)), 11.8)
+    with pytest.raises(NotImplementedError, match="implicit_diff=True"):
+        mbs.solve(mbs.Sample(eps, 0.1), src, implicit_diff=False)
+    eps_t = torch.zeros((3, 3, 4, 4, 4), dtype=torch.complex64, requires_grad=True)
+    with pytest.raises(NotImplementedError, match="permittivity tensor"):
+        mbs.solve(mbs.Sample(eps_t, 0.1), src)
