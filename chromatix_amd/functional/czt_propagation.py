@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from .. import _lib as L
-from ..core.field import ChromaticScalarField, Field, crop, pad, shift_grid
+from ..core.field import ChromaticScalarField, Field, ScalarField, crop, pad, shift_grid
 from ..utils.czt import czt_axis, scaled_conv_axis, use_zoom_mfma, zoom_dft
 from .optics import fft as _fft_shifted
 from .propagation import (
@@ -37,7 +37,7 @@ def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=
     ``output_shape`` centred on ``shift_yx`` (propagation.py:395-443)."""
     L.refuse_grad("kernel_propagate with an output grid", field.u, propagator)
     if isinstance(field, ChromaticScalarField):
-        raise NotImplementedError("output-grid propagation: monochromatic scalar fields only")
+        return _czt_chromatic(field, propagator, output_dx, output_shape, shift_yx, use_czt)
     ny, nx = field.spatial_shape
     if output_shape is None:
         output_shape = (ny, nx)
@@ -94,6 +94,39 @@ def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=
         m, a, w, nmod, scale = pars[d]
         res = czt_axis(res, m, a, w, d, mod_n=nmod, scale=scale)
     return out_field.replace(u=res)
+
+
+def _czt_chromatic(field: ChromaticScalarField, propagator, output_dx, output_shape, shift_yx,
+                   use_czt) -> ChromaticScalarField:
+    """The output-grid branch per wavelength: each wavelength on its own
+    input grid (dx[w], extent dx[w] N) and output grid (output_dx[w], or
+    output_dx for all), a propagator with a trailing wavelength axis taken
+    per wavelength.  The reference's branch indexes the field extent by axis
+    (T[d], propagation.py:420-424), which for a (wv, 2) extent of a
+    chromatic field selects a wavelength instead of an axis; per wavelength
+    the result is the monochromatic one of the same grid."""
+    nwv = field.spectrum.size
+    ny, nx = field.spatial_shape
+    H = torch.as_tensor(propagator)
+    per_wv = H.ndim >= 3 and H.shape[-1] == nwv and tuple(H.shape[-3:-1]) == (ny, nx)
+    if output_dx is None:
+        odx = field.dx.copy()
+    else:
+        o = np.asarray(output_dx, dtype=np.float64)
+        if o.ndim == 2 and o.shape != (nwv, 2):
+            raise ValueError(f"output_dx of a chromatic field: (2,) or ({nwv}, 2), got {o.shape}")
+        odx = np.broadcast_to(o if o.ndim == 2 else np.broadcast_to(o, (2,)), (nwv, 2)).copy()
+    outs, first = [], None
+    for w in range(nwv):
+        mono = ScalarField(field.u[..., w], field.dx[w], field.origin[w],
+                           float(field.spectrum.wavelength[w]))
+        out = czt_kernel_propagate(mono, H[..., w] if per_wv else H, odx[w], output_shape, shift_yx,
+                                   use_czt)
+        if first is None:
+            first = out
+        outs.append(out.u)
+    u = torch.stack(outs, dim=-1)
+    return field.replace(u=u, dx=odx, origin=np.broadcast_to(first.origin, (nwv, 2)).copy())
 
 
 def czt_kernel_propagate_fields(method: str, field: Field, z, n, pad_width, cval, kykx, shift_yx,

@@ -37,7 +37,7 @@ from .core.field import ChromaticScalarField, ScalarField
 from .core.spectrum import Spectrum
 
 __all__ = ["PropRequest", "SasCztRequest", "BornRequest", "ServingEngine", "run_plain",
-           "latency_stats"]
+           "latency_stats", "validate", "Admission"]
 
 
 @dataclass
@@ -54,6 +54,7 @@ class PropRequest:
     arrival: float = 0.0
     result: torch.Tensor | None = None
     done: float | None = None
+    error: str | None = None
 
 
 @dataclass
@@ -71,6 +72,7 @@ class SasCztRequest:
     arrival: float = 0.0
     result: torch.Tensor | None = None
     done: float | None = None
+    error: str | None = None
 
 
 @dataclass
@@ -88,6 +90,82 @@ class BornRequest:
     result: torch.Tensor | None = None
     done: float | None = None
     iterations: int = 0
+    error: str | None = None
+
+
+def _finite_pos(x) -> bool:
+    return bool(np.isfinite(x)) and x > 0
+
+
+def validate(r) -> None:
+    """Raise ValueError for a malformed request: shapes, dtypes, devices and
+    the finiteness of its parameters (host checks; a Born request's
+    permittivity and source are checked for non-finite values too, since
+    the iteration on them would never converge)."""
+    if isinstance(r, PropRequest):
+        if r.u.ndim != 2 or r.u.shape[0] != r.u.shape[1] or r.u.dtype != torch.complex64:
+            raise ValueError(f"field must be (N, N) complex64, got {tuple(r.u.shape)} {r.u.dtype}")
+        if tuple(r.phase.shape) != tuple(r.u.shape) or r.phase.dtype != torch.float32:
+            raise ValueError(f"phase mask must be {tuple(r.u.shape)} float32, got "
+                             f"{tuple(r.phase.shape)} {r.phase.dtype}")
+        if r.phase.device != r.u.device:
+            raise ValueError("field and phase mask on different devices")
+        z = np.asarray(r.z, dtype=np.float64)
+        if z.ndim != 1 or z.size == 0 or z.size > 4096 or not np.all(np.isfinite(z)):
+            raise ValueError("z must be a non-empty list of finite distances")
+        if not (_finite_pos(r.dx) and _finite_pos(r.wavelength) and _finite_pos(r.n)):
+            raise ValueError("dx, wavelength and n must be finite and positive")
+    elif isinstance(r, SasCztRequest):
+        wl = np.asarray(r.wavelengths, dtype=np.float64).reshape(-1)
+        if (r.u.ndim != 3 or r.u.shape[0] != r.u.shape[1] or r.u.shape[2] != wl.size
+                or r.u.dtype != torch.complex64):
+            raise ValueError(f"field must be (N, N, {wl.size}) complex64, got {tuple(r.u.shape)} {r.u.dtype}")
+        if not np.all(np.isfinite(wl)) or np.any(wl <= 0):
+            raise ValueError("wavelengths must be finite and positive")
+        if not np.isfinite(r.z) or r.z == 0:
+            raise ValueError("z must be finite and non-zero")
+        if not (_finite_pos(r.dx) and _finite_pos(r.n) and _finite_pos(r.zoom)) or int(r.m) < 2:
+            raise ValueError("dx, n and zoom must be finite and positive, m >= 2")
+    elif isinstance(r, BornRequest):
+        eps, J = r.permittivity, r.current
+        if eps.ndim != 3 or not eps.is_complex():
+            raise ValueError(f"permittivity must be (Nx, Ny, Nz) complex, got {tuple(eps.shape)} {eps.dtype}")
+        if tuple(J.shape) != tuple(eps.shape) + (3,):
+            raise ValueError(f"current must be {tuple(eps.shape) + (3,)}, got {tuple(J.shape)}")
+        if J.device != eps.device:
+            raise ValueError("permittivity and current on different devices")
+        if not (_finite_pos(r.spacing) and _finite_pos(r.k0) and _finite_pos(r.rtol)):
+            raise ValueError("spacing, k0 and rtol must be finite and positive")
+        if not (bool(torch.isfinite(torch.view_as_real(eps)).all())
+                and bool(torch.isfinite(torch.view_as_real(J.to(torch.complex64))).all())):
+            raise ValueError("permittivity and current must be finite")
+    else:
+        raise ValueError(f"unknown request type {type(r).__name__}")
+
+
+class Admission:
+    """Admission by an HBM budget.  A request is admitted while the bytes
+    of the admitted, unfinished requests plus its own stay within the
+    budget; one that does not fit waits, and nothing that arrived after it
+    overtakes it, until completions free memory; one larger than the whole
+    budget runs alone once nothing else is in flight.  A request never
+    fails for lack of budget: it waits."""
+
+    def __init__(self, budget: int):
+        self.budget = int(budget)
+        self.in_use = 0
+        self.n_active = 0
+
+    def fits(self, need: int) -> bool:
+        return self.n_active == 0 or self.in_use + need <= self.budget
+
+    def admit(self, need: int) -> None:
+        self.in_use += need
+        self.n_active += 1
+
+    def release(self, need: int) -> None:
+        self.in_use -= need
+        self.n_active -= 1
 
 
 _ONES = L.doubles([1.0] * 32)  # placeholder z list for size queries (the table carries the distances)
@@ -203,12 +281,8 @@ def _sas_czt(r: SasCztRequest, stream):
         s = transform_propagate_sas(f, r.z, r.n)
         N = r.u.shape[0]
         ones = torch.ones((N, N), dtype=torch.complex64, device=r.u.device)
-        outs = []
-        for w in range(len(r.wavelengths)):
-            fw = ScalarField(s.u[..., w].resolve_conj().contiguous(), s.dx[w], 0.0, float(r.wavelengths[w]))
-            z = czt_kernel_propagate(fw, ones, output_dx=fw.dx / r.zoom, output_shape=(r.m, r.m))
-            outs.append(z.u)
-        return torch.stack(outs, dim=-1)
+        # per wavelength a zoom onto its own output spacing dx_w / zoom
+        return czt_kernel_propagate(s, ones, output_dx=s.dx / r.zoom, output_shape=(r.m, r.m)).u
 
 
 def _born(r: BornRequest):
@@ -314,6 +388,10 @@ class ServingEngine:
         """Capture the graphs the request mix will need (outside the timed run)."""
         seen = set()
         for r in requests:
+            try:
+                validate(r)
+            except ValueError:
+                continue
             if isinstance(r, PropRequest):
                 bmax = self._batch_size(r.u.shape[-1], len(r.z), self.max_batch)
                 for B in (1, 2, 4, 8):
@@ -336,32 +414,53 @@ class ServingEngine:
         queue: dict[tuple, list[PropRequest]] = {}
         side: list = []
         born: list[_BornJob] = []
-        in_use = 0
+        adm = Admission(self.hbm_budget)
         active = []
         finished = []
         self.t0 = time.perf_counter()
 
+        def reject(r, e):
+            r.error, r.result, r.done = f"{type(e).__name__}: {e}", None, self._now()
+            finished.append(r)
+
         while pending or queue or active or side or born:
             now = self._now()
-            # admit arrivals within the HBM budget
+            # admit arrivals within the HBM budget; malformed ones are rejected
             while pending and pending[0].arrival <= now:
                 r = pending[0]
+                try:
+                    validate(r)
+                except ValueError as e:
+                    pending.pop(0)
+                    reject(r, e)
+                    continue
                 need = self._bytes(r)
-                if in_use + need > self.hbm_budget and (active or side or born):
+                if not adm.fits(need):
                     break
                 pending.pop(0)
-                in_use += need
                 if isinstance(r, PropRequest):
+                    adm.admit(need)
                     queue.setdefault((r.u.shape[-1], len(r.z), r.dx, r.wavelength, r.n), []).append(r)
                 elif isinstance(r, SasCztRequest):
                     self.side_stream.wait_stream(torch.cuda.current_stream(self.device))
-                    with torch.cuda.stream(self.side_stream):
-                        r.result = _sas_czt(r, self.side_stream)
-                        ev = torch.cuda.Event()
-                        ev.record(self.side_stream)
+                    try:
+                        with torch.cuda.stream(self.side_stream):
+                            r.result = _sas_czt(r, self.side_stream)
+                            ev = torch.cuda.Event()
+                            ev.record(self.side_stream)
+                    except (ValueError, NotImplementedError, L.CxmError) as e:
+                        reject(r, e)
+                        continue
+                    adm.admit(need)
                     side.append((r, ev))
                 else:
-                    born.append(_BornJob(r, self.device))
+                    try:
+                        job = _BornJob(r, self.device)
+                    except (ValueError, NotImplementedError, L.CxmError) as e:
+                        reject(r, e)
+                        continue
+                    adm.admit(need)
+                    born.append(job)
             # schedule batches
             for key in list(queue):
                 q = queue[key]
@@ -382,7 +481,7 @@ class ServingEngine:
                 done = obj.poll(self.keep_results, self._now)
                 if done:
                     for r in done:
-                        in_use -= self._bytes(r)
+                        adm.release(self._bytes(r))
                     finished += done
                 else:
                     still.append((kind, obj, t))
@@ -393,7 +492,7 @@ class ServingEngine:
                     job.r.done = self._now()
                     if not self.keep_results:
                         job.r.result = None
-                    in_use -= self._bytes(job.r)
+                    adm.release(self._bytes(job.r))
                     finished.append(job.r)
                 else:
                     keep_b.append(job)
@@ -404,7 +503,7 @@ class ServingEngine:
                     r.done = self._now()
                     if not self.keep_results:
                         r.result = None
-                    in_use -= self._bytes(r)
+                    adm.release(self._bytes(r))
                     finished.append(r)
                 else:
                     keep.append((r, ev))
@@ -419,7 +518,8 @@ class ServingEngine:
 
 def run_plain(requests, keep_results: bool = False):
     """Each request alone, in arrival order, one call per operation,
-    synchronised before the next starts."""
+    synchronised before the next starts; malformed requests are rejected
+    (.error set) as by the engine."""
     from .functional.propagation import asm_propagate
     from .functional.phase_masks import phase_change
 
@@ -429,6 +529,11 @@ def run_plain(requests, keep_results: bool = False):
         wait = r.arrival - (time.perf_counter() - t0)
         if wait > 0:
             time.sleep(wait)
+        try:
+            validate(r)
+        except ValueError as e:
+            r.error, r.result, r.done = f"{type(e).__name__}: {e}", None, time.perf_counter() - t0
+            continue
         if isinstance(r, PropRequest):
             f = ScalarField(r.u, r.dx, 0.0, r.wavelength)
             f = phase_change(f, r.phase)
@@ -447,14 +552,17 @@ def run_plain(requests, keep_results: bool = False):
 
 def latency_stats(reqs) -> dict:
     """p50 / p99 latency (arrival to completion), throughput over the span
-    from the first arrival to the last completion, and per request type."""
-    lat = np.array([r.done - r.arrival for r in reqs if r.done is not None])
-    span = max(r.done for r in reqs) - min(r.arrival for r in reqs)
-    out = {"n": int(lat.size), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+    from the first arrival to the last completion, and per request type;
+    rejected requests are counted apart and take no part in the rest."""
+    ok = [r for r in reqs if r.done is not None and r.error is None]
+    lat = np.array([r.done - r.arrival for r in ok])
+    span = max(r.done for r in ok) - min(r.arrival for r in ok)
+    out = {"n": int(lat.size), "rejected": sum(1 for r in reqs if r.error is not None),
+           "p50_ms": float(np.percentile(lat, 50) * 1e3),
            "p99_ms": float(np.percentile(lat, 99) * 1e3), "throughput_rps": float(lat.size / span),
            "span_s": float(span)}
     for cls in (PropRequest, SasCztRequest, BornRequest):
-        lt = np.array([r.done - r.arrival for r in reqs if isinstance(r, cls) and r.done is not None])
+        lt = np.array([r.done - r.arrival for r in ok if isinstance(r, cls)])
         if lt.size:
             out[cls.__name__] = {"n": int(lt.size), "p50_ms": float(np.percentile(lt, 50) * 1e3),
                                  "p99_ms": float(np.percentile(lt, 99) * 1e3)}

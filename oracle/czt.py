@@ -54,16 +54,13 @@ def zoomed_fft(x, k_start, k_end, output_shape, axes=(-2, -1), include_end=True)
     return x
 
 
-def czt_propagate(u, spacing, wavelength, z, n, output_dx, output_shape, shift_yx=(0.0, 0.0),
-                  kykx=(0.0, 0.0), pad=0, mode="full", remove_evanescent=False, bandlimit=False):
-    """asm_propagate(..., output_dx, output_shape) with use_czt (FP64).
-    Returns (u_out, dx_out, origin_out)."""
-    py, px = (pad, pad) if np.isscalar(pad) else pad
-    u = np.pad(np.asarray(u, np.complex128), [(0, 0)] * (np.ndim(u) - 2) + [(py, py), (px, px)])
+def czt_kernel_propagate(u, H, spacing, output_dx, output_shape, shift_yx=(0.0, 0.0)):
+    """The CZT branch of kernel_propagate (propagation.py:395-443) for a
+    propagator H in FFT order, by the defining sums (FP64).  Returns
+    (u_out, dx_out, origin_out)."""
+    u = np.asarray(u, np.complex128)
     ny, nx = u.shape[-2:]
-    H = asm_propagator((ny, nx), spacing, wavelength, z, n, kykx, bandlimit=bandlimit,
-                       remove_evanescent=remove_evanescent)
-    T = np.asarray(spacing, np.float64) * np.array([ny, nx])
+    T = np.broadcast_to(np.asarray(spacing, np.float64), (2,)) * np.array([ny, nx])
     df = 1.0 / T
     ax = (-2, -1)
     spec = np.fft.fftshift(H * np.fft.fft2(np.fft.ifftshift(u, axes=ax), axes=ax), axes=ax)
@@ -80,10 +77,22 @@ def czt_propagate(u, spacing, wavelength, z, n, output_dx, output_shape, shift_y
         N = (m - 1) // 2
         C = w ** (-N * np.arange(m)) * a**N
         spec = np.moveaxis(np.moveaxis(spec, ax[d], -1) * C, -1, ax[d])
-    out = spec * np.prod(1 / alpha)
+    return spec * np.prod(1 / alpha), odx, np.asarray(shift_yx, np.float64)
+
+
+def czt_propagate(u, spacing, wavelength, z, n, output_dx, output_shape, shift_yx=(0.0, 0.0),
+                  kykx=(0.0, 0.0), pad=0, mode="full", remove_evanescent=False, bandlimit=False):
+    """asm_propagate(..., output_dx, output_shape) with use_czt (FP64).
+    Returns (u_out, dx_out, origin_out)."""
+    py, px = (pad, pad) if np.isscalar(pad) else pad
+    u = np.pad(np.asarray(u, np.complex128), [(0, 0)] * (np.ndim(u) - 2) + [(py, py), (px, px)])
+    ny, nx = u.shape[-2:]
+    H = asm_propagator((ny, nx), spacing, wavelength, z, n, kykx, bandlimit=bandlimit,
+                       remove_evanescent=remove_evanescent)
+    out, odx, origin = czt_kernel_propagate(u, H, spacing, output_dx, output_shape, shift_yx)
     if mode == "same":
         out = out[..., py: out.shape[-2] - py, px: out.shape[-1] - px]
-    return out, odx, np.asarray(shift_yx, np.float64)
+    return out, odx, origin
 
 
 def _freq_1d(n, d):

@@ -81,6 +81,32 @@ def test_asm_propagate_czt_output_grid(n, m, odx, shift, path, monkeypatch):
     np.testing.assert_allclose(out.origin, org)
 
 
+@pytest.mark.parametrize("per_wv_dx", [False, True])
+def test_chromatic_czt_output_grid(per_wv_dx):
+    """A chromatic field on the CZT output grid: each wavelength on its own
+    input and output spacing equals the monochromatic oracle of that
+    wavelength."""
+    rng = np.random.default_rng(6)
+    wl = np.array([0.45, 0.532, 0.65])
+    dx = np.array([[0.5, 0.5], [0.45, 0.5], [0.4, 0.42]]) if per_wv_dx else 0.5
+    u = np.exp(2j * np.pi * rng.random((256, 256, 3))).astype(np.complex64)
+    f = cx.ChromaticScalarField(torch.as_tensor(u).cuda(), dx, 0.0, cx.Spectrum(wl))
+    odx = np.array([[0.2, 0.2], [0.25, 0.22], [0.3, 0.3]]) if per_wv_dx else 0.2
+    out = cf.asm_propagate(f, 80.0, 1.33, 0, output_dx=odx, output_shape=(64, 48),
+                           shift_yx=(2.0, -1.0), phase_mode="accurate")
+    assert out.u.shape == (64, 48, 3)
+    for w in range(3):
+        dxw = f.dx[w]
+        odxw = np.broadcast_to(np.asarray(odx, np.float64), (3, 2))[w]
+        ref, dxo, org = oc.czt_propagate(u[..., w], dxw, wl[w], 80.0, 1.33, odxw, (64, 48),
+                                         shift_yx=(2.0, -1.0))
+        err = rel_l2(out.u[..., w].cpu().numpy(), ref)
+        print(f"chromatic czt wavelength {wl[w]}: {err:.2e}")
+        assert err < TOL
+        np.testing.assert_allclose(out.dx[w], dxo)
+        np.testing.assert_allclose(out.origin[w], org)
+
+
 @pytest.mark.parametrize("shape,odx,shift", [
     ((256, 256), 0.45, (0.0, 0.0)),
     ((512, 512), (0.2, 0.35), (3.0, -1.5)),
