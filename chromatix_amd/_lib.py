@@ -365,6 +365,10 @@ def _declare(lib):
                                                   ctypes.POINTER(FluorDesc)]),
         "cxm_yu_axis_workspace_size": (sz, [ctypes.POINTER(YuDesc)]),
         "cxm_yu_axis_c64": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(YuDesc)]),
+        "cxm_czt_matrix_c64": (i32, [vp, ctypes.POINTER(CztDesc), ctypes.c_int32, vp, ctypes.c_int64,
+                                     ctypes.c_int64, ctypes.c_int32]),
+        "cxm_yu_matrix_c64": (i32, [vp, ctypes.POINTER(YuDesc), vp, ctypes.c_int64, ctypes.c_int64,
+                                    ctypes.c_int32]),
         "cxm_high_na_defocus_c64": (i32, [vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_double]),

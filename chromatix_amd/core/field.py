@@ -141,11 +141,19 @@ class Field:
 
     @property
     def spatial_limits(self) -> tuple[tuple[float, float], tuple[float, float]]:
-        g = self.grid
-        return (float(g[..., 0].min()), float(g[..., 0].max())), (
-            float(g[..., 1].min()),
-            float(g[..., 1].max()),
-        )
+        """(y_min, y_max), (x_min, x_max) of ``grid`` (field.py:221-229).  The
+        coordinates are monotone in the index, so the extremes are those of
+        the first and last sample of each axis (and wavelength), computed
+        with the same float64 operations as ``grid`` instead of the whole
+        grid."""
+        sp = np.asarray(self.dx, dtype=np.float64)
+        org = np.broadcast_to(np.asarray(self.origin, dtype=np.float64), sp.shape)
+        out = []
+        for d, n in enumerate(self.spatial_shape):
+            ends = np.array([0.0, float(n - 1)]) - n / 2
+            v = sp[..., d:d + 1] * ends + org[..., d:d + 1]
+            out.append((float(v.min()), float(v.max())))
+        return out[0], out[1]
 
     @property
     def wavelength(self) -> np.ndarray:

@@ -250,6 +250,29 @@ __global__ void yu_chirps_kernel(float2* __restrict__ out, YuParams c) {
   }
 }
 
+// The n x n matrix of one Yu axis pass, element (j, k) at out[j * ld_r +
+// k * ld_c] (conjugated on request), in FP64:
+//   out_dx / alpha exp(i pi (x_j^2 / alpha + alpha f_k^2 - alpha f_t^2)),
+//   t = j + s - k in [0, n) (0 otherwise), s = (n - 1) / 2
+__global__ void yu_matrix_kernel(float2* __restrict__ out, YuParams c, long long ld_r,
+                                 long long ld_c, double sg) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = c.n, h = c.n / 2, sw = (c.n - 1) / 2;
+  if (i >= (long long)n * n) return;
+  const int j = (int)(i / n), k = (int)(i - (long long)j * n);
+  const int t = j + sw - k;
+  float2 v = make_float2(0.f, 0.f);
+  if (t >= 0 && t < n) {
+    constexpr double kPi = 3.14159265358979323846;
+    const double x = c.x0 + (double)j * c.out_dx;
+    const double fk = (double)(k - h) * c.df, ft = (double)(t - h) * c.df;
+    const double ph = kPi * (x * x / c.alpha + c.alpha * (fk * fk - ft * ft));
+    const double m = c.out_dx / c.alpha;
+    v = make_float2((float)(m * cos(ph)), (float)(sg * m * sin(ph)));
+  }
+  out[(long long)j * ld_r + (long long)k * ld_c] = v;
+}
+
 int yu_geom(const cxm_yu_desc* d, int& n, int& L) {
   if (!d) return fail(CXM_ERR_INVALID, "null descriptor");
   if (d->batch < 0 || d->height <= 0 || d->width <= 0)
@@ -347,4 +370,23 @@ extern "C" int cxm_yu_axis_c64(void* stream, const void* in, void* out, void* wo
   a.out_line_stride = d->width;
   a.out_line_len = d->width;
   return launch_pipe_col(L, a, s);
+}
+
+extern "C" int cxm_yu_matrix_c64(void* stream, const cxm_yu_desc* d, void* out, int64_t ld_r,
+                                 int64_t ld_c, int32_t conj) {
+  int n, L;
+  int rc = yu_geom(d, n, L);
+  if (rc) return rc;
+  if (!out) return fail(CXM_ERR_INVALID, "null buffer");
+  if ((long long)n * n > (1ll << 31)) return fail(CXM_ERR_UNSUPPORTED, "matrix too large");
+  if ((rc = ensure_init())) return rc;
+  YuParams yp;
+  yp.alpha = d->alpha; yp.df = d->df; yp.out_dx = d->out_dx; yp.x0 = d->x0;
+  yp.n = n; yp.L = L;
+  const long long tot = (long long)n * n;
+  hipLaunchKernelGGL(yu_matrix_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (float2*)out, yp, (long long)ld_r, (long long)ld_c,
+                     conj ? -1.0 : 1.0);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
 }

@@ -369,3 +369,33 @@ extern "C" int cxm_cgemm_mfma_c64(void* stream, const void* A, const void* B, vo
   g.sc = stride_c;
   return launch_cgemm(g, batch, (hipStream_t)stream);
 }
+
+// Matrix of a cxm_czt_axis_c64 pass (parameter set `param`), element (k, j)
+// at out[k * ld_r + j * ld_c], conjugated on request: exp((j - N)(k Lw - La))
+// scale with the logs conjugated is its conjugate (j, k, N and scale real).
+extern "C" int cxm_czt_matrix_c64(void* stream, const cxm_czt_desc* d, int32_t param, void* out,
+                                  int64_t ld_r, int64_t ld_c, int32_t conj) {
+  if (!d || !out || !d->a || !d->w) return fail(CXM_ERR_INVALID, "null descriptor or buffer");
+  if (d->height <= 0 || d->width <= 0 || d->m <= 0 || (d->axis != 0 && d->axis != 1))
+    return fail(CXM_ERR_INVALID, "invalid shape parameters");
+  if (param < 0 || param >= d->num_param) return fail(CXM_ERR_INVALID, "param out of range");
+  if (d->mod_n >= d->m) return fail(CXM_ERR_INVALID, "mod_n must be smaller than m");
+  const int n = d->axis == 0 ? d->height : d->width;
+  const long long total = (long long)d->m * n;
+  if (total > (1ll << 31)) return fail(CXM_ERR_UNSUPPORTED, "CZT matrix too large");
+  const double* a = d->a + 2 * param;
+  const double* w = d->w + 2 * param;
+  if (!(std::hypot(a[0], a[1]) > 0.0) || !(std::hypot(w[0], w[1]) > 0.0))
+    return fail(CXM_ERR_INVALID, "a and w must be non-zero");
+  int rc;
+  if ((rc = ensure_init())) return rc;
+  double la[2], lw[2];
+  complex_log(a, la[0], la[1]);
+  complex_log(w, lw[0], lw[1]);
+  const double sg = conj ? -1.0 : 1.0;
+  hipLaunchKernelGGL(zoom_matrix_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (float2*)out, d->m, n, (long long)ld_r, (long long)ld_c,
+                     la[0], sg * la[1], lw[0], sg * lw[1], d->mod_n, d->scale);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}

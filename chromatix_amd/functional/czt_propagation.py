@@ -31,11 +31,38 @@ from .propagation import (
 __all__ = ["czt_kernel_propagate", "czt_kernel_propagate_fields"]
 
 
+def _shifted_mul(a, b):
+    """a * fftshift(b) per plane (plane p of a uses b[p % len(b)])."""
+    a = a.to(torch.complex64).resolve_conj().contiguous()
+    ny, nx = a.shape[-2:]
+    out = torch.empty_like(a)
+    L.check(L.lib().cxm_cmul_c64(L.stream_ptr(a.device), a.data_ptr(), b.data_ptr(), out.data_ptr(),
+                                 a.numel() // (ny * nx), ny, nx, b.shape[0], 1))
+    return out
+
+
+class _ShiftedMulFn(torch.autograd.Function):
+    """The propagator multiply of the CZT branch, spec * fftshift(H); its
+    reverse-mode rule in spec multiplies by conj(fftshift(H)) (H is a
+    constant of the rule: a propagator that requires grad is refused)."""
+
+    @staticmethod
+    def forward(ctx, spec, H):
+        ctx.save_for_backward(H)
+        return _shifted_mul(spec, H)
+
+    @staticmethod
+    def backward(ctx, g):
+        (H,) = ctx.saved_tensors
+        return _shifted_mul(g, torch.conj_physical(H)), None
+
+
 def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=None,
                          shift_yx=(0.0, 0.0), use_czt: bool = True) -> Field:
     """kernel_propagate with an output grid of spacing ``output_dx`` and shape
     ``output_shape`` centred on ``shift_yx`` (propagation.py:395-443)."""
-    L.refuse_grad("kernel_propagate with an output grid", field.u, propagator)
+    L.refuse_grad("kernel_propagate with an output grid (gradient with respect to the propagator)",
+                  propagator)
     if isinstance(field, ChromaticScalarField):
         return _czt_chromatic(field, propagator, output_dx, output_shape, shift_yx, use_czt)
     ny, nx = field.spatial_shape
@@ -59,10 +86,7 @@ def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=
     H = H.reshape((-1, ny, nx)).resolve_conj().contiguous()
     u = field.u.resolve_conj().contiguous()
     L.ensure_device(u)
-    spec = _fft_shifted(u, axes=(-2, -1), shift=True)
-    n_planes = spec.numel() // (ny * nx)
-    L.check(L.lib().cxm_cmul_c64(L.stream_ptr(u.device), spec.data_ptr(), H.data_ptr(),
-                                 spec.data_ptr(), n_planes, ny, nx, H.shape[0], 1))
+    spec = _ShiftedMulFn.apply(_fft_shifted(u, axes=(-2, -1), shift=True), H)
     out_field = Field.empty_like(shift_grid(field, shift_yx), dx=out_dx, shape=output_shape)
     alpha = out_field.dx / df
     if not use_czt:
@@ -136,7 +160,6 @@ def czt_kernel_propagate_fields(method: str, field: Field, z, n, pad_width, cval
     pad, propagator on the padded grid (no output shift when the CZT places
     the window), CZT branch of kernel_propagate, crop in "same" mode
     (propagation.py:229-252, 325-352)."""
-    L.refuse_grad(f"{method}_propagate with an output grid", field.u)
     py, px = _pair(pad_width)
     padded = pad(field, (py, px), cval=cval)
     z = float(np.asarray(z).reshape(()))

@@ -312,6 +312,37 @@ def ifft(x, axes=(1, 2), shift: bool = False) -> torch.Tensor:
     return _raw(x, axes, shift, inverse=True)
 
 
+def _defocus(u, pars):
+    u = u.to(torch.complex64).resolve_conj().contiguous()
+    ny, nx = u.shape[-2:]
+    out = torch.empty_like(u)
+    L.check(L.lib().cxm_high_na_defocus_c64(L.stream_ptr(u.device), u.data_ptr(), out.data_ptr(),
+                                            u.numel() // (ny * nx), ny, nx, *pars))
+    return out
+
+
+class _DefocusFn(torch.autograd.Function):
+    """The defocus factor of high_na_ff_lens; its reverse-mode rule
+    multiplies by the conjugate factor (the factor regenerated from a plane
+    of ones)."""
+
+    @staticmethod
+    def forward(ctx, u, pars):
+        ctx.pars = pars
+        return _defocus(u, pars)
+
+    @staticmethod
+    def backward(ctx, g):
+        g = g.to(torch.complex64).resolve_conj().contiguous()
+        ny, nx = g.shape[-2:]
+        fac = torch.conj_physical(_defocus(torch.ones((1, ny, nx), dtype=torch.complex64,
+                                                      device=g.device), ctx.pars))
+        out = torch.empty_like(g)
+        L.check(L.lib().cxm_cmul_c64(L.stream_ptr(g.device), g.data_ptr(), fac.data_ptr(),
+                                     out.data_ptr(), g.numel() // (ny * nx), ny, nx, 1, 0))
+        return out, None
+
+
 def high_na_ff_lens(field: Field, f, n, NA, output_shape=None, output_dx=None) -> Field:
     """High-NA lens a distance ``f`` after a scalar, single-wavelength, square
     field (lenses.py:98-156): the field times the defocus factor
@@ -324,7 +355,6 @@ def high_na_ff_lens(field: Field, f, n, NA, output_shape=None, output_dx=None) -
     cartesian_to_spherical branch) are not part of this library."""
     from ..utils.czt import zoomed_fft
 
-    L.refuse_grad("high_na_ff_lens", field.u)
     if isinstance(field, ChromaticScalarField):
         raise NotImplementedError("high_na_ff_lens: single-wavelength scalar fields only "
                                   "(the reference assumes one wavelength)")
@@ -337,13 +367,8 @@ def high_na_ff_lens(field: Field, f, n, NA, output_shape=None, output_dx=None) -
     output_shape = (int(output_shape[0]), int(output_shape[1]))
     wl = float(field.spectrum.wavelength[0])
     zoom = 2.0 * float(NA) * output_shape[0] * output_dx / ((nx - 1) * wl)
-    u = field.u
-    L.ensure_device(u)
-    u = u.to(torch.complex64).resolve_conj().contiguous()
-    n_planes = u.numel() // (ny * nx)
-    tmp = torch.empty_like(u)
-    L.check(L.lib().cxm_high_na_defocus_c64(L.stream_ptr(u.device), u.data_ptr(), tmp.data_ptr(),
-                                            n_planes, ny, nx, float(field.dx[0]), float(field.dx[1]),
-                                            wl, float(n), float(NA), float(f)))
+    L.ensure_device(field.u)
+    pars = (float(field.dx[0]), float(field.dx[1]), wl, float(n), float(NA), float(f))
+    tmp = _DefocusFn.apply(field.u, pars)
     out = zoomed_fft(tmp, -zoom * np.pi, zoom * np.pi, output_shape, include_end=True)
     return field.replace(u=out, dx=np.full(2, output_dx))

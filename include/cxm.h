@@ -4,7 +4,13 @@
  *
  * Conventions for every entry point:
  *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
- *     calls only enqueue work on it and never synchronise the host.
+ *     calls only enqueue work on it and never synchronise the host, with
+ *     one exception: cxm_enclosing_circle_c64 returns its result in host
+ *     memory and so waits for `stream` (its host part runs on the pruned
+ *     candidates).
+ *   - library state is per process: cxm_set_option values and the timing
+ *     switch (cxm_timing_*) apply to every later call on every thread and
+ *     device; set them before concurrent work starts.
  *   - buffers are device pointers owned by the caller; complex data is
  *     interleaved complex64 (float re, float im), C order.
  *   - scratch comes from the caller: query *_workspace_size(), pass a device
@@ -406,6 +412,28 @@ typedef struct cxm_yu_desc {
 size_t cxm_yu_axis_workspace_size(const cxm_yu_desc* d);
 int cxm_yu_axis_c64(void* stream, const void* in, void* out, void* workspace,
                     size_t workspace_bytes, const cxm_yu_desc* d);
+
+/*
+ * Dense matrices of the one-axis passes, for their reverse-mode rules: a
+ * pass along an axis is a product with its matrix, so its adjoint (the
+ * vector-Jacobian product torch's autograd asks for) is the product with
+ * the conjugate transpose, one cxm_cgemm_mfma_c64 call.  The reference
+ * gets these rules from JAX's automatic differentiation of czt and of the
+ * scaled convolution (src/chromatix/utils/czt.py:7-52,
+ * src/chromatix/functional/propagation.py:427-458); here they are explicit.
+ * Element (r, c) goes to out[r * ld_r + c * ld_c], conjugated when conj != 0
+ * (ld_r = 1, ld_c = rows stores the conjugate transpose row-major).
+ *   cxm_czt_matrix_c64: m x n, A[k][j] = post_k scale a^-j w^(j k), the
+ *     values of cxm_czt_axis_c64 with parameter set `param` (FP64, rounded
+ *     to complex64).
+ *   cxm_yu_matrix_c64: n x n, A[j][k] = out_dx exp(i pi x_j^2 / alpha)
+ *     (1/alpha) exp(i pi alpha f_k^2) kern[j + s - k], the values of
+ *     cxm_yu_axis_c64 (above).
+ */
+int cxm_czt_matrix_c64(void* stream, const cxm_czt_desc* d, int32_t param, void* out,
+                       int64_t ld_r, int64_t ld_c, int32_t conj);
+int cxm_yu_matrix_c64(void* stream, const cxm_yu_desc* d, void* out, int64_t ld_r,
+                      int64_t ld_c, int32_t conj);
 
 /*
  * The scalar part of high_na_ff_lens (src/chromatix/functional/lenses.py:98-156):

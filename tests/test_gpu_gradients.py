@@ -245,3 +245,121 @@ def test_microscope_phase_mask_gradient_finite_differences():
     ad = float((g * v).sum().item())
     print(f"microscope d loss: autograd {ad:.6e}  central difference {fd:.6e}")
     assert abs(ad - fd) <= 2e-3 * abs(fd)
+
+
+# ---- chirp-z, zoom DFT, scaled convolution, output-grid propagation ----------
+def _czt_matrix_oracle(n, m, a, w):
+    """A^T (n x m) of the CZT by the defining sum (oracle)."""
+    from oracle import czt as oc
+
+    return oc.czt_direct(np.eye(n, dtype=np.complex128), m, a, w, axis=-1)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("n,m,num_param,mod_n", [(100, 37, 1, -1), (256, 64, 2, 31), (96, 160, 1, 79)])
+def test_czt_axis_gradient_matches_oracle(axis, n, m, num_param, mod_n):
+    """The CZT pass's reverse-mode rule against A^H c from the FP64 defining
+    sum, per parameter set, with and without the output modulation."""
+    from chromatix_amd.utils import czt as uc
+
+    rng = np.random.default_rng(n + m + axis)
+    a = np.exp(-1j * rng.uniform(0, 2, num_param))
+    w = np.exp(1j * 2 * np.pi / (1.3 * m) * rng.uniform(0.8, 1.2, num_param))
+    B, other = 4, 24
+    xs = (B, n, other) if axis == 0 else (B, other, n)
+    ys = (B, m, other) if axis == 0 else (B, other, m)
+    x, c = _rand_c(rng, xs), _rand_c(rng, ys)
+    scale = 0.37
+    (g,) = _grads_gpu(lambda t: uc.czt_axis(t, m, a, w, axis, num_param=num_param, mod_n=mod_n,
+                                            scale=scale), [x], c)
+    ref = np.empty(xs, np.complex128)
+    for p in range(B):
+        q = p % num_param
+        M = _czt_matrix_oracle(n, m, a[q], w[q])  # (n, m): y = x @ M along the axis
+        if mod_n >= 0:
+            M = M * (w[q] ** (-mod_n * np.arange(m)) * a[q] ** mod_n)[None, :]
+        M = M * scale
+        cp = c[p].astype(np.complex128)
+        ref[p] = (np.conj(M) @ cp) if axis == 0 else (cp @ np.conj(M).T)
+    err = rel_l2(g, ref)
+    print(f"czt axis {axis} gradient n={n} m={m} P={num_param} mod={mod_n}: {err:.2e}")
+    assert err < TOL
+
+
+@pytest.mark.parametrize("shape,m,mod_n,scale", [((64, 48), (20, 16), (-1, -1), (1.0, 1.0)),
+                                                 ((256, 256), (64, 64), (31, 31), (0.5, 2.0))])
+def test_zoom_dft_gradient_matches_oracle(shape, m, mod_n, scale):
+    from chromatix_amd.utils import czt as uc
+
+    rng = np.random.default_rng(7)
+    a = (np.exp(-0.4j), np.exp(0.9j))
+    w = (np.exp(2j * np.pi / (1.7 * m[0])), np.exp(2j * np.pi / (2.3 * m[1])))
+    x, c = _rand_c(rng, (3,) + shape), _rand_c(rng, (3,) + m)
+    (g,) = _grads_gpu(lambda t: uc.zoom_dft(t, m, a, w, mod_n=mod_n, scale=scale), [x], c)
+    mats = []
+    for d in range(2):
+        M = _czt_matrix_oracle(shape[d], m[d], a[d], w[d])
+        if mod_n[d] >= 0:
+            M = M * (w[d] ** (-mod_n[d] * np.arange(m[d])) * a[d] ** mod_n[d])[None, :]
+        mats.append(M * scale[d])  # (n_d, m_d) = A_d^T
+    ref = np.conj(mats[0]) @ c.astype(np.complex128) @ np.conj(mats[1]).T
+    err = rel_l2(g, ref)
+    print(f"zoom dft gradient {shape} -> {m}: {err:.2e}")
+    assert err < TOL
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+@pytest.mark.parametrize("n", [64, 100])
+def test_scaled_conv_axis_gradient_matches_oracle(axis, n):
+    """The Yu pass's rule against M^H c, M from the oracle's direct double
+    sum applied to the identity."""
+    from chromatix_amd.utils import czt as uc
+    from oracle import czt as oc
+
+    rng = np.random.default_rng(n + axis)
+    alpha, df, out_dx, x0 = 1.5, 1.0 / (n * 0.3), 0.45, -(n / 2) * 0.45 + 0.7
+    xs = (2, n, 20) if axis == 0 else (2, 20, n)
+    x, c = _rand_c(rng, xs), _rand_c(rng, xs)
+    (g,) = _grads_gpu(lambda t: uc.scaled_conv_axis(t, axis, alpha, df, out_dx, x0), [x], c)
+    f_in = (np.arange(n) - n // 2) * df
+    x_out = x0 + np.arange(n) * out_dx
+    # M[j, k]: the pass applied to unit vectors along the axis
+    M = oc.yu_axis_direct(np.eye(n, dtype=np.complex128), alpha, f_in, x_out, out_dx, 0)
+    cc = c.astype(np.complex128)
+    ref = (np.einsum("jk,pjw->pkw", np.conj(M), cc) if axis == 0
+           else np.einsum("jk,pwj->pwk", np.conj(M), cc))
+    err = rel_l2(g, ref)
+    print(f"yu axis {axis} gradient n={n}: {err:.2e}")
+    assert err < TOL
+
+
+@pytest.mark.parametrize("kind", ["czt_bluestein", "czt_mfma", "yu", "high_na", "chromatic_czt"])
+def test_output_grid_propagation_adjoint_identity(kind, monkeypatch):
+    """<F u, y> = <u, F^H y> for the output-grid propagations, F^H from
+    autograd: the CZT branch of asm_propagate (both CZT paths), the scaled
+    convolution, high_na_ff_lens, and a chromatic field on the CZT grid."""
+    if kind == "czt_mfma":
+        monkeypatch.setenv("CXM_ZOOM_PATH", "mfma")
+    elif kind == "czt_bluestein":
+        monkeypatch.setenv("CXM_ZOOM_PATH", "bluestein")
+    N = 128
+    if kind == "chromatic_czt":
+        mk = lambda u: cx.ChromaticScalarField(u, 0.5, 0.0, cx.Spectrum(WL3))
+        fwd = lambda f: cf.asm_propagate(f, 60.0, 1.33, 0, output_dx=0.3, output_shape=(40, 48),
+                                         shift_yx=(1.0, -2.0)).u
+        err = _adjoint_identity(mk, fwd, (N, N, 3), (40, 48, 3), 11)
+    else:
+        mk = lambda u: cx.ScalarField(u, 0.5, 0.0, 0.532)
+        if kind == "yu":
+            fwd = lambda f: cf.asm_propagate(f, 60.0, 1.33, 0, output_dx=0.7, use_czt=False).u
+            out_shape = (2, N, N)
+        elif kind == "high_na":
+            fwd = lambda f: cf.high_na_ff_lens(f, 100.0, 1.33, 0.8, output_shape=(48, 48), output_dx=0.25).u
+            out_shape = (2, 48, 48)
+        else:
+            fwd = lambda f: cf.asm_propagate(f, 60.0, 1.33, 16, output_dx=0.3, output_shape=(40, 48),
+                                             shift_yx=(1.0, -2.0)).u
+            out_shape = (2, 40, 48)
+        err = _adjoint_identity(mk, fwd, (2, N, N), out_shape, 12)
+    print(f"output-grid adjoint identity ({kind}): {err:.2e}")
+    assert err < TOL
