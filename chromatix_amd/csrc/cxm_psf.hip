@@ -209,8 +209,12 @@ using namespace cxm;
 
 // Plane groups of the phase-gradient pass (run in parallel, complex partial
 // sums reduced in a fixed order afterwards, where the plane-invariant
-// operators are applied once per element).
+// operators are applied once per element): at most kGradGroups, by default
+// kGradGroupsDefault.  11 planes of 1280^2, per gradient: 1 group 0.760 ms,
+// 2: 0.710, 3: 0.686, 4: 0.689, 11: 0.751 (fewer groups: too few waves for
+// a latency-bound pass; more: partial planes written and summed).
 constexpr int kGradGroups = 12;
+constexpr int kGradGroupsDefault = 4;
 
 namespace cxm {
 namespace {
@@ -349,7 +353,8 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
     ga.in = (const float2*)workspace;
     ga.n_planes = kc;
     ga.plane0 = (int)p0;
-    const int groups = std::min<int>(kGradGroups, kc);
+    const int og = option(CXM_OPT_GRAD_GROUPS);
+    const int groups = std::min<int>(og > 0 ? std::min(og, kGradGroups) : kGradGroupsDefault, kc);
     if ((rc = launch_pipe_phasegrad(d->width, ga, s, groups))) return rc;
     hipLaunchKernelGGL(sum_groups_kernel, dim3((unsigned)((plane_px + 255) / 256)), dim3(256), 0, s,
                        (const float2*)partial, grad_phase, d->height, d->width, groups, p0 > 0 ? 1 : 0,

@@ -824,6 +824,11 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         Green's projection and the inverse x-axis FFT run
  *                         as one pass where the x length has one; 2 = as
  *                         three launches.  Equal to float32 rounding.
+ *   CXM_OPT_GRAD_GROUPS   cxm_psf4f_bwd: 0 = the default number of plane
+ *                         groups of the phase-gradient pass (4), 1..12 =
+ *                         that many (each group sums its planes in
+ *                         registers and writes one complex partial plane).
+ *                         Equal to float32 rounding.
  */
 #define CXM_OPT_COL_VARIANT 1
 #define CXM_OPT_CHUNK_PLANES 2
@@ -839,6 +844,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_COL_PERSIST 12
 #define CXM_OPT_COL_PERSIST_E 13
 #define CXM_OPT_COL_PERSIST_MODE 14
+#define CXM_OPT_GRAD_GROUPS 15
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 
