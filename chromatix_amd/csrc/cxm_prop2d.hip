@@ -174,6 +174,7 @@ struct Geom {
 };
 
 bool use_h_table(const cxm_prop2d_desc* d);
+bool use_persist(const cxm_prop2d_desc* d);
 size_t table_elems(const Geom& g);
 int generic_geom(const cxm_prop2d_desc* d, Geom& g, size_t limit_bytes);
 
@@ -219,9 +220,15 @@ int validate(const cxm_prop2d_desc* d, Geom& g, size_t limit_bytes = 0) {
   g.plane_ws = (long long)g.Rws * g.Nx;
   // Chunk so that the workspace of one chunk stays <= 128 MiB, half the 256 MiB
   // Infinity Cache: the intermediate planes are re-read from it, not from HBM.
+  // With the persistent 1024-point column pass, 256 MiB (32 planes): its
+  // workgroups then hold each strip for twice the planes, so H per strip is
+  // evaluated half as often per plane, and the passes have half the tails
+  // (64 planes: 0.584 ms per step against 0.610 at 16 planes per chunk and
+  // 0.618 / 0.623 at 48 / 64).
   const int nout = d->z_zip ? 1 : d->num_z;
   const long long per_plane = (nout == 1 ? 1 : 1 + nout) * g.plane_ws;
-  const long long budget = (128ll << 20) / 8;
+  const bool big_chunk = use_persist(d) && d->height + 2 * d->pad_y == 1024;
+  const long long budget = ((big_chunk ? 256ll : 128ll) << 20) / 8;
   long long ch = std::max(1ll, budget / std::max(1ll, per_plane));
   if (option(CXM_OPT_CHUNK_PLANES) > 0) ch = option(CXM_OPT_CHUNK_PLANES);
   const long long plane_rm = g.plane_ws;  // row-major workspace plane

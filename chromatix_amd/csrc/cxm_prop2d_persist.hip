@@ -40,13 +40,27 @@ col_prop_persist_kernel(ColArgs a, int n_items) {
     for (int i = threadIdx.x; i < N; i += THREADS) twl[i] = a.tw[i];
     __syncthreads();
   }
+  // Strips whose rows all lie in the valid input (output) rows and whose
+  // columns all exist -- every strip of an unpadded, uncropped call -- load
+  // (store) without per-value guards, at one byte offset per thread plus a
+  // scalar step per value.
+  const bool rows_in = a.in_off == 0 && a.in_len >= N;
+  const bool rows_out = a.out_off == 0 && a.out_len >= N;
+  const unsigned rstep = (unsigned)(T * a.row_stride) * 8u;
   // column values of item `it` in the transform layout (zeros outside the
   // valid rows and columns)
   auto load = [&](float2 (&v)[E], int it, int ty) {
     const int strip = it / a.n_planes;
     const int p = it - strip * a.n_planes;
-    const bool ok = strip * C + c < a.ncols;
     const float2* src = a.in + p * a.in_plane_stride + (long long)strip * sstride;
+    if (rows_in && strip * C + C <= a.ncols) {
+      const unsigned off = (unsigned)(ty * a.row_stride + c) * 8u;
+#pragma unroll
+      for (int k = 0; k < E; ++k)
+        v[k] = *(const float2*)((const char*)src + (off + (unsigned)k * rstep));
+      return;
+    }
+    const bool ok = strip * C + c < a.ncols;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const int ys = ty + k * T - a.in_off;
@@ -100,8 +114,12 @@ col_prop_persist_kernel(ColArgs a, int n_items) {
 #pragma unroll
     for (int k = 0; k < E; ++k) u[k] = cmul(u[k], h[k]);
     fft<N, E, 1, C, true>(u, lds + c, ty, tw);
-    if (colok) {
-      float2* dst = a.out + p * a.out_plane_stride + (long long)strip * sstride;
+    float2* dst = a.out + p * a.out_plane_stride + (long long)strip * sstride;
+    if (rows_out && strip * C + C <= a.ncols) {
+      const unsigned off = (unsigned)(ty * a.row_stride + c) * 8u;
+#pragma unroll
+      for (int k = 0; k < E; ++k) *(float2*)((char*)dst + (off + (unsigned)k * rstep)) = u[k];
+    } else if (colok) {
 #pragma unroll
       for (int k = 0; k < E; ++k) {
         const int yo = ty + k * T - a.out_off;
