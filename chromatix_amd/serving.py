@@ -18,7 +18,15 @@ SAS + CZT requests (a chromatic field -> scalable angular spectrum
 propagation -> per wavelength a CZT zoom onto a smaller output window) run
 eagerly on their own stream; Born requests (modified Born series solves)
 advance in chunks of iterations on their own streams, their residuals read
-back through pinned memory and events, so the loop never blocks on them.
+back through pinned memory and events, so the loop never waits for an
+iteration.  It does wait once per Born request, at admission: the shift of
+get_shift_and_scale (cxm_enclosing_circle_c64) returns to the host, which
+waits for the permittivity's stream.
+
+Admission keeps the bytes of admitted, unfinished requests within an HBM
+budget (a request that does not fit waits, in arrival order; one larger
+than the budget runs alone), and malformed requests are rejected with a
+message on the request (`validate`) while the others complete.
 
 The plain form for comparison runs every request alone, in arrival order,
 one library call per operation, waiting for each to finish.
