@@ -138,6 +138,11 @@ def czt(x, m: int, a, w, axis: int = -1) -> torch.Tensor:
     k < m (czt.py:7-52, Bluestein)."""
     x = torch.as_tensor(x)
     ax = axis % x.ndim
+    if x.ndim >= 2 and ax == x.ndim - 2:
+        # the column transform of the planes of x: no transpose
+        H, W = x.shape[-2:]
+        y = czt_axis(x.reshape((-1, H, W)), m, a, w, 0)
+        return y.reshape(tuple(x.shape[:-2]) + (int(m), W))
     xm = x.movedim(ax, -1)
     lead = xm.shape[:-1]
     y = czt_axis(xm.reshape(1, -1, xm.shape[-1]), m, a, w, 1)
