@@ -66,11 +66,12 @@ def test_converged_solution_solves_helmholtz():
     assert pde < 1e-4
 
 
-def test_c4_volume_converges():
-    """128^3 with 20 spheres and a 16-voxel absorbing boundary (160^3), to a
-    relative residual of 1e-4; the host FP64 evaluation of the PDE residual
-    confirms the solution."""
-    eps, spacing, J = _setup(128, 16, 20, 4)
+@pytest.mark.parametrize("seed", [4, 5])
+def test_c4_volume_converges(seed):
+    """Two of the c4 volumes (128^3 with 20 spheres and a 16-voxel absorbing
+    boundary, 160^3), each to a relative residual of 1e-4; the host FP64
+    evaluation of the PDE residual confirms the solution (4e-5 measured)."""
+    eps, spacing, J = _setup(128, 16, 20, seed)
     S = om.source_field(J, K0)
     sample = mbs.Sample(torch.as_tensor(eps.astype(np.complex64)).cuda(), spacing)
     src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
@@ -80,7 +81,7 @@ def test_c4_volume_converges():
     pde = m.pde_residual(res.E.cpu().numpy().astype(np.complex128), S)
     print(f"mbs 160^3: {res.iterations} iterations, residual {res.residual:.2e}, PDE {pde:.2e}")
     assert res.iterations < 3000
-    assert pde < 1e-2
+    assert pde < 2e-4
 
 
 @pytest.mark.parametrize("iters", [1, 3, 6])
