@@ -47,6 +47,7 @@ CXM_OPT_COL_PERSIST = 12
 CXM_OPT_COL_PERSIST_E = 13
 CXM_OPT_COL_PERSIST_MODE = 14
 CXM_OPT_GRAD_GROUPS = 15
+CXM_OPT_MBS_ZFUSE = 16
 CXM_H_TABLE_MIN_PLANES = 4
 
 PHASE_MODES = {"ref32": CXM_PHASE_REF32, "accurate": CXM_PHASE_ACCURATE}
@@ -350,6 +351,8 @@ def _declare(lib):
         "cxm_mbs_workspace_size": (sz, [ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_precondition": (i32, [vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_iterate": (i32, [vp, vp, vp, vp, ctypes.c_int32, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
+        "cxm_mbs_iterate_stop": (i32, [vp, vp, vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                       ctypes.c_double, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_output": (i32, [vp, vp, vp, vp, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_bicgstab_state_size": (sz, [ctypes.POINTER(MbsDesc)]),
         "cxm_enclosing_circle_workspace_size": (sz, [ctypes.c_int64]),

@@ -206,13 +206,14 @@ extern "C" int cxm_timing_get(int i, char* name, int name_len, double* total_ms,
 // Tuning options (launch-shape variants of the kernels; see cxm.h).
 // ---------------------------------------------------------------------------
 namespace cxm {
-namespace { int g_opts[16] = {0}; }
-int option(int key) { return (key >= 0 && key < 16) ? g_opts[key] : 0; }
+constexpr int kNumOpts = 32;
+namespace { int g_opts[kNumOpts] = {0}; }
+int option(int key) { return (key >= 0 && key < kNumOpts) ? g_opts[key] : 0; }
 
 }  // namespace cxm
 
 extern "C" int cxm_set_option(int key, int value) {
-  if (key < 0 || key >= 16) return cxm::fail(CXM_ERR_INVALID, "unknown option");
+  if (key < 0 || key >= cxm::kNumOpts) return cxm::fail(CXM_ERR_INVALID, "unknown option");
   cxm::g_opts[key] = value;
   return CXM_OK;
 }

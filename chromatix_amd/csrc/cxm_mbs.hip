@@ -45,6 +45,10 @@ int mbs_geom(const cxm_mbs_desc* d, MbsGeom& g) {
 
 struct C2 { float re, im; };
 
+// A solver's device-side stop flag (null: never stopped); every kernel of an
+// iteration returns at once when it is set.
+CXM_DEV bool stopped(const double* stop) { return stop && *stop != 0.0; }
+
 CXM_DEV float2 cmulf(float2 a, C2 b) { return make_float2(a.x * b.re - a.y * b.im, a.x * b.im + a.y * b.re); }
 
 // V applied to the field vector v at voxel i: o = (s eps - kb) v, eps a
@@ -77,7 +81,8 @@ CXM_DEV void apply_v(const float2* __restrict__ eps, int tensor, long long i, lo
 // the plain scaling of the source)
 __global__ void mbs_v_kernel(const float2* __restrict__ src, const float2* __restrict__ eps,
                              float2* __restrict__ out, long long vox, int interleaved, int tensor, C2 s,
-                             C2 kb, C2 pre) {
+                             C2 kb, C2 pre, const double* __restrict__ stop) {
+  if (stopped(stop)) return;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vox) return;
   float2 v[3], o[3];
@@ -96,7 +101,8 @@ __global__ void mbs_v_kernel(const float2* __restrict__ src, const float2* __res
 // In-place Green multiply of the spectrum Y (3 planar components):
 // Y <- P_T Y / (kb - s |K|^2) + P_L Y / kb, P_L = K K^T / |K|^2 (0 at K = 0)
 __global__ void mbs_green_kernel(float2* __restrict__ y, long long vox, int nx, int ny, int nz,
-                                 float kscale, C2 s, C2 kb) {
+                                 float kscale, C2 s, C2 kb, const double* __restrict__ stop) {
+  if (stopped(stop)) return;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= vox) return;
   const int iz = (int)(i % nz);
@@ -129,6 +135,7 @@ __global__ void mbs_green_kernel(float2* __restrict__ y, long long vox, int nx, 
 }
 
 constexpr int kRedThreads = 256;
+constexpr int kRedBlocks = 1024;
 
 // x pass of the Green's operator, fused: for a strip of C columns of the
 // (nx x ny nz) matrix, the three components of each column are transformed
@@ -141,7 +148,8 @@ constexpr int kRedThreads = 256;
 template <int N, int E, int C>
 __global__ void __launch_bounds__((N / E) * 3 * C)
 mbs_xgreen_kernel(float2* __restrict__ v, const float2* __restrict__ tw, long long vox, int ncols,
-                  int ny, int nz, float kscale, C2 s, C2 kb, float scale) {
+                  int ny, int nz, float kscale, C2 s, C2 kb, float scale, const double* __restrict__ stop) {
+  if (stopped(stop)) return;
   constexpr int T = N / E;
   constexpr int CC = 3 * C;
   __shared__ float2 lds[lds_span<N, CC>()];
@@ -220,14 +228,14 @@ bool xgreen_supported(int n) {
 }
 
 int launch_xgreen(int n, float2* v, long long vox, int ncols, int ny, int nz, float kscale, C2 s, C2 kb,
-                  float scale, hipStream_t st) {
+                  float scale, const double* stop, hipStream_t st) {
   switch (n) {
 #define CXM_XG_LAUNCH(N, E, C)                                                                        \
   case N: {                                                                                           \
     const long long blocks = ((long long)ncols + (C)-1) / (C);                                        \
     if (blocks > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "fused Green pass grid too large");     \
     hipLaunchKernelGGL((mbs_xgreen_kernel<N, E, C>), dim3((unsigned)blocks), dim3((N / E) * 3 * (C)), 0, \
-                       st, v, twiddles_for(N), vox, ncols, ny, nz, kscale, s, kb, scale);             \
+                       st, v, twiddles_for(N), vox, ncols, ny, nz, kscale, s, kb, scale, stop);       \
     CXM_LAUNCH_CHECK();                                                                               \
     return CXM_OK;                                                                                    \
   }
@@ -238,11 +246,231 @@ int launch_xgreen(int n, float2* v, long long vox, int ncols, int ny, int nz, fl
   }
 }
 
+// ---- z passes fused with V and with the update -------------------------------
+// z is the innermost axis, so the z rows (ix, iy) of a component are
+// consecutive in memory and R rows form one contiguous run of R*N values.  A
+// workgroup takes R rows of all three components: it reads the runs linearly
+// (coalesced, every voxel's three components and eps in one thread), places
+// the values in LDS as 3R interleaved transforms (transform c*R + r, stride
+// S = 3R rounded up to an odd count so that consecutive z land on different
+// banks), transforms each (row, component) along z with T = N/E threads, and
+// goes back through LDS for a linear pass over the voxels.
+constexpr int pow2_ceil(int n) {
+  int p = 1;
+  while (p < n) p *= 2;
+  return p;
+}
+
+template <int N, int E>
+struct ZCfg {
+  static constexpr int T = N / E;
+  static constexpr int R = (T >= 64) ? 1 : 64 / T;
+  static constexpr int S = (3 * R) | 1;
+  static constexpr int THREADS = 3 * R * T;
+};
+
+// LDS slot of value z of row r, component c
+template <int N, int E>
+CXM_DEV int zslot(int z, int r, int c) {
+  using Cfg = ZCfg<N, E>;
+  return lds_at<N, Cfg::S>(z) + c * Cfg::R + r;
+}
+
+// transform of (row, component) `tr` along z, values from and back to LDS
+template <int N, int E, int DIR>
+CXM_DEV void ztransform(float2* lds, const float2* __restrict__ tw) {
+  using Cfg = ZCfg<N, E>;
+  constexpr int T = Cfg::T;
+  const int tr = threadIdx.x / T;
+  const int t = threadIdx.x - tr * T;
+  float2 u[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) u[k] = lds[lds_at<N, Cfg::S>(t + k * T) + tr];
+  __syncthreads();  // every initial value has been read before the first scatter
+  fft<N, E, DIR, Cfg::S, true>(u, lds + tr, t, tw);
+#pragma unroll
+  for (int k = 0; k < E; ++k) lds[lds_at<N, Cfg::S>(t + k * T) + tr] = u[k];
+  __syncthreads();
+}
+
+// gb[c] = FFT_z( V(x)[c] ) for the rows of this workgroup (the first pass of
+// G(V x); unnormalised, like pass_z)
+template <int N, int E>
+__global__ void __launch_bounds__((ZCfg<N, E>::THREADS))
+mbs_zv_kernel(const float2* __restrict__ x, const float2* __restrict__ eps, float2* __restrict__ gb,
+              const float2* __restrict__ tw, long long vox, long long nrows, int tensor, C2 s, C2 kb,
+              const double* __restrict__ stop) {
+  if (stopped(stop)) return;
+  using Cfg = ZCfg<N, E>;
+  constexpr int R = Cfg::R;
+  __shared__ float2 lds[lds_span<N, Cfg::S>()];
+  const long long row0 = (long long)blockIdx.x * R;
+  const int nv = (int)min((long long)R, nrows - row0) * N;
+  const long long base = row0 * N;
+  for (int j = threadIdx.x; j < R * N; j += Cfg::THREADS) {
+    const int r = j / N, z = j - r * N;
+    float2 o[3] = {make_float2(0.f, 0.f), make_float2(0.f, 0.f), make_float2(0.f, 0.f)};
+    if (j < nv) {
+      float2 v[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = x[c * vox + base + j];
+      apply_v(eps, tensor, base + j, vox, s, kb, v, o);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) lds[zslot<N, E>(z, r, c)] = o[c];
+  }
+  __syncthreads();
+  ztransform<N, E, -1>(lds, tw);
+  for (int j = threadIdx.x; j < nv; j += Cfg::THREADS) {
+    const int r = j / N, z = j - r * N;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gb[c * vox + base + j] = lds[zslot<N, E>(z, r, c)];
+  }
+}
+
+// The last pass of G(V x) fused with the fixed-point update: g = IFFT_z(gb)
+// (unnormalised; the 1/(nx ny nz) is applied by the x pass), then per voxel
+// r = prec_y + V(g + x), x <- x + r, and the FP64 sum of |r|^2 of the
+// workgroup's rows in partial[block].
+template <int N, int E>
+__global__ void __launch_bounds__((ZCfg<N, E>::THREADS))
+mbs_zupdate_kernel(const float2* __restrict__ gb, const float2* __restrict__ eps,
+                   const float2* __restrict__ py, float2* __restrict__ x, const float2* __restrict__ tw,
+                   long long vox, long long nrows, int tensor, C2 s, C2 kb, double* __restrict__ partial,
+                   const double* __restrict__ stop) {
+  if (stopped(stop)) return;
+  using Cfg = ZCfg<N, E>;
+  constexpr int R = Cfg::R;
+  __shared__ float2 lds[lds_span<N, Cfg::S>()];
+  const long long row0 = (long long)blockIdx.x * R;
+  const int nv = (int)min((long long)R, nrows - row0) * N;
+  const long long base = row0 * N;
+  for (int j = threadIdx.x; j < R * N; j += Cfg::THREADS) {
+    const int r = j / N, z = j - r * N;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      lds[zslot<N, E>(z, r, c)] = j < nv ? gb[c * vox + base + j] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  ztransform<N, E, 1>(lds, tw);
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < nv; j += Cfg::THREADS) {
+    const int r = j / N, z = j - r * N;
+    const long long i = base + j;
+    float2 xv[3], t[3], v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      xv[c] = x[c * vox + i];
+      const float2 gv = lds[zslot<N, E>(z, r, c)];
+      t[c] = make_float2(gv.x + xv[c].x, gv.y + xv[c].y);
+    }
+    apply_v(eps, tensor, i, vox, s, kb, t, v);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const long long jj = c * vox + i;
+      const float2 rr = make_float2(py[jj].x + v[c].x, py[jj].y + v[c].y);
+      x[jj] = make_float2(xv[c].x + rr.x, xv[c].y + rr.y);
+      acc += (double)rr.x * rr.x + (double)rr.y * rr.y;
+    }
+  }
+  // fixed-order tree over the threads (padded to a power of two), in the
+  // transform's LDS once every thread has read its values
+  constexpr int P = pow2_ceil(Cfg::THREADS);
+  static_assert(P * sizeof(double) <= lds_span<N, Cfg::S>() * sizeof(float2), "reduction exceeds LDS");
+  __syncthreads();
+  double* sh = reinterpret_cast<double*>(lds);
+  sh[threadIdx.x] = acc;
+  if (threadIdx.x + Cfg::THREADS < P) sh[threadIdx.x + Cfg::THREADS] = 0.0;
+  __syncthreads();
+  for (int w = P / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// (length, values per thread) of the fused z passes: the lengths of the
+// fused Green pass, with its thread split
+#define CXM_ZFUSE_SIZES(X)                                                     \
+  X(8, 8) X(16, 16) X(32, 16) X(64, 16) X(128, 16) X(256, 16) X(20, 20)         \
+  X(40, 20) X(80, 20) X(160, 20) X(12, 12) X(24, 12) X(48, 12) X(96, 12) X(192, 12)
+
+bool zfuse_supported(int n) {
+  switch (n) {
+#define CXM_ZF_CASE(N, E) case N:
+    CXM_ZFUSE_SIZES(CXM_ZF_CASE)
+#undef CXM_ZF_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+// z rows per workgroup of the fused passes (0: no fused pass for n)
+int zfuse_rows(int n) {
+  switch (n) {
+#define CXM_ZF_ROWS(N, E) \
+  case N:                 \
+    return ZCfg<N, E>::R;
+    CXM_ZFUSE_SIZES(CXM_ZF_ROWS)
+#undef CXM_ZF_ROWS
+    default:
+      return 0;
+  }
+}
+
+// x -> gb = FFT_z(V x)
+int launch_zv(int n, const float2* x, const float2* eps, float2* gb, long long vox, long long nrows,
+              int tensor, C2 s, C2 kb, const double* stop, hipStream_t st) {
+  switch (n) {
+#define CXM_ZV_LAUNCH(N, E)                                                                         \
+  case N: {                                                                                         \
+    using Cfg = ZCfg<N, E>;                                                                         \
+    const long long blocks = (nrows + Cfg::R - 1) / Cfg::R;                                         \
+    if (blocks > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "fused z pass grid too large");        \
+    hipLaunchKernelGGL((mbs_zv_kernel<N, E>), dim3((unsigned)blocks), dim3(Cfg::THREADS), 0, st, x, \
+                       eps, gb, twiddles_for(N), vox, nrows, tensor, s, kb, stop);                  \
+    CXM_LAUNCH_CHECK();                                                                             \
+    return CXM_OK;                                                                                  \
+  }
+    CXM_ZFUSE_SIZES(CXM_ZV_LAUNCH)
+#undef CXM_ZV_LAUNCH
+    default:
+      return fail(CXM_ERR_UNSUPPORTED, "no fused z pass for length " + std::to_string(n));
+  }
+}
+
+// gb -> g = IFFT_z(gb); x <- x + prec_y + V(g + x); partial sums of |r|^2 in
+// partial[0, *nblocks)
+int launch_zupdate(int n, const float2* gb, const float2* eps, const float2* py, float2* x, long long vox,
+                   long long nrows, int tensor, C2 s, C2 kb, double* partial, int* nblocks, const double* stop,
+                   hipStream_t st) {
+  switch (n) {
+#define CXM_ZU_LAUNCH(N, E)                                                                          \
+  case N: {                                                                                          \
+    using Cfg = ZCfg<N, E>;                                                                          \
+    const long long groups = (nrows + Cfg::R - 1) / Cfg::R;                                          \
+    if (groups > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "fused z pass grid too large");         \
+    const int blocks = (int)groups;                                                                  \
+    hipLaunchKernelGGL((mbs_zupdate_kernel<N, E>), dim3(blocks), dim3(Cfg::THREADS), 0, st, gb, eps, \
+                       py, x, twiddles_for(N), vox, nrows, tensor, s, kb, partial, stop);            \
+    CXM_LAUNCH_CHECK();                                                                              \
+    *nblocks = blocks;                                                                               \
+    return CXM_OK;                                                                                   \
+  }
+    CXM_ZFUSE_SIZES(CXM_ZU_LAUNCH)
+#undef CXM_ZU_LAUNCH
+    default:
+      return fail(CXM_ERR_UNSUPPORTED, "no fused z pass for length " + std::to_string(n));
+  }
+}
+
 // r = prec_y + V(g + x);  x <- x + r;  partial[block] = sum |r|^2 (FP64)
 __global__ void __launch_bounds__(kRedThreads)
 mbs_update_kernel(const float2* __restrict__ g, const float2* __restrict__ eps,
                   const float2* __restrict__ py, float2* __restrict__ x, long long vox, int tensor,
-                  C2 s, C2 kb, double* __restrict__ partial) {
+                  C2 s, C2 kb, double* __restrict__ partial, const double* __restrict__ stop) {
+  if (stopped(stop)) return;
   double acc = 0.0;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < vox;
        i += (long long)gridDim.x * blockDim.x) {
@@ -286,6 +514,35 @@ sum_partials_kernel(const double* __restrict__ partial, int n, double* __restric
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+// The stopping test of cxm_mbs_iterate_stop after every iteration, on the
+// device: status = [|r|^2 of the last iteration, iterations performed,
+// stopped (0/1), |prec_y|^2]; |r|^2 is the fixed-order sum of the update
+// pass's partials; stop once |r| <= target (rtol >= 0: rtol |prec_y|, else
+// tol) or after maxiter iterations, the test of jaxopt.FixedPointIteration
+// (solver.py:136-142).  Nothing changes once stopped.
+__global__ void __launch_bounds__(kRedThreads)
+mbs_stop_kernel(const double* __restrict__ partial, int n, double* __restrict__ status, int maxiter,
+                double tol, double rtol) {
+  if (status[2] != 0.0) return;
+  __shared__ double sh[kRedThreads];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += kRedThreads) acc += partial[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kRedThreads / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double rs = sh[0];
+    const double k = status[1] + 1.0;
+    const double target = rtol >= 0.0 ? rtol * sqrt(status[3]) : tol;
+    status[0] = rs;
+    status[1] = k;
+    if (sqrt(rs) <= target || k >= (double)maxiter) status[2] = 1.0;
+  }
 }
 
 // partial[block] = max |v| over 3 planar components (as FP64)
@@ -342,13 +599,12 @@ __global__ void mbs_output_kernel(const float2* __restrict__ x, float2* __restri
   }
 }
 
-constexpr int kRedBlocks = 1024;
-
 unsigned blocks_for(long long n) { return (unsigned)std::max(1LL, std::min<long long>((n + 255) / 256, 0x7fffffffLL)); }
 
 // One axis of the 3D FFT of the 3 planar component volumes, in place
 // (dir -1 forward, +1 inverse, unnormalised; `scale` on the output).
-int pass_z(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
+int pass_z(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s,
+           const double* stop = nullptr) {
   // z: contiguous rows
   PipeArgs a;
   memset(&a, 0, sizeof(a));
@@ -364,10 +620,12 @@ int pass_z(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
   a.seq = dir < 0 ? SEQ_F : SEQ_I;
   a.scale = scale;
   a.mid_scale = 1.0f;
+  a.stop = stop;
   return launch_pipe_row(g.nz, a, s);
 }
 
-int pass_y(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
+int pass_y(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s,
+           const double* stop = nullptr) {
   // y: column strips of each (ny x nz) slab, 3 nx planes
   PipeArgs a;
   memset(&a, 0, sizeof(a));
@@ -384,10 +642,12 @@ int pass_y(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
   a.seq = dir < 0 ? SEQ_F : SEQ_I;
   a.scale = scale;
   a.mid_scale = 1.0f;
+  a.stop = stop;
   return launch_pipe_col(g.ny, a, s);
 }
 
-int pass_x(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
+int pass_x(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s,
+           const double* stop = nullptr) {
   // x: column strips of the (nx x ny nz) matrix, 3 planes
   PipeArgs a;
   memset(&a, 0, sizeof(a));
@@ -404,6 +664,7 @@ int pass_x(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
   a.seq = dir < 0 ? SEQ_F : SEQ_I;
   a.scale = scale;
   a.mid_scale = 1.0f;
+  a.stop = stop;
   return launch_pipe_col(g.nx, a, s);
 }
 
@@ -411,24 +672,37 @@ int pass_x(const MbsGeom& g, float2* v, int dir, float scale, hipStream_t s) {
 // forward, the fused x pass (x FFT, Green multiply, inverse x FFT, the
 // 1/(nx ny nz) scale) where its length has one, else the x pass, the Green
 // kernel and the inverse x pass; then the y and z passes inverse.
-int green_apply(const MbsGeom& g, const cxm_mbs_desc* d, float2* gb, C2 sc, C2 kb, hipStream_t s) {
+// The passes between the first and the last z pass of G: y forward, x
+// (fused with the Green multiply where possible), y inverse.
+int green_inner(const MbsGeom& g, const cxm_mbs_desc* d, float2* gb, C2 sc, C2 kb, hipStream_t s,
+                const double* stop = nullptr) {
   int rc;
   const float inv = (float)(1.0 / ((double)g.nx * g.ny * g.nz));
   const float kscale = (float)(2.0 * M_PI / d->spacing / d->k0);
-  if ((rc = pass_z(g, gb, -1, 1.0f, s))) return rc;
-  if ((rc = pass_y(g, gb, -1, 1.0f, s))) return rc;
+  if ((rc = pass_y(g, gb, -1, 1.0f, s, stop))) return rc;
   if (xgreen_supported(g.nx) && option(CXM_OPT_MBS_XGREEN) != 2) {
-    if ((rc = launch_xgreen(g.nx, gb, g.vox, g.ny * g.nz, g.ny, g.nz, kscale, sc, kb, inv, s))) return rc;
+    if ((rc = launch_xgreen(g.nx, gb, g.vox, g.ny * g.nz, g.ny, g.nz, kscale, sc, kb, inv, stop, s))) return rc;
   } else {
-    if ((rc = pass_x(g, gb, -1, 1.0f, s))) return rc;
+    if ((rc = pass_x(g, gb, -1, 1.0f, s, stop))) return rc;
     hipLaunchKernelGGL(mbs_green_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, gb, g.vox, g.nx, g.ny,
-                       g.nz, kscale, sc, kb);
+                       g.nz, kscale, sc, kb, stop);
     CXM_LAUNCH_CHECK();
-    if ((rc = pass_x(g, gb, 1, inv, s))) return rc;
+    if ((rc = pass_x(g, gb, 1, inv, s, stop))) return rc;
   }
-  if ((rc = pass_y(g, gb, 1, 1.0f, s))) return rc;
-  return pass_z(g, gb, 1, 1.0f, s);
+  return pass_y(g, gb, 1, 1.0f, s, stop);
 }
+
+int green_apply(const MbsGeom& g, const cxm_mbs_desc* d, float2* gb, C2 sc, C2 kb, hipStream_t s,
+                const double* stop = nullptr) {
+  int rc;
+  if ((rc = pass_z(g, gb, -1, 1.0f, s, stop))) return rc;
+  if ((rc = green_inner(g, d, gb, sc, kb, s, stop))) return rc;
+  return pass_z(g, gb, 1, 1.0f, s, stop);
+}
+
+// The fixed-point iteration runs its first and last z passes fused with V
+// and with the update where the z length has them (CXM_OPT_MBS_ZFUSE 0).
+bool zfused(const MbsGeom& g) { return zfuse_supported(g.nz) && option(CXM_OPT_MBS_ZFUSE) != 1; }
 
 C2 c2(double re, double im) { return C2{(float)re, (float)im}; }
 
@@ -770,11 +1044,15 @@ bi_end_kernel(const double* __restrict__ partial, BiScal* st, double* __restrict
 
 using namespace cxm;
 
-// workspace: gbuf (3 vox c64) | partial (kRedBlocks f64) | scalars (4 f64)
+// workspace: gbuf (3 vox c64) | partial (f64: kRedBlocks, or one per row
+// group of the fused z pass if more) | scalars (4 f64)
 extern "C" size_t cxm_mbs_workspace_size(const cxm_mbs_desc* d) {
   MbsGeom g;
   if (mbs_geom(d, g)) return 0;
-  return (size_t)3 * g.vox * sizeof(float2) + (kRedBlocks + 4) * sizeof(double);
+  long long parts = kRedBlocks;
+  const int zr = zfuse_rows(g.nz);
+  if (zr > 0) parts = std::max(parts, ((long long)g.nx * g.ny + zr - 1) / zr);
+  return (size_t)3 * g.vox * sizeof(float2) + (size_t)(parts + 4) * sizeof(double);
 }
 
 extern "C" int cxm_mbs_precondition(void* stream, const void* eps, const void* source,
@@ -795,11 +1073,11 @@ extern "C" int cxm_mbs_precondition(void* stream, const void* eps, const void* s
   double* partial = (double*)((char*)workspace + (size_t)3 * g.vox * sizeof(float2));
   // t = s * source (planar), G(t), prec_y = -V(G(t))
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)source,
-                     nullptr, py, g.vox, 1, 0, sc, kb, sc);
+                     nullptr, py, g.vox, 1, 0, sc, kb, sc, nullptr);
   CXM_LAUNCH_CHECK();
   if ((rc = green_apply(g, d, py, sc, kb, s))) return rc;
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)py,
-                     (const float2*)eps, py, g.vox, 0, d->eps_tensor, sc, kb, C2{-1.0f, 0.0f});
+                     (const float2*)eps, py, g.vox, 0, d->eps_tensor, sc, kb, C2{-1.0f, 0.0f}, nullptr);
   CXM_LAUNCH_CHECK();
   // num_scale = max |prec_y|; prec_y /= num_scale
   hipLaunchKernelGGL(maxabs_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, (const float2*)py,
@@ -813,6 +1091,60 @@ extern "C" int cxm_mbs_precondition(void* stream, const void* eps, const void* s
   return CXM_OK;
 }
 
+namespace cxm {
+namespace {
+// `iterations` fixed-point steps; with `status` (cxm_mbs_iterate_stop) each
+// step is followed by the device-side stopping test and every kernel of a
+// step returns at once after the stop, else |r|^2 of the last step goes to
+// residual_sq.
+int mbs_steps(const MbsGeom& g, const cxm_mbs_desc* d, const void* eps, const void* prec_y, void* x,
+              int iterations, void* workspace, double* residual_sq, double* status, int maxiter, double tol,
+              double rtol, hipStream_t s) {
+  int rc;
+  const C2 sc = c2(d->scale_re, d->scale_im);
+  const C2 kb = c2(d->bias_re * d->scale_re - d->bias_im * d->scale_im + 1.0,
+                   d->bias_re * d->scale_im + d->bias_im * d->scale_re);
+  float2* gb = (float2*)workspace;
+  double* partial = (double*)((char*)workspace + (size_t)3 * g.vox * sizeof(float2));
+  const double* stop = status ? status + 2 : nullptr;
+  const bool fz = zfused(g);
+  const long long nrows = (long long)g.nx * g.ny;
+  int nparts = kRedBlocks;
+  for (int it = 0; it < iterations; ++it) {
+    if (fz) {
+      if ((rc = launch_zv(g.nz, (const float2*)x, (const float2*)eps, gb, g.vox, nrows, d->eps_tensor, sc,
+                          kb, stop, s)))
+        return rc;
+      if ((rc = green_inner(g, d, gb, sc, kb, s, stop))) return rc;
+      if ((rc = launch_zupdate(g.nz, gb, (const float2*)eps, (const float2*)prec_y, (float2*)x, g.vox, nrows,
+                               d->eps_tensor, sc, kb, partial, &nparts, stop, s)))
+        return rc;
+    } else {
+      hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)x,
+                         (const float2*)eps, gb, g.vox, 0, d->eps_tensor, sc, kb, C2{1.0f, 0.0f}, stop);
+      CXM_LAUNCH_CHECK();
+      if ((rc = green_apply(g, d, gb, sc, kb, s, stop))) return rc;
+      hipLaunchKernelGGL(mbs_update_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s,
+                         (const float2*)gb, (const float2*)eps, (const float2*)prec_y, (float2*)x,
+                         g.vox, d->eps_tensor, sc, kb, partial, stop);
+      CXM_LAUNCH_CHECK();
+    }
+    if (status) {
+      hipLaunchKernelGGL(mbs_stop_kernel, dim3(1), dim3(kRedThreads), 0, s, (const double*)partial, nparts,
+                         status, maxiter, tol, rtol);
+      CXM_LAUNCH_CHECK();
+    }
+  }
+  if (!status && iterations > 0) {
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, nparts,
+                       residual_sq);
+    CXM_LAUNCH_CHECK();
+  }
+  return CXM_OK;
+}
+}  // namespace
+}  // namespace cxm
+
 extern "C" int cxm_mbs_iterate(void* stream, const void* eps, const void* prec_y, void* x,
                                int32_t iterations, double* residual_sq, void* workspace,
                                size_t workspace_bytes, const cxm_mbs_desc* d) {
@@ -824,28 +1156,25 @@ extern "C" int cxm_mbs_iterate(void* stream, const void* eps, const void* prec_y
   if (!workspace || workspace_bytes < cxm_mbs_workspace_size(d))
     return fail(CXM_ERR_WORKSPACE, "workspace too small");
   if ((rc = ensure_init())) return rc;
-  hipStream_t s = (hipStream_t)stream;
-  const C2 sc = c2(d->scale_re, d->scale_im);
-  const C2 kb = c2(d->bias_re * d->scale_re - d->bias_im * d->scale_im + 1.0,
-                   d->bias_re * d->scale_im + d->bias_im * d->scale_re);
-  float2* gb = (float2*)workspace;
-  double* partial = (double*)((char*)workspace + (size_t)3 * g.vox * sizeof(float2));
-  for (int it = 0; it < iterations; ++it) {
-    hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)x,
-                       (const float2*)eps, gb, g.vox, 0, d->eps_tensor, sc, kb, C2{1.0f, 0.0f});
-    CXM_LAUNCH_CHECK();
-    if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
-    hipLaunchKernelGGL(mbs_update_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s,
-                       (const float2*)gb, (const float2*)eps, (const float2*)prec_y, (float2*)x,
-                       g.vox, d->eps_tensor, sc, kb, partial);
-    CXM_LAUNCH_CHECK();
-  }
-  if (iterations > 0) {
-    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kRedThreads), 0, s, partial, kRedBlocks,
-                       residual_sq);
-    CXM_LAUNCH_CHECK();
-  }
-  return CXM_OK;
+  return mbs_steps(g, d, eps, prec_y, x, iterations, workspace, residual_sq, nullptr, 0, 0.0, -1.0,
+                   (hipStream_t)stream);
+}
+
+extern "C" int cxm_mbs_iterate_stop(void* stream, const void* eps, const void* prec_y, void* x,
+                                    int32_t iterations, int32_t maxiter, double tol, double rtol,
+                                    double* status, void* workspace, size_t workspace_bytes,
+                                    const cxm_mbs_desc* d) {
+  MbsGeom g;
+  int rc = mbs_geom(d, g);
+  if (rc) return rc;
+  if (iterations < 0 || maxiter < 0) return fail(CXM_ERR_INVALID, "iterations and maxiter must be >= 0");
+  if (!(tol >= 0.0)) return fail(CXM_ERR_INVALID, "tol must be >= 0");
+  if (!eps || !prec_y || !x || !status) return fail(CXM_ERR_INVALID, "null buffer");
+  if (!workspace || workspace_bytes < cxm_mbs_workspace_size(d))
+    return fail(CXM_ERR_WORKSPACE, "workspace too small");
+  if ((rc = ensure_init())) return rc;
+  return mbs_steps(g, d, eps, prec_y, x, iterations, workspace, nullptr, status, maxiter, tol,
+                   rtol >= 0.0 ? rtol : -1.0, (hipStream_t)stream);
 }
 
 extern "C" int cxm_mbs_output(void* stream, const void* x, const double* num_scale, void* e,
@@ -922,7 +1251,7 @@ extern "C" int cxm_mbs_bicgstab_init(void* stream, const void* eps, const void* 
   const BiLayout b = bi_layout(g, state);
   // r0 = b - A x0 = b + V(G(V(x0)) + x0)
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)x,
-                     (const float2*)eps, gb, g.vox, 0, d->eps_tensor, sc, kb, C2{1.0f, 0.0f});
+                     (const float2*)eps, gb, g.vox, 0, d->eps_tensor, sc, kb, C2{1.0f, 0.0f}, nullptr);
   CXM_LAUNCH_CHECK();
   if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
   hipLaunchKernelGGL(bi_r0_kernel, dim3(kRedBlocks), dim3(kRedThreads), 0, s, (const float2*)gb,
@@ -956,7 +1285,7 @@ extern "C" int cxm_mbs_bicgstab_iterate(void* stream, const void* eps, void* x, 
     hipLaunchKernelGGL(bi_p_kernel, rg, rt, 0, s, (const float2*)b.r, b.p, (const float2*)b.q, ep, gb,
                        g.vox, d->eps_tensor, sc, kb, (const BiScal*)b.st);
     CXM_LAUNCH_CHECK();
-    if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
+    if ((rc = green_apply(g, d, gb, sc, kb, s, &b.st->done))) return rc;
     hipLaunchKernelGGL(bi_q_kernel, rg, rt, 0, s, (const float2*)gb, (const float2*)b.p,
                        (const float2*)b.rh, ep, b.q, g.vox, d->eps_tensor, sc, kb, (const BiScal*)b.st,
                        b.partial);
@@ -968,7 +1297,7 @@ extern "C" int cxm_mbs_bicgstab_iterate(void* stream, const void* eps, void* x, 
     CXM_LAUNCH_CHECK();
     hipLaunchKernelGGL(bi_early_kernel, dim3(1), rt, 0, s, (const double*)b.partial, b.st);
     CXM_LAUNCH_CHECK();
-    if ((rc = green_apply(g, d, gb, sc, kb, s))) return rc;
+    if ((rc = green_apply(g, d, gb, sc, kb, s, &b.st->done))) return rc;
     hipLaunchKernelGGL(bi_t_kernel, rg, rt, 0, s, gb, (const float2*)b.r, ep, g.vox, d->eps_tensor, sc,
                        kb, (const BiScal*)b.st, b.partial);
     CXM_LAUNCH_CHECK();
@@ -1335,12 +1664,12 @@ extern "C" int cxm_mbs_vjp(void* stream, const void* eps, const void* source, co
   float2* mu = gss + 3 * g.vox;
   // G(V E)
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)E,
-                     (const float2*)eps, gve, g.vox, 1, 0, sc, kb, C2{1.0f, 0.0f});
+                     (const float2*)eps, gve, g.vox, 1, 0, sc, kb, C2{1.0f, 0.0f}, nullptr);
   CXM_LAUNCH_CHECK();
   if ((rc = green_apply(g, d, gve, sc, kb, s))) return rc;
   // G(s S)
   hipLaunchKernelGGL(mbs_v_kernel, dim3(blocks_for(g.vox)), dim3(256), 0, s, (const float2*)source,
-                     nullptr, gss, g.vox, 1, 0, sc, kb, sc);
+                     nullptr, gss, g.vox, 1, 0, sc, kb, sc, nullptr);
   CXM_LAUNCH_CHECK();
   if ((rc = green_apply(g, d, gss, sc, kb, s))) return rc;
   // mu = G^H(V^H lam)

@@ -50,6 +50,9 @@ struct PipeArgs {
   long long mid_plane_stride;
   int mid_row_stride, mid_y0, mid_x0, mid_h, mid_w;
   float scale, mid_scale;
+  // optional: the pass does nothing once *stop != 0 (a solver's device-side
+  // stop flag, so that iterations enqueued past convergence cost no work)
+  const double* stop;
   OpList pre, mid, post;
   OpTable tab;
 };
@@ -167,6 +170,7 @@ struct PipeRowCfg {
 template <int N, int E>
 __global__ void __launch_bounds__((PipeRowCfg<N, E>::THREADS))
 pipe_row_kernel(PipeArgs a) {
+  if (a.stop && *a.stop != 0.0) return;
   using Cfg = PipeRowCfg<N, E>;
   constexpr int T = Cfg::T;
   constexpr int SPAN = lds_span<N, 1>();
@@ -346,6 +350,7 @@ struct PipeColCfg {
 template <int N, int E, int C>
 __global__ void __launch_bounds__((PipeColCfg<N, E, C>::THREADS))
 pipe_col_kernel(PipeArgs a) {
+  if (a.stop && *a.stop != 0.0) return;
   constexpr int T = N / E;
   __shared__ float2 lds[lds_span<N, C>()];
   const int c = threadIdx.x % C;

@@ -239,8 +239,15 @@ def _desc(sample: Sample, k0: float, shift: complex, scale: complex):
 
 class Solver:
     """Stepwise solve: the work is enqueued in chunks of iterations and the
-    residual comes back through pinned memory and an event, so a caller can
-    interleave other work without blocking (the serving loop does)."""
+    stopping test runs on the device after every iteration
+    (cxm_mbs_iterate_stop, jaxopt.FixedPointIteration's rule: stop once the
+    update norm is at most the tolerance), so iterations enqueued past the
+    stop cost nothing.  The status of each chunk comes back through pinned
+    memory and an event, so a caller can interleave other work without
+    blocking (the serving loop does) or keep the next chunk queued while it
+    waits for the last one (``_run`` does)."""
+
+    _RING = 2  # chunks whose status read-back may be in flight at once
 
     def __init__(self, sample: Sample, source: Source, initial_E=None, *, shift_scale=None,
                  tol: float = 1e-3, rtol: float | None = None, maxiter: int = 1000):
@@ -267,45 +274,67 @@ class Solver:
             self.x = (E0.permute(3, 0, 1, 2) / self.num[0].to(torch.float32)).resolve_conj().contiguous()
         self.rtol, self.tol = rtol, float(tol)
         self.maxiter = int(maxiter)
-        # [residual^2, |prec_y|^2], device and pinned host copies
-        self.res_dev = torch.zeros(2, dtype=torch.float64, device=self.dev)
-        self.res_host = torch.zeros(2, dtype=torch.float64, pin_memory=True)
+        # device status: [|r|^2 of the last iteration, iterations, stopped, |prec_y|^2]
+        self.status_dev = torch.zeros(4, dtype=torch.float64, device=self.dev)
         if rtol is not None:
             pws = torch.empty(max(1, L.lib().cxm_plane_sumsq_workspace_size(1)), dtype=torch.uint8,
                               device=self.dev)
-            L.check(L.lib().cxm_plane_sumsq_c64(s, self.py.data_ptr(), self.res_dev[1:].data_ptr(), 1,
+            L.check(L.lib().cxm_plane_sumsq_c64(s, self.py.data_ptr(), self.status_dev[3:].data_ptr(), 1,
                                                 self.py.numel(), 1, None, pws.data_ptr(), pws.numel()))
             self._pws = pws
-        self.target = float("nan")
-        self.event = torch.cuda.Event()
+        self._ring = [(torch.zeros(4, dtype=torch.float64, pin_memory=True), torch.cuda.Event())
+                      for _ in range(self._RING)]
+        self._inflight: list[int] = []
+        self._next = 0
+        self.event = self._ring[0][1]
+        self.enqueued = 0
         self.iterations, self.residual = 0, float("inf")
-        self.pending = False
+        self.stopped = self.maxiter <= 0
+
+    @property
+    def pending(self) -> bool:
+        return bool(self._inflight)
 
     def step(self, k: int) -> None:
-        """Enqueue k iterations and the residual read-back."""
-        k = min(int(k), self.maxiter - self.iterations)
-        if k <= 0:
+        """Enqueue k iterations (fewer at maxiter) and the status read-back;
+        a no-op once the solve is known to have stopped, or while the
+        read-back ring is full."""
+        k = min(int(k), self.maxiter - self.enqueued)
+        if k <= 0 or self.stopped or len(self._inflight) >= self._RING:
             return
-        L.check(L.lib().cxm_mbs_iterate(L.stream_ptr(self.dev), self.eps.data_ptr(), self.py.data_ptr(),
-                                        self.x.data_ptr(), k, self.res_dev.data_ptr(), self.ws.data_ptr(),
-                                        self.ws.numel(), ctypes.byref(self.d)))
-        self.res_host.copy_(self.res_dev, non_blocking=True)
-        self.event.record()
-        self.iterations += k
-        self.pending = True
+        L.check(L.lib().cxm_mbs_iterate_stop(
+            L.stream_ptr(self.dev), self.eps.data_ptr(), self.py.data_ptr(), self.x.data_ptr(), k,
+            self.maxiter, self.tol, -1.0 if self.rtol is None else float(self.rtol),
+            self.status_dev.data_ptr(), self.ws.data_ptr(), self.ws.numel(), ctypes.byref(self.d)))
+        host, ev = self._ring[self._next]
+        host.copy_(self.status_dev, non_blocking=True)
+        ev.record()
+        self.event = ev
+        self._inflight.append(self._next)
+        self._next = (self._next + 1) % self._RING
+        self.enqueued += k
+
+    def _take(self, slot: int) -> None:
+        r2, it, stop, _ = self._ring[slot][0].tolist()
+        self.residual = float(np.sqrt(r2)) if it > 0 else float("inf")
+        self.iterations = int(it)
+        self.stopped = self.stopped or bool(stop)
 
     def poll(self) -> bool:
-        """True once the last step's residual has arrived (non-blocking)."""
-        if self.pending and self.event.query():
-            r2, p2 = self.res_host.tolist()
-            self.residual = float(np.sqrt(r2))
-            self.target = self.rtol * float(np.sqrt(p2)) if self.rtol is not None else self.tol
-            self.pending = False
-        return not self.pending
+        """True once every enqueued chunk's status has arrived (non-blocking)."""
+        while self._inflight and self._ring[self._inflight[0]][1].query():
+            self._take(self._inflight.pop(0))
+        return not self._inflight
+
+    def wait(self) -> None:
+        """Block until the oldest chunk in flight has finished."""
+        if self._inflight:
+            self._ring[self._inflight[0]][1].synchronize()
+            self._take(self._inflight.pop(0))
 
     @property
     def converged(self) -> bool:
-        return self.residual < self.target or self.iterations >= self.maxiter
+        return self.stopped or self.iterations >= self.maxiter
 
     def result(self) -> torch.Tensor:
         E = torch.empty(self.shape + (3,), dtype=torch.complex64, device=self.dev)
@@ -338,10 +367,14 @@ class BiCGStabSolver(Solver):
         self.k, self.stopped = 0, False
         self._readback()
 
+    @property
+    def pending(self) -> bool:
+        return self._bpending
+
     def _readback(self):
         self.status_host.copy_(self.status_dev, non_blocking=True)
         self.event.record()
-        self.pending = True
+        self._bpending = True
 
     def step(self, k: int) -> None:
         k = min(int(k), self.maxiter - self.iterations)
@@ -356,12 +389,16 @@ class BiCGStabSolver(Solver):
         self._readback()
 
     def poll(self) -> bool:
-        if self.pending and self.event.query():
+        if self._bpending and self.event.query():
             rs, k, stopped, _ = self.status_host.tolist()
             self.residual = float(np.sqrt(rs))
             self.k, self.stopped = int(k), bool(stopped)
-            self.pending = False
-        return not self.pending
+            self._bpending = False
+        return not self._bpending
+
+    def wait(self) -> None:
+        self.event.synchronize()
+        self.poll()
 
     @property
     def converged(self) -> bool:
@@ -385,12 +422,10 @@ def _run(sample: Sample, source: Source, initial_E, maxiter, tol, use_bicgstab, 
         return sv, sv.result()
     sv = Solver(sample, source, initial_E, shift_scale=shift_scale, tol=tol, rtol=rtol,
                 maxiter=maxiter)
-    while True:
-        sv.step(check_every)
-        sv.event.synchronize()
-        sv.poll()
-        if sv.converged:
-            break
+    sv.step(check_every)
+    while not sv.converged:
+        sv.step(check_every)  # the next chunk stays queued while the host waits
+        sv.wait()
     return sv, sv.result()
 
 
@@ -436,14 +471,19 @@ def _adjoint_solve(eps, d, Ebar, *, tol, rtol, maxiter, use_bicgstab, check_ever
                                                  ctypes.byref(da)))
             it += k
     else:
-        target = float(tol) if rtol is None else float(rtol) * float(torch.linalg.vector_norm(py))
-        res = torch.zeros(1, dtype=torch.float64, device=dev)
+        # the forward's stopping rule, tested on the device after every iteration
+        status = torch.zeros(4, dtype=torch.float64, device=dev)
+        if rtol is not None:
+            pws = torch.empty(max(1, lib.cxm_plane_sumsq_workspace_size(1)), dtype=torch.uint8, device=dev)
+            L.check(lib.cxm_plane_sumsq_c64(s, py.data_ptr(), status[3:].data_ptr(), 1, py.numel(), 1, None,
+                                            pws.data_ptr(), pws.numel()))
         while it < maxiter:
             k = min(check_every, maxiter - it)
-            L.check(lib.cxm_mbs_iterate(s, eps_c.data_ptr(), py.data_ptr(), x.data_ptr(), k,
-                                        res.data_ptr(), ws.data_ptr(), ws.numel(), ctypes.byref(da)))
+            L.check(lib.cxm_mbs_iterate_stop(s, eps_c.data_ptr(), py.data_ptr(), x.data_ptr(), k, maxiter,
+                                             float(tol), -1.0 if rtol is None else float(rtol),
+                                             status.data_ptr(), ws.data_ptr(), ws.numel(), ctypes.byref(da)))
             it += k
-            if float(res.item()) ** 0.5 < target:
+            if bool(status[2].item()):
                 break
     return x * norm
 
@@ -506,12 +546,14 @@ def solve(sample: Sample, source: Source, initial_E=None, maxiter: int = 1000, t
           check_every: int = 8, shift_scale=None, return_info: bool = False,
           adjoint_tol: float | None = None):
     """Electric field E (Nx, Ny, Nz, 3) solving the scattering problem
-    (solver.py:101-146).  Fixed-point iteration (default): stops when the l2
-    norm of the update of the normalised iterate falls below ``tol``
-    (jaxopt's FixedPointIteration criterion) or, with ``rtol``, below
-    rtol |prec_y|.  ``use_bicgstab``: BiCGStab until |r| <= tol |prec_y|
-    (jax.scipy.sparse.linalg.bicgstab's criterion) or breakdown.  The norm
-    is read back every ``check_every`` iterations.
+    (solver.py:101-146).  Fixed-point iteration (default): stops after the
+    first iteration whose update of the normalised iterate has l2 norm at
+    most ``tol`` (jaxopt's FixedPointIteration criterion, tested on the
+    device after every iteration) or, with ``rtol``, at most rtol |prec_y|.
+    ``use_bicgstab``: BiCGStab until |r| <= tol |prec_y|
+    (jax.scipy.sparse.linalg.bicgstab's criterion) or breakdown.  The host
+    enqueues ``check_every`` iterations at a time and reads the status of
+    each chunk back while the next one runs.
 
     Differentiable in the permittivity and the source field (scalar
     permittivity) by implicit differentiation: the backward pass solves the

@@ -635,6 +635,17 @@ int cxm_multislice_bwd(void* stream, const void* in, const float* dn, const floa
  *   cxm_mbs_iterate:      `iterations` steps x <- x + prec_y - prec_forward(x) on the
  *                         planar x; residual_sq (device double) = |r|^2 of the last
  *                         step, r = x_new - x
+ *   cxm_mbs_iterate_stop: the same steps with the stopping test of
+ *                         jaxopt.FixedPointIteration (run by solve,
+ *                         solver.py:136-142: after every step, stop once the
+ *                         update norm |r| <= tol, or after maxiter steps) on
+ *                         the device: status (4 device doubles, zero-initialised
+ *                         by the caller except [3]) = |r|^2 of the last step,
+ *                         steps performed, stopped (0/1), |prec_y|^2 (set by the
+ *                         caller, used when rtol >= 0); the tolerance is
+ *                         rtol |prec_y| when rtol >= 0, else tol; steps enqueued
+ *                         after the stop leave x and status unchanged and cost
+ *                         one early return per launch
  *   cxm_mbs_output:       E = x num_scale, nx x ny x nz x 3
  * Workspace: cxm_mbs_workspace_size (one field plus reduction scratch).
  */
@@ -654,6 +665,9 @@ int cxm_mbs_precondition(void* stream, const void* eps, const void* source, void
 int cxm_mbs_iterate(void* stream, const void* eps, const void* prec_y, void* x, int32_t iterations,
                     double* residual_sq, void* workspace, size_t workspace_bytes,
                     const cxm_mbs_desc* d);
+int cxm_mbs_iterate_stop(void* stream, const void* eps, const void* prec_y, void* x,
+                         int32_t iterations, int32_t maxiter, double tol, double rtol, double* status,
+                         void* workspace, size_t workspace_bytes, const cxm_mbs_desc* d);
 int cxm_mbs_output(void* stream, const void* x, const double* num_scale, void* e,
                    const cxm_mbs_desc* d);
 
@@ -824,6 +838,12 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         Green's projection and the inverse x-axis FFT run
  *                         as one pass where the x length has one; 2 = as
  *                         three launches.  Equal to float32 rounding.
+ *   CXM_OPT_MBS_ZFUSE     modified Born series fixed-point iteration: 0 = the
+ *                         first z-axis FFT pass fused with the V multiply and
+ *                         the last with the update and its residual partial
+ *                         sums, where the z length has the fused passes;
+ *                         1 = separate V, z-pass and update launches.  Same
+ *                         iterates; the residual sum's order differs.
  *   CXM_OPT_GRAD_GROUPS   cxm_psf4f_bwd: 0 = the default number of plane
  *                         groups of the phase-gradient pass (4), 1..12 =
  *                         that many (each group sums its planes in
@@ -845,6 +865,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_COL_PERSIST_E 13
 #define CXM_OPT_COL_PERSIST_MODE 14
 #define CXM_OPT_GRAD_GROUPS 15
+#define CXM_OPT_MBS_ZFUSE 16
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

@@ -84,6 +84,38 @@ def test_c4_volume_converges(seed):
     assert pde < 2e-4
 
 
+@pytest.mark.parametrize("tol", [1e-2, 1e-3])
+def test_fixed_point_stops_where_the_oracle_does(tol):
+    """The device-side stopping test (after every iteration, |r| <= tol,
+    jaxopt.FixedPointIteration's rule) stops at the iteration the FP64
+    oracle's per-iteration test picks, and the iterate there matches."""
+    eps, spacing, J = _setup(24, 4, 2, 3)  # 32^3 with the boundary
+    S = om.source_field(J, K0)
+    shift, scale = om.get_shift_and_scale(eps)
+    m = om.MBS(eps, spacing, K0, shift, scale)
+    py, num = m.prec_y(S)
+    x, k_ref = py.copy(), 0
+    norms = []
+    while k_ref < 500:
+        x, r = m.iterate(x, py, 1)
+        k_ref += 1
+        norms.append(np.linalg.norm(r))
+        if norms[-1] <= tol:
+            break
+    sample = mbs.Sample(torch.as_tensor(eps.astype(np.complex64)).cuda(), spacing)
+    src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
+    res = mbs.solve(sample, src, maxiter=500, tol=tol, shift_scale=(shift, scale), check_every=8,
+                    return_info=True)
+    print(f"fixed point tol {tol}: {res.iterations} iterations (oracle {k_ref}), "
+          f"residual {res.residual:.4e} (oracle {norms[-1]:.4e}, previous {norms[-2]:.4e})")
+    # the two norms straddle tol by far more than float32 rounding here
+    assert norms[-1] < tol * (1 - 1e-3) and norms[-2] > tol * (1 + 1e-3)
+    assert res.iterations == k_ref
+    # r = x_new - x of float32 iterates: cancellation leaves ~1e-3 relative at |r| ~ 1e-3
+    assert abs(res.residual - norms[-1]) <= 5e-3 * norms[-1]
+    assert rel_l2(res.E.cpu().numpy(), x * num) < 1e-5
+
+
 @pytest.mark.parametrize("iters", [1, 3, 6])
 def test_bicgstab_iterates_match_oracle(iters):
     # solve(use_bicgstab=True) with maxiter k and tol 0 runs exactly k
@@ -251,6 +283,39 @@ def test_fused_green_pass_matches_separate_passes(n, bc):
     finally:
         L.check(L.lib().cxm_set_option(L.CXM_OPT_MBS_XGREEN, 0))
     assert rel_l2(out[0], out[1]) < 1e-6
+
+
+@pytest.mark.parametrize("n,bc,tensor", [(24, 4, False), (40, 4, False), (64, 8, False), (120, 20, False),
+                                          (24, 4, True)])
+def test_fused_z_passes_match_separate_passes(n, bc, tensor):
+    # the first z pass fused with V and the last with the update
+    # (CXM_OPT_MBS_ZFUSE 0) against separate launches (1): the same
+    # iteration to float32 rounding, and the same residual to FP64 order
+    from chromatix_amd import _lib as L
+
+    eps, spacing, J = _setup(n, bc, 2, 13)
+    if tensor:
+        T = np.zeros((3, 3) + eps.shape, np.complex64)
+        for a in range(3):
+            T[a, a] = eps
+        T[0, 1] = T[1, 0] = 0.01
+        eps = T
+    shift, scale = om.get_shift_and_scale_tensor(eps) if tensor else om.get_shift_and_scale(eps)
+    sample = mbs.Sample(torch.as_tensor(eps.astype(np.complex64)).cuda(), spacing)
+    src = mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0)
+    out, res = [], []
+    try:
+        for opt in (0, 1):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_MBS_ZFUSE, opt))
+            r = mbs.solve(sample, src, maxiter=16, tol=0.0, shift_scale=(shift, scale), return_info=True)
+            out.append(r.E.cpu().numpy())
+            res.append(float(r.residual))
+    finally:
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_MBS_ZFUSE, 0))
+    err = rel_l2(out[0], out[1])
+    print(f"fused z passes {n + 2 * bc}^3 tensor={tensor}: {err:.2e}, residuals {res[0]:.6e} {res[1]:.6e}")
+    assert err < 1e-6
+    assert abs(res[0] - res[1]) <= 1e-5 * abs(res[1])
 
 
 @pytest.mark.parametrize("bicg", [False, True])
