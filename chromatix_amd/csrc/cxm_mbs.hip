@@ -98,6 +98,13 @@ __global__ void mbs_v_kernel(const float2* __restrict__ src, const float2* __res
   for (int c = 0; c < 3; ++c) out[c * vox + i] = cmulf(o[c], pre);
 }
 
+// K along an axis of n bins: bin b (signed) times kscale / n
+CXM_DEV float k_step(float kscale, int n) { return kscale / (float)n; }
+// 1/x by the hardware reciprocal (within 1 ulp; the Green's operator's
+// factors are the only place it is used, where IEEE division cost a third
+// of the fused x pass's arithmetic)
+CXM_DEV float rcp_approx(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // In-place Green multiply of the spectrum Y (3 planar components):
 // Y <- P_T Y / (kb - s |K|^2) + P_L Y / kb, P_L = K K^T / |K|^2 (0 at K = 0)
 __global__ void mbs_green_kernel(float2* __restrict__ y, long long vox, int nx, int ny, int nz,
@@ -109,18 +116,18 @@ __global__ void mbs_green_kernel(float2* __restrict__ y, long long vox, int nx, 
   const int iy = (int)((i / nz) % ny);
   const int ix = (int)(i / ((long long)nz * ny));
   // K = 2 pi fftfreq / spacing / k0
-  const float kx = kscale * (float)signed_bin(ix, nx) / (float)nx;
-  const float ky = kscale * (float)signed_bin(iy, ny) / (float)ny;
-  const float kz = kscale * (float)signed_bin(iz, nz) / (float)nz;
+  const float kx = (float)signed_bin(ix, nx) * k_step(kscale, nx);
+  const float ky = (float)signed_bin(iy, ny) * k_step(kscale, ny);
+  const float kz = (float)signed_bin(iz, nz) * k_step(kscale, nz);
   const float k2 = kx * kx + ky * ky + kz * kz;
   const float2 y0 = y[i], y1 = y[vox + i], y2 = y[2 * vox + i];
   float2 p = make_float2(kx * y0.x + ky * y1.x + kz * y2.x, kx * y0.y + ky * y1.y + kz * y2.y);
-  const float inv = k2 != 0.0f ? 1.0f / k2 : 0.0f;
+  const float inv = k2 != 0.0f ? rcp_approx(k2) : 0.0f;
   p = make_float2(p.x * inv, p.y * inv);
   // 1 / (kb - s k2) and 1 / kb
   const float dre = kb.re - s.re * k2, dim = kb.im - s.im * k2;
-  const float dn = dre * dre + dim * dim;
-  const C2 gt{dre / dn, -dim / dn};
+  const float rdn = rcp_approx(dre * dre + dim * dim);
+  const C2 gt{dre * rdn, -dim * rdn};
   const float bn = kb.re * kb.re + kb.im * kb.im;
   const C2 gl{kb.re / bn, -kb.im / bn};
   const float kk[3] = {kx, ky, kz};
@@ -172,24 +179,25 @@ mbs_xgreen_kernel(float2* __restrict__ v, const float2* __restrict__ tw, long lo
   __syncthreads();
   const int iy = colok ? col / nz : 0;
   const int iz = colok ? col - iy * nz : 0;
-  const float ky = kscale * (float)signed_bin(iy, ny) / (float)ny;
-  const float kz = kscale * (float)signed_bin(iz, nz) / (float)nz;
+  const float ky = (float)signed_bin(iy, ny) * k_step(kscale, ny);
+  const float kz = (float)signed_bin(iz, nz) * k_step(kscale, nz);
+  const float kxs = k_step(kscale, N);
   const float bn = kb.re * kb.re + kb.im * kb.im;
   const C2 gl{kb.re / bn, -kb.im / bn};
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int i = ty + k * T;
-    const float kx = kscale * (float)signed_bin(i, N) / (float)N;
+    const float kx = (float)signed_bin(i, N) * kxs;
     const float k2 = kx * kx + ky * ky + kz * kz;
     const float2 y0 = lds[c + lds_at<N, CC>(i)];
     const float2 y1 = lds[C + c + lds_at<N, CC>(i)];
     const float2 y2 = lds[2 * C + c + lds_at<N, CC>(i)];
     float2 p = make_float2(kx * y0.x + ky * y1.x + kz * y2.x, kx * y0.y + ky * y1.y + kz * y2.y);
-    const float inv = k2 != 0.0f ? 1.0f / k2 : 0.0f;
+    const float inv = k2 != 0.0f ? rcp_approx(k2) : 0.0f;
     p = make_float2(p.x * inv, p.y * inv);
     const float dre = kb.re - s.re * k2, dim = kb.im - s.im * k2;
-    const float dn = dre * dre + dim * dim;
-    const C2 gt{dre / dn, -dim / dn};
+    const float rdn = rcp_approx(dre * dre + dim * dim);
+    const C2 gt{dre * rdn, -dim * rdn};
     const float kq = q == 0 ? kx : (q == 1 ? ky : kz);
     const float2 yq = q == 0 ? y0 : (q == 1 ? y1 : y2);
     const float2 yl = make_float2(kq * p.x, kq * p.y);
