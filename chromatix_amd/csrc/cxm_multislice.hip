@@ -21,7 +21,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "cxm_pipe_host.h"
 
@@ -60,6 +62,124 @@ int ms_geom(const cxm_multislice_desc* d, MsGeom& g) {
   return CXM_OK;
 }
 
+// ---- dedicated row pass of a slice -------------------------------------------
+// In place on row-transformed padded planes: inverse row FFT, the thin-sample
+// factor exp(i 2 pi (dn + i abs) t / lam) (forward) or its adjoint with the
+// dn / absorption gradient k Im(g conj(u)), -k Re(g conj(u)) (ADJ, u the
+// stored forward field after the sample, g the adjoint value before the
+// factor), forward row FFT.  The arithmetic of the operator pipeline's
+// OP_THIN / OP_THINADJ row pass (same transforms for the same E), with the
+// factor applied to the values in registers instead of through the
+// pipeline's operator stage.  One row per workgroup of N/E threads.
+struct MsRowArgs {
+  float2* ws;
+  const float2* tw;
+  long long plane;  // elements per padded plane
+  int Ny;
+  const float* dn;  // slice arrays, h x w, centred at (pad_y, pad_x)
+  const float* ab;  // may be null
+  int h, w, pad_y, pad_x;
+  float t, lam, k;  // thickness, wavelength, 2 pi t / lam (ADJ)
+  // forward: optional store of the field after the sample, grid position
+  // (y, x) -> mid_out[p * mid_pstride + (y - mid_y0) * mid_rstride + x - mid_x0]
+  float2* mid_out;
+  long long mid_pstride;
+  int mid_rstride, mid_y0, mid_x0, mid_h, mid_w;
+  const float2* u_fwd;  // ADJ: field after the sample (one padded plane)
+  float* g_dn;          // ADJ: gradients (h x w), added to when accum
+  float* g_ab;
+  int accum;
+};
+
+template <int N, int E, bool ADJ>
+__global__ void __launch_bounds__(N / E) ms_row_kernel(MsRowArgs a) {
+  constexpr int T = N / E;
+  static_assert(T % 64 == 0, "one row per workgroup of whole wavefronts");
+  __shared__ float2 lds[lds_span<N, 1>()];
+  const int p = blockIdx.x / a.Ny;
+  const int y = blockIdx.x - p * a.Ny;
+  const int t = threadIdx.x;
+  float2* row = a.ws + p * a.plane + (long long)y * N;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) v[k] = row[t + k * T];
+  fft<N, E, 1, 1, true>(v, lds, t, a.tw);
+  const int ay = y - a.pad_y;
+  if (ay >= 0 && ay < a.h) {
+    const long long rb = (long long)ay * a.w;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int x = t + k * T;
+      const int ax = x - a.pad_x;
+      if (ax >= 0 && ax < a.w) {
+        const float dn = a.dn[rb + ax];
+        const float ab = a.ab ? a.ab[rb + ax] : 0.0f;
+        float ang, m, sn, cs;
+        thin_angle(dn, ab, a.t, a.lam, &ang, &m);
+        sincos_exact(ang, &sn, &cs);
+        if constexpr (ADJ) {
+          const float2 u = a.u_fwd[(long long)y * N + x];
+          const float gdn = a.k * (v[k].y * u.x - v[k].x * u.y);
+          float* g0 = a.g_dn + rb + ax;
+          *g0 = a.accum ? *g0 + gdn : gdn;
+          if (a.g_ab) {
+            const float gab = -a.k * (v[k].x * u.x + v[k].y * u.y);
+            float* g1 = a.g_ab + rb + ax;
+            *g1 = a.accum ? *g1 + gab : gab;
+          }
+          v[k] = cmul(v[k], make_float2(m * cs, -m * sn));
+        } else {
+          v[k] = cmul(v[k], make_float2(m * cs, m * sn));
+        }
+      }
+    }
+  }
+  if constexpr (!ADJ) {
+    if (a.mid_out) {
+      const int yl = y - a.mid_y0;
+      if (yl >= 0 && yl < a.mid_h) {
+        float2* mo = a.mid_out + p * a.mid_pstride + (long long)yl * a.mid_rstride;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const int xl = t + k * T - a.mid_x0;
+          if (xl >= 0 && xl < a.mid_w) mo[xl] = v[k];
+        }
+      }
+    }
+  }
+  fft<N, E, -1, 1, true>(v, lds, t, a.tw);
+#pragma unroll
+  for (int k = 0; k < E; ++k) row[t + k * T] = v[k];
+}
+
+// Row lengths with a dedicated pass (values per thread; 1024 as the
+// pipeline's single-plane shape, so the two agree bit for bit)
+#define CXM_MS_ROW_SIZES(X) X(512, 8) X(1024, 4) X(2048, 8)
+
+// -1 when N has no dedicated pass
+int launch_ms_row(int N, bool adj, const MsRowArgs& a, int planes, hipStream_t s) {
+  const unsigned blocks = (unsigned)((long long)planes * a.Ny);
+  if (blocks == 0) return CXM_OK;
+  switch (N) {
+#define CXM_MSR(NN, EE)                                                                        \
+  case NN:                                                                                     \
+    if (adj)                                                                                   \
+      hipLaunchKernelGGL((ms_row_kernel<NN, EE, true>), dim3(blocks), dim3(NN / EE), 0, s, a); \
+    else                                                                                       \
+      hipLaunchKernelGGL((ms_row_kernel<NN, EE, false>), dim3(blocks), dim3(NN / EE), 0, s, a);\
+    CXM_LAUNCH_CHECK();                                                                        \
+    return CXM_OK;
+    CXM_MS_ROW_SIZES(CXM_MSR)
+#undef CXM_MSR
+    default:
+      return -1;
+  }
+}
+
+// The dedicated row passes and the H tables (CXM_OPT_MS_PASSES 0), or the
+// operator-pipeline row passes with H evaluated per bin (1).
+bool ms_dedicated() { return option(CXM_OPT_MS_PASSES) != 1; }
+
 // Angular-spectrum parameters for distance z on the padded grid
 // (compute_asm_propagator, src/chromatix/functional/propagation.py:499-580).
 int asm_hparam(const cxm_multislice_desc* d, const MsGeom& g, double z, bool per_slice,
@@ -85,16 +205,45 @@ int asm_hparam(const cxm_multislice_desc* d, const MsGeom& g, double z, bool per
   return build_hparams(&p, g.Ny, g.Nx, &hp, 1);
 }
 
+// H(z) of the per-slice propagation (the parameters asm_hparam gives) into
+// one padded plane in FFT order, for the table-reading column pass.
+int slice_table(const cxm_multislice_desc* d, const MsGeom& g, double z, float2* out, hipStream_t s) {
+  cxm_prop2d_desc p;
+  memset(&p, 0, sizeof(p));
+  p.batch = 1;
+  p.height = g.Ny;
+  p.width = g.Nx;
+  p.num_z = 1;
+  p.kind = CXM_H_ASM;
+  p.phase_mode = d->phase_mode;
+  p.remove_evanescent = d->remove_evanescent;
+  p.bandlimit = d->bandlimit;
+  p.num_wv = 1;
+  p.dy = d->dy;
+  p.dx = d->dx;
+  p.wavelength = d->wavelength;
+  p.n = d->n;
+  p.ky = d->ky;
+  p.kx = d->kx;
+  p.z = &z;
+  return cxm_propagator_c64((void*)s, out, &p);
+}
+
 // Column pass over `planes` padded planes in place, H for distance z; rows
 // [in_off, in_off + in_len) of the FFT grid are read from `in` rows
 // [0, in_len) (the rest are zero).
 int col_pass(const cxm_multislice_desc* d, const MsGeom& g, const float2* in, float2* out,
              int planes, double z, bool per_slice, bool na, int in_off, int in_len,
-             hipStream_t s) {
+             hipStream_t s, const float2* H = nullptr) {
   ColArgs ca;
   memset(&ca, 0, sizeof(ca));
   int rc = asm_hparam(d, g, z, per_slice, ca.hp[0]);
   if (rc) return rc;
+  if (H && !na) {
+    // H(z) read from a table of the same values (one padded plane, FFT order)
+    ca.kind = CXM_H_ARRAY;
+    ca.H = H;
+  }
   ca.nhp = 1;
   ca.in = in;
   ca.out = out;
@@ -110,7 +259,7 @@ int col_pass(const cxm_multislice_desc* d, const MsGeom& g, const float2* in, fl
   ca.in_len = in_len;
   ca.out_off = 0;
   ca.out_len = g.Ny;
-  ca.kind = CXM_H_ASM;
+  if (ca.kind != CXM_H_ARRAY) ca.kind = CXM_H_ASM;
   ca.phase_mode = d->phase_mode;
   ca.remove_evanescent = per_slice ? d->remove_evanescent : 0;
   ca.bandlimit = per_slice ? d->bandlimit : 0;
@@ -158,6 +307,32 @@ void slice_arrays(OpDesc* o, const cxm_multislice_desc* d, const MsGeom& g, cons
 int sample_pass(const cxm_multislice_desc* d, const MsGeom& g, float2* ws, int planes,
                 const float* dn, const float* ab, int i, float2* mid_out, long long mid_pstride,
                 int mid_rstride, int y0, int x0, int mh, int mw, hipStream_t s) {
+  if (ms_dedicated()) {
+    const long long sl = (long long)g.h * g.w;
+    MsRowArgs r;
+    memset(&r, 0, sizeof(r));
+    r.ws = ws;
+    r.tw = twiddles_for(g.Nx);
+    r.plane = g.plane;
+    r.Ny = g.Ny;
+    r.dn = dn + i * sl;
+    r.ab = ab ? ab + i * sl : nullptr;
+    r.h = g.h;
+    r.w = g.w;
+    r.pad_y = d->pad_y;
+    r.pad_x = d->pad_x;
+    r.t = (float)d->thickness;
+    r.lam = (float)d->wavelength;
+    r.mid_out = mid_out;
+    r.mid_pstride = mid_pstride;
+    r.mid_rstride = mid_rstride;
+    r.mid_y0 = y0;
+    r.mid_x0 = x0;
+    r.mid_h = mh;
+    r.mid_w = mw;
+    const int rc = launch_ms_row(g.Nx, false, r, planes, s);
+    if (rc >= 0) return rc;
+  }
   PipeArgs a;
   memset(&a, 0, sizeof(a));
   Ops b(a);
@@ -208,10 +383,11 @@ int copy_planes(const float2* src, float2* dst, long long n, hipStream_t s) {
 
 using namespace cxm;
 
+// workspace: the batch's padded planes | H(t) (one plane)
 extern "C" size_t cxm_multislice_workspace_size(const cxm_multislice_desc* d) {
   MsGeom g;
   if (ms_geom(d, g)) return 0;
-  return (size_t)std::max(1, d->batch) * g.plane * sizeof(float2);
+  return (size_t)(std::max(1, d->batch) + 1) * g.plane * sizeof(float2);
 }
 
 extern "C" size_t cxm_multislice_checkpoint_size(const cxm_multislice_desc* d) {
@@ -229,7 +405,7 @@ extern "C" int cxm_multislice_fwd(void* stream, const void* in, const float* dn,
   if (rc) return rc;
   if (d->batch == 0) return CXM_OK;
   if (!in || !dn || !out) return fail(CXM_ERR_INVALID, "null buffer");
-  const size_t need = (size_t)d->batch * g.plane * sizeof(float2);
+  const size_t need = cxm_multislice_workspace_size(d);
   if (!workspace || workspace_bytes < need)
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   if ((rc = ensure_init())) return rc;
@@ -237,12 +413,17 @@ extern "C" int cxm_multislice_fwd(void* stream, const void* in, const float* dn,
   const int B = d->batch;
   float2* ws = (float2*)workspace;
   const long long fld = (long long)g.h * g.w;
+  float2* Ht = nullptr;
+  if (ms_dedicated()) {
+    Ht = ws + (long long)B * g.plane;
+    if ((rc = slice_table(d, g, d->thickness, Ht, s))) return rc;
+  }
   if ((rc = input_pass(d, g, (const float2*)in, ws, B, s))) return rc;
   for (int i = 0; i < g.D; ++i) {
     // the first column pass reads only the input rows (the rest is padding)
     const bool first = i == 0;
     if ((rc = col_pass(d, g, first ? ws + (long long)d->pad_y * g.Nx : ws, ws, B, d->thickness,
-                       true, false, first ? d->pad_y : 0, first ? g.h : g.Ny, s)))
+                       true, false, first ? d->pad_y : 0, first ? g.h : g.Ny, s, Ht)))
       return rc;
     float2* stack = d->return_stack ? (float2*)out + (long long)i * B * fld : nullptr;
     if ((rc = sample_pass(d, g, ws, B, dn, absorption, i, stack, fld, g.w, d->pad_y, d->pad_x,
@@ -276,10 +457,65 @@ extern "C" int cxm_multislice_fwd(void* stream, const void* in, const float* dn,
 extern "C" size_t cxm_multislice_bwd_workspace_size(const cxm_multislice_desc* d) {
   MsGeom g;
   if (ms_geom(d, g)) return 0;
-  // forward state, adjoint state, one segment of fields, internal checkpoints
-  return (size_t)(2 + g.k + (g.nseg - 1)) * g.plane * sizeof(float2);
+  // forward state, adjoint state, two segments of fields (double-buffered),
+  // internal checkpoints, H(t) and H(-t)
+  return (size_t)(2 + 2 * g.k + (g.nseg - 1) + 2) * g.plane * sizeof(float2);
 }
 
+namespace cxm {
+namespace {
+// A second stream per device for the adjoint's segment recomputation (default
+// priority, non-blocking; created once and kept until the process ends).
+hipStream_t ms_aux_stream() {
+  static std::mutex mu;
+  static hipStream_t st[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!st[dev]) {
+    hipStream_t x;
+    if (hipStreamCreateWithFlags(&x, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    st[dev] = x;
+  }
+  return st[dev];
+}
+
+// Events of one call (timing disabled); destroyed at scope exit, which HIP
+// defers until the work they mark has completed.
+struct EventSet {
+  std::vector<hipEvent_t> ev;
+  int make(int n) {
+    ev.assign(n, nullptr);
+    for (auto& e : ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+        return fail(CXM_ERR_HIP, "hipEventCreateWithFlags failed");
+    return CXM_OK;
+  }
+  ~EventSet() {
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
+// `waiter` waits for the work enqueued on `src` so far (nothing when both are
+// the same stream).
+int ms_join(hipStream_t src, hipStream_t waiter, hipEvent_t e) {
+  if (src == waiter) return CXM_OK;
+  if (hipEventRecord(e, src) != hipSuccess || hipStreamWaitEvent(waiter, e, 0) != hipSuccess)
+    return fail(CXM_ERR_HIP, "event record / wait failed");
+  return CXM_OK;
+}
+}  // namespace
+}  // namespace cxm
+
+// The adjoint sweeps the segments of k slices from the last to the first.
+// A segment's forward fields are recomputed from its checkpoint (stream F)
+// into one of two segment buffers while the reverse sweep of the segment
+// after it (the caller's stream R) reads the other: the two single-plane
+// chains run side by side.  F recomputes segment sg once R has finished the
+// segment sg + 2, which used the same buffer; R sweeps segment sg once F has
+// recomputed it.  F's last work is waited for on R, so the caller's stream
+// orders everything.
 extern "C" int cxm_multislice_bwd(void* stream, const void* in, const float* dn,
                                   const float* absorption, const void* checkpoints,
                                   const void* cot, float* grad_dn, float* grad_absorption,
@@ -297,57 +533,84 @@ extern "C" int cxm_multislice_bwd(void* stream, const void* in, const float* dn,
   if (!workspace || workspace_bytes < need)
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   if ((rc = ensure_init())) return rc;
-  hipStream_t s = (hipStream_t)stream;
+  hipStream_t R = (hipStream_t)stream;
+  hipStream_t F = R;
+  if (g.nseg > 1) {
+    F = ms_aux_stream();
+    if (!F) F = R;
+  }
   const int B = d->batch;
   const long long fld = (long long)g.h * g.w;
-  float2* wf = (float2*)workspace;  // forward state
-  float2* wg = wf + g.plane;        // adjoint state
-  float2* seg = wg + g.plane;       // fields after each slice of a segment
-  float2* ick = seg + (long long)g.k * g.plane;  // internal checkpoints (one field)
+  const long long sl = fld;
+  float2* wf = (float2*)workspace;  // forward state (F)
+  float2* wg = wf + g.plane;        // adjoint state (R)
+  float2* segs = wg + g.plane;      // two buffers of k fields after each slice
+  float2* ick = segs + 2LL * g.k * g.plane;  // internal checkpoints (one field)
+  float2* Hf = ick + (long long)(g.nseg - 1) * g.plane;  // H(t), H(-t) tables
+  float2* Hb = Hf + g.plane;
   const float kk = (float)(2.0 * M_PI * d->thickness / d->wavelength);
+  const bool ded = ms_dedicated();
+  if (!ded) Hf = Hb = nullptr;
+  EventSet es;
+  // per field: start, one "recomputed" and one "swept" event per segment
+  const int ne = 1 + 2 * g.nseg;
+  if ((rc = es.make(B * ne))) return rc;
+  if (ded) {
+    if ((rc = slice_table(d, g, d->thickness, Hf, R))) return rc;
+    if ((rc = slice_table(d, g, -d->thickness, Hb, R))) return rc;
+  }
+  auto segbuf = [&](int sg) { return segs + (long long)(sg & 1) * g.k * g.plane; };
   for (int b = 0; b < B; ++b) {
+    hipEvent_t* E = es.ev.data() + (long long)b * ne;
+    hipEvent_t* e_rec = E + 1;             // F: segment sg recomputed
+    hipEvent_t* e_swept = E + 1 + g.nseg;  // R: segment sg swept
     const float2* inb = (const float2*)in + b * fld;
+    // F starts after everything enqueued on R so far (tables, previous field)
+    if ((rc = ms_join(R, F, E[0]))) return rc;
     // states before slices c k, c = 1 .. nseg-1, of field b
     auto state = [&](int c) -> const float2* {
       if (checkpoints) return (const float2*)checkpoints + ((long long)(c - 1) * B + b) * g.plane;
       return ick + (long long)(c - 1) * g.plane;
     };
     if (!checkpoints && g.nseg > 1) {
-      if ((rc = input_pass(d, g, inb, wf, 1, s))) return rc;
+      if ((rc = input_pass(d, g, inb, wf, 1, F))) return rc;
       for (int i = 0; i + 1 < (g.nseg - 1) * g.k + 1 && i < g.D; ++i) {
         const bool first = i == 0;
         if ((rc = col_pass(d, g, first ? wf + (long long)d->pad_y * g.Nx : wf, wf, 1,
-                           d->thickness, true, false, first ? d->pad_y : 0, first ? g.h : g.Ny, s)))
+                           d->thickness, true, false, first ? d->pad_y : 0, first ? g.h : g.Ny, F, Hf)))
           return rc;
-        if ((rc = sample_pass(d, g, wf, 1, dn, absorption, i, nullptr, 0, 0, 0, 0, 0, 0, s)))
+        if ((rc = sample_pass(d, g, wf, 1, dn, absorption, i, nullptr, 0, 0, 0, 0, 0, 0, F)))
           return rc;
         if ((i + 1) % g.k == 0 && i + 1 < g.D &&
-            (rc = copy_planes(wf, ick + (long long)((i + 1) / g.k - 1) * g.plane, g.plane, s)))
+            (rc = copy_planes(wf, ick + (long long)((i + 1) / g.k - 1) * g.plane, g.plane, F)))
           return rc;
       }
     }
     for (int sg = g.nseg - 1; sg >= 0; --sg) {
       const int i0 = sg * g.k, i1 = std::min(g.D, i0 + g.k);
-      // recompute the segment, keeping the field after each slice
+      float2* seg = segbuf(sg);
+      // ---- F: recompute the segment, keeping the field after each slice
+      if (sg + 2 <= g.nseg - 1 && F != R && hipStreamWaitEvent(F, e_swept[sg + 2], 0) != hipSuccess)
+        return fail(CXM_ERR_HIP, "hipStreamWaitEvent failed");
       if (sg == 0) {
-        if ((rc = input_pass(d, g, inb, wf, 1, s))) return rc;
-      } else if ((rc = copy_planes(state(sg), wf, g.plane, s))) {
+        if ((rc = input_pass(d, g, inb, wf, 1, F))) return rc;
+      } else if ((rc = copy_planes(state(sg), wf, g.plane, F))) {
         return rc;
       }
       for (int i = i0; i < i1; ++i) {
         const bool first = i == 0;
         if ((rc = col_pass(d, g, first ? wf + (long long)d->pad_y * g.Nx : wf, wf, 1,
-                           d->thickness, true, false, first ? d->pad_y : 0, first ? g.h : g.Ny, s)))
+                           d->thickness, true, false, first ? d->pad_y : 0, first ? g.h : g.Ny, F, Hf)))
           return rc;
         if ((rc = sample_pass(d, g, wf, 1, dn, absorption, i, seg + (long long)(i - i0) * g.plane,
-                              g.plane, g.Nx, 0, 0, g.Ny, g.Nx, s)))
+                              g.plane, g.Nx, 0, 0, g.Ny, g.Nx, F)))
           return rc;
       }
       if (sg == g.nseg - 1) {
         // output cotangent: back-propagate, inverse rows, cotangent of the
         // cropped output, forward rows -> adjoint state; then the adjoint of
         // the back-propagation
-        if ((rc = col_pass(d, g, wf, wf, 1, back_distance(d), false, d->use_na != 0, 0, g.Ny, s)))
+        if ((rc = col_pass(d, g, wf, wf, 1, back_distance(d), false, d->use_na != 0, 0, g.Ny, F)))
           return rc;
         PipeArgs a;
         memset(&a, 0, sizeof(a));
@@ -367,12 +630,41 @@ extern "C" int cxm_multislice_bwd(void* stream, const void* in, const float* dn,
         a.in = wf;
         a.out = wg;
         a.seq = SEQ_IF;
-        if ((rc = launch_pipe_row(g.Nx, a, s))) return rc;
-        if ((rc = col_pass(d, g, wg, wg, 1, -back_distance(d), false, d->use_na != 0, 0, g.Ny, s)))
+        if ((rc = launch_pipe_row(g.Nx, a, F))) return rc;
+        if ((rc = col_pass(d, g, wg, wg, 1, -back_distance(d), false, d->use_na != 0, 0, g.Ny, F)))
           return rc;
       }
-      // reverse through the segment
+      // ---- R: reverse through the segment once F has recomputed it
+      if ((rc = ms_join(F, R, e_rec[sg]))) return rc;
       for (int i = i1 - 1; i >= i0; --i) {
+        if (ded) {
+          MsRowArgs r;
+          memset(&r, 0, sizeof(r));
+          r.ws = wg;
+          r.tw = twiddles_for(g.Nx);
+          r.plane = g.plane;
+          r.Ny = g.Ny;
+          r.dn = dn + i * sl;
+          r.ab = absorption ? absorption + i * sl : nullptr;
+          r.h = g.h;
+          r.w = g.w;
+          r.pad_y = d->pad_y;
+          r.pad_x = d->pad_x;
+          r.t = (float)d->thickness;
+          r.lam = (float)d->wavelength;
+          r.k = kk;
+          r.u_fwd = seg + (long long)(i - i0) * g.plane;
+          r.g_dn = grad_dn + (long long)i * fld;
+          r.g_ab = grad_absorption ? grad_absorption + (long long)i * fld : nullptr;
+          r.accum = b > 0;
+          rc = launch_ms_row(g.Nx, true, r, 1, R);
+          if (rc > 0) return rc;
+          if (rc == 0) {
+            if (i > 0 && (rc = col_pass(d, g, wg, wg, 1, -d->thickness, true, false, 0, g.Ny, R, Hb)))
+              return rc;
+            continue;
+          }
+        }
         PipeArgs a;
         memset(&a, 0, sizeof(a));
         Ops ob(a);
@@ -390,10 +682,12 @@ extern "C" int cxm_multislice_bwd(void* stream, const void* in, const float* dn,
         a.in = wg;
         a.out = wg;
         a.seq = SEQ_IF;
-        if ((rc = launch_pipe_row(g.Nx, a, s))) return rc;
-        if (i > 0 && (rc = col_pass(d, g, wg, wg, 1, -d->thickness, true, false, 0, g.Ny, s)))
+        if ((rc = launch_pipe_row(g.Nx, a, R))) return rc;
+        if (i > 0 && (rc = col_pass(d, g, wg, wg, 1, -d->thickness, true, false, 0, g.Ny, R, Hb)))
           return rc;
       }
+      if (F != R && hipEventRecord(e_swept[sg], R) != hipSuccess)
+        return fail(CXM_ERR_HIP, "hipEventRecord failed");
     }
   }
   return CXM_OK;

@@ -96,6 +96,15 @@ CXM_DEV float bin_freq(int k, int n, float inv_d) {
   return __fmul_rn(inv_d, __fdiv_rn((float)signed_bin(k, n), (float)n));
 }
 
+// The thin-sample factor exp(i 2 pi (dn + i ab) t / lam) = m (cos ang + i sin ang)
+// of thin_sample (samples.py:126-133), in the reference's float32 order;
+// ang and m only (the caller evaluates the sine and cosine).
+CXM_DEV void thin_angle(float dn, float ab, float t, float lam, float* ang, float* m) {
+  const float re = __fdiv_rn(__fmul_rn(-__fmul_rn(kTwoPiF, ab), t), lam);
+  *ang = __fdiv_rn(__fmul_rn(__fmul_rn(kTwoPiF, dn), t), lam);
+  *m = (re == 0.0f) ? 1.0f : expf(re);
+}
+
 CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, int iy, int ix,
                         int ny, int nx) {
   const int q = (o.pmod > 1) ? (plane / o.pdiv) % o.pmod : 0;
@@ -153,10 +162,7 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
         // thin sample (samples.py:126-133): exp(i 2 pi (dn + i abs) t / lam)
         const float dn = ((const float*)o.arr0)[idx];
         const float ab = o.arr1 ? ((const float*)o.arr1)[idx] : 0.0f;
-        const float lam = P[0].x;
-        const float re = __fdiv_rn(__fmul_rn(-__fmul_rn(kTwoPiF, ab), o.t), lam);
-        ang = __fdiv_rn(__fmul_rn(__fmul_rn(kTwoPiF, dn), o.t), lam);
-        m = (re == 0.0f) ? 1.0f : expf(re);
+        thin_angle(dn, ab, o.t, P[0].x, &ang, &m);
       } else {  // fluorescence f exp(i r), added
         f = ((const float*)o.arr0)[idx];
         ang = ((const float*)o.arr1)[idx];
@@ -237,12 +243,10 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
       const long long idx = (long long)ay * o.arr_row_stride + ax;
       const float dn = ((const float*)o.arr0)[idx];
       const float ab = o.arr1 ? ((const float*)o.arr1)[idx] : 0.0f;
-      const float lam = P[0].x, k = P[0].y;
-      const float re = __fdiv_rn(__fmul_rn(-__fmul_rn(kTwoPiF, ab), o.t), lam);
-      const float im = __fdiv_rn(__fmul_rn(__fmul_rn(kTwoPiF, dn), o.t), lam);
-      float s, c;
+      const float k = P[0].y;
+      float im, m, s, c;
+      thin_angle(dn, ab, o.t, P[0].x, &im, &m);
       sincos_exact(im, &s, &c);
-      const float m = (re == 0.0f) ? 1.0f : expf(re);
       const float2 u = ((const float2*)o.arr2)[(long long)plane * o.arr2_plane_stride +
                                                 (long long)iy * o.arr2_row_stride + ix];
       const float gdn = k * (v.y * u.x - v.x * u.y);

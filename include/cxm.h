@@ -844,6 +844,13 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         sums, where the z length has the fused passes;
  *                         1 = separate V, z-pass and update launches.  Same
  *                         iterates; the residual sum's order differs.
+ *   CXM_OPT_MS_PASSES     multislice (cxm_multislice_fwd / _bwd): 0 = the
+ *                         per-slice column passes read H(+t) / H(-t) from
+ *                         tables built once per call, and the slice's row
+ *                         pass (thin sample or its adjoint) is a dedicated
+ *                         kernel where the row length has one (512, 1024,
+ *                         2048); 1 = the operator-pipeline row passes with H
+ *                         evaluated per bin.  Equal to float32 rounding.
  *   CXM_OPT_GRAD_GROUPS   cxm_psf4f_bwd: 0 = the default number of plane
  *                         groups of the phase-gradient pass (4), 1..12 =
  *                         that many (each group sums its planes in
@@ -866,6 +873,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_COL_PERSIST_MODE 14
 #define CXM_OPT_GRAD_GROUPS 15
 #define CXM_OPT_MBS_ZFUSE 16
+#define CXM_OPT_MS_PASSES 17
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

@@ -251,3 +251,40 @@ def test_fluorescent_multislice_seeded_monte_carlo():
     assert torch.equal(a, b)
     assert not torch.equal(a, c)
     assert bool((a >= 0).all()) and bool(torch.isfinite(a).all())
+
+
+@pytest.mark.parametrize("pm", ["ref32", "accurate"])
+@pytest.mark.parametrize("shape,pad", [((480, 448), (16, 32)), ((1024, 1024), 0), ((2000, 2016), (24, 16))])
+def test_dedicated_passes_match_pipeline_passes(pm, shape, pad):
+    """The dedicated slice row passes and the H tables (CXM_OPT_MS_PASSES 0)
+    against the operator-pipeline passes with H per bin (1): forward field
+    and the dn / absorption gradients, with absorption, kykx and an NA mask
+    on the final back-propagation (padded grids 512^2, 1024^2, 2048^2)."""
+    from chromatix_amd import _lib as L
+
+    rng = np.random.default_rng(21)
+    h, w = shape
+    D = 6
+    u = (np.exp(2j * np.pi * rng.random((h, w)))).astype(np.complex64)
+    dn = (0.02 * rng.standard_normal((D, h, w))).astype(np.float32)
+    ab = (0.002 * rng.random((D, h, w))).astype(np.float32)
+    c = (rng.standard_normal((h, w)) + 1j * rng.standard_normal((h, w))).astype(np.complex64)
+    outs, gdns, gabs = [], [], []
+    try:
+        for opt in (0, 1):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_MS_PASSES, opt))
+            dn_t = torch.as_tensor(dn).cuda().requires_grad_(True)
+            ab_t = torch.as_tensor(ab).cuda().requires_grad_(True)
+            f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, LAM)
+            out = cf.multislice_thick_sample(f, ab_t, dn_t, NM, T, pad, NA=0.8, kykx=(0.05, -0.1),
+                                             phase_mode=pm, checkpoint_every=4)
+            loss = torch.sum(torch.real(torch.conj(torch.as_tensor(c).cuda()) * out.u))
+            loss.backward()
+            outs.append(out.u.detach().cpu().numpy())
+            gdns.append(dn_t.grad.cpu().numpy())
+            gabs.append(ab_t.grad.cpu().numpy())
+    finally:
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_MS_PASSES, 0))
+    e = (rel_l2(outs[0], outs[1]), rel_l2(gdns[0], gdns[1]), rel_l2(gabs[0], gabs[1]))
+    print(f"dedicated vs pipeline {pm} {shape} pad {pad}: fwd {e[0]:.2e} dn {e[1]:.2e} abs {e[2]:.2e}")
+    assert max(e) < 1e-6
