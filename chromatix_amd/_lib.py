@@ -49,6 +49,7 @@ CXM_OPT_COL_PERSIST_MODE = 14
 CXM_OPT_GRAD_GROUPS = 15
 CXM_OPT_MBS_ZFUSE = 16
 CXM_OPT_MS_PASSES = 17
+CXM_OPT_MS_ROW_E = 18
 CXM_H_TABLE_MIN_PLANES = 4
 
 PHASE_MODES = {"ref32": CXM_PHASE_REF32, "accurate": CXM_PHASE_ACCURATE}

@@ -156,10 +156,29 @@ __global__ void __launch_bounds__(N / E) ms_row_kernel(MsRowArgs a) {
 // pipeline's single-plane shape, so the two agree bit for bit)
 #define CXM_MS_ROW_SIZES(X) X(512, 8) X(1024, 4) X(2048, 8)
 
+template <int N, int E>
+int launch_ms_row_t(bool adj, const MsRowArgs& a, unsigned blocks, hipStream_t s) {
+  if (adj)
+    hipLaunchKernelGGL((ms_row_kernel<N, E, true>), dim3(blocks), dim3(N / E), 0, s, a);
+  else
+    hipLaunchKernelGGL((ms_row_kernel<N, E, false>), dim3(blocks), dim3(N / E), 0, s, a);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
 // -1 when N has no dedicated pass
 int launch_ms_row(int N, bool adj, const MsRowArgs& a, int planes, hipStream_t s) {
   const unsigned blocks = (unsigned)((long long)planes * a.Ny);
   if (blocks == 0) return CXM_OK;
+  if (N == 1024) {
+    // values per thread of the 1024-point rows (CXM_OPT_MS_ROW_E)
+    switch (option(CXM_OPT_MS_ROW_E)) {
+      case 2: return launch_ms_row_t<1024, 2>(adj, a, blocks, s);
+      case 8: return launch_ms_row_t<1024, 8>(adj, a, blocks, s);
+      case 16: return launch_ms_row_t<1024, 16>(adj, a, blocks, s);
+      default: break;
+    }
+  }
   switch (N) {
 #define CXM_MSR(NN, EE)                                                                        \
   case NN:                                                                                     \
@@ -177,8 +196,10 @@ int launch_ms_row(int N, bool adj, const MsRowArgs& a, int planes, hipStream_t s
 }
 
 // The dedicated row passes and the H tables (CXM_OPT_MS_PASSES 0), or the
-// operator-pipeline row passes with H evaluated per bin (1).
+// operator-pipeline row passes with H evaluated per bin (1); 2: the
+// dedicated row passes with H evaluated per bin.
 bool ms_dedicated() { return option(CXM_OPT_MS_PASSES) != 1; }
+bool ms_tables() { return option(CXM_OPT_MS_PASSES) == 0; }
 
 // Angular-spectrum parameters for distance z on the padded grid
 // (compute_asm_propagator, src/chromatix/functional/propagation.py:499-580).
@@ -414,7 +435,7 @@ extern "C" int cxm_multislice_fwd(void* stream, const void* in, const float* dn,
   float2* ws = (float2*)workspace;
   const long long fld = (long long)g.h * g.w;
   float2* Ht = nullptr;
-  if (ms_dedicated()) {
+  if (ms_tables()) {
     Ht = ws + (long long)B * g.plane;
     if ((rc = slice_table(d, g, d->thickness, Ht, s))) return rc;
   }
@@ -550,12 +571,12 @@ extern "C" int cxm_multislice_bwd(void* stream, const void* in, const float* dn,
   float2* Hb = Hf + g.plane;
   const float kk = (float)(2.0 * M_PI * d->thickness / d->wavelength);
   const bool ded = ms_dedicated();
-  if (!ded) Hf = Hb = nullptr;
+  if (!ms_tables()) Hf = Hb = nullptr;
   EventSet es;
   // per field: start, one "recomputed" and one "swept" event per segment
   const int ne = 1 + 2 * g.nseg;
   if ((rc = es.make(B * ne))) return rc;
-  if (ded) {
+  if (Hf) {
     if ((rc = slice_table(d, g, d->thickness, Hf, R))) return rc;
     if ((rc = slice_table(d, g, -d->thickness, Hb, R))) return rc;
   }

@@ -850,7 +850,10 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         pass (thin sample or its adjoint) is a dedicated
  *                         kernel where the row length has one (512, 1024,
  *                         2048); 1 = the operator-pipeline row passes with H
- *                         evaluated per bin.  Equal to float32 rounding.
+ *                         evaluated per bin; 2 = the dedicated row passes with
+ *                         H evaluated per bin.  Equal to float32 rounding.
+ *   CXM_OPT_MS_ROW_E      values per thread of the dedicated 1024-point slice
+ *                         row pass: 0 = 4, or 2, 8, 16
  *   CXM_OPT_GRAD_GROUPS   cxm_psf4f_bwd: 0 = the default number of plane
  *                         groups of the phase-gradient pass (4), 1..12 =
  *                         that many (each group sums its planes in
@@ -874,6 +877,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_GRAD_GROUPS 15
 #define CXM_OPT_MBS_ZFUSE 16
 #define CXM_OPT_MS_PASSES 17
+#define CXM_OPT_MS_ROW_E 18
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 
