@@ -301,7 +301,8 @@ def bench_c1(args):
 def bench_c2(args):
     """4f PSF stack (1280^2, 11 planes, z in linspace(-5, 5) um) forward and
     the phase-mask gradient of the MSE against a target stack; one step =
-    forward + gradient."""
+    forward (the intensity stack) + gradient, computed together
+    (psf_4f_mse_grad(return_intensity=True))."""
     import torch
 
     import chromatix_amd.functional as cf
@@ -317,8 +318,9 @@ def bench_c2(args):
     target = (I0 * torch.as_tensor(rng2.uniform(0.5, 1.5, (Z, N, N)).astype(np.float32)).cuda())
 
     def step():
-        cf.psf_4f(phase, dx, lam, z, f, nimm, NA, ft)
-        cf.psf_4f_mse_grad(phase, target, dx, lam, z, f, nimm, NA, ft)
+        # the loss's forward (the intensity stack) and its gradient in one
+        # sweep, as jax.value_and_grad computes them
+        cf.psf_4f_mse_grad(phase, target, dx, lam, z, f, nimm, NA, ft, return_intensity=True)
 
     steps, warm = max(10, args.steps // 4), args.warmup
     ms = _timed(torch, step, steps, warm)

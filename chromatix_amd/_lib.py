@@ -50,6 +50,7 @@ CXM_OPT_GRAD_GROUPS = 15
 CXM_OPT_MBS_ZFUSE = 16
 CXM_OPT_MS_PASSES = 17
 CXM_OPT_MS_ROW_E = 18
+CXM_OPT_PSF_PASSES = 19
 CXM_H_TABLE_MIN_PLANES = 4
 
 PHASE_MODES = {"ref32": CXM_PHASE_REF32, "accurate": CXM_PHASE_ACCURATE}
@@ -341,6 +342,7 @@ def _declare(lib):
         "cxm_psf4f_workspace_size": (sz, [ctypes.POINTER(Psf4fDesc)]),
         "cxm_psf4f_fwd": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(Psf4fDesc)]),
         "cxm_psf4f_bwd": (i32, [vp, vp, vp, vp, vp, sz, ctypes.POINTER(Psf4fDesc)]),
+        "cxm_psf4f_value_and_grad": (i32, [vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(Psf4fDesc)]),
         "cxm_point_source_c64": (i32, [vp, vp, ctypes.POINTER(Psf4fDesc)]),
         "cxm_phase_change_c64": (i32, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                        ctypes.c_int32, dp]),

@@ -49,6 +49,11 @@ struct PipeArgs {
   float2* mid_out;
   long long mid_plane_stride;
   int mid_row_stride, mid_y0, mid_x0, mid_h, mid_w;
+  // optional store of |x|^2 of the value entering the `mid` operators
+  // (column pass only), at the fftshifted grid position:
+  // mid_int[plane * mid_int_plane_stride + fshift(y) * grid_nx + fshift(x)]
+  float* mid_int;
+  long long mid_int_plane_stride;
   float scale, mid_scale;
   // optional: the pass does nothing once *stop != 0 (a solver's device-side
   // stop flag, so that iterations enqueued past convergence cost no work)
@@ -128,6 +133,10 @@ CXM_DEV void run_seq(const PipeArgs& a, float2 (&v)[E], float2* lds, int t, int 
         const int j = t + k * T;
         float2* slot = lds + lds_at<N, S>(j);
         float2 x = cscale(*slot, a.mid_scale);
+        if (a.mid_int && !is_row)
+          a.mid_int[(long long)(plane - a.plane0) * a.mid_int_plane_stride +
+                    (long long)fshift_dst(j, a.grid_ny) * a.grid_nx + fshift_dst(line_grid, a.grid_nx)] =
+              cabs2(x);
         if (a.mid.n)
           x = is_row ? apply_ops(a.mid, a.tab, x, plane, line_grid, j, a.grid_ny, a.grid_nx)
                      : apply_ops(a.mid, a.tab, x, plane, j, line_grid, a.grid_ny, a.grid_nx);

@@ -202,6 +202,231 @@ int psf_geom(const cxm_psf4f_desc* d) {
   return CXM_OK;
 }
 
+// ---- dedicated passes over tables -------------------------------------------
+// The generated source of plane j at grid position (y, x) is
+//   src_j(y, x) = P(y, x) c_j(y, x),
+// P the plane-invariant factor (pupil, amplitude / power scale, tube-lens
+// norm, phase mask exp(i phase)) and c_j the defocus chirp
+// exp(i a_j |g - o|^2 / L^2).  P is evaluated once per call into a table by
+// the pipeline's own operators; the chirp is a product of a row factor
+// cy_j(y) = exp(i a_j (gy - oy)^2 / L^2) and a column factor cx_j(x) (equal in
+// real arithmetic; in float32 each factor's phase is rounded on its own, as
+// the per-element phase is).  A pass then costs two complex products per
+// value instead of a pupil test, two sine/cosine pairs and the operator
+// stage.
+struct PsfTables {
+  float2* P;   // height x width
+  float2* cy;  // num_z x height
+  float2* cx;  // num_z x width
+};
+
+// One axis of the CHIRP operator (cxm_ops.h OP_CHIRP, parameters of plane
+// slot q): exp(i a (g - o)^2 / b) (ref32) or exp(2 pi i frac((a_hi + a_lo)
+// (g - o)^2)) (double-float turns), g the axis' grid coordinate.
+CXM_DEV float2 chirp_axis(const OpDesc& o, const OpTable& T, int q, int i, int n, int axis) {
+  const float4* P = &T.p[o.pbase + q * o.pstride];
+  const float4 p0 = P[0], p1 = P[1], p2 = P[2];
+  const float d = axis == 0 ? p1.x : p1.y, org = axis == 0 ? p1.z : p1.w, off = axis == 0 ? p2.x : p2.y;
+  float s, c;
+  if (o.mode == CXM_PHASE_REF32) {
+    const float g = __fsub_rn(grid_coord(i, n, d, org), off);
+    float ph = __fmul_rn(p0.x, __fmul_rn(g, g));
+    if (p0.y != 0.0f) ph = __fdiv_rn(ph, p0.y);
+    sincos_exact(ph, &s, &c);
+  } else {
+    const df g = df_sub(df_add(df_mul_f(df_make(d, 0.f), (float)i - 0.5f * (float)n), df_make(org, 0.f)),
+                        df_make(off, 0.f));
+    const float tr = df_frac(df_mul(df_make(p0.x, p0.z), df_mul(g, g)));
+    sincospif(2.0f * tr, &s, &c);
+  }
+  return make_float2(c, s);
+}
+
+// cy[j][y] (i < ny) and cx[j][x] (i >= ny) for plane slot j = blockIdx.y
+__global__ void psf_chirp_kernel(float2* __restrict__ cy, float2* __restrict__ cx, int ny, int nx,
+                                 OpDesc ch, OpTable tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i < ny) cy[(long long)j * ny + i] = chirp_axis(ch, tab, j, i, ny, 0);
+  else if (i < ny + nx) cx[(long long)j * nx + i - ny] = chirp_axis(ch, tab, j, i - ny, nx, 1);
+}
+
+// P(y, x) = the plane-invariant operators applied to 1 (grid order)
+__global__ void psf_p_kernel(float2* __restrict__ P, int ny, int nx, OpList L, OpTable T) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)ny * nx) return;
+  const int iy = (int)(i / nx), ix = (int)(i - (long long)iy * nx);
+  P[i] = apply_ops(L, T, make_float2(1.f, 0.f), 0, iy, ix, ny, nx);
+}
+
+struct PsfRowArgs {
+  PsfTables t;
+  const float2* tw;
+  float2* out;  // chunk planes (row-transformed)
+  long long plane_stride;
+  int ny, plane0;
+};
+
+// Row pass: src_j at (r, ishift(i)) for the row's positions i, row FFT.  One
+// row per workgroup of N / E threads.
+template <int N, int E>
+__global__ void __launch_bounds__(N / E) psf_row_kernel(PsfRowArgs a) {
+  constexpr int T = N / E;
+  constexpr bool SYNC = T > 64;
+  __shared__ float2 lds[lds_span<N, 1>()];
+  const int p = blockIdx.x / a.ny;
+  const int r = blockIdx.x - p * a.ny;
+  const int pg = a.plane0 + p;
+  const int t = threadIdx.x;
+  const float2 fy = a.t.cy[(long long)pg * a.ny + r];
+  const float2* Prow = a.t.P + (long long)r * N;
+  const float2* cxp = a.t.cx + (long long)pg * N;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int xf = ishift_src(t + k * T, N);
+    v[k] = cmul(cmul(Prow[xf], cxp[xf]), fy);
+  }
+  fft<N, E, -1, 1, SYNC>(v, lds, t, a.tw);
+  float2* dst = a.out + p * a.plane_stride + (long long)r * N;
+#pragma unroll
+  for (int k = 0; k < E; ++k) dst[t + k * T] = v[k];
+}
+
+struct PsfGradArgs {
+  PsfTables t;
+  const float2* tw;
+  const float2* in;  // chunk planes after the column adjoint
+  float2* partial;   // groups x height x width
+  long long plane_stride;
+  int ny, plane0, n_planes;
+};
+
+// Phase-gradient row pass: for row r and the planes of group blockIdx.y,
+// g_p = IFFT_row(in[p, r, :]) (the adjoint of the forward row FFT) and
+//   partial[group][r][x] = sum_p g_p(x) conj(c_p(r, x)),  x = ishift(i);
+// the plane-invariant factor is applied once per element in the group sum.
+template <int N, int E>
+__global__ void __launch_bounds__(N / E) psf_grad_kernel(PsfGradArgs a) {
+  constexpr int T = N / E;
+  constexpr bool SYNC = T > 64;
+  __shared__ float2 lds[lds_span<N, 1>()];
+  const int r = blockIdx.x;
+  const int grp = blockIdx.y, groups = gridDim.y;
+  const int pa = (int)((long long)grp * a.n_planes / groups);
+  const int pb = (int)((long long)(grp + 1) * a.n_planes / groups);
+  const int t = threadIdx.x;
+  float2 acc[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) acc[k] = make_float2(0.f, 0.f);
+  for (int p = pa; p < pb; ++p) {
+    // the thread index laundered per plane: the transform's twiddle loads
+    // are then not hoisted out of the loop (hoisted, every stage's twiddles
+    // stay live next to the plane sum: 256 VGPRs, one wave per SIMD)
+    int tl = t;
+    asm volatile("" : "+v"(tl));
+    const int pg = a.plane0 + p;
+    const float2* src = a.in + p * a.plane_stride + (long long)r * N;
+    float2 v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) v[k] = src[tl + k * T];
+    fft<N, E, 1, 1, SYNC>(v, lds, tl, a.tw);
+    const float2 fy = a.t.cy[(long long)pg * a.ny + r];
+    const float2* cxp = a.t.cx + (long long)pg * N;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const float2 c = cmul(cxp[ishift_src(tl + k * T, N)], fy);
+      acc[k] = cadd(acc[k], cmulc(v[k], c));
+    }
+  }
+  float2* dst = a.partial + (long long)grp * a.ny * N + (long long)r * N;
+#pragma unroll
+  for (int k = 0; k < E; ++k) dst[ishift_src(t + k * T, N)] = acc[k];
+}
+
+// grad[i] (+)= Im(conj(P[i]) sum_g partial[g][i]), g in order
+__global__ void psf_sum_kernel(const float2* __restrict__ partial, const float2* __restrict__ P,
+                               float* __restrict__ grad, long long n, int groups, int accum) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float2 c = make_float2(0.f, 0.f);
+  for (int g = 0; g < groups; ++g) c = cadd(c, partial[g * n + i]);
+  const float2 w = P[i];
+  const float v = w.x * c.y - w.y * c.x;
+  grad[i] = accum ? grad[i] + v : v;
+}
+
+// (row length, values per thread) of the dedicated passes: whole wavefronts
+// per row
+#define CXM_PSF_SIZES(X) X(512, 8) X(1024, 16) X(1280, 20) X(1536, 24) X(2048, 16)
+
+bool psf_dedicated_size(int n) {
+  switch (n) {
+#define CXM_PS_CASE(N, E) case N:
+    CXM_PSF_SIZES(CXM_PS_CASE)
+#undef CXM_PS_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+int launch_psf_row(int n, const PsfRowArgs& a, int planes, hipStream_t s) {
+  const unsigned blocks = (unsigned)((long long)planes * a.ny);
+  switch (n) {
+#define CXM_PS_ROW(N, E)                                                                      \
+  case N:                                                                                     \
+    hipLaunchKernelGGL((psf_row_kernel<N, E>), dim3(blocks), dim3(N / E), 0, s, a);           \
+    CXM_LAUNCH_CHECK();                                                                       \
+    return CXM_OK;
+    CXM_PSF_SIZES(CXM_PS_ROW)
+#undef CXM_PS_ROW
+    default:
+      return fail(CXM_ERR_UNSUPPORTED, "no dedicated PSF row pass for " + std::to_string(n));
+  }
+}
+
+int launch_psf_grad(int n, const PsfGradArgs& a, int groups, hipStream_t s) {
+  switch (n) {
+#define CXM_PS_GRAD(N, E)                                                                            \
+  case N:                                                                                            \
+    hipLaunchKernelGGL((psf_grad_kernel<N, E>), dim3((unsigned)a.ny, (unsigned)groups), dim3(N / E), \
+                       0, s, a);                                                                     \
+    CXM_LAUNCH_CHECK();                                                                              \
+    return CXM_OK;
+    CXM_PSF_SIZES(CXM_PS_GRAD)
+#undef CXM_PS_GRAD
+    default:
+      return fail(CXM_ERR_UNSUPPORTED, "no dedicated PSF gradient pass for " + std::to_string(n));
+  }
+}
+
+// The dedicated passes where the row length has them (CXM_OPT_PSF_PASSES 0)
+bool psf_dedicated(const cxm_psf4f_desc* d) {
+  return option(CXM_OPT_PSF_PASSES) != 1 && psf_dedicated_size(d->width);
+}
+
+// Fill P and the chirp factors for the call: the operators of psf_ops split
+// into the chirp (its parameter slots) and the rest (applied to 1).
+int psf_tables(const PipeArgs& ra, const cxm_psf4f_desc* d, const PsfTables& t, hipStream_t s) {
+  OpList inv;
+  memset(&inv, 0, sizeof(inv));
+  const OpDesc* ch = nullptr;
+  for (int i = 0; i < ra.pre.n; ++i) {
+    if (ra.pre.op[i].kind == OP_CHIRP) ch = &ra.pre.op[i];
+    else inv.op[inv.n++] = ra.pre.op[i];
+  }
+  if (!ch) return fail(CXM_ERR_INVALID, "no chirp operator");
+  const long long n = (long long)d->height * d->width;
+  hipLaunchKernelGGL(psf_p_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t.P, d->height,
+                     d->width, inv, ra.tab);
+  CXM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(psf_chirp_kernel, dim3((unsigned)((d->height + d->width + 255) / 256), (unsigned)d->num_z),
+                     dim3(256), 0, s, t.cy, t.cx, d->height, d->width, *ch, ra.tab);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
 }  // namespace
 }  // namespace cxm
 
@@ -218,9 +443,21 @@ constexpr int kGradGroupsDefault = 4;
 
 namespace cxm {
 namespace {
+// workspace: one chunk of planes | kGradGroups partial planes | the tables
+// of the dedicated passes (P: one plane; cy, cx: num_z rows and columns)
 size_t psf_ws_bytes(const cxm_psf4f_desc* d) {
   const size_t plane = (size_t)d->height * d->width;
-  return (size_t)psf_chunk(d) * plane * sizeof(float2) + kGradGroups * plane * sizeof(float2);
+  return ((size_t)psf_chunk(d) * plane + kGradGroups * plane + plane +
+          (size_t)d->num_z * (d->height + d->width)) * sizeof(float2);
+}
+
+PsfTables psf_table_ptrs(const cxm_psf4f_desc* d, void* workspace) {
+  const long long plane = (long long)d->height * d->width;
+  PsfTables t;
+  t.P = (float2*)workspace + (psf_chunk(d) + kGradGroups) * plane;
+  t.cy = t.P + plane;
+  t.cx = t.cy + (long long)d->num_z * d->height;
+  return t;
 }
 
 // grad[i] (+)= Im(conj(w) sum_g partial[g][i]), g in order, w = the
@@ -258,7 +495,7 @@ extern "C" int cxm_psf4f_fwd(void* stream, const float* phase, void* out, void* 
     return fail(CXM_ERR_INVALID, "invalid output kind");
   if (!phase || !out) return fail(CXM_ERR_INVALID, "null buffer");
   const long long chunk = psf_chunk(d);
-  const size_t need = (size_t)chunk * d->height * d->width * sizeof(float2);
+  const size_t need = psf_ws_bytes(d);
   if (!workspace || workspace_bytes < need)
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   if ((rc = ensure_init())) return rc;
@@ -269,6 +506,17 @@ extern "C" int cxm_psf4f_fwd(void* stream, const float* phase, void* out, void* 
   Ops rb(ra);
   if ((rc = psf_ops(rb, ra, d, phase))) return rc;
   row_args(ra, d);
+  const bool ded = psf_dedicated(d);
+  PsfRowArgs pr;
+  memset(&pr, 0, sizeof(pr));
+  if (ded) {
+    pr.t = psf_table_ptrs(d, workspace);
+    if ((rc = psf_tables(ra, d, pr.t, s))) return rc;
+    pr.tw = twiddles_for(d->width);
+    pr.out = (float2*)workspace;
+    pr.plane_stride = (long long)d->height * d->width;
+    pr.ny = d->height;
+  }
   col_args(ca, d);
   ca.out_line_shift = 1;
   ca.intensity = d->out_kind == CXM_OUT_INTENSITY;
@@ -278,7 +526,12 @@ extern "C" int cxm_psf4f_fwd(void* stream, const float* phase, void* out, void* 
     ra.out = workspace;
     ra.n_planes = kc;
     ra.plane0 = (int)p0;
-    if ((rc = launch_pipe_row(d->width, ra, s))) return rc;
+    if (ded) {
+      pr.plane0 = (int)p0;
+      if ((rc = launch_psf_row(d->width, pr, kc, s))) return rc;
+    } else if ((rc = launch_pipe_row(d->width, ra, s))) {
+      return rc;
+    }
     ca.in = (const float2*)workspace;
     ca.out = (char*)out + (size_t)p0 * ca.out_plane_stride * esz;
     ca.n_planes = kc;
@@ -288,8 +541,12 @@ extern "C" int cxm_psf4f_fwd(void* stream, const float* phase, void* out, void* 
   return CXM_OK;
 }
 
-extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot, float* grad_phase,
-                             void* workspace, size_t workspace_bytes, const cxm_psf4f_desc* d) {
+namespace cxm {
+namespace {
+// The adjoint passes; with `intensity`, the column pass also stores the
+// forward intensity stack it forms on the way (|u|^2 entering the cotangent).
+int psf_bwd(void* stream, const float* phase, const float* cot, float* grad_phase, float* intensity,
+            void* workspace, size_t workspace_bytes, const cxm_psf4f_desc* d) {
   int rc = psf_geom(d);
   if (rc) return rc;
   if (d->grad_kind != CXM_GRAD_UPSTREAM && d->grad_kind != CXM_GRAD_MSE)
@@ -339,22 +596,60 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
   ga.out = partial;
   ga.out_plane_stride = plane_px;
   ga.out_line_stride = d->width;
+  const bool ded = psf_dedicated(d);
+  PsfRowArgs pr;
+  PsfGradArgs pg;
+  memset(&pr, 0, sizeof(pr));
+  memset(&pg, 0, sizeof(pg));
+  if (ded) {
+    pr.t = psf_table_ptrs(d, workspace);
+    if ((rc = psf_tables(ra, d, pr.t, s))) return rc;
+    pr.tw = twiddles_for(d->width);
+    pr.out = (float2*)workspace;
+    pr.plane_stride = plane_px;
+    pr.ny = d->height;
+    pg.t = pr.t;
+    pg.tw = pr.tw;
+    pg.in = (const float2*)workspace;
+    pg.partial = partial;
+    pg.plane_stride = plane_px;
+    pg.ny = d->height;
+  }
   for (long long p0 = 0; p0 < d->num_z; p0 += chunk) {
     const int kc = (int)std::min<long long>(chunk, d->num_z - p0);
     ra.out = workspace;
     ra.n_planes = kc;
     ra.plane0 = (int)p0;
-    if ((rc = launch_pipe_row(d->width, ra, s))) return rc;
+    if (ded) {
+      pr.plane0 = (int)p0;
+      if ((rc = launch_psf_row(d->width, pr, kc, s))) return rc;
+    } else if ((rc = launch_pipe_row(d->width, ra, s))) {
+      return rc;
+    }
     ca.in = (const float2*)workspace;
     ca.out = workspace;  // in place: a strip is fully loaded before any store, rows only permute
     ca.n_planes = kc;
     ca.plane0 = (int)p0;
+    if (intensity) {
+      ca.mid_int = intensity + p0 * plane_px;
+      ca.mid_int_plane_stride = plane_px;
+    }
     if ((rc = launch_pipe_col(d->height, ca, s))) return rc;
     ga.in = (const float2*)workspace;
     ga.n_planes = kc;
     ga.plane0 = (int)p0;
     const int og = option(CXM_OPT_GRAD_GROUPS);
     const int groups = std::min<int>(og > 0 ? std::min(og, kGradGroups) : kGradGroupsDefault, kc);
+    if (ded) {
+      pg.plane0 = (int)p0;
+      pg.n_planes = kc;
+      if ((rc = launch_psf_grad(d->width, pg, groups, s))) return rc;
+      hipLaunchKernelGGL(psf_sum_kernel, dim3((unsigned)((plane_px + 255) / 256)), dim3(256), 0, s,
+                         (const float2*)partial, (const float2*)pr.t.P, grad_phase, plane_px, groups,
+                         p0 > 0 ? 1 : 0);
+      CXM_LAUNCH_CHECK();
+      continue;
+    }
     if ((rc = launch_pipe_phasegrad(d->width, ga, s, groups))) return rc;
     hipLaunchKernelGGL(sum_groups_kernel, dim3((unsigned)((plane_px + 255) / 256)), dim3(256), 0, s,
                        (const float2*)partial, grad_phase, d->height, d->width, groups, p0 > 0 ? 1 : 0,
@@ -362,6 +657,20 @@ extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot,
     CXM_LAUNCH_CHECK();
   }
   return CXM_OK;
+}
+}  // namespace
+}  // namespace cxm
+
+extern "C" int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot, float* grad_phase,
+                             void* workspace, size_t workspace_bytes, const cxm_psf4f_desc* d) {
+  return psf_bwd(stream, phase, cot, grad_phase, nullptr, workspace, workspace_bytes, d);
+}
+
+extern "C" int cxm_psf4f_value_and_grad(void* stream, const float* phase, const float* cot,
+                                        float* grad_phase, float* intensity, void* workspace,
+                                        size_t workspace_bytes, const cxm_psf4f_desc* d) {
+  if (!intensity) return fail(CXM_ERR_INVALID, "null buffer");
+  return psf_bwd(stream, phase, cot, grad_phase, intensity, workspace, workspace_bytes, d);
 }
 
 // ---------------------------------------------------------------------------

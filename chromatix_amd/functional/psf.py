@@ -89,15 +89,22 @@ def _forward(phase, d, out_kind, out=None) -> torch.Tensor:
     return out
 
 
-def _backward(phase, cot, d, kind) -> torch.Tensor:
+def _backward(phase, cot, d, kind, intensity=None) -> torch.Tensor:
+    """dL/dphase; with ``intensity`` (a (num_z, H, W) float32 tensor) the
+    forward intensity stack is stored on the way (cxm_psf4f_value_and_grad)."""
     d.grad_kind = kind
     cot = cot.to(torch.float32).resolve_conj().contiguous()
     if tuple(cot.shape) != (d.num_z, d.height, d.width):
         raise ValueError(f"cotangent / target must have shape {(d.num_z, d.height, d.width)}")
     grad = torch.empty((d.height, d.width), dtype=torch.float32, device=phase.device)
     ws = _workspace(d, phase.device)
-    L.check(L.lib().cxm_psf4f_bwd(L.stream_ptr(phase.device), phase.data_ptr(), cot.data_ptr(),
-                                  grad.data_ptr(), ws.data_ptr(), ws.numel(), ctypes.byref(d)))
+    if intensity is None:
+        L.check(L.lib().cxm_psf4f_bwd(L.stream_ptr(phase.device), phase.data_ptr(), cot.data_ptr(),
+                                      grad.data_ptr(), ws.data_ptr(), ws.numel(), ctypes.byref(d)))
+    else:
+        L.check(L.lib().cxm_psf4f_value_and_grad(L.stream_ptr(phase.device), phase.data_ptr(),
+                                                 cot.data_ptr(), grad.data_ptr(), intensity.data_ptr(),
+                                                 ws.data_ptr(), ws.numel(), ctypes.byref(d)))
     return grad
 
 
@@ -162,20 +169,28 @@ def psf_4f(phase, spacing, wavelength, z, f, n, NA, f_tube, power=1.0, amplitude
 
 
 def psf_4f_mse_grad(phase, target, spacing, wavelength, z, f, n, NA, f_tube, power=1.0,
-                    amplitude=1.0, offset=(0.0, 0.0), *, phase_mode=None) -> torch.Tensor:
+                    amplitude=1.0, offset=(0.0, 0.0), *, phase_mode=None, return_intensity=False):
     """d/dphase of mean((psf_4f(phase, ...) - target)^2), fused: the forward
-    is recomputed inside the adjoint and the intensity never stored."""
+    runs inside the adjoint.  With ``return_intensity`` the forward's
+    intensity stack (what psf_4f returns) comes back too, stored by the
+    adjoint's column pass as it forms it -- a value-and-gradient step in one
+    sweep: returns (grad, intensity)."""
     ph = _phase_tensor(phase).detach().resolve_conj().contiguous()
     args = (spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode)
     chunks = _z_chunks(args)
     nz = chunks[-1][0].stop
     t = torch.as_tensor(target, device=ph.device).reshape((nz,) + tuple(ph.shape[-2:]))
+    inten = (torch.empty((nz,) + tuple(ph.shape[-2:]), dtype=torch.float32, device=ph.device)
+             if return_intensity else None)
     grad = None
     for sl, a in chunks:
         d, keep, zs = _desc(ph, *a)
-        gc = _backward(ph, t[sl], d, L.CXM_GRAD_MSE)
+        gc = _backward(ph, t[sl], d, L.CXM_GRAD_MSE, None if inten is None else inten[sl])
         if len(chunks) > 1:
             # the library's mean is over its chunk's planes
             gc.mul_((sl.stop - sl.start) / nz)
         grad = gc if grad is None else grad.add_(gc)
+    if return_intensity:
+        single = np.ndim(np.asarray(z)) == 0 or np.asarray(z).size == 1
+        return grad, (inten[0] if single else inten)
     return grad

@@ -276,6 +276,16 @@ int cxm_psf4f_fwd(void* stream, const float* phase, void* out, void* workspace,
  */
 int cxm_psf4f_bwd(void* stream, const float* phase, const float* cot, float* grad_phase,
                   void* workspace, size_t workspace_bytes, const cxm_psf4f_desc* d);
+/*
+ * The loss value's forward and its gradient in one sweep (jax.value_and_grad
+ * of a loss of Optical4FSystemPSF's intensity, microscopes.py:334-355): the
+ * passes of cxm_psf4f_bwd, whose column pass also stores the intensity stack
+ * it forms (num_z x height x width float32, as cxm_psf4f_fwd with
+ * CXM_OUT_INTENSITY gives it).
+ */
+int cxm_psf4f_value_and_grad(void* stream, const float* phase, const float* cot, float* grad_phase,
+                             float* intensity, void* workspace, size_t workspace_bytes,
+                             const cxm_psf4f_desc* d);
 
 /*
  * objective_point_source (src/chromatix/functional/sources.py:122-209),
@@ -854,6 +864,14 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         H evaluated per bin.  Equal to float32 rounding.
  *   CXM_OPT_MS_ROW_E      values per thread of the dedicated 1024-point slice
  *                         row pass: 0 = 4, or 2, 8, 16
+ *   CXM_OPT_PSF_PASSES    cxm_psf4f_fwd / _bwd / _value_and_grad: 0 = the
+ *                         generated source from per-call tables (the
+ *                         plane-invariant factor, and the defocus chirp as
+ *                         row x column factors) in dedicated row and
+ *                         phase-gradient passes, where the row length has them
+ *                         (512, 1024, 1280, 1536, 2048); 1 = the operator
+ *                         pipeline with the source regenerated per element.
+ *                         Equal to float32 rounding.
  *   CXM_OPT_GRAD_GROUPS   cxm_psf4f_bwd: 0 = the default number of plane
  *                         groups of the phase-gradient pass (4), 1..12 =
  *                         that many (each group sums its planes in
@@ -878,6 +896,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_MBS_ZFUSE 16
 #define CXM_OPT_MS_PASSES 17
 #define CXM_OPT_MS_ROW_E 18
+#define CXM_OPT_PSF_PASSES 19
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

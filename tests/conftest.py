@@ -32,6 +32,11 @@ def dev():
 def rel_l2(a, b):
     import numpy as np
 
+    # in FP64: float32 norms of tiny values (gradients of unnormalised
+    # fields, ~1e-18) underflow and report 0
     a = np.asarray(a)
     b = np.asarray(b)
+    wide = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    a = a.astype(wide)
+    b = b.astype(wide)
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))

@@ -66,6 +66,56 @@ def test_psf_mse_gradient(pm, n, nz):
     assert err < TOL
 
 
+@pytest.mark.parametrize("n,nz", [(256, 3), (1280, 11), (96, 40)])
+def test_psf_value_and_gradient(n, nz):
+    """psf_4f_mse_grad(return_intensity=True): the gradient of the plain
+    call bit for bit, and the intensity stack the adjoint stores on the way
+    equal to psf_4f's bit for bit (same transforms of the same source) and to
+    the FP64 oracle; 40 planes run as two library calls."""
+    phase, dx, z, rng = _case(n, nz, pitch=6.5 * 1280 / n)
+    args = ((dx, dx), LAM, z, FOBJ, NIMM, NA, FTUBE)
+    I_ref = op.psf4f(phase.astype(np.float64), *args)
+    target = torch.as_tensor((I_ref * rng.uniform(0.5, 1.5, I_ref.shape)).astype(np.float32)).cuda()
+    ph = torch.as_tensor(phase).cuda()
+    g0 = cf.psf_4f_mse_grad(ph, target, dx, LAM, z, FOBJ, NIMM, NA, FTUBE)
+    g1, I1 = cf.psf_4f_mse_grad(ph, target, dx, LAM, z, FOBJ, NIMM, NA, FTUBE, return_intensity=True)
+    I0 = cf.psf_4f(ph, dx, LAM, z, FOBJ, NIMM, NA, FTUBE)
+    assert torch.equal(g0, g1)
+    assert torch.equal(I0, I1)
+    assert rel_l2(I1.cpu().numpy(), I_ref) < TOL
+
+
+@pytest.mark.parametrize("pm", ["ref32", "accurate"])
+@pytest.mark.parametrize("n,nz", [(512, 5), (1280, 11), (2048, 3)])
+def test_dedicated_psf_passes_match_pipeline(pm, n, nz):
+    """The table-driven dedicated row and phase-gradient passes
+    (CXM_OPT_PSF_PASSES 0) against the operator pipeline (1): intensity,
+    the MSE gradient and the autograd gradient, with a point-source offset
+    and without power normalisation, to float32 rounding."""
+    from chromatix_amd import _lib as L
+
+    phase, dx, z, rng = _case(n, nz, pitch=6.5 * 1280 / n)
+    ph = torch.as_tensor(phase).cuda()
+    w = torch.as_tensor(rng.standard_normal((nz, n, n)).astype(np.float32)).cuda()
+    res = []
+    try:
+        for opt in (0, 1):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_PSF_PASSES, opt))
+            I = cf.psf_4f(ph, dx, LAM, z, FOBJ, NIMM, NA, FTUBE, power=None, amplitude=2.0,
+                          offset=(0.4, -0.7), phase_mode=pm)
+            g, I2 = cf.psf_4f_mse_grad(ph, I * 1.1, dx, LAM, z, FOBJ, NIMM, NA, FTUBE, power=None,
+                                       amplitude=2.0, offset=(0.4, -0.7), phase_mode=pm,
+                                       return_intensity=True)
+            phr = ph.clone().requires_grad_(True)
+            (cf.psf_4f(phr, dx, LAM, z, FOBJ, NIMM, NA, FTUBE, phase_mode=pm) * w).sum().backward()
+            res.append((I.cpu().numpy(), g.cpu().numpy(), I2.cpu().numpy(), phr.grad.cpu().numpy()))
+    finally:
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_PSF_PASSES, 0))
+    errs = [rel_l2(a, b) for a, b in zip(res[0], res[1])]
+    print(f"dedicated vs pipeline psf n={n} nz={nz} {pm}: " + " ".join(f"{e:.2e}" for e in errs))
+    assert max(errs) < 2e-6
+
+
 def test_psf_more_planes_than_one_call():
     # 70 planes run as chunks of 32: intensity, field, autograd and the
     # MSE gradient (whose mean covers all planes) against the oracle
