@@ -298,6 +298,8 @@ struct PsfGradArgs {
   const float2* tw;
   const float2* in;  // chunk planes after the column adjoint
   float2* partial;   // groups x height x width
+  float* grad;       // one group: the gradient itself, Im(conj(P) sum)
+  int accum;         // one group: add to grad
   long long plane_stride;
   int ny, plane0, n_planes;
 };
@@ -339,6 +341,19 @@ __global__ void __launch_bounds__(N / E) psf_grad_kernel(PsfGradArgs a) {
       acc[k] = cadd(acc[k], cmulc(v[k], c));
     }
   }
+  if (groups == 1) {
+    // the plane-invariant factor applied here: grad (+)= Im(conj(P) sum)
+    const float2* Prow = a.t.P + (long long)r * N;
+    float* g = a.grad + (long long)r * N;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int xf = ishift_src(t + k * T, N);
+      const float2 w = Prow[xf];
+      const float v = w.x * acc[k].y - w.y * acc[k].x;
+      g[xf] = a.accum ? g[xf] + v : v;
+    }
+    return;
+  }
   float2* dst = a.partial + (long long)grp * a.ny * N + (long long)r * N;
 #pragma unroll
   for (int k = 0; k < E; ++k) dst[ishift_src(t + k * T, N)] = acc[k];
@@ -356,9 +371,103 @@ __global__ void psf_sum_kernel(const float2* __restrict__ partial, const float2*
   grad[i] = accum ? grad[i] + v : v;
 }
 
+struct PsfColArgs {
+  float2* ws;  // chunk planes, row-transformed
+  const float2* tw;
+  long long plane_stride;
+  int nx, plane0;
+  void* out;        // forward: chunk output (fftshifted), float32 |u|^2 or complex64
+  int intensity;
+  const float* cot; // backward: dL/dI or the target, all planes (fftshifted)
+  float* inten;     // backward: optional forward intensity of the chunk
+  int mse;
+  float p0;         // MSE: 2 / element count
+};
+
+// Column pass on strips of C columns (transform length N = height): the
+// column FFT (ifftshift folded into the load); forward: |u|^2 or u stored at
+// the fftshifted position; backward: at that position the cotangent of u,
+// 2 g u with g = dL/dI or 2 (|u|^2 - target) / count (the intensity stored
+// on the way when asked), then the inverse column FFT back into the strip.
+// The arithmetic of the pipeline's column pass with OP_IGRAD, in registers.
+template <int N, int E, int C, bool BWD>
+__global__ void __launch_bounds__((N / E) * C) psf_col_kernel(PsfColArgs a) {
+  constexpr int T = N / E;
+  __shared__ float2 lds[lds_span<N, C>()];
+  const int c = threadIdx.x % C;
+  const int ty = threadIdx.x / C;
+  const int nstrips = a.nx / C;
+  const int nb = gridDim.x;
+  const int L = ((nb & 7) == 0) ? (int)((blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3)) : (int)blockIdx.x;
+  const int p = L / nstrips;
+  const int col = (L - p * nstrips) * C + c;
+  float2* colp = a.ws + p * a.plane_stride + col;
+  const int xo = fshift_dst(col, a.nx);
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) v[k] = colp[(long long)ishift_src(ty + k * T, N) * a.nx];
+  // backward: the cotangents are loaded with the strip, so that their
+  // latency overlaps the forward transform instead of following it
+  float cv[BWD ? E : 1];
+  if constexpr (BWD) {
+    const long long cbase = (long long)(a.plane0 + p) * a.plane_stride + xo;
+#pragma unroll
+    for (int k = 0; k < E; ++k) cv[k] = a.cot[cbase + (long long)fshift_dst(ty + k * T, N) * a.nx];
+  }
+  fft<N, E, -1, C, true>(v, lds + c, ty, a.tw);
+  if constexpr (!BWD) {
+    const long long obase = p * a.plane_stride + xo;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const long long idx = obase + (long long)fshift_dst(ty + k * T, N) * a.nx;
+      if (a.intensity) ((float*)a.out)[idx] = cabs2(v[k]);
+      else ((float2*)a.out)[idx] = v[k];
+    }
+  } else {
+    const long long ibase = p * a.plane_stride + xo;
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const long long yo = (long long)fshift_dst(ty + k * T, N) * a.nx;
+      const float I = cabs2(v[k]);
+      if (a.inten) a.inten[ibase + yo] = I;
+      const float g = a.mse ? a.p0 * (I - cv[k]) : cv[k];
+      v[k] = cscale(v[k], 2.0f * g);
+    }
+    // the inverse transform's twiddle loads stay after the cotangents
+    int tyl = ty;
+    asm volatile("" : "+v"(tyl));
+    fft<N, E, 1, C, true>(v, lds + c, tyl, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k) colp[(long long)fshift_dst(ty + k * T, N) * a.nx] = v[k];
+  }
+}
+
+constexpr int kPsfColC = 4;  // columns per column-pass workgroup
+
 // (row length, values per thread) of the dedicated passes: whole wavefronts
 // per row
 #define CXM_PSF_SIZES(X) X(512, 8) X(1024, 16) X(1280, 20) X(1536, 24) X(2048, 16)
+
+int launch_psf_col(int n, bool bwd, const PsfColArgs& a, int planes, hipStream_t s) {
+  const long long blocks = (long long)planes * (a.nx / kPsfColC);
+  if (blocks > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "column pass grid too large");
+  switch (n) {
+#define CXM_PS_COL(N, E)                                                                          \
+  case N:                                                                                         \
+    if (bwd)                                                                                      \
+      hipLaunchKernelGGL((psf_col_kernel<N, E, kPsfColC, true>), dim3((unsigned)blocks),          \
+                         dim3((N / E) * kPsfColC), 0, s, a);                                      \
+    else                                                                                          \
+      hipLaunchKernelGGL((psf_col_kernel<N, E, kPsfColC, false>), dim3((unsigned)blocks),         \
+                         dim3((N / E) * kPsfColC), 0, s, a);                                      \
+    CXM_LAUNCH_CHECK();                                                                           \
+    return CXM_OK;
+    CXM_PSF_SIZES(CXM_PS_COL)
+#undef CXM_PS_COL
+    default:
+      return fail(CXM_ERR_UNSUPPORTED, "no dedicated PSF column pass for " + std::to_string(n));
+  }
+}
 
 bool psf_dedicated_size(int n) {
   switch (n) {
@@ -401,9 +510,13 @@ int launch_psf_grad(int n, const PsfGradArgs& a, int groups, hipStream_t s) {
   }
 }
 
-// The dedicated passes where the row length has them (CXM_OPT_PSF_PASSES 0)
+// The dedicated passes where the row length has them (CXM_OPT_PSF_PASSES 0),
+// and the dedicated column pass where the column length has one too
 bool psf_dedicated(const cxm_psf4f_desc* d) {
   return option(CXM_OPT_PSF_PASSES) != 1 && psf_dedicated_size(d->width);
+}
+bool psf_dedicated_col(const cxm_psf4f_desc* d) {
+  return psf_dedicated(d) && psf_dedicated_size(d->height) && d->width % kPsfColC == 0;
 }
 
 // Fill P and the chirp factors for the call: the operators of psf_ops split
@@ -536,6 +649,19 @@ extern "C" int cxm_psf4f_fwd(void* stream, const float* phase, void* out, void* 
     ca.out = (char*)out + (size_t)p0 * ca.out_plane_stride * esz;
     ca.n_planes = kc;
     ca.plane0 = (int)p0;
+    if (psf_dedicated_col(d)) {
+      PsfColArgs pc;
+      memset(&pc, 0, sizeof(pc));
+      pc.ws = (float2*)workspace;
+      pc.tw = twiddles_for(d->height);
+      pc.plane_stride = (long long)d->height * d->width;
+      pc.nx = d->width;
+      pc.plane0 = (int)p0;
+      pc.out = ca.out;
+      pc.intensity = ca.intensity;
+      if ((rc = launch_psf_col(d->height, false, pc, kc, s))) return rc;
+      continue;
+    }
     if ((rc = launch_pipe_col(d->height, ca, s))) return rc;
   }
   return CXM_OK;
@@ -634,18 +760,39 @@ int psf_bwd(void* stream, const float* phase, const float* cot, float* grad_phas
       ca.mid_int = intensity + p0 * plane_px;
       ca.mid_int_plane_stride = plane_px;
     }
-    if ((rc = launch_pipe_col(d->height, ca, s))) return rc;
+    if (psf_dedicated_col(d)) {
+      PsfColArgs pc;
+      memset(&pc, 0, sizeof(pc));
+      pc.ws = (float2*)workspace;
+      pc.tw = twiddles_for(d->height);
+      pc.plane_stride = plane_px;
+      pc.nx = d->width;
+      pc.plane0 = (int)p0;
+      pc.cot = cot;
+      pc.inten = intensity ? intensity + p0 * plane_px : nullptr;
+      pc.mse = d->grad_kind == CXM_GRAD_MSE;
+      pc.p0 = (float)(2.0 / count);
+      if ((rc = launch_psf_col(d->height, true, pc, kc, s))) return rc;
+    } else if ((rc = launch_pipe_col(d->height, ca, s))) {
+      return rc;
+    }
     ga.in = (const float2*)workspace;
     ga.n_planes = kc;
     ga.plane0 = (int)p0;
     const int og = option(CXM_OPT_GRAD_GROUPS);
     const int groups = std::min<int>(og > 0 ? std::min(og, kGradGroups) : kGradGroupsDefault, kc);
     if (ded) {
+      // one plane group by default (measured 355 us per 1280^2 x 11 step,
+      // 358 with 3 or 4 groups and their partial-sum pass)
+      const int dg = std::min<int>(og > 0 ? std::min(og, kGradGroups) : 1, kc);
       pg.plane0 = (int)p0;
       pg.n_planes = kc;
-      if ((rc = launch_psf_grad(d->width, pg, groups, s))) return rc;
+      pg.grad = grad_phase;
+      pg.accum = p0 > 0 ? 1 : 0;
+      if ((rc = launch_psf_grad(d->width, pg, dg, s))) return rc;
+      if (dg == 1) continue;
       hipLaunchKernelGGL(psf_sum_kernel, dim3((unsigned)((plane_px + 255) / 256)), dim3(256), 0, s,
-                         (const float2*)partial, (const float2*)pr.t.P, grad_phase, plane_px, groups,
+                         (const float2*)partial, (const float2*)pr.t.P, grad_phase, plane_px, dg,
                          p0 > 0 ? 1 : 0);
       CXM_LAUNCH_CHECK();
       continue;
