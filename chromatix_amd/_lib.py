@@ -167,6 +167,25 @@ class Psf4fDesc(ctypes.Structure):
 CXM_GRAD_COMPLEX = 2
 
 
+class ResampleDesc(ctypes.Structure):
+    _fields_ = [
+        ("batch", ctypes.c_int64),
+        ("in_h", ctypes.c_int32),
+        ("in_w", ctypes.c_int32),
+        ("out_h", ctypes.c_int32),
+        ("out_w", ctypes.c_int32),
+        ("ky", ctypes.c_int32),
+        ("kx", ctypes.c_int32),
+        ("mask_on_output", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("y_start", ctypes.c_void_p),
+        ("wy", ctypes.c_void_p),
+        ("x_start", ctypes.c_void_p),
+        ("wx", ctypes.c_void_p),
+        ("mask", ctypes.c_void_p),
+    ]
+
+
 class MbsDesc(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int32),
@@ -352,6 +371,8 @@ def _declare(lib):
         "cxm_cmul_c64": (i32, [vp, vp, vp, vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                ctypes.c_int32, ctypes.c_int32]),
         "cxm_czt_axis_workspace_size": (sz, [ctypes.POINTER(CztDesc)]),
+        "cxm_resample_workspace_size": (sz, [ctypes.POINTER(ResampleDesc)]),
+        "cxm_resample_f32": (i32, [vp, vp, vp, vp, sz, ctypes.POINTER(ResampleDesc)]),
         "cxm_mbs_workspace_size": (sz, [ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_precondition": (i32, [vp, vp, vp, vp, vp, vp, sz, ctypes.POINTER(MbsDesc)]),
         "cxm_mbs_iterate": (i32, [vp, vp, vp, vp, ctypes.c_int32, vp, vp, sz, ctypes.POINTER(MbsDesc)]),

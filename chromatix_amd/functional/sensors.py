@@ -1,11 +1,12 @@
 """The basic sensor on libcxm (mirrors src/chromatix/functional/sensors.py).
 
 ``basic_sensor`` (sensors.py:14-93): intensity of a field (or a given
-intensity), a sum over ``reduce_axis`` (cxm_sum_axis_f32, in index order),
-a sum across processes (the reference's ``jax.lax.psum`` over a pmapped
-axis: here an all-reduce over a ``torch.distributed`` process group, one
-process per GPU), then shot noise.  Resampling to the sensor's pixels is
-not implemented: the Microscope's imaging path calls the sensor without it.
+intensity), resampled to the sensor's pixels by a resampler of
+``ops.resample.init_plane_resample`` (cxm_resample_f32), summed over
+``reduce_axis`` (cxm_sum_axis_f32, in index order) and across processes
+(the reference's ``jax.lax.psum`` over a pmapped axis: here an all-reduce
+over a ``torch.distributed`` process group, one process per GPU), then
+shot noise.
 """
 from __future__ import annotations
 
@@ -88,14 +89,16 @@ def basic_sensor(sensor_input, shot_noise_mode=None, resampler=None, reduce_axis
     "poisson"; ``noise_key`` a torch.Generator or an integer seed)."""
     if isinstance(sensor_input, Field):
         intensity = sensor_input.intensity
+        # the spacing of the first wavelength (sensors.py:67-70)
+        input_spacing = sensor_input.central_rectangular_dx
     else:
         intensity = torch.as_tensor(sensor_input)
     if resampler is not None:
         if input_spacing is None:
             raise ValueError("Must provide input_spacing for intensity array")
-        raise NotImplementedError("basic_sensor: resampling to the sensor's pixels is not "
-                                  "implemented")
-    image = intensity
+        image = resampler(intensity, input_spacing)
+    else:
+        image = intensity
     if reduce_axis is not None:
         image = sum_axis(image, reduce_axis)
     if reduce_parallel_axis_name is not None:

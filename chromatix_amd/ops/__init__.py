@@ -20,7 +20,7 @@ import math
 import numpy as np
 import torch
 
-from . import _lib as L
+from .. import _lib as L
 
 __all__ = ["fourier_convolution", "planewise_convolution_sum", "approximate_shot_noise",
            "shot_noise", "next_order"]

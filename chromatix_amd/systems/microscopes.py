@@ -6,9 +6,8 @@ Optical4FSystemPSF (microscopes.py:300-365) on libcxm.
 sensor without resampling; when a BasicSensor sums the planes of a (D, H, W)
 sample, the sum is taken inside the convolution
 (``ops.planewise_convolution_sum``).  ``Microscope.__call__`` computes the PSF,
-crops its padding and passes it to the sensor's ``resample``; tapering the
-PSF edges (``taper_width`` > 0) and resampling to a different pitch are not
-implemented.
+crops its padding, tapers its edges (``utils.sigmoid_taper``, a mask applied
+inside the resampling call) and resamples it to the sensor's pixels.
 
 ``Optical4FSystemPSF`` takes any object with the objective's ``f``, ``n``,
 ``NA``, ``spectrum`` and ``padding_ratio`` (a ``Microscope`` has them).
@@ -20,10 +19,30 @@ import torch
 
 from ..core.field import ScalarField
 from ..core.spectrum import MonoSpectrum, Spectrum
-from ..elements.sensors import BasicSensor
+from ..elements.sensors import BasicSensor, apply_mask
 from ..functional.psf import psf_4f
 from ..functional.sensors import basic_sensor
 from ..ops import fourier_convolution, planewise_convolution_sum
+
+
+_TAPERS: dict = {}
+
+
+def _taper(shape, width, device) -> torch.Tensor:
+    """sigmoid_taper (src/chromatix/utils/utils.py:76-81): 2 (sigmoid(d /
+    width) - 1/2), d the Euclidean distance to a one-pixel zero border
+    (scipy's exact distance transform, FP64 on the host), as a float32
+    device mask; cached per shape, width and device."""
+    key = (shape, width, str(device))
+    t = _TAPERS.get(key)
+    if t is None:
+        from scipy.ndimage import distance_transform_edt
+
+        dist = distance_transform_edt(np.pad(np.ones((shape[0] - 2, shape[1] - 2)), 1))
+        mask = 2.0 * (1.0 / (1.0 + np.exp(-dist / width)) - 0.5)
+        t = torch.as_tensor(mask.astype(np.float32), device=device)
+        _TAPERS[key] = t
+    return t
 
 
 class Microscope:
@@ -87,8 +106,13 @@ class Microscope:
             # center_crop by padding // 2 on each side (utils.py:26-41)
             cy, cx = padding[0] // 2, padding[1] // 2
             psf = psf[..., cy: shape[0] - cy, cx: shape[1] - cx]
+        taper = None
         if self.taper_width > 0:
-            raise NotImplementedError("Microscope: PSF edge tapering is not implemented")
+            taper = _taper(tuple(psf.shape[-2:]), float(self.taper_width), psf.device)
+        if isinstance(self.sensor, BasicSensor):
+            return self.sensor.resample(psf, spacing, mask=taper)
+        if taper is not None:
+            psf = apply_mask(psf, taper)
         return self.sensor.resample(psf, spacing)
 
     def image(self, sample, psf, key=None):

@@ -900,6 +900,39 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 
+/*
+ * Separable resampling of a batch of float32 planes (batch x in_h x in_w ->
+ * batch x out_h x out_w) by per-output tap tables:
+ *   out[b, i, j] = sum_u sum_v wy[i][u] wx[j][v] in[b, y0[i] + u, x0[j] + v]
+ * (taps outside the input are skipped), with an optional mask (in_h x in_w,
+ * multiplying the input; or out_h x out_w multiplying the output when
+ * mask_on_output).  The caller builds the tables: sum pooling, or the
+ * weight matrices of jax.image.scale_and_translate divided by the scale
+ * product, which replaces the sensor's resampler (init_plane_resample,
+ * src/chromatix/ops/resample.py:10-128, called by basic_sensor,
+ * src/chromatix/functional/sensors.py:14-93, and Microscope._process_psf,
+ * src/chromatix/systems/microscopes.py:229-269, whose PSF taper
+ * utils.sigmoid_taper is the input mask); with the transposed tables and
+ * the mask on the output it is the resampler's adjoint.
+ * Workspace: cxm_resample_workspace_size (the row-resampled intermediate).
+ */
+typedef struct cxm_resample_desc {
+  int64_t batch;
+  int32_t in_h, in_w, out_h, out_w;
+  int32_t ky, kx;           /* taps per output row / column (table widths) */
+  int32_t mask_on_output;   /* 0: mask multiplies the input; 1: the output */
+  int32_t reserved;
+  const int32_t* y_start;   /* out_h first input rows (device) */
+  const float* wy;          /* out_h x ky weights (device) */
+  const int32_t* x_start;   /* out_w first input columns (device) */
+  const float* wx;          /* out_w x kx weights (device) */
+  const float* mask;        /* optional (device) */
+} cxm_resample_desc;
+
+size_t cxm_resample_workspace_size(const cxm_resample_desc* d);
+int cxm_resample_f32(void* stream, const float* in, float* out, void* workspace,
+                     size_t workspace_bytes, const cxm_resample_desc* d);
+
 /* Library state. */
 int cxm_init(int device);            /* build twiddle tables on `device` */
 const char* cxm_last_error(void);    /* thread-local text of the last error */
