@@ -161,6 +161,7 @@ class Psf4fDesc(ctypes.Structure):
         ("offset_y", ctypes.c_double),
         ("offset_x", ctypes.c_double),
         ("z", ctypes.POINTER(ctypes.c_double)),
+        ("phase_scale", ctypes.c_double),
     ]
 
 

@@ -132,6 +132,8 @@ float2 tube_norm(const cxm_psf4f_desc* d) {
   return make_float2(0.f, (float)(-(d->dy * d->dx) / (d->wavelength * d->f_tube / d->n)));
 }
 
+float phase_scale(const cxm_psf4f_desc* d) { return d->phase_scale == 0.0 ? 1.0f : (float)d->phase_scale; }
+
 int psf_ops(Ops& B, PipeArgs& ra, const cxm_psf4f_desc* d, const float* phase) {
   int rc = source_ops(B, ra.pre, d, tube_norm(d));
   if (rc) return rc;
@@ -142,7 +144,9 @@ int psf_ops(Ops& B, PipeArgs& ra, const cxm_psf4f_desc* d, const float* phase) {
   ph->arr_row_stride = d->width;
   ph->arr_rows = d->height;
   ph->arr_cols = d->width;
-  B.p(ph, 0, 0) = make_float4(1.0f, 0.f, 0.f, 0.f);  // spectral modulation of a mono field
+  // the phase in float32 as fl(phase * m), m the spectral modulation of the
+  // wavelength (phase_masks.py:105-124; 0 in the descriptor means 1)
+  B.p(ph, 0, 0) = make_float4(phase_scale(d), 0.f, 0.f, 0.f);
   return CXM_OK;
 }
 
@@ -298,8 +302,9 @@ struct PsfGradArgs {
   const float2* tw;
   const float2* in;  // chunk planes after the column adjoint
   float2* partial;   // groups x height x width
-  float* grad;       // one group: the gradient itself, Im(conj(P) sum)
+  float* grad;       // one group: the gradient itself, m Im(conj(P) sum)
   int accum;         // one group: add to grad
+  float m;           // the phase's spectral modulation (d/dphase = m d/d(m phase))
   long long plane_stride;
   int ny, plane0, n_planes;
 };
@@ -349,7 +354,7 @@ __global__ void __launch_bounds__(N / E) psf_grad_kernel(PsfGradArgs a) {
     for (int k = 0; k < E; ++k) {
       const int xf = ishift_src(t + k * T, N);
       const float2 w = Prow[xf];
-      const float v = w.x * acc[k].y - w.y * acc[k].x;
+      const float v = a.m * (w.x * acc[k].y - w.y * acc[k].x);
       g[xf] = a.accum ? g[xf] + v : v;
     }
     return;
@@ -361,13 +366,13 @@ __global__ void __launch_bounds__(N / E) psf_grad_kernel(PsfGradArgs a) {
 
 // grad[i] (+)= Im(conj(P[i]) sum_g partial[g][i]), g in order
 __global__ void psf_sum_kernel(const float2* __restrict__ partial, const float2* __restrict__ P,
-                               float* __restrict__ grad, long long n, int groups, int accum) {
+                               float* __restrict__ grad, long long n, int groups, int accum, float m) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   float2 c = make_float2(0.f, 0.f);
   for (int g = 0; g < groups; ++g) c = cadd(c, partial[g * n + i]);
   const float2 w = P[i];
-  const float v = w.x * c.y - w.y * c.x;
+  const float v = m * (w.x * c.y - w.y * c.x);
   grad[i] = accum ? grad[i] + v : v;
 }
 
@@ -577,7 +582,7 @@ PsfTables psf_table_ptrs(const cxm_psf4f_desc* d, void* workspace) {
 // plane-invariant operators at grid position i
 __global__ void sum_groups_kernel(const float2* __restrict__ partial, float* __restrict__ grad,
                                   int ny, int nx, int groups, int accum, int plane0, OpList mid,
-                                  OpTable tab) {
+                                  OpTable tab, float m) {
   const long long n = (long long)ny * nx;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -589,7 +594,7 @@ __global__ void sum_groups_kernel(const float2* __restrict__ partial, float* __r
   const int iy = (int)(i / nx), ix = (int)(i - (long long)iy * nx);
   const float2 w = mid.n ? apply_ops(mid, tab, make_float2(1.f, 0.f), plane0, iy, ix, ny, nx)
                          : make_float2(1.f, 0.f);
-  const float v = w.x * c.y - w.y * c.x;
+  const float v = m * (w.x * c.y - w.y * c.x);
   grad[i] = accum ? grad[i] + v : v;
 }
 }  // namespace
@@ -789,18 +794,19 @@ int psf_bwd(void* stream, const float* phase, const float* cot, float* grad_phas
       pg.n_planes = kc;
       pg.grad = grad_phase;
       pg.accum = p0 > 0 ? 1 : 0;
+      pg.m = phase_scale(d);
       if ((rc = launch_psf_grad(d->width, pg, dg, s))) return rc;
       if (dg == 1) continue;
       hipLaunchKernelGGL(psf_sum_kernel, dim3((unsigned)((plane_px + 255) / 256)), dim3(256), 0, s,
                          (const float2*)partial, (const float2*)pr.t.P, grad_phase, plane_px, dg,
-                         p0 > 0 ? 1 : 0);
+                         p0 > 0 ? 1 : 0, phase_scale(d));
       CXM_LAUNCH_CHECK();
       continue;
     }
     if ((rc = launch_pipe_phasegrad(d->width, ga, s, groups))) return rc;
     hipLaunchKernelGGL(sum_groups_kernel, dim3((unsigned)((plane_px + 255) / 256)), dim3(256), 0, s,
                        (const float2*)partial, grad_phase, d->height, d->width, groups, p0 > 0 ? 1 : 0,
-                       (int)p0, ga.mid, ga.tab);
+                       (int)p0, ga.mid, ga.tab, phase_scale(d));
     CXM_LAUNCH_CHECK();
   }
   return CXM_OK;

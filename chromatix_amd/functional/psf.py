@@ -32,11 +32,13 @@ def psf_output_spacing(shape, spacing, wavelength, f_tube, n) -> np.ndarray:
                      wavelength * f_tube / (n * shape[1] * dx)])
 
 
-def _desc(phase, spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode):
+def _desc(phase, spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode,
+          modulation=1.0):
     H, W = phase.shape[-2:]
     d, keep, zs = psf_desc((H, W), spacing, wavelength, z, f, n, NA, power, amplitude, offset,
                            phase_mode)
     d.f_tube = float(f_tube)
+    d.phase_scale = float(modulation)
     return d, keep, zs
 
 
@@ -138,7 +140,7 @@ def _phase_tensor(phase) -> torch.Tensor:
 
 
 def psf_4f(phase, spacing, wavelength, z, f, n, NA, f_tube, power=1.0, amplitude=1.0,
-           offset=(0.0, 0.0), *, output: str = "intensity", phase_mode=None):
+           offset=(0.0, 0.0), *, output: str = "intensity", phase_mode=None, spectral_modulation=1.0):
     """PSF stack of a 4f system with ``phase`` (radians, (H, W), on the GPU)
     in the pupil plane of an objective (focal length ``f``, numerical
     aperture ``NA``, immersion index ``n``) followed by a tube lens of focal
@@ -146,13 +148,16 @@ def psf_4f(phase, spacing, wavelength, z, f, n, NA, f_tube, power=1.0, amplitude
 
     output="intensity": (Z, H, W) float32, differentiable w.r.t. ``phase``.
     output="field": a ScalarField (Z, H, W) with the back-focal-plane spacing.
-    A scalar ``z`` drops the Z axis, like objective_point_source."""
+    A scalar ``z`` drops the Z axis, like objective_point_source.
+    ``spectral_modulation``: the phase enters as phase * m (phase_change's
+    lam_0 / lam for one wavelength of a chromatic field)."""
     ph = phase if isinstance(phase, torch.Tensor) and phase.dtype == torch.float32 else _phase_tensor(phase)
     if ph.ndim != 2:
         raise ValueError("phase must be a (height, width) array")
     L.ensure_device(ph)
     ph_c = ph.resolve_conj().contiguous()
-    args = (spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode)
+    args = (spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode,
+            spectral_modulation)
     single = np.ndim(np.asarray(z)) == 0 or np.asarray(z).size == 1
     if output == "intensity":
         if ph_c.requires_grad:
@@ -169,14 +174,16 @@ def psf_4f(phase, spacing, wavelength, z, f, n, NA, f_tube, power=1.0, amplitude
 
 
 def psf_4f_mse_grad(phase, target, spacing, wavelength, z, f, n, NA, f_tube, power=1.0,
-                    amplitude=1.0, offset=(0.0, 0.0), *, phase_mode=None, return_intensity=False):
+                    amplitude=1.0, offset=(0.0, 0.0), *, phase_mode=None, return_intensity=False,
+                    spectral_modulation=1.0):
     """d/dphase of mean((psf_4f(phase, ...) - target)^2), fused: the forward
     runs inside the adjoint.  With ``return_intensity`` the forward's
     intensity stack (what psf_4f returns) comes back too, stored by the
     adjoint's column pass as it forms it -- a value-and-gradient step in one
     sweep: returns (grad, intensity)."""
     ph = _phase_tensor(phase).detach().resolve_conj().contiguous()
-    args = (spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode)
+    args = (spacing, wavelength, z, f, n, NA, f_tube, power, amplitude, offset, phase_mode,
+            spectral_modulation)
     chunks = _z_chunks(args)
     nz = chunks[-1][0].stop
     t = torch.as_tensor(target, device=ph.device).reshape((nz,) + tuple(ph.shape[-2:]))

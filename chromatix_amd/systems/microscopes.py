@@ -17,11 +17,11 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from ..core.field import ScalarField
+from ..core.field import ChromaticScalarField, ScalarField
 from ..core.spectrum import MonoSpectrum, Spectrum
 from ..elements.sensors import BasicSensor, apply_mask
 from ..functional.psf import psf_4f
-from ..functional.sensors import basic_sensor
+from ..functional.sensors import basic_sensor, sum_axis
 from ..ops import fourier_convolution, planewise_convolution_sum
 
 
@@ -167,21 +167,74 @@ class Optical4FSystemPSF:
         padding = tuple(int(s * getattr(microscope, "padding_ratio", 0)) for s in self.shape)
         padded = tuple(s + p for s, p in zip(self.shape, padding))
         spectrum = Spectrum.build(microscope.spectrum)
-        if not isinstance(spectrum, MonoSpectrum):
-            raise NotImplementedError("Optical4FSystemPSF: monochromatic spectra only")
-        lam = float(spectrum.wavelength[0])
-        dx = self.compute_required_spacing(padded[0], self.spacing, self.f_tube, microscope.n, lam)
         phase = torch.as_tensor(self.phase)
         if tuple(phase.shape) != padded:
             raise ValueError(f"phase must have the simulation shape {padded}")
-        return phase, dx, lam
+        return phase, padded, spectrum
 
-    def __call__(self, microscope, z) -> ScalarField:
-        phase, dx, lam = self._setup(microscope)
-        return psf_4f(phase, dx, lam, z, microscope.f, microscope.n, microscope.NA, self.f_tube,
-                      output="field")
+    def _wavelengths(self, microscope, padded, spectrum, with_density):
+        """Per wavelength k of the spectrum: (lam_k, dx_k, amplitude_k, m_k)
+        for one monochromatic PSF call each.  dx_k = f_tube lam_k / (n H
+        spacing) keeps every wavelength's output spacing at ``spacing``; the
+        point sources of all wavelengths are normalised as one chromatic
+        field (sources.py:206-208, field.py:729-733: u_k *= sqrt(P / (dx_k^2
+        S)) with S = sum_k w_k M_k / L_k^4, M_k the samples inside the pupil,
+        L_k^2 = lam_k f / n), so the call takes the amplitude 1 * sqrt(P /
+        (dx_k^2 S)) without its own normalisation; ``with_density`` folds
+        sqrt(w_k) in as well, which makes the intensity call return w_k
+        |out_k|^2 (field.py:735-740); m_k = lam_0 / lam_k the phase
+        modulation (phase_masks.py:105-124)."""
+        lam = np.asarray(spectrum.wavelength, np.float64)
+        w = np.asarray(spectrum.density, np.float64)
+        f, n, NA = float(microscope.f), float(microscope.n), float(microscope.NA)
+        dxs = [self.compute_required_spacing(padded[0], self.spacing, self.f_tube, n, lk) for lk in lam]
+        S = sum(wk * _pupil_samples(padded, dk, f, NA, n) / (lk * f / n) ** 2
+                for wk, dk, lk in zip(w, dxs, lam))
+        out = []
+        for lk, dk, wk, mk in zip(lam, dxs, w, spectrum.spectral_modulation):
+            amp = float(np.sqrt(1.0 / (dk * dk * S)))  # power 1 (objective_point_source's default)
+            if with_density:
+                amp *= float(np.sqrt(wk))
+            out.append((float(lk), dk, amp, float(mk)))
+        return out
+
+    def __call__(self, microscope, z):
+        phase, padded, spectrum = self._setup(microscope)
+        f, n, NA = microscope.f, microscope.n, microscope.NA
+        if isinstance(spectrum, MonoSpectrum):
+            lam = float(spectrum.wavelength[0])
+            dx = self.compute_required_spacing(padded[0], self.spacing, self.f_tube, n, lam)
+            return psf_4f(phase, dx, lam, z, f, n, NA, self.f_tube, output="field")
+        fields = [psf_4f(phase, dk, lk, z, f, n, NA, self.f_tube, power=None, amplitude=ak,
+                         spectral_modulation=mk, output="field")
+                  for lk, dk, ak, mk in self._wavelengths(microscope, padded, spectrum, False)]
+        u = torch.stack([fl.u for fl in fields], dim=-1)  # (..., H, W, wv)
+        du = np.stack([np.asarray(fl.dx, np.float64).reshape(2) for fl in fields])
+        return ChromaticScalarField(u, du, 0.0, spectrum)
 
     def intensity(self, microscope, z) -> torch.Tensor:
-        """|field|^2 of __call__, differentiable with respect to ``phase``."""
-        phase, dx, lam = self._setup(microscope)
-        return psf_4f(phase, dx, lam, z, microscope.f, microscope.n, microscope.NA, self.f_tube)
+        """|field|^2 of __call__ (for a chromatic spectrum, sum_k w_k
+        |out_k|^2), differentiable with respect to ``phase``."""
+        phase, padded, spectrum = self._setup(microscope)
+        f, n, NA = microscope.f, microscope.n, microscope.NA
+        if isinstance(spectrum, MonoSpectrum):
+            lam = float(spectrum.wavelength[0])
+            dx = self.compute_required_spacing(padded[0], self.spacing, self.f_tube, n, lam)
+            return psf_4f(phase, dx, lam, z, f, n, NA, self.f_tube)
+        planes = [psf_4f(phase, dk, lk, z, f, n, NA, self.f_tube, power=None, amplitude=ak,
+                         spectral_modulation=mk)
+                  for lk, dk, ak, mk in self._wavelengths(microscope, padded, spectrum, True)]
+        return sum_axis(torch.stack(planes, dim=0), 0)
+
+
+def _pupil_samples(shape, dx, f, NA, n) -> int:
+    """Samples of a (H, W) grid of spacing dx inside the circular pupil of
+    diameter 2 f NA / n (pupils.py:17-23), in the grid's float32 arithmetic
+    (the count the point-source kernel's pupil test gives)."""
+    H, W = shape
+    d = np.float32(dx)
+    gy = d * (np.arange(H, dtype=np.float32) - np.float32(0.5 * H))
+    gx = d * (np.arange(W, dtype=np.float32) - np.float32(0.5 * W))
+    r = np.float32(2.0 * f * NA / n) / np.float32(2.0)
+    l2 = gy[:, None] * gy[:, None] + gx[None, :] * gx[None, :]
+    return int(np.count_nonzero(l2 <= r * r))

@@ -261,6 +261,10 @@ typedef struct cxm_psf4f_desc {
   double power, amplitude;
   double offset_y, offset_x; /* lateral point-source offset */
   const double* z;        /* num_z defocus distances, host memory */
+  double phase_scale;     /* the phase mask enters as fl(phase * phase_scale), the
+                             spectral modulation lam_0 / lam of phase_change
+                             (phase_masks.py:105-124) for one wavelength of a
+                             chromatic field; 0 means 1 */
 } cxm_psf4f_desc;
 
 size_t cxm_psf4f_workspace_size(const cxm_psf4f_desc* d);

@@ -3,6 +3,8 @@ infrastructure, FP64).
 
 Follows (paths relative to the chromatix repository root):
   Optical4FSystemPSF.__call__   src/chromatix/systems/microscopes.py:334-355
+  (chromatic)                   with field.power / .intensity, core/field.py:729-740,
+                                and spectrally_modulate_phase, phase_masks.py:105-124
   compute_required_spacing      src/chromatix/systems/microscopes.py:357-365
   objective_point_source        src/chromatix/functional/sources.py:122-209
   phase_change                  src/chromatix/functional/phase_masks.py:29-68
@@ -33,6 +35,35 @@ def psf4f_field(phase, spacing, wavelength, z, f, n, NA, f_tube, power=1.0):
     u = np.asarray(u).reshape((-1,) + shape)
     u = u * np.exp(1j * np.asarray(phase, np.float64))
     return ff_lens(u, spacing, wavelength, f_tube, n)
+
+
+def psf4f_chromatic(phase, output_spacing, wavelengths, density, z, f, n, NA, f_tube, power=1.0):
+    """Intensity stack (Z, H, W) of Optical4FSystemPSF for a chromatic
+    spectrum (microscopes.py:334-355 with a ChromaticScalarField): per
+    wavelength k the pupil spacing dx_k = required_spacing(..., lam_k); the
+    point source of every wavelength, unnormalised, then normalised as one
+    chromatic field (sources.py:206-208 with field.power, field.py:729-733:
+    power_k = dx_k^2 sum_{y,x} sum_k' w_k' |u_k'|^2, so u_k *= sqrt(P /
+    power_k)); the phase scaled by lam_0 / lam_k (phase_masks.py:105-124);
+    ff_lens per wavelength; intensity sum_k w_k |out_k|^2 (field.py:735-740)
+    with the density w normalised to sum 1 (Spectrum.build)."""
+    shape = np.shape(phase)
+    lam = np.asarray(wavelengths, np.float64).reshape(-1)
+    w = np.asarray(density, np.float64).reshape(-1)
+    w = w / w.sum()
+    zs = np.atleast_1d(np.asarray(z, np.float64))
+    dxs = [required_spacing(shape[0], output_spacing, f_tube, n, lk) for lk in lam]
+    us = [np.asarray(objective_point_source(shape, (dk, dk), lk, zs, f, n, NA, None)).reshape((-1,) + shape)
+          for lk, dk in zip(lam, dxs)]
+    S = sum(wk * np.sum(np.abs(u) ** 2, axis=(-2, -1)) for wk, u in zip(w, us))  # per z plane
+    out = 0.0
+    for lk, dk, wk, u in zip(lam, dxs, w, us):
+        if power is not None:
+            u = u * np.sqrt(power / (dk * dk * S))[:, None, None]
+        u = u * np.exp(1j * np.asarray(phase, np.float64) * (lam[0] / lk))
+        o, _ = ff_lens(u, (dk, dk), lk, f_tube, n)
+        out = out + wk * np.abs(o) ** 2
+    return out
 
 
 def psf4f(phase, spacing, wavelength, z, f, n, NA, f_tube, power=1.0):

@@ -89,3 +89,43 @@ def test_point_source_defocus_phase():
     np.testing.assert_allclose(np.angle(u[inside] / u0[inside]),
                                np.angle(np.exp(-1j * np.pi * (z / F_OBJ) * r2[inside] / L2)),
                                atol=1e-12)
+
+
+def test_chromatic_psf_reduces_to_monochromatic():
+    # one wavelength, and the same wavelength listed twice with any weights:
+    # the chromatic composition (per-wavelength spacing and modulation, the
+    # whole field normalised, density-weighted intensity) gives the
+    # monochromatic PSF exactly
+    rng = np.random.default_rng(0)
+    n, pitch, ft = 64, 6.5 * 20, 200000.0
+    phase = rng.uniform(-np.pi, np.pi, (n, n))
+    z = np.array([-2.0, 1.5])
+    dx = op.required_spacing(n, pitch, ft, 1.33, 0.532)
+    mono = op.psf4f(phase, (dx, dx), 0.532, z, 8000.0, 1.33, 0.6, ft)
+    one = op.psf4f_chromatic(phase, pitch, [0.532], [1.0], z, 8000.0, 1.33, 0.6, ft)
+    two = op.psf4f_chromatic(phase, pitch, [0.532, 0.532], [0.3, 0.9], z, 8000.0, 1.33, 0.6, ft)
+    np.testing.assert_allclose(one, mono, rtol=1e-12, atol=1e-12 * mono.max())
+    np.testing.assert_allclose(two, mono, rtol=1e-12, atol=1e-12 * mono.max())
+
+
+def test_chromatic_psf_power_and_weights():
+    # the normalised chromatic source has power P in the reference's sense at
+    # every wavelength's area, and the lens conserves each wavelength's
+    # energy: sum_k w_k dx_k^2 sum|u_k|^2 = ... the output intensity summed
+    # over the plane times the output pixel area equals sum_k w_k P_k with
+    # P_k = P S_k / S (S_k = sum |u_k|^2 before the lens, S = sum_k w_k S_k)
+    rng = np.random.default_rng(1)
+    n, pitch, ft, P = 96, 6.5 * 10, 200000.0, 2.0
+    lam, wts = np.array([0.5, 0.6]), np.array([1.0, 3.0])
+    phase = rng.uniform(-np.pi, np.pi, (n, n))
+    I = op.psf4f_chromatic(phase, pitch, lam, wts, 0.0, 8000.0, 1.33, 0.6, ft, power=P)[0]
+    w = wts / wts.sum()
+    from oracle.sources import objective_point_source
+
+    dxs = [op.required_spacing(n, pitch, ft, 1.33, l) for l in lam]
+    Sk = [np.sum(np.abs(objective_point_source((n, n), (d, d), l, 0.0, 8000.0, 1.33, 0.6, None)) ** 2)
+          for l, d in zip(lam, dxs)]
+    S = np.dot(w, Sk)
+    # per wavelength, after normalisation, sum |u_k|^2 dx_k^2 = P S_k / S
+    expect = sum(wk * P * sk / S for wk, sk in zip(w, Sk))
+    assert abs(I.sum() * pitch**2 / expect - 1.0) < 1e-9
