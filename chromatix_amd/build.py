@@ -102,5 +102,91 @@ def build(verbose: bool = True, jobs: int | None = None) -> Path:
     return LIB
 
 
+# ---------------------------------------------------------------------------
+# LDS barrier check (a maintainer tool, not part of build()): every exchange
+# barrier (block_sync / wave_sync in csrc/cxm_common.h) carries an asm marker,
+# so the final assembly can be cut into the same barrier-delimited segments
+# as the optimised IR.  A segment that mixes LDS reads and writes in the
+# assembly but not in the IR means a machine pass moved an LDS access across
+# a barrier -- a cross-wave race.
+# ---------------------------------------------------------------------------
+_MARK = "cxm sink fence"
+
+
+def _lds_segments_isa(text: str) -> dict[str, list[set]]:
+    out, name, segs = {}, None, None
+    for line in text.splitlines():
+        m = re.match(r"^(_Z\S+):", line)
+        if m:
+            name, segs = m.group(1), [set()]
+            continue
+        if name is None:
+            continue
+        if line.startswith(".Lfunc_end"):
+            out[name], name = segs, None
+            continue
+        s = line.strip()
+        if s.startswith(("ds_read", "ds_load")):
+            segs[-1].add("R")
+        elif s.startswith(("ds_write", "ds_store")):
+            segs[-1].add("W")
+        elif s.startswith("s_barrier") or _MARK in s:
+            segs.append(set())
+    return out
+
+
+def _lds_segments_ir(text: str) -> dict[str, list[set]]:
+    out = {}
+    for m in re.finditer(r"^define [^@]*@(_Z\S+)\(", text, re.M):
+        segs = [set()]
+        end = text.index("\n}\n", m.start())
+        for line in text[m.start():end].splitlines():
+            if "addrspace(3)" in line and re.search(r"= load ", line):
+                segs[-1].add("R")
+            elif "addrspace(3)" in line and re.match(r"\s*store ", line):
+                segs[-1].add("W")
+            elif ("llvm.amdgcn.s.barrier" in line and "call" in line) or _MARK in line:
+                segs.append(set())
+        out[m.group(1)] = segs
+    return out
+
+
+def _check_source(src: Path) -> list[str]:
+    d = OBJ_DIR / "lds_check"
+    d.mkdir(parents=True, exist_ok=True)
+    flags = [f for f in COMMON_FLAGS if f != "-fPIC"] + ["--cuda-device-only", "-S"]
+    texts = []
+    for suffix, extra in ((".s", []), (".ll", ["-emit-llvm"])):
+        out = d / (src.stem + suffix)
+        res = subprocess.run([HIPCC, *flags, *extra, str(src), "-o", str(out)], capture_output=True,
+                             text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {src.name}:\n{res.stderr[-4000:]}")
+        texts.append(out.read_text())
+    isa, ir = _lds_segments_isa(texts[0]), _lds_segments_ir(texts[1])
+    bad = []
+    for k, segs in isa.items():
+        mixed = sum(1 for g in segs if g == {"R", "W"})
+        mixed_ir = sum(1 for g in ir.get(k, []) if g == {"R", "W"})
+        if mixed > mixed_ir:
+            bad.append(f"{src.name}: {k}: {mixed} mixed LDS segments (IR: {mixed_ir})")
+    return bad
+
+
+def check_lds_barriers(jobs: int | None = None) -> int:
+    """Compile every source to assembly and IR and report kernels whose LDS
+    accesses crossed a barrier; returns the number reported."""
+    srcs = sources()
+    jobs = jobs or min(len(srcs), max(1, (os.cpu_count() or 2)))
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        bad = [b for res in ex.map(_check_source, srcs) for b in res]
+    for b in bad:
+        print(b, file=sys.stderr)
+    print(f"LDS barrier check: {len(srcs)} sources, {len(bad)} kernels flagged", file=sys.stderr)
+    return len(bad)
+
+
 if __name__ == "__main__":
+    if "--check-lds" in sys.argv[1:]:
+        sys.exit(1 if check_lds_barriers() else 0)
     build()

@@ -39,6 +39,36 @@ template <int DIR> CXM_DEV float2 mul_i(float2 a) {
   else return make_float2(-a.y, a.x);
 }
 CXM_DEV float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
+
+#ifndef CXM_CPU_CHECK
+// ---------------------------------------------------------------------------
+// Barriers for LDS exchanges.  s_barrier and the workgroup fences around it
+// carry no memory operand, and LLVM's machine sinking moves a load below
+// them into a successor block when nothing in between may store (seen: the
+// last exchange reads of a transform issued after the barrier that lets the
+// other waves write the next transform's stage into the same slots -- a
+// cross-wave race that corrupted whole column strips now and then).  An asm
+// statement that clobbers memory counts as a store there, so loads stay on
+// their side.  It emits no instruction; the marker comment lets the built
+// assembly be checked for LDS operations on the wrong side.
+// ---------------------------------------------------------------------------
+CXM_DEV void sink_fence() { asm volatile("; cxm sink fence" ::: "memory"); }
+// __syncthreads with no load moved across it in either direction
+CXM_DEV void block_sync() {
+  sink_fence();
+  __syncthreads();
+  sink_fence();
+}
+// The same for the waves of one wavefront only (a transform that one wave
+// computes alone): the LDS writes before it are visible to every lane after.
+CXM_DEV void wave_sync() {
+  sink_fence();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  sink_fence();
+}
+#endif
 // Correctly rounded float32 square root.  (HIP's __fsqrt_rn maps to the
 // approximate native sqrt unless OCML rounded operations are enabled; the
 // builtin is correctly rounded under HIP's default

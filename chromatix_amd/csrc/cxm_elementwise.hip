@@ -65,10 +65,10 @@ plane_sumsq_stage1(const float2* __restrict__ u, double* __restrict__ partial,
   }
   __shared__ double s[kSumThreads];
   s[threadIdx.x] = acc;
-  __syncthreads();
+  block_sync();
   for (int w = kSumThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) partial[(long long)plane * kSumChunks + chunk] = s[0];
 }

@@ -131,10 +131,10 @@ plane_ss_kernel(const float* __restrict__ x, long long plane_elems, double* __re
     acc += (double)px[i] * px[i];
   __shared__ double sh[256];
   sh[threadIdx.x] = acc;
-  __syncthreads();
+  block_sync();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) partial[(long long)plane * kSsBlocks + blockIdx.x] = sh[0];
 }
@@ -245,10 +245,10 @@ pair_row_kernel(PairRowArgs a) {
     v[k] = make_float2(re, im);
   }
   fft<N, E, -1, 1, SYNC>(v, st, t, a.tw);
-  __syncthreads();  // every wave of the row has read the transform's last exchange
+  block_sync();  // every wave of the row has read the transform's last exchange
 #pragma unroll
   for (int k = 0; k < E; ++k) st[lds_at_split<N, 1>(t, k * T)] = v[k];
-  __syncthreads();
+  block_sync();
   pair_tiles_exchange<N, PITCH, true>(lds, a.out + p * a.out_plane + (long long)(row0 - p * a.rows) * kTileW,
                                       a.strip_stride, 4 * T);
 }
@@ -279,9 +279,9 @@ CXM_DEV void pair_col_plane(const PairColArgs& a, int p, const float2* colp, boo
     u[k] = (valid && i < a.in_len) ? colp[(long long)i * kTileW] : make_float2(0.f, 0.f);
   }
   fft<N, E, -1, C, true>(u, lds + c, ty, a.tw);
-  __syncthreads();  // the transform's last exchange has been read by every thread
+  block_sync();  // the transform's last exchange has been read by every thread
   stage_out<N, E, C>(u, lds + c, ty);
-  __syncthreads();
+  block_sync();
   const float f = a.norm * a.inv4s[a.plane0 + p];  // 0 for a zero plane: exact zeros
 #pragma unroll
   for (int k = 0; k < E; ++k) {
@@ -295,7 +295,7 @@ CXM_DEV void pair_col_plane(const PairColArgs& a, int p, const float2* colp, boo
     const float dy = 2.0f * (z.x * z.y - z2.x * z2.y);
     u[k] = make_float2(dy * f, -dx * f);
   }
-  __syncthreads();  // the partner values are read before the inverse transform writes LDS
+  block_sync();  // the partner values are read before the inverse transform writes LDS
   fft<N, E, 1, C, true>(u, lds + c, ty, a.tw);
 }
 
@@ -368,7 +368,7 @@ pair_inv_kernel(PairInvArgs a) {
   const int r = row0 - p * a.rows + g;
   pair_tiles_exchange<N, PITCH, false>(lds, (float2*)a.in + p * a.in_plane + (long long)(row0 - p * a.rows) * kTileW,
                                        a.strip_stride, 4 * T);
-  __syncthreads();
+  block_sync();
   float2* st = lds + g * PITCH;
   float2 v[E];
 #pragma unroll

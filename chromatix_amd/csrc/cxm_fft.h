@@ -295,12 +295,12 @@ CXM_DEV void stage_body(float2 (&v)[E], float2* lds, int t, const StageTw<N, E, 
 template <int N, int E, int S, bool SYNC>
 CXM_DEV void stage_exchange(float2 (&v)[E], float2* lds, int t) {
   constexpr int T = N / E;
-  if constexpr (SYNC) __syncthreads();
-  else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+  if constexpr (SYNC) block_sync();
+  else wave_sync();
 #pragma unroll
   for (int e = 0; e < E; ++e) v[e] = lds[lds_at_split<N, S>(t, e * T)];
-  if constexpr (SYNC) __syncthreads();
-  else { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup"); __builtin_amdgcn_wave_barrier(); __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup"); }
+  if constexpr (SYNC) block_sync();
+  else wave_sync();
 }
 
 template <int N, int E, int DIR, int Ns, int S, bool SYNC>

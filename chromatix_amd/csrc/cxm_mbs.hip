@@ -174,9 +174,9 @@ mbs_xgreen_kernel(float2* __restrict__ v, const float2* __restrict__ tw, long lo
   for (int k = 0; k < E; ++k)
     u[k] = colok ? vq[(long long)(ty + k * T) * ncols] : make_float2(0.f, 0.f);
   fft<N, E, -1, CC, true>(u, lds + cc, ty, tw);
-  __syncthreads();  // the transform's last exchange has been read by every thread
+  block_sync();  // the transform's last exchange has been read by every thread
   stage_out<N, E, CC>(u, lds + cc, ty);
-  __syncthreads();
+  block_sync();
   const int iy = colok ? col / nz : 0;
   const int iz = colok ? col - iy * nz : 0;
   const float ky = (float)signed_bin(iy, ny) * k_step(kscale, ny);
@@ -207,7 +207,7 @@ mbs_xgreen_kernel(float2* __restrict__ v, const float2* __restrict__ tw, long lo
   }
   // the inverse transform's first stage writes this thread's column before
   // any barrier: every thread must have read the other components first
-  __syncthreads();
+  block_sync();
   fft<N, E, 1, CC, true>(u, lds + cc, ty, tw);
   if (!colok) return;
 #pragma unroll
@@ -294,11 +294,11 @@ CXM_DEV void ztransform(float2* lds, const float2* __restrict__ tw) {
   float2 u[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) u[k] = lds[lds_at<N, Cfg::S>(t + k * T) + tr];
-  __syncthreads();  // every initial value has been read before the first scatter
+  block_sync();  // every initial value has been read before the first scatter
   fft<N, E, DIR, Cfg::S, true>(u, lds + tr, t, tw);
 #pragma unroll
   for (int k = 0; k < E; ++k) lds[lds_at<N, Cfg::S>(t + k * T) + tr] = u[k];
-  __syncthreads();
+  block_sync();
 }
 
 // gb[c] = FFT_z( V(x)[c] ) for the rows of this workgroup (the first pass of
@@ -327,7 +327,7 @@ mbs_zv_kernel(const float2* __restrict__ x, const float2* __restrict__ eps, floa
 #pragma unroll
     for (int c = 0; c < 3; ++c) lds[zslot<N, E>(z, r, c)] = o[c];
   }
-  __syncthreads();
+  block_sync();
   ztransform<N, E, -1>(lds, tw);
   for (int j = threadIdx.x; j < nv; j += Cfg::THREADS) {
     const int r = j / N, z = j - r * N;
@@ -359,7 +359,7 @@ mbs_zupdate_kernel(const float2* __restrict__ gb, const float2* __restrict__ eps
     for (int c = 0; c < 3; ++c)
       lds[zslot<N, E>(z, r, c)] = j < nv ? gb[c * vox + base + j] : make_float2(0.f, 0.f);
   }
-  __syncthreads();
+  block_sync();
   ztransform<N, E, 1>(lds, tw);
   double acc = 0.0;
   for (int j = threadIdx.x; j < nv; j += Cfg::THREADS) {
@@ -385,14 +385,14 @@ mbs_zupdate_kernel(const float2* __restrict__ gb, const float2* __restrict__ eps
   // transform's LDS once every thread has read its values
   constexpr int P = pow2_ceil(Cfg::THREADS);
   static_assert(P * sizeof(double) <= lds_span<N, Cfg::S>() * sizeof(float2), "reduction exceeds LDS");
-  __syncthreads();
+  block_sync();
   double* sh = reinterpret_cast<double*>(lds);
   sh[threadIdx.x] = acc;
   if (threadIdx.x + Cfg::THREADS < P) sh[threadIdx.x + Cfg::THREADS] = 0.0;
-  __syncthreads();
+  block_sync();
   for (int w = P / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
@@ -500,10 +500,10 @@ mbs_update_kernel(const float2* __restrict__ g, const float2* __restrict__ eps,
   }
   __shared__ double sh[kRedThreads];
   sh[threadIdx.x] = acc;
-  __syncthreads();
+  block_sync();
   for (int w = kRedThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
@@ -516,10 +516,10 @@ sum_partials_kernel(const double* __restrict__ partial, int n, double* __restric
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += kRedThreads) acc += partial[i];
   sh[threadIdx.x] = acc;
-  __syncthreads();
+  block_sync();
   for (int w = kRedThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) out[0] = sh[0];
 }
@@ -538,10 +538,10 @@ mbs_stop_kernel(const double* __restrict__ partial, int n, double* __restrict__ 
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += kRedThreads) acc += partial[i];
   sh[threadIdx.x] = acc;
-  __syncthreads();
+  block_sync();
   for (int w = kRedThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) {
     const double rs = sh[0];
@@ -564,10 +564,10 @@ maxabs_kernel(const float2* __restrict__ v, long long n, double* __restrict__ pa
   }
   __shared__ double sh[kRedThreads];
   sh[threadIdx.x] = m;
-  __syncthreads();
+  block_sync();
   for (int w = kRedThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
@@ -578,10 +578,10 @@ max_partials_kernel(const double* __restrict__ partial, int n, double* __restric
   double m = 0.0;
   for (int i = threadIdx.x; i < n; i += kRedThreads) m = fmax(m, partial[i]);
   sh[threadIdx.x] = m;
-  __syncthreads();
+  block_sync();
   for (int w = kRedThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) out[0] = sh[0];
 }
@@ -762,13 +762,13 @@ CXM_DEV void block_sums(double (&v)[NV], double* partial) {
   __shared__ double sh[NV][kRedThreads];
 #pragma unroll
   for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] = v[j];
-  __syncthreads();
+  block_sync();
   for (int w = kRedThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
 #pragma unroll
       for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + w];
     }
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -821,13 +821,13 @@ CXM_DEV void sum_partials(const double* partial, double (&out)[NV]) {
   }
 #pragma unroll
   for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] = a[j];
-  __syncthreads();
+  block_sync();
   for (int w = kRedThreads / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) {
 #pragma unroll
       for (int j = 0; j < NV; ++j) sh[j][threadIdx.x] += sh[j][threadIdx.x + w];
     }
-    __syncthreads();
+    block_sync();
   }
 #pragma unroll
   for (int j = 0; j < NV; ++j) out[j] = sh[j][0];
@@ -1372,19 +1372,19 @@ ec_extreme_kernel(const float2* __restrict__ v, long long n, double* __restrict_
   for (int d = 0; d < kEcDirs; ++d) {
     sb[threadIdx.x] = best[d];
     sa[threadIdx.x] = arg[d];
-    __syncthreads();
+    block_sync();
     for (int w = 128; w > 0; w >>= 1) {
       if ((int)threadIdx.x < w && sb[threadIdx.x + w] > sb[threadIdx.x]) {
         sb[threadIdx.x] = sb[threadIdx.x + w];
         sa[threadIdx.x] = sa[threadIdx.x + w];
       }
-      __syncthreads();
+      block_sync();
     }
     if (threadIdx.x == 0) {
       partial[(blockIdx.x * kEcDirs + d) * 2] = sb[0];
       partial[(blockIdx.x * kEcDirs + d) * 2 + 1] = (double)sa[0];
     }
-    __syncthreads();
+    block_sync();
   }
 }
 
@@ -1408,7 +1408,7 @@ ec_extreme_final_kernel(const float2* __restrict__ v, const double* __restrict__
   __shared__ int sb[256];
   sv[threadIdx.x] = best;
   sb[threadIdx.x] = bb;
-  __syncthreads();
+  block_sync();
   for (int w = 16; w > 0; w >>= 1) {
     if (l < w) {
       const double o = sv[threadIdx.x + w];
@@ -1419,7 +1419,7 @@ ec_extreme_final_kernel(const float2* __restrict__ v, const double* __restrict__
         sb[threadIdx.x] = ob;
       }
     }
-    __syncthreads();
+    block_sync();
   }
   if (l == 0) {
     const int b = sb[threadIdx.x];
@@ -1471,10 +1471,10 @@ ec_radius_kernel(const float2* __restrict__ v, long long n, double cx, double cy
   }
   __shared__ double sh[256];
   sh[threadIdx.x] = m;
-  __syncthreads();
+  block_sync();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
+    block_sync();
   }
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }

@@ -126,7 +126,7 @@ CXM_DEV void run_seq(const PipeArgs& a, float2 (&v)[E], float2* lds, int t, int 
   else fft<N, E, 1, S, SYNC>(v, lds, t, a.tw);
   if (a.seq == SEQ_FI || a.seq == SEQ_IF) {
     if (a.mid.n || (a.mid_out && is_row)) {
-      __syncthreads();  // the FFT's last exchange has been read by every thread
+      block_sync();  // the FFT's last exchange has been read by every thread
       stage_out<N, E, S>(v, lds, t);
 #pragma unroll 1
       for (int k = 0; k < E; ++k) {
@@ -149,7 +149,7 @@ CXM_DEV void run_seq(const PipeArgs& a, float2 (&v)[E], float2* lds, int t, int 
         *slot = x;
       }
       stage_in<N, E, S>(v, lds, t);
-      __syncthreads();  // before the next FFT scatters into other threads' slots
+      block_sync();  // before the next FFT scatters into other threads' slots
     } else {
 #pragma unroll
       for (int k = 0; k < E; ++k) v[k] = cscale(v[k], a.mid_scale);
@@ -209,7 +209,7 @@ pipe_row_kernel(PipeArgs a) {
       st[lds_at<N, 1>(i)] = x;
     }
     stage_in<N, E, 1>(v, st, t);
-    __syncthreads();
+    block_sync();
   } else {
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -222,7 +222,7 @@ pipe_row_kernel(PipeArgs a) {
   run_seq<N, E, 1, true>(a, v, st, t, plane, yg, true);
   const long long obase = p * a.out_plane_stride + (long long)r * a.out_line_stride;
   if (a.post.n) {
-    __syncthreads();
+    block_sync();
     stage_out<N, E, 1>(v, st, t);
     if (!ok) return;
 #pragma unroll 1
@@ -292,7 +292,7 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
 #pragma unroll
     for (int k = 0; k < E; ++k) v[k] = ok ? src[t + k * T] : make_float2(0.f, 0.f);
     fft<N, E, 1, 1, true>(v, st, t, a.tw);
-    __syncthreads();
+    block_sync();
     stage_out<N, E, 1>(v, st, t);
 #pragma unroll 1
     for (int k = 0; k < E; ++k) {
@@ -313,7 +313,7 @@ pipe_row_phasegrad_kernel(PipeArgs a) {
       const float2 q = st[lds_at<N, 1>(t + k * T)];
       acc[k] = make_float2(acc[k].x + q.x, acc[k].y + q.y);
     }
-    __syncthreads();  // before the next plane's FFT reuses the slots
+    block_sync();  // before the next plane's FFT reuses the slots
   }
   if (!ok) return;
   if (a.partial_complex) {
@@ -388,7 +388,7 @@ pipe_col_kernel(PipeArgs a) {
       st[lds_at<N, C>(i)] = x;
     }
     stage_in<N, E, C>(v, st, ty);
-    __syncthreads();
+    block_sync();
   } else {
 #pragma unroll
     for (int k = 0; k < E; ++k) {
@@ -405,7 +405,7 @@ pipe_col_kernel(PipeArgs a) {
   const bool store = colok && xl >= 0 && xl < a.out_line_len;
   const long long obase = p * a.out_plane_stride + xl;
   if (a.post.n) {
-    __syncthreads();
+    block_sync();
     stage_out<N, E, C>(v, st, ty);
     if (!store) return;
 #pragma unroll 1

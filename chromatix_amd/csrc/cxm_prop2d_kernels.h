@@ -141,7 +141,7 @@ row_fft_kernel(RowArgs a) {
             a.in + p0 * a.in_plane_stride + (long long)(row0 - p0 * a.rows_per_plane) * kTileW,
             a.in_strip_stride, lds);
       }
-      __syncthreads();
+      block_sync();
 #pragma unroll
       for (int k = 0; k < E; ++k) v[k] = st[lds_at_split<N, 1>(t, k * T)];
       loaded = true;
@@ -161,7 +161,7 @@ row_fft_kernel(RowArgs a) {
       // each wave rewrites only its own row, which no other wave reads
 #pragma unroll
       for (int k = 0; k < E; ++k) st[lds_at_split<N, 1>(t, k * T)] = cscale(v[k], a.scale);
-      __syncthreads();
+      block_sync();
       const int row0 = blockIdx.x * G;
       if (row0 < a.n_rows) {
         const int p0 = row0 / a.rows_per_plane;
@@ -249,7 +249,7 @@ row_fft_loop_kernel(RowArgs a) {
     float2* st = lds + g * PITCH;
     float2 v[E];
     if (a.in_tiled) {
-      __syncthreads();  // the previous group's LDS reads are done
+      block_sync();  // the previous group's LDS reads are done
 #pragma unroll
       for (int i = 0; i < E / 2; ++i) {
         const int c = i * 256 + tid;
@@ -258,7 +258,7 @@ row_fft_loop_kernel(RowArgs a) {
         d[0] = make_float2(pf[i].x, pf[i].y);
         d[1] = make_float2(pf[i].z, pf[i].w);
       }
-      __syncthreads();
+      block_sync();
 #pragma unroll
       for (int k = 0; k < E; ++k) v[k] = st[lds_at_split<N, 1>(t, k * T)];
     } else {
@@ -275,7 +275,7 @@ row_fft_loop_kernel(RowArgs a) {
     if (a.out_tiled) {
 #pragma unroll
       for (int k = 0; k < E; ++k) st[lds_at_split<N, 1>(t, k * T)] = cscale(v[k], a.scale);
-      __syncthreads();
+      block_sync();
       const int p0 = row0 / a.rows_per_plane;
       float2* base =
           a.out_c + p0 * a.out_plane_stride + (long long)(row0 - p0 * a.rows_per_plane) * kTileW;
@@ -287,7 +287,7 @@ row_fft_loop_kernel(RowArgs a) {
         *(float4*)(base + (c >> 3) * a.out_strip_stride + 2 * q) =
             make_float4(d[0].x, d[0].y, d[1].x, d[1].y);
       }
-      __syncthreads();  // before the next group's transforms reuse the rows
+      block_sync();  // before the next group's transforms reuse the rows
     } else {
       const int row = row0 + g;
       if (row < a.n_rows) {

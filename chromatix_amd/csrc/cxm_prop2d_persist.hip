@@ -38,7 +38,7 @@ col_prop_persist_kernel(ColArgs a, int n_items) {
   const bool na = a.na_r2 > 0.0f;
   if constexpr (TWL) {
     for (int i = threadIdx.x; i < N; i += THREADS) twl[i] = a.tw[i];
-    __syncthreads();
+    block_sync();
   }
   // Strips whose rows all lie in the valid input (output) rows and whose
   // columns all exist -- every strip of an unpadded, uncropped call -- load
@@ -129,7 +129,7 @@ col_prop_persist_kernel(ColArgs a, int n_items) {
     }
     // the next item's first exchange must not overwrite LDS a slower wave
     // still reads in this item's last one
-    __syncthreads();
+    block_sync();
   }
 }
 

@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256) cgemm_mfma_kernel(CgemmArgs g) {
       br[b_k * kBP + b_col + q] = pb[q].x;
       bi[b_k * kBP + b_col + q] = pb[q].y;
     }
-    __syncthreads();
+    block_sync();
     if (k0 + kBK < kend) fetch(k0 + kBK);
 #pragma unroll
     for (int s = 0; s < kBK / 4; ++s) {
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(256) cgemm_mfma_kernel(CgemmArgs g) {
           acc_i[ti][tj] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa_i[ti], xb_r[tj], acc_i[ti][tj], 0, 0, 0);
         }
     }
-    __syncthreads();
+    block_sync();
   }
 #pragma unroll
   for (int ti = 0; ti < 2; ++ti)
