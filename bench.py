@@ -44,6 +44,8 @@ def parse():
     p.add_argument("--phase-mode", default="ref32", choices=["ref32", "accurate"])
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--mbs-concurrency", type=int, default=2,
+                   help="c4/c10: volumes solved at once, each on its own stream")
     p.add_argument("--config", default="m0",
                    choices=["m0", "c1", "c2", "c3", "c4", "c5", "c6", "c7", "c8", "c9", "c10"],
                    help="m0: the headline batched 1024^2 AS propagation (default); c1-c10: the "
@@ -82,13 +84,22 @@ def hbm_copy_gbps(torch, L, nbytes=1 << 30, reps=10):
 
 
 def _cpu_model() -> str:
+    """The host CPU's model name with its family / model ids (a virtual
+    machine may report only the brand in the name)."""
+    info = {}
     try:
         for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
+            if ":" in line:
+                k, v = (s.strip() for s in line.split(":", 1))
+                info.setdefault(k, v)
+            elif info:
+                break
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    if "cpu family" in info and "model" in info:
+        name += f" (family {info['cpu family']}, model {info['model']})"
+    return name
 
 
 def cpu_baseline(args):
@@ -422,24 +433,24 @@ def bench_c4(args, use_bicgstab=False):
     lam = 0.532
     k0, sp = 2 * np.pi / lam, lam / 4
     n, bc, vols = 128, 16, 8
-    times, iters = [], []
+    kw = dict(tol=1e-4, use_bicgstab=True) if use_bicgstab else dict(rtol=1e-4)
     # untimed warm-up (library initialisation, first launches of every
-    # kernel): short solves of the first volume
+    # kernel, the side streams): short solves of the first volume
     for sample, src in _mbs_volumes(torch, mbs, n, bc, 1, k0, sp):
-        kw = dict(tol=1e-4, use_bicgstab=True) if use_bicgstab else dict(rtol=1e-4)
+        c = max(1, args.mbs_concurrency)
         for _ in range(max(1, args.warmup)):
-            mbs.solve(sample, src, maxiter=16, **kw)
+            mbs.solve_batch([sample] * c, [src] * c, maxiter=16, concurrency=c, **kw)
         torch.cuda.synchronize()
-    for sample, src in _mbs_volumes(torch, mbs, n, bc, vols, k0, sp):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        if use_bicgstab:
-            res = mbs.solve(sample, src, maxiter=3000, tol=1e-4, use_bicgstab=True, return_info=True)
-        else:
-            res = mbs.solve(sample, src, maxiter=3000, rtol=1e-4, return_info=True)
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t0)
-        iters.append(res.iterations)
+    # the volumes are independent: solve_batch runs them args.mbs_concurrency
+    # at a time, each on its own stream (1: one after another)
+    pairs = list(_mbs_volumes(torch, mbs, n, bc, vols, k0, sp))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = mbs.solve_batch([p[0] for p in pairs], [p[1] for p in pairs], maxiter=3000,
+                          concurrency=args.mbs_concurrency, return_info=True, **kw)
+    torch.cuda.synchronize()
+    times = [(time.perf_counter() - t0) / vols] * vols
+    iters = [r.iterations for r in res]
     N3 = (n + 2 * bc) ** 3
     per_it = sum(times) / sum(iters)
     if use_bicgstab:
@@ -453,14 +464,16 @@ def bench_c4(args, use_bicgstab=False):
                        "8 volumes 160^3 (128^3 + 16-voxel boundary), 20 spheres, plane-wave current "
                        "sheet, BiCGStab to relative residual 1e-4",
                        per_iter_bytes * N3 * (sum(iters) / vols), {}, None,
-                       {"iterations": iters, "ms_per_iteration": per_it * 1e3})
+                       {"iterations": iters, "ms_per_iteration": per_it * 1e3,
+                        "concurrent_solves": args.mbs_concurrency})
     return _result(args, "modified Born series solves/s (128^3 + boundary, rtol 1e-4)",
                    vols / sum(times), "solves/s", sum(times) / vols * 1e3, vols, max(1, args.warmup),
                    "complex64",
                    "8 volumes 160^3 (128^3 + 16-voxel boundary), 20 spheres, plane-wave current "
                    "sheet, to relative residual 1e-4",
                    272 * N3 * (sum(iters) / vols), {}, None,
-                   {"iterations": iters, "ms_per_iteration": per_it * 1e3})
+                   {"iterations": iters, "ms_per_iteration": per_it * 1e3,
+                    "concurrent_solves": args.mbs_concurrency})
 
 
 def bench_c10(args):

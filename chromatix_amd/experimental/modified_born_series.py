@@ -29,7 +29,7 @@ import torch
 from .. import _lib as L
 
 __all__ = ["Sample", "EmptySample", "Source", "add_absorbing_bc", "get_shift_and_scale", "solve",
-           "MBSResult", "Solver", "BiCGStabSolver"]
+           "solve_batch", "MBSResult", "Solver", "BiCGStabSolver"]
 
 
 class Sample:
@@ -427,6 +427,75 @@ def _run(sample: Sample, source: Source, initial_E, maxiter, tol, use_bicgstab, 
         sv.step(check_every)  # the next chunk stays queued while the host waits
         sv.wait()
     return sv, sv.result()
+
+
+def solve_batch(samples, sources, maxiter: int = 1000, tol: float = 1e-3, use_bicgstab: bool = False,
+                *, rtol: float | None = None, check_every: int = 8, concurrency: int = 2,
+                return_info: bool = False) -> list:
+    """Independent solves of (sample, source) pairs -- what ``jax.vmap`` of
+    solve (solver.py:101-146) gives in the reference -- run ``concurrency``
+    at a time, each on a stream of its own, so that one volume's setup and
+    the gaps between its launches overlap the other volumes' iterations.
+    Each result is that of solve() for its pair: the same kernels in the same
+    order on its stream.  Forward only (no gradients).  Returns the list of
+    E (or MBSResult with ``return_info``), in input order, ready on the
+    caller's current stream."""
+    samples, sources = list(samples), list(sources)
+    if len(samples) != len(sources):
+        raise ValueError("solve_batch needs one source per sample")
+    if use_bicgstab and rtol is not None:
+        raise ValueError("rtol applies to the fixed-point iteration; BiCGStab's tol is relative")
+    if concurrency < 1:
+        raise ValueError("concurrency must be >= 1")
+    if torch.is_grad_enabled() and any(
+            s.permittivity.requires_grad or torch.as_tensor(c.field).requires_grad
+            for s, c in zip(samples, sources)):
+        raise NotImplementedError("solve_batch: no gradients; use solve() per volume")
+    out: list = [None] * len(samples)
+    if not samples:
+        return out
+    dev = torch.as_tensor(samples[0].permittivity).device
+    L.ensure_device(torch.as_tensor(samples[0].permittivity))
+    caller = torch.cuda.current_stream(dev)
+    streams = [torch.cuda.Stream(device=dev) for _ in range(min(concurrency, len(samples)))]
+    for st in streams:
+        st.wait_stream(caller)  # the inputs may still be in flight on the caller's stream
+    todo = list(range(len(samples)))
+    active: dict = {}
+
+    def start(slot):
+        i = todo.pop(0)
+        with torch.cuda.stream(streams[slot]):
+            if use_bicgstab:
+                sv = BiCGStabSolver(samples[i], sources[i], tol=tol, maxiter=maxiter)
+            else:
+                sv = Solver(samples[i], sources[i], tol=tol, rtol=rtol, maxiter=maxiter)
+            # two chunks queued: the stream stays busy while the host reads
+            # the first chunk's status
+            sv.step(check_every)
+            sv.step(check_every)
+        active[slot] = (i, sv)
+
+    for slot in range(len(streams)):
+        start(slot)
+    while active:
+        for slot in list(active):
+            i, sv = active[slot]
+            with torch.cuda.stream(streams[slot]):
+                sv.wait()
+                if not sv.converged:
+                    sv.step(check_every)
+                    continue
+                E = sv.result()
+                E.record_stream(caller)
+            its = sv.k if use_bicgstab else sv.iterations
+            out[i] = MBSResult(E, its, sv.residual) if return_info else E
+            del active[slot]
+            if todo:
+                start(slot)
+    for st in streams:
+        caller.wait_stream(st)
+    return out
 
 
 def _adjoint_desc(d):

@@ -356,3 +356,19 @@ def test_solve_gradient_flows_to_current_density():
     E.abs().square().sum().backward()
     assert J_t.grad is not None and torch.isfinite(torch.view_as_real(J_t.grad)).all()
     assert float(J_t.grad.abs().max()) > 0
+
+
+@pytest.mark.parametrize("bicg", [False, True])
+def test_solve_batch_matches_solve(bicg):
+    """solve_batch (volumes on concurrent streams, two at a time) gives each
+    volume's solve() result bit for bit, with the same iteration count."""
+    cases = [_setup(n, bc, 3, seed) for n, bc, seed in [(32, 4, 11), (40, 4, 12), (32, 8, 13)]]
+    samples = [mbs.Sample(torch.as_tensor(e.astype(np.complex64)).cuda(), sp) for e, sp, _ in cases]
+    sources = [mbs.Source(torch.as_tensor(J.astype(np.complex64)).cuda(), K0) for _, _, J in cases]
+    kw = dict(tol=1e-4, use_bicgstab=True) if bicg else dict(rtol=1e-4)
+    batch = mbs.solve_batch(samples, sources, maxiter=2000, concurrency=2, return_info=True, **kw)
+    for s, c, b in zip(samples, sources, batch):
+        one = mbs.solve(s, c, maxiter=2000, return_info=True, **kw)
+        print(f"solve_batch {tuple(s.shape)} bicgstab={bicg}: {b.iterations} / {one.iterations} iterations")
+        assert b.iterations == one.iterations
+        assert torch.equal(b.E, one.E)
