@@ -311,57 +311,91 @@ struct PsfGradArgs {
 
 // Phase-gradient row pass: for row r and the planes of group blockIdx.y,
 // g_p = IFFT_row(in[p, r, :]) (the adjoint of the forward row FFT) and
-//   partial[group][r][x] = sum_p g_p(x) conj(c_p(r, x)),  x = ishift(i);
-// the plane-invariant factor is applied once per element in the group sum.
-template <int N, int E>
-__global__ void __launch_bounds__(N / E) psf_grad_kernel(PsfGradArgs a) {
+//   sum[x] = sum_p g_p(x) conj(c_p(r, x)),  x = ishift(i),
+// the plane-invariant factor applied once per element to the sum.  W row
+// transforms per workgroup: transform w takes planes pa + w, pa + w + W, ...
+// and keeps its sum in registers; the W sums meet in LDS and are added in
+// the order w = 0 .. W-1.  Every transform group runs the same number of
+// transforms (the last with zeros where it has no plane), so the
+// block-wide barriers of transforms longer than a wave match.
+template <int N, int E, int W>
+__global__ void __launch_bounds__((N / E) * W) psf_grad_kernel(PsfGradArgs a) {
   constexpr int T = N / E;
+  // a transform of one wave exchanges within its own LDS region; only the
+  // final reduction of the W sums needs the whole workgroup
   constexpr bool SYNC = T > 64;
-  __shared__ float2 lds[lds_span<N, 1>()];
+  constexpr int SPAN = lds_span<N, 1>();
+  static_assert(W * SPAN >= W * N, "the partial sums reuse the transforms' LDS");
+  __shared__ float2 lds[W * SPAN];
   const int r = blockIdx.x;
   const int grp = blockIdx.y, groups = gridDim.y;
   const int pa = (int)((long long)grp * a.n_planes / groups);
   const int pb = (int)((long long)(grp + 1) * a.n_planes / groups);
-  const int t = threadIdx.x;
+  const int w = threadIdx.x / T;
+  const int t = threadIdx.x - w * T;
+  float2* st = lds + w * SPAN;
   float2 acc[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) acc[k] = make_float2(0.f, 0.f);
-  for (int p = pa; p < pb; ++p) {
+  const int rounds = (pb - pa + W - 1) / W;
+  for (int i = 0; i < rounds; ++i) {
     // the thread index laundered per plane: the transform's twiddle loads
     // are then not hoisted out of the loop (hoisted, every stage's twiddles
     // stay live next to the plane sum: 256 VGPRs, one wave per SIMD)
     int tl = t;
     asm volatile("" : "+v"(tl));
-    const int pg = a.plane0 + p;
-    const float2* src = a.in + p * a.plane_stride + (long long)r * N;
+    const int p = pa + i * W + w;
+    const bool have = p < pb;
+    const int pg = a.plane0 + (have ? p : pa);
+    const float2* src = a.in + (have ? p : pa) * a.plane_stride + (long long)r * N;
     float2 v[E];
 #pragma unroll
-    for (int k = 0; k < E; ++k) v[k] = src[tl + k * T];
-    fft<N, E, 1, 1, SYNC>(v, lds, tl, a.tw);
-    const float2 fy = a.t.cy[(long long)pg * a.ny + r];
-    const float2* cxp = a.t.cx + (long long)pg * N;
+    for (int k = 0; k < E; ++k) v[k] = have ? src[tl + k * T] : make_float2(0.f, 0.f);
+    fft<N, E, 1, 1, SYNC>(v, st, tl, a.tw);
+    if (have) {
+      const float2 fy = a.t.cy[(long long)pg * a.ny + r];
+      const float2* cxp = a.t.cx + (long long)pg * N;
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const float2 c = cmul(cxp[ishift_src(tl + k * T, N)], fy);
-      acc[k] = cadd(acc[k], cmulc(v[k], c));
+      for (int k = 0; k < E; ++k) {
+        const float2 c = cmul(cxp[ishift_src(tl + k * T, N)], fy);
+        acc[k] = cadd(acc[k], cmulc(v[k], c));
+      }
     }
   }
-  if (groups == 1) {
-    // the plane-invariant factor applied here: grad (+)= Im(conj(P) sum)
-    const float2* Prow = a.t.P + (long long)r * N;
-    float* g = a.grad + (long long)r * N;
+  float2* part = lds;  // W partial sums in grid order, over the transforms' LDS
+  if constexpr (W > 1) {
+    block_sync();  // every transform's last exchange has been read
 #pragma unroll
-    for (int k = 0; k < E; ++k) {
-      const int xf = ishift_src(t + k * T, N);
-      const float2 w = Prow[xf];
-      const float v = a.m * (w.x * acc[k].y - w.y * acc[k].x);
-      g[xf] = a.accum ? g[xf] + v : v;
-    }
-    return;
+    for (int k = 0; k < E; ++k) part[w * N + ishift_src(t + k * T, N)] = acc[k];
+    block_sync();
   }
+  const float2* Prow = a.t.P + (long long)r * N;
+  float* g = a.grad + (long long)r * N;
   float2* dst = a.partial + (long long)grp * a.ny * N + (long long)r * N;
+  // element x of the row: thread threadIdx.x takes x = threadIdx.x + j * (T W)
 #pragma unroll
-  for (int k = 0; k < E; ++k) dst[ishift_src(t + k * T, N)] = acc[k];
+  for (int j = 0; j < E; ++j) {
+    float2 c;
+    int xf;
+    if constexpr (W > 1) {
+      xf = threadIdx.x + j * T * W;
+      if (xf >= N) break;
+      c = part[xf];
+#pragma unroll
+      for (int u = 1; u < W; ++u) c = cadd(c, part[u * N + xf]);
+    } else {
+      xf = ishift_src(t + j * T, N);
+      c = acc[j];
+    }
+    if (groups == 1) {
+      // the plane-invariant factor applied here: grad (+)= m Im(conj(P) sum)
+      const float2 q = Prow[xf];
+      const float v = a.m * (q.x * c.y - q.y * c.x);
+      g[xf] = a.accum ? g[xf] + v : v;
+    } else {
+      dst[xf] = c;
+    }
+  }
 }
 
 // grad[i] (+)= Im(conj(P[i]) sum_g partial[g][i]), g in order
@@ -500,14 +534,39 @@ int launch_psf_row(int n, const PsfRowArgs& a, int planes, hipStream_t s) {
   }
 }
 
+// Row transforms per workgroup of the phase-gradient pass
+// (CXM_OPT_PSF_GRAD_WAVES; 0 = the default).  1280^2 x 11 planes, per
+// value-and-gradient step: W = 1 0.363 ms, 2 0.394, 4 0.371 -- more waves per
+// row do not shorten the pass, so one wave per row sums every plane.
+constexpr int kPsfGradWavesDefault = 1;
+
+template <int N, int E>
+int launch_psf_grad_n(const PsfGradArgs& a, int groups, int waves, hipStream_t s) {
+  const dim3 grid((unsigned)a.ny, (unsigned)groups);
+  switch (waves) {
+    case 1:
+      hipLaunchKernelGGL((psf_grad_kernel<N, E, 1>), grid, dim3(N / E), 0, s, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((psf_grad_kernel<N, E, 2>), grid, dim3(2 * (N / E)), 0, s, a);
+      break;
+    case 4:
+      hipLaunchKernelGGL((psf_grad_kernel<N, E, 4>), grid, dim3(4 * (N / E)), 0, s, a);
+      break;
+    default:
+      return fail(CXM_ERR_INVALID, "phase-gradient transforms per workgroup must be 1, 2 or 4");
+  }
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
 int launch_psf_grad(int n, const PsfGradArgs& a, int groups, hipStream_t s) {
+  const int ow = option(CXM_OPT_PSF_GRAD_WAVES);
+  const int waves = ow > 0 ? ow : kPsfGradWavesDefault;
   switch (n) {
-#define CXM_PS_GRAD(N, E)                                                                            \
-  case N:                                                                                            \
-    hipLaunchKernelGGL((psf_grad_kernel<N, E>), dim3((unsigned)a.ny, (unsigned)groups), dim3(N / E), \
-                       0, s, a);                                                                     \
-    CXM_LAUNCH_CHECK();                                                                              \
-    return CXM_OK;
+#define CXM_PS_GRAD(N, E) \
+  case N:                 \
+    return launch_psf_grad_n<N, E>(a, groups, waves, s);
     CXM_PSF_SIZES(CXM_PS_GRAD)
 #undef CXM_PS_GRAD
     default:

@@ -876,6 +876,10 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         (512, 1024, 1280, 1536, 2048); 1 = the operator
  *                         pipeline with the source regenerated per element.
  *                         Equal to float32 rounding.
+ *   CXM_OPT_PSF_GRAD_WAVES  dedicated phase-gradient pass: row transforms
+ *                         per workgroup (0 = 1, or 2, 4), each summing every
+ *                         W-th plane; the W sums are added in LDS.  Equal
+ *                         to float32 rounding.
  *   CXM_OPT_GRAD_GROUPS   cxm_psf4f_bwd: 0 = the default number of plane
  *                         groups of the phase-gradient pass (4), 1..12 =
  *                         that many (each group sums its planes in
@@ -901,6 +905,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_MS_PASSES 17
 #define CXM_OPT_MS_ROW_E 18
 #define CXM_OPT_PSF_PASSES 19
+#define CXM_OPT_PSF_GRAD_WAVES 20
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 
