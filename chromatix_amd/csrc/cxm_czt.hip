@@ -112,6 +112,157 @@ size_t czt_ws(const CztGeom& g) {
   return (size_t)g.P * ((size_t)g.n + g.L + g.m) * sizeof(float2);
 }
 
+// ---- dedicated chirp-convolution passes ---------------------------------------
+// The pass both entry points run, without the operator interpreter: per line
+// (a row, or one column of a strip) of plane p, with chirp set q = p mod P,
+//   v_i = x_i wA[q][i] (i < n; zero to length L),  v = FFT(v),
+//   v_k = (v_k / L) wH[q][k],  v = IFFT(v),
+//   out_j = v_{j + out_off} wP[q][j]  (j < m).
+// The multiplies sit in registers around the two transforms; the arithmetic
+// is the operator pipeline's (the same products in the same order), so both
+// give the same bits.
+struct ChirpConvArgs {
+  const float2* in;
+  float2* out;
+  const float2* tw;
+  const float2* wA;  // P x n
+  const float2* wH;  // P x L (the spectrum of the chirp kernel)
+  const float2* wP;  // P x m
+  long long in_plane_stride, out_plane_stride;
+  int lines;            // rows (row pass) or columns (column pass) of a plane
+  int width;            // elements between consecutive rows of in (and of out, column pass)
+  int out_line_stride;  // row pass: elements between output rows
+  int n, m, out_off, P;
+  float mid_scale;      // 1 / L
+};
+
+template <int L, int E>
+__global__ void __launch_bounds__(L / E) chirp_row_kernel(ChirpConvArgs a) {
+  constexpr int T = L / E;
+  constexpr bool SYNC = T > 64;
+  __shared__ float2 lds[lds_span<L, 1>()];
+  const int t = threadIdx.x;
+  const int p = blockIdx.x / a.lines;
+  const int r = blockIdx.x - p * a.lines;
+  const int q = a.P > 1 ? p % a.P : 0;
+  const float2* src = a.in + p * a.in_plane_stride + (long long)r * a.width;
+  const float2* A = a.wA + (long long)q * a.n;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = t + k * T;
+    v[k] = i < a.n ? cmul(src[i], A[i]) : make_float2(0.f, 0.f);
+  }
+  fft<L, E, -1, 1, SYNC>(v, lds, t, a.tw);
+  // the thread index laundered after the first transform: the spectrum and
+  // the second transform's twiddle loads are then not hoisted above it (held
+  // across it they take the register count to 256)
+  int tl = t;
+  asm volatile("" : "+v"(tl));
+  const float2* H = a.wH + (long long)q * L;
+#pragma unroll
+  for (int k = 0; k < E; ++k) v[k] = cmul(cscale(v[k], a.mid_scale), H[tl + k * T]);
+  fft<L, E, 1, 1, SYNC>(v, lds, tl, a.tw);
+  const float2* Pq = a.wP + (long long)q * a.m;
+  float2* dst = a.out + p * a.out_plane_stride + (long long)r * a.out_line_stride;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int j = tl + k * T - a.out_off;
+    if (j >= 0 && j < a.m) dst[j] = cmul(v[k], Pq[j]);
+  }
+}
+
+template <int L, int E, int C>
+__global__ void __launch_bounds__((L / E) * C) chirp_col_kernel(ChirpConvArgs a) {
+  constexpr int T = L / E;
+  __shared__ float2 lds[lds_span<L, C>()];
+  const int c = threadIdx.x % C;
+  const int ty = threadIdx.x / C;
+  const int nstrips = (a.lines + C - 1) / C;
+  const int nb = gridDim.x;
+  // consecutive strips of a plane on one XCD
+  const int b = ((nb & 7) == 0) ? (int)((blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3)) : (int)blockIdx.x;
+  const int p = b / nstrips;
+  const int col = (b - p * nstrips) * C + c;
+  const bool colok = col < a.lines;
+  const int q = a.P > 1 ? p % a.P : 0;
+  const float2* src = a.in + p * a.in_plane_stride + col;
+  const float2* A = a.wA + (long long)q * a.n;
+  float2 v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int i = ty + k * T;
+    v[k] = (colok && i < a.n) ? cmul(src[(long long)i * a.width], A[i]) : make_float2(0.f, 0.f);
+  }
+  fft<L, E, -1, C, true>(v, lds + c, ty, a.tw);
+  int tyl = ty;  // laundered as in the row pass
+  asm volatile("" : "+v"(tyl));
+  const float2* H = a.wH + (long long)q * L;
+#pragma unroll
+  for (int k = 0; k < E; ++k) v[k] = cmul(cscale(v[k], a.mid_scale), H[tyl + k * T]);
+  fft<L, E, 1, C, true>(v, lds + c, tyl, a.tw);
+  if (!colok) return;
+  const float2* Pq = a.wP + (long long)q * a.m;
+  float2* dst = a.out + p * a.out_plane_stride + col;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int j = tyl + k * T - a.out_off;
+    if (j >= 0 && j < a.m) dst[(long long)j * a.width] = cmul(v[k], Pq[j]);
+  }
+}
+
+// (length, values per thread) of the dedicated row passes and (length,
+// values per thread, columns per workgroup) of the column passes: the
+// shapes of the operator pipeline's passes at these lengths
+#define CXM_CHIRP_ROW_SIZES(X) \
+  X(1024, 16) X(1280, 20) X(1536, 24) X(2048, 16) X(2560, 20) X(3072, 24) X(4096, 16) X(5120, 20)
+#define CXM_CHIRP_COL_SIZES(X) \
+  X(1024, 16, 4) X(1280, 20, 4) X(1536, 24, 4) X(2048, 16, 4) X(2560, 20, 4) X(3072, 24, 4)
+
+// The dedicated pass along `axis` (0: columns, 1: rows) for `planes` planes:
+// 1 if there is none for length L (the caller runs the operator pipeline),
+// else the launch status.
+int chirp_conv_pass(int axis, int L, int planes, const ChirpConvArgs& a, hipStream_t s) {
+  if (option(CXM_OPT_CZT_PASSES) == 1) return 1;
+  const long long lines = (long long)planes * a.lines;
+  if (axis == 1) {
+    if (lines > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "chirp-z row pass grid too large");
+    if (L == 1024 && lines <= 4096) {
+      // few rows: 256 threads per row, four times the waves of the default
+      // shape (the operator pipeline's choice for these launches too)
+      hipLaunchKernelGGL((chirp_row_kernel<1024, 4>), dim3((unsigned)lines), dim3(256), 0, s, a);
+      CXM_LAUNCH_CHECK();
+      return CXM_OK;
+    }
+    switch (L) {
+#define CXM_CR(NN, EE)                                                                          \
+  case NN:                                                                                      \
+    hipLaunchKernelGGL((chirp_row_kernel<NN, EE>), dim3((unsigned)lines), dim3(NN / EE), 0, s, a); \
+    CXM_LAUNCH_CHECK();                                                                         \
+    return CXM_OK;
+      CXM_CHIRP_ROW_SIZES(CXM_CR)
+#undef CXM_CR
+      default:
+        return 1;
+    }
+  }
+  switch (L) {
+#define CXM_CC(NN, EE, CC)                                                                        \
+  case NN: {                                                                                      \
+    const long long blocks = (long long)planes * ((a.lines + CC - 1) / CC);                       \
+    if (blocks > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "chirp-z column pass grid too large"); \
+    hipLaunchKernelGGL((chirp_col_kernel<NN, EE, CC>), dim3((unsigned)blocks), dim3((NN / EE) * CC), \
+                       0, s, a);                                                                  \
+    CXM_LAUNCH_CHECK();                                                                           \
+    return CXM_OK;                                                                                \
+  }
+    CXM_CHIRP_COL_SIZES(CXM_CC)
+#undef CXM_CC
+    default:
+      return 1;
+  }
+}
+
 }  // namespace
 }  // namespace cxm
 
@@ -174,10 +325,30 @@ extern "C" int cxm_czt_axis_c64(void* stream, const void* in, void* out, void* w
     f.scale = f.mid_scale = 1.0f;
     if ((rc = launch_pipe_row(L, f, s))) return rc;
   }
+  const int ax = d->axis == 1 ? 1 : 0;  // VEC index axis: x for rows, y for columns
+  {
+    ChirpConvArgs c;
+    c.in = (const float2*)in;
+    c.out = (float2*)out;
+    c.tw = twiddles_for(L);
+    c.wA = wA;
+    c.wH = wH;
+    c.wP = wP;
+    c.in_plane_stride = (long long)d->height * d->width;
+    c.out_plane_stride = ax ? (long long)d->height * m : (long long)m * d->width;
+    c.lines = ax ? d->height : d->width;
+    c.width = d->width;
+    c.out_line_stride = ax ? m : d->width;
+    c.n = n;
+    c.m = m;
+    c.out_off = n - 1;
+    c.P = P;
+    c.mid_scale = (float)(1.0 / L);
+    if ((rc = chirp_conv_pass(ax, L, d->batch, c, s)) != 1) return rc;
+  }
   PipeArgs a;
   memset(&a, 0, sizeof(a));
   Ops b(a);
-  const int ax = d->axis == 1 ? 1 : 0;  // VEC index axis: x for rows, y for columns
   OpDesc* oA = b.add(a.pre, OP_VEC, P, 0);
   OpDesc* oH = b.add(a.mid, OP_VEC, P, 0);
   OpDesc* oP = b.add(a.post, OP_VEC, P, 0);
@@ -333,15 +504,34 @@ extern "C" int cxm_yu_axis_c64(void* stream, const void* in, void* out, void* wo
     f.scale = f.mid_scale = 1.0f;
     if ((rc = launch_pipe_row(L, f, s))) return rc;
   }
+  const int ax = d->axis == 1 ? 1 : 0;
+  const int sw = (n - 1) / 2;  // "same" window of the full convolution
+  {
+    ChirpConvArgs c;
+    c.in = (const float2*)in;
+    c.out = (float2*)out;
+    c.tw = twiddles_for(L);
+    c.wA = wA;
+    c.wH = wH;
+    c.wP = wP;
+    c.in_plane_stride = c.out_plane_stride = (long long)d->height * d->width;
+    c.lines = ax ? d->height : d->width;
+    c.width = d->width;
+    c.out_line_stride = d->width;
+    c.n = n;
+    c.m = n;
+    c.out_off = sw;
+    c.P = 1;
+    c.mid_scale = (float)(1.0 / L);
+    if ((rc = chirp_conv_pass(ax, L, d->batch, c, s)) != 1) return rc;
+  }
   PipeArgs a;
   memset(&a, 0, sizeof(a));
   Ops b(a);
-  const int ax = d->axis == 1 ? 1 : 0;
   OpDesc* oA = b.add(a.pre, OP_VEC, 1, 0);
   OpDesc* oH = b.add(a.mid, OP_VEC, 1, 0);
   OpDesc* oP = b.add(a.post, OP_VEC, 1, 0);
   if (!oA || !oH || !oP) return fail(CXM_ERR_INVALID, "operator table overflow");
-  const int sw = (n - 1) / 2;  // "same" window of the full convolution
   oA->arr0 = wA; oA->arr_plane_stride = n; oA->arr_cols = n; oA->vec_axis = ax;
   oH->arr0 = wH; oH->arr_plane_stride = L; oH->arr_cols = L; oH->vec_axis = ax;
   oP->arr0 = wP; oP->arr_plane_stride = n; oP->arr_cols = n; oP->vec_axis = ax;

@@ -166,26 +166,29 @@ struct HighNaParams {
   int ny, nx;
 };
 
+// One thread per grid position: the factor depends on the position only, so
+// it is evaluated once (FP64) and applied to that position of every plane.
 __global__ void high_na_defocus_kernel(const float2* __restrict__ in, float2* __restrict__ out,
-                                       long long n_total, HighNaParams c) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_total) return;
+                                       long long n_planes, HighNaParams c) {
   const long long plane = (long long)c.ny * c.nx;
-  const int r = (int)(i % plane);
-  const int y = r / c.nx, x = r - y * c.nx;
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= plane) return;
+  const int y = (int)(r / c.nx), x = (int)(r - (long long)y * c.nx);
   // fftshift(fftfreq(N)) / d = (index - N/2 rounded down) / (N d)
   const double sy = (double)(y - c.ny / 2) * c.fy_step * c.s_scale;
   const double sx = (double)(x - c.nx / 2) * c.fx_step * c.s_scale;
   const double sz2 = 1.0 - c.NA2 * (sy * sy + sx * sx);
   const double sz = sqrt(sz2 > 0.0 ? sz2 : 0.0);
-  float2 v = in[i];
   if (sz == 0.0) {
-    out[i] = make_float2(0.f, 0.f);
+    for (long long p = 0; p < n_planes; ++p) out[p * plane + r] = make_float2(0.f, 0.f);
     return;
   }
   const double ph = c.k * sz * c.f;
   const double re = cos(ph) / sz, im = sin(ph) / sz;
-  out[i] = make_float2((float)(v.x * re - v.y * im), (float)(v.x * im + v.y * re));
+  for (long long p = 0; p < n_planes; ++p) {
+    const float2 v = in[p * plane + r];
+    out[p * plane + r] = make_float2((float)(v.x * re - v.y * im), (float)(v.x * im + v.y * re));
+  }
 }
 }  // namespace cxm
 
@@ -206,11 +209,11 @@ extern "C" int cxm_high_na_defocus_c64(void* stream, const void* in, void* out, 
   c.f = f;
   c.ny = height;
   c.nx = width;
-  const long long tot = n_planes * (long long)height * width;
-  if ((tot + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "too many elements for one launch");
+  const long long plane = (long long)height * width;
+  if ((plane + 255) / 256 > 0x7fffffffLL) return fail(CXM_ERR_INVALID, "plane too large for one launch");
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(high_na_defocus_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s,
-                     (const float2*)in, (float2*)out, tot, c);
+  hipLaunchKernelGGL(high_na_defocus_kernel, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s,
+                     (const float2*)in, (float2*)out, (long long)n_planes, c);
   CXM_LAUNCH_CHECK();
   return CXM_OK;
 }

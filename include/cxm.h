@@ -880,6 +880,12 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         per workgroup (0 = 1, or 2, 4), each summing every
  *                         W-th plane; the W sums are added in LDS.  Equal
  *                         to float32 rounding.
+ *   CXM_OPT_CZT_PASSES    cxm_czt_axis_c64 / cxm_yu_axis_c64: 0 = the
+ *                         chirp multiplies in registers around the two
+ *                         transforms in dedicated row and column passes,
+ *                         where the FFT length has them (rows 1024-5120,
+ *                         columns 1024-3072); 1 = the operator pipeline.
+ *                         Bit-identical.
  *   CXM_OPT_GRAD_GROUPS   cxm_psf4f_bwd: 0 = the default number of plane
  *                         groups of the phase-gradient pass (4), 1..12 =
  *                         that many (each group sums its planes in
@@ -906,6 +912,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_MS_ROW_E 18
 #define CXM_OPT_PSF_PASSES 19
 #define CXM_OPT_PSF_GRAD_WAVES 20
+#define CXM_OPT_CZT_PASSES 21
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

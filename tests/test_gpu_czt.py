@@ -169,3 +169,33 @@ def test_high_na_ff_lens(n, out_shape, odx, f_len, NA, path, monkeypatch):
     assert out.u.shape == ref.shape
     assert err < TOL
     np.testing.assert_allclose(out.dx, dxo)
+
+
+@pytest.mark.parametrize("n,m", [(1024, 512), (700, 300), (1200, 900), (2048, 512), (3000, 40), (600, 500)])
+@pytest.mark.parametrize("axis", [0, 1])
+def test_dedicated_chirp_passes_match_pipeline(n, m, axis):
+    """The dedicated chirp-z row and column passes (CXM_OPT_CZT_PASSES 0;
+    FFT lengths 1024 ... 5120, the chirp multiplies in registers) give the
+    operator pipeline's bits, with two chirp parameter sets; and the Yu
+    scaled-convolution pass likewise."""
+    from chromatix_amd import _lib as L
+
+    shape = (3, n, 20) if axis == 0 else (3, 20, n)
+    x = torch.as_tensor(_x(shape, n + m)).cuda()
+    aa = np.array([np.exp(-0.9j), np.exp(-0.4j)])
+    ww = np.array([np.exp(2j * np.pi / (1.7 * m)), np.exp(2j * np.pi / (2.3 * m))])
+    outs, yus = [], []
+    try:
+        for opt in (0, 1):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_CZT_PASSES, opt))
+            outs.append(uc.czt_axis(x, m, aa, ww, axis, num_param=2, mod_n=n // 2, scale=0.5))
+            yus.append(uc._yu_raw(x, axis, 1.5, 0.01, 0.3, 0.2))
+    finally:
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_CZT_PASSES, 0))
+    print(f"dedicated chirp passes n={n} m={m} axis={axis}: czt {bool(torch.equal(outs[0], outs[1]))}, "
+          f"yu {bool(torch.equal(yus[0], yus[1]))}")
+    assert torch.equal(outs[0], outs[1])
+    assert torch.equal(yus[0], yus[1])
+    ref = oc.czt_direct(x.cpu().numpy()[1].astype(np.complex128), m, aa[1], ww[1], axis=axis)
+    ref = ref * ww[1] ** (-(n // 2) * np.arange(m)).reshape((-1, 1) if axis == 0 else (1, -1)) * aa[1] ** (n // 2) * 0.5
+    assert rel_l2(outs[0].cpu().numpy()[1], ref) < TOL
