@@ -267,11 +267,13 @@ struct PairColArgs {
   int first;                 // sum mode: acc is written, not added to
 };
 
-// Column pass of one plane p: forward transform of the slot's column, the
-// product A K of the pair, inverse transform, in u.
+// Column pass of one plane p up to the spectrum: forward transform of the
+// slot's column and the product A K of the pair, in u (scaled by the
+// plane's 1 / (4 s) and the transform norm); ends with a barrier, so the
+// next transform may write the LDS.
 template <int N, int E, int C>
-CXM_DEV void pair_col_plane(const PairColArgs& a, int p, const float2* colp, bool valid, int c,
-                            int ty, int partner, float2* lds, float2 (&u)[E]) {
+CXM_DEV void pair_col_product(const PairColArgs& a, int p, const float2* colp, bool valid, int c,
+                              int ty, int partner, float2* lds, float2 (&u)[E]) {
   constexpr int T = N / E;
 #pragma unroll
   for (int k = 0; k < E; ++k) {
@@ -295,15 +297,25 @@ CXM_DEV void pair_col_plane(const PairColArgs& a, int p, const float2* colp, boo
     const float dy = 2.0f * (z.x * z.y - z2.x * z2.y);
     u[k] = make_float2(dy * f, -dx * f);
   }
-  block_sync();  // the partner values are read before the inverse transform writes LDS
+  block_sync();  // the partner values are read before the next transform writes LDS
+}
+
+// The whole column pass of one plane p: the product, inverse transform, in u.
+template <int N, int E, int C>
+CXM_DEV void pair_col_plane(const PairColArgs& a, int p, const float2* colp, bool valid, int c,
+                            int ty, int partner, float2* lds, float2 (&u)[E]) {
+  pair_col_product<N, E, C>(a, p, colp, valid, c, ty, partner, lds, u);
   fft<N, E, 1, C, true>(u, lds + c, ty, a.tw);
 }
 
 // C consecutive slots per workgroup (C even: both columns of each pair);
 // slot 2 j + side pairs with its neighbour, pair 0's columns with themselves.
 // One workgroup per (plane, C slots), the kept rows stored in place; SUM:
-// one plane per launch (n_planes 1), its kept rows written to acc (first)
-// or added to it, so that launches in plane order sum the planes in order.
+// one workgroup per C slots of every plane of the launch: the planes'
+// spectral products are added in registers in plane order (the inverse
+// transform is linear), one inverse transform, and the kept rows written
+// to acc (first) or added to it, so that launches in chunk order sum the
+// planes in order.
 template <int N, int E, int C, bool SUM>
 __global__ void __launch_bounds__((N / E) * C)
 pair_col_kernel(PairColArgs a) {
@@ -318,9 +330,28 @@ pair_col_kernel(PairColArgs a) {
   const int partner = (slot >> 1) == 0 ? c : (c ^ 1);
   const long long soff = (long long)(slot >> 2) * a.strip_stride + (slot & 3);
   float2 u[E];
-  const int p = SUM ? 0 : blockIdx.x / nblk;
-  float2* colp = a.buf + (long long)p * a.plane_stride + soff;
-  pair_col_plane<N, E, C>(a, p, colp, valid, c, ty, partner, lds, u);
+  if constexpr (SUM) {
+    float2 s[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) s[k] = make_float2(0.f, 0.f);
+    for (int p = 0; p < a.n_planes; ++p) {
+      // the thread index laundered per plane: the transforms' twiddle loads
+      // are then not hoisted out of the plane loop
+      int tl = ty;
+      asm volatile("" : "+v"(tl));
+      pair_col_product<N, E, C>(a, p, a.buf + (long long)p * a.plane_stride + soff, valid, c, tl, partner,
+                                lds, u);
+#pragma unroll
+      for (int k = 0; k < E; ++k) s[k] = make_float2(s[k].x + u[k].x, s[k].y + u[k].y);
+    }
+    fft<N, E, 1, C, true>(s, lds + c, ty, a.tw);
+#pragma unroll
+    for (int k = 0; k < E; ++k) u[k] = s[k];
+  } else {
+    const int p = blockIdx.x / nblk;
+    pair_col_plane<N, E, C>(a, p, a.buf + (long long)p * a.plane_stride + soff, valid, c, ty, partner, lds,
+                            u);
+  }
   if (!valid) return;
   if constexpr (SUM) {
     float2* accp = a.acc + soff;
@@ -333,6 +364,7 @@ pair_col_kernel(PairColArgs a) {
       }
     }
   } else {
+    float2* colp = a.buf + (long long)(blockIdx.x / nblk) * a.plane_stride + soff;
 #pragma unroll
     for (int k = 0; k < E; ++k) {
       const int yl = ty + k * T - a.out_off;
@@ -650,15 +682,11 @@ extern "C" int cxm_fconv_c64(void* stream, const void* image, const void* kernel
       ca.plane0 = p0;
       ca.norm = (float)(1.0 / ((double)Ly * (double)Lx));
       if (d->sum_batch) {
-        // one launch per plane, in plane order (the sum's order)
+        // one launch per chunk, the planes summed in plane order; chunks in
+        // order (the sum's order)
         ca.acc = acc;
-        ca.n_planes = 1;
-        for (int p = 0; p < kc; ++p) {
-          ca.buf = ws + p * q.plane;
-          ca.plane0 = p0 + p;
-          ca.first = p0 + p == 0;
-          if ((rc = launch_pair(Ly, PAIR_COL, nullptr, &ca, nullptr, s))) return rc;
-        }
+        ca.first = p0 == 0;
+        if ((rc = launch_pair(Ly, PAIR_COL, nullptr, &ca, nullptr, s))) return rc;
         continue;
       }
       if ((rc = launch_pair(Ly, PAIR_COL, nullptr, &ca, nullptr, s))) return rc;
