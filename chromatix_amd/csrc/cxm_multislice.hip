@@ -766,6 +766,126 @@ void fl_arrays(OpDesc* o, const cxm_fluor_desc* d, const FlGeom& g, const float*
   o->t = (float)d->thickness;
 }
 
+// ---- dedicated row passes of the fluorescent sample -------------------------
+// The row passes of the sweep without the operator interpreter (the same
+// arithmetic in the same order, so the bits of the operator pipeline):
+//   forward: row r of W: inverse row FFT; inside the slice window the thin
+//     sample factor exp(i 2 pi dn t / lam) and the fluorescence f exp(i r)
+//     added; forward row FFT, in place;
+//   backward: row r of the spectrum u: u *= Pb (stored back), inverse row
+//     FFT scaled by 1 / Nx, into W (the propagator multiply fused in).
+// Rows shorter than a wavefront run 64 / T rows per wavefront, like the
+// operator pipeline's row pass.
+struct FlRowArgs {
+  float2* W;
+  float2* u;          // backward
+  const float2* Pb;   // backward
+  const float2* tw;
+  int Ny;
+  const float* dn;    // forward: slice arrays, h x w at (pad_y, pad_x)
+  const float* fl;
+  const float* ph;
+  int h, w, pad_y, pad_x;
+  float t, lam;
+  float scale;        // backward: 1 / Nx
+};
+
+template <int N, int E>
+struct FlRowCfg {
+  static constexpr int T = N / E;
+  static constexpr int G = T >= 64 ? 1 : 64 / T;
+};
+
+template <int N, int E, bool BWD>
+__global__ void __launch_bounds__((FlRowCfg<N, E>::G * (N / E))) fl_row_kernel(FlRowArgs a) {
+  constexpr int T = N / E;
+  constexpr int G = FlRowCfg<N, E>::G;
+  constexpr bool SYNC = T > 64;
+  constexpr int SPAN = lds_span<N, 1>();
+  __shared__ float2 lds[G * SPAN];
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x - g * T;
+  const int r = blockIdx.x * G + g;
+  const bool ok = r < a.Ny;
+  const long long rb = (long long)(ok ? r : 0) * N;
+  float2* st = lds + g * SPAN;
+  float2 v[E];
+  if constexpr (BWD) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      const int j = t + k * T;
+      v[k] = cmul(a.u[rb + j], a.Pb[rb + j]);
+      if (ok) a.u[rb + j] = v[k];
+    }
+    fft<N, E, 1, 1, SYNC>(v, st, t, a.tw);
+    if (!ok) return;
+#pragma unroll
+    for (int k = 0; k < E; ++k) a.W[rb + t + k * T] = cscale(v[k], a.scale);
+  } else {
+#pragma unroll
+    for (int k = 0; k < E; ++k) v[k] = a.W[rb + t + k * T];
+    fft<N, E, 1, 1, SYNC>(v, st, t, a.tw);
+    // the thread index laundered: the forward transform's twiddle loads are
+    // not hoisted above the slice's operators
+    int tl = t;
+    asm volatile("" : "+v"(tl));
+    const int ay = r - a.pad_y;
+    if (ok && ay >= 0 && ay < a.h) {
+#pragma unroll
+      for (int k = 0; k < E; ++k) {
+        const int ax = tl + k * T - a.pad_x;
+        if (ax < 0 || ax >= a.w) continue;
+        const long long idx = (long long)ay * a.w + ax;
+        float ang, m, sn, cs;
+        thin_angle(a.dn[idx], 0.0f, a.t, a.lam, &ang, &m);
+        sincos_exact(ang, &sn, &cs);
+        float2 x = cmul(v[k], make_float2(m * cs, m * sn));
+        const float f = a.fl[idx];
+        sincos_exact(a.ph[idx], &sn, &cs);
+        v[k] = make_float2(__fadd_rn(__fmul_rn(f, cs), x.x), __fadd_rn(__fmul_rn(f, sn), x.y));
+      }
+    }
+    fft<N, E, -1, 1, SYNC>(v, st, tl, a.tw);
+    if (!ok) return;
+#pragma unroll
+    for (int k = 0; k < E; ++k) a.W[rb + tl + k * T] = v[k];
+  }
+}
+
+// (row length, values per thread): the operator pipeline's row shapes for
+// one plane (1024: 256 threads per row)
+#define CXM_FL_ROW_SIZES(X) X(512, 16) X(640, 20) X(1024, 4) X(1280, 20) X(2048, 16)
+
+bool fl_dedicated(int n) {
+  if (!ms_dedicated()) return false;
+  switch (n) {
+#define CXM_FL_CASE(N, E) case N:
+    CXM_FL_ROW_SIZES(CXM_FL_CASE)
+#undef CXM_FL_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+int launch_fl_row(int n, bool bwd, const FlRowArgs& a, hipStream_t s) {
+  switch (n) {
+#define CXM_FL_LAUNCH(N, E)                                                                      \
+  case N: {                                                                                      \
+    constexpr int G = FlRowCfg<N, E>::G;                                                         \
+    const dim3 grid((unsigned)((a.Ny + G - 1) / G)), block((unsigned)(G * (N / E)));             \
+    if (bwd) hipLaunchKernelGGL((fl_row_kernel<N, E, true>), grid, block, 0, s, a);              \
+    else hipLaunchKernelGGL((fl_row_kernel<N, E, false>), grid, block, 0, s, a);                 \
+    CXM_LAUNCH_CHECK();                                                                          \
+    return CXM_OK;                                                                               \
+  }
+    CXM_FL_ROW_SIZES(CXM_FL_LAUNCH)
+#undef CXM_FL_LAUNCH
+    default:
+      return fail(CXM_ERR_UNSUPPORTED, "no dedicated fluorescent row pass for " + std::to_string(n));
+  }
+}
+
 // Thin sample and fluorescence of slice i as operators of list L.
 int fl_slice_ops(Ops& b, OpList& L, const cxm_fluor_desc* d, const FlGeom& g, const float* dn,
                  const float* fl, const float* ph, int i) {
@@ -870,6 +990,22 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
     Pbw = Pb;
   }
   const float inv_n = (float)(1.0 / ((double)g.Ny * (double)g.Nx));
+  const bool ded = fl_dedicated(g.Nx);
+  FlRowArgs fr;
+  memset(&fr, 0, sizeof(fr));
+  fr.W = W;
+  fr.u = u;
+  fr.Pb = Pbw;
+  fr.tw = twiddles_for(g.Nx);
+  fr.Ny = g.Ny;
+  fr.h = g.h;
+  fr.w = g.w;
+  fr.pad_y = d->pad_y;
+  fr.pad_x = d->pad_x;
+  fr.t = (float)d->thickness;
+  fr.lam = (float)d->wavelength;
+  fr.scale = (float)(1.0 / (double)g.Nx);
+  const long long sl = (long long)g.h * g.w;
   // forward, slice 0: thin sample + fluorescence, row FFT (state -> W)
   {
     PipeArgs a;
@@ -902,6 +1038,13 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
       ca.scale = inv_n;
       if ((rc = launch_col_prop(g.Ny, ca, s))) return rc;
       // row IFFT, slice i+1's thin sample + fluorescence, row FFT, in place
+      if (ded) {
+        fr.dn = dn + (i + 1) * sl;
+        fr.fl = fluorescence + (i + 1) * sl;
+        fr.ph = phases + (i + 1) * sl;
+        if ((rc = launch_fl_row(g.Nx, false, fr, s))) return rc;
+        continue;
+      }
       PipeArgs a;
       memset(&a, 0, sizeof(a));
       Ops b(a);
@@ -930,17 +1073,21 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
     }
   }
   // backward sweep
-  const long long sl = (long long)g.h * g.w;
   for (int i = 0; i < g.D; ++i) {
-    if ((rc = cxm_cmul_c64((void*)s, u, Pbw, u, 1, g.Ny, g.Nx, 1, 0))) return rc;
-    PipeArgs r;
-    memset(&r, 0, sizeof(r));
-    fl_row_common(r, g);
-    r.in = u;
-    r.out = W;
-    r.seq = SEQ_I;
-    r.scale = (float)(1.0 / (double)g.Nx);
-    if ((rc = launch_pipe_row(g.Nx, r, s))) return rc;
+    if (ded) {
+      // u *= Pb and the inverse row FFT in one pass
+      if ((rc = launch_fl_row(g.Nx, true, fr, s))) return rc;
+    } else {
+      if ((rc = cxm_cmul_c64((void*)s, u, Pbw, u, 1, g.Ny, g.Nx, 1, 0))) return rc;
+      PipeArgs r;
+      memset(&r, 0, sizeof(r));
+      fl_row_common(r, g);
+      r.in = u;
+      r.out = W;
+      r.seq = SEQ_I;
+      r.scale = (float)(1.0 / (double)g.Nx);
+      if ((rc = launch_pipe_row(g.Nx, r, s))) return rc;
+    }
     PipeArgs c;
     memset(&c, 0, sizeof(c));
     fl_col_common(c, g);

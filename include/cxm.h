@@ -866,6 +866,12 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         2048); 1 = the operator-pipeline row passes with H
  *                         evaluated per bin; 2 = the dedicated row passes with
  *                         H evaluated per bin.  Equal to float32 rounding.
+ *                         cxm_fluor_multislice_sample_c64: 0 or 2 = the
+ *                         slice's row pass (thin sample and fluorescence) and
+ *                         the backward sweep's propagator multiply with its
+ *                         inverse row FFT as dedicated kernels where the row
+ *                         length has them (512, 640, 1024, 1280, 2048);
+ *                         1 = the operator pipeline.  Bit-identical.
  *   CXM_OPT_MS_ROW_E      values per thread of the dedicated 1024-point slice
  *                         row pass: 0 = 4, or 2, 8, 16
  *   CXM_OPT_PSF_PASSES    cxm_psf4f_fwd / _bwd / _value_and_grad: 0 = the

@@ -288,3 +288,32 @@ def test_dedicated_passes_match_pipeline_passes(pm, shape, pad):
     e = (rel_l2(outs[0], outs[1]), rel_l2(gdns[0], gdns[1]), rel_l2(gabs[0], gabs[1]))
     print(f"dedicated vs pipeline {pm} {shape} pad {pad}: fwd {e[0]:.2e} dn {e[1]:.2e} abs {e[2]:.2e}")
     assert max(e) < 1e-6
+
+
+@pytest.mark.parametrize("H,pad", [(448, 32), (512, 64), (1024, 0)])
+def test_dedicated_fluorescent_passes_match_pipeline(H, pad):
+    """The dedicated fluorescent row passes (the slice's thin sample and
+    fluorescence between the row transforms; the backward propagator
+    multiply fused into the inverse row transform) give the operator
+    pipeline's bits (CXM_OPT_MS_PASSES 1), at padded lengths 512, 640 and
+    1024, two Monte-Carlo samples; and the 512 case matches the oracle."""
+    from chromatix_amd import _lib as L
+
+    D, samples, t = 3, 2, 1.5
+    u, fl, dn, rng = _fluor_case(H, D, samples)
+    ph = rng.uniform(0, 2 * np.pi, (samples, D, H, H)).astype(np.float32)
+    outs = []
+    try:
+        for opt in (0, 1):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_MS_PASSES, opt))
+            f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, 0.532)
+            outs.append(cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, t, pad, num_samples=samples,
+                                                               random_phases=ph, phase_mode="accurate"))
+    finally:
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_MS_PASSES, 0))
+    print(f"dedicated vs pipeline fluorescent {H + 2 * pad}: {bool(torch.equal(outs[0], outs[1]))}")
+    assert torch.equal(outs[0], outs[1])
+    if H + 2 * pad == 512:
+        ref = om.fluorescent_multislice(u, fl.astype(np.float64), dn.astype(np.float64), (0.3, 0.3),
+                                        0.532, 1.33, t, pad, ph.astype(np.float64))
+        assert rel_l2(outs[0].cpu().numpy(), ref) < TOL
