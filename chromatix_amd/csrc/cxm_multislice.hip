@@ -720,9 +720,10 @@ extern "C" int cxm_multislice_bwd(void* stream, const void* in, const float* dn,
 // i's row pass applies the thin sample and adds f_i exp(i r_i) between the
 // row IFFT and FFT, the column pass multiplies by P(+t); the last column
 // pass stops after the column FFT and the multiply, which is fft2 of the
-// field after the last slice.  Backward per step: U *= P(-t) in place, row
-// IFFT, column IFFT storing |.|^2 of the cropped window accumulated into
-// the slice's intensity plane.
+// field after the last slice.  Backward step i: U *= P(-t), row IFFT,
+// column IFFT storing |.|^2 of the cropped window accumulated into
+// intensity plane depth-1-i; the steps' multiplies form an elementwise
+// chain, so the transforms of many steps run as one batched pass.
 // ---------------------------------------------------------------------------
 namespace cxm {
 namespace {
@@ -769,25 +770,22 @@ void fl_arrays(OpDesc* o, const cxm_fluor_desc* d, const FlGeom& g, const float*
 // ---- dedicated row passes of the fluorescent sample -------------------------
 // The row passes of the sweep without the operator interpreter (the same
 // arithmetic in the same order, so the bits of the operator pipeline):
-//   forward: row r of W: inverse row FFT; inside the slice window the thin
-//     sample factor exp(i 2 pi dn t / lam) and the fluorescence f exp(i r)
-//     added; forward row FFT, in place;
-//   backward: row r of the spectrum u: u *= Pb (stored back), inverse row
-//     FFT scaled by 1 / Nx, into W (the propagator multiply fused in).
+//   row r of W: inverse row FFT; inside the slice window the thin sample
+//   factor exp(i 2 pi dn t / lam) and the fluorescence f exp(i r) added;
+//   forward row FFT, in place.  Slice 0 (FIRST) reads the spatial field
+//   and skips the inverse transform.
 // Rows shorter than a wavefront run 64 / T rows per wavefront, like the
 // operator pipeline's row pass.
 struct FlRowArgs {
   float2* W;
-  float2* u;          // backward
-  const float2* Pb;   // backward
+  const float2* src;  // FIRST: the spatial field of slice 0
   const float2* tw;
   int Ny;
-  const float* dn;    // forward: slice arrays, h x w at (pad_y, pad_x)
+  const float* dn;    // slice arrays, h x w at (pad_y, pad_x)
   const float* fl;
   const float* ph;
   int h, w, pad_y, pad_x;
   float t, lam;
-  float scale;        // backward: 1 / Nx
 };
 
 template <int N, int E>
@@ -796,7 +794,7 @@ struct FlRowCfg {
   static constexpr int G = T >= 64 ? 1 : 64 / T;
 };
 
-template <int N, int E, bool BWD>
+template <int N, int E, bool FIRST>
 __global__ void __launch_bounds__((FlRowCfg<N, E>::G * (N / E))) fl_row_kernel(FlRowArgs a) {
   constexpr int T = N / E;
   constexpr int G = FlRowCfg<N, E>::G;
@@ -810,51 +808,53 @@ __global__ void __launch_bounds__((FlRowCfg<N, E>::G * (N / E))) fl_row_kernel(F
   const long long rb = (long long)(ok ? r : 0) * N;
   float2* st = lds + g * SPAN;
   float2 v[E];
-  if constexpr (BWD) {
+#pragma unroll
+  for (int k = 0; k < E; ++k) v[k] = (FIRST ? a.src : a.W)[rb + t + k * T];
+  // the slice arrays loaded before the first transform, which hides their
+  // latency
+  const int ay = r - a.pad_y;
+  const bool rin = ok && ay >= 0 && ay < a.h;
+  float sd[E], sf[E], sp[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int ax = t + k * T - a.pad_x;
+    const bool in = rin && ax >= 0 && ax < a.w;
+    const long long idx = in ? (long long)ay * a.w + ax : 0;
+    sd[k] = in ? a.dn[idx] : 0.0f;
+    sf[k] = in ? a.fl[idx] : 0.0f;
+    sp[k] = in ? a.ph[idx] : 0.0f;
+  }
+  if constexpr (!FIRST) fft<N, E, 1, 1, SYNC>(v, st, t, a.tw);
+  // the thread index laundered: the forward transform's twiddle loads are
+  // not hoisted above the slice's operators
+  int tl = t;
+  asm volatile("" : "+v"(tl));
+  if (rin) {
 #pragma unroll
     for (int k = 0; k < E; ++k) {
-      const int j = t + k * T;
-      v[k] = cmul(a.u[rb + j], a.Pb[rb + j]);
-      if (ok) a.u[rb + j] = v[k];
+      const int ax = tl + k * T - a.pad_x;
+      if (ax < 0 || ax >= a.w) continue;
+      float ang, m, sn, cs;
+      thin_angle(sd[k], 0.0f, a.t, a.lam, &ang, &m);
+      sincos_exact(ang, &sn, &cs);
+      float2 x = cmul(v[k], make_float2(m * cs, m * sn));
+      const float f = sf[k];
+      sincos_exact(sp[k], &sn, &cs);
+      v[k] = make_float2(__fadd_rn(__fmul_rn(f, cs), x.x), __fadd_rn(__fmul_rn(f, sn), x.y));
     }
-    fft<N, E, 1, 1, SYNC>(v, st, t, a.tw);
-    if (!ok) return;
-#pragma unroll
-    for (int k = 0; k < E; ++k) a.W[rb + t + k * T] = cscale(v[k], a.scale);
-  } else {
-#pragma unroll
-    for (int k = 0; k < E; ++k) v[k] = a.W[rb + t + k * T];
-    fft<N, E, 1, 1, SYNC>(v, st, t, a.tw);
-    // the thread index laundered: the forward transform's twiddle loads are
-    // not hoisted above the slice's operators
-    int tl = t;
-    asm volatile("" : "+v"(tl));
-    const int ay = r - a.pad_y;
-    if (ok && ay >= 0 && ay < a.h) {
-#pragma unroll
-      for (int k = 0; k < E; ++k) {
-        const int ax = tl + k * T - a.pad_x;
-        if (ax < 0 || ax >= a.w) continue;
-        const long long idx = (long long)ay * a.w + ax;
-        float ang, m, sn, cs;
-        thin_angle(a.dn[idx], 0.0f, a.t, a.lam, &ang, &m);
-        sincos_exact(ang, &sn, &cs);
-        float2 x = cmul(v[k], make_float2(m * cs, m * sn));
-        const float f = a.fl[idx];
-        sincos_exact(a.ph[idx], &sn, &cs);
-        v[k] = make_float2(__fadd_rn(__fmul_rn(f, cs), x.x), __fadd_rn(__fmul_rn(f, sn), x.y));
-      }
-    }
-    fft<N, E, -1, 1, SYNC>(v, st, tl, a.tw);
-    if (!ok) return;
-#pragma unroll
-    for (int k = 0; k < E; ++k) a.W[rb + tl + k * T] = v[k];
   }
+  fft<N, E, -1, 1, SYNC>(v, st, tl, a.tw);
+  if (!ok) return;
+#pragma unroll
+  for (int k = 0; k < E; ++k) a.W[rb + tl + k * T] = v[k];
 }
 
 // (row length, values per thread): the operator pipeline's row shapes for
-// one plane (1024: 256 threads per row)
-#define CXM_FL_ROW_SIZES(X) X(512, 16) X(640, 20) X(1024, 4) X(1280, 20) X(2048, 16)
+// one plane (1024: 256 threads per row), except 640 (10 values per thread,
+// one 64-thread row per wavefront: twice the pipeline's waves, 97 -> 114
+// Monte-Carlo stacks/s at 512^2 + pad 64; CXM_OPT_FL_ROW_E 20 gives the
+// pipeline's shape and bits)
+#define CXM_FL_ROW_SIZES(X) X(512, 16) X(640, 10) X(1024, 4) X(1280, 20) X(2048, 16)
 
 bool fl_dedicated(int n) {
   if (!ms_dedicated()) return false;
@@ -868,22 +868,140 @@ bool fl_dedicated(int n) {
   }
 }
 
-int launch_fl_row(int n, bool bwd, const FlRowArgs& a, hipStream_t s) {
+template <int N, int E>
+int launch_fl_row_t(bool first, const FlRowArgs& a, hipStream_t s) {
+  constexpr int G = FlRowCfg<N, E>::G;
+  const dim3 grid((unsigned)((a.Ny + G - 1) / G)), block((unsigned)(G * (N / E)));
+  if (first) hipLaunchKernelGGL((fl_row_kernel<N, E, true>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((fl_row_kernel<N, E, false>), grid, block, 0, s, a);
+  CXM_LAUNCH_CHECK();
+  return CXM_OK;
+}
+
+// first: slice 0 from the spatial field a.src (no inverse transform)
+int launch_fl_row(int n, bool first, const FlRowArgs& a, hipStream_t s) {
+  if (n == 640 && option(CXM_OPT_FL_ROW_E) == 20) return launch_fl_row_t<640, 20>(first, a, s);
   switch (n) {
-#define CXM_FL_LAUNCH(N, E)                                                                      \
-  case N: {                                                                                      \
-    constexpr int G = FlRowCfg<N, E>::G;                                                         \
-    const dim3 grid((unsigned)((a.Ny + G - 1) / G)), block((unsigned)(G * (N / E)));             \
-    if (bwd) hipLaunchKernelGGL((fl_row_kernel<N, E, true>), grid, block, 0, s, a);              \
-    else hipLaunchKernelGGL((fl_row_kernel<N, E, false>), grid, block, 0, s, a);                 \
-    CXM_LAUNCH_CHECK();                                                                          \
-    return CXM_OK;                                                                               \
-  }
+#define CXM_FL_LAUNCH(N, E) \
+  case N: return launch_fl_row_t<N, E>(first, a, s);
     CXM_FL_ROW_SIZES(CXM_FL_LAUNCH)
 #undef CXM_FL_LAUNCH
     default:
       return fail(CXM_ERR_UNSUPPORTED, "no dedicated fluorescent row pass for " + std::to_string(n));
   }
+}
+
+// ---- backward sweep, batched over its steps --------------------------------
+// U_{i+1} = U_i * Pb is an elementwise chain, so the steps' inverse
+// transforms do not depend on one another.  Workgroup (row group, chunk c)
+// loads its rows of U_{i0} and Pb, advances them through c * chunk
+// multiplies in registers (the products of the step-by-step sweep, in its
+// order, so the same bits), then for each of its steps multiplies once
+// more, inverse-transforms a copy along the row and stores it scaled by
+// 1 / Nx in the step's workspace plane (planes in reverse step order, so
+// plane p belongs to intensity plane depth - i1 + p).  The last chunk
+// stores U_{i1} in `dst` (a buffer other than `src`: every chunk of a row
+// reads `src`).
+struct FlBwdArgs {
+  const float2* src;
+  float2* dst;
+  const float2* Pb;
+  float2* W;
+  const float2* tw;
+  int Ny;
+  int steps;          // i1 - i0
+  int chunk;          // steps per workgroup
+  long long plane;    // Ny * Nx
+  float scale;        // 1 / Nx
+};
+
+template <int N, int E>
+__global__ void __launch_bounds__((FlRowCfg<N, E>::G * (N / E))) fl_bwd_rows_kernel(FlBwdArgs a) {
+  constexpr int T = N / E;
+  constexpr int G = FlRowCfg<N, E>::G;
+  constexpr bool SYNC = T > 64;
+  constexpr int SPAN = lds_span<N, 1>();
+  __shared__ float2 lds[G * SPAN];
+  const int g = threadIdx.x / T;
+  const int t = threadIdx.x - g * T;
+  const int r = blockIdx.x * G + g;
+  const bool ok = r < a.Ny;
+  const long long rb = (long long)(ok ? r : 0) * N;
+  float2* st = lds + g * SPAN;
+  const int i0 = blockIdx.y * a.chunk;
+  const int i1 = min(i0 + a.chunk, a.steps);
+  float2 u[E], p[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    u[k] = a.src[rb + t + k * T];
+    p[k] = a.Pb[rb + t + k * T];
+  }
+  for (int i = 0; i < i0; ++i) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) u[k] = cmul(u[k], p[k]);
+  }
+  for (int i = i0; i < i1; ++i) {
+    // the thread index laundered per step: twiddle loads stay in the loop
+    int tl = t;
+    asm volatile("" : "+v"(tl));
+    float2 v[E];
+#pragma unroll
+    for (int k = 0; k < E; ++k) {
+      u[k] = cmul(u[k], p[k]);
+      v[k] = u[k];
+    }
+    fft<N, E, 1, 1, SYNC>(v, st, tl, a.tw);
+    if (ok) {
+      float2* w = a.W + (long long)(a.steps - 1 - i) * a.plane + rb;
+#pragma unroll
+      for (int k = 0; k < E; ++k) w[tl + k * T] = cscale(v[k], a.scale);
+    }
+  }
+  if (ok && i1 == a.steps) {
+#pragma unroll
+    for (int k = 0; k < E; ++k) a.dst[rb + t + k * T] = u[k];
+  }
+}
+
+// Steps per workgroup of the batched backward rows (CXM_OPT_FL_BWD_CHUNK).
+int fl_bwd_chunk() {
+  const int c = option(CXM_OPT_FL_BWD_CHUNK);
+  return c > 0 ? c : 8;
+}
+
+int launch_fl_bwd_rows(int n, const FlBwdArgs& a, hipStream_t s) {
+  const unsigned chunks = (unsigned)((a.steps + a.chunk - 1) / a.chunk);
+  if (n == 640 && option(CXM_OPT_FL_ROW_E) == 20) {
+    constexpr int G = FlRowCfg<640, 20>::G;
+    const dim3 grid((unsigned)((a.Ny + G - 1) / G), chunks), block((unsigned)(G * 32));
+    hipLaunchKernelGGL((fl_bwd_rows_kernel<640, 20>), grid, block, 0, s, a);
+    CXM_LAUNCH_CHECK();
+    return CXM_OK;
+  }
+  switch (n) {
+#define CXM_FL_BWD_LAUNCH(N, E)                                                                  \
+  case N: {                                                                                      \
+    constexpr int G = FlRowCfg<N, E>::G;                                                         \
+    const dim3 grid((unsigned)((a.Ny + G - 1) / G), chunks), block((unsigned)(G * (N / E)));    \
+    hipLaunchKernelGGL((fl_bwd_rows_kernel<N, E>), grid, block, 0, s, a);                        \
+    CXM_LAUNCH_CHECK();                                                                          \
+    return CXM_OK;                                                                               \
+  }
+    CXM_FL_ROW_SIZES(CXM_FL_BWD_LAUNCH)
+#undef CXM_FL_BWD_LAUNCH
+    default:
+      return fail(CXM_ERR_UNSUPPORTED, "no batched backward row pass for " + std::to_string(n));
+  }
+}
+
+// Steps of the backward sweep per batched launch pair: the workspace holds
+// that many planes, at most 512 MiB of them (at least one), or the number
+// CXM_OPT_FL_BWD_BATCH sets.
+int fl_bwd_batch(const FlGeom& g) {
+  const int o = option(CXM_OPT_FL_BWD_BATCH);
+  if (o > 0) return std::min(g.D, o);
+  const long long cap = (512ll << 20) / (g.plane * (long long)sizeof(float2));
+  return (int)std::max(1ll, std::min((long long)g.D, cap));
 }
 
 // Thin sample and fluorescence of slice i as operators of list L.
@@ -958,7 +1076,9 @@ int fl_propagator(const cxm_fluor_desc* d, const FlGeom& g, double z, void* out,
 extern "C" size_t cxm_fluor_multislice_workspace_size(const cxm_fluor_desc* d) {
   FlGeom g;
   if (fl_geom(d, g)) return 0;
-  return (size_t)3 * g.plane * sizeof(float2);  // row-transformed field, P(+t), P(-t)
+  // P(+t), P(-t), the carried spectrum's second buffer, then the row-transformed
+  // field (forward) or the backward steps' planes
+  return (size_t)(3 + fl_bwd_batch(g)) * g.plane * sizeof(float2);
 }
 
 extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const float* fluorescence,
@@ -970,15 +1090,17 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
   if (rc) return rc;
   if (!state || !fluorescence || !dn || !phases || !intensity)
     return fail(CXM_ERR_INVALID, "null buffer");
-  const size_t need = (size_t)3 * g.plane * sizeof(float2);
+  const int B = fl_bwd_batch(g);
+  const size_t need = (size_t)(3 + B) * g.plane * sizeof(float2);
   if (!workspace || workspace_bytes < need)
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   if ((rc = ensure_init())) return rc;
   hipStream_t s = (hipStream_t)stream;
   float2* u = (float2*)state;
-  float2* W = (float2*)workspace;
-  float2* Pf = W + g.plane;
+  float2* Pf = (float2*)workspace;
   float2* Pb = Pf + g.plane;
+  float2* U2 = Pb + g.plane;
+  float2* W = U2 + g.plane;
   const float2* Pfw = (const float2*)d->prop_fwd;
   const float2* Pbw = (const float2*)d->prop_bwd;
   if (!Pfw) {
@@ -994,8 +1116,6 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
   FlRowArgs fr;
   memset(&fr, 0, sizeof(fr));
   fr.W = W;
-  fr.u = u;
-  fr.Pb = Pbw;
   fr.tw = twiddles_for(g.Nx);
   fr.Ny = g.Ny;
   fr.h = g.h;
@@ -1004,10 +1124,15 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
   fr.pad_x = d->pad_x;
   fr.t = (float)d->thickness;
   fr.lam = (float)d->wavelength;
-  fr.scale = (float)(1.0 / (double)g.Nx);
   const long long sl = (long long)g.h * g.w;
   // forward, slice 0: thin sample + fluorescence, row FFT (state -> W)
-  {
+  if (ded) {
+    fr.src = u;
+    fr.dn = dn;
+    fr.fl = fluorescence;
+    fr.ph = phases;
+    if ((rc = launch_fl_row(g.Nx, true, fr, s))) return rc;
+  } else {
     PipeArgs a;
     memset(&a, 0, sizeof(a));
     Ops b(a);
@@ -1072,27 +1197,15 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
       if ((rc = launch_pipe_col(g.Ny, a, s))) return rc;
     }
   }
-  // backward sweep
-  for (int i = 0; i < g.D; ++i) {
-    if (ded) {
-      // u *= Pb and the inverse row FFT in one pass
-      if ((rc = launch_fl_row(g.Nx, true, fr, s))) return rc;
-    } else {
-      if ((rc = cxm_cmul_c64((void*)s, u, Pbw, u, 1, g.Ny, g.Nx, 1, 0))) return rc;
-      PipeArgs r;
-      memset(&r, 0, sizeof(r));
-      fl_row_common(r, g);
-      r.in = u;
-      r.out = W;
-      r.seq = SEQ_I;
-      r.scale = (float)(1.0 / (double)g.Nx);
-      if ((rc = launch_pipe_row(g.Nx, r, s))) return rc;
-    }
+  // backward sweep: the inverse column FFT of `planes` workspace planes with
+  // |.|^2 of the cropped window accumulated into intensity planes from `first`
+  auto bwd_cols = [&](int first, int planes) {
     PipeArgs c;
     memset(&c, 0, sizeof(c));
     fl_col_common(c, g);
+    c.n_planes = planes;
     c.in = W;
-    c.out = intensity + (long long)(g.D - 1 - i) * sl;
+    c.out = intensity + (long long)first * sl;
     c.seq = SEQ_I;
     c.scale = (float)(1.0 / (double)g.Ny);
     c.intensity = 2;
@@ -1103,7 +1216,47 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
     c.out_len = g.h;
     c.out_line_off = d->pad_x;
     c.out_line_len = g.w;
-    if ((rc = launch_pipe_col(g.Ny, c, s))) return rc;
+    return launch_pipe_col(g.Ny, c, s);
+  };
+  if (ded) {
+    // B steps per launch pair: every step's U *= Pb and inverse row FFT in
+    // one batched pass, then the column pass over the B planes
+    FlBwdArgs ba;
+    memset(&ba, 0, sizeof(ba));
+    ba.Pb = Pbw;
+    ba.W = W;
+    ba.tw = twiddles_for(g.Nx);
+    ba.Ny = g.Ny;
+    ba.chunk = fl_bwd_chunk();
+    ba.plane = g.plane;
+    ba.scale = (float)(1.0 / (double)g.Nx);
+    float2* cur = u;  // holds the carried spectrum
+    for (int i0 = 0; i0 < g.D; i0 += B) {
+      float2* next = cur == u ? U2 : u;
+      ba.src = cur;
+      ba.dst = next;
+      ba.steps = std::min(B, g.D - i0);
+      if ((rc = launch_fl_bwd_rows(g.Nx, ba, s))) return rc;
+      if ((rc = bwd_cols(g.D - i0 - ba.steps, ba.steps))) return rc;
+      cur = next;
+    }
+    // the carried spectrum back in the caller's state
+    if (cur != u && hipMemcpyAsync(u, cur, g.plane * sizeof(float2), hipMemcpyDeviceToDevice,
+                                   s) != hipSuccess)
+      return fail(CXM_ERR_HIP, "copy of the carried spectrum failed");
+    return CXM_OK;
+  }
+  for (int i = 0; i < g.D; ++i) {
+    if ((rc = cxm_cmul_c64((void*)s, u, Pbw, u, 1, g.Ny, g.Nx, 1, 0))) return rc;
+    PipeArgs r;
+    memset(&r, 0, sizeof(r));
+    fl_row_common(r, g);
+    r.in = u;
+    r.out = W;
+    r.seq = SEQ_I;
+    r.scale = (float)(1.0 / (double)g.Nx);
+    if ((rc = launch_pipe_row(g.Nx, r, s))) return rc;
+    if ((rc = bwd_cols(g.D - 1 - i, 1))) return rc;
   }
   return CXM_OK;
 }

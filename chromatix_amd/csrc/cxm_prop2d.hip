@@ -54,22 +54,29 @@ int launch_col_variant_1024(int v, const ColArgs& a, hipStream_t s);
 int launch_col_variant_2048(int v, const ColArgs& a, hipStream_t s);
 int col_variant_width_1024(int v);
 int col_variant_width_2048(int v);
+int launch_col_variant_640(int v, const ColArgs& a, hipStream_t s);
+int col_variant_width_640(int v);
 
 // Launch-shape variant of the column pass: CXM_OPT_COL_VARIANT, or for a
-// single plane or a few (the multislice loop) 8 values per thread and 4
-// columns, twice the waves of the batched default (20 -> 18 us per plane).
+// single plane or a few (the multislice loops) twice the waves of the
+// batched default: 1024 points 8 values per thread and 4 columns (20 -> 18
+// us per plane), 640 points 10 and 4 (16.7 -> 14.2 us).
 int col_variant(int N, int n_planes) {
   int v = option(CXM_OPT_COL_VARIANT);
   if (v == 0 && N == 1024 && n_planes <= 4) v = 2;
+  if (v == 0 && N == 640 && n_planes <= 4) v = 1;
   if (N == 1024 && col_variant_width_1024(v) > 0) return v;
   if (N == 2048 && col_variant_width_2048(v) > 0) return v;
+  if (N == 640 && col_variant_width_640(v) > 0) return v;
   return 0;
 }
 
 // Columns per column-pass workgroup (the strip width C) for this launch.
 int col_width(int N, int n_planes) {
   const int v = col_variant(N, n_planes);
-  if (v > 0) return N == 1024 ? col_variant_width_1024(v) : col_variant_width_2048(v);
+  if (v > 0)
+    return N == 1024 ? col_variant_width_1024(v)
+                     : N == 2048 ? col_variant_width_2048(v) : col_variant_width_640(v);
   switch (N) {
 #define CXM_CASE(NN, ER, EC, CC) case NN: return CC;
     CXM_FOR_EACH_SIZE(CXM_CASE)
@@ -94,6 +101,7 @@ int launch_col_prop(int N, const ColArgs& a, hipStream_t s) {
   const int v = col_variant(N, a.n_planes);
   if (v > 0 && N == 1024 && (rc = launch_col_variant_1024(v, a, s)) >= 0) return rc;
   if (v > 0 && N == 2048 && (rc = launch_col_variant_2048(v, a, s)) >= 0) return rc;
+  if (v > 0 && N == 640 && (rc = launch_col_variant_640(v, a, s)) >= 0) return rc;
   if ((rc = launch_col_g0(N, a, s)) >= 0) return rc;
   if ((rc = launch_col_g1(N, a, s)) >= 0) return rc;
   if ((rc = launch_col_g2(N, a, s)) >= 0) return rc;

@@ -329,7 +329,8 @@ def fluorescent_multislice_thick_sample(field: Field, fluorescence_stack, dn_sta
         if ph_all is not None:
             ph = ph_all[smp].resolve_conj().contiguous()
         else:
-            ph = torch.rand((D, H, W), generator=gen, device=dev, dtype=torch.float32) * (2 * np.pi)
+            ph = torch.empty((D, H, W), device=dev, dtype=torch.float32).uniform_(0.0, 2 * np.pi,
+                                                                                   generator=gen)
         L.check(L.lib().cxm_fluor_multislice_sample_c64(stream, state.data_ptr(), fl.data_ptr(),
                                                          dn.data_ptr(), ph.data_ptr(), out.data_ptr(),
                                                          ws.data_ptr(), nbytes, ctypes.byref(d)))

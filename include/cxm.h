@@ -595,7 +595,9 @@ int cxm_shot_noise_approx_f32(void* stream, const float* image, const float* nor
  * P(z): the angular-spectrum propagator of the padded grid (remove_evanescent,
  * bandlimit as given), or the caller's prop_fwd / prop_bwd (Ny x Nx, FFT
  * order, device) when non-null.  intensity: depth x height x width float32,
- * accumulated.
+ * accumulated.  The workspace (cxm_fluor_multislice_workspace_size bytes)
+ * holds 3 + B padded planes, B = min(depth, planes in 512 MiB): the backward
+ * sweep transforms B steps per batched launch.
  */
 typedef struct cxm_fluor_desc {
   int32_t height, width;      /* unpadded field and slices */
@@ -793,8 +795,9 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  * Launch-shape tuning (0 = the built-in choice, which for 1024-point
  * passes over at most a few planes picks a shape with more waves).  No
  * counterpart in the reference (performance tuning).
- *   CXM_OPT_COL_VARIANT   alternative column-pass shape for 1024- and
- *                         2048-point columns
+ *   CXM_OPT_COL_VARIANT   alternative column-pass shape for 640-, 1024- and
+ *                         2048-point columns (640 and 1024: a narrower
+ *                         default for up to 4 planes)
  *   CXM_OPT_CHUNK_PLANES  planes per chunk of the propagation passes
  *                         (default: a workspace of at most 128 MiB)
  *   CXM_OPT_ROW_VARIANT   alternative row-pass shape for 1024-point rows
@@ -868,10 +871,14 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         H evaluated per bin.  Equal to float32 rounding.
  *                         cxm_fluor_multislice_sample_c64: 0 or 2 = the
  *                         slice's row pass (thin sample and fluorescence) and
- *                         the backward sweep's propagator multiply with its
- *                         inverse row FFT as dedicated kernels where the row
- *                         length has them (512, 640, 1024, 1280, 2048);
- *                         1 = the operator pipeline.  Bit-identical.
+ *                         the backward sweep batched over its steps (each
+ *                         workgroup replays the propagator multiplies in
+ *                         registers, then inverse-transforms its steps'
+ *                         rows; one column pass over the steps' planes) as
+ *                         dedicated kernels where the row length has them
+ *                         (512, 640, 1024, 1280, 2048); 1 = the operator
+ *                         pipeline step by step.  Bit-identical (640-point
+ *                         rows: with CXM_OPT_FL_ROW_E 20).
  *   CXM_OPT_MS_ROW_E      values per thread of the dedicated 1024-point slice
  *                         row pass: 0 = 4, or 2, 8, 16
  *   CXM_OPT_PSF_PASSES    cxm_psf4f_fwd / _bwd / _value_and_grad: 0 = the
@@ -891,6 +898,17 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         transforms in dedicated row and column passes,
  *                         where the FFT length has them (rows 1024-5120,
  *                         columns 1024-3072); 1 = the operator pipeline.
+ *                         Bit-identical.
+ *   CXM_OPT_FL_BWD_CHUNK  cxm_fluor_multislice_sample_c64's batched
+ *                         backward rows: steps per workgroup (0 = 8).
+ *                         Bit-identical.
+ *   CXM_OPT_FL_ROW_E      the same, 640-point rows: values per thread of
+ *                         the dedicated row passes (0 = 10, 20 = the operator
+ *                         pipeline's shape and bits).  Equal to float32
+ *                         rounding.
+ *   CXM_OPT_FL_BWD_BATCH  the same: steps per batched launch pair (0 = the
+ *                         planes that fit 512 MiB); sizes the workspace, so
+ *                         set it before cxm_fluor_multislice_workspace_size.
  *                         Bit-identical.
  *   CXM_OPT_GRAD_GROUPS   cxm_psf4f_bwd: 0 = the default number of plane
  *                         groups of the phase-gradient pass (4), 1..12 =
@@ -919,6 +937,9 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_PSF_PASSES 19
 #define CXM_OPT_PSF_GRAD_WAVES 20
 #define CXM_OPT_CZT_PASSES 21
+#define CXM_OPT_FL_BWD_CHUNK 22
+#define CXM_OPT_FL_BWD_BATCH 23
+#define CXM_OPT_FL_ROW_E 24
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

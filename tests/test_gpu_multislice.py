@@ -292,11 +292,13 @@ def test_dedicated_passes_match_pipeline_passes(pm, shape, pad):
 
 @pytest.mark.parametrize("H,pad", [(448, 32), (512, 64), (1024, 0)])
 def test_dedicated_fluorescent_passes_match_pipeline(H, pad):
-    """The dedicated fluorescent row passes (the slice's thin sample and
-    fluorescence between the row transforms; the backward propagator
-    multiply fused into the inverse row transform) give the operator
-    pipeline's bits (CXM_OPT_MS_PASSES 1), at padded lengths 512, 640 and
-    1024, two Monte-Carlo samples; and the 512 case matches the oracle."""
+    """The dedicated fluorescent passes (the slice's thin sample and
+    fluorescence between the row transforms; the backward sweep batched
+    over its steps) give the operator pipeline's bits (CXM_OPT_MS_PASSES 1)
+    at the pipeline's row shapes (CXM_OPT_FL_ROW_E 20), at padded lengths
+    512, 640 and 1024, two Monte-Carlo samples; the default 640-point shape
+    (10 values per thread) agrees to float32 rounding; and the 512 case
+    matches the oracle."""
     from chromatix_amd import _lib as L
 
     D, samples, t = 3, 2, 1.5
@@ -304,16 +306,53 @@ def test_dedicated_fluorescent_passes_match_pipeline(H, pad):
     ph = rng.uniform(0, 2 * np.pi, (samples, D, H, H)).astype(np.float32)
     outs = []
     try:
-        for opt in (0, 1):
-            L.check(L.lib().cxm_set_option(L.CXM_OPT_MS_PASSES, opt))
+        for ms, row_e in ((0, 20), (1, 0), (0, 0)):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_MS_PASSES, ms))
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_FL_ROW_E, row_e))
             f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, 0.532)
             outs.append(cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, t, pad, num_samples=samples,
                                                                random_phases=ph, phase_mode="accurate"))
     finally:
         L.check(L.lib().cxm_set_option(L.CXM_OPT_MS_PASSES, 0))
-    print(f"dedicated vs pipeline fluorescent {H + 2 * pad}: {bool(torch.equal(outs[0], outs[1]))}")
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_FL_ROW_E, 0))
+    e = rel_l2(outs[2].cpu().numpy(), outs[1].cpu().numpy())
+    print(f"dedicated vs pipeline fluorescent {H + 2 * pad}: {bool(torch.equal(outs[0], outs[1]))}, "
+          f"default shape {e:.2e}")
     assert torch.equal(outs[0], outs[1])
+    assert e < 1e-6
     if H + 2 * pad == 512:
         ref = om.fluorescent_multislice(u, fl.astype(np.float64), dn.astype(np.float64), (0.3, 0.3),
                                         0.532, 1.33, t, pad, ph.astype(np.float64))
         assert rel_l2(outs[0].cpu().numpy(), ref) < TOL
+
+
+@pytest.mark.parametrize("chunk,batch", [(1, 0), (3, 0), (2, 3), (3, 2), (1, 1)])
+def test_batched_fluorescent_backward_chunking(chunk, batch):
+    """The batched backward sweep gives the step-by-step pipeline's bits for
+    every split of its steps: steps per workgroup (CXM_OPT_FL_BWD_CHUNK,
+    ragged last chunk) and steps per launch pair (CXM_OPT_FL_BWD_BATCH: 3
+    batches of a 7-slice sample end with the carried spectrum in the second
+    buffer and copied back, 4 end in the caller's; one step per batch), two
+    Monte-Carlo samples so the carried spectrum is checked too (the
+    pipeline's 640-point row shape, CXM_OPT_FL_ROW_E 20)."""
+    from chromatix_amd import _lib as L
+
+    H, pad, D, samples, t = 448, 32, 7, 2, 1.5
+    u, fl, dn, rng = _fluor_case(H, D, samples)
+    ph = rng.uniform(0, 2 * np.pi, (samples, D, H, H)).astype(np.float32)
+    outs = []
+    try:
+        L.check(L.lib().cxm_set_option(L.CXM_OPT_FL_ROW_E, 20))
+        for ms, c, b in ((1, 0, 0), (0, chunk, batch)):
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_MS_PASSES, ms))
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_FL_BWD_CHUNK, c))
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_FL_BWD_BATCH, b))
+            f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, 0.532)
+            outs.append(cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, t, pad, num_samples=samples,
+                                                               random_phases=ph, phase_mode="accurate"))
+    finally:
+        for key in (L.CXM_OPT_MS_PASSES, L.CXM_OPT_FL_BWD_CHUNK, L.CXM_OPT_FL_BWD_BATCH,
+                    L.CXM_OPT_FL_ROW_E):
+            L.check(L.lib().cxm_set_option(key, 0))
+    print(f"batched backward chunk {chunk} batch {batch}: {bool(torch.equal(outs[0], outs[1]))}")
+    assert torch.equal(outs[0], outs[1])
