@@ -137,6 +137,7 @@ class OptFftDesc(ctypes.Structure):
         ("wavelengths", ctypes.POINTER(ctypes.c_double)),
         ("spacing", ctypes.POINTER(ctypes.c_double)),
         ("origin", ctypes.POINTER(ctypes.c_double)),
+        ("spec_mul", ctypes.c_void_p),
     ]
 
 

@@ -147,6 +147,13 @@ CXM_DEV float2 apply_op(const OpDesc& o, const OpTable& T, float2 v, int plane, 
     case OP_THIN:
     case OP_CARRAY:
     case OP_FLUOR: {
+      if (o.kind == OP_CARRAY && o.mode == 1) {
+        // an array in FFT order at fftshifted coordinates: back to its index
+        iy += ny - ny / 2;
+        if (iy >= ny) iy -= ny;
+        ix += nx - nx / 2;
+        if (ix >= nx) ix -= nx;
+      }
       const int ay = iy - o.arr_row_off, ax = ix - o.arr_col_off;
       if (ay < 0 || ay >= o.arr_rows || ax < 0 || ax >= o.arr_cols)
         return o.outside_zero ? make_float2(0.f, 0.f) : v;

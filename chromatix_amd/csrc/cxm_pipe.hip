@@ -257,6 +257,18 @@ extern "C" int cxm_optical_fft_c64(void* stream, const void* in, void* out, void
       if (q2) chirp_slot(cb, q2, w, c, c.du_y, c.du_x, c.du_y32, c.du_x32, v, mode);
     }
   }
+  if (d->spec_mul) {
+    // the spectrum multiply on the column pass's store
+    if (optical || adj || d->out_kind != CXM_OUT_COMPLEX)
+      return fail(CXM_ERR_INVALID, "spec_mul applies to the raw transforms with complex output");
+    OpDesc* m = cb.add(ca.post, OP_CARRAY, 1, 0, shift);
+    if (!m) return fail(CXM_ERR_INVALID, "operator table overflow");
+    m->arr0 = d->spec_mul;
+    m->arr_plane_stride = 0;
+    m->arr_row_stride = g.Nx;
+    m->arr_rows = g.Ny;
+    m->arr_cols = g.Nx;
+  }
   // transform direction: the forward DFT, or for the raw inverse and the
   // adjoint of the forward modes (z >= 0) the inverse one; the adjoint of
   // the raw inverse is the forward DFT scaled by 1/N, that of the raw

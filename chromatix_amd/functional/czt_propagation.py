@@ -21,6 +21,7 @@ import torch
 from .. import _lib as L
 from ..core.field import ChromaticScalarField, Field, ScalarField, crop, pad, shift_grid
 from ..utils.czt import czt_axis, scaled_conv_axis, use_zoom_mfma, zoom_dft
+from .optics import _raw as _raw_fft
 from .optics import fft as _fft_shifted
 from .propagation import (
     _pair,
@@ -86,7 +87,11 @@ def czt_kernel_propagate(field: Field, propagator, output_dx=None, output_shape=
     H = H.reshape((-1, ny, nx)).resolve_conj().contiguous()
     u = field.u.resolve_conj().contiguous()
     L.ensure_device(u)
-    spec = _ShiftedMulFn.apply(_fft_shifted(u, axes=(-2, -1), shift=True), H)
+    if H.shape[0] == 1 and not (torch.is_grad_enabled() and u.requires_grad):
+        # the propagator multiply on the shifted transform's column store
+        spec = _raw_fft(u, (-2, -1), True, False, spec_mul=H[0])
+    else:
+        spec = _ShiftedMulFn.apply(_fft_shifted(u, axes=(-2, -1), shift=True), H)
     out_field = Field.empty_like(shift_grid(field, shift_yx), dx=out_dx, shape=output_shape)
     alpha = out_field.dx / df
     if not use_czt:

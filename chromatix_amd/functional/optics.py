@@ -287,7 +287,10 @@ def compute_sas_precompensation(field: Field, z, n, *, phase_mode=None) -> torch
     return out[0]
 
 
-def _raw(x: torch.Tensor, axes, shift: bool, inverse: bool) -> torch.Tensor:
+def _raw(x: torch.Tensor, axes, shift: bool, inverse: bool, spec_mul=None) -> torch.Tensor:
+    """fft2 / ifft2 over `axes`; spec_mul (a contiguous complex64 device
+    array of the two axes' shape, FFT order; not differentiated) multiplies
+    the output by fftshift(spec_mul) (shift) or spec_mul in the same pass."""
     x = torch.as_tensor(x)
     ax = tuple(a % x.ndim for a in axes)
     if len(ax) != 2:
@@ -298,6 +301,11 @@ def _raw(x: torch.Tensor, axes, shift: bool, inverse: bool) -> torch.Tensor:
     f = ScalarField(xm, 1.0, 0.0, 1.0)
     d, keep = _desc(f, L.CXM_OFFT_RAW_INV if inverse else L.CXM_OFFT_RAW_FWD, 0.0, 1.0, None)
     d.shift = int(bool(shift))
+    if spec_mul is not None:
+        if torch.is_grad_enabled() and xm.requires_grad:
+            raise RuntimeError("the fused spectrum multiply has no reverse-mode rule")
+        d.spec_mul = spec_mul.data_ptr()
+        keep = (keep, spec_mul)
     out = _apply(f, d, keep, "optical_fft", f.spatial_shape)
     return out.movedim((-2, -1), ax)
 

@@ -196,6 +196,11 @@ typedef struct cxm_optfft_desc {
   const double* wavelengths;  /* num_wv, host */
   const double* spacing;      /* num_wv x (dy, dx), host */
   const double* origin;       /* num_wv x (oy, ox), host */
+  const void* spec_mul;       /* RAW_FWD / RAW_INV, not adjoint, complex output:
+                                 optional height x width complex64 array (device, FFT
+                                 order) multiplied onto every output plane, at the
+                                 output's fftshifted positions when shift is set --
+                                 fft2 (shifted) times fftshift(spec_mul) in one pass */
 } cxm_optfft_desc;
 
 size_t cxm_optical_fft_workspace_size(const cxm_optfft_desc* d);

@@ -199,3 +199,24 @@ def test_dedicated_chirp_passes_match_pipeline(n, m, axis):
     ref = oc.czt_direct(x.cpu().numpy()[1].astype(np.complex128), m, aa[1], ww[1], axis=axis)
     ref = ref * ww[1] ** (-(n // 2) * np.arange(m)).reshape((-1, 1) if axis == 0 else (1, -1)) * aa[1] ** (n // 2) * 0.5
     assert rel_l2(outs[0].cpu().numpy()[1], ref) < TOL
+
+
+@pytest.mark.parametrize("shape,use_czt", [((1024, 1024), False), ((96, 160), False), ((96, 160), True)])
+def test_fused_propagator_multiply_matches_separate(shape, use_czt):
+    """kernel_propagate's output-grid branches without gradients multiply the
+    shifted spectrum by fftshift(H) on the transform's column store; with
+    gradients (the autograd path) in a separate pass.  Same bits."""
+    ny, nx = shape
+    rng = np.random.default_rng(3)
+    u = torch.polar(torch.ones(2, ny, nx), torch.as_tensor(rng.uniform(0, 2 * np.pi, (2, ny, nx)),
+                                                            dtype=torch.float32)).cuda()
+    f = cx.ScalarField(u, 0.3, 0.0, 0.532)
+    H = cf.compute_asm_propagator(f, 50.0, 1.33)
+    kw = dict(output_dx=0.45, use_czt=use_czt)
+    if use_czt:
+        kw["output_shape"] = (64, 48)
+    fused = cf.kernel_propagate(f, H, **kw).u
+    ug = u.clone().requires_grad_(True)
+    sep = cf.kernel_propagate(cx.ScalarField(ug, 0.3, 0.0, 0.532), H, **kw).u.detach()
+    print(f"fused vs separate propagator multiply {shape} czt={use_czt}: {bool(torch.equal(fused, sep))}")
+    assert torch.equal(fused, sep)
