@@ -56,6 +56,7 @@ CXM_OPT_CZT_PASSES = 21
 CXM_OPT_FL_BWD_CHUNK = 22
 CXM_OPT_FL_BWD_BATCH = 23
 CXM_OPT_FL_ROW_E = 24
+CXM_OPT_FL_FACTOR_BLOCK = 25
 CXM_H_TABLE_MIN_PLANES = 4
 
 PHASE_MODES = {"ref32": CXM_PHASE_REF32, "accurate": CXM_PHASE_ACCURATE}

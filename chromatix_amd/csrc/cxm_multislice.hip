@@ -781,12 +781,28 @@ struct FlRowArgs {
   const float2* src;  // FIRST: the spatial field of slice 0
   const float2* tw;
   int Ny;
-  const float* dn;    // slice arrays, h x w at (pad_y, pad_x)
-  const float* fl;
-  const float* ph;
+  const float4* tf;   // the slice's factors, h x w at (pad_y, pad_x)
   int h, w, pad_y, pad_x;
-  float t, lam;
 };
+
+// The slice factors of a block of slices, which do not depend on the field:
+// tf = (T, F), T = exp(i 2 pi dn t / lam) (thin_sample's factor, its float32
+// order), F = f exp(i r) (the fluorescence term) -- the values the row pass
+// combines with the field as x = v T, v = F + x.  One full-chip pass per
+// block, so that the sine/cosine work leaves the slice chain.
+__global__ void fl_factors_kernel(const float* __restrict__ dn, const float* __restrict__ fl,
+                                  const float* __restrict__ ph, float4* __restrict__ tf, long long n,
+                                  float t, float lam) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float ang, m, sn, cs;
+  thin_angle(dn[i], 0.0f, t, lam, &ang, &m);
+  sincos_exact(ang, &sn, &cs);
+  const float tx = m * cs, ty = m * sn;
+  const float f = fl[i];
+  sincos_exact(ph[i], &sn, &cs);
+  tf[i] = make_float4(tx, ty, __fmul_rn(f, cs), __fmul_rn(f, sn));
+}
 
 template <int N, int E>
 struct FlRowCfg {
@@ -810,19 +826,16 @@ __global__ void __launch_bounds__((FlRowCfg<N, E>::G * (N / E))) fl_row_kernel(F
   float2 v[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) v[k] = (FIRST ? a.src : a.W)[rb + t + k * T];
-  // the slice arrays loaded before the first transform, which hides their
+  // the slice factors loaded before the first transform, which hides their
   // latency
   const int ay = r - a.pad_y;
   const bool rin = ok && ay >= 0 && ay < a.h;
-  float sd[E], sf[E], sp[E];
+  float4 f4[E];
 #pragma unroll
   for (int k = 0; k < E; ++k) {
     const int ax = t + k * T - a.pad_x;
     const bool in = rin && ax >= 0 && ax < a.w;
-    const long long idx = in ? (long long)ay * a.w + ax : 0;
-    sd[k] = in ? a.dn[idx] : 0.0f;
-    sf[k] = in ? a.fl[idx] : 0.0f;
-    sp[k] = in ? a.ph[idx] : 0.0f;
+    f4[k] = in ? a.tf[(long long)ay * a.w + ax] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
   if constexpr (!FIRST) fft<N, E, 1, 1, SYNC>(v, st, t, a.tw);
   // the thread index laundered: the forward transform's twiddle loads are
@@ -834,13 +847,8 @@ __global__ void __launch_bounds__((FlRowCfg<N, E>::G * (N / E))) fl_row_kernel(F
     for (int k = 0; k < E; ++k) {
       const int ax = tl + k * T - a.pad_x;
       if (ax < 0 || ax >= a.w) continue;
-      float ang, m, sn, cs;
-      thin_angle(sd[k], 0.0f, a.t, a.lam, &ang, &m);
-      sincos_exact(ang, &sn, &cs);
-      float2 x = cmul(v[k], make_float2(m * cs, m * sn));
-      const float f = sf[k];
-      sincos_exact(sp[k], &sn, &cs);
-      v[k] = make_float2(__fadd_rn(__fmul_rn(f, cs), x.x), __fadd_rn(__fmul_rn(f, sn), x.y));
+      const float2 x = cmul(v[k], make_float2(f4[k].x, f4[k].y));
+      v[k] = make_float2(__fadd_rn(f4[k].z, x.x), __fadd_rn(f4[k].w, x.y));
     }
   }
   fft<N, E, -1, 1, SYNC>(v, st, tl, a.tw);
@@ -1004,6 +1012,26 @@ int fl_bwd_batch(const FlGeom& g) {
   return (int)std::max(1ll, std::min((long long)g.D, cap));
 }
 
+// Slices per block of precomputed slice factors (16 bytes per slice
+// element): at most 512 MiB of them (at least one slice), or the number
+// CXM_OPT_FL_FACTOR_BLOCK sets.
+int fl_factor_block(const FlGeom& g) {
+  const int o = option(CXM_OPT_FL_FACTOR_BLOCK);
+  if (o > 0) return std::min(g.D, o);
+  const long long cap = (512ll << 20) / ((long long)g.h * g.w * (long long)sizeof(float4));
+  return (int)std::max(1ll, std::min((long long)g.D, cap));
+}
+
+// Byte offset of the factor block in the workspace (256-byte aligned) and
+// the workspace size.
+size_t fl_factor_offset(const FlGeom& g) {
+  const size_t planes = (size_t)(3 + fl_bwd_batch(g)) * g.plane * sizeof(float2);
+  return (planes + 255) / 256 * 256;
+}
+size_t fl_workspace(const FlGeom& g) {
+  return fl_factor_offset(g) + (size_t)fl_factor_block(g) * g.h * g.w * sizeof(float4);
+}
+
 // Thin sample and fluorescence of slice i as operators of list L.
 int fl_slice_ops(Ops& b, OpList& L, const cxm_fluor_desc* d, const FlGeom& g, const float* dn,
                  const float* fl, const float* ph, int i) {
@@ -1076,9 +1104,10 @@ int fl_propagator(const cxm_fluor_desc* d, const FlGeom& g, double z, void* out,
 extern "C" size_t cxm_fluor_multislice_workspace_size(const cxm_fluor_desc* d) {
   FlGeom g;
   if (fl_geom(d, g)) return 0;
-  // P(+t), P(-t), the carried spectrum's second buffer, then the row-transformed
-  // field (forward) or the backward steps' planes
-  return (size_t)(3 + fl_bwd_batch(g)) * g.plane * sizeof(float2);
+  // P(+t), P(-t), the carried spectrum's second buffer, the row-transformed
+  // field (forward) or the backward steps' planes, then a block of slice
+  // factors
+  return fl_workspace(g);
 }
 
 extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const float* fluorescence,
@@ -1091,7 +1120,7 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
   if (!state || !fluorescence || !dn || !phases || !intensity)
     return fail(CXM_ERR_INVALID, "null buffer");
   const int B = fl_bwd_batch(g);
-  const size_t need = (size_t)(3 + B) * g.plane * sizeof(float2);
+  const size_t need = fl_workspace(g);
   if (!workspace || workspace_bytes < need)
     return fail(CXM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
   if ((rc = ensure_init())) return rc;
@@ -1101,6 +1130,7 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
   float2* Pb = Pf + g.plane;
   float2* U2 = Pb + g.plane;
   float2* W = U2 + g.plane;
+  float4* TF = (float4*)((char*)workspace + fl_factor_offset(g));
   const float2* Pfw = (const float2*)d->prop_fwd;
   const float2* Pbw = (const float2*)d->prop_bwd;
   if (!Pfw) {
@@ -1122,15 +1152,22 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
   fr.w = g.w;
   fr.pad_y = d->pad_y;
   fr.pad_x = d->pad_x;
-  fr.t = (float)d->thickness;
-  fr.lam = (float)d->wavelength;
   const long long sl = (long long)g.h * g.w;
+  // the factors of slices s0 .. s0 + NB - 1 (at TF + (slice % NB) * sl)
+  const int NB = fl_factor_block(g);
+  auto factors = [&](int s0) -> int {
+    const long long n = (long long)std::min(NB, g.D - s0) * sl;
+    hipLaunchKernelGGL(fl_factors_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       dn + s0 * sl, fluorescence + s0 * sl, phases + s0 * sl, TF, n,
+                       (float)d->thickness, (float)d->wavelength);
+    CXM_LAUNCH_CHECK();
+    return CXM_OK;
+  };
   // forward, slice 0: thin sample + fluorescence, row FFT (state -> W)
   if (ded) {
+    if ((rc = factors(0))) return rc;
     fr.src = u;
-    fr.dn = dn;
-    fr.fl = fluorescence;
-    fr.ph = phases;
+    fr.tf = TF;
     if ((rc = launch_fl_row(g.Nx, true, fr, s))) return rc;
   } else {
     PipeArgs a;
@@ -1164,9 +1201,8 @@ extern "C" int cxm_fluor_multislice_sample_c64(void* stream, void* state, const 
       if ((rc = launch_col_prop(g.Ny, ca, s))) return rc;
       // row IFFT, slice i+1's thin sample + fluorescence, row FFT, in place
       if (ded) {
-        fr.dn = dn + (i + 1) * sl;
-        fr.fl = fluorescence + (i + 1) * sl;
-        fr.ph = phases + (i + 1) * sl;
+        if ((i + 1) % NB == 0 && (rc = factors(i + 1))) return rc;
+        fr.tf = TF + (long long)((i + 1) % NB) * sl;
         if ((rc = launch_fl_row(g.Nx, false, fr, s))) return rc;
         continue;
       }

@@ -911,6 +911,10 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
  *                         the dedicated row passes (0 = 10, 20 = the operator
  *                         pipeline's shape and bits).  Equal to float32
  *                         rounding.
+ *   CXM_OPT_FL_FACTOR_BLOCK  the same: slices per block of precomputed
+ *                         slice factors (0 = the slices that fit 512 MiB);
+ *                         sizes the workspace like CXM_OPT_FL_BWD_BATCH.
+ *                         Bit-identical.
  *   CXM_OPT_FL_BWD_BATCH  the same: steps per batched launch pair (0 = the
  *                         planes that fit 512 MiB); sizes the workspace, so
  *                         set it before cxm_fluor_multislice_workspace_size.
@@ -945,6 +949,7 @@ int cxm_timing_get(int i, char* name, int name_len, double* total_ms, int64_t* l
 #define CXM_OPT_FL_BWD_CHUNK 22
 #define CXM_OPT_FL_BWD_BATCH 23
 #define CXM_OPT_FL_ROW_E 24
+#define CXM_OPT_FL_FACTOR_BLOCK 25
 #define CXM_H_TABLE_MIN_PLANES 4
 int cxm_set_option(int key, int value);
 

@@ -326,15 +326,17 @@ def test_dedicated_fluorescent_passes_match_pipeline(H, pad):
         assert rel_l2(outs[0].cpu().numpy(), ref) < TOL
 
 
-@pytest.mark.parametrize("chunk,batch", [(1, 0), (3, 0), (2, 3), (3, 2), (1, 1)])
-def test_batched_fluorescent_backward_chunking(chunk, batch):
+@pytest.mark.parametrize("chunk,batch,block", [(1, 0, 0), (3, 0, 3), (2, 3, 0), (3, 2, 2), (1, 1, 1)])
+def test_batched_fluorescent_backward_chunking(chunk, batch, block):
     """The batched backward sweep gives the step-by-step pipeline's bits for
     every split of its steps: steps per workgroup (CXM_OPT_FL_BWD_CHUNK,
     ragged last chunk) and steps per launch pair (CXM_OPT_FL_BWD_BATCH: 3
     batches of a 7-slice sample end with the carried spectrum in the second
     buffer and copied back, 4 end in the caller's; one step per batch), two
-    Monte-Carlo samples so the carried spectrum is checked too (the
-    pipeline's 640-point row shape, CXM_OPT_FL_ROW_E 20)."""
+    Monte-Carlo samples so the carried spectrum is checked too; and the
+    slice factors precomputed in blocks of 1, 2, 3 or all 7 slices
+    (CXM_OPT_FL_FACTOR_BLOCK; the pipeline's 640-point row shape,
+    CXM_OPT_FL_ROW_E 20)."""
     from chromatix_amd import _lib as L
 
     H, pad, D, samples, t = 448, 32, 7, 2, 1.5
@@ -347,12 +349,14 @@ def test_batched_fluorescent_backward_chunking(chunk, batch):
             L.check(L.lib().cxm_set_option(L.CXM_OPT_MS_PASSES, ms))
             L.check(L.lib().cxm_set_option(L.CXM_OPT_FL_BWD_CHUNK, c))
             L.check(L.lib().cxm_set_option(L.CXM_OPT_FL_BWD_BATCH, b))
+            L.check(L.lib().cxm_set_option(L.CXM_OPT_FL_FACTOR_BLOCK, block if ms == 0 else 0))
             f = cx.ScalarField(torch.as_tensor(u).cuda(), 0.3, 0.0, 0.532)
             outs.append(cf.fluorescent_multislice_thick_sample(f, fl, dn, 1.33, t, pad, num_samples=samples,
                                                                random_phases=ph, phase_mode="accurate"))
     finally:
         for key in (L.CXM_OPT_MS_PASSES, L.CXM_OPT_FL_BWD_CHUNK, L.CXM_OPT_FL_BWD_BATCH,
-                    L.CXM_OPT_FL_ROW_E):
+                    L.CXM_OPT_FL_ROW_E, L.CXM_OPT_FL_FACTOR_BLOCK):
             L.check(L.lib().cxm_set_option(key, 0))
-    print(f"batched backward chunk {chunk} batch {batch}: {bool(torch.equal(outs[0], outs[1]))}")
+    print(f"batched backward chunk {chunk} batch {batch} factor block {block}: "
+          f"{bool(torch.equal(outs[0], outs[1]))}")
     assert torch.equal(outs[0], outs[1])
