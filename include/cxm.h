@@ -602,7 +602,8 @@ int cxm_shot_noise_approx_f32(void* stream, const float* image, const float* nor
  * order, device) when non-null.  intensity: depth x height x width float32,
  * accumulated.  The workspace (cxm_fluor_multislice_workspace_size bytes)
  * holds 3 + B padded planes, B = min(depth, planes in 512 MiB): the backward
- * sweep transforms B steps per batched launch.
+ * sweep transforms B steps per batched launch; and the precomputed factors
+ * (16 bytes per slice element) of up to 512 MiB of slices.
  */
 typedef struct cxm_fluor_desc {
   int32_t height, width;      /* unpadded field and slices */
